@@ -1,0 +1,339 @@
+"""barney_amd -- MI355X (gfx950) ray-tracing and volume backend for barney.
+
+Python mirror of the C-ABI in include/bnx_rt.h (loaded with ctypes from the
+in-tree barney_amd/libbnx.so).  The objects follow barney's rtcore backend
+interface: Device, Buffer, Geom, Group (triangles / user geometry /
+instances), plus the World, Frame and render() of the native layer.
+
+There is no fallback: if the HIP library is missing or no GPU is present,
+every call raises.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+from . import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "libbnx.so")
+_lib = None
+
+vp = C.c_void_p
+
+
+class BnxError(RuntimeError):
+    def __init__(self, code: int, msg: str):
+        super().__init__(f"bnx error {code}: {msg}")
+        self.code = code
+
+
+# (name, argtypes, restype)
+_PROTOS = [
+    ("bnx_last_error", [], C.c_char_p),
+    ("bnx_abi_version", [], C.c_int),
+    ("bnx_device_create", [C.c_int, C.POINTER(vp)], C.c_int),
+    ("bnx_device_destroy", [vp], C.c_int),
+    ("bnx_device_sync", [vp], C.c_int),
+    ("bnx_device_mem_info", [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+    ("bnx_buffer_create", [vp, C.c_size_t, vp, C.POINTER(vp)], C.c_int),
+    ("bnx_buffer_upload", [vp, vp, C.c_size_t, C.c_size_t], C.c_int),
+    ("bnx_buffer_download", [vp, vp, C.c_size_t, C.c_size_t], C.c_int),
+    ("bnx_buffer_destroy", [vp], C.c_int),
+    ("bnx_buffer_dptr", [vp], vp),
+    ("bnx_buffer_size", [vp], C.c_size_t),
+    ("bnx_geom_create_triangles", [vp, C.POINTER(abi.TrianglesDesc), C.POINTER(vp)], C.c_int),
+    ("bnx_geom_create_spheres", [vp, C.POINTER(abi.SpheresDesc), C.POINTER(vp)], C.c_int),
+    ("bnx_geom_destroy", [vp], C.c_int),
+    ("bnx_group_create_triangles", [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
+    ("bnx_group_create_user", [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
+    ("bnx_group_create_instances", [vp, C.POINTER(vp), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int,
+                                    C.POINTER(vp)], C.c_int),
+    ("bnx_group_build", [vp], C.c_int),
+    ("bnx_group_bvh_info", [vp, C.POINTER(abi.BvhInfo)], C.c_int),
+    ("bnx_group_download_bvh", [vp, vp, C.POINTER(C.c_int32)], C.c_int),
+    ("bnx_group_destroy", [vp], C.c_int),
+    ("bnx_bvh_build_boxes", [vp, vp, C.c_int, C.c_int, vp, C.POINTER(C.c_int32), C.POINTER(abi.BvhInfo)], C.c_int),
+    ("bnx_trace", [vp, vp, C.c_int, vp, C.c_uint32], C.c_int),
+    ("bnx_world_create", [vp, vp, C.POINTER(abi.DirLight), C.c_int, C.POINTER(vp)], C.c_int),
+    ("bnx_world_destroy", [vp], C.c_int),
+    ("bnx_camera_perspective", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                C.POINTER(abi.Camera)], C.c_int),
+    ("bnx_frame_create", [vp, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
+    ("bnx_frame_reset", [vp], C.c_int),
+    ("bnx_frame_read", [vp, C.c_int, vp], C.c_int),
+    ("bnx_frame_accum_id", [vp], C.c_int),
+    ("bnx_frame_destroy", [vp], C.c_int),
+    ("bnx_render", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.POINTER(abi.RenderStats)],
+     C.c_int),
+]
+EXPORTED = [p[0] for p in _PROTOS]
+
+
+def lib():
+    """The loaded HIP library; raises when it has not been built."""
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"HIP library missing: {LIB_PATH} (run `python -m barney_amd.build`)")
+        L = C.CDLL(LIB_PATH)
+        for name, args, res in _PROTOS:
+            f = getattr(L, name)
+            f.argtypes = args
+            f.restype = res
+        _lib = L
+    return _lib
+
+
+def check(rc: int):
+    if rc != abi.BNX_OK:
+        raise BnxError(rc, lib().bnx_last_error().decode())
+
+
+def _handle_array(objs):
+    arr = (vp * max(1, len(objs)))()
+    for i, o in enumerate(objs):
+        arr[i] = o.h
+    return arr
+
+
+class Device:
+    """rtc::Device: one GPU and its stream."""
+
+    def __init__(self, gpu: int = 0):
+        self.L = lib()
+        h = vp()
+        check(self.L.bnx_device_create(gpu, C.byref(h)))
+        self.h = h
+
+    def sync(self):
+        check(self.L.bnx_device_sync(self.h))
+
+    def mem_info(self):
+        f, t = C.c_size_t(), C.c_size_t()
+        check(self.L.bnx_device_mem_info(self.h, C.byref(f), C.byref(t)))
+        return f.value, t.value
+
+    def close(self):
+        if self.h:
+            self.L.bnx_device_destroy(self.h)
+            self.h = None
+
+
+class Buffer:
+    """rtc::Buffer: device memory."""
+
+    def __init__(self, dev: Device, nbytes: int = 0, data: np.ndarray | None = None):
+        self.dev, self.L = dev, dev.L
+        if data is not None:
+            data = np.ascontiguousarray(data)
+            nbytes = data.nbytes
+        h = vp()
+        check(self.L.bnx_buffer_create(dev.h, nbytes, data.ctypes.data if data is not None else None, C.byref(h)))
+        self.h = h
+        self.nbytes = nbytes
+
+    @property
+    def dptr(self) -> int:
+        return self.L.bnx_buffer_dptr(self.h)
+
+    def upload(self, data: np.ndarray, offset: int = 0):
+        data = np.ascontiguousarray(data)
+        check(self.L.bnx_buffer_upload(self.h, data.ctypes.data, data.nbytes, offset))
+
+    def download(self, out: np.ndarray, offset: int = 0) -> np.ndarray:
+        check(self.L.bnx_buffer_download(self.h, out.ctypes.data, out.nbytes, offset))
+        return out
+
+    def close(self):
+        if self.h:
+            self.L.bnx_buffer_destroy(self.h)
+            self.h = None
+
+
+class Geom:
+    def __init__(self, dev, h, keep):
+        self.dev, self.h, self._keep = dev, h, keep
+
+    @staticmethod
+    def triangles(dev: Device, vertices, indices, colors=None, user_id=0, material=None) -> "Geom":
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        d = abi.TrianglesDesc(abi.fptr(v), len(v), len(i), abi.iptr(i), abi.fptr(c), user_id, 0,
+                              material or abi.matte())
+        h = vp()
+        check(dev.L.bnx_geom_create_triangles(dev.h, C.byref(d), C.byref(h)))
+        return Geom(dev, h, None)
+
+    @staticmethod
+    def spheres(dev: Device, centers, radii=None, colors=None, user_id=1, material=None,
+                default_radius=0.1) -> "Geom":
+        ce = np.ascontiguousarray(centers, dtype=np.float32)
+        r = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        d = abi.SpheresDesc(abi.fptr(ce), abi.fptr(r), abi.fptr(c), len(ce), default_radius, user_id, 0,
+                            material or abi.matte())
+        h = vp()
+        check(dev.L.bnx_geom_create_spheres(dev.h, C.byref(d), C.byref(h)))
+        return Geom(dev, h, None)
+
+
+TRIANGLES, USER = 0, 1
+
+
+class Group:
+    """rtc::Group: a triangles group, a user-geometry group, or an instance group."""
+
+    def __init__(self, dev: Device, h, members):
+        self.dev, self.h, self.members = dev, h, members
+
+    @staticmethod
+    def geoms(dev: Device, kind: int, geoms) -> "Group":
+        arr = _handle_array(geoms)
+        h = vp()
+        fn = dev.L.bnx_group_create_triangles if kind == TRIANGLES else dev.L.bnx_group_create_user
+        check(fn(dev.h, arr, len(geoms), C.byref(h)))
+        return Group(dev, h, list(geoms))
+
+    @staticmethod
+    def instances(dev: Device, groups, inst_ids=None, xfms=None) -> "Group":
+        arr = _handle_array(groups)
+        ids = (C.c_int32 * max(1, len(groups)))(*(inst_ids if inst_ids is not None else range(len(groups))))
+        x = None
+        if xfms is not None:
+            xa = np.ascontiguousarray(xfms, dtype=np.float32).reshape(-1)
+            x = (C.c_float * len(xa))(*xa)
+        h = vp()
+        check(dev.L.bnx_group_create_instances(dev.h, arr, ids, x, len(groups), C.byref(h)))
+        return Group(dev, h, list(groups))
+
+    def build(self):
+        check(self.dev.L.bnx_group_build(self.h))
+        return self
+
+    def info(self) -> abi.BvhInfo:
+        inf = abi.BvhInfo()
+        check(self.dev.L.bnx_group_bvh_info(self.h, C.byref(inf)))
+        return inf
+
+    def bvh(self):
+        inf = self.info()
+        nodes = np.zeros(max(1, inf.num_nodes), dtype=abi.NODE_DTYPE)
+        order = np.zeros(max(1, inf.num_prims), dtype=np.int32)
+        check(self.dev.L.bnx_group_download_bvh(self.h, nodes.ctypes.data, abi.iptr(order)))
+        return nodes[: inf.num_nodes], order[: inf.num_prims], inf.max_depth
+
+    def trace(self, rays: np.ndarray, any_hit: bool = False) -> np.ndarray:
+        """Closest (or any) hit of host rays (n x 8 float32, bnx_ray layout)."""
+        rays = np.ascontiguousarray(rays, dtype=np.float32).reshape(-1, 8)
+        n = len(rays)
+        hits = np.zeros(n, dtype=abi.HIT_DTYPE)
+        if n == 0:
+            return hits
+        rb = Buffer(self.dev, data=rays)
+        hb = Buffer(self.dev, nbytes=hits.nbytes)
+        try:
+            check(self.dev.L.bnx_trace(self.h, rb.dptr, n, hb.dptr,
+                                       abi.BNX_TRACE_ANY if any_hit else abi.BNX_TRACE_CLOSEST))
+            hb.download(hits)
+        finally:
+            rb.close()
+            hb.close()
+        return hits
+
+
+def bvh_build_boxes(dev: Device, boxes: np.ndarray, max_leaf: int = 4):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(max(1, n), dtype=abi.NODE_DTYPE)
+    order = np.zeros(max(1, n), dtype=np.int32)
+    inf = abi.BvhInfo()
+    b = Buffer(dev, data=boxes if n else np.zeros(6, np.float32))
+    try:
+        check(dev.L.bnx_bvh_build_boxes(dev.h, b.dptr, n, max_leaf, nodes.ctypes.data, abi.iptr(order),
+                                        C.byref(inf)))
+    finally:
+        b.close()
+    return nodes[: inf.num_nodes], order[:n], inf
+
+
+def camera_perspective(position, direction, up, fovy_deg) -> abi.Camera:
+    L = lib()
+    cam = abi.Camera()
+    p = (C.c_float * 3)(*position)
+    d = (C.c_float * 3)(*direction)
+    u = (C.c_float * 3)(*up)
+    check(L.bnx_camera_perspective(p, d, u, fovy_deg, C.byref(cam)))
+    return cam
+
+
+class World:
+    def __init__(self, dev: Device, instances: Group, dir_lights=()):
+        self.dev, self.instances = dev, instances
+        arr = abi.dir_lights(list(dir_lights))
+        h = vp()
+        check(dev.L.bnx_world_create(dev.h, instances.h, arr, len(dir_lights), C.byref(h)))
+        self.h = h
+
+
+class Frame:
+    def __init__(self, dev: Device, w: int, h: int):
+        self.dev, self.w, self.h_ = dev, w, None
+        self.hgt = h
+        hh = vp()
+        check(dev.L.bnx_frame_create(dev.h, w, h, C.byref(hh)))
+        self.h_ = hh
+
+    @property
+    def accum_id(self) -> int:
+        return self.dev.L.bnx_frame_accum_id(self.h_)
+
+    def reset(self):
+        check(self.dev.L.bnx_frame_reset(self.h_))
+
+    def read(self, channel: int) -> np.ndarray:
+        if channel == abi.BNX_FB_COLOR:
+            out = np.zeros((self.hgt, self.w, 4), np.float32)
+        elif channel == abi.BNX_FB_NORMAL:
+            out = np.zeros((self.hgt, self.w, 3), np.float32)
+        elif channel == abi.BNX_FB_DEPTH:
+            out = np.zeros((self.hgt, self.w), np.float32)
+        else:
+            out = np.zeros((self.hgt, self.w), np.int32)
+        check(self.dev.L.bnx_frame_read(self.h_, channel, out.ctypes.data))
+        return out
+
+    def close(self):
+        if self.h_:
+            self.dev.L.bnx_frame_destroy(self.h_)
+            self.h_ = None
+
+
+def render(world: World, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp: int) -> dict:
+    st = abi.RenderStats()
+    check(world.dev.L.bnx_render(world.h, C.byref(cam), C.byref(rend), frame.h_, spp, C.byref(st)))
+    return dict(rays=st.rays_traced, shadow=st.shadow_rays, generations=st.generations,
+                volume_samples=st.volume_samples, ms=st.ms_total, nodes=st.node_visits, tris=st.tri_tests,
+                spheres=st.sphere_tests, trace_ms=st.trace_ms, trace_launches=st.trace_launches)
+
+
+class SurfaceWorld:
+    """GPU world for a barney_amd.scenes.SurfaceScene, flattened like
+    native/ModelSlot.cpp:128-155: user-geometry group (spheres) first, then
+    the triangles group, both as instances of user instance 0."""
+
+    def __init__(self, dev: Device, sc):
+        self.dev = dev
+        self.sph = [Geom.spheres(dev, s.centers, s.radii, s.colors, user_id=s.user_id) for s in sc.spheres]
+        self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id) for m in sc.meshes]
+        groups = []
+        if self.sph:
+            groups.append(Group.geoms(dev, USER, self.sph).build())
+        if self.tri:
+            groups.append(Group.geoms(dev, TRIANGLES, self.tri).build())
+        self.groups = groups
+        self.insts = Group.instances(dev, groups, [0] * len(groups)).build()
+        self.world = World(dev, self.insts, sc.dir_lights)

@@ -1,0 +1,110 @@
+"""ctypes mirror of the POD records of include/bnx_rt.h."""
+from __future__ import annotations
+
+import ctypes as C
+
+import numpy as np
+
+BNX_OK = 0
+BNX_MATERIAL_MATTE = 0
+BNX_CAMERA_PERSPECTIVE = 1
+BNX_RENDER_COMPAT_SPHERE_INSTID13 = 1
+BNX_FB_COLOR, BNX_FB_DEPTH, BNX_FB_PRIMID, BNX_FB_INSTID, BNX_FB_OBJID, BNX_FB_NORMAL = range(6)
+BNX_TRACE_CLOSEST = 0
+BNX_TRACE_ANY = 1
+
+f3 = C.c_float * 3
+f4 = C.c_float * 4
+fp = C.POINTER(C.c_float)
+ip = C.POINTER(C.c_int32)
+
+
+class Material(C.Structure):
+    _fields_ = [("type", C.c_int32), ("color_from_attribute", C.c_int32), ("color", f4),
+                ("opacity", C.c_float), ("_pad", C.c_float)]
+
+
+class TrianglesDesc(C.Structure):
+    _fields_ = [("vertices", fp), ("num_vertices", C.c_int32), ("num_triangles", C.c_int32),
+                ("indices", ip), ("prim_colors", fp), ("user_id", C.c_int32), ("_pad", C.c_int32),
+                ("material", Material)]
+
+
+class SpheresDesc(C.Structure):
+    _fields_ = [("centers", fp), ("radii", fp), ("colors", fp), ("num_spheres", C.c_int32),
+                ("default_radius", C.c_float), ("user_id", C.c_int32), ("_pad", C.c_int32),
+                ("material", Material)]
+
+
+class Camera(C.Structure):
+    _fields_ = [("type", C.c_int32), ("dir_00", f3), ("dir_du", f3), ("dir_dv", f3), ("lens_00", f3),
+                ("aperture_radius", C.c_float), ("focus_distance", C.c_float)]
+
+
+class Renderer(C.Structure):
+    _fields_ = [("bg_color", f4), ("ambient_radiance", C.c_float), ("cut_plane", f4), ("flags", C.c_uint32),
+                ("max_generations", C.c_int32), ("_pad", C.c_int32)]
+
+
+class DirLight(C.Structure):
+    _fields_ = [("direction", f3), ("radiance", C.c_float), ("color", f3)]
+
+
+class RenderStats(C.Structure):
+    _fields_ = [("rays_traced", C.c_int64), ("shadow_rays", C.c_int64), ("generations", C.c_int64),
+                ("volume_samples", C.c_int64), ("ms_total", C.c_double), ("node_visits", C.c_int64),
+                ("tri_tests", C.c_int64), ("sphere_tests", C.c_int64), ("trace_ms", C.c_double),
+                ("trace_launches", C.c_int64)]
+
+
+class BvhInfo(C.Structure):
+    _fields_ = [("num_nodes", C.c_int32), ("num_prims", C.c_int32), ("max_depth", C.c_int32),
+                ("root_is_leaf", C.c_int32), ("root_lower", f3), ("root_upper", f3), ("build_ms", C.c_double)]
+
+
+assert C.sizeof(Material) == 32 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
+
+# 64-byte BVH node: child boxes lo0,hi0,lo1,hi1 then ref[2], cnt[2]
+NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3), ("hi1", "<f4", 3),
+                       ("ref", "<i4", 2), ("cnt", "<i4", 2)])
+assert NODE_DTYPE.itemsize == 64
+
+HIT_DTYPE = np.dtype([("t", "<f4"), ("u", "<f4"), ("v", "<f4"), ("prim_id", "<i4"), ("geom_id", "<i4"),
+                      ("inst", "<i4"), ("user_inst_id", "<i4"), ("obj_id", "<i4")])
+assert HIT_DTYPE.itemsize == 32
+
+
+def fptr(a: np.ndarray):
+    return a.ctypes.data_as(fp) if a is not None else None
+
+
+def iptr(a: np.ndarray):
+    return a.ctypes.data_as(ip) if a is not None else None
+
+
+def matte(color=(0.8, 0.8, 0.8, 1.0), from_attribute=True, opacity=1.0) -> Material:
+    m = Material()
+    m.type = BNX_MATERIAL_MATTE
+    m.color_from_attribute = 1 if from_attribute else 0
+    m.color = f4(*color)
+    m.opacity = opacity
+    return m
+
+
+def renderer(bg=(0.0, 0.0, 0.0, 1.0), ambient=1.0, flags=0, max_generations=0) -> Renderer:
+    r = Renderer()
+    r.bg_color = f4(*bg)
+    r.ambient_radiance = ambient
+    r.cut_plane = f4(0.0, 0.0, 0.0, -1e30)
+    r.flags = flags
+    r.max_generations = max_generations
+    return r
+
+
+def dir_lights(lights) -> "C.Array":
+    arr = (DirLight * max(1, len(lights)))()
+    for i, (d, rad, col) in enumerate(lights):
+        arr[i].direction = f3(*[float(x) for x in d])
+        arr[i].radiance = float(rad)
+        arr[i].color = f3(*[float(x) for x in col])
+    return arr

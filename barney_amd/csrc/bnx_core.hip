@@ -1,0 +1,833 @@
+// bnx_core.hip -- the C-ABI of include/bnx_rt.h: devices, buffers,
+// geometries, groups (BLAS/TLAS builds), worlds, frames and the frame loop.
+#include "bnx_internal.h"
+#include "bnx_render.h"
+
+#include <chrono>
+#include <cmath>
+#include <cstring>
+#include <new>
+
+namespace bnx {
+
+static thread_local std::string g_last_error = "no error";
+void set_error(const std::string &msg) { g_last_error = msg; }
+
+namespace {
+
+inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+// ------------------------------------------------------------ build prep
+struct GeomBuild {
+  int type;
+  int base;             // first flattened prim of this geometry
+  int num_prims;
+  int pad;
+  const float *vertices;
+  const int *indices;
+  const float4 *spheres;
+};
+
+__global__ void k_prim_boxes(const GeomBuild *gb, int num_geoms, int n, float4 *lo, float4 *hi, int *pgeom)
+{
+  int p = blockIdx.x * blockDim.x + threadIdx.x;
+  if (p >= n) return;
+  int g = 0;
+  while (g + 1 < num_geoms && gb[g + 1].base <= p) g++;
+  const GeomBuild G = gb[g];
+  int prim = p - G.base;
+  pgeom[p] = g;
+  if (G.type == 0) {
+    // createTriangleBounds (rtcore/cuda/Group.cu:91-112)
+    const int *ix = G.indices + 3 * prim;
+    float l[3] = {INFINITY, INFINITY, INFINITY}, h[3] = {-INFINITY, -INFINITY, -INFINITY};
+    for (int k = 0; k < 3; k++) {
+      const float *v = G.vertices + 3 * ix[k];
+      for (int a = 0; a < 3; a++) {
+        l[a] = fmn(l[a], v[a]);
+        h[a] = fmx(h[a], v[a]);
+      }
+    }
+    lo[p] = make_float4(l[0], l[1], l[2], 0.f);
+    hi[p] = make_float4(h[0], h[1], h[2], 0.f);
+  } else {
+    // SpheresPrograms::bounds (Spheres.dev.cu:29-40)
+    float4 s = G.spheres[prim];
+    lo[p] = make_float4(s.x - s.w, s.y - s.w, s.z - s.w, 0.f);
+    hi[p] = make_float4(s.x + s.w, s.y + s.w, s.z + s.w, 0.f);
+  }
+}
+
+// BVH-ordered primitive records (reorderPrims, rtcore/cuda/Group.cu:114-123)
+__global__ void k_prim_records(const GeomBuild *gb, const int *order, const int *pgeom, int n, float4 *prims,
+                               int2 *refs)
+{
+  int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  int p = order[s];
+  int g = pgeom[p];
+  const GeomBuild G = gb[g];
+  int prim = p - G.base;
+  if (G.type == 0) {
+    const int *ix = G.indices + 3 * prim;
+    const float *a = G.vertices + 3 * ix[0], *b = G.vertices + 3 * ix[1], *c = G.vertices + 3 * ix[2];
+    prims[3 * s + 0] = make_float4(a[0], a[1], a[2], __int_as_float(prim));
+    prims[3 * s + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], __int_as_float(g));
+    prims[3 * s + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
+  } else {
+    prims[s] = G.spheres[prim];
+    refs[s] = make_int2(g, prim);
+  }
+}
+
+__global__ void k_boxes_from_floats(const float *b, int n, float4 *lo, float4 *hi)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  lo[i] = make_float4(b[6 * i + 0], b[6 * i + 1], b[6 * i + 2], 0.f);
+  hi[i] = make_float4(b[6 * i + 3], b[6 * i + 4], b[6 * i + 5], 0.f);
+}
+
+template <typename T> int upload(Device *dev, DevMem &m, const T *host, size_t count)
+{
+  BNX_HIP(m.ensure(sizeof(T) * (count ? count : 1)));
+  if (count) BNX_HIP(hipMemcpyAsync(m.ptr, host, sizeof(T) * count, hipMemcpyHostToDevice, dev->stream));
+  return BNX_OK;
+}
+
+int build_blas(Group *g)
+{
+  Device *dev = g->dev;
+  std::vector<GeomBuild> gb;
+  int n = 0;
+  for (Geom *geom : g->geoms) {
+    GeomBuild b{};
+    b.type = geom->type;
+    b.base = n;
+    b.num_prims = geom->num_prims;
+    b.vertices = geom->vertices.as<float>();
+    b.indices = geom->indices.as<int>();
+    b.spheres = geom->spheres.as<float4>();
+    gb.push_back(b);
+    n += geom->num_prims;
+  }
+  DevMem d_gb, lo, hi, pgeom;
+  int rc;
+  if ((rc = upload(dev, d_gb, gb.data(), gb.size())) != BNX_OK) return rc;
+  BNX_HIP(lo.ensure(sizeof(float4) * (size_t)(n ? n : 1)));
+  BNX_HIP(hi.ensure(sizeof(float4) * (size_t)(n ? n : 1)));
+  BNX_HIP(pgeom.ensure(sizeof(int) * (size_t)(n ? n : 1)));
+  if (n) k_prim_boxes<<<nblk(n, 256), 256, 0, dev->stream>>>(d_gb.as<GeomBuild>(), (int)gb.size(), n, lo.as<float4>(),
+                                                             hi.as<float4>(), pgeom.as<int>());
+  if ((rc = build_bvh(dev, lo.as<float4>(), hi.as<float4>(), n, 4, g->bvh)) != BNX_OK) return rc;
+  size_t per = g->kind == GROUP_TRIANGLES ? 3 : 1;
+  BNX_HIP(g->prims.ensure(sizeof(float4) * per * (size_t)(n ? n : 1)));
+  BNX_HIP(g->refs.ensure(sizeof(int2) * (size_t)(n ? n : 1)));
+  if (n)
+    k_prim_records<<<nblk(n, 256), 256, 0, dev->stream>>>(d_gb.as<GeomBuild>(), g->bvh.order.as<int>(),
+                                                          pgeom.as<int>(), n, g->prims.as<float4>(),
+                                                          g->refs.as<int2>());
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  BNX_HIP(hipGetLastError());
+  g->built = true;
+  return BNX_OK;
+}
+
+// inverse of an affine (columns vx, vy, vz, p) in double precision
+void affine_inverse(const float *a, float *out)
+{
+  double m[3][3], p[3];
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) m[r][c] = a[3 * c + r];
+  for (int r = 0; r < 3; r++) p[r] = a[9 + r];
+  double det = m[0][0] * (m[1][1] * m[2][2] - m[1][2] * m[2][1]) - m[0][1] * (m[1][0] * m[2][2] - m[1][2] * m[2][0]) +
+               m[0][2] * (m[1][0] * m[2][1] - m[1][1] * m[2][0]);
+  double inv[3][3];
+  inv[0][0] = (m[1][1] * m[2][2] - m[1][2] * m[2][1]) / det;
+  inv[0][1] = (m[0][2] * m[2][1] - m[0][1] * m[2][2]) / det;
+  inv[0][2] = (m[0][1] * m[1][2] - m[0][2] * m[1][1]) / det;
+  inv[1][0] = (m[1][2] * m[2][0] - m[1][0] * m[2][2]) / det;
+  inv[1][1] = (m[0][0] * m[2][2] - m[0][2] * m[2][0]) / det;
+  inv[1][2] = (m[0][2] * m[1][0] - m[0][0] * m[1][2]) / det;
+  inv[2][0] = (m[1][0] * m[2][1] - m[1][1] * m[2][0]) / det;
+  inv[2][1] = (m[0][1] * m[2][0] - m[0][0] * m[2][1]) / det;
+  inv[2][2] = (m[0][0] * m[1][1] - m[0][1] * m[1][0]) / det;
+  for (int c = 0; c < 3; c++)
+    for (int r = 0; r < 3; r++) out[3 * c + r] = (float)inv[r][c];
+  for (int r = 0; r < 3; r++) {
+    double s = -(inv[r][0] * p[0] + inv[r][1] * p[1] + inv[r][2] * p[2]);
+    out[9 + r] = (float)s;
+  }
+}
+
+void to_rows(const float *a, float4 *rows)
+{
+  // columns (vx, vy, vz, p) -> rows (x: vx.x vy.x vz.x p.x, ...)
+  rows[0] = make_float4(a[0], a[3], a[6], a[9]);
+  rows[1] = make_float4(a[1], a[4], a[7], a[10]);
+  rows[2] = make_float4(a[2], a[5], a[8], a[11]);
+}
+
+bool is_identity(const float *a)
+{
+  static const float I[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+  for (int k = 0; k < 12; k++)
+    if (a[k] != I[k]) return false;
+  return true;
+}
+
+int build_tlas(Group *g)
+{
+  Device *dev = g->dev;
+  const int n = (int)g->groups.size();
+  std::vector<Group *> blas_groups;
+  std::vector<InstDev> insts(n);
+  std::vector<float4> lo(n ? n : 1), hi(n ? n : 1);
+  std::vector<GeomDev> geoms;
+  std::vector<int> geom_base;
+  g->max_blas_depth = 0;
+  for (int i = 0; i < n; i++) {
+    Group *bg = g->groups[i];
+    if (!bg || !bg->built || bg->kind == GROUP_INSTANCES) {
+      set_error("bnx_group_build: instance refers to a group that is not a built geometry group");
+      return BNX_ERR_STATE;
+    }
+    int bi = -1;
+    for (int k = 0; k < (int)blas_groups.size(); k++)
+      if (blas_groups[k] == bg) bi = k;
+    if (bi < 0) {
+      bi = (int)blas_groups.size();
+      blas_groups.push_back(bg);
+      geom_base.push_back((int)geoms.size());
+      for (Geom *gm : bg->geoms) {
+        GeomDev gd{};
+        gd.type = gm->type;
+        gd.user_id = gm->user_id;
+        gd.color_from_attr = gm->mat.color_from_attribute;
+        gd.num_prims = gm->num_prims;
+        gd.color = make_float4(gm->mat.color[0], gm->mat.color[1], gm->mat.color[2], gm->mat.color[3]);
+        gd.vertices = gm->vertices.as<float>();
+        gd.indices = gm->indices.as<int>();
+        gd.colors = gm->colors.ptr ? gm->colors.as<float>() : nullptr;
+        gd.spheres = gm->spheres.as<float4>();
+        geoms.push_back(gd);
+      }
+      g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.max_depth);
+    }
+    const float *x = &g->xfms[12 * i];
+    InstDev &I = insts[i];
+    I.blas = bi;
+    I.user_id = g->inst_ids[i];
+    I.identity = is_identity(x) ? 1 : 0;
+    I.geom_base = geom_base[bi];
+    float inv[12];
+    affine_inverse(x, inv);
+    to_rows(x, I.o2w);
+    to_rows(inv, I.w2o);
+    // computeInstanceBounds (rtcore/cuda/Group.cu:125-137)
+    const float *rl = bg->bvh.root_lo, *rh = bg->bvh.root_hi;
+    float bl[3] = {INFINITY, INFINITY, INFINITY}, bh[3] = {-INFINITY, -INFINITY, -INFINITY};
+    if (I.identity) {
+      for (int k = 0; k < 3; k++) { bl[k] = rl[k]; bh[k] = rh[k]; }
+    } else if (rl[0] <= rh[0]) {
+      for (int c = 0; c < 8; c++) {
+        float pc[3] = {(c & 1) ? rh[0] : rl[0], (c & 2) ? rh[1] : rl[1], (c & 4) ? rh[2] : rl[2]};
+        for (int r = 0; r < 3; r++) {
+          // ((vx*px + vy*py) + vz*pz) + p
+          float v = ((x[r] * pc[0] + x[3 + r] * pc[1]) + x[6 + r] * pc[2]) + x[9 + r];
+          bl[r] = std::min(bl[r], v);
+          bh[r] = std::max(bh[r], v);
+        }
+      }
+    }
+    lo[i] = make_float4(bl[0], bl[1], bl[2], 0.f);
+    hi[i] = make_float4(bh[0], bh[1], bh[2], 0.f);
+  }
+  std::vector<BlasDev> blas(blas_groups.size());
+  for (size_t k = 0; k < blas_groups.size(); k++) {
+    Group *bg = blas_groups[k];
+    blas[k].nodes = bg->bvh.nodes.as<Node>();
+    blas[k].prims = bg->prims.as<float4>();
+    blas[k].refs = bg->refs.as<int2>();
+    blas[k].type = bg->kind == GROUP_TRIANGLES ? BLAS_TRIANGLES : BLAS_SPHERES;
+    blas[k].num_prims = bg->bvh.num_prims;
+  }
+  DevMem d_lo, d_hi;
+  int rc;
+  if ((rc = upload(dev, d_lo, lo.data(), n)) != BNX_OK) return rc;
+  if ((rc = upload(dev, d_hi, hi.data(), n)) != BNX_OK) return rc;
+  if ((rc = build_bvh(dev, d_lo.as<float4>(), d_hi.as<float4>(), n, 1, g->bvh)) != BNX_OK) return rc;
+  if ((rc = upload(dev, g->d_insts, insts.data(), insts.size())) != BNX_OK) return rc;
+  if ((rc = upload(dev, g->d_blas, blas.data(), blas.size())) != BNX_OK) return rc;
+  if ((rc = upload(dev, g->d_geoms, geoms.data(), geoms.size())) != BNX_OK) return rc;
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  if (g->bvh.max_depth + 1 + g->max_blas_depth + 1 > trace_stack_capacity()) {
+    set_error("bnx_group_build: BVH deeper than the traversal stack");
+    return BNX_ERR_UNSUPPORTED;
+  }
+  g->scene.tlas_nodes = g->bvh.nodes.as<Node>();
+  g->scene.tlas_order = g->bvh.order.as<int>();
+  g->scene.insts = g->d_insts.as<InstDev>();
+  g->scene.blas = g->d_blas.as<BlasDev>();
+  g->scene.geoms = g->d_geoms.as<GeomDev>();
+  g->scene.num_insts = n;
+  g->built = true;
+  return BNX_OK;
+}
+
+Queues &queues_for(Device *dev)
+{
+  if (!dev->queues) dev->queues = new Queues;
+  return *(Queues *)dev->queues;
+}
+
+} // namespace
+
+int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp, bnx_render_stats *stats)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  Device *dev = w->dev;
+  Group *g = w->instances;
+  const int W = f->w, H = f->h;
+  const long np = (long)W * H;
+  int64_t rays = 0, gens = 0;
+  Queues &Q = queues_for(dev);
+  // samples in flight per wave: about 2M pixel-samples
+  int S = (int)std::max(1L, std::min({(long)spp, (2L << 20) / std::max(1L, np), 4096L}));
+  int rc;
+  if ((rc = Q.ensure((size_t)(2 * np * S))) != BNX_OK) return rc;
+  int *counts = Q.counts.as<int>();
+  unsigned long long *st = (unsigned long long *)(counts + 4);   // [0] shadow rays, [1..3] traversal counts
+  BNX_HIP(hipMemsetAsync(counts, 0, 64, dev->stream));
+  if (!Q.ev_begin) {
+    BNX_HIP(hipEventCreate(&Q.ev_begin));
+    BNX_HIP(hipEventCreate(&Q.ev_end));
+  }
+  double trace_ms = 0.0;
+  int64_t trace_launches = 0;
+  ShadeParams P{};
+  P.rr = rr;
+  P.dir_lights = w->d_dir_lights.as<bnx_dir_light>();
+  P.num_dir_lights = (int)w->dir_lights.size();
+  P.W = W;
+  P.H = H;
+  P.stats = st;
+  FrameDev fd = frame_dev(f);
+  const int max_gen = rr.max_generations > 0 ? rr.max_generations : 1 << 30;
+  for (int s0 = 0; s0 < spp; s0 += S) {
+    const int nS = std::min(S, spp - s0);
+    int cur = 0;
+    int count = (int)(np * nS);
+    if ((rc = launch_raygen(dev, cam, rr, W, H, f->accum_id, nS, Q.q(cur))) != BNX_OK) return rc;
+    BNX_HIP(hipMemcpyAsync(counts + cur, &count, sizeof(int), hipMemcpyHostToDevice, dev->stream));
+    P.accum_base = f->accum_id;
+    for (int gen = 0; count > 0 && gen < max_gen; gen++) {
+      RayQueue in = Q.q(cur), out = Q.q(1 - cur);
+      TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
+      BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
+      if ((rc = launch_trace_queue(g, tq, counts + cur, count, st + 1)) != BNX_OK) return rc;
+      BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
+      rays += count;
+      gens++;
+      BNX_HIP(hipMemsetAsync(counts + (1 - cur), 0, sizeof(int), dev->stream));
+      P.generation = gen;
+      if ((rc = launch_shade(dev, g->scene, P, in, counts + cur, count, out, counts + (1 - cur), fd)) != BNX_OK)
+        return rc;
+      BNX_HIP(hipMemcpyAsync(&count, counts + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, dev->stream));
+      BNX_HIP(hipStreamSynchronize(dev->stream));
+      float ems = 0.f;
+      BNX_HIP(hipEventElapsedTime(&ems, Q.ev_begin, Q.ev_end));
+      trace_ms += ems;
+      trace_launches++;
+      cur = 1 - cur;
+    }
+    f->accum_id += nS;
+  }
+  unsigned long long host_st[4] = {0, 0, 0, 0};
+  BNX_HIP(hipMemcpyAsync(host_st, st, sizeof(host_st), hipMemcpyDeviceToHost, dev->stream));
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  BNX_HIP(hipGetLastError());
+  if (stats) {
+    stats->rays_traced = rays;
+    stats->shadow_rays = (int64_t)host_st[0];
+    stats->generations = gens;
+    stats->volume_samples = 0;
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->node_visits = (int64_t)host_st[1];
+    stats->tri_tests = (int64_t)host_st[2];
+    stats->sphere_tests = (int64_t)host_st[3];
+    stats->trace_ms = trace_ms;
+    stats->trace_launches = trace_launches;
+  }
+  return BNX_OK;
+}
+
+} // namespace bnx
+
+using namespace bnx;
+
+struct bnx_device_s : Device {};
+struct bnx_buffer_s : Buffer {};
+struct bnx_geom_s : Geom {};
+struct bnx_group_s : Group {};
+struct bnx_world_s : World {};
+struct bnx_frame_s : Frame {};
+
+#define GUARD_BEGIN try {
+#define GUARD_END                                                                                                    \
+  }                                                                                                                  \
+  catch (const std::bad_alloc &) {                                                                                   \
+    set_error("host allocation failed");                                                                             \
+    return BNX_ERR_OOM;                                                                                              \
+  }                                                                                                                  \
+  catch (const std::exception &e) {                                                                                  \
+    set_error(e.what());                                                                                             \
+    return BNX_ERR_INVALID;                                                                                          \
+  }
+
+extern "C" {
+
+const char *bnx_last_error(void) { return g_last_error.c_str(); }
+int bnx_abi_version(void) { return BNX_ABI_VERSION; }
+
+int bnx_device_create(int gpu_id, bnx_device *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(out, BNX_ERR_INVALID, "bnx_device_create: null out");
+  int count = 0;
+  BNX_HIP(hipGetDeviceCount(&count));
+  BNX_CHECK(gpu_id >= 0 && gpu_id < count, BNX_ERR_INVALID, "bnx_device_create: no such GPU");
+  BNX_HIP(hipSetDevice(gpu_id));
+  bnx_device d = new bnx_device_s;
+  d->gpu = gpu_id;
+  BNX_HIP(hipStreamCreateWithFlags(&d->stream, hipStreamNonBlocking));
+  hipDeviceProp_t prop;
+  BNX_HIP(hipGetDeviceProperties(&prop, gpu_id));
+  d->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  *out = d;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_device_destroy(bnx_device d)
+{
+  BNX_CHECK(d, BNX_ERR_INVALID, "bnx_device_destroy: null device");
+  (void)hipStreamSynchronize(d->stream);
+  delete (Queues *)d->queues;
+  d->queues = nullptr;
+  (void)hipStreamDestroy(d->stream);
+  delete d;
+  return BNX_OK;
+}
+
+int bnx_device_sync(bnx_device d)
+{
+  BNX_CHECK(d, BNX_ERR_INVALID, "bnx_device_sync: null device");
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  return BNX_OK;
+}
+
+int bnx_device_mem_info(bnx_device d, size_t *free_bytes, size_t *total_bytes)
+{
+  BNX_CHECK(d && free_bytes && total_bytes, BNX_ERR_INVALID, "bnx_device_mem_info: null argument");
+  BNX_HIP(hipSetDevice(d->gpu));
+  BNX_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return BNX_OK;
+}
+
+// ------------------------------------------------------------------ buffers
+int bnx_buffer_create(bnx_device d, size_t bytes, const void *init, bnx_buffer *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && out, BNX_ERR_INVALID, "bnx_buffer_create: null argument");
+  bnx_buffer b = new bnx_buffer_s;
+  b->dev = d;
+  hipError_t e = b->mem.ensure(bytes);
+  if (e != hipSuccess) {
+    delete b;
+    set_error(std::string("bnx_buffer_create: ") + hipGetErrorString(e));
+    return BNX_ERR_OOM;
+  }
+  if (init && bytes) BNX_HIP(hipMemcpy(b->mem.ptr, init, bytes, hipMemcpyHostToDevice));
+  *out = b;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_buffer_upload(bnx_buffer b, const void *host, size_t bytes, size_t offset)
+{
+  BNX_CHECK(b && host, BNX_ERR_INVALID, "bnx_buffer_upload: null argument");
+  BNX_CHECK(offset + bytes <= b->mem.bytes, BNX_ERR_INVALID, "bnx_buffer_upload: out of range");
+  BNX_HIP(hipMemcpyAsync((char *)b->mem.ptr + offset, host, bytes, hipMemcpyHostToDevice, b->dev->stream));
+  BNX_HIP(hipStreamSynchronize(b->dev->stream));
+  return BNX_OK;
+}
+
+int bnx_buffer_download(bnx_buffer b, void *host, size_t bytes, size_t offset)
+{
+  BNX_CHECK(b && host, BNX_ERR_INVALID, "bnx_buffer_download: null argument");
+  BNX_CHECK(offset + bytes <= b->mem.bytes, BNX_ERR_INVALID, "bnx_buffer_download: out of range");
+  BNX_HIP(hipMemcpyAsync(host, (char *)b->mem.ptr + offset, bytes, hipMemcpyDeviceToHost, b->dev->stream));
+  BNX_HIP(hipStreamSynchronize(b->dev->stream));
+  return BNX_OK;
+}
+
+int bnx_buffer_destroy(bnx_buffer b)
+{
+  BNX_CHECK(b, BNX_ERR_INVALID, "bnx_buffer_destroy: null buffer");
+  (void)hipStreamSynchronize(b->dev->stream);
+  delete b;
+  return BNX_OK;
+}
+
+void *bnx_buffer_dptr(bnx_buffer b) { return b ? b->mem.ptr : nullptr; }
+size_t bnx_buffer_size(bnx_buffer b) { return b ? b->mem.bytes : 0; }
+
+// ------------------------------------------------------------------- geoms
+static void set_mat(Geom *g, const bnx_material &m) { g->mat = m; }
+
+int bnx_geom_create_triangles(bnx_device d, const bnx_triangles_desc *desc, bnx_geom *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && desc && out, BNX_ERR_INVALID, "bnx_geom_create_triangles: null argument");
+  BNX_CHECK(desc->num_triangles >= 0 && desc->num_vertices >= 0, BNX_ERR_INVALID, "negative count");
+  BNX_CHECK(desc->num_triangles == 0 || (desc->vertices && desc->indices), BNX_ERR_INVALID,
+            "bnx_geom_create_triangles: missing vertices or indices");
+  BNX_CHECK(desc->material.type == BNX_MATERIAL_MATTE, BNX_ERR_UNSUPPORTED, "only matte materials are provided");
+  BNX_CHECK(desc->material.opacity >= 1.f, BNX_ERR_UNSUPPORTED,
+            "stochastic opacity (Triangles.dev.cu:120-133) is not provided; opacity must be 1");
+  for (int i = 0; i < 3 * desc->num_triangles; i++)
+    BNX_CHECK(desc->indices[i] >= 0 && desc->indices[i] < desc->num_vertices, BNX_ERR_INVALID,
+              "bnx_geom_create_triangles: index out of range");
+  bnx_geom g = new bnx_geom_s;
+  g->dev = d;
+  g->type = 0;
+  g->user_id = desc->user_id;
+  set_mat(g, desc->material);
+  g->num_prims = desc->num_triangles;
+  g->num_vertices = desc->num_vertices;
+  int rc;
+  if ((rc = upload(d, g->vertices, desc->vertices, 3 * (size_t)desc->num_vertices)) != BNX_OK ||
+      (rc = upload(d, g->indices, desc->indices, 3 * (size_t)desc->num_triangles)) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  if (desc->prim_colors && (rc = upload(d, g->colors, desc->prim_colors, 3 * (size_t)desc->num_triangles)) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  *out = g;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_geom_create_spheres(bnx_device d, const bnx_spheres_desc *desc, bnx_geom *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && desc && out, BNX_ERR_INVALID, "bnx_geom_create_spheres: null argument");
+  BNX_CHECK(desc->num_spheres >= 0, BNX_ERR_INVALID, "negative count");
+  BNX_CHECK(desc->num_spheres == 0 || desc->centers, BNX_ERR_INVALID, "bnx_geom_create_spheres: missing centers");
+  BNX_CHECK(desc->material.type == BNX_MATERIAL_MATTE, BNX_ERR_UNSUPPORTED, "only matte materials are provided");
+  bnx_geom g = new bnx_geom_s;
+  g->dev = d;
+  g->type = 1;
+  g->user_id = desc->user_id;
+  set_mat(g, desc->material);
+  g->num_prims = desc->num_spheres;
+  g->num_vertices = 0;
+  std::vector<float4> sp((size_t)desc->num_spheres);
+  for (int i = 0; i < desc->num_spheres; i++)
+    sp[i] = make_float4(desc->centers[3 * i], desc->centers[3 * i + 1], desc->centers[3 * i + 2],
+                        desc->radii ? desc->radii[i] : desc->default_radius);
+  int rc;
+  if ((rc = upload(d, g->spheres, sp.data(), sp.size())) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  if (desc->colors && (rc = upload(d, g->colors, desc->colors, 3 * (size_t)desc->num_spheres)) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  *out = g;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_geom_destroy(bnx_geom g)
+{
+  BNX_CHECK(g, BNX_ERR_INVALID, "bnx_geom_destroy: null geom");
+  (void)hipStreamSynchronize(g->dev->stream);
+  delete g;
+  return BNX_OK;
+}
+
+// ------------------------------------------------------------------ groups
+static int create_geom_group(bnx_device d, const bnx_geom *geoms, int n, int kind, bnx_group *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && out && (n == 0 || geoms), BNX_ERR_INVALID, "group create: null argument");
+  bnx_group g = new bnx_group_s;
+  g->dev = d;
+  g->kind = kind;
+  for (int i = 0; i < n; i++) {
+    if (!geoms[i] || geoms[i]->type != (kind == GROUP_TRIANGLES ? 0 : 1)) {
+      delete g;
+      set_error("group create: geometry type does not match the group kind");
+      return BNX_ERR_INVALID;
+    }
+    g->geoms.push_back(geoms[i]);
+  }
+  *out = g;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_group_create_triangles(bnx_device d, const bnx_geom *geoms, int n, bnx_group *out)
+{
+  return create_geom_group(d, geoms, n, GROUP_TRIANGLES, out);
+}
+
+int bnx_group_create_user(bnx_device d, const bnx_geom *geoms, int n, bnx_group *out)
+{
+  return create_geom_group(d, geoms, n, GROUP_USER, out);
+}
+
+int bnx_group_create_instances(bnx_device d, const bnx_group *groups, const int32_t *inst_ids, const float *xfms,
+                               int n, bnx_group *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && out && (n == 0 || groups), BNX_ERR_INVALID, "bnx_group_create_instances: null argument");
+  bnx_group g = new bnx_group_s;
+  g->dev = d;
+  g->kind = GROUP_INSTANCES;
+  for (int i = 0; i < n; i++) {
+    g->groups.push_back(groups[i]);
+    g->inst_ids.push_back(inst_ids ? inst_ids[i] : i);
+    static const float I[12] = {1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0};
+    const float *x = xfms ? xfms + 12 * i : I;
+    g->xfms.insert(g->xfms.end(), x, x + 12);
+  }
+  *out = g;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_group_build(bnx_group g)
+{
+  GUARD_BEGIN
+  BNX_CHECK(g, BNX_ERR_INVALID, "bnx_group_build: null group");
+  BNX_HIP(hipSetDevice(g->dev->gpu));
+  if (g->kind == GROUP_INSTANCES) return build_tlas(g);
+  return build_blas(g);
+  GUARD_END
+}
+
+int bnx_group_bvh_info(bnx_group g, bnx_bvh_info *info)
+{
+  BNX_CHECK(g && info, BNX_ERR_INVALID, "bnx_group_bvh_info: null argument");
+  BNX_CHECK(g->built, BNX_ERR_STATE, "bnx_group_bvh_info: group not built");
+  info->num_nodes = g->bvh.num_nodes;
+  info->num_prims = g->bvh.num_prims;
+  info->max_depth = g->bvh.max_depth;
+  info->root_is_leaf = g->bvh.num_prims <= (g->kind == GROUP_INSTANCES ? 1 : 4) ? 1 : 0;
+  for (int k = 0; k < 3; k++) {
+    info->root_lower[k] = g->bvh.root_lo[k];
+    info->root_upper[k] = g->bvh.root_hi[k];
+  }
+  info->build_ms = g->bvh.build_ms;
+  return BNX_OK;
+}
+
+int bnx_group_download_bvh(bnx_group g, void *nodes, int32_t *order)
+{
+  BNX_CHECK(g, BNX_ERR_INVALID, "bnx_group_download_bvh: null group");
+  BNX_CHECK(g->built, BNX_ERR_STATE, "bnx_group_download_bvh: group not built");
+  if (nodes)
+    BNX_HIP(hipMemcpy(nodes, g->bvh.nodes.ptr, sizeof(Node) * (size_t)g->bvh.num_nodes, hipMemcpyDeviceToHost));
+  if (order && g->bvh.num_prims)
+    BNX_HIP(hipMemcpy(order, g->bvh.order.ptr, sizeof(int) * (size_t)g->bvh.num_prims, hipMemcpyDeviceToHost));
+  return BNX_OK;
+}
+
+int bnx_group_destroy(bnx_group g)
+{
+  BNX_CHECK(g, BNX_ERR_INVALID, "bnx_group_destroy: null group");
+  (void)hipStreamSynchronize(g->dev->stream);
+  delete g;
+  return BNX_OK;
+}
+
+int bnx_bvh_build_boxes(bnx_device d, const float *d_boxes, int n, int max_leaf, void *nodes_host,
+                        int32_t *order_host, bnx_bvh_info *info)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && (n == 0 || d_boxes) && n >= 0 && max_leaf >= 1 && max_leaf <= 7, BNX_ERR_INVALID,
+            "bnx_bvh_build_boxes: bad argument");
+  DevMem lo, hi;
+  BNX_HIP(lo.ensure(sizeof(float4) * (size_t)(n ? n : 1)));
+  BNX_HIP(hi.ensure(sizeof(float4) * (size_t)(n ? n : 1)));
+  if (n) k_boxes_from_floats<<<nblk(n, 256), 256, 0, d->stream>>>(d_boxes, n, lo.as<float4>(), hi.as<float4>());
+  Bvh bvh;
+  int rc = build_bvh(d, lo.as<float4>(), hi.as<float4>(), n, max_leaf, bvh);
+  if (rc != BNX_OK) return rc;
+  if (nodes_host)
+    BNX_HIP(hipMemcpy(nodes_host, bvh.nodes.ptr, sizeof(Node) * (size_t)bvh.num_nodes, hipMemcpyDeviceToHost));
+  if (order_host && n) BNX_HIP(hipMemcpy(order_host, bvh.order.ptr, sizeof(int) * (size_t)n, hipMemcpyDeviceToHost));
+  if (info) {
+    info->num_nodes = bvh.num_nodes;
+    info->num_prims = n;
+    info->max_depth = bvh.max_depth;
+    info->root_is_leaf = n <= max_leaf;
+    for (int k = 0; k < 3; k++) {
+      info->root_lower[k] = bvh.root_lo[k];
+      info->root_upper[k] = bvh.root_hi[k];
+    }
+    info->build_ms = bvh.build_ms;
+  }
+  return BNX_OK;
+  GUARD_END
+}
+
+// ------------------------------------------------------------------- trace
+int bnx_trace(bnx_group g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags)
+{
+  BNX_CHECK(g && n >= 0 && (n == 0 || (d_rays && d_hits)), BNX_ERR_INVALID, "bnx_trace: bad argument");
+  BNX_CHECK(g->kind == GROUP_INSTANCES && g->built, BNX_ERR_STATE, "bnx_trace: needs a built instance group");
+  if (n == 0) return BNX_OK;
+  return launch_trace_api(g, d_rays, n, d_hits, flags);
+}
+
+// ------------------------------------------------------------------- world
+int bnx_world_create(bnx_device d, bnx_group instances, const bnx_dir_light *lights, int n, bnx_world *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && instances && out && n >= 0 && (n == 0 || lights), BNX_ERR_INVALID, "bnx_world_create: bad argument");
+  BNX_CHECK(instances->kind == GROUP_INSTANCES && instances->built, BNX_ERR_STATE,
+            "bnx_world_create: needs a built instance group");
+  bnx_world w = new bnx_world_s;
+  w->dev = d;
+  w->instances = instances;
+  w->dir_lights.assign(lights, lights + n);
+  int rc = upload(d, w->d_dir_lights, w->dir_lights.data(), w->dir_lights.size());
+  if (rc != BNX_OK) {
+    delete w;
+    return rc;
+  }
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  *out = w;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_world_destroy(bnx_world w)
+{
+  BNX_CHECK(w, BNX_ERR_INVALID, "bnx_world_destroy: null world");
+  delete w;
+  return BNX_OK;
+}
+
+// ------------------------------------------------------------------ camera
+int bnx_camera_perspective(const float position[3], const float direction[3], const float up[3], float fovy_deg,
+                           bnx_camera *out)
+{
+  BNX_CHECK(position && direction && up && out, BNX_ERR_INVALID, "bnx_camera_perspective: null argument");
+  // PerspectiveCamera::commit (native/Camera.cu:82-101), host float math
+  auto nrm = [](const float *v, float *o) {
+    float l = 1.f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+    o[0] = v[0] * l;
+    o[1] = v[1] * l;
+    o[2] = v[2] * l;
+  };
+  auto crs = [](const float *a, const float *b, float *o) {
+    o[0] = a[1] * b[2] - a[2] * b[1];
+    o[1] = a[2] * b[0] - a[0] * b[2];
+    o[2] = a[0] * b[1] - a[1] * b[0];
+  };
+  float d00[3], du[3], dv[3], t[3];
+  nrm(direction, d00);
+  crs(d00, up, t);
+  nrm(t, du);
+  crs(du, d00, t);
+  nrm(t, dv);
+  float half_angle = ((0.5f * fovy_deg) * (float)M_PI) / 180.0f;
+  float sc = (float)(1.f / (2.0f * tanf(half_angle)));
+  memset(out, 0, sizeof(*out));
+  out->type = BNX_CAMERA_PERSPECTIVE;
+  for (int k = 0; k < 3; k++) {
+    out->dir_00[k] = d00[k] * sc;
+    out->dir_du[k] = du[k];
+    out->dir_dv[k] = dv[k];
+    out->lens_00[k] = position[k];
+  }
+  out->aperture_radius = 0.f;
+  out->focus_distance = 1.f;
+  return BNX_OK;
+}
+
+// ------------------------------------------------------------------- frame
+int bnx_frame_create(bnx_device d, int w, int h, bnx_frame *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && out && w > 0 && h > 0, BNX_ERR_INVALID, "bnx_frame_create: bad argument");
+  bnx_frame f = new bnx_frame_s;
+  f->dev = d;
+  f->w = w;
+  f->h = h;
+  size_t np = (size_t)w * h;
+  hipError_t e;
+  if ((e = f->accum.ensure(32 * np)) != hipSuccess || (e = f->depth.ensure(4 * np)) != hipSuccess ||
+      (e = f->prim.ensure(4 * np)) != hipSuccess || (e = f->inst.ensure(4 * np)) != hipSuccess ||
+      (e = f->obj.ensure(4 * np)) != hipSuccess || (e = f->normal.ensure(16 * np)) != hipSuccess) {
+    delete f;
+    set_error(std::string("bnx_frame_create: ") + hipGetErrorString(e));
+    return BNX_ERR_OOM;
+  }
+  int rc = frame_clear(f);
+  if (rc != BNX_OK) {
+    delete f;
+    return rc;
+  }
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  *out = f;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_frame_reset(bnx_frame f)
+{
+  BNX_CHECK(f, BNX_ERR_INVALID, "bnx_frame_reset: null frame");
+  return frame_clear(f);
+}
+
+int bnx_frame_read(bnx_frame f, int channel, void *host)
+{
+  BNX_CHECK(f && host, BNX_ERR_INVALID, "bnx_frame_read: null argument");
+  return frame_read(f, channel, host);
+}
+
+int bnx_frame_accum_id(bnx_frame f) { return f ? f->accum_id : -1; }
+
+int bnx_frame_destroy(bnx_frame f)
+{
+  BNX_CHECK(f, BNX_ERR_INVALID, "bnx_frame_destroy: null frame");
+  (void)hipStreamSynchronize(f->dev->stream);
+  delete f;
+  return BNX_OK;
+}
+
+// ------------------------------------------------------------------ render
+int bnx_render(bnx_world w, const bnx_camera *cam, const bnx_renderer *r, bnx_frame f, int spp,
+               bnx_render_stats *stats)
+{
+  GUARD_BEGIN
+  BNX_CHECK(w && cam && r && f && spp >= 0, BNX_ERR_INVALID, "bnx_render: bad argument");
+  BNX_CHECK(f->dev == w->dev, BNX_ERR_INVALID, "bnx_render: frame and world live on different devices");
+  BNX_CHECK(cam->type == BNX_CAMERA_PERSPECTIVE && cam->aperture_radius == 0.f, BNX_ERR_UNSUPPORTED,
+            "bnx_render: only the pinhole perspective camera is provided");
+  return render_frame(w, *cam, *r, f, spp, stats);
+  GUARD_END
+}
+
+} // extern "C"

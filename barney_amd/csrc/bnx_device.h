@@ -1,0 +1,189 @@
+// bnx_device.h -- device-side types and math of the gfx950 path.
+//
+// Float arithmetic is written operation by operation (the library is built
+// with -ffp-contract=off and correctly rounded division / square root), so
+// every hit distance, position and normal is bit-identical to a scalar CPU
+// evaluation of the same expressions.
+#pragma once
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/bnx_rt.h"
+#include "bnx_numeric.h"
+
+#define BNX_D __device__ __forceinline__
+
+namespace bnx {
+
+// ------------------------------------------------------------------ vec3
+struct v3 {
+  float x, y, z;
+};
+BNX_D v3 mk(float a, float b, float c) { return v3{a, b, c}; }
+BNX_D v3 splat(float a) { return v3{a, a, a}; }
+BNX_D v3 operator+(v3 a, v3 b) { return v3{a.x + b.x, a.y + b.y, a.z + b.z}; }
+BNX_D v3 operator-(v3 a, v3 b) { return v3{a.x - b.x, a.y - b.y, a.z - b.z}; }
+BNX_D v3 operator-(v3 a) { return v3{-a.x, -a.y, -a.z}; }
+BNX_D v3 operator*(v3 a, v3 b) { return v3{a.x * b.x, a.y * b.y, a.z * b.z}; }
+BNX_D v3 operator*(v3 a, float s) { return v3{a.x * s, a.y * s, a.z * s}; }
+BNX_D v3 operator*(float s, v3 a) { return v3{s * a.x, s * a.y, s * a.z}; }
+BNX_D v3 operator/(v3 a, float s) { return v3{a.x / s, a.y / s, a.z / s}; }
+BNX_D bool operator==(v3 a, v3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+BNX_D float fmn(float a, float b) { return a < b ? a : b; }
+BNX_D float fmx(float a, float b) { return a > b ? a : b; }
+BNX_D v3 vmin(v3 a, v3 b) { return v3{fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)}; }
+BNX_D float dot(v3 a, v3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+BNX_D v3 cross(v3 a, v3 b) { return v3{a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x}; }
+BNX_D float length(v3 a) { return sqrtf(dot(a, a)); }
+BNX_D v3 normalize(v3 a) { return a * (1.f / sqrtf(dot(a, a))); }
+BNX_D float reduce_max(v3 a) { return fmx(fmx(a.x, a.y), a.z); }
+BNX_D v3 vabs(v3 a) { return v3{fabsf(a.x), fabsf(a.y), fabsf(a.z)}; }
+BNX_D v3 xyz(float4 f) { return v3{f.x, f.y, f.z}; }
+BNX_D v3 ld3(const float *p) { return v3{p[0], p[1], p[2]}; }
+
+// direction fix of traceRays.dev.cu:41-43 and rtcore/cuda/TraceInterface.h:37-39
+BNX_D float fix1(float d)
+{
+  if (d == 0.f) d = 1e-6f;
+  if (fabsf(d) < 1e-6f) d = 1e-6f;
+  return d;
+}
+BNX_D v3 fix_dir(v3 d) { return v3{fix1(d.x), fix1(d.y), fix1(d.z)}; }
+
+// --------------------------------------------------------------- half
+// vec3h storage (render/Ray.h:17-23,76): hardware IEEE RNE conversion
+BNX_D uint32_t f2h(float f)
+{
+  _Float16 h = (_Float16)f;
+  return (uint32_t)__builtin_bit_cast(unsigned short, h);
+}
+BNX_D float h2f(uint32_t b)
+{
+  _Float16 h = __builtin_bit_cast(_Float16, (unsigned short)(b & 0xffffu));
+  return (float)h;
+}
+BNX_D v3 quant_half(v3 v) { return v3{h2f(f2h(v.x)), h2f(f2h(v.y)), h2f(f2h(v.z))}; }
+
+// ------------------------------------------------------------------ RNG
+// RNGSeed / Random2 (native/common/barney-common.h:21-67)
+BNX_D uint32_t rng_next(uint64_t &state)
+{
+  uint64_t x = state;
+  unsigned count = (unsigned)(x >> 61);
+  state = x * 6364136223846793005ull;
+  x ^= x >> 22;
+  return (uint32_t)(x >> (22 + count));
+}
+BNX_D void rng_seed(uint64_t &state, uint32_t a, uint32_t b)
+{
+  state ^= (uint64_t)(uint32_t)(2u * a + 1u);
+  rng_next(state);
+  state ^= (uint64_t)(uint32_t)(2u * b);
+  rng_next(state);
+}
+BNX_D uint32_t rng_next_h(uint64_t &state, uint64_t h)
+{
+  state ^= (h << 1);
+  return rng_next(state);
+}
+struct Random2 {
+  uint64_t state;
+  BNX_D Random2(uint32_t a, uint32_t b) : state(0xcafef00dd15ea5e5ull) { rng_seed(state, a, b); }
+  BNX_D Random2(uint64_t &parent, uint32_t b)
+  {
+    state = parent;
+    rng_next(parent);
+    rng_next_h(state, (uint64_t)(b ? b : 290374u));
+  }
+  BNX_D float operator()() { return (float)(rng_next(state) & 0x00FFFFFFu) * (1.f / (float)0x01000000); }
+};
+
+// ------------------------------------------------------------------ BVH
+// 64-byte node: boxes of both children plus refs (see bnx_group_download_bvh)
+struct __align__(16) Node {
+  float lo0x, lo0y, lo0z, hi0x;
+  float hi0y, hi0z, lo1x, lo1y;
+  float lo1z, hi1x, hi1y, hi1z;
+  int ref0, ref1, cnt0, cnt1;
+};
+static_assert(sizeof(Node) == 64, "node");
+
+// traversal encoding of a child: inner node -> its index (< 2^30);
+// leaf -> 0x80000000 | first << 3 | count
+constexpr uint32_t LEAF_BIT = 0x80000000u;
+constexpr uint32_t LEAVE_BLAS = 0x7fffffffu;
+BNX_D uint32_t enc_child(int ref, int cnt)
+{
+  return cnt > 0 ? (LEAF_BIT | ((uint32_t)ref << 3) | (uint32_t)cnt) : (uint32_t)ref;
+}
+
+// ----------------------------------------------------------- scene data
+enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1 };
+
+struct BlasDev {
+  const Node *nodes;
+  const float4 *prims;    // triangles: 3 per slot (v0|prim, e1|geom, e2); spheres: (c, r)
+  const int2 *refs;       // spheres: (geom, prim) per slot
+  int type;
+  int num_prims;
+};
+
+struct InstDev {
+  float4 w2o[3];          // rows of the world->object affine (x,y,z,p)
+  float4 o2w[3];          // rows of the object->world affine
+  int blas;
+  int user_id;
+  int identity;
+  int geom_base;          // first entry of this instance's group in the geometry table
+};
+
+// per-geometry shading data
+struct GeomDev {
+  int type;               // 0 triangles, 1 spheres
+  int user_id;
+  int color_from_attr;
+  int num_prims;
+  float4 color;           // constant material color
+  const float *vertices;  // triangles: float3
+  const int *indices;     //            int3
+  const float *colors;    // per-prim float3 or null
+  const float4 *spheres;  // spheres: (center, radius)
+};
+
+struct SceneDev {
+  const Node *tlas_nodes;
+  const int *tlas_order;
+  const InstDev *insts;
+  const BlasDev *blas;
+  const GeomDev *geoms;
+  int num_insts;
+};
+
+BNX_D v3 xfm_point_rows(const float4 *m, v3 p)
+{
+  // rows (a,b,c,d): x' = a*px + b*py + c*pz + d, grouping as owl xfmPoint:
+  // ((vx*px + vy*py) + vz*pz) + p per component
+  return v3{((m[0].x * p.x + m[0].y * p.y) + m[0].z * p.z) + m[0].w,
+            ((m[1].x * p.x + m[1].y * p.y) + m[1].z * p.z) + m[1].w,
+            ((m[2].x * p.x + m[2].y * p.y) + m[2].z * p.z) + m[2].w};
+}
+BNX_D v3 xfm_vector_rows(const float4 *m, v3 v)
+{
+  return v3{(m[0].x * v.x + m[0].y * v.y) + m[0].z * v.z, (m[1].x * v.x + m[1].y * v.y) + m[1].z * v.z,
+            (m[2].x * v.x + m[2].y * v.y) + m[2].z * v.z};
+}
+
+// ------------------------------------------------------- ordered floats
+BNX_D int f2o(float f)
+{
+  int i = __float_as_int(f);
+  return i >= 0 ? i : (i ^ 0x7fffffff);
+}
+BNX_D float o2f(int i) { return __int_as_float(i >= 0 ? i : (i ^ 0x7fffffff)); }
+
+// ----------------------------------------------------------- wave helpers
+BNX_D int lane_id() { return __lane_id(); }
+BNX_D uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+
+} // namespace bnx

@@ -1,0 +1,146 @@
+// bnx_internal.h -- host-side objects behind the C-ABI handles.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <string>
+#include <vector>
+
+#include "../../include/bnx_rt.h"
+#include "bnx_device.h"
+
+namespace bnx {
+
+void set_error(const std::string &msg);
+
+#define BNX_HIP(call)                                                                 \
+  do {                                                                                \
+    hipError_t e_ = (call);                                                           \
+    if (e_ != hipSuccess) {                                                           \
+      ::bnx::set_error(std::string(#call) + ": " + hipGetErrorString(e_));             \
+      return BNX_ERR_HIP;                                                             \
+    }                                                                                 \
+  } while (0)
+
+#define BNX_CHECK(cond, code, msg)                                                    \
+  do {                                                                                \
+    if (!(cond)) {                                                                    \
+      ::bnx::set_error(msg);                                                          \
+      return code;                                                                    \
+    }                                                                                 \
+  } while (0)
+
+// device memory owned by a C++ object
+struct DevMem {
+  void *ptr = nullptr;
+  size_t bytes = 0;
+  ~DevMem() { release(); }
+  void release()
+  {
+    if (ptr) (void)hipFree(ptr);
+    ptr = nullptr;
+    bytes = 0;
+  }
+  // (re)allocate if smaller than needed; contents are not preserved
+  hipError_t ensure(size_t b)
+  {
+    if (b <= bytes && ptr) return hipSuccess;
+    release();
+    hipError_t e = hipMalloc(&ptr, b ? b : 16);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <typename T> T *as() const { return (T *)ptr; }
+};
+
+struct Device;
+
+// scratch arrays of the BVH builder, kept per device and grown on demand
+struct BuildScratch {
+  DevMem lo, hi, order, order_tmp, slot_task, flags, scan, cub_tmp;
+  DevMem tasks, tasks_next, bins, big_map, small_tasks, counters, bounds, split;
+};
+
+struct Device {
+  int gpu = 0;
+  hipStream_t stream = nullptr;
+  int num_cus = 0;
+  BuildScratch build;
+  DevMem counters;      // small device counters (trace work queue etc.)
+  void *queues = nullptr;   // render queues (bnx_render.h Queues)
+};
+
+struct Buffer {
+  Device *dev;
+  DevMem mem;
+};
+
+struct Geom {
+  Device *dev;
+  int type;             // 0 triangles, 1 spheres
+  int user_id;
+  bnx_material mat;
+  int num_prims;
+  int num_vertices;
+  DevMem vertices, indices, colors, spheres;
+};
+
+struct Bvh {
+  DevMem nodes;         // Node[num_nodes] (capacity max(1,n))
+  DevMem order;         // int[num_prims]
+  int num_nodes = 0;
+  int num_prims = 0;
+  int max_depth = 0;
+  float root_lo[3] = {0, 0, 0}, root_hi[3] = {0, 0, 0};
+  double build_ms = 0.0;
+};
+
+enum GroupKind { GROUP_TRIANGLES = 0, GROUP_USER = 1, GROUP_INSTANCES = 2 };
+
+struct Group {
+  Device *dev;
+  int kind;
+  std::vector<Geom *> geoms;
+  // BLAS data
+  Bvh bvh;
+  DevMem prims, refs;   // BVH-ordered primitive records
+  bool built = false;
+  // instance-group data
+  std::vector<Group *> groups;
+  std::vector<int> inst_ids;
+  std::vector<float> xfms;   // 12 floats per instance, owl affine3f layout
+  DevMem d_insts, d_blas, d_geoms;
+  SceneDev scene{};
+  int max_blas_depth = 0;
+};
+
+struct World {
+  Device *dev;
+  Group *instances;
+  std::vector<bnx_dir_light> dir_lights;
+  DevMem d_dir_lights;
+};
+
+struct Frame {
+  Device *dev;
+  int w, h;
+  int accum_id = 0;
+  DevMem accum;         // int64 x 4 per pixel, 2^-32 fixed point
+  DevMem depth, prim, inst, obj, normal;
+  DevMem out;           // staging for reads
+};
+
+// builder entry (bnx_bvh.hip): boxes in d_lo/d_hi (float4, n); returns host info
+int build_bvh(Device *dev, const float4 *d_lo, const float4 *d_hi, int n, int max_leaf, Bvh &out);
+
+// traversal entry (bnx_trace.hip)
+struct TraceQueue {
+  float4 *org;          // org.xyz, tmax (tmax replaced by hit t on hit)
+  const float4 *dir;    // dir.xyz, flags (bit 0: shadow ray -> occlusion query)
+  int4 *hit;            // prim, geom, inst, bits(u)
+  float *hit_v;         // v
+};
+int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
+int launch_trace_queue(Group *g, const TraceQueue &q, const int *d_count, int max_n, unsigned long long *counters);
+
+} // namespace bnx

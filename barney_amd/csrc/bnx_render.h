@@ -1,0 +1,89 @@
+// bnx_render.h -- wavefront queue, shading parameters and frame views.
+#pragma once
+
+#include "bnx_internal.h"
+
+namespace bnx {
+
+// structure-of-arrays ray queue (Ray + PathState + hit of render/Ray.h:17-93)
+struct RayQueue {
+  float4 *org;          // org.xyz, tMax (hit t after trace)
+  float4 *dir;          // dir.xyz, flags: bit0 shadow, bits 16.. sample index in the batch
+  uint64_t *rng;        // RNGSeed state
+  uint4 *ps;            // throughput (3 x fp16), diffuse bounces, pixel, misWeight
+  int4 *hit;            // prim, geom, inst, bits(u)
+  float *hit_v;
+};
+
+struct ShadeParams {
+  bnx_renderer rr;
+  const bnx_dir_light *dir_lights;
+  int num_dir_lights;
+  int W, H;
+  int accum_base;
+  int generation;
+  unsigned long long *stats;   // [0] shadow rays written
+};
+
+struct FrameDev {
+  unsigned long long *accum;   // 4 x int64 per pixel, 2^-32 fixed point
+  int *depth;                  // ordered-int float
+  int *prim, *inst, *obj;
+  float4 *normal;
+};
+
+inline FrameDev frame_dev(Frame *f)
+{
+  FrameDev d;
+  d.accum = f->accum.as<unsigned long long>();
+  d.depth = f->depth.as<int>();
+  d.prim = f->prim.as<int>();
+  d.inst = f->inst.as<int>();
+  d.obj = f->obj.as<int>();
+  d.normal = f->normal.as<float4>();
+  return d;
+}
+
+// per-device ray queues (two, ping-pong) and counters
+struct Queues {
+  DevMem org[2], dir[2], rng[2], ps[2], hit[2], hit_v[2];
+  DevMem counts;      // [0], [1]: queue counts; [4..]: statistics
+  size_t cap = 0;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  int ensure(size_t n)
+  {
+    if (n <= cap) return BNX_OK;
+    for (int k = 0; k < 2; k++) {
+      BNX_HIP(org[k].ensure(sizeof(float4) * n));
+      BNX_HIP(dir[k].ensure(sizeof(float4) * n));
+      BNX_HIP(rng[k].ensure(sizeof(uint64_t) * n));
+      BNX_HIP(ps[k].ensure(sizeof(uint4) * n));
+      BNX_HIP(hit[k].ensure(sizeof(int4) * n));
+      BNX_HIP(hit_v[k].ensure(sizeof(float) * n));
+    }
+    BNX_HIP(counts.ensure(64));
+    cap = n;
+    return BNX_OK;
+  }
+  RayQueue q(int k)
+  {
+    RayQueue r;
+    r.org = org[k].as<float4>();
+    r.dir = dir[k].as<float4>();
+    r.rng = rng[k].as<uint64_t>();
+    r.ps = ps[k].as<uint4>();
+    r.hit = hit[k].as<int4>();
+    r.hit_v = hit_v[k].as<float>();
+    return r;
+  }
+};
+
+int frame_clear(Frame *f);
+int frame_read(Frame *f, int channel, void *host);
+int launch_raygen(Device *dev, const bnx_camera &cam, const bnx_renderer &rr, int W, int H, int accum_base, int S,
+                  const RayQueue &q);
+int launch_shade(Device *dev, const SceneDev &S, const ShadeParams &P, const RayQueue &in, const int *in_count,
+                 int max_in, const RayQueue &out, int *out_count, const FrameDev &fb);
+int trace_stack_capacity();
+
+} // namespace bnx

@@ -1,0 +1,591 @@
+// bnx_render.hip -- wavefront frame loop on gfx950: ray generation, shade
+// (hit programs + path-tracing bounce), accumulation and frame read-back.
+//
+// Replaces Context::renderTiles (native/Context.cpp:139-182) with
+// _generateRays (kernels/generateRays.cu:42-245), _shadeRays
+// (kernels/shadeRays.cu:883-1041) and linearizeColorAndNormalKernel
+// (fb/TiledFB.cu:12-52).
+//
+// Layout: the ray queue is structure-of-arrays (16-byte records per
+// field), so each wave reads and writes whole cache lines; several samples
+// per pixel are in flight in one queue (slot = sample * pixels + pixel at
+// generation 0).  Shade appends its shadow and continuation rays with one
+// atomic per wave (ballot + popcount prefix).  The frame accumulates in
+// 2^-32 fixed point with 64-bit integer atomics, so a pixel's sum does not
+// depend on the order in which rays of different samples or requests
+// arrive.
+#include "bnx_internal.h"
+#include "bnx_render.h"
+
+#include <chrono>
+
+namespace bnx {
+namespace {
+
+constexpr float BNX_INF = __builtin_huge_valf();
+constexpr int MAX_PATH_DEPTH = 10;      // shadeRays.cu:34
+constexpr int MAX_DIFFUSE_BOUNCES = 1;  // shadeRays.cu:17
+constexpr float CLAMP_F_R = 13.f;       // shadeRays.cu:27
+constexpr float ONE_PI = 3.14159265358979323846f;
+constexpr float TWO_PI = 2.f * ONE_PI;
+constexpr float FOUR_PI = 4.f * ONE_PI;
+constexpr float ONE_OVER_PI = 1.f / ONE_PI;
+constexpr float ONE_OVER_TWO_PI = 1.f / TWO_PI;
+constexpr float ONE_OVER_FOUR_PI = 1.f / FOUR_PI;
+
+// ---------------------------------------------------------------- raygen
+__global__ void k_raygen(bnx_camera cam, bnx_renderer rr, int W, int H, int accum_base, int S, RayQueue q)
+{
+  int idx = blockIdx.x * blockDim.x + threadIdx.x;
+  int np = W * H;
+  if (idx >= np * S) return;
+  int s = idx / np, pix = idx - s * np;
+  int ix = pix % W, iy = pix / W;
+  int accumID = accum_base + s;
+  Random2 rnd((uint32_t)(ix + W * accumID), (uint32_t)(iy + H * accumID));
+  uint64_t seed = 0xcafef00dd15ea5e5ull;
+  rng_seed(seed, (uint32_t)(ix + accumID * W), (uint32_t)iy);
+  float pu = (accumID == 0) ? .5f : rnd();
+  float pv = (accumID == 0) ? .5f : rnd();
+  float image_u = ((float)ix + pu) / (float)W;
+  float image_v = ((float)iy + pv) / (float)H;
+  float aspect = (float)W / (float)H;
+  v3 org = ld3(cam.lens_00);
+  v3 d = (ld3(cam.dir_00) + ((1.f * aspect) * (image_u - .5f)) * ld3(cam.dir_du)) +
+         (1.f * (image_v - .5f)) * ld3(cam.dir_dv);
+  v3 dir = normalize(d);
+  float tmax = 1e30f;
+  if (rr.cut_plane[3] > -1e28f) {
+    v3 N = ld3(rr.cut_plane);
+    float dist_org = dot(N, org) + rr.cut_plane[3];
+    float denom = dot(N, dir);
+    if (dist_org >= 0.f) {
+      if (denom < 0.f) tmax = fmn(tmax, -dist_org / denom);
+    } else {
+      if (denom > 0.f) org = org + (-dist_org / denom) * dir;
+      else tmax = 0.f;
+    }
+  }
+  q.org[idx] = make_float4(org.x, org.y, org.z, tmax);
+  q.dir[idx] = make_float4(dir.x, dir.y, dir.z, __uint_as_float((uint32_t)s << 16));
+  q.rng[idx] = seed;
+  uint32_t one = f2h(1.f);
+  q.ps[idx] = make_uint4(one | (one << 16), one, (uint32_t)pix, __float_as_uint(0.f));
+}
+
+// ----------------------------------------------------------------- shade
+struct SurfHit {
+  v3 P, N;          // N: fp16-rounded world normal
+  v3 albedo;
+};
+
+BNX_D void onb(v3 n, v3 &b1, v3 &b2)
+{
+  // Duff et al., "Building an Orthonormal Basis, Revisited", JCGT 6(1) 2017
+  float sign = copysignf(1.0f, n.z);
+  float a = -1.0f / (sign + n.z);
+  float b = (n.x * n.y) * a;
+  b1 = v3{1.0f + ((sign * n.x) * n.x) * a, sign * b, -sign * n.x};
+  b2 = v3{b, sign + (n.y * n.y) * a, -n.y};
+}
+
+BNX_D v3 random_direction(Random2 &rng)
+{
+  while (true) {
+    v3 v;
+    v.x = 1.f - 2.f * rng();
+    v.y = 1.f - 2.f * rng();
+    v.z = 1.f - 2.f * rng();
+    if (dot(v, v) <= 1.f) return normalize(v);
+  }
+}
+
+struct LightSample {
+  v3 direction;
+  float distance;
+  v3 radiance;
+  float pdf;
+};
+
+BNX_D bool sample_dir(LightSample &ls, const bnx_dir_light *lights, int n, v3 N, Random2 &random)
+{
+  if (n == 0) return false;
+  int lID[2];
+  float weights[2];
+  float sum = 0.f;
+  for (int i = 0; i < 2; i++) {
+    int l = (int)(random() * (float)n);
+    lID[i] = l < n - 1 ? l : n - 1;
+    weights[i] = 0.f;
+    const bnx_dir_light &L = lights[lID[i]];
+    v3 rad = ld3(L.color) * L.radiance;
+    v3 ldir = -ld3(L.direction);
+    float w = (N == splat(0.f)) ? 1.f : dot(ldir, N);
+    if (w <= 1e-3f) continue;
+    w *= reduce_max(rad);
+    if (w <= 1e-3f) continue;
+    weights[i] = w;
+    sum += w;
+  }
+  if (sum == 0.f) return false;
+  float r = random() * sum;
+  int i = 0;
+  while (i < 2 && r >= weights[i]) {
+    r -= weights[i];
+    ++i;
+  }
+  if (i == 2) return false;
+  const bnx_dir_light &L = lights[lID[i]];
+  ls.direction = -ld3(L.direction);
+  ls.distance = BNX_INF;
+  float denom = (weights[i] / sum) * (2.f / (float)n);
+  ls.radiance = ((ld3(L.color) * L.radiance) * 1.f) / denom;
+  ls.pdf = BNX_INF;
+  return weights[i] != 0.f;
+}
+
+BNX_D bool sample_lights(LightSample &ls, const ShadeParams &P, v3 N, Random2 &random, bool &needs_mis,
+                         bool &is_dir)
+{
+  needs_mis = false;
+  LightSample els, dls;
+  // sampleEnvLight without texture (shadeRays.cu:265-291)
+  els.direction = random_direction(random);
+  els.distance = BNX_INF;
+  if (!(N == splat(0.f))) {
+    els.radiance = splat(TWO_PI * P.rr.ambient_radiance);
+    if (dot(els.direction, N) < 0.f) els.direction = -els.direction;
+    els.pdf = ONE_OVER_TWO_PI;
+  } else {
+    els.radiance = splat(FOUR_PI * P.rr.ambient_radiance);
+    els.pdf = ONE_OVER_FOUR_PI;
+  }
+  float elsW = reduce_max(els.radiance);
+  float alsW = 0.f;
+  float dlsW = sample_dir(dls, P.dir_lights, P.num_dir_lights, N, random) ? reduce_max(dls.radiance) : 0.f;
+  float plsW = 0.f;
+  float sum = ((alsW + dlsW) + elsW) + plsW;
+  if (sum == 0.f) return false;
+  elsW *= 1.f / sum;
+  alsW *= 1.f / sum;
+  dlsW *= 1.f / sum;
+  plsW *= 1.f / sum;
+  float rr = random();
+  if (rr <= alsW) {
+    return false;
+  } else if (rr <= alsW + elsW) {
+    ls = els;
+    ls.radiance = ls.radiance * (1.f / elsW);
+    needs_mis = true;
+  } else if (rr <= (alsW + elsW) + plsW) {
+    return false;
+  } else {
+    ls = dls;
+    ls.radiance = ls.radiance * (1.f / dlsW);
+    is_dir = true;
+  }
+  if (isnan(ls.pdf) || ls.pdf <= 0.f) return false;
+  return true;
+}
+
+BNX_D float lambert_pdf(v3 Ng, v3 wi)
+{
+  v3 N = Ng;
+  if (dot(wi, N) < 0.f) N = -N;
+  float c = fmx(dot(wi, N), 0.f);
+  return c * ONE_OVER_PI;
+}
+
+// hit programs: Triangles.dev.cu:19-135 / Spheres.dev.cu:42-107 + AnariMatte
+BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
+{
+  SurfHit sh;
+  const InstDev &I = S.insts[hit.z];
+  const GeomDev &G = S.geoms[I.geom_base + hit.y];
+  const int prim = hit.x;
+  const float hu = __int_as_float(hit.w);
+  float color[4] = {0.f, 0.f, 0.f, 1.f};
+  if (G.colors) {
+    color[0] = G.colors[3 * prim + 0];
+    color[1] = G.colors[3 * prim + 1];
+    color[2] = G.colors[3 * prim + 2];
+  }
+  v3 P, Nw;
+  if (G.type == 0) {
+    const int *ix = G.indices + 3 * prim;
+    v3 v0 = ld3(G.vertices + 3 * ix[0]), v1 = ld3(G.vertices + 3 * ix[1]), v2 = ld3(G.vertices + 3 * ix[2]);
+    v3 n = normalize(cross(v1 - v0, v2 - v0));
+    v3 osN = normalize(n);
+    v3 osP = (((1.f - hu) - hv) * v0 + hu * v1) + hv * v2;
+    if (I.identity) {
+      P = osP;
+      Nw = osN;
+    } else {
+      P = xfm_point_rows(I.o2w, osP);
+      // transpose of the world-to-object linear part: columns of the rows
+      Nw = v3{(I.w2o[0].x * osN.x + I.w2o[1].x * osN.y) + I.w2o[2].x * osN.z,
+              (I.w2o[0].y * osN.x + I.w2o[1].y * osN.y) + I.w2o[2].y * osN.z,
+              (I.w2o[0].z * osN.x + I.w2o[1].z * osN.y) + I.w2o[2].z * osN.z};
+    }
+  } else {
+    v3 o = rorg, d = fix_dir(rdir);
+    if (!I.identity) {
+      o = xfm_point_rows(I.w2o, rorg);
+      d = fix_dir(xfm_vector_rows(I.w2o, d));
+    }
+    float4 sp = G.spheres[prim];
+    v3 ctr = xyz(sp);
+    float rad = sp.w;
+    float t_move = fmx(0.f, length(ctr - o) - 3.f * rad);
+    v3 om = o + t_move * d;
+    v3 objP = om + hu * d;
+    P = I.identity ? objP : xfm_point_rows(I.o2w, objP);
+    v3 objN = (objP == ctr) ? v3{1.f, 0.f, 0.f} : (objP - ctr);
+    v3 wN = I.identity ? objN : xfm_vector_rows(I.o2w, objN);
+    Nw = normalize(wN);
+  }
+  float base0, base1, base2;
+  if (G.color_from_attr) {
+    base0 = color[0]; base1 = color[1]; base2 = color[2];
+  } else {
+    base0 = G.color.x; base1 = G.color.y; base2 = G.color.z;
+  }
+  sh.albedo = (0.85f * v3{base0, base1, base2}) * ONE_OVER_PI;
+  sh.P = P;
+  sh.N = quant_half(Nw);
+  return sh;
+}
+
+// primary-ray miss color (generateRays.cu:205-216): background or gradient
+BNX_D float4 miss_color(const ShadeParams &P, int pixel)
+{
+  if (P.rr.bg_color[3] >= 0.f) return make_float4(P.rr.bg_color[0], P.rr.bg_color[1], P.rr.bg_color[2], P.rr.bg_color[3]);
+  int iy = pixel / P.W;
+  float t = ((float)iy + .5f) / (float)P.H;
+  return make_float4((1.0f - t) * 0.9f + t * 0.15f, (1.0f - t) * 0.9f + t * 0.25f, (1.0f - t) * 0.9f + t * .8f,
+                     (1.0f - t) * 1.f + t * 1.f);
+}
+
+BNX_D int64_t to_fixed(float f) { return (int64_t)(f * 0x1p32f); }
+BNX_D void acc_add(unsigned long long *p, float f) { atomicAdd(p, (unsigned long long)to_fixed(f)); }
+
+struct OutRay {
+  float4 org, dir;
+  uint64_t rng;
+  uint4 ps;
+};
+
+BNX_D void pack_ps(uint4 &ps, v3 tp, int ndiff, int pixel, float mis)
+{
+  ps.x = f2h(tp.x) | (f2h(tp.y) << 16);
+  ps.y = f2h(tp.z) | ((uint32_t)ndiff << 16);
+  ps.z = (uint32_t)pixel;
+  ps.w = __float_as_uint(mis);
+}
+
+__global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQueue in, const int *in_count,
+                                               RayQueue out, int *out_count, FrameDev fb)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  const int n = *in_count;
+  const bool valid = i < n;
+  bool w_sh = false, w_ray = false;
+  OutRay o_sh, o_ray;
+  if (valid) {
+    const int gen = P.generation;
+    float4 ro = in.org[i], rd = in.dir[i];
+    uint64_t rng = in.rng[i];
+    uint4 ps = in.ps[i];
+    int4 hit = in.hit[i];
+    float hv = in.hit_v[i];
+    uint32_t flags = __float_as_uint(rd.w);
+    const bool is_shadow = (flags & 1u) != 0;
+    const int sample = (int)(flags >> 16);
+    const int accumID = P.accum_base + sample;
+    v3 tp = v3{h2f(ps.x), h2f(ps.x >> 16), h2f(ps.y)};
+    int ndiff = (int)(ps.y >> 16);
+    const int pixel = (int)ps.z;
+    const float mis = __uint_as_float(ps.w);
+    v3 rorg = xyz(ro), rdir = xyz(rd);
+    float tmax = ro.w;
+    const bool had_hit = hit.x >= 0 && !is_shadow;
+    const bool shadow_hit = hit.x >= 0;
+
+    SurfHit sh{};
+    if (had_hit) sh = eval_hit(S, hit, hv, rorg, rdir);
+
+    const float4 mc = miss_color(P, pixel);
+    float alpha = (gen == 0) ? (had_hit ? 1.f : mc.w) : 0.f;
+    float inZ = tmax;
+    v3 inN = had_hit ? sh.N : splat(0.f);
+    v3 frag = splat(0.f);
+
+    // ------------------------------------------------ bounce (shadeRays.cu:474-880)
+    const float EPS = 1e-4f;
+    float sh_tmax = -1.f, ray_tmax = tmax;
+    if (is_shadow) {
+      if (!shadow_hit) frag = tp * mis;
+      ray_tmax = -1.f;
+    } else {
+      v3 Ng = sh.N;
+      const bool is_vol = (Ng == splat(0.f));
+      if (!is_vol) Ng = normalize(Ng);
+      const bool front = dot(rdir, Ng) < 0.f;
+      v3 Ngff = front ? Ng : -Ng;
+      if (!had_hit) {
+        if (gen == 0) {
+          frag = xyz(mc);
+        } else {
+          frag = (tp * splat(P.rr.ambient_radiance)) * mis;
+        }
+        ray_tmax = -1.f;
+      } else {
+        Random2 random(rng, __float_as_uint(tmax));
+        v3 dgP = sh.P;
+        v3 wo = -normalize(rdir);
+        const float off_eps = fmx(EPS, 1e-6f * reduce_max(vabs(dgP)));
+        v3 ff_off = is_vol ? wo : Ngff;
+        LightSample ls;
+        bool needs_mis = false, is_dir = false;
+        if (sample_lights(ls, P, Ngff, random, needs_mis, is_dir)) {
+          float c = fmx(dot(ls.direction, Ng), 0.f);
+          float fpdf = c * ONE_OVER_PI;
+          v3 fv = (sh.albedo * ONE_OVER_PI) * c;
+          if (!(fpdf > 0.f) || reduce_max(fv) < 1e-4f) {
+            sh_tmax = -1.f;
+          } else {
+            fv = vmin(fv, splat(CLAMP_F_R));
+            float s1 = isinf(ls.pdf) ? 1.f : 1.f / ls.pdf;
+            float s2 = is_vol ? 1.f : ONE_OVER_PI * fabsf(dot(Ng, ls.direction));
+            v3 tp_sr = (((tp * s1) * fv) * ls.radiance) * s2;
+            v3 so = dgP + off_eps * ff_off;
+            sh_tmax = ls.distance * (1.f - 2.f * off_eps);
+            o_sh.org = make_float4(so.x, so.y, so.z, sh_tmax);
+            o_sh.dir = make_float4(ls.direction.x, ls.direction.y, ls.direction.z,
+                                   __uint_as_float(1u | ((uint32_t)sample << 16)));
+            o_sh.rng = rng;
+            rng_next_h(rng, (uint64_t)__float_as_uint(tmax));
+            float smis = 1.f;
+            if (!is_dir && needs_mis) {
+              float pl = ONE_OVER_FOUR_PI;
+              float psd = lambert_pdf(Ng, ls.direction);
+              smis = pl / ((pl + psd) + 1e-10f);
+              if (smis < 1e-5f) sh_tmax = -1.f;
+            }
+            pack_ps(o_sh.ps, quant_half(tp_sr), 0, pixel, smis);
+          }
+        }
+        if (gen >= MAX_PATH_DEPTH) {
+          ray_tmax = -1.f;
+        } else {
+          ray_tmax = BNX_INF;
+          // Lambertian::scatter (packedBSDF/Lambertian.h:70-85)
+          v3 Nsc = Ng;
+          if (dot(wo, Nsc) < 0.f) Nsc = -Nsc;
+          float sx = random();
+          float sy = random();
+          float phi = TWO_PI * sx;
+          float cosT = sqrtf(sy);
+          float sinT = sqrtf(1.0f - sy);
+          float sp, cp;
+          bnx_num::sincos(phi, sp, cp);
+          v3 local = v3{cp * sinT, sp * sinT, cosT};
+          v3 b1, b2;
+          onb(Nsc, b1, b2);
+          v3 sdir = (b1 * local.x + b2 * local.y) + Nsc * local.z;
+          float spdf = local.z * ONE_OVER_PI;
+          if (!(spdf > 0.f) || spdf <= 1e-6f) {
+            // invalid scatter: the ray goes on unchanged (shadeRays.cu:774-775)
+          } else if (ndiff >= MAX_DIFFUSE_BOUNCES) {
+            ray_tmax = -1.f;
+          } else {
+            ndiff = ndiff + 1;
+            rorg = dgP + ((1.f * off_eps) * ff_off);
+            rdir = normalize(sdir);
+            float cf = is_vol ? 1.f : fabsf(dot(Ng, normalize(rdir)));
+            float den = isinf(spdf) ? 1.f : spdf + 1e-10f;
+            v3 sf = (sh.albedo * cf) / den;
+            sf = vmin(sf, splat(2.5f));
+            tp = quant_half(tp * sf);
+            float nmis;
+            if (needs_mis && !isinf(spdf)) {
+              float psd = lambert_pdf(Ng, rdir);
+              float pl = ONE_OVER_FOUR_PI;
+              nmis = psd / (psd + pl);
+            } else {
+              nmis = 1.f;
+            }
+            pack_ps(ps, tp, ndiff, pixel, nmis);
+          }
+          if (ray_tmax > 0.f) {
+            o_ray.org = make_float4(rorg.x, rorg.y, rorg.z, ray_tmax);
+            o_ray.dir = make_float4(rdir.x, rdir.y, rdir.z, __uint_as_float((uint32_t)sample << 16));
+            o_ray.rng = rng;
+            o_ray.ps = ps;
+          }
+        }
+      }
+    }
+    w_sh = sh_tmax > 0.f;
+    w_ray = ray_tmax > 0.f;
+
+    // ------------------------------------------------ accumulate (shadeRays.cu:993-1040)
+    const float clampMax = 10.f * (float)(1 + accumID);
+    frag = vmin(frag, splat(clampMax));
+    unsigned long long *acc = fb.accum + 4 * (size_t)pixel;
+    if (gen == 0) {
+      if (accumID == 0) {
+        fb.prim[pixel] = hit.x >= 0 ? hit.x : -1;
+        int inst = -1, obj = -1;
+        if (hit.x >= 0) {
+          const InstDev &I = S.insts[hit.z];
+          const GeomDev &G = S.geoms[I.geom_base + hit.y];
+          inst = (G.type == 1 && (P.rr.flags & BNX_RENDER_COMPAT_SPHERE_INSTID13)) ? 13 : I.user_id;
+          obj = G.user_id;
+        }
+        fb.inst[pixel] = inst;
+        fb.obj[pixel] = obj;
+        fb.normal[pixel] = make_float4(inN.x, inN.y, inN.z, 0.f);
+      }
+      atomicMin(fb.depth + pixel, f2o(inZ));
+      if (alpha > 0.f) acc_add(acc + 3, alpha);
+    }
+    if (frag.x > 0.f) acc_add(acc + 0, frag.x);
+    if (frag.y > 0.f) acc_add(acc + 1, frag.y);
+    if (frag.z > 0.f) acc_add(acc + 2, frag.z);
+  }
+  // ------------------------------------------------ wave-compacted append
+  const uint64_t b1 = __ballot(w_sh);
+  const uint64_t b2 = __ballot(w_ray);
+  const uint64_t lt = lanemask_lt();
+  const int pre = __popcll(b1 & lt) + __popcll(b2 & lt);
+  const int tot = __popcll(b1) + __popcll(b2);
+  int base = 0;
+  if (lane_id() == 0 && tot > 0) base = atomicAdd(out_count, tot);
+  base = __shfl(base, 0);
+  int slot = base + pre;
+  if (w_sh) {
+    out.org[slot] = o_sh.org;
+    out.dir[slot] = o_sh.dir;
+    out.rng[slot] = o_sh.rng;
+    out.ps[slot] = o_sh.ps;
+    slot++;
+  }
+  if (w_ray) {
+    out.org[slot] = o_ray.org;
+    out.dir[slot] = o_ray.dir;
+    out.rng[slot] = o_ray.rng;
+    out.ps[slot] = o_ray.ps;
+  }
+  if (lane_id() == 0 && tot > 0 && P.stats) atomicAdd(P.stats, (unsigned long long)__popcll(b1));
+}
+
+// ----------------------------------------------------------- frame ops
+__global__ void k_frame_clear(FrameDev fb, int np)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  for (int k = 0; k < 4; k++) fb.accum[4 * (size_t)i + k] = 0ull;
+  fb.depth[i] = f2o(BNX_INF);
+  fb.prim[i] = -1;
+  fb.inst[i] = -1;
+  fb.obj[i] = -1;
+  fb.normal[i] = make_float4(0.f, 0.f, 0.f, 0.f);
+}
+
+// linearizeColorAndNormalKernel (fb/TiledFB.cu:12-52): accum * (1/accumID)
+__global__ void k_frame_color(FrameDev fb, int np, float scale, float4 *out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const unsigned long long *a = fb.accum + 4 * (size_t)i;
+  float c[4];
+  for (int k = 0; k < 4; k++) c[k] = (float)((double)(long long)a[k] * 0x1p-32) * scale;
+  out[i] = make_float4(c[0], c[1], c[2], c[3]);
+}
+
+__global__ void k_frame_depth(FrameDev fb, int np, float *out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < np) out[i] = o2f(fb.depth[i]);
+}
+
+inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+} // namespace
+
+int frame_clear(Frame *f)
+{
+  int np = f->w * f->h;
+  if (np > 0) k_frame_clear<<<nblk(np, 256), 256, 0, f->dev->stream>>>(frame_dev(f), np);
+  BNX_HIP(hipGetLastError());
+  f->accum_id = 0;
+  return BNX_OK;
+}
+
+int frame_read(Frame *f, int channel, void *host)
+{
+  int np = f->w * f->h;
+  hipStream_t st = f->dev->stream;
+  FrameDev fd = frame_dev(f);
+  if (np == 0) return BNX_OK;
+  switch (channel) {
+  case BNX_FB_COLOR: {
+    BNX_HIP(f->out.ensure(sizeof(float4) * (size_t)np));
+    float scale = f->accum_id > 0 ? 1.f / (float)f->accum_id : 0.f;
+    k_frame_color<<<nblk(np, 256), 256, 0, st>>>(fd, np, scale, f->out.as<float4>());
+    BNX_HIP(hipMemcpyAsync(host, f->out.ptr, sizeof(float4) * (size_t)np, hipMemcpyDeviceToHost, st));
+    break;
+  }
+  case BNX_FB_DEPTH:
+    BNX_HIP(f->out.ensure(sizeof(float) * (size_t)np));
+    k_frame_depth<<<nblk(np, 256), 256, 0, st>>>(fd, np, f->out.as<float>());
+    BNX_HIP(hipMemcpyAsync(host, f->out.ptr, sizeof(float) * (size_t)np, hipMemcpyDeviceToHost, st));
+    break;
+  case BNX_FB_PRIMID:
+    BNX_HIP(hipMemcpyAsync(host, fd.prim, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+    break;
+  case BNX_FB_INSTID:
+    BNX_HIP(hipMemcpyAsync(host, fd.inst, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+    break;
+  case BNX_FB_OBJID:
+    BNX_HIP(hipMemcpyAsync(host, fd.obj, 4 * (size_t)np, hipMemcpyDeviceToHost, st));
+    break;
+  case BNX_FB_NORMAL: {
+    std::vector<float4> tmp((size_t)np);
+    BNX_HIP(hipMemcpyAsync(tmp.data(), fd.normal, sizeof(float4) * (size_t)np, hipMemcpyDeviceToHost, st));
+    BNX_HIP(hipStreamSynchronize(st));
+    float *o = (float *)host;
+    for (int i = 0; i < np; i++) {
+      o[3 * i + 0] = tmp[i].x;
+      o[3 * i + 1] = tmp[i].y;
+      o[3 * i + 2] = tmp[i].z;
+    }
+    return BNX_OK;
+  }
+  default:
+    set_error("bnx_frame_read: unknown channel");
+    return BNX_ERR_INVALID;
+  }
+  BNX_HIP(hipStreamSynchronize(st));
+  return BNX_OK;
+}
+
+int launch_raygen(Device *dev, const bnx_camera &cam, const bnx_renderer &rr, int W, int H, int accum_base, int S,
+                  const RayQueue &q)
+{
+  long total = (long)W * H * S;
+  if (total > 0) k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(cam, rr, W, H, accum_base, S, q);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_shade(Device *dev, const SceneDev &S, const ShadeParams &P, const RayQueue &in, const int *in_count,
+                 int max_in, const RayQueue &out, int *out_count, const FrameDev &fb)
+{
+  if (max_in > 0) k_shade<<<nblk(max_in, 256), 256, 0, dev->stream>>>(S, P, in, in_count, out, out_count, fb);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+} // namespace bnx

@@ -1,0 +1,344 @@
+// bnx_trace.hip -- two-level closest-hit / any-hit traversal on gfx950.
+//
+// Replaces TraceInterface::traceRay (rtcore/cuda/TraceInterface.h:24-139,
+// CPU: rtcore/cpu/Device.cpp:112-201) together with the trace kernel of
+// kernels/traceRays.dev.cu:26-54.
+//
+// Design (wave64):
+//   * persistent waves: lane 0 of each wave takes 64 rays at a time from a
+//     device counter, so long rays do not hold a whole block hostage;
+//   * nearest-child-first binary BVH traversal over 64-byte nodes (both
+//     child boxes in one fetch = four 16-byte loads);
+//   * short stack: a 16-entry ring per lane in LDS laid out [entry][lane]
+//     (bank-conflict free), spilling its oldest entries to a per-lane
+//     private array when deeper;
+//   * two levels: a TLAS leaf pushes a LEAVE marker and switches the ray to
+//     object space (enterBlas, rtcore/cuda/TraceInterface.h:107-122);
+//   * primitive programs are a closed type switch (triangles / spheres)
+//     instead of SBT function pointers (rtcore/cuda/Geom.h:21-36);
+//   * ties in t go to the lower (instance, geometry, primitive) ID, and a
+//     box whose entry distance equals the current best is still visited.
+#include "bnx_internal.h"
+
+namespace bnx {
+namespace {
+
+constexpr int TB = 256;        // threads per block
+constexpr int SHORT = 16;      // LDS ring entries per lane (power of two)
+constexpr int DEEP = 176;      // private spill entries per lane
+constexpr int STACK_CAP = SHORT + DEEP;
+
+struct HitRec {
+  float t, u, v;
+  int prim, geom, inst;
+};
+
+BNX_D bool id_less(int i0, int g0, int p0, int i1, int g1, int p1)
+{
+  if (i0 != i1) return i0 < i1;
+  if (g0 != g1) return g0 < g1;
+  return p0 < p1;
+}
+
+BNX_D bool accept_t(float t, const HitRec &b, int inst, int geom, int prim)
+{
+  if (t < b.t) return true;
+  if (t > b.t) return false;
+  return b.prim < 0 ? false : id_less(inst, geom, prim, b.inst, b.geom, b.prim);
+}
+
+// slab test, (lo - o) * inv per axis, conservative by 1e-5 relative on the exit
+BNX_D bool box_test(v3 o, v3 inv, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax,
+                    float &tnear)
+{
+  float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+  float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+  float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+  float tn = fmx(fmx(fmx(tmin, fmn(t0x, t1x)), fmn(t0y, t1y)), fmn(t0z, t1z));
+  float tf = fmn(fmn(fmn(tmax, fmx(t0x, t1x)), fmx(t0y, t1y)), fmx(t0z, t1z));
+  tnear = tn;
+  return tn <= tf * 1.00001f;
+}
+
+// barney's triangle test (rtcore/hiprt/TraceInterface.h:38-76) on a BVH-
+// ordered record (v0|prim, e1|geom, e2); acceptance under the tie rule
+BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int inst)
+{
+  float4 A = rec[0], B = rec[1], C = rec[2];
+  v3 v0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
+  int prim = __float_as_int(A.w), geom = __float_as_int(B.w);
+  v3 N = cross(e1, e2);
+  float dn = dot(d, N);
+  if (fabsf(dn) < 1e-12f) return false;
+  float t = -dot(o - v0, N) / dn;
+  if (t <= tmin) return false;
+  if (!accept_t(t, best, inst, geom, prim)) return false;
+  v3 P = (o - v0) + t * d;
+  float e1u, e2u, Pu, e1v, e2v, Pv;
+  if (fabsf(N.x) >= fmx(fabsf(N.y), fabsf(N.z))) {
+    e1u = e1.y; e2u = e2.y; Pu = P.y;
+    e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else if (fabsf(N.y) > fabsf(N.z)) {
+    e1u = e1.x; e2u = e2.x; Pu = P.x;
+    e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else {
+    e1u = e1.x; e2u = e2.x; Pu = P.x;
+    e1v = e1.y; e2v = e2.y; Pv = P.y;
+  }
+  float den = e1u * e2v - e1v * e2u;
+  if (den == 0.f) return false;
+  float u = (Pu * e2v - Pv * e2u) / den;
+  float v = (e1u * Pv - e1v * Pu) / den;
+  if (u < 0.f || v < 0.f || (u + v) >= 1.f) return false;
+  best = HitRec{t, u, v, prim, geom, inst};
+  return true;
+}
+
+// SpheresPrograms::intersect (Spheres.dev.cu:114-166), "move the origin"
+BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst)
+{
+  v3 center = xyz(S);
+  float radius = S.w;
+  float t_move = fmx(0.f, length(center - o) - 3.f * radius);
+  v3 om = o + t_move * d;
+  float tmin_l = fmx(0.f, tmin - t_move);
+  v3 oc = om - center;
+  float a = dot(d, d);
+  float b = dot(oc, d);
+  float c = dot(oc, oc) - radius * radius;
+  float disc = b * b - a * c;
+  if (disc < 0.f) return false;
+  float sq = sqrtf(disc);
+  float r1 = (-b - sq) / a;
+  float r2 = (-b + sq) / a;
+  float cand;
+  if (r1 > tmin_l) cand = r1;
+  else if (r2 > tmin_l) cand = r2;
+  else return false;
+  float t = cand + t_move;
+  if (t > best.t) return false;
+  int2 r = *ref;
+  if (!accept_t(t, best, inst, r.x, r.y)) return false;
+  best = HitRec{t, cand, 0.f, r.y, r.x, inst};
+  return true;
+}
+
+struct Stack {
+  uint32_t *lds;         // &ring[0][lane-in-block]
+  uint32_t deep[DEEP];
+  int sp;
+  BNX_D void push(uint32_t v)
+  {
+    int s = sp & (SHORT - 1);
+    if (sp >= SHORT) deep[sp - SHORT] = lds[s * TB];
+    lds[s * TB] = v;
+    sp++;
+  }
+  BNX_D uint32_t pop()
+  {
+    sp--;
+    int s = sp & (SHORT - 1);
+    uint32_t v = lds[s * TB];
+    if (sp >= SHORT) lds[s * TB] = deep[sp - SHORT];
+    return v;
+  }
+};
+
+struct Counts {
+  uint32_t nodes, tris, spheres;
+};
+
+BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float tmax, bool any_hit, Stack &st,
+                      Counts &cnt)
+{
+  const v3 wdir = fix_dir(wdir_in);
+  const v3 winv = v3{1.f / wdir.x, 1.f / wdir.y, 1.f / wdir.z};
+  HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
+  v3 o = worg, d = wdir, inv = winv;
+  const Node *nodes = S.tlas_nodes;
+  BlasDev blas{};
+  bool in_blas = false;
+  int inst = -1;
+  st.sp = 0;
+  uint32_t cur = 0;
+  uint32_t nvis = 0;
+  // every node is entered at most once per ray, so a ray can take no more
+  // loop trips than the scene has nodes and leaves; the cap only bounds the
+  // loop if the structure were corrupted
+  for (uint32_t trip = 0; trip < (1u << 28); trip++) {
+    if (cur < LEAVE_BLAS) {
+      // ------------------------------------------------------- inner node
+      const float4 *np = (const float4 *)(nodes + cur);
+      float4 a = np[0], b = np[1], c = np[2];
+      int4 r = ((const int4 *)np)[3];
+      nvis++;
+      float tn0 = 0.f, tn1 = 0.f;
+      bool h0 = r.z >= 0 && box_test(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, tmin, best.t, tn0);
+      bool h1 = r.w >= 0 && box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, tmin, best.t, tn1);
+      if (h0 && h1) {
+        bool n1 = tn1 < tn0;
+        uint32_t c0 = enc_child(r.x, r.z), c1 = enc_child(r.y, r.w);
+        st.push(n1 ? c0 : c1);
+        cur = n1 ? c1 : c0;
+        continue;
+      }
+      if (h0) { cur = enc_child(r.x, r.z); continue; }
+      if (h1) { cur = enc_child(r.y, r.w); continue; }
+    } else if (cur == LEAVE_BLAS) {
+      in_blas = false;
+      o = worg; d = wdir; inv = winv;
+      nodes = S.tlas_nodes;
+    } else {
+      // ------------------------------------------------------------ leaf
+      uint32_t first = (cur & ~LEAF_BIT) >> 3;
+      int count = (int)(cur & 7u);
+      if (!in_blas) {
+        inst = S.tlas_order[first];                  // TLAS leaves hold one instance
+        const InstDev &I = S.insts[inst];
+        blas = S.blas[I.blas];
+        if (!I.identity) {
+          o = xfm_point_rows(I.w2o, worg);
+          d = fix_dir(xfm_vector_rows(I.w2o, wdir));
+          inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+        }
+        st.push(LEAVE_BLAS);
+        in_blas = true;
+        nodes = blas.nodes;
+        cur = 0;
+        continue;
+      }
+      bool found = false;
+      if (blas.type == BLAS_TRIANGLES) {
+        cnt.tris += count;
+        for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, tmin, best, inst);
+      } else {
+        cnt.spheres += count;
+        for (int k = 0; k < count; k++)
+          found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst);
+      }
+      if (any_hit && found) break;
+    }
+    if (st.sp == 0) break;
+    cur = st.pop();
+  }
+  cnt.nodes += nvis;
+  return best;
+}
+
+// take 64 rays per wave from a device counter
+BNX_D int wave_take(int *work)
+{
+  int base = 0;
+  if (lane_id() == 0) base = atomicAdd(work, 64);
+  return __shfl(base, 0);
+}
+
+__global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *rays, int n, bnx_hit *hits,
+                                                  uint32_t flags, int *work)
+{
+  __shared__ uint32_t ring[SHORT * TB];
+  Stack st;
+  st.lds = ring + threadIdx.x;
+  while (true) {
+    int base = wave_take(work);
+    if (base >= n) break;
+    int i = base + lane_id();
+    if (i >= n) continue;
+    const float4 *rp = (const float4 *)(rays + i);
+    float4 r0 = rp[0], r1 = rp[1];
+    Counts cnt{0, 0, 0};
+    HitRec h = traverse(S, xyz(r0), xyz(r1), r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, st, cnt);
+    bnx_hit out;
+    out.t = h.t;
+    out.u = h.prim >= 0 ? h.u : 0.f;
+    out.v = h.prim >= 0 ? h.v : 0.f;
+    out.prim_id = h.prim;
+    if (h.prim >= 0) {
+      const InstDev &I = S.insts[h.inst];
+      out.geom_id = h.geom;
+      out.inst = h.inst;
+      out.user_inst_id = I.user_id;
+      out.obj_id = S.geoms[I.geom_base + h.geom].user_id;
+    } else {
+      out.geom_id = out.inst = out.user_inst_id = out.obj_id = -1;
+    }
+    hits[i] = out;
+  }
+}
+
+BNX_D uint32_t wave_sum(uint32_t v)
+{
+  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
+  return v;
+}
+
+// counters[0..2]: node visits, triangle tests, sphere tests (64-bit)
+__global__ __launch_bounds__(TB) void k_trace_queue(SceneDev S, TraceQueue q, const int *d_count, int *work,
+                                                    unsigned long long *counters)
+{
+  __shared__ uint32_t ring[SHORT * TB];
+  Stack st;
+  st.lds = ring + threadIdx.x;
+  const int n = *d_count;
+  Counts cnt{0, 0, 0};
+  while (true) {
+    int base = wave_take(work);
+    if (base >= n) break;
+    int i = base + lane_id();
+    if (i >= n) continue;
+    float4 o = q.org[i];
+    float4 d = q.dir[i];
+    bool shadow = (__float_as_uint(d.w) & 1u) != 0;
+    HitRec h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, st, cnt);
+    if (h.prim >= 0) q.org[i].w = h.t;
+    q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
+    q.hit_v[i] = h.v;
+  }
+  uint32_t a = wave_sum(cnt.nodes), b = wave_sum(cnt.tris), c = wave_sum(cnt.spheres);
+  if (lane_id() == 0 && counters) {
+    atomicAdd(counters + 0, (unsigned long long)a);
+    atomicAdd(counters + 1, (unsigned long long)b);
+    atomicAdd(counters + 2, (unsigned long long)c);
+  }
+}
+
+__global__ void k_zero(int *p, int n)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0;
+}
+
+int grid_for(Device *dev, int n)
+{
+  int want = (n + TB - 1) / TB;
+  int cap = dev->num_cus * 8;
+  return want < cap ? (want > 0 ? want : 1) : cap;
+}
+
+} // namespace
+
+int trace_stack_capacity() { return STACK_CAP; }
+
+int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags)
+{
+  Device *dev = g->dev;
+  BNX_HIP(dev->counters.ensure(256));
+  int *work = dev->counters.as<int>();
+  k_zero<<<1, 64, 0, dev->stream>>>(work, 1);
+  k_trace_api<<<grid_for(dev, n), TB, 0, dev->stream>>>(g->scene, d_rays, n, d_hits, flags, work);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_trace_queue(Group *g, const TraceQueue &q, const int *d_count, int max_n, unsigned long long *counters)
+{
+  Device *dev = g->dev;
+  BNX_HIP(dev->counters.ensure(256));
+  int *work = dev->counters.as<int>() + 1;
+  k_zero<<<1, 64, 0, dev->stream>>>(work, 1);
+  k_trace_queue<<<grid_for(dev, max_n), TB, 0, dev->stream>>>(g->scene, q, d_count, work, counters);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+} // namespace bnx

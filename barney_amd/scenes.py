@@ -1,0 +1,175 @@
+"""Seeded synthetic scenes for the benchmark configurations.
+
+All random draws come from PCG32 (M. E. O'Neill, "PCG: A Family of Simple
+Fast Space-Efficient Statistically Good Algorithms for Random Number
+Generation", 2014): 64-bit LCG with multiplier 6364136223846793005, odd
+increment ``(stream << 1) | 1`` and XSH-RR output.  ``U[0,1) = (u32 >> 8) *
+2^-24``.  Every primitive draws from its own stream, keyed by
+``(kind << 32) | index``, so generation is vectorised and independent of
+the order in which primitives are produced.
+"""
+from __future__ import annotations
+
+import math
+from dataclasses import dataclass, field
+
+import numpy as np
+
+MULT = np.uint64(6364136223846793005)
+SEEDS = {"c1": 0xB4A20001, "c2": 0xB4A20002, "c3": 0xB4A20003, "c4": 0xB4A20004, "c5": 0xB4A20005}
+
+
+def _u64(x):
+    return np.asarray(x, dtype=np.uint64)
+
+
+class PCG32:
+    """Vectorised PCG32 over many streams (one state per stream)."""
+
+    def __init__(self, seed: int, streams):
+        streams = _u64(streams)
+        self.inc = (streams << np.uint64(1)) | np.uint64(1)
+        with np.errstate(over="ignore"):
+            self.state = np.zeros_like(streams)
+            self.state = self.state * MULT + self.inc
+            self.state = self.state + np.uint64(seed & 0xFFFFFFFFFFFFFFFF)
+            self.state = self.state * MULT + self.inc
+
+    def next_u32(self) -> np.ndarray:
+        old = self.state
+        with np.errstate(over="ignore"):
+            self.state = old * MULT + self.inc
+        xorshifted = (((old >> np.uint64(18)) ^ old) >> np.uint64(27)).astype(np.uint32)
+        rot = (old >> np.uint64(59)).astype(np.uint32)
+        return (xorshifted >> rot) | (xorshifted << ((np.uint32(32) - rot) & np.uint32(31)))
+
+    def uniform(self) -> np.ndarray:
+        return (self.next_u32() >> np.uint32(8)).astype(np.float64) * (1.0 / 16777216.0)
+
+
+def streams(kind: int, n: int) -> np.ndarray:
+    return (np.uint64(kind) << np.uint64(32)) | np.arange(n, dtype=np.uint64)
+
+
+def uniform_table(seed: int, kind: int, n: int, k: int) -> np.ndarray:
+    """n x k uniforms: row i is the first k draws of stream (kind, i)."""
+    g = PCG32(seed, streams(kind, n))
+    return np.stack([g.uniform() for _ in range(k)], axis=1)
+
+
+# --------------------------------------------------------------------------
+# icosphere
+# --------------------------------------------------------------------------
+def icosphere(subdiv: int, radius: float = 1.0):
+    """Icosahedron subdivided `subdiv` times by midpoint split with a
+    shared-edge cache; child faces of (a,b,c) are emitted in the fixed order
+    (a,ab,ca), (b,bc,ab), (c,ca,bc), (ab,bc,ca).  Faces wind counter-
+    clockwise seen from outside, so cross(v1-v0, v2-v0) points outward.
+    subdiv=5 gives 10,242 vertices and 20,480 triangles."""
+    t = (1.0 + math.sqrt(5.0)) / 2.0
+    verts = [(-1, t, 0), (1, t, 0), (-1, -t, 0), (1, -t, 0),
+             (0, -1, t), (0, 1, t), (0, -1, -t), (0, 1, -t),
+             (t, 0, -1), (t, 0, 1), (-t, 0, -1), (-t, 0, 1)]
+    verts = [np.array(v, dtype=np.float64) / np.linalg.norm(v) for v in verts]
+    faces = [(0, 11, 5), (0, 5, 1), (0, 1, 7), (0, 7, 10), (0, 10, 11),
+             (1, 5, 9), (5, 11, 4), (11, 10, 2), (10, 7, 6), (7, 1, 8),
+             (3, 9, 4), (3, 4, 2), (3, 2, 6), (3, 6, 8), (3, 8, 9),
+             (4, 9, 5), (2, 4, 11), (6, 2, 10), (8, 6, 7), (9, 8, 1)]
+    for _ in range(subdiv):
+        cache = {}
+
+        def mid(a, b):
+            key = (a, b) if a < b else (b, a)
+            if key not in cache:
+                m = verts[a] + verts[b]
+                verts.append(m / np.linalg.norm(m))
+                cache[key] = len(verts) - 1
+            return cache[key]
+
+        nf = []
+        for a, b, c in faces:
+            ab, bc, ca = mid(a, b), mid(b, c), mid(c, a)
+            nf += [(a, ab, ca), (b, bc, ab), (c, ca, bc), (ab, bc, ca)]
+        faces = nf
+    v = (np.array(verts) * radius).astype(np.float32)
+    f = np.array(faces, dtype=np.int32)
+    return v, f
+
+
+# --------------------------------------------------------------------------
+# scene containers
+# --------------------------------------------------------------------------
+@dataclass
+class TriMesh:
+    vertices: np.ndarray          # (nv,3) float32
+    indices: np.ndarray           # (nt,3) int32
+    colors: np.ndarray | None     # (nt,3) float32 per-prim
+    user_id: int = 0
+
+
+@dataclass
+class SphereSet:
+    centers: np.ndarray           # (n,3) float32
+    radii: np.ndarray             # (n,) float32
+    colors: np.ndarray | None     # (n,3) float32
+    user_id: int = 1
+
+
+@dataclass
+class SurfaceScene:
+    meshes: list = field(default_factory=list)
+    spheres: list = field(default_factory=list)
+    dir_lights: list = field(default_factory=list)   # (direction(3), radiance, color(3))
+    ambient: float = 1.0
+    bg_color: tuple = (0.0, 0.0, 0.0, 1.0)
+    camera: dict = field(default_factory=dict)
+    width: int = 512
+    height: int = 512
+    spp: int = 1
+
+    @property
+    def num_prims(self):
+        return sum(len(m.indices) for m in self.meshes) + sum(len(s.radii) for s in self.spheres)
+
+
+def config1(width: int = 512, height: int = 512, spp: int = 1, num_spheres: int = 1000,
+            subdiv: int = 5) -> SurfaceScene:
+    """Plain CPU-reference scene: icosphere(5) + 1,000 spheres, Lambertian
+    albedo U[0.2,0.8]^3 per primitive, one directional light toward
+    (1,1,1)/sqrt(3) (emission direction -(1,1,1)/sqrt(3)), ambient 1,
+    pinhole at (0,0,5) looking at the origin, fovy 40 degrees."""
+    seed = SEEDS["c1"]
+    v, f = icosphere(subdiv)
+    tri_col = (0.2 + 0.6 * uniform_table(seed, 3, len(f), 3)).astype(np.float32)
+    g = uniform_table(seed, 1, num_spheres, 4)
+    centers = (-2.0 + 4.0 * g[:, :3]).astype(np.float32)
+    radii = (0.01 + 0.04 * g[:, 3]).astype(np.float32)
+    sph_col = (0.2 + 0.6 * uniform_table(seed, 2, num_spheres, 3)).astype(np.float32)
+    s = SurfaceScene(width=width, height=height, spp=spp)
+    s.meshes.append(TriMesh(v, f, tri_col, user_id=0))
+    s.spheres.append(SphereSet(centers, radii, sph_col, user_id=1))
+    d = -np.ones(3, dtype=np.float64) / math.sqrt(3.0)
+    s.dir_lights.append((d.astype(np.float32), 1.0, np.ones(3, dtype=np.float32)))
+    s.ambient = 1.0
+    s.bg_color = (0.0, 0.0, 0.0, 1.0)
+    s.camera = dict(position=(0.0, 0.0, 5.0), direction=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=40.0)
+    return s
+
+
+def random_rays(seed: int, n: int, center=(0.0, 0.0, 0.0), radius: float = 4.0, spread: float = 1.5):
+    """n rays from points on a sphere of `radius` aimed at jittered points
+    near `center` (bnx_ray layout: org, tmin, dir, tmax)."""
+    u = uniform_table(seed, 7, n, 6)
+    z = 1.0 - 2.0 * u[:, 0]
+    phi = 2.0 * math.pi * u[:, 1]
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    org = np.stack([r * np.cos(phi), r * np.sin(phi), z], 1) * radius + np.asarray(center)
+    tgt = np.asarray(center) + spread * (u[:, 2:5] - 0.5) * 2.0
+    d = tgt - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), dtype=np.float32)
+    rays[:, 0:3] = org
+    rays[:, 3] = 0.0
+    rays[:, 4:7] = d
+    rays[:, 7] = 1e30
+    return rays

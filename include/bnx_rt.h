@@ -1,0 +1,279 @@
+/* bnx_rt.h -- C-ABI of the MI355X (gfx950) ray-tracing + volume backend.
+ *
+ * This is the drop-in boundary underneath barney's `rtcore` backend interface
+ * (namespace BARNEY_NS::rtc).  Every entry point below names the reference
+ * interface it replaces (paths relative to the barney v1.4.0 source tree).
+ * All functions return BNX_OK (0) on success or a negative BNX_ERR_* code;
+ * bnx_last_error() returns a message for the most recent failure on the
+ * calling thread.  A C++ adapter converts codes to std::runtime_error, which
+ * is what BARNEY_RAISE does in rtcore/cudaCommon/cuda-helper.h:9-42.
+ *
+ * Threading: one host thread drives a bnx_device; every call is enqueued on
+ * that device's single HIP stream (rtcore/cudaCommon/Device.cu:109-122).
+ * Pointers named d_* are device pointers (from bnx_buffer_dptr); every other
+ * pointer is a host pointer.  All structs are plain-old-data with the sizes
+ * asserted below.
+ */
+#ifndef BNX_RT_H
+#define BNX_RT_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BNX_ABI_VERSION 1
+
+/* ---------------------------------------------------------------- errors */
+#define BNX_OK               0
+#define BNX_ERR_INVALID     -1  /* bad handle / argument                       */
+#define BNX_ERR_HIP         -2  /* HIP runtime error                           */
+#define BNX_ERR_OOM         -3  /* device allocation failed / over budget      */
+#define BNX_ERR_UNSUPPORTED -4  /* feature not provided by this backend         */
+#define BNX_ERR_STATE       -5  /* call out of order (e.g. trace before build) */
+
+/* Message for the last failed call on this thread (never NULL). */
+const char *bnx_last_error(void);
+int bnx_abi_version(void);
+
+/* ------------------------------------------------------------- handles */
+typedef struct bnx_device_s *bnx_device;
+typedef struct bnx_buffer_s *bnx_buffer;
+typedef struct bnx_geom_s   *bnx_geom;
+typedef struct bnx_group_s  *bnx_group;
+typedef struct bnx_world_s  *bnx_world;
+typedef struct bnx_frame_s  *bnx_frame;
+
+/* ------------------------------------------------------------- records */
+/* One ray of a trace batch: traceRays.dev.cu:38-50 passes (org, dir, 0, tMax). */
+typedef struct bnx_ray {
+  float org[3];
+  float tmin;
+  float dir[3];
+  float tmax;
+} bnx_ray;
+
+/* Closest hit of one ray.  prim_id is local to its geometry (Triangles.dev.cu:
+ * 31, Spheres.dev.cu:181); geom_id is the geometry's index inside its group;
+ * inst is the rtc instance index and user_inst_id the user instance ID it maps
+ * to (rtcore/cpu/TraceInterface.h:107-108); obj_id is the geometry's userID
+ * (Triangles.dev.cu:57).  For triangles (u,v) are the barycentrics; for spheres
+ * u is the hit distance measured from the moved origin (Spheres.dev.cu:132-166)
+ * and v is 0.  A miss has prim_id == -1 and t == the ray's tmax. */
+typedef struct bnx_hit {
+  float t;
+  float u;
+  float v;
+  int32_t prim_id;
+  int32_t geom_id;
+  int32_t inst;
+  int32_t user_inst_id;
+  int32_t obj_id;
+} bnx_hit;
+
+/* ----------------------------------------------------------- materials */
+#define BNX_MATERIAL_MATTE 0        /* AnariMatte -> Lambertian (material/AnariMatte.h:51-76) */
+
+typedef struct bnx_material {
+  int32_t type;                     /* BNX_MATERIAL_*                              */
+  int32_t color_from_attribute;     /* 1: color = geometry's per-prim 'color'      */
+  float   color[4];                 /* constant color when not from the attribute  */
+  float   opacity;                  /* AnariMatte 'opacity'                        */
+  float   _pad;
+} bnx_material;
+
+/* ---------------------------------------------------------- geometries */
+/* Triangles: rtc::Geom::setVertices/setIndices + Triangles::DD
+ * (geometry/Triangles.h:26-32, native/geometry/Triangles.cpp:50-82). */
+typedef struct bnx_triangles_desc {
+  const float   *vertices;          /* float3[num_vertices]                        */
+  int32_t        num_vertices;
+  int32_t        num_triangles;
+  const int32_t *indices;           /* int3[num_triangles]                         */
+  const float   *prim_colors;       /* float3[num_triangles] or NULL               */
+  int32_t        user_id;           /* objID reported in hit IDs                   */
+  int32_t        _pad;
+  bnx_material   material;
+} bnx_triangles_desc;
+
+/* Spheres: Spheres::DD (geometry/Spheres.h:15-20). */
+typedef struct bnx_spheres_desc {
+  const float   *centers;           /* float3[num_spheres]                         */
+  const float   *radii;             /* float[num_spheres] or NULL                  */
+  const float   *colors;            /* float3[num_spheres] or NULL                 */
+  int32_t        num_spheres;
+  float          default_radius;
+  int32_t        user_id;
+  int32_t        _pad;
+  bnx_material   material;
+} bnx_spheres_desc;
+
+/* Camera::DD perspective member (native/Camera.h:30-65). */
+#define BNX_CAMERA_PERSPECTIVE 1
+typedef struct bnx_camera {
+  int32_t type;
+  float   dir_00[3];
+  float   dir_du[3];
+  float   dir_dv[3];
+  float   lens_00[3];
+  float   aperture_radius;
+  float   focus_distance;
+} bnx_camera;
+
+/* Renderer::DD + the renderer constants of kernels/shadeRays.cu:17-34. */
+#define BNX_RENDER_COMPAT_SPHERE_INSTID13 1u  /* reproduce Spheres.dev.cu:193-194 */
+typedef struct bnx_renderer {
+  float    bg_color[4];             /* 'background' (anari/Renderer.cpp:29)        */
+  float    ambient_radiance;        /* 'ambientRadiance' (anari/Renderer.cpp:24)   */
+  float    cut_plane[4];            /* disabled when cut_plane[3] <= -1e28         */
+  uint32_t flags;                   /* BNX_RENDER_*                                */
+  int32_t  max_generations;         /* host loop bound; 0 = MAX_PATH_DEPTH+1        */
+  int32_t  _pad;
+} bnx_renderer;
+
+/* DirLight::DD (light/DirLight.h:13-17): 'direction' is the emission direction. */
+typedef struct bnx_dir_light {
+  float direction[3];
+  float radiance;
+  float color[3];
+} bnx_dir_light;
+
+/* Frame-buffer channels (fb/FrameBuffer.cu:315-394 bnFrameBufferRead). */
+#define BNX_FB_COLOR   0   /* float4 per pixel: accum * 1/accumID           */
+#define BNX_FB_DEPTH   1   /* float per pixel: primary-ray tMax             */
+#define BNX_FB_PRIMID  2   /* int32 per pixel                               */
+#define BNX_FB_INSTID  3   /* int32 per pixel                               */
+#define BNX_FB_OBJID   4   /* int32 per pixel                               */
+#define BNX_FB_NORMAL  5   /* float3 per pixel (first-sample normal)        */
+
+/* Counters of one bnx_render call (device-side counts, read after sync). */
+typedef struct bnx_render_stats {
+  int64_t rays_traced;              /* sum over generations of rays entering trace */
+  int64_t shadow_rays;
+  int64_t generations;              /* generation launches that had rays           */
+  int64_t volume_samples;           /* density evaluations (volumes)               */
+  double  ms_total;                 /* wall time of the call, host clock          */
+  int64_t node_visits;              /* BVH inner-node fetches (64 B each)          */
+  int64_t tri_tests;                /* triangle tests (48 B records)               */
+  int64_t sphere_tests;             /* sphere tests (16 B records)                 */
+  double  trace_ms;                 /* summed trace-kernel time (HIP events)       */
+  int64_t trace_launches;
+} bnx_render_stats;
+
+/* BVH summary of one BLAS / TLAS (for tests and the byte model). */
+typedef struct bnx_bvh_info {
+  int32_t num_nodes;                /* inner nodes (64 B each)                     */
+  int32_t num_prims;
+  int32_t max_depth;
+  int32_t root_is_leaf;
+  float   root_lower[3];
+  float   root_upper[3];
+  double  build_ms;
+} bnx_bvh_info;
+
+/* ------------------------------------------------------------- device */
+/* rtc::Device(gpuID) -- rtcore/cudaCommon/Device.h, rtcore/cuda/Device.h:23-56 */
+int bnx_device_create(int gpu_id, bnx_device *out);
+int bnx_device_destroy(bnx_device dev);
+/* rtc::Device::sync (rtcore/cudaCommon/Device.cu) */
+int bnx_device_sync(bnx_device dev);
+/* free / total device memory in bytes (used by memory-based admission) */
+int bnx_device_mem_info(bnx_device dev, size_t *free_bytes, size_t *total_bytes);
+
+/* ------------------------------------------------------------- buffers */
+/* rtc::Device::createBuffer / rtc::Buffer::upload / freeBuffer */
+int   bnx_buffer_create(bnx_device dev, size_t bytes, const void *init, bnx_buffer *out);
+int   bnx_buffer_upload(bnx_buffer buf, const void *host, size_t bytes, size_t offset);
+int   bnx_buffer_download(bnx_buffer buf, void *host, size_t bytes, size_t offset);
+int   bnx_buffer_destroy(bnx_buffer buf);
+void *bnx_buffer_dptr(bnx_buffer buf);
+size_t bnx_buffer_size(bnx_buffer buf);
+
+/* --------------------------------------------------------------- geoms */
+/* GeomType::createGeom + Geom::setVertices/setIndices/setDD for the
+ * 'Triangles' type (RTC_EXPORT_TRIANGLES_GEOM, Triangles.dev.cu:139). */
+int bnx_geom_create_triangles(bnx_device dev, const bnx_triangles_desc *desc, bnx_geom *out);
+/* ... and for the 'Spheres' user type (RTC_EXPORT_USER_GEOM, Spheres.dev.cu:205). */
+int bnx_geom_create_spheres(bnx_device dev, const bnx_spheres_desc *desc, bnx_geom *out);
+/* rtc::Device::freeGeom */
+int bnx_geom_destroy(bnx_geom geom);
+
+/* -------------------------------------------------------------- groups */
+/* rtc::Device::createTrianglesGroup (rtcore/cuda/Group.cu:261-413) */
+int bnx_group_create_triangles(bnx_device dev, const bnx_geom *geoms, int num_geoms, bnx_group *out);
+/* rtc::Device::createUserGeomsGroup (rtcore/cuda/Group.cu:418-562) */
+int bnx_group_create_user(bnx_device dev, const bnx_geom *geoms, int num_geoms, bnx_group *out);
+/* rtc::Device::createInstanceGroup(groups, instIDs, xfms) (rtcore/cuda/Group.cu:144-258).
+ * xfms: num_instances affine3f, 12 floats each, column-major like owl's
+ * affine3f: {vx.x,vx.y,vx.z, vy.x,vy.y,vy.z, vz.x,vz.y,vz.z, p.x,p.y,p.z}. */
+int bnx_group_create_instances(bnx_device dev, const bnx_group *groups, const int32_t *inst_ids,
+                               const float *xfms, int num_instances, bnx_group *out);
+/* rtc::Group::buildAccel: binned-SAH BVH built on the GPU. For an instance
+ * group every referenced group must have been built first. */
+int bnx_group_build(bnx_group group);
+int bnx_group_bvh_info(bnx_group group, bnx_bvh_info *info);
+/* Download the BVH: nodes (64 B each, num_nodes) and the prim order
+ * (num_prims int32: original prim index for each BVH-order slot). */
+int bnx_group_download_bvh(bnx_group group, void *nodes, int32_t *prim_order);
+/* rtc::Device::freeGroup */
+int bnx_group_destroy(bnx_group group);
+
+/* Binned-SAH build over raw boxes (d_boxes: n x {lo.xyz, hi.xyz} floats).
+ * Writes the same node layout as bnx_group_download_bvh. For tests. */
+int bnx_bvh_build_boxes(bnx_device dev, const float *d_boxes, int n, int max_leaf,
+                        void *nodes_host, int32_t *prim_order_host, bnx_bvh_info *info);
+
+/* --------------------------------------------------------------- trace */
+#define BNX_TRACE_CLOSEST 0u
+#define BNX_TRACE_ANY     1u   /* occlusion: any opaque hit ends the ray */
+/* TraceKernel2D::launch of native::TraceRays (kernels/traceRays.dev.cu:26-54,
+ * rtcore/cuda/TraceInterface.h:24-139): closest hit for n rays, lower
+ * (inst, geom, prim) wins a tie in t. Asynchronous on the device stream. */
+int bnx_trace(bnx_group instances, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
+
+/* --------------------------------------------------------------- world */
+/* World::DD (render/World.h:32-47): instances + lights. */
+int bnx_world_create(bnx_device dev, bnx_group instances, const bnx_dir_light *dir_lights,
+                     int num_dir_lights, bnx_world *out);
+int bnx_world_destroy(bnx_world world);
+
+/* -------------------------------------------------------------- camera */
+/* PerspectiveCamera::commit (native/Camera.cu:82-101); fovy in degrees. */
+int bnx_camera_perspective(const float position[3], const float direction[3], const float up[3],
+                           float fovy_deg, bnx_camera *out);
+
+/* --------------------------------------------------------------- frame */
+/* FrameBuffer/TiledFB resize (fb/TiledFB.cu:249-319) */
+int bnx_frame_create(bnx_device dev, int width, int height, bnx_frame *out);
+/* bnAccumReset (fb/FrameBuffer.h:147) */
+int bnx_frame_reset(bnx_frame fb);
+/* bnFrameBufferRead (fb/FrameBuffer.cu:315-394) + linearize (fb/TiledFB.cu:12-111) */
+int bnx_frame_read(bnx_frame fb, int channel, void *host);
+int bnx_frame_accum_id(bnx_frame fb);
+int bnx_frame_destroy(bnx_frame fb);
+
+/* -------------------------------------------------------------- render */
+/* Context::renderTiles (native/Context.cpp:139-182): spp samples per pixel,
+ * each through raygen -> (trace, shade)* generations, accumulated into fb. */
+int bnx_render(bnx_world world, const bnx_camera *camera, const bnx_renderer *renderer,
+               bnx_frame fb, int spp, bnx_render_stats *stats);
+
+#ifdef __cplusplus
+} /* extern "C" */
+#endif
+
+#ifdef __cplusplus
+static_assert(sizeof(bnx_ray) == 32, "bnx_ray layout");
+static_assert(sizeof(bnx_hit) == 32, "bnx_hit layout");
+static_assert(sizeof(bnx_material) == 32, "bnx_material layout");
+static_assert(sizeof(bnx_camera) == 60, "bnx_camera layout");
+static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
+static_assert(sizeof(bnx_triangles_desc) == 72, "bnx_triangles_desc layout");
+static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
+static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
+static_assert(sizeof(bnx_render_stats) == 80, "bnx_render_stats layout");
+#endif
+
+#endif /* BNX_RT_H */

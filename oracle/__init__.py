@@ -1,0 +1,211 @@
+"""ctypes binding of the CPU oracle (oracle/_build/liboracle.so).
+
+TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and
+bench.py's cpu_baseline leg, always as the checker and never as the path
+being measured.  The product package (barney_amd) never imports it.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+import subprocess
+
+import numpy as np
+
+from barney_amd import abi
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "_build", "liboracle.so")
+_lib = None
+
+
+def build(force: bool = False) -> str:
+    """Compile the oracle with its Makefile (g++)."""
+    if force and os.path.exists(LIB_PATH):
+        os.remove(LIB_PATH)
+    subprocess.run(["make", "-C", _HERE, "-s"], check=True)
+    return LIB_PATH
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise RuntimeError(f"oracle library missing: {LIB_PATH} (run `make -C oracle`)")
+        L = C.CDLL(LIB_PATH)
+        vp = C.c_void_p
+        L.orc_scene_create.restype = vp
+        L.orc_scene_destroy.argtypes = [vp]
+        L.orc_scene_add_triangles.argtypes = [vp, C.POINTER(abi.TrianglesDesc)]
+        L.orc_scene_add_spheres.argtypes = [vp, C.POINTER(abi.SpheresDesc)]
+        L.orc_scene_add_group.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
+        L.orc_scene_add_instance.argtypes = [vp, C.c_int, C.c_int, abi.fp, abi.fp]
+        L.orc_scene_set_dir_lights.argtypes = [vp, C.POINTER(abi.DirLight), C.c_int]
+        L.orc_scene_build.argtypes = [vp]
+        L.orc_scene_build.restype = C.c_double
+        L.orc_trace.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_uint64), C.c_int]
+        L.orc_bvh_build_boxes.argtypes = [abi.fp, C.c_int, C.c_int, vp, abi.ip, abi.ip]
+        L.orc_scene_bvh.argtypes = [vp, C.c_int, vp, abi.ip, abi.ip]
+        L.orc_camera_perspective.argtypes = [abi.fp, abi.fp, abi.fp, C.c_float, C.POINTER(abi.Camera)]
+        L.orc_frame_create.restype = vp
+        L.orc_frame_create.argtypes = [C.c_int, C.c_int]
+        L.orc_frame_destroy.argtypes = [vp]
+        L.orc_frame_reset.argtypes = [vp]
+        L.orc_render.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.c_int,
+                                 C.POINTER(C.c_uint64)]
+        L.orc_render.restype = C.c_double
+        L.orc_frame_read.argtypes = [vp, C.c_int, vp]
+        L.orc_sincos.argtypes = [C.c_float, abi.fp, abi.fp]
+        L.orc_f32_to_f16.argtypes = [C.c_float]
+        L.orc_f32_to_f16.restype = C.c_uint16
+        L.orc_f16_to_f32.argtypes = [C.c_uint16]
+        L.orc_f16_to_f32.restype = C.c_float
+        L.orc_rng_seq.argtypes = [C.c_uint32, C.c_uint32, C.c_int, C.POINTER(C.c_uint32)]
+        L.orc_random2_seq.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, abi.fp, C.POINTER(C.c_uint64)]
+        _lib = L
+    return _lib
+
+
+def default_threads() -> int:
+    return max(1, min(16, os.cpu_count() or 1))
+
+
+class Scene:
+    """Oracle scene: geometries -> groups -> instances, as in rtc."""
+
+    def __init__(self):
+        self.L = lib()
+        self.h = self.L.orc_scene_create()
+        self._keep = []
+
+    def __del__(self):
+        try:
+            self.L.orc_scene_destroy(self.h)
+        except Exception:
+            pass
+
+    def add_triangles(self, vertices, indices, colors=None, user_id=0, material=None):
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        self._keep += [v, i, c]
+        d = abi.TrianglesDesc(abi.fptr(v), len(v), len(i), abi.iptr(i), abi.fptr(c), user_id, 0,
+                              material or abi.matte())
+        return self.L.orc_scene_add_triangles(self.h, C.byref(d))
+
+    def add_spheres(self, centers, radii, colors=None, user_id=1, material=None, default_radius=0.1):
+        ce = np.ascontiguousarray(centers, dtype=np.float32)
+        r = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        self._keep += [ce, r, c]
+        d = abi.SpheresDesc(abi.fptr(ce), abi.fptr(r), abi.fptr(c), len(ce), default_radius, user_id, 0,
+                            material or abi.matte())
+        return self.L.orc_scene_add_spheres(self.h, C.byref(d))
+
+    def add_group(self, kind: int, geoms):
+        g = (C.c_int * len(geoms))(*geoms)
+        return self.L.orc_scene_add_group(self.h, kind, g, len(geoms))
+
+    def add_instance(self, group: int, user_id: int = 0, o2w=None, w2o=None):
+        a = None if o2w is None else np.ascontiguousarray(o2w, dtype=np.float32)
+        b = None if w2o is None else np.ascontiguousarray(w2o, dtype=np.float32)
+        self._keep += [a, b]
+        return self.L.orc_scene_add_instance(self.h, group, user_id, abi.fptr(a), abi.fptr(b))
+
+    def set_dir_lights(self, lights):
+        arr = abi.dir_lights(lights)
+        self._keep.append(arr)
+        self.L.orc_scene_set_dir_lights(self.h, arr, len(lights))
+
+    def build(self) -> float:
+        return self.L.orc_scene_build(self.h)
+
+    def trace(self, rays: np.ndarray, mode: int = 0, threads: int | None = None):
+        rays = np.ascontiguousarray(rays, dtype=np.float32)
+        n = len(rays)
+        hits = np.zeros(n, dtype=abi.HIT_DTYPE)
+        ctr = (C.c_uint64 * 4)()
+        self.L.orc_trace(self.h, rays.ctypes.data, n, hits.ctypes.data, mode, ctr,
+                         threads or default_threads())
+        return hits, dict(nodes=ctr[0], tris=ctr[1], spheres=ctr[2], rays=ctr[3])
+
+    def bvh(self, group: int):
+        info = (C.c_int32 * 3)()
+        self.L.orc_scene_bvh(self.h, group, None, None, info)
+        nodes = np.zeros(info[0], dtype=abi.NODE_DTYPE)
+        order = np.zeros(info[2], dtype=np.int32)
+        self.L.orc_scene_bvh(self.h, group, nodes.ctypes.data, abi.iptr(order), info)
+        return nodes, order, int(info[1])
+
+
+def bvh_build_boxes(boxes: np.ndarray, max_leaf: int = 4):
+    boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
+    n = len(boxes)
+    nodes = np.zeros(max(1, n), dtype=abi.NODE_DTYPE)
+    order = np.zeros(max(1, n), dtype=np.int32)
+    info = (C.c_int32 * 2)()
+    lib().orc_bvh_build_boxes(abi.fptr(boxes), n, max_leaf, nodes.ctypes.data, abi.iptr(order), info)
+    return nodes[: info[0]], order[:n], int(info[1])
+
+
+def camera_perspective(position, direction, up, fovy_deg) -> abi.Camera:
+    cam = abi.Camera()
+    p = np.asarray(position, np.float32)
+    d = np.asarray(direction, np.float32)
+    u = np.asarray(up, np.float32)
+    lib().orc_camera_perspective(abi.fptr(p), abi.fptr(d), abi.fptr(u), fovy_deg, C.byref(cam))
+    return cam
+
+
+class Frame:
+    def __init__(self, w: int, h: int):
+        self.L = lib()
+        self.w, self.h = w, h
+        self.h_ = self.L.orc_frame_create(w, h)
+
+    def __del__(self):
+        try:
+            self.L.orc_frame_destroy(self.h_)
+        except Exception:
+            pass
+
+    def reset(self):
+        self.L.orc_frame_reset(self.h_)
+
+    def read(self, channel: int) -> np.ndarray:
+        n = self.w * self.h
+        if channel == abi.BNX_FB_COLOR:
+            out = np.zeros((self.h, self.w, 4), np.float32)
+        elif channel == abi.BNX_FB_NORMAL:
+            out = np.zeros((self.h, self.w, 3), np.float32)
+        elif channel == abi.BNX_FB_DEPTH:
+            out = np.zeros((self.h, self.w), np.float32)
+        else:
+            out = np.zeros((self.h, self.w), np.int32)
+        assert out.size >= n
+        self.L.orc_frame_read(self.h_, channel, out.ctypes.data)
+        return out
+
+
+def render(scene: Scene, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp: int,
+           threads: int | None = None):
+    st = (C.c_uint64 * 7)()
+    ms = scene.L.orc_render(scene.h, C.byref(cam), C.byref(rend), frame.h_, spp, threads or default_threads(), st)
+    return ms, dict(rays=st[0], shadow=st[1], generations=st[2], volume_samples=st[3], nodes=st[4],
+                    tris=st[5], spheres=st[6])
+
+
+def surface_scene(sc) -> Scene:
+    """Oracle scene for a barney_amd.scenes.SurfaceScene: one user instance
+    (ID 0) flattened into a user-geometry group (spheres) and a triangles
+    group, in the order of native/ModelSlot.cpp:128-155."""
+    s = Scene()
+    sph = [s.add_spheres(x.centers, x.radii, x.colors, user_id=x.user_id) for x in sc.spheres]
+    tri = [s.add_triangles(m.vertices, m.indices, m.colors, user_id=m.user_id) for m in sc.meshes]
+    if sph:
+        s.add_instance(s.add_group(1, sph), 0)
+    if tri:
+        s.add_instance(s.add_group(0, tri), 0)
+    s.set_dir_lights(sc.dir_lights)
+    s.build()
+    return s

@@ -1,0 +1,242 @@
+// capi.cpp -- C entry points of the CPU oracle, loaded by tests/ and by
+// bench.py's cpu_baseline leg through ctypes.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+#include "oracle.h"
+
+#include <chrono>
+#include <cstdio>
+
+namespace orc {
+void camera_perspective(const float *pos, const float *dir, const float *up, float fovy, bnx_camera &out);
+}
+
+using namespace orc;
+
+struct orc_scene {
+  Scene s;
+};
+struct orc_frame {
+  Frame f;
+};
+
+static void set_material(Material &m, const bnx_material &d)
+{
+  m.type = d.type;
+  m.color_from_attribute = d.color_from_attribute != 0;
+  memcpy(m.color, d.color, sizeof(m.color));
+  m.opacity = d.opacity;
+}
+
+extern "C" {
+
+orc_scene *orc_scene_create(void) { return new orc_scene; }
+void orc_scene_destroy(orc_scene *s) { delete s; }
+
+int orc_scene_add_triangles(orc_scene *sc, const bnx_triangles_desc *d)
+{
+  Geom g;
+  g.type = 0;
+  g.user_id = d->user_id;
+  set_material(g.mat, d->material);
+  for (int i = 0; i < d->num_vertices; i++) g.vertices.push_back(load3(d->vertices + 3 * i));
+  g.indices.assign(d->indices, d->indices + 3 * (size_t)d->num_triangles);
+  if (d->prim_colors)
+    for (int i = 0; i < d->num_triangles; i++) g.colors.push_back(load3(d->prim_colors + 3 * i));
+  sc->s.geoms.push_back(std::move(g));
+  return (int)sc->s.geoms.size() - 1;
+}
+
+int orc_scene_add_spheres(orc_scene *sc, const bnx_spheres_desc *d)
+{
+  Geom g;
+  g.type = 1;
+  g.user_id = d->user_id;
+  set_material(g.mat, d->material);
+  g.default_radius = d->default_radius;
+  for (int i = 0; i < d->num_spheres; i++) g.centers.push_back(load3(d->centers + 3 * i));
+  if (d->radii) g.radii.assign(d->radii, d->radii + d->num_spheres);
+  if (d->colors)
+    for (int i = 0; i < d->num_spheres; i++) g.colors.push_back(load3(d->colors + 3 * i));
+  sc->s.geoms.push_back(std::move(g));
+  return (int)sc->s.geoms.size() - 1;
+}
+
+int orc_scene_add_group(orc_scene *sc, int type, const int *geoms, int n)
+{
+  Group g;
+  g.type = type;
+  g.geoms.assign(geoms, geoms + n);
+  sc->s.groups.push_back(std::move(g));
+  return (int)sc->s.groups.size() - 1;
+}
+
+// xfm: 12 floats (vx, vy, vz, p) or NULL for identity; w2o is its inverse
+int orc_scene_add_instance(orc_scene *sc, int group, int user_id, const float *o2w, const float *w2o)
+{
+  Instance in;
+  in.group = group;
+  in.user_id = user_id;
+  if (o2w && w2o) {
+    Affine a, b;
+    a.vx = load3(o2w); a.vy = load3(o2w + 3); a.vz = load3(o2w + 6); a.p = load3(o2w + 9);
+    b.vx = load3(w2o); b.vy = load3(w2o + 3); b.vz = load3(w2o + 6); b.p = load3(w2o + 9);
+    a.identity = b.identity = false;
+    in.o2w = a;
+    in.w2o = b;
+  }
+  sc->s.insts.push_back(in);
+  return (int)sc->s.insts.size() - 1;
+}
+
+void orc_scene_set_dir_lights(orc_scene *sc, const bnx_dir_light *l, int n)
+{
+  sc->s.dir_lights.assign(l, l + n);
+}
+
+double orc_scene_build(orc_scene *sc)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  build_scene(sc->s);
+  return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+}
+
+// mode 0: BVH closest, 1: brute force closest, 2: BVH any hit.
+// counters: nodes, tris, spheres, rays
+int orc_trace(orc_scene *sc, const bnx_ray *rays, int n, bnx_hit *hits, int mode, uint64_t *counters, int threads)
+{
+  std::atomic<uint64_t> cn{0}, ct{0}, cs{0};
+  parallel_for(n, threads, [&](int i) {
+    const bnx_ray &r = rays[i];
+    Hit h;
+    TraceCounters c;
+    if (mode == 1) h = trace_brute(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax);
+    else h = trace_bvh(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax, mode == 2, &c);
+    bnx_hit &o = hits[i];
+    o.t = h.t;
+    o.u = h.prim >= 0 ? h.u : 0.f;
+    o.v = h.prim >= 0 ? h.v : 0.f;
+    o.prim_id = h.prim;
+    o.geom_id = h.prim >= 0 ? h.geom : -1;
+    o.inst = h.prim >= 0 ? h.inst : -1;
+    o.user_inst_id = h.prim >= 0 ? sc->s.insts[h.inst].user_id : -1;
+    o.obj_id = h.prim >= 0 ? sc->s.geoms[sc->s.groups[sc->s.insts[h.inst].group].geoms[h.geom]].user_id : -1;
+    cn += c.nodes;
+    ct += c.tris;
+    cs += c.spheres;
+  });
+  if (counters) {
+    counters[0] = cn;
+    counters[1] = ct;
+    counters[2] = cs;
+    counters[3] = (uint64_t)n;
+  }
+  return 0;
+}
+
+// BVH over raw boxes (n x 6 floats); nodes_out holds max(1, n) nodes.
+int orc_bvh_build_boxes(const float *boxes, int n, int max_leaf, void *nodes_out, int32_t *order_out, int32_t *info)
+{
+  std::vector<Box> b(n);
+  for (int i = 0; i < n; i++) b[i] = Box(load3(boxes + 6 * i), load3(boxes + 6 * i + 3));
+  Bvh bvh;
+  build_bvh(b, max_leaf, bvh);
+  memcpy(nodes_out, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node));
+  if (n) memcpy(order_out, bvh.order.data(), (size_t)n * sizeof(int32_t));
+  info[0] = (int32_t)bvh.nodes.size();
+  info[1] = bvh.max_depth;
+  return 0;
+}
+
+int orc_scene_bvh(orc_scene *sc, int group, void *nodes_out, int32_t *order_out, int32_t *info)
+{
+  const Bvh &bvh = group < 0 ? sc->s.tlas : sc->s.groups[group].bvh;
+  if (nodes_out) memcpy(nodes_out, bvh.nodes.data(), bvh.nodes.size() * sizeof(Node));
+  if (order_out && !bvh.order.empty()) memcpy(order_out, bvh.order.data(), bvh.order.size() * sizeof(int32_t));
+  info[0] = (int32_t)bvh.nodes.size();
+  info[1] = bvh.max_depth;
+  info[2] = (int32_t)bvh.order.size();
+  return 0;
+}
+
+void orc_camera_perspective(const float *pos, const float *dir, const float *up, float fovy, bnx_camera *out)
+{
+  camera_perspective(pos, dir, up, fovy, *out);
+}
+
+orc_frame *orc_frame_create(int w, int h)
+{
+  orc_frame *f = new orc_frame;
+  f->f.resize(w, h);
+  return f;
+}
+void orc_frame_destroy(orc_frame *f) { delete f; }
+void orc_frame_reset(orc_frame *f) { f->f.reset(); }
+
+// stats: rays, shadow rays, generations, volume samples, nodes, tris, spheres
+double orc_render(orc_scene *sc, const bnx_camera *cam, const bnx_renderer *r, orc_frame *fb, int spp, int threads,
+                  uint64_t *stats)
+{
+  RenderStats st;
+  auto t0 = std::chrono::steady_clock::now();
+  render(sc->s, *cam, *r, fb->f, spp, threads, stats ? &st : nullptr);
+  double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  if (stats) {
+    stats[0] = st.rays;
+    stats[1] = st.shadow;
+    stats[2] = st.generations;
+    stats[3] = st.volume_samples;
+    stats[4] = st.ctr.nodes;
+    stats[5] = st.ctr.tris;
+    stats[6] = st.ctr.spheres;
+  }
+  return ms;
+}
+
+// channel layout as bnx_frame_read
+int orc_frame_read(orc_frame *fb, int channel, void *host)
+{
+  const Frame &f = fb->f;
+  size_t np = (size_t)f.w * f.h;
+  if (channel == BNX_FB_COLOR) {
+    float *o = (float *)host;
+    float scale = 1.f / (float)f.accum_id;
+    for (size_t i = 0; i < 4 * np; i++) o[i] = (float)((double)f.accum[i] * 0x1p-32) * scale;
+  } else if (channel == BNX_FB_DEPTH) {
+    memcpy(host, f.depth.data(), np * 4);
+  } else if (channel == BNX_FB_PRIMID) {
+    memcpy(host, f.prim.data(), np * 4);
+  } else if (channel == BNX_FB_INSTID) {
+    memcpy(host, f.inst.data(), np * 4);
+  } else if (channel == BNX_FB_OBJID) {
+    memcpy(host, f.obj.data(), np * 4);
+  } else if (channel == BNX_FB_NORMAL) {
+    memcpy(host, f.normal.data(), np * 12);
+  } else {
+    return -1;
+  }
+  return 0;
+}
+
+// --- numeric helpers for the known-answer tests
+void orc_sincos(float x, float *s, float *c) { bnx_num::sincos(x, *s, *c); }
+uint16_t orc_f32_to_f16(float f) { return f32_to_f16(f); }
+float orc_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
+// RNG: seed(a,b) from the default state, then n outputs of next()
+void orc_rng_seq(uint32_t a, uint32_t b, int n, uint32_t *out)
+{
+  RNGSeed r;
+  r.seed(a, b);
+  for (int i = 0; i < n; i++) out[i] = r.next();
+}
+// Random2(parent seeded (a,b), hash) -> n floats; also returns parent state
+void orc_random2_seq(uint32_t a, uint32_t b, uint32_t hash, int n, float *out, uint64_t *parent_state)
+{
+  RNGSeed p;
+  p.seed(a, b);
+  Random2 r(p, hash);
+  for (int i = 0; i < n; i++) out[i] = r();
+  *parent_state = p.state;
+}
+
+} // extern "C"

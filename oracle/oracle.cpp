@@ -1,0 +1,523 @@
+// oracle.cpp -- CPU oracle: BVH spec, traversal, ray generation, shading.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+//
+// Parity status per function (how each is pinned; tests/ holds the checks):
+//   RNGSeed/Random2, fnv_hash ....... closed form: tests/test_oracle_cpu.py
+//                                     re-derives the sequences in Python
+//                                     integers from barney-common.h:21-108.
+//   f32_to_f16 / f16_to_f32 ......... numpy.float16 (IEEE RNE) on all
+//                                     65536 halves and random floats.
+//   camera (Camera.cu:82-101) ....... closed form: centre pixel looks
+//                                     along -z; corner directions from the
+//                                     fovy formula.
+//   tri_test (hiprt/TraceInterface.h:38-76) ... closed forms (axis-aligned
+//                                     quads, t = 4 on the unit icosphere at
+//                                     the centre pixel), brute force.
+//   sphere_test (Spheres.dev.cu:114-166) ..... analytic roots.
+//   build_bvh ....................... invariants (each prim in exactly one
+//                                     leaf, child box == union of its prims,
+//                                     leaf size), SAH cost <= median split.
+//   trace_bvh ....................... == trace_brute bit for bit on random
+//                                     scenes, including constructed ties.
+//   shade (shadeRays.cu:474-880) .... white-furnace and closed-form
+//                                     expectations on analytic scenes; no
+//                                     reference-held vectors exist for it.
+//   owl::common::frame(N) ........... parity unpinned against the reference
+//                                     (un-vendored OWL); restated as Duff et
+//                                     al. 2017 and pinned by orthonormality.
+#include "oracle.h"
+
+#include <algorithm>
+#include <thread>
+#include <cstdio>
+
+namespace orc {
+
+// ======================================================================
+// half <-> float, IEEE round to nearest even
+// ======================================================================
+uint16_t f32_to_f16(float f)
+{
+  uint32_t x = fbits(f);
+  uint32_t sign = (x >> 16) & 0x8000u;
+  uint32_t mag = x & 0x7fffffffu;
+  if (mag >= 0x7f800000u) {                       // inf / nan
+    return (uint16_t)(sign | 0x7c00u | (mag > 0x7f800000u ? 0x200u : 0u));
+  }
+  if (mag >= 0x477ff000u) return (uint16_t)(sign | 0x7c00u);   // rounds to inf
+  if (mag < 0x38800000u) {                        // result is subnormal or zero
+    if (mag < 0x33000000u) return (uint16_t)sign; // < half of smallest subnormal
+    uint32_t e = mag >> 23;
+    uint32_t m = (mag & 0x7fffffu) | 0x800000u;
+    uint32_t shift = 126u - e;                    // 14 + (113 - e) - ... see below
+    // value = m * 2^(e-150); subnormal unit = 2^-24 -> q = m * 2^(e-126)
+    uint32_t q = m >> shift;
+    uint32_t rem = m & ((1u << shift) - 1u);
+    uint32_t halfway = 1u << (shift - 1u);
+    if (rem > halfway || (rem == halfway && (q & 1u))) q++;
+    return (uint16_t)(sign | q);
+  }
+  uint32_t e = (mag >> 23) - 112u;                // rebias 127 -> 15
+  uint32_t m = mag & 0x7fffffu;
+  uint32_t q = (e << 10) | (m >> 13);
+  uint32_t rem = m & 0x1fffu;
+  if (rem > 0x1000u || (rem == 0x1000u && (q & 1u))) q++;
+  return (uint16_t)(sign | q);
+}
+
+float f16_to_f32(uint16_t h)
+{
+  uint32_t sign = ((uint32_t)h & 0x8000u) << 16;
+  uint32_t e = (h >> 10) & 0x1fu;
+  uint32_t m = h & 0x3ffu;
+  uint32_t x;
+  if (e == 0) {
+    if (m == 0) x = sign;
+    else {                                        // subnormal: normalize
+      int k = 0;
+      while (!(m & 0x400u)) { m <<= 1; k++; }
+      m &= 0x3ffu;
+      x = sign | ((uint32_t)(113 - k) << 23) | (m << 13);
+    }
+  } else if (e == 31) {
+    x = sign | 0x7f800000u | (m << 13);
+  } else {
+    x = sign | ((e + 112u) << 23) | (m << 13);
+  }
+  float f;
+  memcpy(&f, &x, 4);
+  return f;
+}
+
+// ======================================================================
+// binned-SAH BVH (the build specification the GPU builder implements)
+//
+// Reference: cuBQL gpuBuilder with leaf <= 4 (rtcore/cuda/Group.cu:382,
+// 503-504) and leaf == 1 for instances (:198); cuBQL's own algorithm is
+// un-vendored, so this is the build's own binned SAH: 16 bins per axis over
+// the task's centroid bounds, cost = area(L)*|L| + area(R)*|R|, lowest cost
+// wins (ties: lower axis, then lower split), stable partition; median split
+// when no axis has a valid plane or the depth reaches MAX_SAH_DEPTH.
+// ======================================================================
+namespace {
+
+struct BuildCtx {
+  const std::vector<Box> *boxes;
+  std::vector<vec3> cent;
+  std::vector<int32_t> order, tmp;
+  int max_leaf;
+  Bvh *out;
+};
+
+void task_bounds(const BuildCtx &c, int b, int e, Box &box, Box &cbox)
+{
+  box = Box();
+  cbox = Box();
+  for (int i = b; i < e; i++) {
+    int p = c.order[i];
+    box.grow((*c.boxes)[p]);
+    cbox.grow(c.cent[p]);
+  }
+}
+
+inline int bin_of(float c, float lo, float scale)
+{
+  int b = (int)((c - lo) * scale);
+  return b > SAH_BINS - 1 ? SAH_BINS - 1 : b;
+}
+
+// returns split position (b < mid < e)
+int split_task(BuildCtx &c, int b, int e, int depth, const Box &cbox)
+{
+  int n = e - b;
+  int best_axis = -1, best_bin = -1;
+  float best_cost = INFINITY;
+  if (depth < MAX_SAH_DEPTH) {
+    for (int a = 0; a < 3; a++) {
+      float ext = cbox.hi[a] - cbox.lo[a];
+      if (!(ext >= 1e-30f)) continue;
+      float scale = (float)SAH_BINS / ext;
+      Box bb[SAH_BINS];
+      int bc[SAH_BINS] = {0};
+      for (int i = b; i < e; i++) {
+        int p = c.order[i];
+        int k = bin_of(c.cent[p][a], cbox.lo[a], scale);
+        bc[k]++;
+        bb[k].grow((*c.boxes)[p]);
+      }
+      float areaL[SAH_BINS], areaR[SAH_BINS];
+      int cntL[SAH_BINS], cntR[SAH_BINS];
+      Box acc;
+      int cn = 0;
+      for (int j = 0; j < SAH_BINS - 1; j++) {
+        acc.grow(bb[j]);
+        cn += bc[j];
+        areaL[j] = half_area(acc);
+        cntL[j] = cn;
+      }
+      acc = Box();
+      cn = 0;
+      for (int j = SAH_BINS - 1; j > 0; j--) {
+        acc.grow(bb[j]);
+        cn += bc[j];
+        areaR[j - 1] = half_area(acc);
+        cntR[j - 1] = cn;
+      }
+      for (int j = 0; j < SAH_BINS - 1; j++) {
+        if (cntL[j] == 0 || cntR[j] == 0) continue;
+        float cost = areaL[j] * (float)cntL[j] + areaR[j] * (float)cntR[j];
+        if (cost < best_cost) {
+          best_cost = cost;
+          best_axis = a;
+          best_bin = j;
+        }
+      }
+    }
+  }
+  if (best_axis < 0) return b + n / 2;           // median split in current order
+  float ext = cbox.hi[best_axis] - cbox.lo[best_axis];
+  float scale = (float)SAH_BINS / ext;
+  int nl = 0, nr = 0;
+  for (int i = b; i < e; i++) {                   // stable partition
+    int p = c.order[i];
+    if (bin_of(c.cent[p][best_axis], cbox.lo[best_axis], scale) <= best_bin) c.order[b + nl++] = p;
+    else c.tmp[nr++] = p;
+  }
+  for (int i = 0; i < nr; i++) c.order[b + nl + i] = c.tmp[i];
+  return b + nl;
+}
+
+void set_child(Node &nd, int side, const Box &box, int ref, int cnt)
+{
+  float *lo = side == 0 ? nd.lo0 : nd.lo1;
+  float *hi = side == 0 ? nd.hi0 : nd.hi1;
+  for (int k = 0; k < 3; k++) { lo[k] = box.lo[k]; hi[k] = box.hi[k]; }
+  nd.ref[side] = ref;
+  nd.cnt[side] = cnt;
+}
+
+void build_rec(BuildCtx &c, int b, int e, int depth, const Box &cbox, int node)
+{
+  int mid = split_task(c, b, e, depth, cbox);
+  int rng[2][2] = {{b, mid}, {mid, e}};
+  for (int s = 0; s < 2; s++) {
+    int cb = rng[s][0], ce = rng[s][1];
+    Box box, cb_box;
+    task_bounds(c, cb, ce, box, cb_box);
+    if (ce - cb <= c.max_leaf) {
+      set_child(c.out->nodes[node], s, box, cb, ce - cb);
+      c.out->max_depth = std::max(c.out->max_depth, depth + 1);
+    } else {
+      int idx = (int)c.out->nodes.size();
+      c.out->nodes.push_back(Node());
+      set_child(c.out->nodes[node], s, box, idx, 0);
+      build_rec(c, cb, ce, depth + 1, cb_box, idx);
+    }
+  }
+}
+
+} // namespace
+
+void build_bvh(const std::vector<Box> &boxes, int max_leaf, Bvh &out)
+{
+  int n = (int)boxes.size();
+  out.nodes.clear();
+  out.max_depth = 0;
+  out.nodes.push_back(Node());
+  BuildCtx c;
+  c.boxes = &boxes;
+  c.max_leaf = max_leaf;
+  c.out = &out;
+  c.cent.resize(n);
+  for (int i = 0; i < n; i++)
+    c.cent[i] = (boxes[i].lo + boxes[i].hi) * 0.5f;
+  c.order.resize(n);
+  c.tmp.resize(n);
+  for (int i = 0; i < n; i++) c.order[i] = i;
+  Box box, cbox;
+  task_bounds(c, 0, n, box, cbox);
+  out.root = box;
+  Node &root = out.nodes[0];
+  set_child(root, 0, Box(), 0, -1);
+  set_child(root, 1, Box(), 0, -1);
+  if (n == 0) {
+  } else if (n <= max_leaf) {
+    set_child(out.nodes[0], 0, box, 0, n);
+    out.max_depth = 1;
+  } else {
+    build_rec(c, 0, n, 0, cbox, 0);
+  }
+  out.order = c.order;
+}
+
+// ======================================================================
+// scene build: BLAS per group, TLAS over instances
+// (ModelSlot.cpp:128-155 flattening; rtcore/cuda/Group.cu:144-562)
+// ======================================================================
+
+void build_scene(Scene &s)
+{
+  for (auto &g : s.groups) {
+    g.prim_geom.clear();
+    g.prim_id.clear();
+    std::vector<Box> boxes;
+    for (int gi = 0; gi < (int)g.geoms.size(); gi++) {
+      const Geom &geom = s.geoms[g.geoms[gi]];
+      int np = geom.num_prims();
+      for (int p = 0; p < np; p++) {
+        g.prim_geom.push_back(gi);
+        g.prim_id.push_back(p);
+        Box b;
+        if (geom.type == 0) {
+          // createTriangleBounds (rtcore/cuda/Group.cu:91-112)
+          for (int k = 0; k < 3; k++) b.grow(geom.vertices[geom.indices[3 * p + k]]);
+        } else {
+          // SpheresPrograms::bounds (Spheres.dev.cu:29-40)
+          vec3 ctr = geom.centers[p];
+          float r = geom.radii.empty() ? geom.default_radius : geom.radii[p];
+          b = Box(ctr - vec3(r), ctr + vec3(r));
+        }
+        boxes.push_back(b);
+      }
+    }
+    build_bvh(boxes, 4, g.bvh);
+  }
+  // computeInstanceBounds (rtcore/cuda/Group.cu:125-137)
+  std::vector<Box> ib;
+  for (auto &in : s.insts) {
+    const Box &rb = s.groups[in.group].bvh.root;
+    Box b;
+    if (in.o2w.identity) b = rb;
+    else if (rb.lo.x <= rb.hi.x) {
+      for (int k = 0; k < 8; k++) {
+        vec3 c((k & 1) ? rb.hi.x : rb.lo.x, (k & 2) ? rb.hi.y : rb.lo.y, (k & 4) ? rb.hi.z : rb.lo.z);
+        b.grow(xfm_point(in.o2w, c));
+      }
+    }
+    ib.push_back(b);
+  }
+  build_bvh(ib, 1, s.tlas);
+}
+
+// ======================================================================
+// traversal
+// ======================================================================
+namespace {
+
+
+// lexicographic (inst, geom, prim) order: the lower ID wins a tie in t
+inline bool id_less(int i0, int g0, int p0, int i1, int g1, int p1)
+{
+  if (i0 != i1) return i0 < i1;
+  if (g0 != g1) return g0 < g1;
+  return p0 < p1;
+}
+
+inline bool accept_t(float t, const Hit &best, int inst, int geom, int prim)
+{
+  if (t < best.t) return true;
+  if (t > best.t) return false;
+  return best.prim < 0 ? false : id_less(inst, geom, prim, best.inst, best.geom, best.prim);
+}
+
+// barney's triangle test (rtcore/hiprt/TraceInterface.h:38-76), with the
+// acceptance bound replaced by the tie rule above
+bool tri_test(vec3 org, vec3 dir, float tmin, vec3 v0, vec3 v1, vec3 v2, Hit &best, int inst, int geom,
+              int prim)
+{
+  vec3 e1 = v1 - v0;
+  vec3 e2 = v2 - v0;
+  vec3 N = cross(e1, e2);
+  float dn = dot(dir, N);
+  if (fabsf(dn) < 1e-12f) return false;
+  float t = -dot(org - v0, N) / dn;
+  if (t <= tmin) return false;
+  if (!accept_t(t, best, inst, geom, prim)) return false;
+  vec3 P = (org - v0) + t * dir;
+  float e1u, e2u, Pu, e1v, e2v, Pv;
+  if (fabsf(N.x) >= fmx(fabsf(N.y), fabsf(N.z))) {
+    e1u = e1.y; e2u = e2.y; Pu = P.y;
+    e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else if (fabsf(N.y) > fabsf(N.z)) {
+    e1u = e1.x; e2u = e2.x; Pu = P.x;
+    e1v = e1.z; e2v = e2.z; Pv = P.z;
+  } else {
+    e1u = e1.x; e2u = e2.x; Pu = P.x;
+    e1v = e1.y; e2v = e2.y; Pv = P.y;
+  }
+  float den = e1u * e2v - e1v * e2u;
+  if (den == 0.f) return false;
+  float u = (Pu * e2v - Pv * e2u) / den;
+  float v = (e1u * Pv - e1v * Pu) / den;
+  if (u < 0.f || v < 0.f || (u + v) >= 1.f) return false;
+  best.t = t;
+  best.u = u;
+  best.v = v;
+  best.inst = inst;
+  best.geom = geom;
+  best.prim = prim;
+  return true;
+}
+
+// SpheresPrograms::intersect (Spheres.dev.cu:114-166), "move the origin";
+// the nearest root above tmin is accepted under the tie rule
+bool sphere_test(vec3 org, vec3 dir, float tmin, vec3 center, float radius, Hit &best, int inst, int geom,
+                 int prim)
+{
+  float t_move = fmx(0.f, length(center - org) - 3.f * radius);
+  vec3 o = org + t_move * dir;
+  float tmin_l = fmx(0.f, tmin - t_move);
+  vec3 oc = o - center;
+  float a = dot(dir, dir);
+  float b = dot(oc, dir);
+  float c = dot(oc, oc) - radius * radius;
+  float disc = b * b - a * c;
+  if (disc < 0.f) return false;
+  float sq = sqrtf(disc);
+  float r1 = (-b - sq) / a;
+  float r2 = (-b + sq) / a;
+  float cand;
+  if (r1 > tmin_l) cand = r1;
+  else if (r2 > tmin_l) cand = r2;
+  else return false;
+  float t = cand + t_move;
+  if (!accept_t(t, best, inst, geom, prim)) return false;
+  best.t = t;
+  best.u = cand;
+  best.v = 0.f;
+  best.inst = inst;
+  best.geom = geom;
+  best.prim = prim;
+  return true;
+}
+
+// slab test, conservative by a relative 1e-5 on the far distance
+inline bool box_test(vec3 org, vec3 inv, const float *lo, const float *hi, float tmin, float tmax, float &tnear)
+{
+  float tn = tmin, tf = tmax;
+  for (int k = 0; k < 3; k++) {
+    float t0 = (lo[k] - org[k]) * inv[k];
+    float t1 = (hi[k] - org[k]) * inv[k];
+    tn = fmx(tn, fmn(t0, t1));
+    tf = fmn(tf, fmx(t0, t1));
+  }
+  tnear = tn;
+  return tn <= tf * 1.00001f;
+}
+
+bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, float tmin, Hit &best, int inst,
+               TraceCounters *ctr)
+{
+  int pidx = g.bvh.order[slot];
+  int gi = g.prim_geom[pidx];
+  int pid = g.prim_id[pidx];
+  const Geom &geom = s.geoms[g.geoms[gi]];
+  if (geom.type == 0) {
+    if (ctr) ctr->tris++;
+    const int32_t *ix = &geom.indices[3 * pid];
+    return tri_test(org, dir, tmin, geom.vertices[ix[0]], geom.vertices[ix[1]], geom.vertices[ix[2]], best, inst,
+                    gi, pid);
+  } else {
+    if (ctr) ctr->spheres++;
+    float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
+    return sphere_test(org, dir, tmin, geom.centers[pid], r, best, inst, gi, pid);
+  }
+}
+
+} // namespace
+
+Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool any_hit, TraceCounters *ctr)
+{
+  vec3 dir = fix_dir(dir0);
+  Hit best;
+  best.t = tmax;
+  best.u = best.v = 0.f;
+  if (ctr) ctr->rays++;
+  vec3 winv(1.f / dir.x, 1.f / dir.y, 1.f / dir.z);
+  // stack entries: cnt == 0 inner node, cnt > 0 leaf [ref, ref+cnt),
+  // ENTER: instance 'ref' (TLAS leaf), LEAVE: back to the TLAS
+  struct Entry { int32_t ref, cnt; };
+  const int32_t LEAVE = -2, ENTER = -3;
+  Entry stack[512];
+  int sp = 0;
+  stack[sp++] = {0, 0};
+  bool in_blas = false;
+  int inst = -1;
+  vec3 o = org, d = dir, inv = winv;
+  while (sp > 0) {
+    Entry en = stack[--sp];
+    if (en.cnt == LEAVE) {
+      in_blas = false;
+      o = org; d = dir; inv = winv;
+      continue;
+    }
+    if (en.cnt == ENTER) {
+      // enterBlas (rtcore/cuda/TraceInterface.h:107-122)
+      inst = en.ref;
+      in_blas = true;
+      const Instance &in = s.insts[inst];
+      if (in.w2o.identity) {
+        o = org; d = dir; inv = winv;
+      } else {
+        o = xfm_point(in.w2o, org);
+        d = fix_dir(xfm_vector(in.w2o, dir));
+        inv = vec3(1.f / d.x, 1.f / d.y, 1.f / d.z);
+      }
+      stack[sp++] = {0, LEAVE};
+      stack[sp++] = {0, 0};
+      continue;
+    }
+    const Bvh &bvh = in_blas ? s.groups[s.insts[inst].group].bvh : s.tlas;
+    if (en.cnt == 0) {
+      const Node &nd = bvh.nodes[en.ref];
+      if (ctr) ctr->nodes++;
+      float tn[2] = {0.f, 0.f};
+      bool h[2];
+      h[0] = nd.cnt[0] >= 0 && box_test(o, inv, nd.lo0, nd.hi0, tmin, best.t, tn[0]);
+      h[1] = nd.cnt[1] >= 0 && box_test(o, inv, nd.lo1, nd.hi1, tmin, best.t, tn[1]);
+      if (h[0] && h[1]) {
+        int nr = tn[1] < tn[0] ? 1 : 0;
+        stack[sp++] = {nd.ref[1 - nr], nd.cnt[1 - nr]};
+        stack[sp++] = {nd.ref[nr], nd.cnt[nr]};
+      } else if (h[0]) {
+        stack[sp++] = {nd.ref[0], nd.cnt[0]};
+      } else if (h[1]) {
+        stack[sp++] = {nd.ref[1], nd.cnt[1]};
+      }
+      continue;
+    }
+    if (!in_blas) {
+      for (int k = en.ref + en.cnt - 1; k >= en.ref; k--) stack[sp++] = {s.tlas.order[k], ENTER};
+      continue;
+    }
+    const Group &g = s.groups[s.insts[inst].group];
+    for (int k = en.ref; k < en.ref + en.cnt; k++) {
+      bool hit = prim_test(s, g, k, o, d, tmin, best, inst, ctr);
+      if (hit && any_hit) return best;
+    }
+  }
+  return best;
+}
+
+Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax)
+{
+  vec3 dir = fix_dir(dir0);
+  Hit best;
+  best.t = tmax;
+  best.u = best.v = 0.f;
+  for (int inst = 0; inst < (int)s.insts.size(); inst++) {
+    const Instance &in = s.insts[inst];
+    vec3 o = org, d = dir;
+    if (!in.w2o.identity) {
+      o = xfm_point(in.w2o, org);
+      d = fix_dir(xfm_vector(in.w2o, dir));
+    }
+    const Group &g = s.groups[in.group];
+    for (int k = 0; k < (int)g.bvh.order.size(); k++) prim_test(s, g, k, o, d, tmin, best, inst, nullptr);
+  }
+  return best;
+}
+
+// ======================================================================
+}  // namespace orc

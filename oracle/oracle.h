@@ -1,0 +1,284 @@
+// oracle.h -- CPU oracle for barney's single-GPU ray-tracing hot path.
+//
+// TEST INFRASTRUCTURE ONLY.  This is a scalar C++ restatement of the
+// reference CPU (embree/cuBQL) path, used by tests/ and by bench.py's
+// cpu_baseline leg as the checker.  The product (barney_amd/) never
+// includes, links or calls anything in this directory.
+//
+// Every function cites the reference file:line it restates (paths relative
+// to the barney v1.4.0 tree).  Arithmetic is written operation by operation
+// in the order the reference writes it and is compiled with
+// -ffp-contract=off, so results are reproducible bit for bit.
+//
+// Shared with the product: only barney_amd/csrc/bnx_numeric.h (sin/cos with
+// a fixed operation sequence, pinned against libm by tests/test_numeric.py)
+// and the POD layouts of include/bnx_rt.h.
+//
+// Parity status of each part is listed in the table at the top of
+// oracle.cpp.
+#pragma once
+
+#include <stdint.h>
+#include <string.h>
+#include <math.h>
+#include <vector>
+#include <atomic>
+#include <thread>
+
+#include "../include/bnx_rt.h"
+#include "../barney_amd/csrc/bnx_numeric.h"
+
+namespace orc {
+
+// ------------------------------------------------------------------ vec3
+struct vec3 {
+  float x, y, z;
+  vec3() : x(0), y(0), z(0) {}
+  vec3(float a) : x(a), y(a), z(a) {}
+  vec3(float a, float b, float c) : x(a), y(b), z(c) {}
+  float operator[](int i) const { return i == 0 ? x : (i == 1 ? y : z); }
+  float &operator[](int i) { return i == 0 ? x : (i == 1 ? y : z); }
+};
+inline vec3 operator+(vec3 a, vec3 b) { return vec3(a.x + b.x, a.y + b.y, a.z + b.z); }
+inline vec3 operator-(vec3 a, vec3 b) { return vec3(a.x - b.x, a.y - b.y, a.z - b.z); }
+inline vec3 operator-(vec3 a) { return vec3(-a.x, -a.y, -a.z); }
+inline vec3 operator*(vec3 a, vec3 b) { return vec3(a.x * b.x, a.y * b.y, a.z * b.z); }
+inline vec3 operator*(vec3 a, float s) { return vec3(a.x * s, a.y * s, a.z * s); }
+inline vec3 operator*(float s, vec3 a) { return vec3(s * a.x, s * a.y, s * a.z); }
+inline vec3 operator/(vec3 a, float s) { return vec3(a.x / s, a.y / s, a.z / s); }
+inline bool operator==(vec3 a, vec3 b) { return a.x == b.x && a.y == b.y && a.z == b.z; }
+inline bool operator!=(vec3 a, vec3 b) { return !(a == b); }
+inline float fmn(float a, float b) { return a < b ? a : b; }
+inline float fmx(float a, float b) { return a > b ? a : b; }
+inline vec3 vmin(vec3 a, vec3 b) { return vec3(fmn(a.x, b.x), fmn(a.y, b.y), fmn(a.z, b.z)); }
+inline vec3 vmax(vec3 a, vec3 b) { return vec3(fmx(a.x, b.x), fmx(a.y, b.y), fmx(a.z, b.z)); }
+inline float dot(vec3 a, vec3 b) { return a.x * b.x + a.y * b.y + a.z * b.z; }
+inline vec3 cross(vec3 a, vec3 b)
+{
+  return vec3(a.y * b.z - a.z * b.y, a.z * b.x - a.x * b.z, a.x * b.y - a.y * b.x);
+}
+inline float length(vec3 a) { return sqrtf(dot(a, a)); }
+inline vec3 normalize(vec3 a) { return a * (1.f / sqrtf(dot(a, a))); }
+inline float reduce_max(vec3 a) { return fmx(fmx(a.x, a.y), a.z); }
+inline float reduce_min(vec3 a) { return fmn(fmn(a.x, a.y), a.z); }
+inline vec3 vabs(vec3 a) { return vec3(fabsf(a.x), fabsf(a.y), fabsf(a.z)); }
+inline vec3 load3(const float *p) { return vec3(p[0], p[1], p[2]); }
+
+inline uint32_t fbits(float f) { uint32_t u; memcpy(&u, &f, 4); return u; }
+
+static const float ONE_PI = (float)M_PI;
+static const float TWO_PI = 2.f * ONE_PI;
+static const float FOUR_PI = 4.f * ONE_PI;
+static const float ONE_OVER_PI = 1.f / ONE_PI;
+static const float ONE_OVER_TWO_PI = 1.f / TWO_PI;
+static const float ONE_OVER_FOUR_PI = 1.f / FOUR_PI;
+
+// ------------------------------------------------------------------- RNG
+// RNGSeed / Random2 -- native/common/barney-common.h:21-67
+struct RNGSeed {
+  uint64_t state = 0xcafef00dd15ea5e5ull;
+  void seed(uint32_t a, uint32_t b)               // :22-28
+  {
+    state ^= (uint64_t)(uint32_t)(2u * a + 1u);  // 32-bit unsigned arithmetic, then widened
+    next();
+    state ^= (uint64_t)(uint32_t)(2u * b);
+    next();
+  }
+  uint32_t next()                                 // :30-38
+  {
+    uint64_t x = state;
+    unsigned count = (unsigned)(x >> 61);
+    state = x * 6364136223846793005ull;
+    x ^= x >> 22;
+    return (uint32_t)(x >> (22 + count));
+  }
+  uint32_t next_h(uint64_t hash)                  // :43-49
+  {
+    state ^= (hash << 1);
+    return next();
+  }
+};
+struct Random2 : RNGSeed {
+  Random2(uint32_t a, uint32_t b) { seed(a, b); } // :53-55
+  Random2(RNGSeed &parent, uint32_t b)            // :56-58
+  {
+    state = parent.state;
+    parent.next();
+    next_h((uint64_t)(b ? b : 290374u));
+  }
+  float operator()() { return (float)(next() & 0x00FFFFFF) * (1.f / (float)0x01000000); } // :63-66
+};
+
+// FNV-1 style hash -- barney-common.h:88-108
+inline uint64_t fnv_hash(uint32_t v) { return 0xcbf29ce484222325ull ^ (0x100000001b3ull * v); }
+inline uint64_t fnv_hash(uint64_t h, uint32_t v) { return h * 0x100000001b3ull ^ v; }
+inline uint64_t fnv_hash3(uint32_t a, uint32_t b, uint32_t c) { return fnv_hash(fnv_hash(fnv_hash(a), b), c); }
+
+// ---------------------------------------------------------------- half
+// vec3h storage of Ray::N and PathState::throughput (render/Ray.h:17-23,76).
+// IEEE round-to-nearest-even, the device's conversion (common/half.h:48-52).
+uint16_t f32_to_f16(float f);
+float f16_to_f32(uint16_t h);
+inline vec3 quantize_half(vec3 v)
+{
+  return vec3(f16_to_f32(f32_to_f16(v.x)), f16_to_f32(f32_to_f16(v.y)), f16_to_f32(f32_to_f16(v.z)));
+}
+
+// ------------------------------------------------------------------ box
+struct Box {
+  vec3 lo, hi;
+  Box() : lo(INFINITY), hi(-INFINITY) {}
+  Box(vec3 a, vec3 b) : lo(a), hi(b) {}
+  void grow(const Box &b) { lo = vmin(lo, b.lo); hi = vmax(hi, b.hi); }
+  void grow(vec3 p) { lo = vmin(lo, p); hi = vmax(hi, p); }
+};
+inline float half_area(const Box &b)
+{
+  vec3 d = b.hi - b.lo;
+  return (d.x * d.y + d.y * d.z) + d.z * d.x;
+}
+
+// ------------------------------------------------------------------ BVH
+// Node layout shared with the GPU builder: two child boxes + refs.  A child
+// with count > 0 is a leaf over prim slots [ref, ref+count); count == 0 is
+// the inner node 'ref'; count < 0 is an empty child.
+struct Node {
+  float lo0[3], hi0[3], lo1[3], hi1[3];
+  int32_t ref[2];
+  int32_t cnt[2];
+};
+static_assert(sizeof(Node) == 64, "node layout");
+
+struct Bvh {
+  std::vector<Node> nodes;
+  std::vector<int32_t> order;   // BVH slot -> original prim index
+  Box root;
+  int max_depth = 0;
+};
+
+enum { SAH_BINS = 16, MAX_SAH_DEPTH = 48 };
+void build_bvh(const std::vector<Box> &boxes, int max_leaf, Bvh &out);
+
+// --------------------------------------------------------------- scene
+struct Material {
+  int type = BNX_MATERIAL_MATTE;
+  bool color_from_attribute = false;
+  float color[4] = {0.8f, 0.8f, 0.8f, 1.f};
+  float opacity = 1.f;
+};
+
+struct Geom {
+  int type = 0;                     // 0 triangles, 1 spheres
+  int user_id = 0;
+  Material mat;
+  std::vector<vec3> vertices;
+  std::vector<int32_t> indices;     // 3 per triangle
+  std::vector<vec3> centers;
+  std::vector<float> radii;
+  float default_radius = 0.1f;
+  std::vector<vec3> colors;         // per prim, may be empty
+  int num_prims() const { return type == 0 ? (int)indices.size() / 3 : (int)centers.size(); }
+};
+
+struct Group {
+  int type = 0;                     // 0 triangles group, 1 user group
+  std::vector<int> geoms;           // indices into Scene::geoms
+  std::vector<int32_t> prim_geom;   // flattened prim list: geom index in group
+  std::vector<int32_t> prim_id;     //                      prim index in geom
+  Bvh bvh;
+};
+
+struct Affine {                       // owl affine3f: linear (columns vx,vy,vz) + p
+  vec3 vx{1, 0, 0}, vy{0, 1, 0}, vz{0, 0, 1}, p{0, 0, 0};
+  bool identity = true;
+};
+
+inline vec3 xfm_point(const Affine &a, vec3 p) { return ((a.vx * p.x + a.vy * p.y) + a.vz * p.z) + a.p; }
+inline vec3 xfm_vector(const Affine &a, vec3 v) { return (a.vx * v.x + a.vy * v.y) + a.vz * v.z; }
+
+struct Instance {
+  int group;
+  int user_id;
+  Affine o2w, w2o;
+};
+
+struct Scene {
+  std::vector<Geom> geoms;
+  std::vector<Group> groups;
+  std::vector<Instance> insts;
+  Bvh tlas;
+  std::vector<bnx_dir_light> dir_lights;
+};
+
+void build_scene(Scene &s);        // BLAS per group + TLAS
+
+// ------------------------------------------------------------- tracing
+// direction fix: traceRays.dev.cu:41-43 (exact zero -> 1e-6), then
+// rtcore/cuda/TraceInterface.h:37-39 (|d| < 1e-6 -> 1e-6)
+inline vec3 fix_dir(vec3 d)
+{
+  for (int k = 0; k < 3; k++) {
+    if (d[k] == 0.f) d[k] = 1e-6f;
+    if (fabsf(d[k]) < 1e-6f) d[k] = 1e-6f;
+  }
+  return d;
+}
+
+struct TraceCounters {
+  uint64_t nodes = 0, tris = 0, spheres = 0, rays = 0;
+};
+
+struct Hit {
+  float t;
+  float u, v;
+  int prim = -1, geom = -1, inst = -1;
+};
+
+// closest hit (any_hit=false) or occlusion (any_hit=true)
+Hit trace_bvh(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, bool any_hit, TraceCounters *ctr);
+Hit trace_brute(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax);
+
+// ------------------------------------------------------------- render
+struct Frame {
+  int w = 0, h = 0, accum_id = 0;
+  std::vector<int64_t> accum;      // 4 fixed-point channels per pixel (2^-32 units)
+  std::vector<float> depth;
+  std::vector<int32_t> prim, inst, obj;
+  std::vector<float> normal;       // 3 per pixel
+  void resize(int w, int h);
+  void reset();
+};
+
+struct RenderStats {
+  uint64_t rays = 0, shadow = 0, generations = 0, volume_samples = 0;
+  TraceCounters ctr;
+};
+
+void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &r, Frame &fb, int spp,
+            int num_threads, RenderStats *stats);
+
+// thread pool over [0,n) in chunks (rtcore/cpu/Compute.cpp:17-80: std::thread
+// workers pulling task indices from an atomic counter)
+template <typename F> void parallel_for(int n, int num_threads, F &&f)
+{
+  if (num_threads <= 1 || n < 256) {
+    for (int i = 0; i < n; i++) f(i);
+    return;
+  }
+  std::atomic<int> next{0};
+  const int chunk = 64;
+  auto worker = [&]() {
+    while (true) {
+      int b = next.fetch_add(chunk);
+      if (b >= n) break;
+      int e = b + chunk < n ? b + chunk : n;
+      for (int i = b; i < e; i++) f(i);
+    }
+  };
+  std::vector<std::thread> ts;
+  for (int t = 0; t < num_threads - 1; t++) ts.emplace_back(worker);
+  worker();
+  for (auto &t : ts) t.join();
+}
+
+} // namespace orc

@@ -1,0 +1,523 @@
+// render.cpp -- CPU oracle of barney's wavefront frame loop: ray
+// generation, hit programs, path-tracing shade and accumulation.
+//
+// TEST INFRASTRUCTURE ONLY (see oracle.h).
+#include "oracle.h"
+
+#include <algorithm>
+
+namespace orc {
+
+namespace {
+
+const float BNX_INF = INFINITY;
+enum { MAX_PATH_DEPTH = 10, MAX_DIFFUSE_BOUNCES = 1 };   // shadeRays.cu:17,34
+const float CLAMP_F_R = 13.f;                            // shadeRays.cu:27
+
+// Ray + PathState (render/Ray.h:17-93), scalar form
+struct PathRay {
+  vec3 org, dir;
+  float tmax;
+  RNGSeed rng;
+  bool had_hit = false;
+  vec3 P, N;                 // N holds the fp16-rounded normal (vec3h)
+  vec3 albedo;               // Lambertian::albedo (the only BSDF of these scenes)
+  bool is_shadow = false, is_specular = false, in_medium = false;
+  float miss[4];
+  vec3 tp;                   // PathState::throughput, fp16-rounded (vec3h)
+  int pixel = 0;
+  float mis = 0.f;
+  int ndiff = 0;
+};
+
+struct HitIDs {
+  float depth;
+  int prim, inst, obj;
+};
+
+// Owl's frame(N) is un-vendored: orthonormal basis of Duff, Burgess,
+// Christensen, Hery, Kensler, Liani, Villemin, "Building an Orthonormal
+// Basis, Revisited", JCGT 6(1), 2017 (listing 3).
+void onb(vec3 n, vec3 &b1, vec3 &b2)
+{
+  float sign = copysignf(1.0f, n.z);
+  float a = -1.0f / (sign + n.z);
+  float b = (n.x * n.y) * a;
+  b1 = vec3(1.0f + ((sign * n.x) * n.x) * a, sign * b, -sign * n.x);
+  b2 = vec3(b, sign + (n.y * n.y) * a, -n.y);
+}
+
+// randomDirection (render/DG.h:186-196)
+vec3 random_direction(Random2 &rng)
+{
+  while (true) {
+    vec3 v;
+    v.x = 1.f - 2.f * rng();
+    v.y = 1.f - 2.f * rng();
+    v.z = 1.f - 2.f * rng();
+    if (dot(v, v) <= 1.f) return normalize(v);
+  }
+}
+
+struct LightSample {
+  vec3 direction;
+  float distance;
+  vec3 radiance;
+  float pdf;
+};
+
+// sampleEnvLight without an env-map texture (shadeRays.cu:265-291)
+bool sample_env(LightSample &ls, float ambient, vec3 N, Random2 &random)
+{
+  ls.direction = random_direction(random);
+  ls.distance = BNX_INF;
+  if (N != vec3(0.f)) {
+    ls.radiance = vec3(TWO_PI * ambient);
+    if (dot(ls.direction, N) < 0.f) ls.direction = -ls.direction;
+    ls.pdf = ONE_OVER_TWO_PI;
+  } else {
+    ls.radiance = vec3(FOUR_PI * ambient);
+    ls.pdf = ONE_OVER_FOUR_PI;
+  }
+  return true;
+}
+
+// sampleDirLights (shadeRays.cu:138-189), reservoir of 2
+bool sample_dir(LightSample &ls, const std::vector<bnx_dir_light> &lights, vec3 N, Random2 &random)
+{
+  int n = (int)lights.size();
+  if (n == 0) return false;
+  int lID[2];
+  float weights[2];
+  float sum = 0.f;
+  for (int i = 0; i < 2; i++) {
+    lID[i] = std::min((int)(random() * (float)n), n - 1);
+    weights[i] = 0.f;
+    const bnx_dir_light &L = lights[lID[i]];
+    vec3 rad = load3(L.color) * L.radiance;
+    vec3 ldir = -load3(L.direction);
+    float w = (N == vec3(0.f)) ? 1.f : dot(ldir, N);
+    if (w <= 1e-3f) continue;
+    w *= reduce_max(rad);
+    if (w <= 1e-3f) continue;
+    weights[i] = w;
+    sum += w;
+  }
+  if (sum == 0.f) return false;
+  float r = random() * sum;
+  int i = 0;
+  while (i < 2 && r >= weights[i]) {
+    r -= weights[i];
+    ++i;
+  }
+  if (i == 2) return false;
+  const bnx_dir_light &L = lights[lID[i]];
+  ls.direction = -load3(L.direction);
+  ls.distance = BNX_INF;
+  float denom = (weights[i] / sum) * (2.f / (float)n);
+  ls.radiance = ((load3(L.color) * L.radiance) * 1.f) / denom;
+  ls.pdf = BNX_INF;
+  return weights[i] != 0.f;
+}
+
+// sampleLights (shadeRays.cu:293-402); no quad or point lights here
+bool sample_lights(LightSample &ls, const Scene &s, const bnx_renderer &r, vec3 N, Random2 &random,
+                   bool &needs_mis, bool &is_dir)
+{
+  needs_mis = false;
+  LightSample els, dls;
+  float elsW = sample_env(els, r.ambient_radiance, N, random) ? reduce_max(els.radiance) : 0.f;
+  float alsW = 0.f;
+  float dlsW = sample_dir(dls, s.dir_lights, N, random) ? reduce_max(dls.radiance) : 0.f;
+  float plsW = 0.f;
+  float sum = ((alsW + dlsW) + elsW) + plsW;
+  if (sum == 0.f) return false;
+  elsW *= 1.f / sum;
+  alsW *= 1.f / sum;
+  dlsW *= 1.f / sum;
+  plsW *= 1.f / sum;
+  float rr = random();
+  if (rr <= alsW) {
+    return false;                         // no area lights in these scenes
+  } else if (rr <= alsW + elsW) {
+    ls = els;
+    ls.radiance = ls.radiance * (1.f / elsW);
+    needs_mis = true;
+  } else if (rr <= (alsW + elsW) + plsW) {
+    return false;                         // no point lights
+  } else {
+    ls = dls;
+    ls.radiance = ls.radiance * (1.f / dlsW);
+    is_dir = true;
+  }
+  if (std::isnan(ls.pdf) || ls.pdf <= 0.f) return false;
+  return true;
+}
+
+// Lambertian (packedBSDF/Lambertian.h:46-85)
+struct DGeo {
+  vec3 Ng, Ns, wo, P;
+};
+float lambert_pdf(const DGeo &dg, vec3 wi)
+{
+  vec3 N = dg.Ng;
+  if (dot(wi, N) < 0.f) N = -N;
+  float c = fmx(dot(wi, N), 0.f);
+  return c * ONE_OVER_PI;
+}
+void lambert_eval(vec3 albedo, const DGeo &dg, vec3 wi, vec3 &value, float &pdf)
+{
+  float c = fmx(dot(wi, dg.Ns), 0.f);
+  pdf = c * ONE_OVER_PI;
+  value = (albedo * ONE_OVER_PI) * c;
+}
+void lambert_scatter(vec3 albedo, const DGeo &dg, Random2 &random, vec3 &dir, float &pdf, vec3 &f_r)
+{
+  vec3 N = dg.Ng;
+  if (dot(dg.wo, N) < 0.f) N = -N;
+  float sx = random();
+  float sy = random();
+  // cosineSampleHemisphere (render/DG.h:158-164)
+  float phi = TWO_PI * sx;
+  float cosT = sqrtf(sy);
+  float sinT = sqrtf(1.0f - sy);
+  float sp, cp;
+  bnx_num::sincos(phi, sp, cp);
+  vec3 local(cp * sinT, sp * sinT, cosT);
+  vec3 b1, b2;
+  onb(N, b1, b2);
+  dir = (b1 * local.x + b2 * local.y) + N * local.z;
+  pdf = local.z * ONE_OVER_PI;
+  f_r = albedo;
+}
+
+// bounce() (shadeRays.cu:474-880) for surfaces with a Lambertian BSDF
+void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, PathRay &shadow, int depth)
+{
+  const float EPS = 1e-4f;
+  const bool no_hit = !ray.had_hit;
+  const vec3 in_tp = ray.tp;
+  if (ray.is_shadow) {
+    if (no_hit) frag = ray.tp * ray.mis;
+    shadow.tmax = -1.f;
+    ray.tmax = -1.f;
+    return;
+  }
+  vec3 Ng = ray.N;
+  const bool is_vol = (Ng == vec3(0.f));
+  if (!is_vol) Ng = normalize(Ng);
+  const bool front = dot(ray.dir, Ng) < 0.f;
+  vec3 Ngff = front ? Ng : -Ng;
+  if (no_hit) {
+    if (depth == 0) frag = vec3(ray.miss[0], ray.miss[1], ray.miss[2]);
+    else frag = (ray.tp * vec3(rr.ambient_radiance)) * ray.mis;
+    ray.tmax = -1.f;
+    return;
+  }
+  Random2 random(ray.rng, fbits(ray.tmax));
+  DGeo dg;
+  dg.P = ray.P;
+  dg.Ng = Ng;
+  dg.Ns = Ng;
+  dg.wo = -normalize(ray.dir);
+  const float off_eps = fmx(EPS, 1e-6f * reduce_max(vabs(dg.P)));   // safe_eps (:36-41)
+  vec3 ff_off = is_vol ? dg.wo : Ngff;
+
+  LightSample ls;
+  bool needs_mis = false, is_dir = false;
+  if (sample_lights(ls, s, rr, Ngff, random, needs_mis, is_dir)) {
+    vec3 fv;
+    float fpdf;
+    lambert_eval(ray.albedo, dg, ls.direction, fv, fpdf);
+    if (!(fpdf > 0.f) || reduce_max(fv) < 1e-4f) {
+      shadow.tmax = -1.f;
+    } else {
+      fv = vmin(fv, vec3(CLAMP_F_R));
+      float s1 = std::isinf(ls.pdf) ? 1.f : 1.f / ls.pdf;
+      float s2 = is_vol ? 1.f : ONE_OVER_PI * fabsf(dot(dg.Ng, ls.direction));
+      vec3 tp_sr = (((in_tp * s1) * fv) * ls.radiance) * s2;
+      // makeShadowRay (render/Ray.h:146-157)
+      shadow.had_hit = false;
+      shadow.is_shadow = true;
+      shadow.dir = ls.direction;
+      shadow.org = dg.P + off_eps * ff_off;
+      shadow.tmax = ls.distance * (1.f - 2.f * off_eps);
+      shadow.tp = quantize_half(tp_sr);
+      shadow.rng = ray.rng;
+      ray.rng.next_h((uint64_t)fbits(ray.tmax));
+      shadow.pixel = ray.pixel;
+      shadow.mis = 1.f;
+      if (!is_dir && needs_mis) {
+        float pl = ONE_OVER_FOUR_PI;      // EnvMapLight::DD::pdf without texture (light/EnvMap.h:142-151)
+        float ps = lambert_pdf(dg, ls.direction);
+        shadow.mis = pl / ((pl + ps) + 1e-10f);
+        if (shadow.mis < 1e-5f) shadow.tmax = -1.f;
+      }
+    }
+  }
+  if (depth >= MAX_PATH_DEPTH) {
+    ray.tmax = -1.f;
+    return;
+  }
+  ray.had_hit = false;
+  ray.tmax = BNX_INF;
+  vec3 sdir, f_r;
+  float spdf;
+  lambert_scatter(ray.albedo, dg, random, sdir, spdf, f_r);
+  if (!(spdf > 0.f) || spdf <= 1e-6f) return;
+  // DIFFUSE scatter
+  if (ray.ndiff >= MAX_DIFFUSE_BOUNCES) {
+    ray.tmax = -1.f;
+    return;
+  }
+  ray.ndiff = ray.ndiff + 1;
+  ray.is_specular = false;
+  ray.org = dg.P + ((1.f * off_eps) * ff_off);
+  ray.dir = normalize(sdir);
+  float cosf_ = is_vol ? 1.f : fabsf(dot(dg.Ng, normalize(ray.dir)));
+  float den = std::isinf(spdf) ? 1.f : spdf + 1e-10f;
+  vec3 sf = (f_r * cosf_) / den;
+  sf = vmin(sf, vec3(2.5f));
+  ray.tp = quantize_half(ray.tp * sf);
+  if (needs_mis && !std::isinf(spdf)) {
+    float ps = lambert_pdf(dg, ray.dir);
+    float pl = ONE_OVER_FOUR_PI;
+    ray.mis = ps / (ps + pl);
+  } else {
+    ray.mis = 1.f;
+  }
+}
+
+// Hit programs: TrianglesPrograms::anyHit (Triangles.dev.cu:19-135) and
+// SpheresPrograms::closestHit (Spheres.dev.cu:42-107) ending in
+// DeviceMaterial::setHit (material/DeviceMaterial.h:64-72) with AnariMatte.
+void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compat13)
+{
+  if (h.prim < 0) return;
+  const Instance &in = s.insts[h.inst];
+  const Group &g = s.groups[in.group];
+  const Geom &geom = s.geoms[g.geoms[h.geom]];
+  vec3 P, Nw;
+  float color[4] = {0.f, 0.f, 0.f, 1.f};         // HitAttributes default (render/HitAttributes.h:49-58)
+  bool have_attr = !geom.colors.empty();
+  if (have_attr) {
+    vec3 c = geom.colors[h.prim];
+    color[0] = c.x; color[1] = c.y; color[2] = c.z;
+  }
+  if (geom.type == 0) {
+    const int32_t *ix = &geom.indices[3 * h.prim];
+    vec3 v0 = geom.vertices[ix[0]], v1 = geom.vertices[ix[1]], v2 = geom.vertices[ix[2]];
+    vec3 n = normalize(cross(v1 - v0, v2 - v0));
+    vec3 osN = normalize(n);
+    vec3 osP = (((1.f - h.u) - h.v) * v0 + h.u * v1) + h.v * v2;
+    if (in.o2w.identity) {
+      P = osP;
+      Nw = osN;
+    } else {
+      P = xfm_point(in.o2w, osP);
+      // normal: transpose of the world-to-object linear part
+      const Affine &w = in.w2o;
+      Nw = vec3(w.vx.x * osN.x + w.vx.y * osN.y + w.vx.z * osN.z,
+                w.vy.x * osN.x + w.vy.y * osN.y + w.vy.z * osN.z,
+                w.vz.x * osN.x + w.vz.y * osN.y + w.vz.z * osN.z);
+    }
+  } else {
+    vec3 o = r.org, d = fix_dir(r.dir);
+    if (!in.w2o.identity) {
+      o = xfm_point(in.w2o, r.org);
+      d = fix_dir(xfm_vector(in.w2o, d));
+    }
+    vec3 ctr = geom.centers[h.prim];
+    float rad = geom.radii.empty() ? geom.default_radius : geom.radii[h.prim];
+    float t_move = fmx(0.f, length(ctr - o) - 3.f * rad);
+    vec3 om = o + t_move * d;
+    vec3 objP = om + h.u * d;
+    P = in.o2w.identity ? objP : xfm_point(in.o2w, objP);
+    vec3 objN = (objP == ctr) ? vec3(1.f, 0.f, 0.f) : (objP - ctr);
+    vec3 wN = in.o2w.identity ? objN : xfm_vector(in.o2w, objN);
+    Nw = normalize(wN);
+  }
+  const Material &m = geom.mat;
+  float base[4];
+  if (m.color_from_attribute) memcpy(base, color, sizeof(base));
+  else memcpy(base, m.color, sizeof(base));
+  vec3 albedo = (0.85f * vec3(base[0], base[1], base[2])) * ONE_OVER_PI;   // AnariMatte.h:58-63
+  r.had_hit = true;
+  r.tmax = h.t;
+  r.P = P;
+  r.N = quantize_half(Nw);
+  r.albedo = albedo;
+  if (ids) {
+    ids->depth = h.t;
+    ids->prim = h.prim;
+    ids->inst = (geom.type == 1 && compat13) ? 13 : in.user_id;
+    ids->obj = geom.user_id;
+  }
+}
+
+inline void atomic_add64(int64_t *p, int64_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
+inline int64_t to_fixed(float f) { return (int64_t)(f * 0x1p32f); }
+
+} // namespace
+
+void Frame::resize(int w_, int h_)
+{
+  w = w_;
+  h = h_;
+  accum.assign((size_t)w * h * 4, 0);
+  depth.assign((size_t)w * h, BNX_INF);
+  prim.assign((size_t)w * h, -1);
+  inst.assign((size_t)w * h, -1);
+  obj.assign((size_t)w * h, -1);
+  normal.assign((size_t)w * h * 3, 0.f);
+  accum_id = 0;
+}
+
+void Frame::reset() { resize(w, h); }
+
+void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame &fb, int spp, int threads,
+            RenderStats *stats)
+{
+  const int W = fb.w, H = fb.h;
+  const bool compat13 = (rr.flags & BNX_RENDER_COMPAT_SPHERE_INSTID13) != 0;
+  const vec3 dir00 = load3(cam.dir_00), du = load3(cam.dir_du), dv = load3(cam.dir_dv), lens = load3(cam.lens_00);
+  for (int sidx = 0; sidx < spp; sidx++) {
+    const int accumID = fb.accum_id;
+    // ---------------------------------------------- _generateRays (generateRays.cu:42-245)
+    std::vector<PathRay> queue((size_t)W * H);
+    parallel_for(W * H, threads, [&](int pix) {
+      int ix = pix % W, iy = pix / W;
+      PathRay &ray = queue[pix];
+      ray.mis = 0.f;
+      ray.pixel = pix;
+      Random2 rand((uint32_t)(ix + W * accumID), (uint32_t)(iy + H * accumID));
+      ray.rng.seed((uint32_t)(ix + accumID * W), (uint32_t)iy);
+      float pu = (accumID == 0) ? .5f : rand();
+      float pv = (accumID == 0) ? .5f : rand();
+      float image_u = ((float)ix + pu) / (float)W;
+      float image_v = ((float)iy + pv) / (float)H;
+      float aspect = (float)W / (float)H;
+      ray.org = lens;
+      vec3 d = (dir00 + ((1.f * aspect) * (image_u - .5f)) * du) + (1.f * (image_v - .5f)) * dv;
+      ray.dir = normalize(d);
+      ray.had_hit = false;
+      ray.is_shadow = false;
+      ray.in_medium = false;
+      ray.tmax = 1e30f;
+      if (rr.cut_plane[3] > -1e28f) {
+        vec3 N = load3(rr.cut_plane);
+        float dist_org = dot(N, ray.org) + rr.cut_plane[3];
+        float denom = dot(N, ray.dir);
+        if (dist_org >= 0.f) {
+          if (denom < 0.f) ray.tmax = fmn(ray.tmax, -dist_org / denom);
+        } else {
+          if (denom > 0.f) ray.org = ray.org + (-dist_org / denom) * ray.dir;
+          else ray.tmax = 0.f;
+        }
+      }
+      ray.ndiff = 0;
+      float t = ((float)iy + .5f) / (float)H;
+      if (rr.bg_color[3] >= 0.f) {
+        memcpy(ray.miss, rr.bg_color, sizeof(ray.miss));
+      } else {
+        float a[4] = {0.9f, 0.9f, 0.9f, 1.f}, b[4] = {0.15f, 0.25f, .8f, 1.f};
+        for (int k = 0; k < 4; k++) ray.miss[k] = (1.0f - t) * a[k] + t * b[k];
+      }
+      ray.tp = vec3(1.f);
+    });
+    for (int gen = 0;; gen++) {
+      if (queue.empty()) break;
+      if (rr.max_generations > 0 && gen >= rr.max_generations) break;
+      const int n = (int)queue.size();
+      std::vector<HitIDs> ids(gen == 0 ? n : 0);
+      std::vector<TraceCounters> ctrs(threads > 0 ? 1 : 1);
+      std::atomic<uint64_t> nn{0}, nt{0}, ns{0}, nsh{0};
+      // ------------------------------------------ trace (traceRays.dev.cu:26-51)
+      parallel_for(n, threads, [&](int i) {
+        PathRay &r = queue[i];
+        TraceCounters c;
+        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, false, stats ? &c : nullptr);
+        if (gen == 0) ids[i] = {BNX_INF, -1, -1, -1};
+        apply_hit(s, r, h, gen == 0 ? &ids[i] : nullptr, compat13);
+        if (stats) {
+          nn += c.nodes;
+          nt += c.tris;
+          ns += c.spheres;
+          if (r.is_shadow) nsh++;
+        }
+      });
+      if (stats) {
+        stats->rays += n;
+        stats->shadow += nsh;
+        stats->generations++;
+        stats->ctr.nodes += nn;
+        stats->ctr.tris += nt;
+        stats->ctr.spheres += ns;
+        stats->ctr.rays += n;
+      }
+      // ------------------------------------------ _shadeRays (shadeRays.cu:883-1041)
+      std::vector<PathRay> out((size_t)2 * n);
+      std::vector<char> live((size_t)2 * n, 0);
+      const float clampMax = 10.f * (float)(1 + accumID);
+      parallel_for(n, threads, [&](int i) {
+        PathRay ray = queue[i];
+        float alpha = (gen == 0) ? (ray.had_hit ? 1.f : ray.miss[3]) : 0.f;
+        float inZ = ray.tmax;
+        vec3 inN = ray.had_hit ? ray.N : vec3(0.f);
+        vec3 frag(0.f);
+        PathRay sh;
+        sh.tmax = -1.f;
+        bounce(s, rr, frag, ray, sh, gen);
+        if (sh.tmax > 0.f) { out[2 * i] = sh; live[2 * i] = 1; }
+        if (ray.tmax > 0.f) { out[2 * i + 1] = ray; live[2 * i + 1] = 1; }
+        frag = vmin(frag, vec3(clampMax));
+        int pix = ray.pixel;
+        if (gen == 0) {
+          if (accumID == 0) {
+            fb.depth[pix] = inZ;
+            fb.prim[pix] = ids[i].prim;
+            fb.inst[pix] = ids[i].inst;
+            fb.obj[pix] = ids[i].obj;
+            fb.normal[3 * pix + 0] = inN.x;
+            fb.normal[3 * pix + 1] = inN.y;
+            fb.normal[3 * pix + 2] = inN.z;
+          } else if (inZ < fb.depth[pix]) {
+            fb.depth[pix] = inZ;
+          }
+          if (alpha > 0.f) atomic_add64(&fb.accum[4 * (size_t)pix + 3], to_fixed(alpha));
+        }
+        if (frag.x > 0.f) atomic_add64(&fb.accum[4 * (size_t)pix + 0], to_fixed(frag.x));
+        if (frag.y > 0.f) atomic_add64(&fb.accum[4 * (size_t)pix + 1], to_fixed(frag.y));
+        if (frag.z > 0.f) atomic_add64(&fb.accum[4 * (size_t)pix + 2], to_fixed(frag.z));
+      });
+      std::vector<PathRay> next;
+      next.reserve(n);
+      for (size_t k = 0; k < out.size(); k++)
+        if (live[k]) next.push_back(out[k]);
+      queue.swap(next);
+    }
+    fb.accum_id++;
+  }
+}
+
+// PerspectiveCamera::commit (native/Camera.cu:82-101), fovy in degrees
+void camera_perspective(const float *pos, const float *dir, const float *up, float fovy, bnx_camera &out)
+{
+  vec3 d00 = normalize(load3(dir));
+  vec3 ddu = normalize(cross(d00, load3(up)));
+  vec3 ddv = normalize(cross(ddu, d00));
+  float sc = (float)(1.f / (2.0f * tanf(((0.5f * fovy) * (float)M_PI) / 180.0f)));
+  d00 = d00 * sc;
+  memset(&out, 0, sizeof(out));
+  out.type = BNX_CAMERA_PERSPECTIVE;
+  for (int k = 0; k < 3; k++) {
+    out.dir_00[k] = d00[k];
+    out.dir_du[k] = ddu[k];
+    out.dir_dv[k] = ddv[k];
+    out.lens_00[k] = pos[k];
+  }
+  out.aperture_radius = 0.f;
+  out.focus_distance = 1.f;
+}
+
+} // namespace orc

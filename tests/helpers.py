@@ -1,0 +1,232 @@
+"""Shared helpers for the parity tests (scene construction on both sides,
+canonical BVH comparison)."""
+import math
+
+import numpy as np
+
+from barney_amd import abi, scenes
+
+
+def canonical_bvh(nodes, order, max_leaf):
+    """DFS serialisation of a BVH independent of node numbering: a list of
+    (child box, leaf slot range or subtree marker) in pre-order."""
+    out = []
+    if len(nodes) == 0:
+        return out
+    stack = [0]
+    while stack:
+        i = stack.pop()
+        nd = nodes[i]
+        rec = []
+        for s in (0, 1):
+            lo = nd["lo0"] if s == 0 else nd["lo1"]
+            hi = nd["hi0"] if s == 0 else nd["hi1"]
+            cnt = int(nd["cnt"][s])
+            ref = int(nd["ref"][s])
+            if cnt < 0:
+                rec.append(("empty",))
+            elif cnt > 0:
+                assert cnt <= max_leaf
+                rec.append(("leaf", tuple(lo.tolist()), tuple(hi.tolist()), ref, cnt))
+            else:
+                rec.append(("node", tuple(lo.tolist()), tuple(hi.tolist())))
+        out.append(tuple(rec))
+        for s in (1, 0):
+            if int(nd["cnt"][s]) == 0:
+                stack.append(int(nd["ref"][s]))
+    return out
+
+
+def random_boxes(seed, n, degenerate=False):
+    rng = np.random.default_rng(seed)
+    if degenerate:
+        c = np.zeros((n, 3), np.float32) + np.float32(0.25)
+    else:
+        c = rng.random((n, 3)).astype(np.float32) * 10
+    e = (rng.random((n, 3)).astype(np.float32) * 0.3 + 0.01).astype(np.float32)
+    return np.concatenate([c - e, c + e], 1).astype(np.float32)
+
+
+def triangle_soup(seed, n, extent=4.0, size=0.4):
+    rng = np.random.default_rng(seed)
+    c = (rng.random((n, 3)) - 0.5) * extent
+    v = c[:, None, :] + (rng.random((n, 3, 3)) - 0.5) * size
+    verts = v.reshape(-1, 3).astype(np.float32)
+    idx = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    col = rng.random((n, 3)).astype(np.float32)
+    return verts, idx, col
+
+
+def tie_scene():
+    """Coplanar, shared-edge and exactly duplicated triangles plus two
+    identical spheres: rays through shared edges and duplicates produce equal
+    t for several prims, so the lower-ID rule decides."""
+    g = np.linspace(-1.0, 1.0, 9, dtype=np.float32)
+    verts, idx = [], []
+    for j in range(8):
+        for i in range(8):
+            a = len(verts)
+            verts += [(g[i], g[j], 0.0), (g[i + 1], g[j], 0.0), (g[i + 1], g[j + 1], 0.0), (g[i], g[j + 1], 0.0)]
+            idx += [(a, a + 1, a + 2), (a, a + 2, a + 3)]
+    verts = np.array(verts, np.float32)
+    idx = np.array(idx, np.int32)
+    idx = np.concatenate([idx, idx[:10]])            # exact duplicates of prims 0..9
+    centers = np.array([[0.0, 0.0, 2.0], [0.0, 0.0, 2.0], [0.5, 0.5, 2.0]], np.float32)
+    radii = np.array([0.25, 0.25, 0.125], np.float32)
+    return verts, idx, centers, radii
+
+
+def grid_rays(z0=5.0, n=64, span=1.2, target_z=0.0):
+    """Rays from a plane above the scene straight down through a grid that
+    includes exact edge and vertex coordinates of tie_scene()."""
+    xs = np.linspace(-span, span, n, dtype=np.float32)
+    X, Y = np.meshgrid(xs, xs)
+    rays = np.zeros((n * n, 8), np.float32)
+    rays[:, 0], rays[:, 1], rays[:, 2] = X.ravel(), Y.ravel(), z0
+    rays[:, 4:7] = (0.0, 0.0, -1.0)
+    rays[:, 7] = 1e30
+    return rays
+
+
+def camera_rays(sc):
+    """Primary rays of config scene `sc` at pixel centres (generateRays.cu:
+    accumID 0), in bnx_ray layout, computed in float32 like the kernels."""
+    import oracle
+    cam = oracle.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"],
+                                    sc.camera["fovy"])
+    W, H = sc.width, sc.height
+    ix, iy = np.meshgrid(np.arange(W, dtype=np.float32), np.arange(H, dtype=np.float32))
+    u = (ix + np.float32(0.5)) / np.float32(W)
+    v = (iy + np.float32(0.5)) / np.float32(H)
+    aspect = np.float32(W) / np.float32(H)
+    d00, du, dv = (np.array(list(x), np.float32) for x in (cam.dir_00, cam.dir_du, cam.dir_dv))
+    su = (np.float32(1.0) * aspect) * (u - np.float32(0.5))
+    sv = np.float32(1.0) * (v - np.float32(0.5))
+    d = (d00[None, None, :] + su[..., None] * du[None, None, :]) + sv[..., None] * dv[None, None, :]
+    d = d.astype(np.float32)
+    nrm = np.float32(1.0) / np.sqrt(((d[..., 0] * d[..., 0] + d[..., 1] * d[..., 1]) + d[..., 2] * d[..., 2]))
+    d = d * nrm[..., None]
+    rays = np.zeros((H * W, 8), np.float32)
+    rays[:, 0:3] = np.array(sc.camera["position"], np.float32)
+    rays[:, 4:7] = d.reshape(-1, 3)
+    rays[:, 7] = 1e30
+    return rays
+
+
+def hits_equal(a, b):
+    """Bit-exact comparison of two bnx_hit arrays; returns mismatch indices."""
+    bad = np.zeros(len(a), bool)
+    for f in ("prim_id", "geom_id", "inst", "user_inst_id", "obj_id"):
+        bad |= a[f] != b[f]
+    for f in ("t", "u", "v"):
+        bad |= a[f].view(np.uint32) != b[f].view(np.uint32)
+    return np.nonzero(bad)[0]
+
+
+class SceneSpec:
+    """A scene described once and built on both sides: groups of geometries
+    (kind 0 triangles, 1 spheres) and instances (group, user id, o2w, w2o)."""
+
+    def __init__(self):
+        self.geoms = []      # (kind, kwargs)
+        self.groups = []     # (kind, [geom indices])
+        self.insts = []      # (group, user_id, o2w or None, w2o or None)
+        self.lights = []
+
+    def tri(self, vertices, indices, colors=None, user_id=0):
+        self.geoms.append((0, dict(vertices=vertices, indices=indices, colors=colors, user_id=user_id)))
+        return len(self.geoms) - 1
+
+    def sph(self, centers, radii, colors=None, user_id=0):
+        self.geoms.append((1, dict(centers=centers, radii=radii, colors=colors, user_id=user_id)))
+        return len(self.geoms) - 1
+
+    def group(self, kind, geoms):
+        self.groups.append((kind, list(geoms)))
+        return len(self.groups) - 1
+
+    def inst(self, group, user_id=0, o2w=None, w2o=None):
+        self.insts.append((group, user_id, o2w, w2o))
+
+    def oracle(self, orc):
+        s = orc.Scene()
+        ids = []
+        for kind, kw in self.geoms:
+            if kind == 0:
+                ids.append(s.add_triangles(kw["vertices"], kw["indices"], kw["colors"], user_id=kw["user_id"]))
+            else:
+                ids.append(s.add_spheres(kw["centers"], kw["radii"], kw["colors"], user_id=kw["user_id"]))
+        gids = [s.add_group(k, [ids[g] for g in gl]) for k, gl in self.groups]
+        for g, uid, o2w, w2o in self.insts:
+            s.add_instance(gids[g], uid, o2w, w2o)
+        s.set_dir_lights(self.lights)
+        s.build()
+        return s
+
+    def gpu(self, dev):
+        import barney_amd as bx
+        geoms = []
+        for kind, kw in self.geoms:
+            if kind == 0:
+                geoms.append(bx.Geom.triangles(dev, kw["vertices"], kw["indices"], kw["colors"],
+                                               user_id=kw["user_id"]))
+            else:
+                geoms.append(bx.Geom.spheres(dev, kw["centers"], kw["radii"], kw["colors"],
+                                             user_id=kw["user_id"]))
+        groups = [bx.Group.geoms(dev, k, [geoms[g] for g in gl]).build() for k, gl in self.groups]
+        xf = None
+        if any(o is not None for _, _, o, _ in self.insts):
+            ident = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, 0, 0, 0], np.float32)
+            xf = np.stack([o if o is not None else ident for _, _, o, _ in self.insts])
+        insts = bx.Group.instances(dev, [groups[g] for g, _, _, _ in self.insts],
+                                   [u for _, u, _, _ in self.insts], xf).build()
+        insts._keep = (geoms, groups)
+        return insts
+
+
+def c1_spec(sc):
+    spec = SceneSpec()
+    sph = [spec.sph(s.centers, s.radii, s.colors, user_id=s.user_id) for s in sc.spheres]
+    tri = [spec.tri(m.vertices, m.indices, m.colors, user_id=m.user_id) for m in sc.meshes]
+    if sph:
+        spec.inst(spec.group(1, sph), 0)
+    if tri:
+        spec.inst(spec.group(0, tri), 0)
+    spec.lights = list(sc.dir_lights)
+    return spec
+
+
+def soup_spec():
+    verts, idx, col = triangle_soup(9, 3000)
+    spec = SceneSpec()
+    g0 = spec.tri(verts, idx[:1500], col[:1500], user_id=0)
+    g1 = spec.tri(verts, idx[1500:], col[1500:], user_id=1)
+    rng = np.random.default_rng(1)
+    cs = ((rng.random((500, 3)) - 0.5) * 4).astype(np.float32)
+    rs = (rng.random(500) * 0.1 + 0.01).astype(np.float32)
+    g2 = spec.sph(cs, rs, None, user_id=2)
+    spec.inst(spec.group(1, [g2]), 0)
+    spec.inst(spec.group(0, [g0, g1]), 0)
+    return spec
+
+
+def tie_spec():
+    v, i, c, r = tie_scene()
+    spec = SceneSpec()
+    ga = spec.tri(v, i, None, user_id=5)
+    gb = spec.tri(v, i[:20], None, user_id=6)
+    gs = spec.sph(c, r, None, user_id=7)
+    spec.inst(spec.group(1, [gs]), 0)
+    spec.inst(spec.group(0, [ga, gb]), 0)
+    return spec
+
+
+def instanced_spec():
+    verts, idx, col = triangle_soup(2, 400, extent=2.0)
+    spec = SceneSpec()
+    grp = spec.group(0, [spec.tri(verts, idx, col, user_id=0)])
+    for k, off in enumerate([(0, 0, 0), (4, 0, 0), (-4, 2, 0), (0, -4, 1)]):
+        o2w = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, *off], np.float32)
+        w2o = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, *[-x for x in off]], np.float32)
+        spec.inst(grp, 10 + k, o2w, w2o)
+    return spec

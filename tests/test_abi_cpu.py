@@ -1,0 +1,49 @@
+"""The C-ABI library loads and exports every entry point include/bnx_rt.h
+declares (no compute calls: this runs without a GPU)."""
+import ctypes as C
+import os
+import re
+
+import pytest
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+
+
+def header_functions():
+    txt = open(os.path.join(ROOT, "include", "bnx_rt.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bnx_[a-z0-9_]+)\s*\(", txt)))
+
+
+def test_header_declares_core_entry_points():
+    fns = header_functions()
+    for f in ["bnx_device_create", "bnx_group_build", "bnx_trace", "bnx_render", "bnx_frame_read"]:
+        assert f in fns
+
+
+def test_library_exports_every_declared_symbol():
+    import barney_amd
+    if not os.path.exists(barney_amd.LIB_PATH):
+        pytest.skip("HIP library not built here")
+    L = C.CDLL(barney_amd.LIB_PATH)
+    missing = [f for f in header_functions() if not hasattr(L, f)]
+    assert not missing, missing
+    assert sorted(barney_amd.EXPORTED) == header_functions()
+    assert L.bnx_abi_version() == 1
+
+
+def test_struct_layouts_match_header():
+    from barney_amd import abi
+    assert C.sizeof(abi.TrianglesDesc) == 72
+    assert C.sizeof(abi.SpheresDesc) == 72
+    assert C.sizeof(abi.Renderer) == 48
+    assert C.sizeof(abi.RenderStats) == 80
+    assert abi.HIT_DTYPE.itemsize == 32 and abi.NODE_DTYPE.itemsize == 64
+
+
+def test_missing_library_fails_loudly(tmp_path, monkeypatch):
+    import barney_amd
+    monkeypatch.setattr(barney_amd, "LIB_PATH", str(tmp_path / "absent.so"))
+    monkeypatch.setattr(barney_amd, "_lib", None)
+    with pytest.raises(RuntimeError):
+        barney_amd.lib()
