@@ -57,6 +57,12 @@ _PROTOS = [
     ("bnx_group_destroy", [vp], C.c_int),
     ("bnx_bvh_build_boxes", [vp, vp, C.c_int, C.c_int, vp, C.POINTER(C.c_int32), C.POINTER(abi.BvhInfo)], C.c_int),
     ("bnx_trace", [vp, vp, C.c_int, vp, C.c_uint32], C.c_int),
+    ("bnx_volume_create_structured", [vp, C.POINTER(abi.StructuredDesc), C.POINTER(abi.TransferFunction), C.c_int32,
+                                      C.POINTER(vp)], C.c_int),
+    ("bnx_volume_set_transfer_function", [vp, C.POINTER(abi.TransferFunction)], C.c_int),
+    ("bnx_volume_majorants", [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)], C.c_int),
+    ("bnx_volume_destroy", [vp], C.c_int),
+    ("bnx_world_add_volume", [vp, vp, C.c_int32], C.c_int),
     ("bnx_world_create", [vp, vp, C.POINTER(abi.DirLight), C.c_int, C.POINTER(vp)], C.c_int),
     ("bnx_world_destroy", [vp], C.c_int),
     ("bnx_camera_perspective", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
@@ -270,6 +276,29 @@ def camera_perspective(position, direction, up, fovy_deg) -> abi.Camera:
     return cam
 
 
+class Volume:
+    """A structured-data volume with its transfer function (MCVolumeAccel)."""
+
+    def __init__(self, dev: Device, vs, user_id: int = 0):
+        self.dev = dev
+        d, t, keep = abi.structured_desc(vs)
+        h = vp()
+        check(dev.L.bnx_volume_create_structured(dev.h, C.byref(d), C.byref(t), user_id, C.byref(h)))
+        self.h = h
+
+    def set_transfer_function(self, tf, domain=(0.0, 1.0), base_density=1.0):
+        tf = np.ascontiguousarray(tf, dtype=np.float32)
+        t = abi.TransferFunction(abi.fptr(tf), len(tf), (C.c_float * 2)(*domain), base_density)
+        check(self.dev.L.bnx_volume_set_transfer_function(self.h, C.byref(t)))
+
+    def majorants(self):
+        dims = (C.c_int32 * 3)()
+        check(self.dev.L.bnx_volume_majorants(self.h, dims, None))
+        m = np.zeros(dims[0] * dims[1] * dims[2], np.float32)
+        check(self.dev.L.bnx_volume_majorants(self.h, dims, abi.fptr(m)))
+        return np.array(list(dims)), m
+
+
 class World:
     def __init__(self, dev: Device, instances: Group, dir_lights=()):
         self.dev, self.instances = dev, instances
@@ -277,6 +306,11 @@ class World:
         h = vp()
         check(dev.L.bnx_world_create(dev.h, instances.h, arr, len(dir_lights), C.byref(h)))
         self.h = h
+        self.volumes = []
+
+    def add_volume(self, vol: Volume, user_inst_id: int = 0):
+        check(self.dev.L.bnx_world_add_volume(self.h, vol.h, user_inst_id))
+        self.volumes.append(vol)
 
 
 class Frame:
@@ -337,3 +371,15 @@ class SurfaceWorld:
         self.groups = groups
         self.insts = Group.instances(dev, groups, [0] * len(groups)).build()
         self.world = World(dev, self.insts, sc.dir_lights)
+
+
+class VolumeWorld:
+    """GPU world for a barney_amd.scenes.StructuredVolumeScene: no surfaces,
+    one volume as user instance 0."""
+
+    def __init__(self, dev: Device, vs):
+        self.dev = dev
+        self.insts = Group.instances(dev, [], []).build()
+        self.volume = Volume(dev, vs, user_id=vs.user_id)
+        self.world = World(dev, self.insts, vs.dir_lights)
+        self.world.add_volume(self.volume, 0)

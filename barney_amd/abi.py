@@ -50,6 +50,26 @@ class DirLight(C.Structure):
     _fields_ = [("direction", f3), ("radiance", C.c_float), ("color", f3)]
 
 
+class StructuredDesc(C.Structure):
+    _fields_ = [("scalars", fp), ("dims", C.c_int32 * 3), ("cells_per_mc", C.c_int32), ("grid_origin", f3),
+                ("grid_spacing", f3)]
+
+
+class TransferFunction(C.Structure):
+    _fields_ = [("values", fp), ("num_values", C.c_int32), ("domain", C.c_float * 2), ("base_density", C.c_float)]
+
+
+def structured_desc(vs):
+    """(StructuredDesc, TransferFunction, keep-alive arrays) for a
+    scenes.StructuredVolumeScene."""
+    sc = np.ascontiguousarray(vs.scalars, dtype=np.float32)
+    tf = np.ascontiguousarray(vs.tf, dtype=np.float32)
+    d = StructuredDesc(fptr(sc), (C.c_int32 * 3)(*vs.dims), vs.cells_per_mc, f3(*vs.grid_origin),
+                       f3(*vs.grid_spacing))
+    t = TransferFunction(fptr(tf), len(tf), (C.c_float * 2)(*vs.tf_domain), vs.base_density)
+    return d, t, (sc, tf)
+
+
 class RenderStats(C.Structure):
     _fields_ = [("rays_traced", C.c_int64), ("shadow_rays", C.c_int64), ("generations", C.c_int64),
                 ("volume_samples", C.c_int64), ("ms_total", C.c_double), ("node_visits", C.c_int64),

@@ -2,6 +2,7 @@
 // geometries, groups (BLAS/TLAS builds), worlds, frames and the frame loop.
 #include "bnx_internal.h"
 #include "bnx_render.h"
+#include "bnx_volume.h"
 
 #include <chrono>
 #include <cmath>
@@ -283,6 +284,27 @@ Queues &queues_for(Device *dev)
 
 } // namespace
 
+int world_volumes(World *w, VolumeList &vl)
+{
+  if (w->volumes_dirty) {
+    std::vector<VolumeDev> vd;
+    const int nsurf = w->instances ? (int)w->instances->groups.size() : 0;
+    for (size_t k = 0; k < w->volumes.size(); k++) {
+      VolumeDev d = w->volumes[k]->dd;
+      d.rtc_inst = nsurf + (int)k;
+      d.user_inst = w->volume_inst_ids[k];
+      vd.push_back(d);
+    }
+    int rc = upload(w->dev, w->d_volumes, vd.data(), vd.size());
+    if (rc != BNX_OK) return rc;
+    BNX_HIP(hipStreamSynchronize(w->dev->stream));
+    w->volumes_dirty = false;
+  }
+  vl.vols = w->d_volumes.as<VolumeDev>();
+  vl.count = (int)w->volumes.size();
+  return BNX_OK;
+}
+
 int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp, bnx_render_stats *stats)
 {
   auto t0 = std::chrono::steady_clock::now();
@@ -313,6 +335,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
   P.H = H;
   P.stats = st;
   FrameDev fd = frame_dev(f);
+  VolumeList vl{};
+  if ((rc = world_volumes(w, vl)) != BNX_OK) return rc;
+  P.vols = vl.vols;
+  P.num_vols = vl.count;
   const int max_gen = rr.max_generations > 0 ? rr.max_generations : 1 << 30;
   for (int s0 = 0; s0 < spp; s0 += S) {
     const int nS = std::min(S, spp - s0);
@@ -327,6 +353,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
       if ((rc = launch_trace_queue(g, tq, counts + cur, count, st + 1)) != BNX_OK) return rc;
       BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
+      if ((rc = launch_volume_march(dev, vl, in, counts + cur, count, st + 4)) != BNX_OK) return rc;
       rays += count;
       gens++;
       BNX_HIP(hipMemsetAsync(counts + (1 - cur), 0, sizeof(int), dev->stream));
@@ -343,7 +370,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     }
     f->accum_id += nS;
   }
-  unsigned long long host_st[4] = {0, 0, 0, 0};
+  unsigned long long host_st[5] = {0, 0, 0, 0, 0};
   BNX_HIP(hipMemcpyAsync(host_st, st, sizeof(host_st), hipMemcpyDeviceToHost, dev->stream));
   BNX_HIP(hipStreamSynchronize(dev->stream));
   BNX_HIP(hipGetLastError());
@@ -351,7 +378,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     stats->rays_traced = rays;
     stats->shadow_rays = (int64_t)host_st[0];
     stats->generations = gens;
-    stats->volume_samples = 0;
+    stats->volume_samples = (int64_t)host_st[4];
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
     stats->node_visits = (int64_t)host_st[1];
     stats->tri_tests = (int64_t)host_st[2];
@@ -372,6 +399,7 @@ struct bnx_geom_s : Geom {};
 struct bnx_group_s : Group {};
 struct bnx_world_s : World {};
 struct bnx_frame_s : Frame {};
+struct bnx_volume_s : Volume {};
 
 #define GUARD_BEGIN try {
 #define GUARD_END                                                                                                    \
@@ -699,7 +727,106 @@ int bnx_trace(bnx_group g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32
   return launch_trace_api(g, d_rays, n, d_hits, flags);
 }
 
+// ----------------------------------------------------------------- volumes
+static int set_tf(Volume *v, const bnx_transfer_function *tf)
+{
+  BNX_CHECK(tf && tf->values && tf->num_values >= 2, BNX_ERR_INVALID, "transfer function needs >= 2 values");
+  BNX_CHECK(tf->domain[1] > tf->domain[0], BNX_ERR_INVALID, "transfer function domain is empty");
+  int rc = upload(v->dev, v->tf, (const float4 *)tf->values, (size_t)tf->num_values);
+  if (rc != BNX_OK) return rc;
+  v->dd.tf = v->tf.as<float4>();
+  v->dd.ntf = tf->num_values;
+  v->dd.dom0 = tf->domain[0];
+  v->dd.dom1 = tf->domain[1];
+  v->dd.base_density = tf->base_density;
+  return BNX_OK;
+}
+
+int bnx_volume_create_structured(bnx_device d, const bnx_structured_desc *f, const bnx_transfer_function *tf,
+                                 int32_t user_id, bnx_volume *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && f && tf && out && f->scalars, BNX_ERR_INVALID, "bnx_volume_create_structured: null argument");
+  BNX_CHECK(f->dims[0] >= 2 && f->dims[1] >= 2 && f->dims[2] >= 2, BNX_ERR_INVALID,
+            "bnx_volume_create_structured: dims must be >= 2");
+  BNX_CHECK(f->grid_spacing[0] > 0.f && f->grid_spacing[1] > 0.f && f->grid_spacing[2] > 0.f, BNX_ERR_INVALID,
+            "bnx_volume_create_structured: spacing must be positive");
+  bnx_volume v = new bnx_volume_s;
+  v->dev = d;
+  VolumeDev &dd = v->dd;
+  const size_t n = (size_t)f->dims[0] * f->dims[1] * f->dims[2];
+  int rc = upload(d, v->scalars, f->scalars, n);
+  if (rc != BNX_OK) {
+    delete v;
+    return rc;
+  }
+  dd.scalars = v->scalars.as<float>();
+  const int c = f->cells_per_mc > 0 ? f->cells_per_mc : 8;
+  dd.cells_per_mc = c;
+  for (int k = 0; k < 3; k++) {
+    dd.dims[k] = f->dims[k];
+    dd.ncells[k] = f->dims[k] - 1;
+    dd.origin[k] = f->grid_origin[k];
+    dd.rcp_spacing[k] = 1.f / f->grid_spacing[k];
+    // StructuredData::commit (StructuredData.cu:177-181)
+    dd.lo[k] = f->grid_origin[k];
+    dd.hi[k] = f->grid_origin[k] + f->grid_spacing[k] * (float)dd.ncells[k];
+    // buildMCs (StructuredData.cu:71-78)
+    dd.mc[k] = (dd.ncells[k] + c - 1) / c;
+    dd.mc_origin[k] = dd.lo[k];
+    dd.mc_rcp_spacing[k] = 1.f / ((float)c * f->grid_spacing[k]);
+  }
+  dd.user_id = user_id;
+  if ((rc = set_tf(v, tf)) != BNX_OK || (rc = volume_build(v)) != BNX_OK) {
+    delete v;
+    return rc;
+  }
+  *out = v;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_volume_set_transfer_function(bnx_volume v, const bnx_transfer_function *tf)
+{
+  GUARD_BEGIN
+  BNX_CHECK(v, BNX_ERR_INVALID, "bnx_volume_set_transfer_function: null volume");
+  int rc = set_tf(v, tf);
+  if (rc != BNX_OK) return rc;
+  return volume_rebuild_majorants(v);
+  GUARD_END
+}
+
+int bnx_volume_majorants(bnx_volume v, int32_t dims[3], float *maj)
+{
+  BNX_CHECK(v && dims, BNX_ERR_INVALID, "bnx_volume_majorants: null argument");
+  for (int k = 0; k < 3; k++) dims[k] = v->dd.mc[k];
+  if (maj)
+    BNX_HIP(hipMemcpy(maj, v->majorants.ptr, sizeof(float) * (size_t)dims[0] * dims[1] * dims[2],
+                      hipMemcpyDeviceToHost));
+  return BNX_OK;
+}
+
+int bnx_volume_destroy(bnx_volume v)
+{
+  BNX_CHECK(v, BNX_ERR_INVALID, "bnx_volume_destroy: null volume");
+  (void)hipStreamSynchronize(v->dev->stream);
+  delete v;
+  return BNX_OK;
+}
+
 // ------------------------------------------------------------------- world
+int bnx_world_add_volume(bnx_world w, bnx_volume v, int32_t user_inst_id)
+{
+  GUARD_BEGIN
+  BNX_CHECK(w && v, BNX_ERR_INVALID, "bnx_world_add_volume: null argument");
+  BNX_CHECK(w->dev == v->dev, BNX_ERR_INVALID, "bnx_world_add_volume: volume lives on another device");
+  w->volumes.push_back(v);
+  w->volume_inst_ids.push_back(user_inst_id);
+  w->volumes_dirty = true;
+  return BNX_OK;
+  GUARD_END
+}
+
 int bnx_world_create(bnx_device d, bnx_group instances, const bnx_dir_light *lights, int n, bnx_world *out)
 {
   GUARD_BEGIN

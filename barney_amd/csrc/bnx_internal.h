@@ -114,11 +114,16 @@ struct Group {
   int max_blas_depth = 0;
 };
 
+struct Volume;
 struct World {
   Device *dev;
   Group *instances;
   std::vector<bnx_dir_light> dir_lights;
   DevMem d_dir_lights;
+  std::vector<Volume *> volumes;
+  std::vector<int> volume_inst_ids;
+  DevMem d_volumes;           // VolumeDev[volumes.size()]
+  bool volumes_dirty = true;
 };
 
 struct Frame {

@@ -13,10 +13,14 @@ struct RayQueue {
   uint4 *ps;            // throughput (3 x fp16), diffuse bounces, pixel, misWeight
   int4 *hit;            // prim, geom, inst, bits(u)
   float *hit_v;
+  float4 *vcol;         // rgb of a volume event
 };
 
+struct VolumeDev;
 struct ShadeParams {
   bnx_renderer rr;
+  const VolumeDev *vols;       // world volumes (slot = hit geom - VOLUME_GEOM_BASE)
+  int num_vols;
   const bnx_dir_light *dir_lights;
   int num_dir_lights;
   int W, H;
@@ -46,7 +50,7 @@ inline FrameDev frame_dev(Frame *f)
 
 // per-device ray queues (two, ping-pong) and counters
 struct Queues {
-  DevMem org[2], dir[2], rng[2], ps[2], hit[2], hit_v[2];
+  DevMem org[2], dir[2], rng[2], ps[2], hit[2], hit_v[2], vcol[2];
   DevMem counts;      // [0], [1]: queue counts; [4..]: statistics
   size_t cap = 0;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr;
@@ -60,6 +64,7 @@ struct Queues {
       BNX_HIP(ps[k].ensure(sizeof(uint4) * n));
       BNX_HIP(hit[k].ensure(sizeof(int4) * n));
       BNX_HIP(hit_v[k].ensure(sizeof(float) * n));
+      BNX_HIP(vcol[k].ensure(sizeof(float4) * n));
     }
     BNX_HIP(counts.ensure(64));
     cap = n;
@@ -74,6 +79,7 @@ struct Queues {
     r.ps = ps[k].as<uint4>();
     r.hit = hit[k].as<int4>();
     r.hit_v = hit_v[k].as<float>();
+    r.vcol = vcol[k].as<float4>();
     return r;
   }
 };

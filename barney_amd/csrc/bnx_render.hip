@@ -16,6 +16,7 @@
 // arrive.
 #include "bnx_internal.h"
 #include "bnx_render.h"
+#include "bnx_volume.h"
 
 #include <chrono>
 
@@ -75,8 +76,9 @@ __global__ void k_raygen(bnx_camera cam, bnx_renderer rr, int W, int H, int accu
 
 // ----------------------------------------------------------------- shade
 struct SurfHit {
-  v3 P, N;          // N: fp16-rounded world normal
-  v3 albedo;
+  v3 P, N;          // N: fp16-rounded world normal (zero for a volume event)
+  v3 albedo;        // Lambertian albedo or SimplePhase albedo
+  bool phase;       // volume event: SimplePhase (packedBSDF/Phase.h:119-172)
 };
 
 BNX_D void onb(v3 n, v3 &b1, v3 &b2)
@@ -253,6 +255,19 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
   sh.albedo = (0.85f * v3{base0, base1, base2}) * ONE_OVER_PI;
   sh.P = P;
   sh.N = quant_half(Nw);
+  sh.phase = false;
+  return sh;
+}
+
+// Ray::setVolumeHit (render/Ray.h:137-144): SimplePhase(0.7 * rgb), N = 0
+BNX_D SurfHit eval_volume_hit(float t, float4 rgb, v3 rorg, v3 rdir)
+{
+  SurfHit sh;
+  v3 d = fix_dir(rdir);
+  sh.P = rorg + t * d;
+  sh.N = splat(0.f);
+  sh.albedo = .7f * v3{rgb.x, rgb.y, rgb.z};
+  sh.phase = true;
   return sh;
 }
 
@@ -312,7 +327,9 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQue
     const bool shadow_hit = hit.x >= 0;
 
     SurfHit sh{};
-    if (had_hit) sh = eval_hit(S, hit, hv, rorg, rdir);
+    const bool vol_hit = had_hit && hit.y >= VOLUME_GEOM_BASE;
+    if (vol_hit) sh = eval_volume_hit(tmax, in.vcol[i], rorg, rdir);
+    else if (had_hit) sh = eval_hit(S, hit, hv, rorg, rdir);
 
     const float4 mc = miss_color(P, pixel);
     float alpha = (gen == 0) ? (had_hit ? 1.f : mc.w) : 0.f;
@@ -348,9 +365,16 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQue
         LightSample ls;
         bool needs_mis = false, is_dir = false;
         if (sample_lights(ls, P, Ngff, random, needs_mis, is_dir)) {
-          float c = fmx(dot(ls.direction, Ng), 0.f);
-          float fpdf = c * ONE_OVER_PI;
-          v3 fv = (sh.albedo * ONE_OVER_PI) * c;
+          float fpdf;
+          v3 fv;
+          if (sh.phase) {
+            fpdf = ONE_OVER_FOUR_PI;
+            fv = ONE_OVER_FOUR_PI * sh.albedo;
+          } else {
+            float c = fmx(dot(ls.direction, Ng), 0.f);
+            fpdf = c * ONE_OVER_PI;
+            fv = (sh.albedo * ONE_OVER_PI) * c;
+          }
           if (!(fpdf > 0.f) || reduce_max(fv) < 1e-4f) {
             sh_tmax = -1.f;
           } else {
@@ -368,14 +392,15 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQue
             float smis = 1.f;
             if (!is_dir && needs_mis) {
               float pl = ONE_OVER_FOUR_PI;
-              float psd = lambert_pdf(Ng, ls.direction);
+              float psd = sh.phase ? ONE_OVER_FOUR_PI : lambert_pdf(Ng, ls.direction);
               smis = pl / ((pl + psd) + 1e-10f);
               if (smis < 1e-5f) sh_tmax = -1.f;
             }
             pack_ps(o_sh.ps, quant_half(tp_sr), 0, pixel, smis);
           }
         }
-        if (gen >= MAX_PATH_DEPTH) {
+        if (gen >= MAX_PATH_DEPTH || sh.phase) {
+          // SimplePhase scatters VOLUME, which SCI_VIS_MODE ends (shadeRays.cu:779-789)
           ray_tmax = -1.f;
         } else {
           ray_tmax = BNX_INF;
@@ -437,7 +462,11 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQue
       if (accumID == 0) {
         fb.prim[pixel] = hit.x >= 0 ? hit.x : -1;
         int inst = -1, obj = -1;
-        if (hit.x >= 0) {
+        if (vol_hit) {
+          const VolumeDev &V = P.vols[hit.y - VOLUME_GEOM_BASE];
+          inst = V.user_inst;
+          obj = V.user_id;
+        } else if (hit.x >= 0) {
           const InstDev &I = S.insts[hit.z];
           const GeomDev &G = S.geoms[I.geom_base + hit.y];
           inst = (G.type == 1 && (P.rr.flags & BNX_RENDER_COMPAT_SPHERE_INSTID13)) ? 13 : I.user_id;

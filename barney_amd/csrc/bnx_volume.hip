@@ -1,0 +1,157 @@
+// bnx_volume.hip -- structured volumes on gfx950: macrocell value ranges,
+// majorants, and the DDA + Woodcock (delta tracking) march.
+//
+// Replaces MCVolumeAccel<StructuredDataSampler> (volume/MCAccelerator.h:
+// 155-195, 550-689), StructuredData_computeMCs (volume/StructuredData.cu:
+// 30-62), mapMCs (volume/MCGrid.cu:96-113), dda3 (volume/DDA.h:60-201) and
+// Woodcock::sampleRange (volume/Volume.h:154-203).
+//
+// Design: volumes are marched in their own wavefront kernel after the
+// surface trace (one thread per queued ray, persistent waves), clipped to
+// the surface hit.  The march is long, data-dependent and texture-bound, so
+// keeping it out of the BVH traversal kernel keeps that kernel's registers
+// and divergence low.  Trilinear filtering is done in software on the raw
+// float grid (hardware texture filtering uses 8-bit fixed-point weights);
+// a 256^3 field is 64 MiB and stays resident in the 256 MiB Infinity Cache.
+#include "bnx_internal.h"
+#include "bnx_render.h"
+#include "bnx_volume.h"
+
+namespace bnx {
+namespace {
+
+inline int nblk(long n, int b) { return (int)((n + b - 1) / b); }
+
+// StructuredData_computeMCs: min/max over the (cpm+1)^3 vertices of a macrocell
+__global__ void k_mc_ranges(VolumeDev v, float2 *ranges)
+{
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = v.mc[0] * v.mc[1] * v.mc[2];
+  if (tid >= n) return;
+  int mx = tid % v.mc[0], my = (tid / v.mc[0]) % v.mc[1], mz = tid / (v.mc[0] * v.mc[1]);
+  const int c = v.cells_per_mc;
+  float lo = INFINITY, hi = -INFINITY;
+  for (int iz = 0; iz <= c; iz++) {
+    int sz = mz * c + iz;
+    if (sz >= v.dims[2]) continue;
+    for (int iy = 0; iy <= c; iy++) {
+      int sy = my * c + iy;
+      if (sy >= v.dims[1]) continue;
+      const float *row = v.scalars + (size_t)v.dims[0] * ((size_t)sy + (size_t)v.dims[1] * sz);
+      for (int ix = 0; ix <= c; ix++) {
+        int sx = mx * c + ix;
+        if (sx >= v.dims[0]) continue;
+        float f = row[sx];
+        lo = fmn(lo, f);
+        hi = fmx(hi, f);
+      }
+    }
+  }
+  ranges[tid] = make_float2(lo, hi);
+}
+
+// mapMCs + TransferFunction::DD::majorant
+__global__ void k_majorants(VolumeDev v, const float2 *ranges, float *maj)
+{
+  int tid = blockIdx.x * blockDim.x + threadIdx.x;
+  int n = v.mc[0] * v.mc[1] * v.mc[2];
+  if (tid >= n) return;
+  float2 r = ranges[tid];
+  maj[tid] = tf_majorant(v, r.x, r.y);
+}
+
+BNX_D uint32_t wave_sum(uint32_t x)
+{
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// march every queued ray through the world's volumes
+__global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q, const int *d_count, int *work,
+                                                      unsigned long long *samples)
+{
+  const int n = *d_count;
+  uint32_t ns = 0;
+  while (true) {
+    int base = 0;
+    if (lane_id() == 0) base = atomicAdd(work, 64);
+    base = __shfl(base, 0);
+    if (base >= n) break;
+    int i = base + lane_id();
+    if (i >= n) continue;
+    float4 o = q.org[i];
+    float4 d = q.dir[i];
+    int4 hit = q.hit[i];
+    const bool shadow = (__float_as_uint(d.w) & 1u) != 0;
+    if (shadow && hit.x >= 0) continue;            // already occluded
+    const v3 org = xyz(o), dir = fix_dir(xyz(d));
+    float tcur = o.w;
+    uint64_t rng = q.rng[i];
+    const uint64_t rng0 = rng;
+    int vk = -1;
+    float vt = 0.f, rgb[3] = {0.f, 0.f, 0.f};
+    for (int k = 0; k < vl.count; k++) {
+      float t, c[3];
+      if (volume_march(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, ns)) {
+        vk = k;
+        vt = t;
+        tcur = t;
+        rgb[0] = c[0]; rgb[1] = c[1]; rgb[2] = c[2];
+        if (shadow) break;
+      }
+    }
+    if (rng != rng0) q.rng[i] = rng;
+    if (vk >= 0) {
+      q.org[i].w = vt;
+      q.hit[i] = make_int4(0, VOLUME_GEOM_BASE + vk, vl.vols[vk].rtc_inst, 0);
+      q.vcol[i] = make_float4(rgb[0], rgb[1], rgb[2], 0.f);
+    }
+  }
+  uint32_t s = wave_sum(ns);
+  if (lane_id() == 0 && samples) atomicAdd(samples, (unsigned long long)s);
+}
+
+__global__ void k_zero1(int *p) { *p = 0; }
+
+} // namespace
+
+int volume_build(Volume *v)
+{
+  Device *dev = v->dev;
+  VolumeDev &d = v->dd;
+  const size_t nmc = (size_t)d.mc[0] * d.mc[1] * d.mc[2];
+  BNX_HIP(v->ranges.ensure(sizeof(float2) * nmc));
+  BNX_HIP(v->majorants.ensure(sizeof(float) * nmc));
+  d.majorants = v->majorants.as<float>();
+  k_mc_ranges<<<nblk((long)nmc, 64), 64, 0, dev->stream>>>(d, v->ranges.as<float2>());
+  k_majorants<<<nblk((long)nmc, 256), 256, 0, dev->stream>>>(d, v->ranges.as<float2>(), v->majorants.as<float>());
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int volume_rebuild_majorants(Volume *v)
+{
+  Device *dev = v->dev;
+  const size_t nmc = (size_t)v->dd.mc[0] * v->dd.mc[1] * v->dd.mc[2];
+  k_majorants<<<nblk((long)nmc, 256), 256, 0, dev->stream>>>(v->dd, v->ranges.as<float2>(),
+                                                             v->majorants.as<float>());
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
+                        unsigned long long *samples)
+{
+  if (vl.count == 0 || max_n == 0) return BNX_OK;
+  BNX_HIP(dev->counters.ensure(256));
+  int *work = dev->counters.as<int>() + 2;
+  k_zero1<<<1, 1, 0, dev->stream>>>(work);
+  int grid = std::min(nblk(max_n, 256), dev->num_cus * 8);
+  k_volume_march<<<grid, 256, 0, dev->stream>>>(vl, q, d_count, work, samples);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+} // namespace bnx

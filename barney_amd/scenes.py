@@ -173,3 +173,80 @@ def random_rays(seed: int, n: int, center=(0.0, 0.0, 0.0), radius: float = 4.0, 
     rays[:, 4:7] = d
     rays[:, 7] = 1e30
     return rays
+
+
+# --------------------------------------------------------------------------
+# config 2: structured volume
+# --------------------------------------------------------------------------
+@dataclass
+class StructuredVolumeScene:
+    scalars: np.ndarray            # (nz, ny, nx) float32, x fastest in memory
+    grid_origin: tuple
+    grid_spacing: tuple
+    tf: np.ndarray                 # (N, 4) float32 RGBA
+    tf_domain: tuple
+    base_density: float
+    cells_per_mc: int
+    dir_lights: list = field(default_factory=list)
+    ambient: float = 0.25
+    bg_color: tuple = (0.0, 0.0, 0.0, 1.0)
+    camera: dict = field(default_factory=dict)
+    width: int = 1024
+    height: int = 1024
+    spp: int = 16
+    user_id: int = 0
+
+    @property
+    def dims(self):
+        nz, ny, nx = self.scalars.shape
+        return (nx, ny, nz)
+
+
+def gaussian_blobs(n: int, num_blobs: int = 32, seed: int = SEEDS["c2"]) -> np.ndarray:
+    """n^3 grid on [0,1]^3 (vertex i at i/(n-1)): sum of isotropic Gaussians
+    (centres U[0,1]^3, sigma U(0.02,0.1), amplitude U(0.5,1)), normalised to
+    [0,1].  Returned as (z, y, x) float32."""
+    u = uniform_table(seed, 4, num_blobs, 5)
+    c = u[:, :3]
+    sigma = 0.02 + 0.08 * u[:, 3]
+    amp = 0.5 + 0.5 * u[:, 4]
+    x = np.linspace(0.0, 1.0, n)
+    f = np.zeros((n, n, n), np.float64)
+    for b in range(num_blobs):
+        gx = np.exp(-((x - c[b, 0]) ** 2) / (2 * sigma[b] ** 2))
+        gy = np.exp(-((x - c[b, 1]) ** 2) / (2 * sigma[b] ** 2))
+        gz = np.exp(-((x - c[b, 2]) ** 2) / (2 * sigma[b] ** 2))
+        f += amp[b] * gz[:, None, None] * gy[None, :, None] * gx[None, None, :]
+    f = (f - f.min()) / (f.max() - f.min())
+    return f.astype(np.float32)
+
+
+def ramp_tf(n: int = 256, max_alpha: float = 0.8) -> np.ndarray:
+    """Piecewise-linear RGBA map: rgb from blue through green-white to red,
+    opacity ramp 0 -> max_alpha."""
+    t = np.linspace(0.0, 1.0, n)
+    r = np.clip(1.5 * t - 0.25, 0, 1)
+    g = np.clip(1.0 - np.abs(2.0 * t - 1.0), 0, 1) * 0.8 + 0.1
+    b = np.clip(1.25 - 1.5 * t, 0, 1)
+    a = max_alpha * t
+    return np.stack([r, g, b, a], 1).astype(np.float32)
+
+
+def config2(n: int = 256, width: int = 1024, height: int = 1024, spp: int = 16,
+            mc_grid: int = 16) -> StructuredVolumeScene:
+    """Structured volume: 256^3 Gaussian-blob field on [0,1]^3, 256-entry TF
+    with opacity ramp 0 -> 0.8, density scale 50 (unitDistance 0.02),
+    macrocell grid mc_grid^3 (cells_per_mc = ceil((n-1)/mc_grid)), one
+    directional light (emission -(1,1,1)/sqrt(3), radiance 2.5) plus ambient
+    0.25, camera outside the unit cube looking at its centre, fovy 35."""
+    f = gaussian_blobs(n)
+    cpm = -(-(n - 1) // mc_grid)
+    sp = 1.0 / (n - 1)
+    s = StructuredVolumeScene(scalars=f, grid_origin=(0.0, 0.0, 0.0), grid_spacing=(sp, sp, sp),
+                              tf=ramp_tf(256, 0.8), tf_domain=(0.0, 1.0), base_density=50.0, cells_per_mc=cpm,
+                              width=width, height=height, spp=spp)
+    d = -np.ones(3, dtype=np.float64) / math.sqrt(3.0)
+    s.dir_lights.append((d.astype(np.float32), 2.5, np.ones(3, dtype=np.float32)))
+    s.camera = dict(position=(1.9, 1.4, 2.6), direction=(0.5 - 1.9, 0.5 - 1.4, 0.5 - 2.6), up=(0.0, 1.0, 0.0),
+                    fovy=35.0)
+    return s

@@ -45,6 +45,7 @@ typedef struct bnx_geom_s   *bnx_geom;
 typedef struct bnx_group_s  *bnx_group;
 typedef struct bnx_world_s  *bnx_world;
 typedef struct bnx_frame_s  *bnx_frame;
+typedef struct bnx_volume_s *bnx_volume;
 
 /* ------------------------------------------------------------- records */
 /* One ray of a trace batch: traceRays.dev.cu:38-50 passes (org, dir, 0, tMax). */
@@ -140,6 +141,28 @@ typedef struct bnx_dir_light {
   float color[3];
 } bnx_dir_light;
 
+/* StructuredData (volume/StructuredData.h:30-74, set3i "dims", set3f
+ * "gridOrigin"/"gridSpacing"): scalars at dims[0]*dims[1]*dims[2] grid
+ * vertices, x fastest; cells_per_mc is the macrocell edge in cells
+ * (StructuredData.cu:26 hard-codes 8). */
+typedef struct bnx_structured_desc {
+  const float *scalars;
+  int32_t      dims[3];
+  int32_t      cells_per_mc;
+  float        grid_origin[3];
+  float        grid_spacing[3];
+} bnx_structured_desc;
+
+/* TransferFunction (volume/TransferFunction.h:20-117): num_values RGBA
+ * entries over [domain[0], domain[1]]; base_density = 1/unitDistance
+ * (anari/Volume.cpp:101-106). */
+typedef struct bnx_transfer_function {
+  const float *values;              /* float4[num_values]                          */
+  int32_t      num_values;
+  float        domain[2];
+  float        base_density;
+} bnx_transfer_function;
+
 /* Frame-buffer channels (fb/FrameBuffer.cu:315-394 bnFrameBufferRead). */
 #define BNX_FB_COLOR   0   /* float4 per pixel: accum * 1/accumID           */
 #define BNX_FB_DEPTH   1   /* float per pixel: primary-ray tMax             */
@@ -233,10 +256,26 @@ int bnx_bvh_build_boxes(bnx_device dev, const float *d_boxes, int n, int max_lea
  * (inst, geom, prim) wins a tie in t. Asynchronous on the device stream. */
 int bnx_trace(bnx_group instances, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
 
+/* ------------------------------------------------------------- volumes */
+/* Volume over a StructuredData field with its MCVolumeAccel (volume/
+ * MCAccelerator.h:155-195): builds the macrocell value ranges
+ * (StructuredData.cu:30-96) and majorants (MCGrid.cu:96-141) on the GPU. */
+int bnx_volume_create_structured(bnx_device dev, const bnx_structured_desc *field,
+                                 const bnx_transfer_function *tf, int32_t user_id, bnx_volume *out);
+/* Volume::setXF + VolumeAccel::rebuildMajorantsOnly */
+int bnx_volume_set_transfer_function(bnx_volume vol, const bnx_transfer_function *tf);
+/* macrocell grid dims and majorants (for tests) */
+int bnx_volume_majorants(bnx_volume vol, int32_t dims[3], float *majorants);
+int bnx_volume_destroy(bnx_volume vol);
+
 /* --------------------------------------------------------------- world */
 /* World::DD (render/World.h:32-47): instances + lights. */
 int bnx_world_create(bnx_device dev, bnx_group instances, const bnx_dir_light *dir_lights,
                      int num_dir_lights, bnx_world *out);
+/* Adds a volume as a user instance (ModelSlot.cpp:128-155 puts volume
+ * groups after the geometry groups of an instance).  Volumes are marched
+ * after the surface trace, in the order they were added. */
+int bnx_world_add_volume(bnx_world world, bnx_volume vol, int32_t user_inst_id);
 int bnx_world_destroy(bnx_world world);
 
 /* -------------------------------------------------------------- camera */

@@ -41,6 +41,17 @@ def lib():
         L.orc_scene_add_group.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
         L.orc_scene_add_instance.argtypes = [vp, C.c_int, C.c_int, abi.fp, abi.fp]
         L.orc_scene_set_dir_lights.argtypes = [vp, C.POINTER(abi.DirLight), C.c_int]
+        L.orc_scene_add_volume.argtypes = [vp, C.POINTER(abi.StructuredDesc), C.POINTER(abi.TransferFunction),
+                                           C.c_int, C.c_int]
+        L.orc_volume_majorants.argtypes = [vp, C.c_int, abi.ip, abi.fp]
+        L.orc_volume_sample.argtypes = [vp, C.c_int, abi.fp]
+        L.orc_volume_sample.restype = C.c_float
+        L.orc_volume_tf_map.argtypes = [vp, C.c_int, C.c_float, abi.fp]
+        L.orc_fast_log.argtypes = [C.c_float]
+        L.orc_fast_log.restype = C.c_float
+        L.orc_volume_march.argtypes = [vp, C.c_int, abi.fp, abi.fp, C.c_float, C.c_uint32, C.c_uint32, abi.fp,
+                                       C.POINTER(C.c_uint64)]
+        L.orc_volume_march.restype = C.c_float
         L.orc_scene_build.argtypes = [vp]
         L.orc_scene_build.restype = C.c_double
         L.orc_trace.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_uint64), C.c_int]
@@ -111,6 +122,36 @@ class Scene:
         b = None if w2o is None else np.ascontiguousarray(w2o, dtype=np.float32)
         self._keep += [a, b]
         return self.L.orc_scene_add_instance(self.h, group, user_id, abi.fptr(a), abi.fptr(b))
+
+    def add_volume(self, vs, user_id=0, user_inst=0):
+        d, t, keep = abi.structured_desc(vs)
+        k = self.L.orc_scene_add_volume(self.h, C.byref(d), C.byref(t), user_id, user_inst)
+        return k
+
+    def volume_majorants(self, k=0):
+        dims = np.zeros(3, np.int32)
+        self.L.orc_volume_majorants(self.h, k, abi.iptr(dims), None)
+        maj = np.zeros(int(np.prod(dims)), np.float32)
+        self.L.orc_volume_majorants(self.h, k, abi.iptr(dims), abi.fptr(maj))
+        return dims, maj
+
+    def volume_sample(self, P, k=0):
+        p = np.ascontiguousarray(P, np.float32)
+        return self.L.orc_volume_sample(self.h, k, abi.fptr(p))
+
+    def tf_map(self, s, k=0):
+        out = np.zeros(4, np.float32)
+        self.L.orc_volume_tf_map(self.h, k, float(s), abi.fptr(out))
+        return out
+
+    def volume_march(self, org, dir, tmax, seed=(1, 2), k=0):
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dir, np.float32)
+        rgb = np.zeros(3, np.float32)
+        ns = C.c_uint64()
+        t = self.L.orc_volume_march(self.h, k, abi.fptr(o), abi.fptr(d), tmax, seed[0], seed[1], abi.fptr(rgb),
+                                    C.byref(ns))
+        return t, rgb, ns.value
 
     def set_dir_lights(self, lights):
         arr = abi.dir_lights(lights)
@@ -207,5 +248,15 @@ def surface_scene(sc) -> Scene:
     if tri:
         s.add_instance(s.add_group(0, tri), 0)
     s.set_dir_lights(sc.dir_lights)
+    s.build()
+    return s
+
+
+def volume_scene(vs) -> Scene:
+    """Oracle scene for a barney_amd.scenes.StructuredVolumeScene: one volume
+    as user instance 0, no surfaces."""
+    s = Scene()
+    s.add_volume(vs, user_id=vs.user_id, user_inst=0)
+    s.set_dir_lights(vs.dir_lights)
     s.build()
     return s

@@ -89,6 +89,54 @@ int orc_scene_add_instance(orc_scene *sc, int group, int user_id, const float *o
   return (int)sc->s.insts.size() - 1;
 }
 
+// structured volume as user instance 'user_inst' (after all surface instances)
+int orc_scene_add_volume(orc_scene *sc, const bnx_structured_desc *f, const bnx_transfer_function *tf, int user_id,
+                         int user_inst)
+{
+  Volume v;
+  for (int k = 0; k < 3; k++) v.dims[k] = f->dims[k];
+  v.cells_per_mc = f->cells_per_mc > 0 ? f->cells_per_mc : 8;
+  v.origin = load3(f->grid_origin);
+  v.spacing = load3(f->grid_spacing);
+  v.scalars.assign(f->scalars, f->scalars + (size_t)f->dims[0] * f->dims[1] * f->dims[2]);
+  v.tf.assign(tf->values, tf->values + 4 * (size_t)tf->num_values);
+  v.dom0 = tf->domain[0];
+  v.dom1 = tf->domain[1];
+  v.base_density = tf->base_density;
+  v.user_id = user_id;
+  v.user_inst = user_inst;
+  build_volume(v);
+  sc->s.volumes.push_back(std::move(v));
+  return (int)sc->s.volumes.size() - 1;
+}
+
+int orc_volume_majorants(orc_scene *sc, int k, int32_t *dims, float *maj)
+{
+  const Volume &v = sc->s.volumes[k];
+  for (int i = 0; i < 3; i++) dims[i] = v.mc[i];
+  if (maj) memcpy(maj, v.majorant.data(), v.majorant.size() * sizeof(float));
+  return 0;
+}
+
+float orc_volume_sample(orc_scene *sc, int k, const float *P) { return volume_sample(sc->s.volumes[k], load3(P)); }
+void orc_volume_tf_map(orc_scene *sc, int k, float s, float *out) { tf_map(sc->s.volumes[k], s, out); }
+float orc_fast_log(float f) { return fast_log(f); }
+
+// march one ray through volume k with a parent seed (a, b): event t or -1
+float orc_volume_march(orc_scene *sc, int k, const float *org, const float *dir, float tmax, uint32_t a, uint32_t b,
+                       float *rgb, uint64_t *samples)
+{
+  RNGSeed p;
+  p.seed(a, b);
+  float t = -1.f;
+  uint64_t ns = 0;
+  if (!volume_march(sc->s.volumes[k], (uint32_t)(sc->s.insts.size() + k), load3(org), fix_dir(load3(dir)), 0.f,
+                    tmax, p, t, rgb, &ns))
+    t = -1.f;
+  if (samples) *samples = ns;
+  return t;
+}
+
 void orc_scene_set_dir_lights(orc_scene *sc, const bnx_dir_light *l, int n)
 {
   sc->s.dir_lights.assign(l, l + n);

@@ -106,6 +106,15 @@ struct Random2 : RNGSeed {
     parent.next();
     next_h((uint64_t)(b ? b : 290374u));
   }
+  // Random2(RNGSeed &, uint64_t) (:59-61): a 64-bit hash
+  static Random2 from64(RNGSeed &parent, uint64_t b)
+  {
+    Random2 r(0u, 0u);
+    r.state = parent.state;
+    parent.next();
+    r.next_h(b ? b : 290374ull);
+    return r;
+  }
   float operator()() { return (float)(next() & 0x00FFFFFF) * (1.f / (float)0x01000000); } // :63-66
 };
 
@@ -202,12 +211,42 @@ struct Instance {
   Affine o2w, w2o;
 };
 
+// StructuredData volume with its macrocell majorants (volume/StructuredData.*,
+// volume/MCGrid.*, volume/TransferFunction.h)
+struct Volume {
+  int dims[3] = {0, 0, 0};
+  int cells_per_mc = 8;
+  vec3 origin, spacing{1, 1, 1};
+  std::vector<float> scalars;
+  std::vector<float> tf;            // rgba per entry
+  float dom0 = 0.f, dom1 = 1.f, base_density = 1.f;
+  int user_id = 0, user_inst = 0;
+  // derived by build_volume
+  int ncells[3];
+  Box bounds;
+  int mc[3];
+  vec3 mc_origin, mc_spacing;
+  std::vector<float> mc_lo, mc_hi, majorant;
+};
+void build_volume(Volume &v);
+void build_majorants(Volume &v);
+float volume_sample(const Volume &v, vec3 P);
+void tf_map(const Volume &v, float s, float out[4]);
+float tf_majorant(const Volume &v, float lo, float hi);
+// MCVolumeAccel::isProg for one volume; 'dir' is the fixed trace direction.
+// Returns true on a scattering event (t_out, rgb); advances 'parent' once when
+// the ray overlaps the volume's bounds.
+bool volume_march(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin, float tmax, RNGSeed &parent,
+                  float &t_out, float rgb[3], uint64_t *samples);
+float fast_log(float f);
+
 struct Scene {
   std::vector<Geom> geoms;
   std::vector<Group> groups;
   std::vector<Instance> insts;
   Bvh tlas;
   std::vector<bnx_dir_light> dir_lights;
+  std::vector<Volume> volumes;      // marched after the surface trace, in order
 };
 
 void build_scene(Scene &s);        // BLAS per group + TLAS

@@ -21,7 +21,8 @@ struct PathRay {
   RNGSeed rng;
   bool had_hit = false;
   vec3 P, N;                 // N holds the fp16-rounded normal (vec3h)
-  vec3 albedo;               // Lambertian::albedo (the only BSDF of these scenes)
+  vec3 albedo;               // Lambertian::albedo or SimplePhase::albedo
+  int bsdf = 0;              // 1 Lambertian, 2 SimplePhase (volume hit)
   bool is_shadow = false, is_specular = false, in_medium = false;
   float miss[4];
   vec3 tp;                   // PathState::throughput, fp16-rounded (vec3h)
@@ -228,7 +229,13 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
   if (sample_lights(ls, s, rr, Ngff, random, needs_mis, is_dir)) {
     vec3 fv;
     float fpdf;
-    lambert_eval(ray.albedo, dg, ls.direction, fv, fpdf);
+    if (ray.bsdf == 2) {
+      // SimplePhase::eval (packedBSDF/Phase.h:147-152)
+      fpdf = ONE_OVER_FOUR_PI;
+      fv = ONE_OVER_FOUR_PI * ray.albedo;
+    } else {
+      lambert_eval(ray.albedo, dg, ls.direction, fv, fpdf);
+    }
     if (!(fpdf > 0.f) || reduce_max(fv) < 1e-4f) {
       shadow.tmax = -1.f;
     } else {
@@ -249,7 +256,7 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
       shadow.mis = 1.f;
       if (!is_dir && needs_mis) {
         float pl = ONE_OVER_FOUR_PI;      // EnvMapLight::DD::pdf without texture (light/EnvMap.h:142-151)
-        float ps = lambert_pdf(dg, ls.direction);
+        float ps = ray.bsdf == 2 ? ONE_OVER_FOUR_PI : lambert_pdf(dg, ls.direction);
         shadow.mis = pl / ((pl + ps) + 1e-10f);
         if (shadow.mis < 1e-5f) shadow.tmax = -1.f;
       }
@@ -261,6 +268,14 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
   }
   ray.had_hit = false;
   ray.tmax = BNX_INF;
+  if (ray.bsdf == 2) {
+    // SimplePhase::scatter (Phase.h:155-170) draws two numbers and returns a
+    // VOLUME scatter, which SCI_VIS_MODE ends (shadeRays.cu:779-789)
+    (void)random();
+    (void)random();
+    ray.tmax = -1.f;
+    return;
+  }
   vec3 sdir, f_r;
   float spdf;
   lambert_scatter(ray.albedo, dg, random, sdir, spdf, f_r);
@@ -347,11 +362,31 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
   r.P = P;
   r.N = quantize_half(Nw);
   r.albedo = albedo;
+  r.bsdf = 1;
   if (ids) {
     ids->depth = h.t;
     ids->prim = h.prim;
     ids->inst = (geom.type == 1 && compat13) ? 13 : in.user_id;
     ids->obj = geom.user_id;
+  }
+}
+
+// Ray::setVolumeHit (render/Ray.h:137-144): SimplePhase(0.7 * rgb), N = 0
+void apply_volume_hit(const Scene &s, PathRay &r, int k, float t, const float *rgb, HitIDs *ids)
+{
+  const Volume &v = s.volumes[k];
+  vec3 d = fix_dir(r.dir);
+  r.had_hit = true;
+  r.tmax = t;
+  r.P = r.org + t * d;
+  r.N = vec3(0.f);
+  r.albedo = .7f * vec3(rgb[0], rgb[1], rgb[2]);
+  r.bsdf = 2;
+  if (ids) {
+    ids->depth = t;
+    ids->prim = 0;
+    ids->inst = v.user_inst;
+    ids->obj = v.user_id;
   }
 }
 
@@ -431,14 +466,31 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
       const int n = (int)queue.size();
       std::vector<HitIDs> ids(gen == 0 ? n : 0);
       std::vector<TraceCounters> ctrs(threads > 0 ? 1 : 1);
-      std::atomic<uint64_t> nn{0}, nt{0}, ns{0}, nsh{0};
+      std::atomic<uint64_t> nn{0}, nt{0}, ns{0}, nsh{0}, nvs{0};
       // ------------------------------------------ trace (traceRays.dev.cu:26-51)
       parallel_for(n, threads, [&](int i) {
         PathRay &r = queue[i];
         TraceCounters c;
         Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, false, stats ? &c : nullptr);
         if (gen == 0) ids[i] = {BNX_INF, -1, -1, -1};
-        apply_hit(s, r, h, gen == 0 ? &ids[i] : nullptr, compat13);
+        // volumes after the surface trace, clipped to the surface hit
+        float tcur = h.prim >= 0 ? h.t : r.tmax;
+        int vk = -1;
+        float vt = 0.f, vrgb[3] = {0.f, 0.f, 0.f};
+        uint64_t vs = 0;
+        const vec3 fd = fix_dir(r.dir);
+        for (int k = 0; k < (int)s.volumes.size(); k++) {
+          float t, rgb[3];
+          if (volume_march(s.volumes[k], (uint32_t)(s.insts.size() + k), r.org, fd, 0.f, tcur, r.rng, t, rgb, &vs)) {
+            vk = k;
+            vt = t;
+            tcur = t;
+            memcpy(vrgb, rgb, sizeof(vrgb));
+          }
+        }
+        if (vk >= 0) apply_volume_hit(s, r, vk, vt, vrgb, gen == 0 ? &ids[i] : nullptr);
+        else apply_hit(s, r, h, gen == 0 ? &ids[i] : nullptr, compat13);
+        if (stats) nvs += vs;
         if (stats) {
           nn += c.nodes;
           nt += c.tris;
@@ -454,6 +506,7 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
         stats->ctr.tris += nt;
         stats->ctr.spheres += ns;
         stats->ctr.rays += n;
+        stats->volume_samples += nvs;
       }
       // ------------------------------------------ _shadeRays (shadeRays.cu:883-1041)
       std::vector<PathRay> out((size_t)2 * n);
