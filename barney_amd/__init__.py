@@ -351,7 +351,9 @@ def render(world: World, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp:
     check(world.dev.L.bnx_render(world.h, C.byref(cam), C.byref(rend), frame.h_, spp, C.byref(st)))
     return dict(rays=st.rays_traced, shadow=st.shadow_rays, generations=st.generations,
                 volume_samples=st.volume_samples, ms=st.ms_total, nodes=st.node_visits, tris=st.tri_tests,
-                spheres=st.sphere_tests, trace_ms=st.trace_ms, trace_launches=st.trace_launches)
+                spheres=st.sphere_tests, trace_ms=st.trace_ms, trace_launches=st.trace_launches,
+                march_ms=st.march_ms, march_launches=st.march_launches, mc_cells=st.mc_cells,
+                march_rays=st.march_rays)
 
 
 class SurfaceWorld:

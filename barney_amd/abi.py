@@ -74,7 +74,8 @@ class RenderStats(C.Structure):
     _fields_ = [("rays_traced", C.c_int64), ("shadow_rays", C.c_int64), ("generations", C.c_int64),
                 ("volume_samples", C.c_int64), ("ms_total", C.c_double), ("node_visits", C.c_int64),
                 ("tri_tests", C.c_int64), ("sphere_tests", C.c_int64), ("trace_ms", C.c_double),
-                ("trace_launches", C.c_int64)]
+                ("trace_launches", C.c_int64), ("march_ms", C.c_double), ("march_launches", C.c_int64),
+                ("mc_cells", C.c_int64), ("march_rays", C.c_int64)]
 
 
 class BvhInfo(C.Structure):

@@ -320,13 +320,17 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
   if ((rc = Q.ensure((size_t)(2 * np * S))) != BNX_OK) return rc;
   int *counts = Q.counts.as<int>();
   unsigned long long *st = (unsigned long long *)(counts + 4);   // [0] shadow rays, [1..3] traversal counts
-  BNX_HIP(hipMemsetAsync(counts, 0, 64, dev->stream));
+  BNX_HIP(hipMemsetAsync(counts, 0, 128, dev->stream));
   if (!Q.ev_begin) {
     BNX_HIP(hipEventCreate(&Q.ev_begin));
     BNX_HIP(hipEventCreate(&Q.ev_end));
   }
-  double trace_ms = 0.0;
-  int64_t trace_launches = 0;
+  if (!Q.ev_vbegin) {
+    BNX_HIP(hipEventCreate(&Q.ev_vbegin));
+    BNX_HIP(hipEventCreate(&Q.ev_vend));
+  }
+  double trace_ms = 0.0, march_ms = 0.0;
+  int64_t trace_launches = 0, march_launches = 0, march_rays = 0;
   ShadeParams P{};
   P.rr = rr;
   P.dir_lights = w->d_dir_lights.as<bnx_dir_light>();
@@ -351,9 +355,14 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       RayQueue in = Q.q(cur), out = Q.q(1 - cur);
       TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
       BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
+      const int count_in = count;
       if ((rc = launch_trace_queue(g, tq, counts + cur, count, st + 1)) != BNX_OK) return rc;
       BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
-      if ((rc = launch_volume_march(dev, vl, in, counts + cur, count, st + 4)) != BNX_OK) return rc;
+      if (vl.count > 0) {
+        BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
+        if ((rc = launch_volume_march(dev, vl, in, counts + cur, count, st + 4)) != BNX_OK) return rc;
+        BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
+      }
       rays += count;
       gens++;
       BNX_HIP(hipMemsetAsync(counts + (1 - cur), 0, sizeof(int), dev->stream));
@@ -366,11 +375,17 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       BNX_HIP(hipEventElapsedTime(&ems, Q.ev_begin, Q.ev_end));
       trace_ms += ems;
       trace_launches++;
+      if (vl.count > 0) {
+        BNX_HIP(hipEventElapsedTime(&ems, Q.ev_vbegin, Q.ev_vend));
+        march_ms += ems;
+        march_launches++;
+        march_rays += count_in;
+      }
       cur = 1 - cur;
     }
     f->accum_id += nS;
   }
-  unsigned long long host_st[5] = {0, 0, 0, 0, 0};
+  unsigned long long host_st[6] = {0, 0, 0, 0, 0, 0};
   BNX_HIP(hipMemcpyAsync(host_st, st, sizeof(host_st), hipMemcpyDeviceToHost, dev->stream));
   BNX_HIP(hipStreamSynchronize(dev->stream));
   BNX_HIP(hipGetLastError());
@@ -385,6 +400,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     stats->sphere_tests = (int64_t)host_st[3];
     stats->trace_ms = trace_ms;
     stats->trace_launches = trace_launches;
+    stats->march_ms = march_ms;
+    stats->march_launches = march_launches;
+    stats->mc_cells = (int64_t)host_st[5];
+    stats->march_rays = march_rays;
   }
   return BNX_OK;
 }

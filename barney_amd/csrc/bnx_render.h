@@ -53,7 +53,7 @@ struct Queues {
   DevMem org[2], dir[2], rng[2], ps[2], hit[2], hit_v[2], vcol[2];
   DevMem counts;      // [0], [1]: queue counts; [4..]: statistics
   size_t cap = 0;
-  hipEvent_t ev_begin = nullptr, ev_end = nullptr;
+  hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_vbegin = nullptr, ev_vend = nullptr;
   int ensure(size_t n)
   {
     if (n <= cap) return BNX_OK;
@@ -66,7 +66,7 @@ struct Queues {
       BNX_HIP(hit_v[k].ensure(sizeof(float) * n));
       BNX_HIP(vcol[k].ensure(sizeof(float4) * n));
     }
-    BNX_HIP(counts.ensure(64));
+    BNX_HIP(counts.ensure(128));
     cap = n;
     return BNX_OK;
   }

@@ -41,8 +41,9 @@ struct Volume {
 
 int volume_build(Volume *v);
 int volume_rebuild_majorants(Volume *v);
+// stats[0]: samples, stats[1]: macrocells stepped
 int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
-                        unsigned long long *samples);
+                        unsigned long long *stats);
 
 // ---------------------------------------------------------------- device
 BNX_D uint64_t fnv3(uint32_t a, uint32_t b, uint32_t c)
@@ -157,7 +158,7 @@ BNX_D float fast_log(float f)
 // child seeded by hash(rtc instance, geometry 0, 0), dda3 over macrocells,
 // Woodcock sampling per non-empty cell.  'dir' is the fixed trace direction.
 BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, float &t_out,
-                        float rgb[3], uint32_t &nsamples)
+                        float rgb[3], uint32_t &nsamples, uint32_t &ncells)
 {
   // boxTest (render/Ray.h:169-217)
   float r0 = tmin, r1 = tmax;
@@ -219,6 +220,7 @@ BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
     const float c0 = ray_t0 + next_begin;
     const float c1 = ray_t0 + fmn(t_closest, dda_tmax);
     const float maj = v.majorants[(size_t)cell[0] + (size_t)v.mc[0] * ((size_t)cell[1] + (size_t)v.mc[1] * cell[2])];
+    ncells++;
     if (maj != 0.f) {
       // Woodcock::sampleRange (volume/Volume.h:168-203)
       const float upper = fmn(c1, ray_tmax);

@@ -68,10 +68,10 @@ BNX_D uint32_t wave_sum(uint32_t x)
 
 // march every queued ray through the world's volumes
 __global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q, const int *d_count, int *work,
-                                                      unsigned long long *samples)
+                                                      unsigned long long *stats)
 {
   const int n = *d_count;
-  uint32_t ns = 0;
+  uint32_t ns = 0, nc = 0;
   while (true) {
     int base = 0;
     if (lane_id() == 0) base = atomicAdd(work, 64);
@@ -92,7 +92,7 @@ __global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q,
     float vt = 0.f, rgb[3] = {0.f, 0.f, 0.f};
     for (int k = 0; k < vl.count; k++) {
       float t, c[3];
-      if (volume_march(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, ns)) {
+      if (volume_march(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, ns, nc)) {
         vk = k;
         vt = t;
         tcur = t;
@@ -107,8 +107,11 @@ __global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q,
       q.vcol[i] = make_float4(rgb[0], rgb[1], rgb[2], 0.f);
     }
   }
-  uint32_t s = wave_sum(ns);
-  if (lane_id() == 0 && samples) atomicAdd(samples, (unsigned long long)s);
+  uint32_t s = wave_sum(ns), c = wave_sum(nc);
+  if (lane_id() == 0 && stats) {
+    atomicAdd(stats, (unsigned long long)s);
+    atomicAdd(stats + 1, (unsigned long long)c);
+  }
 }
 
 __global__ void k_zero1(int *p) { *p = 0; }
@@ -142,14 +145,14 @@ int volume_rebuild_majorants(Volume *v)
 }
 
 int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
-                        unsigned long long *samples)
+                        unsigned long long *stats)
 {
   if (vl.count == 0 || max_n == 0) return BNX_OK;
   BNX_HIP(dev->counters.ensure(256));
   int *work = dev->counters.as<int>() + 2;
   k_zero1<<<1, 1, 0, dev->stream>>>(work);
   int grid = std::min(nblk(max_n, 256), dev->num_cus * 8);
-  k_volume_march<<<grid, 256, 0, dev->stream>>>(vl, q, d_count, work, samples);
+  k_volume_march<<<grid, 256, 0, dev->stream>>>(vl, q, d_count, work, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -183,6 +183,10 @@ typedef struct bnx_render_stats {
   int64_t sphere_tests;             /* sphere tests (16 B records)                 */
   double  trace_ms;                 /* summed trace-kernel time (HIP events)       */
   int64_t trace_launches;
+  double  march_ms;                 /* summed volume-march kernel time             */
+  int64_t march_launches;
+  int64_t mc_cells;                 /* macrocells stepped by the volume DDA        */
+  int64_t march_rays;               /* rays entering the volume-march kernel       */
 } bnx_render_stats;
 
 /* BVH summary of one BLAS / TLAS (for tests and the byte model). */
@@ -312,7 +316,7 @@ static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
 static_assert(sizeof(bnx_triangles_desc) == 72, "bnx_triangles_desc layout");
 static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
-static_assert(sizeof(bnx_render_stats) == 80, "bnx_render_stats layout");
+static_assert(sizeof(bnx_render_stats) == 112, "bnx_render_stats layout");
 #endif
 
 #endif /* BNX_RT_H */
