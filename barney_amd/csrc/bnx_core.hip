@@ -319,8 +319,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
   int rc;
   if ((rc = Q.ensure((size_t)(2 * np * S))) != BNX_OK) return rc;
   int *counts = Q.counts.as<int>();
-  unsigned long long *st = (unsigned long long *)(counts + 4);   // [0] shadow rays, [1..3] traversal counts
+  unsigned long long *st = Q.stats.as<unsigned long long>();
   BNX_HIP(hipMemsetAsync(counts, 0, 128, dev->stream));
+  BNX_HIP(hipMemsetAsync(st, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
+  const bool surfaces = g->scene.num_insts > 0;
   if (!Q.ev_begin) {
     BNX_HIP(hipEventCreate(&Q.ev_begin));
     BNX_HIP(hipEventCreate(&Q.ev_end));
@@ -356,11 +358,11 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
       BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
       const int count_in = count;
-      if ((rc = launch_trace_queue(g, tq, counts + cur, count, st + 1)) != BNX_OK) return rc;
+      if (surfaces && (rc = launch_trace_queue(g, tq, counts + cur, count, st)) != BNX_OK) return rc;
       BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
       if (vl.count > 0) {
         BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
-        if ((rc = launch_volume_march(dev, vl, in, counts + cur, count, st + 4)) != BNX_OK) return rc;
+        if ((rc = launch_volume_march(dev, vl, in, counts + cur, count, surfaces, st)) != BNX_OK) return rc;
         BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
       }
       rays += count;
@@ -373,8 +375,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       BNX_HIP(hipStreamSynchronize(dev->stream));
       float ems = 0.f;
       BNX_HIP(hipEventElapsedTime(&ems, Q.ev_begin, Q.ev_end));
-      trace_ms += ems;
-      trace_launches++;
+      if (surfaces) {
+        trace_ms += ems;
+        trace_launches++;
+      }
       if (vl.count > 0) {
         BNX_HIP(hipEventElapsedTime(&ems, Q.ev_vbegin, Q.ev_vend));
         march_ms += ems;
@@ -385,24 +389,28 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     }
     f->accum_id += nS;
   }
-  unsigned long long host_st[6] = {0, 0, 0, 0, 0, 0};
-  BNX_HIP(hipMemcpyAsync(host_st, st, sizeof(host_st), hipMemcpyDeviceToHost, dev->stream));
+  std::vector<unsigned long long> slots((size_t)STAT_COUNT * STAT_SLOTS);
+  BNX_HIP(hipMemcpyAsync(slots.data(), st, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost,
+                         dev->stream));
   BNX_HIP(hipStreamSynchronize(dev->stream));
   BNX_HIP(hipGetLastError());
+  unsigned long long host_st[STAT_COUNT] = {};
+  for (int k = 0; k < STAT_COUNT; k++)
+    for (int j = 0; j < STAT_SLOTS; j++) host_st[k] += slots[(size_t)k * STAT_SLOTS + j];
   if (stats) {
     stats->rays_traced = rays;
-    stats->shadow_rays = (int64_t)host_st[0];
+    stats->shadow_rays = (int64_t)host_st[STAT_SHADOW];
     stats->generations = gens;
-    stats->volume_samples = (int64_t)host_st[4];
+    stats->volume_samples = (int64_t)host_st[STAT_SAMPLES];
     stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    stats->node_visits = (int64_t)host_st[1];
-    stats->tri_tests = (int64_t)host_st[2];
-    stats->sphere_tests = (int64_t)host_st[3];
+    stats->node_visits = (int64_t)host_st[STAT_NODES];
+    stats->tri_tests = (int64_t)host_st[STAT_TRIS];
+    stats->sphere_tests = (int64_t)host_st[STAT_SPHERES];
     stats->trace_ms = trace_ms;
     stats->trace_launches = trace_launches;
     stats->march_ms = march_ms;
     stats->march_launches = march_launches;
-    stats->mc_cells = (int64_t)host_st[5];
+    stats->mc_cells = (int64_t)host_st[STAT_CELLS];
     stats->march_rays = march_rays;
   }
   return BNX_OK;

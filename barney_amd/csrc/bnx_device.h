@@ -55,6 +55,10 @@ BNX_D v3 fix_dir(v3 d) { return v3{fix1(d.x), fix1(d.y), fix1(d.z)}; }
 // vec3h storage (render/Ray.h:17-23,76): hardware IEEE RNE conversion
 BNX_D uint32_t f2h(float f)
 {
+  // The empty asm pins f as a float32 register value: without it the
+  // compiler fuses a preceding multiply into v_fma_mix*_f16, rounding once to
+  // fp16 instead of to float32 and then to fp16 as the scalar code does.
+  asm volatile("" : "+v"(f));
   _Float16 h = (_Float16)f;
   return (uint32_t)__builtin_bit_cast(unsigned short, h);
 }
@@ -185,5 +189,23 @@ BNX_D float o2f(int i) { return __int_as_float(i >= 0 ? i : (i ^ 0x7fffffff)); }
 // ----------------------------------------------------------- wave helpers
 BNX_D int lane_id() { return __lane_id(); }
 BNX_D uint64_t lanemask_lt() { return (1ull << __lane_id()) - 1ull; }
+BNX_D uint32_t wave_sum(uint32_t x)
+{
+  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
+  return x;
+}
+
+// Statistics counters, spread over STAT_SLOTS addresses per counter so that
+// the per-wave atomics of a launch do not serialise on one L2 line; the host
+// sums the slots.
+constexpr int STAT_SLOTS = 256;
+enum { STAT_SHADOW = 0, STAT_NODES, STAT_TRIS, STAT_SPHERES, STAT_SAMPLES, STAT_CELLS, STAT_COUNT };
+// call with the wave-wide total from lane 0
+BNX_D void stat_add(unsigned long long *stats, int idx, uint32_t v)
+{
+  if (!stats || v == 0) return;
+  uint32_t slot = (blockIdx.x * (blockDim.x >> 6) + (threadIdx.x >> 6)) & (STAT_SLOTS - 1);
+  atomicAdd(stats + idx * STAT_SLOTS + slot, (unsigned long long)v);
+}
 
 } // namespace bnx

@@ -26,7 +26,7 @@ struct ShadeParams {
   int W, H;
   int accum_base;
   int generation;
-  unsigned long long *stats;   // [0] shadow rays written
+  unsigned long long *stats;   // statistics array (STAT_* x STAT_SLOTS)
 };
 
 struct FrameDev {
@@ -51,7 +51,8 @@ inline FrameDev frame_dev(Frame *f)
 // per-device ray queues (two, ping-pong) and counters
 struct Queues {
   DevMem org[2], dir[2], rng[2], ps[2], hit[2], hit_v[2], vcol[2];
-  DevMem counts;      // [0], [1]: queue counts; [4..]: statistics
+  DevMem counts;      // [0], [1]: queue counts
+  DevMem stats;       // STAT_COUNT x STAT_SLOTS 64-bit counters
   size_t cap = 0;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_vbegin = nullptr, ev_vend = nullptr;
   int ensure(size_t n)
@@ -67,6 +68,7 @@ struct Queues {
       BNX_HIP(vcol[k].ensure(sizeof(float4) * n));
     }
     BNX_HIP(counts.ensure(128));
+    BNX_HIP(stats.ensure(sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS));
     cap = n;
     return BNX_OK;
   }

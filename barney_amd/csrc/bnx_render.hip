@@ -298,9 +298,13 @@ BNX_D void pack_ps(uint4 &ps, v3 tp, int ndiff, int pixel, float mis)
   ps.w = __float_as_uint(mis);
 }
 
-__global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQueue in, const int *in_count,
-                                               RayQueue out, int *out_count, FrameDev fb)
+constexpr int SHADE_BLOCK = 1024;
+
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P, RayQueue in, const int *in_count,
+                                                       RayQueue out, int *out_count, FrameDev fb)
 {
+  __shared__ int wave_base[SHADE_BLOCK / 64];
+  __shared__ int block_base;
   const int i = blockIdx.x * blockDim.x + threadIdx.x;
   const int n = *in_count;
   const bool valid = i < n;
@@ -483,16 +487,29 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQue
     if (frag.y > 0.f) acc_add(acc + 1, frag.y);
     if (frag.z > 0.f) acc_add(acc + 2, frag.z);
   }
-  // ------------------------------------------------ wave-compacted append
+  // ------------------------------------------------ compacted append
+  // wave ballot + popcount gives each lane its offset inside the wave, the
+  // wave totals are scanned in LDS, and one atomic per block reserves the
+  // block's range in the output queue
   const uint64_t b1 = __ballot(w_sh);
   const uint64_t b2 = __ballot(w_ray);
   const uint64_t lt = lanemask_lt();
   const int pre = __popcll(b1 & lt) + __popcll(b2 & lt);
   const int tot = __popcll(b1) + __popcll(b2);
-  int base = 0;
-  if (lane_id() == 0 && tot > 0) base = atomicAdd(out_count, tot);
-  base = __shfl(base, 0);
-  int slot = base + pre;
+  const int wid = threadIdx.x >> 6;
+  if (lane_id() == 0) wave_base[wid] = tot;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    int acc = 0;
+    for (int w = 0; w < SHADE_BLOCK / 64; w++) {
+      int c = wave_base[w];
+      wave_base[w] = acc;
+      acc += c;
+    }
+    block_base = acc > 0 ? atomicAdd(out_count, acc) : 0;
+  }
+  __syncthreads();
+  int slot = block_base + wave_base[wid] + pre;
   if (w_sh) {
     out.org[slot] = o_sh.org;
     out.dir[slot] = o_sh.dir;
@@ -506,7 +523,7 @@ __global__ __launch_bounds__(256) void k_shade(SceneDev S, ShadeParams P, RayQue
     out.rng[slot] = o_ray.rng;
     out.ps[slot] = o_ray.ps;
   }
-  if (lane_id() == 0 && tot > 0 && P.stats) atomicAdd(P.stats, (unsigned long long)__popcll(b1));
+  if (lane_id() == 0) stat_add(P.stats, STAT_SHADOW, (uint32_t)__popcll(b1));
 }
 
 // ----------------------------------------------------------- frame ops
@@ -612,7 +629,8 @@ int launch_raygen(Device *dev, const bnx_camera &cam, const bnx_renderer &rr, in
 int launch_shade(Device *dev, const SceneDev &S, const ShadeParams &P, const RayQueue &in, const int *in_count,
                  int max_in, const RayQueue &out, int *out_count, const FrameDev &fb)
 {
-  if (max_in > 0) k_shade<<<nblk(max_in, 256), 256, 0, dev->stream>>>(S, P, in, in_count, out, out_count, fb);
+  if (max_in > 0)
+    k_shade<<<nblk(max_in, SHADE_BLOCK), SHADE_BLOCK, 0, dev->stream>>>(S, P, in, in_count, out, out_count, fb);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -225,25 +225,15 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
   return best;
 }
 
-// take 64 rays per wave from a device counter
-BNX_D int wave_take(int *work)
-{
-  int base = 0;
-  if (lane_id() == 0) base = atomicAdd(work, 64);
-  return __shfl(base, 0);
-}
-
 __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *rays, int n, bnx_hit *hits,
-                                                  uint32_t flags, int *work)
+                                                  uint32_t flags)
 {
   __shared__ uint32_t ring[SHORT * TB];
   Stack st;
   st.lds = ring + threadIdx.x;
-  while (true) {
-    int base = wave_take(work);
-    if (base >= n) break;
-    int i = base + lane_id();
-    if (i >= n) continue;
+  {
+    const int i = blockIdx.x * TB + threadIdx.x;
+    if (i >= n) return;
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
@@ -266,26 +256,19 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
   }
 }
 
-BNX_D uint32_t wave_sum(uint32_t v)
-{
-  for (int off = 32; off > 0; off >>= 1) v += __shfl_xor(v, off);
-  return v;
-}
-
-// counters[0..2]: node visits, triangle tests, sphere tests (64-bit)
-__global__ __launch_bounds__(TB) void k_trace_queue(SceneDev S, TraceQueue q, const int *d_count, int *work,
+// one ray per thread; blocks past the queue count exit at once.
+// counters: statistics array (STAT_NODES, STAT_TRIS, STAT_SPHERES)
+__global__ __launch_bounds__(TB) void k_trace_queue(SceneDev S, TraceQueue q, const int *d_count,
                                                     unsigned long long *counters)
 {
   __shared__ uint32_t ring[SHORT * TB];
+  const int n = *d_count;
+  if ((int)(blockIdx.x * TB) >= n) return;
   Stack st;
   st.lds = ring + threadIdx.x;
-  const int n = *d_count;
   Counts cnt{0, 0, 0};
-  while (true) {
-    int base = wave_take(work);
-    if (base >= n) break;
-    int i = base + lane_id();
-    if (i >= n) continue;
+  const int i = blockIdx.x * TB + threadIdx.x;
+  if (i < n) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
     bool shadow = (__float_as_uint(d.w) & 1u) != 0;
@@ -295,25 +278,14 @@ __global__ __launch_bounds__(TB) void k_trace_queue(SceneDev S, TraceQueue q, co
     q.hit_v[i] = h.v;
   }
   uint32_t a = wave_sum(cnt.nodes), b = wave_sum(cnt.tris), c = wave_sum(cnt.spheres);
-  if (lane_id() == 0 && counters) {
-    atomicAdd(counters + 0, (unsigned long long)a);
-    atomicAdd(counters + 1, (unsigned long long)b);
-    atomicAdd(counters + 2, (unsigned long long)c);
+  if (lane_id() == 0) {
+    stat_add(counters, STAT_NODES, a);
+    stat_add(counters, STAT_TRIS, b);
+    stat_add(counters, STAT_SPHERES, c);
   }
 }
 
-__global__ void k_zero(int *p, int n)
-{
-  int i = blockIdx.x * blockDim.x + threadIdx.x;
-  if (i < n) p[i] = 0;
-}
-
-int grid_for(Device *dev, int n)
-{
-  int want = (n + TB - 1) / TB;
-  int cap = dev->num_cus * 8;
-  return want < cap ? (want > 0 ? want : 1) : cap;
-}
+int grid_for(int n) { return n > 0 ? (n + TB - 1) / TB : 1; }
 
 } // namespace
 
@@ -322,10 +294,7 @@ int trace_stack_capacity() { return STACK_CAP; }
 int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags)
 {
   Device *dev = g->dev;
-  BNX_HIP(dev->counters.ensure(256));
-  int *work = dev->counters.as<int>();
-  k_zero<<<1, 64, 0, dev->stream>>>(work, 1);
-  k_trace_api<<<grid_for(dev, n), TB, 0, dev->stream>>>(g->scene, d_rays, n, d_hits, flags, work);
+  k_trace_api<<<grid_for(n), TB, 0, dev->stream>>>(g->scene, d_rays, n, d_hits, flags);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }
@@ -333,10 +302,7 @@ int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, ui
 int launch_trace_queue(Group *g, const TraceQueue &q, const int *d_count, int max_n, unsigned long long *counters)
 {
   Device *dev = g->dev;
-  BNX_HIP(dev->counters.ensure(256));
-  int *work = dev->counters.as<int>() + 1;
-  k_zero<<<1, 64, 0, dev->stream>>>(work, 1);
-  k_trace_queue<<<grid_for(dev, max_n), TB, 0, dev->stream>>>(g->scene, q, d_count, work, counters);
+  k_trace_queue<<<grid_for(max_n), TB, 0, dev->stream>>>(g->scene, q, d_count, counters);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -41,9 +41,10 @@ struct Volume {
 
 int volume_build(Volume *v);
 int volume_rebuild_majorants(Volume *v);
-// stats[0]: samples, stats[1]: macrocells stepped
+// surfaces: false when the world has no surface instances (the trace kernel
+// was skipped and the hit records are not written yet)
 int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
-                        unsigned long long *stats);
+                        bool surfaces, unsigned long long *stats);
 
 // ---------------------------------------------------------------- device
 BNX_D uint64_t fnv3(uint32_t a, uint32_t b, uint32_t c)

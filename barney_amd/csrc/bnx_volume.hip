@@ -60,61 +60,57 @@ __global__ void k_majorants(VolumeDev v, const float2 *ranges, float *maj)
   maj[tid] = tf_majorant(v, r.x, r.y);
 }
 
-BNX_D uint32_t wave_sum(uint32_t x)
-{
-  for (int off = 32; off > 0; off >>= 1) x += __shfl_xor(x, off);
-  return x;
-}
-
-// march every queued ray through the world's volumes
-__global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q, const int *d_count, int *work,
+// march every queued ray through the world's volumes, one ray per thread.
+// With no surface instances the trace kernel is skipped and every ray
+// enters here as a surface miss.
+__global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q, const int *d_count, int surfaces,
                                                       unsigned long long *stats)
 {
   const int n = *d_count;
+  if ((int)(blockIdx.x * 256) >= n) return;
   uint32_t ns = 0, nc = 0;
-  while (true) {
-    int base = 0;
-    if (lane_id() == 0) base = atomicAdd(work, 64);
-    base = __shfl(base, 0);
-    if (base >= n) break;
-    int i = base + lane_id();
-    if (i >= n) continue;
+  const int i = blockIdx.x * 256 + threadIdx.x;
+  if (i < n) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
-    int4 hit = q.hit[i];
+    int4 hit = surfaces ? q.hit[i] : make_int4(-1, -1, -1, 0);
     const bool shadow = (__float_as_uint(d.w) & 1u) != 0;
-    if (shadow && hit.x >= 0) continue;            // already occluded
-    const v3 org = xyz(o), dir = fix_dir(xyz(d));
-    float tcur = o.w;
-    uint64_t rng = q.rng[i];
-    const uint64_t rng0 = rng;
-    int vk = -1;
-    float vt = 0.f, rgb[3] = {0.f, 0.f, 0.f};
-    for (int k = 0; k < vl.count; k++) {
-      float t, c[3];
-      if (volume_march(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, ns, nc)) {
-        vk = k;
-        vt = t;
-        tcur = t;
-        rgb[0] = c[0]; rgb[1] = c[1]; rgb[2] = c[2];
-        if (shadow) break;
+    if (!(shadow && hit.x >= 0)) {                 // an occluded shadow ray is done
+      const v3 org = xyz(o), dir = fix_dir(xyz(d));
+      float tcur = o.w;
+      uint64_t rng = q.rng[i];
+      const uint64_t rng0 = rng;
+      int vk = -1;
+      float vt = 0.f, rgb[3] = {0.f, 0.f, 0.f};
+      for (int k = 0; k < vl.count; k++) {
+        float t, c[3];
+        if (volume_march(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, ns, nc)) {
+          vk = k;
+          vt = t;
+          tcur = t;
+          rgb[0] = c[0]; rgb[1] = c[1]; rgb[2] = c[2];
+          if (shadow) break;
+        }
       }
-    }
-    if (rng != rng0) q.rng[i] = rng;
-    if (vk >= 0) {
-      q.org[i].w = vt;
-      q.hit[i] = make_int4(0, VOLUME_GEOM_BASE + vk, vl.vols[vk].rtc_inst, 0);
-      q.vcol[i] = make_float4(rgb[0], rgb[1], rgb[2], 0.f);
+      if (rng != rng0) q.rng[i] = rng;
+      if (vk >= 0) {
+        q.org[i].w = vt;
+        hit = make_int4(0, VOLUME_GEOM_BASE + vk, vl.vols[vk].rtc_inst, 0);
+        q.vcol[i] = make_float4(rgb[0], rgb[1], rgb[2], 0.f);
+        q.hit[i] = hit;
+      } else if (!surfaces) {
+        q.hit[i] = hit;
+      }
+    } else if (!surfaces) {
+      q.hit[i] = hit;
     }
   }
   uint32_t s = wave_sum(ns), c = wave_sum(nc);
-  if (lane_id() == 0 && stats) {
-    atomicAdd(stats, (unsigned long long)s);
-    atomicAdd(stats + 1, (unsigned long long)c);
+  if (lane_id() == 0) {
+    stat_add(stats, STAT_SAMPLES, s);
+    stat_add(stats, STAT_CELLS, c);
   }
 }
-
-__global__ void k_zero1(int *p) { *p = 0; }
 
 } // namespace
 
@@ -145,14 +141,10 @@ int volume_rebuild_majorants(Volume *v)
 }
 
 int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
-                        unsigned long long *stats)
+                        bool surfaces, unsigned long long *stats)
 {
   if (vl.count == 0 || max_n == 0) return BNX_OK;
-  BNX_HIP(dev->counters.ensure(256));
-  int *work = dev->counters.as<int>() + 2;
-  k_zero1<<<1, 1, 0, dev->stream>>>(work);
-  int grid = std::min(nblk(max_n, 256), dev->num_cus * 8);
-  k_volume_march<<<grid, 256, 0, dev->stream>>>(vl, q, d_count, work, stats);
+  k_volume_march<<<nblk(max_n, 256), 256, 0, dev->stream>>>(vl, q, d_count, surfaces ? 1 : 0, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }
