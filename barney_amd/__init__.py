@@ -74,6 +74,14 @@ _PROTOS = [
     ("bnx_frame_destroy", [vp], C.c_int),
     ("bnx_render", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.POINTER(abi.RenderStats)],
      C.c_int),
+    ("bnx_server_create", [vp, C.POINTER(abi.ServerConfig), C.POINTER(vp)], C.c_int),
+    ("bnx_server_destroy", [vp], C.c_int),
+    ("bnx_server_submit", [vp, C.POINTER(abi.Request), C.POINTER(C.c_int64)], C.c_int),
+    ("bnx_server_step", [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int)], C.c_int),
+    ("bnx_server_poll", [vp, C.c_int64, C.POINTER(abi.RequestStatus)], C.c_int),
+    ("bnx_server_read", [vp, C.c_int64, C.c_int, vp], C.c_int),
+    ("bnx_server_release", [vp, C.c_int64], C.c_int),
+    ("bnx_server_get_stats", [vp, C.POINTER(abi.ServerStats)], C.c_int),
 ]
 EXPORTED = [p[0] for p in _PROTOS]
 
@@ -354,6 +362,78 @@ def render(world: World, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp:
                 spheres=st.sphere_tests, trace_ms=st.trace_ms, trace_launches=st.trace_launches,
                 march_ms=st.march_ms, march_launches=st.march_launches, mc_cells=st.mc_cells,
                 march_rays=st.march_rays)
+
+
+class Server:
+    """Continuous-batching request server (bnx_server_*): requests of any
+    resident world, resolution and spp share one wavefront on the device."""
+
+    def __init__(self, dev: Device, frame_memory_budget=0, max_paths_in_flight=0, max_active=0):
+        self.dev, self.L = dev, dev.L
+        cfg = abi.ServerConfig(frame_memory_budget, max_paths_in_flight, max_active, 0)
+        h = vp()
+        check(self.L.bnx_server_create(dev.h, C.byref(cfg), C.byref(h)))
+        self.h = h
+        self.sizes = {}
+        self.pending = set()
+        self._done = (C.c_int64 * 4096)()
+
+    def submit(self, world: World, cam: abi.Camera, rend: abi.Renderer, width: int, height: int, spp: int) -> int:
+        r = abi.Request(world.h, cam, width, height, rend, spp, 0)
+        t = C.c_int64()
+        check(self.L.bnx_server_submit(self.h, C.byref(r), C.byref(t)))
+        self.sizes[t.value] = (width, height)
+        self.pending.add(t.value)
+        return t.value
+
+    def step(self) -> list:
+        n = C.c_int()
+        check(self.L.bnx_server_step(self.h, self._done, len(self._done), C.byref(n)))
+        done = [self._done[k] for k in range(min(n.value, len(self._done)))]
+        self.pending.difference_update(done)
+        return done
+
+    def poll(self, ticket: int) -> dict:
+        st = abi.RequestStatus()
+        check(self.L.bnx_server_poll(self.h, ticket, C.byref(st)))
+        return dict(state=st.state, submit_ms=st.submit_ms, admit_ms=st.admit_ms, done_ms=st.done_ms,
+                    paths_started=st.paths_started, paths_total=st.paths_total)
+
+    def read(self, ticket: int, channel: int) -> np.ndarray:
+        w, h = self.sizes[ticket]
+        if channel == abi.BNX_FB_COLOR:
+            out = np.zeros((h, w, 4), np.float32)
+        elif channel == abi.BNX_FB_NORMAL:
+            out = np.zeros((h, w, 3), np.float32)
+        elif channel == abi.BNX_FB_DEPTH:
+            out = np.zeros((h, w), np.float32)
+        else:
+            out = np.zeros((h, w), np.int32)
+        check(self.L.bnx_server_read(self.h, ticket, channel, out.ctypes.data))
+        return out
+
+    def release(self, ticket: int):
+        check(self.L.bnx_server_release(self.h, ticket))
+        self.sizes.pop(ticket, None)
+
+    def stats(self) -> dict:
+        st = abi.ServerStats()
+        check(self.L.bnx_server_get_stats(self.h, C.byref(st)))
+        return {k: getattr(st, k) for k, _ in abi.ServerStats._fields_}
+
+    def run_until_idle(self, max_steps=1_000_000) -> list:
+        """step until every submitted request is done; returns the tickets in completion order"""
+        done = []
+        for _ in range(max_steps):
+            if not self.pending:
+                break
+            done += self.step()
+        return done
+
+    def close(self):
+        if self.h:
+            self.L.bnx_server_destroy(self.h)
+            self.h = None
 
 
 class SurfaceWorld:

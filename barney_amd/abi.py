@@ -83,7 +83,33 @@ class BvhInfo(C.Structure):
                 ("root_is_leaf", C.c_int32), ("root_lower", f3), ("root_upper", f3), ("build_ms", C.c_double)]
 
 
+class ServerConfig(C.Structure):
+    _fields_ = [("frame_memory_budget", C.c_size_t), ("max_paths_in_flight", C.c_int64),
+                ("max_active", C.c_int32), ("_pad", C.c_int32)]
+
+
+class Request(C.Structure):
+    _fields_ = [("world", C.c_void_p), ("camera", Camera), ("width", C.c_int32), ("height", C.c_int32),
+                ("renderer", Renderer), ("spp", C.c_int32), ("_pad", C.c_int32)]
+
+
+class RequestStatus(C.Structure):
+    _fields_ = [("state", C.c_int32), ("_pad", C.c_int32), ("submit_ms", C.c_double), ("admit_ms", C.c_double),
+                ("done_ms", C.c_double), ("paths_started", C.c_int64), ("paths_total", C.c_int64)]
+
+
+class ServerStats(C.Structure):
+    _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("done", C.c_int64), ("released", C.c_int64),
+                ("rays_traced", C.c_int64), ("shadow_rays", C.c_int64), ("volume_samples", C.c_int64),
+                ("generations", C.c_int64), ("peak_queue", C.c_int64), ("active", C.c_int32),
+                ("queued", C.c_int32), ("frame_bytes", C.c_size_t), ("busy_ms", C.c_double)]
+
+
+BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE = 0, 1, 2
+
 assert C.sizeof(Material) == 32 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
+assert C.sizeof(ServerConfig) == 24 and C.sizeof(Request) == 136 and C.sizeof(RequestStatus) == 48
+assert C.sizeof(ServerStats) == 96 and Request.renderer.offset == 76 and Request.spp.offset == 124
 
 # 64-byte BVH node: child boxes lo0,hi0,lo1,hi1 then ref[2], cnt[2]
 NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3), ("hi1", "<f4", 3),

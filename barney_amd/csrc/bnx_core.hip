@@ -276,12 +276,6 @@ int build_tlas(Group *g)
   return BNX_OK;
 }
 
-Queues &queues_for(Device *dev)
-{
-  if (!dev->queues) dev->queues = new Queues;
-  return *(Queues *)dev->queues;
-}
-
 } // namespace
 
 int world_volumes(World *w, VolumeList &vl)
@@ -305,117 +299,6 @@ int world_volumes(World *w, VolumeList &vl)
   return BNX_OK;
 }
 
-int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp, bnx_render_stats *stats)
-{
-  auto t0 = std::chrono::steady_clock::now();
-  Device *dev = w->dev;
-  Group *g = w->instances;
-  const int W = f->w, H = f->h;
-  const long np = (long)W * H;
-  int64_t rays = 0, gens = 0;
-  Queues &Q = queues_for(dev);
-  // samples in flight per wave: about 2M pixel-samples
-  int S = (int)std::max(1L, std::min({(long)spp, (2L << 20) / std::max(1L, np), 4096L}));
-  int rc;
-  if ((rc = Q.ensure((size_t)(2 * np * S))) != BNX_OK) return rc;
-  int *counts = Q.counts.as<int>();
-  unsigned long long *st = Q.stats.as<unsigned long long>();
-  BNX_HIP(hipMemsetAsync(counts, 0, 128, dev->stream));
-  BNX_HIP(hipMemsetAsync(st, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
-  const bool surfaces = g->scene.num_insts > 0;
-  if (!Q.ev_begin) {
-    BNX_HIP(hipEventCreate(&Q.ev_begin));
-    BNX_HIP(hipEventCreate(&Q.ev_end));
-  }
-  if (!Q.ev_vbegin) {
-    BNX_HIP(hipEventCreate(&Q.ev_vbegin));
-    BNX_HIP(hipEventCreate(&Q.ev_vend));
-  }
-  double trace_ms = 0.0, march_ms = 0.0;
-  int64_t trace_launches = 0, march_launches = 0, march_rays = 0;
-  ShadeParams P{};
-  P.rr = rr;
-  P.dir_lights = w->d_dir_lights.as<bnx_dir_light>();
-  P.num_dir_lights = (int)w->dir_lights.size();
-  P.W = W;
-  P.H = H;
-  P.stats = st;
-  FrameDev fd = frame_dev(f);
-  VolumeList vl{};
-  if ((rc = world_volumes(w, vl)) != BNX_OK) return rc;
-  P.vols = vl.vols;
-  P.num_vols = vl.count;
-  const int max_gen = rr.max_generations > 0 ? rr.max_generations : 1 << 30;
-  for (int s0 = 0; s0 < spp; s0 += S) {
-    const int nS = std::min(S, spp - s0);
-    int cur = 0;
-    int count = (int)(np * nS);
-    if ((rc = launch_raygen(dev, cam, rr, W, H, f->accum_id, nS, Q.q(cur))) != BNX_OK) return rc;
-    BNX_HIP(hipMemcpyAsync(counts + cur, &count, sizeof(int), hipMemcpyHostToDevice, dev->stream));
-    P.accum_base = f->accum_id;
-    for (int gen = 0; count > 0 && gen < max_gen; gen++) {
-      RayQueue in = Q.q(cur), out = Q.q(1 - cur);
-      TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
-      BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
-      const int count_in = count;
-      if (surfaces && (rc = launch_trace_queue(g, tq, counts + cur, count, st)) != BNX_OK) return rc;
-      BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
-      if (vl.count > 0) {
-        BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
-        if ((rc = launch_volume_march(dev, vl, in, counts + cur, count, surfaces, st)) != BNX_OK) return rc;
-        BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
-      }
-      rays += count;
-      gens++;
-      BNX_HIP(hipMemsetAsync(counts + (1 - cur), 0, sizeof(int), dev->stream));
-      P.generation = gen;
-      if ((rc = launch_shade(dev, g->scene, P, in, counts + cur, count, out, counts + (1 - cur), fd)) != BNX_OK)
-        return rc;
-      BNX_HIP(hipMemcpyAsync(&count, counts + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, dev->stream));
-      BNX_HIP(hipStreamSynchronize(dev->stream));
-      float ems = 0.f;
-      BNX_HIP(hipEventElapsedTime(&ems, Q.ev_begin, Q.ev_end));
-      if (surfaces) {
-        trace_ms += ems;
-        trace_launches++;
-      }
-      if (vl.count > 0) {
-        BNX_HIP(hipEventElapsedTime(&ems, Q.ev_vbegin, Q.ev_vend));
-        march_ms += ems;
-        march_launches++;
-        march_rays += count_in;
-      }
-      cur = 1 - cur;
-    }
-    f->accum_id += nS;
-  }
-  std::vector<unsigned long long> slots((size_t)STAT_COUNT * STAT_SLOTS);
-  BNX_HIP(hipMemcpyAsync(slots.data(), st, sizeof(unsigned long long) * slots.size(), hipMemcpyDeviceToHost,
-                         dev->stream));
-  BNX_HIP(hipStreamSynchronize(dev->stream));
-  BNX_HIP(hipGetLastError());
-  unsigned long long host_st[STAT_COUNT] = {};
-  for (int k = 0; k < STAT_COUNT; k++)
-    for (int j = 0; j < STAT_SLOTS; j++) host_st[k] += slots[(size_t)k * STAT_SLOTS + j];
-  if (stats) {
-    stats->rays_traced = rays;
-    stats->shadow_rays = (int64_t)host_st[STAT_SHADOW];
-    stats->generations = gens;
-    stats->volume_samples = (int64_t)host_st[STAT_SAMPLES];
-    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
-    stats->node_visits = (int64_t)host_st[STAT_NODES];
-    stats->tri_tests = (int64_t)host_st[STAT_TRIS];
-    stats->sphere_tests = (int64_t)host_st[STAT_SPHERES];
-    stats->trace_ms = trace_ms;
-    stats->trace_launches = trace_launches;
-    stats->march_ms = march_ms;
-    stats->march_launches = march_launches;
-    stats->mc_cells = (int64_t)host_st[STAT_CELLS];
-    stats->march_rays = march_rays;
-  }
-  return BNX_OK;
-}
-
 } // namespace bnx
 
 using namespace bnx;
@@ -427,18 +310,6 @@ struct bnx_group_s : Group {};
 struct bnx_world_s : World {};
 struct bnx_frame_s : Frame {};
 struct bnx_volume_s : Volume {};
-
-#define GUARD_BEGIN try {
-#define GUARD_END                                                                                                    \
-  }                                                                                                                  \
-  catch (const std::bad_alloc &) {                                                                                   \
-    set_error("host allocation failed");                                                                             \
-    return BNX_ERR_OOM;                                                                                              \
-  }                                                                                                                  \
-  catch (const std::exception &e) {                                                                                  \
-    set_error(e.what());                                                                                             \
-    return BNX_ERR_INVALID;                                                                                          \
-  }
 
 extern "C" {
 
@@ -468,8 +339,7 @@ int bnx_device_destroy(bnx_device d)
 {
   BNX_CHECK(d, BNX_ERR_INVALID, "bnx_device_destroy: null device");
   (void)hipStreamSynchronize(d->stream);
-  delete (Queues *)d->queues;
-  d->queues = nullptr;
+  wavefront_destroy(d);
   (void)hipStreamDestroy(d->stream);
   delete d;
   return BNX_OK;

@@ -1,0 +1,617 @@
+// bnx_engine.hip -- host side of the wavefront: the per-request frame loop
+// (bnx_render) and the request server (bnx_server_*).  Both drive the same
+// generation: trace, volume march and shade of one shared ray queue, whose
+// rays may belong to any number of requests (each ray names its request
+// slot; per-request state lives in a slot table in HBM).
+//
+// Replaces Context::renderTiles (native/Context.cpp:139-182) and the
+// RayQueue count read-back (render/RayQueue.cpp:47-57).  The server has no
+// counterpart in the reference: it continuously batches requests of
+// different worlds, resolutions and spp through that one wavefront.
+#include "bnx_internal.h"
+#include "bnx_render.h"
+#include "bnx_volume.h"
+
+#include <algorithm>
+#include <chrono>
+#include <deque>
+#include <map>
+#include <unordered_map>
+
+namespace bnx {
+namespace {
+
+// pinned host memory for read-backs and small uploads
+struct Pinned {
+  void *p = nullptr;
+  size_t bytes = 0;
+  ~Pinned()
+  {
+    if (p) (void)hipHostFree(p);
+  }
+  hipError_t ensure(size_t b)
+  {
+    if (b <= bytes && p) return hipSuccess;
+    if (p) (void)hipHostFree(p);
+    p = nullptr;
+    bytes = 0;
+    hipError_t e = hipHostMalloc(&p, b ? b : 16, hipHostMallocDefault);
+    if (e == hipSuccess) bytes = b;
+    return e;
+  }
+  template <typename T> T *as() const { return (T *)p; }
+};
+
+// queues, tables and staging of one wavefront
+struct Wavefront {
+  Queues Q;
+  DevMem slots, chunks, live, slot_list, live_out;
+  Pinned h_slots, h_chunks, h_list, h_live, h_count;
+  int slot_cap = 0, chunk_cap = 0;
+
+  int ensure(size_t rays, int nslots, int nchunks)
+  {
+    int rc = Q.ensure(rays);
+    if (rc != BNX_OK) return rc;
+    if (!Q.ev_begin) {
+      BNX_HIP(hipEventCreate(&Q.ev_begin));
+      BNX_HIP(hipEventCreate(&Q.ev_end));
+      BNX_HIP(hipEventCreate(&Q.ev_vbegin));
+      BNX_HIP(hipEventCreate(&Q.ev_vend));
+    }
+    if (nslots > slot_cap) {
+      BNX_HIP(slots.ensure(sizeof(SlotDev) * nslots));
+      BNX_HIP(h_slots.ensure(sizeof(SlotDev) * nslots));
+      BNX_HIP(live.ensure(sizeof(unsigned) * 2 * LIVE_SPREAD * nslots));
+      BNX_HIP(hipMemset(live.ptr, 0, sizeof(unsigned) * 2 * LIVE_SPREAD * nslots));
+      BNX_HIP(slot_list.ensure(sizeof(int) * nslots));
+      BNX_HIP(h_list.ensure(sizeof(int) * nslots));
+      BNX_HIP(live_out.ensure(sizeof(unsigned) * 2 * nslots));
+      BNX_HIP(h_live.ensure(sizeof(unsigned) * 2 * nslots));
+      slot_cap = nslots;
+    }
+    if (nchunks > chunk_cap) {
+      BNX_HIP(chunks.ensure(sizeof(RaygenChunk) * nchunks));
+      BNX_HIP(h_chunks.ensure(sizeof(RaygenChunk) * nchunks));
+      chunk_cap = nchunks;
+    }
+    BNX_HIP(h_count.ensure(64));
+    return BNX_OK;
+  }
+};
+
+struct GenTiming {
+  double trace_ms = 0.0, march_ms = 0.0;
+  int64_t trace_launches = 0, march_launches = 0, march_rays = 0;
+};
+
+// Enqueue one generation over queue `cur`, whose count (n on the host) is
+// already in counts[cur] on the device: trace, volume march, shade.  Shade
+// appends to queue 1 - cur, starting from count zero.
+int enqueue_generation(Device *dev, Wavefront &F, int cur, int n, bool surfaces, bool volumes, unsigned *live)
+{
+  Queues &Q = F.Q;
+  RayQueue in = Q.q(cur), out = Q.q(1 - cur);
+  int *counts = Q.counts.as<int>();
+  unsigned long long *st = Q.stats.as<unsigned long long>();
+  const SlotDev *slots = F.slots.as<SlotDev>();
+  TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
+  int rc;
+  // with neither surfaces nor volumes the trace kernel still writes the misses
+  const bool trace = surfaces || !volumes;
+  BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
+  if (trace && (rc = launch_trace_queue(dev, slots, tq, counts + cur, n, st)) != BNX_OK) return rc;
+  BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
+  if (volumes) {
+    BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
+    if ((rc = launch_volume_march(dev, slots, in, counts + cur, n, trace, st)) != BNX_OK) return rc;
+    BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
+  }
+  BNX_HIP(hipMemsetAsync(counts + (1 - cur), 0, sizeof(int), dev->stream));
+  return launch_shade(dev, slots, in, counts + cur, n, out, counts + (1 - cur), live, st);
+}
+
+// after the stream has synchronized
+int collect_timing(Wavefront &F, int n, bool surfaces, bool volumes, GenTiming &t)
+{
+  float ms = 0.f;
+  if (surfaces) {
+    BNX_HIP(hipEventElapsedTime(&ms, F.Q.ev_begin, F.Q.ev_end));
+    t.trace_ms += ms;
+    t.trace_launches++;
+  }
+  if (volumes) {
+    BNX_HIP(hipEventElapsedTime(&ms, F.Q.ev_vbegin, F.Q.ev_vend));
+    t.march_ms += ms;
+    t.march_launches++;
+    t.march_rays += n;
+  }
+  return BNX_OK;
+}
+
+int read_stats(Device *dev, Wavefront &F, unsigned long long out[STAT_COUNT])
+{
+  std::vector<unsigned long long> v((size_t)STAT_COUNT * STAT_SLOTS);
+  BNX_HIP(hipMemcpyAsync(v.data(), F.Q.stats.ptr, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost,
+                         dev->stream));
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  for (int k = 0; k < STAT_COUNT; k++) {
+    out[k] = 0;
+    for (int j = 0; j < STAT_SLOTS; j++) out[k] += v[(size_t)k * STAT_SLOTS + j];
+  }
+  return BNX_OK;
+}
+
+Wavefront &wavefront_for(Device *dev)
+{
+  if (!dev->queues) dev->queues = new Wavefront;
+  return *(Wavefront *)dev->queues;
+}
+
+int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, SlotDev &s)
+{
+  VolumeList vl{};
+  int rc = world_volumes(w, vl);
+  if (rc != BNX_OK) return rc;
+  s = SlotDev{};
+  s.scene = w->instances->scene;
+  s.vols = vl.vols;
+  s.num_vols = vl.count;
+  s.dir_lights = w->d_dir_lights.as<bnx_dir_light>();
+  s.num_dir_lights = (int)w->dir_lights.size();
+  s.rr = rr;
+  s.cam = cam;
+  s.W = f->w;
+  s.H = f->h;
+  s.accum_base = f->accum_id;
+  s.fb = frame_dev(f);
+  return BNX_OK;
+}
+
+} // namespace
+
+void wavefront_destroy(Device *dev)
+{
+  delete (Wavefront *)dev->queues;
+  dev->queues = nullptr;
+}
+
+int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp, bnx_render_stats *stats)
+{
+  auto t0 = std::chrono::steady_clock::now();
+  Device *dev = w->dev;
+  Wavefront &F = wavefront_for(dev);
+  const long long np = (long long)f->w * f->h;
+  // samples per batch: about 2M paths in flight
+  const int S = (int)std::max(1LL, std::min({(long long)spp, (2LL << 20) / std::max(1LL, np),
+                                             (long long)MAX_BATCH_SAMPLES}));
+  int rc;
+  if ((rc = F.ensure((size_t)(2 * np * S), 1, 1)) != BNX_OK) return rc;
+  int *counts = F.Q.counts.as<int>();
+  BNX_HIP(hipMemsetAsync(counts, 0, 128, dev->stream));
+  BNX_HIP(hipMemsetAsync(F.Q.stats.ptr, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
+  SlotDev *hs = F.h_slots.as<SlotDev>();
+  if ((rc = fill_slot(w, cam, rr, f, *hs)) != BNX_OK) return rc;
+  const bool surfaces = hs->scene.num_insts > 0;
+  const bool volumes = hs->num_vols > 0;
+  const bool traced = surfaces || !volumes;
+  RaygenChunk *hc = F.h_chunks.as<RaygenChunk>();
+  int *hn = F.h_count.as<int>();
+  int64_t rays = 0, gens = 0;
+  GenTiming tm;
+  for (int s0 = 0; s0 < spp; s0 += S) {
+    const int nS = std::min(S, spp - s0);
+    hs->accum_base = f->accum_id;
+    hc[0] = RaygenChunk{0, 0, 0, 0};
+    BNX_HIP(hipMemcpyAsync(F.slots.ptr, hs, sizeof(SlotDev), hipMemcpyHostToDevice, dev->stream));
+    BNX_HIP(hipMemcpyAsync(F.chunks.ptr, hc, sizeof(RaygenChunk), hipMemcpyHostToDevice, dev->stream));
+    int cur = 0;
+    int count = (int)(np * nS);
+    if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), 1, F.slots.as<SlotDev>(), 0, count, F.Q.q(cur))) !=
+        BNX_OK)
+      return rc;
+    hn[0] = count;
+    BNX_HIP(hipMemcpyAsync(counts + cur, hn, sizeof(int), hipMemcpyHostToDevice, dev->stream));
+    while (count > 0) {
+      if ((rc = enqueue_generation(dev, F, cur, count, surfaces, volumes, nullptr)) != BNX_OK) return rc;
+      BNX_HIP(hipMemcpyAsync(hn + 1, counts + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, dev->stream));
+      BNX_HIP(hipStreamSynchronize(dev->stream));
+      if ((rc = collect_timing(F, count, traced, volumes, tm)) != BNX_OK) return rc;
+      rays += count;
+      gens++;
+      count = hn[1];
+      cur = 1 - cur;
+    }
+    f->accum_id += nS;
+  }
+  unsigned long long host_st[STAT_COUNT];
+  if ((rc = read_stats(dev, F, host_st)) != BNX_OK) return rc;
+  BNX_HIP(hipGetLastError());
+  if (stats) {
+    stats->rays_traced = rays;
+    stats->shadow_rays = (int64_t)host_st[STAT_SHADOW];
+    stats->generations = gens;
+    stats->volume_samples = (int64_t)host_st[STAT_SAMPLES];
+    stats->ms_total = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+    stats->node_visits = (int64_t)host_st[STAT_NODES];
+    stats->tri_tests = (int64_t)host_st[STAT_TRIS];
+    stats->sphere_tests = (int64_t)host_st[STAT_SPHERES];
+    stats->trace_ms = tm.trace_ms;
+    stats->trace_launches = tm.trace_launches;
+    stats->march_ms = tm.march_ms;
+    stats->march_launches = tm.march_launches;
+    stats->mc_cells = (int64_t)host_st[STAT_CELLS];
+    stats->march_rays = tm.march_rays;
+  }
+  return BNX_OK;
+}
+
+// ------------------------------------------------------------------ server
+namespace {
+
+constexpr size_t FRAME_BYTES_PER_PIXEL = 80;   // 64 B of channels + 16 B read staging
+
+struct Request {
+  int64_t ticket = 0;
+  World *world = nullptr;
+  bnx_camera cam{};
+  bnx_renderer rr{};
+  int W = 0, H = 0, spp = 0;
+  int state = BNX_REQ_QUEUED;
+  int slot = -1;
+  Frame *frame = nullptr;
+  long long next_path = 0, total_paths = 0;
+  long long live_rays = 0, live_paths = 0;
+  double submit_ms = -1.0, admit_ms = -1.0, done_ms = -1.0;
+  bool surfaces = false, volumes = false;
+};
+
+} // namespace
+
+struct Server {
+  Device *dev = nullptr;
+  long long max_paths = 0;
+  size_t budget = 0;
+  int max_active = 0;
+  Wavefront F;
+  std::unordered_map<int64_t, Request *> reqs;
+  std::deque<Request *> queued;
+  std::vector<Request *> active;
+  std::vector<int> free_slots;
+  std::multimap<std::pair<int, int>, Frame *> pool;
+  size_t frame_bytes = 0;
+  int cur = 0, count = 0;
+  long long paths_in_flight = 0;
+  int64_t next_ticket = 1;
+  int64_t submitted = 0, admitted = 0, done = 0, released = 0, rays = 0, gens = 0, peak = 0;
+  double busy_ms = 0.0;
+  std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
+
+  double now_ms() const
+  {
+    return std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  }
+  ~Server()
+  {
+    (void)hipStreamSynchronize(dev->stream);
+    for (auto &kv : reqs) {
+      delete kv.second->frame;
+      delete kv.second;
+    }
+    for (auto &kv : pool) delete kv.second;
+  }
+
+  static size_t bytes_of(int w, int h) { return FRAME_BYTES_PER_PIXEL * (size_t)w * h; }
+
+  // a cleared frame of w x h from the pool or newly allocated; null if the
+  // budget does not allow it right now
+  int take_frame(int w, int h, Frame **out)
+  {
+    *out = nullptr;
+    auto it = pool.find({w, h});
+    if (it != pool.end()) {
+      *out = it->second;
+      pool.erase(it);
+    } else {
+      const size_t need = bytes_of(w, h);
+      while (frame_bytes + need > budget && !pool.empty()) {
+        auto v = pool.begin();
+        frame_bytes -= bytes_of(v->second->w, v->second->h);
+        delete v->second;
+        pool.erase(v);
+      }
+      if (frame_bytes + need > budget) return BNX_OK;
+      Frame *f = new Frame;
+      f->dev = dev;
+      f->w = w;
+      f->h = h;
+      const size_t np = (size_t)w * h;
+      hipError_t e;
+      if ((e = f->accum.ensure(32 * np)) != hipSuccess || (e = f->depth.ensure(4 * np)) != hipSuccess ||
+          (e = f->prim.ensure(4 * np)) != hipSuccess || (e = f->inst.ensure(4 * np)) != hipSuccess ||
+          (e = f->obj.ensure(4 * np)) != hipSuccess || (e = f->normal.ensure(16 * np)) != hipSuccess ||
+          (e = f->out.ensure(16 * np)) != hipSuccess) {
+        delete f;
+        set_error(std::string("bnx_server_step: frame allocation: ") + hipGetErrorString(e));
+        return BNX_ERR_OOM;
+      }
+      frame_bytes += need;
+      *out = f;
+    }
+    return frame_clear(*out);
+  }
+
+  int admit()
+  {
+    while (!queued.empty() && !free_slots.empty()) {
+      Request *r = queued.front();
+      Frame *f = nullptr;
+      int rc = take_frame(r->W, r->H, &f);
+      if (rc != BNX_OK) return rc;
+      if (!f) break;                       // admission is in arrival order
+      queued.pop_front();
+      r->frame = f;
+      r->slot = free_slots.back();
+      free_slots.pop_back();
+      SlotDev *hs = F.h_slots.as<SlotDev>() + r->slot;
+      if ((rc = fill_slot(r->world, r->cam, r->rr, f, *hs)) != BNX_OK) return rc;
+      hs->accum_base = 0;
+      BNX_HIP(hipMemcpyAsync(F.slots.as<SlotDev>() + r->slot, hs, sizeof(SlotDev), hipMemcpyHostToDevice,
+                             dev->stream));
+      r->surfaces = hs->scene.num_insts > 0;
+      r->volumes = hs->num_vols > 0;
+      r->state = BNX_REQ_ACTIVE;
+      r->admit_ms = now_ms();
+      active.push_back(r);
+      admitted++;
+    }
+    return BNX_OK;
+  }
+
+  int step(int64_t *done_tickets, int max_done, int *num_done)
+  {
+    const double ts = now_ms();
+    *num_done = 0;
+    int rc;
+    if ((rc = admit()) != BNX_OK) return rc;
+    if (active.empty()) return BNX_OK;
+    // top the queue up with primary rays, shortest remaining work first
+    std::vector<Request *> order;
+    for (Request *r : active)
+      if (r->next_path < r->total_paths) order.push_back(r);
+    std::stable_sort(order.begin(), order.end(), [](const Request *a, const Request *b) {
+      return a->total_paths - a->next_path < b->total_paths - b->next_path;
+    });
+    long long room = max_paths - paths_in_flight;
+    long long added = 0;
+    int nch = 0;
+    RaygenChunk *hc = F.h_chunks.as<RaygenChunk>();
+    for (Request *r : order) {
+      if (room <= 0) break;
+      const long long take = std::min(room, r->total_paths - r->next_path);
+      hc[nch++] = RaygenChunk{(long long)count + added, r->next_path, r->slot, 0};
+      r->next_path += take;
+      added += take;
+      room -= take;
+    }
+    const int n = count + (int)added;
+    if (n == 0) {
+      // every active request is still waiting for a first sample (cannot
+      // happen with room > 0) -- nothing to launch
+      busy_ms += now_ms() - ts;
+      return BNX_OK;
+    }
+    int *counts = F.Q.counts.as<int>();
+    int *hn = F.h_count.as<int>();
+    if (added > 0) {
+      BNX_HIP(hipMemcpyAsync(F.chunks.ptr, hc, sizeof(RaygenChunk) * nch, hipMemcpyHostToDevice, dev->stream));
+      if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), nch, F.slots.as<SlotDev>(), count, added,
+                              F.Q.q(cur))) != BNX_OK)
+        return rc;
+    }
+    hn[0] = n;
+    BNX_HIP(hipMemcpyAsync(counts + cur, hn, sizeof(int), hipMemcpyHostToDevice, dev->stream));
+    bool surfaces = false, volumes = false;
+    int *hl = F.h_list.as<int>();
+    const int nact = (int)active.size();
+    for (int k = 0; k < nact; k++) {
+      surfaces |= active[k]->surfaces;
+      volumes |= active[k]->volumes;
+      hl[k] = active[k]->slot;
+    }
+    BNX_HIP(hipMemcpyAsync(F.slot_list.ptr, hl, sizeof(int) * nact, hipMemcpyHostToDevice, dev->stream));
+    unsigned *live = F.live.as<unsigned>();
+    if ((rc = enqueue_generation(dev, F, cur, n, surfaces, volumes, live)) != BNX_OK) return rc;
+    if ((rc = launch_live_reduce(dev, live, F.slot_list.as<int>(), nact, F.live_out.as<unsigned>())) != BNX_OK)
+      return rc;
+    unsigned *hv = F.h_live.as<unsigned>();
+    BNX_HIP(hipMemcpyAsync(hv, F.live_out.ptr, sizeof(unsigned) * 2 * nact, hipMemcpyDeviceToHost, dev->stream));
+    BNX_HIP(hipMemcpyAsync(hn + 1, counts + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, dev->stream));
+    BNX_HIP(hipStreamSynchronize(dev->stream));
+    BNX_HIP(hipGetLastError());
+    rays += n;
+    gens++;
+    peak = std::max<int64_t>(peak, n);
+    count = hn[1];
+    cur = 1 - cur;
+    paths_in_flight = 0;
+    const double tnow = now_ms();
+    std::vector<Request *> still;
+    for (int k = 0; k < nact; k++) {
+      Request *r = active[k];
+      r->live_rays = hv[2 * k];
+      r->live_paths = hv[2 * k + 1];
+      paths_in_flight += r->live_paths;
+      if (r->next_path == r->total_paths && r->live_rays == 0) {
+        r->state = BNX_REQ_DONE;
+        r->done_ms = tnow;
+        r->frame->accum_id = r->spp;
+        free_slots.push_back(r->slot);
+        r->slot = -1;
+        done++;
+        if (*num_done < max_done && done_tickets) done_tickets[*num_done] = r->ticket;
+        (*num_done)++;
+      } else {
+        still.push_back(r);
+      }
+    }
+    active.swap(still);
+    busy_ms += now_ms() - ts;
+    return BNX_OK;
+  }
+};
+
+} // namespace bnx
+
+using namespace bnx;
+
+struct bnx_server_s : Server {};
+
+extern "C" {
+
+int bnx_server_create(bnx_device d, const bnx_server_config *cfg, bnx_server *out)
+{
+  GUARD_BEGIN
+  Device *dev = (Device *)d;
+  BNX_CHECK(dev && out, BNX_ERR_INVALID, "bnx_server_create: null argument");
+  bnx_server_config c{};
+  if (cfg) c = *cfg;
+  const long long paths = c.max_paths_in_flight > 0 ? c.max_paths_in_flight : (4LL << 20);
+  const int act = c.max_active > 0 ? c.max_active : 256;
+  BNX_CHECK(act <= 4096 && act <= MAX_SLOTS, BNX_ERR_INVALID, "bnx_server_create: max_active above 4096");
+  BNX_CHECK(paths <= (1LL << 28), BNX_ERR_INVALID, "bnx_server_create: max_paths_in_flight above 2^28");
+  bnx_server s = new bnx_server_s;
+  s->dev = dev;
+  s->max_paths = paths;
+  s->budget = c.frame_memory_budget > 0 ? c.frame_memory_budget : ((size_t)16 << 30);
+  s->max_active = act;
+  int rc = s->F.ensure((size_t)(2 * paths), act, act);
+  if (rc != BNX_OK) {
+    delete s;
+    return rc;
+  }
+  BNX_HIP(hipMemsetAsync(s->F.Q.counts.ptr, 0, 128, dev->stream));
+  BNX_HIP(hipMemsetAsync(s->F.Q.stats.ptr, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
+  BNX_HIP(hipStreamSynchronize(dev->stream));
+  for (int k = act - 1; k >= 0; k--) s->free_slots.push_back(k);
+  *out = s;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_server_destroy(bnx_server s)
+{
+  BNX_CHECK(s, BNX_ERR_INVALID, "bnx_server_destroy: null server");
+  delete s;
+  return BNX_OK;
+}
+
+int bnx_server_submit(bnx_server s, const bnx_request *req, int64_t *ticket)
+{
+  GUARD_BEGIN
+  BNX_CHECK(s && req && ticket, BNX_ERR_INVALID, "bnx_server_submit: null argument");
+  World *w = (World *)req->world;
+  BNX_CHECK(w && w->dev == s->dev, BNX_ERR_INVALID, "bnx_server_submit: world missing or of another device");
+  BNX_CHECK(w->instances && w->instances->built, BNX_ERR_INVALID, "bnx_server_submit: world not built");
+  BNX_CHECK(req->width > 0 && req->height > 0 && (long long)req->width * req->height <= (1LL << 26),
+            BNX_ERR_INVALID, "bnx_server_submit: bad resolution");
+  BNX_CHECK(req->spp >= 1 && req->spp <= MAX_BATCH_SAMPLES, BNX_ERR_INVALID, "bnx_server_submit: spp not in 1..4096");
+  BNX_CHECK(req->camera.type == BNX_CAMERA_PERSPECTIVE && req->camera.aperture_radius == 0.f, BNX_ERR_UNSUPPORTED,
+            "bnx_server_submit: only the pinhole perspective camera is provided");
+  BNX_CHECK(Server::bytes_of(req->width, req->height) <= s->budget, BNX_ERR_INVALID,
+            "bnx_server_submit: frame larger than the server's frame memory budget");
+  Request *r = new Request;
+  r->ticket = s->next_ticket++;
+  r->world = w;
+  r->cam = req->camera;
+  r->rr = req->renderer;
+  r->W = req->width;
+  r->H = req->height;
+  r->spp = req->spp;
+  r->total_paths = (long long)req->width * req->height * req->spp;
+  r->submit_ms = s->now_ms();
+  s->reqs[r->ticket] = r;
+  s->queued.push_back(r);
+  s->submitted++;
+  *ticket = r->ticket;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_server_step(bnx_server s, int64_t *done_tickets, int max_done, int *num_done)
+{
+  GUARD_BEGIN
+  BNX_CHECK(s && num_done && max_done >= 0, BNX_ERR_INVALID, "bnx_server_step: bad argument");
+  return s->step(done_tickets, max_done, num_done);
+  GUARD_END
+}
+
+int bnx_server_poll(bnx_server s, int64_t ticket, bnx_request_status *st)
+{
+  BNX_CHECK(s && st, BNX_ERR_INVALID, "bnx_server_poll: null argument");
+  auto it = s->reqs.find(ticket);
+  BNX_CHECK(it != s->reqs.end(), BNX_ERR_INVALID, "bnx_server_poll: unknown ticket");
+  const Request *r = it->second;
+  *st = bnx_request_status{};
+  st->state = r->state;
+  st->submit_ms = r->submit_ms;
+  st->admit_ms = r->admit_ms;
+  st->done_ms = r->done_ms;
+  st->paths_started = r->next_path;
+  st->paths_total = r->total_paths;
+  return BNX_OK;
+}
+
+int bnx_server_read(bnx_server s, int64_t ticket, int channel, void *host)
+{
+  GUARD_BEGIN
+  BNX_CHECK(s && host, BNX_ERR_INVALID, "bnx_server_read: null argument");
+  auto it = s->reqs.find(ticket);
+  BNX_CHECK(it != s->reqs.end(), BNX_ERR_INVALID, "bnx_server_read: unknown ticket");
+  BNX_CHECK(it->second->state == BNX_REQ_DONE, BNX_ERR_INVALID, "bnx_server_read: request not done");
+  return frame_read(it->second->frame, channel, host);
+  GUARD_END
+}
+
+int bnx_server_release(bnx_server s, int64_t ticket)
+{
+  GUARD_BEGIN
+  BNX_CHECK(s, BNX_ERR_INVALID, "bnx_server_release: null server");
+  auto it = s->reqs.find(ticket);
+  BNX_CHECK(it != s->reqs.end(), BNX_ERR_INVALID, "bnx_server_release: unknown ticket");
+  Request *r = it->second;
+  BNX_CHECK(r->state == BNX_REQ_DONE, BNX_ERR_INVALID, "bnx_server_release: request not done");
+  s->pool.insert({{r->W, r->H}, r->frame});
+  s->reqs.erase(it);
+  delete r;
+  s->released++;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_server_get_stats(bnx_server s, bnx_server_stats *st)
+{
+  GUARD_BEGIN
+  BNX_CHECK(s && st, BNX_ERR_INVALID, "bnx_server_get_stats: null argument");
+  unsigned long long host_st[STAT_COUNT];
+  int rc = read_stats(s->dev, s->F, host_st);
+  if (rc != BNX_OK) return rc;
+  *st = bnx_server_stats{};
+  st->submitted = s->submitted;
+  st->admitted = s->admitted;
+  st->done = s->done;
+  st->released = s->released;
+  st->rays_traced = s->rays;
+  st->shadow_rays = (int64_t)host_st[STAT_SHADOW];
+  st->volume_samples = (int64_t)host_st[STAT_SAMPLES];
+  st->generations = s->gens;
+  st->peak_queue = s->peak;
+  st->active = (int32_t)s->active.size();
+  st->queued = (int32_t)s->queued.size();
+  st->frame_bytes = s->frame_bytes;
+  st->busy_ms = s->busy_ms;
+  return BNX_OK;
+  GUARD_END
+}
+
+} // extern "C"

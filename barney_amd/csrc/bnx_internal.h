@@ -3,6 +3,8 @@
 
 #include <hip/hip_runtime.h>
 
+#include <new>
+#include <stdexcept>
 #include <string>
 #include <vector>
 
@@ -29,6 +31,19 @@ void set_error(const std::string &msg);
       return code;                                                                    \
     }                                                                                 \
   } while (0)
+
+// exception guard of every C-ABI entry point
+#define GUARD_BEGIN try {
+#define GUARD_END                                                                     \
+  }                                                                                   \
+  catch (const std::bad_alloc &) {                                                    \
+    ::bnx::set_error("host allocation failed");                                       \
+    return BNX_ERR_OOM;                                                               \
+  }                                                                                   \
+  catch (const std::exception &e) {                                                   \
+    ::bnx::set_error(e.what());                                                       \
+    return BNX_ERR_INVALID;                                                           \
+  }
 
 // device memory owned by a C++ object
 struct DevMem {
@@ -146,6 +161,8 @@ struct TraceQueue {
   float *hit_v;         // v
 };
 int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
-int launch_trace_queue(Group *g, const TraceQueue &q, const int *d_count, int max_n, unsigned long long *counters);
+struct SlotDev;
+int launch_trace_queue(Device *dev, const SlotDev *d_slots, const TraceQueue &q, const int *d_count, int max_n,
+                       unsigned long long *counters);
 
 } // namespace bnx

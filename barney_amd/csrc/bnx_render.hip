@@ -8,9 +8,12 @@
 //
 // Layout: the ray queue is structure-of-arrays (16-byte records per
 // field), so each wave reads and writes whole cache lines; several samples
-// per pixel are in flight in one queue (slot = sample * pixels + pixel at
-// generation 0).  Shade appends its shadow and continuation rays with one
-// atomic per wave (ballot + popcount prefix).  The frame accumulates in
+// per pixel, and several requests, are in flight in one queue (at
+// generation 0 a request's batch occupies begin + sample * pixels + pixel).
+// Every ray carries its request slot in its flags; per-request state
+// (scene, lights, camera, renderer, frame) lives in a slot table in HBM.
+// Shade appends its shadow and continuation rays with one atomic per block
+// (ballot + popcount prefix inside each wave, LDS scan across waves).  The frame accumulates in
 // 2^-32 fixed point with 64-bit integer atomics, so a pixel's sum does not
 // depend on the order in which rays of different samples or requests
 // arrive.
@@ -35,12 +38,29 @@ constexpr float ONE_OVER_TWO_PI = 1.f / TWO_PI;
 constexpr float ONE_OVER_FOUR_PI = 1.f / FOUR_PI;
 
 // ---------------------------------------------------------------- raygen
-__global__ void k_raygen(bnx_camera cam, bnx_renderer rr, int W, int H, int accum_base, int S, RayQueue q)
+// One thread per (request, sample, pixel) of the batch.  The chunk table
+// lists each request's first queue index in ascending order; a thread finds
+// its request by binary search over it.
+__global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDev *slots, long long q_begin,
+                         long long total, RayQueue q)
 {
-  int idx = blockIdx.x * blockDim.x + threadIdx.x;
-  int np = W * H;
-  if (idx >= np * S) return;
-  int s = idx / np, pix = idx - s * np;
+  const long long idx = q_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (idx >= q_begin + total) return;
+  int lo = 0, hi = num_chunks - 1;
+  while (lo < hi) {
+    int mid = (lo + hi + 1) >> 1;
+    if (chunks[mid].begin <= idx) lo = mid;
+    else hi = mid - 1;
+  }
+  const RaygenChunk ch = chunks[lo];
+  const SlotDev &SL = slots[ch.slot];
+  const bnx_camera &cam = SL.cam;
+  const bnx_renderer &rr = SL.rr;
+  const int W = SL.W, H = SL.H;
+  const int np = W * H;
+  const long long path = ch.first + (idx - ch.begin);
+  const int s = (int)(path / np), pix = (int)(path - (long long)s * np);
+  const int accum_base = SL.accum_base;
   int ix = pix % W, iy = pix / W;
   int accumID = accum_base + s;
   Random2 rnd((uint32_t)(ix + W * accumID), (uint32_t)(iy + H * accumID));
@@ -68,10 +88,10 @@ __global__ void k_raygen(bnx_camera cam, bnx_renderer rr, int W, int H, int accu
     }
   }
   q.org[idx] = make_float4(org.x, org.y, org.z, tmax);
-  q.dir[idx] = make_float4(dir.x, dir.y, dir.z, __uint_as_float((uint32_t)s << 16));
+  q.dir[idx] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(mk_flags(false, s, ch.slot)));
   q.rng[idx] = seed;
   uint32_t one = f2h(1.f);
-  q.ps[idx] = make_uint4(one | (one << 16), one, (uint32_t)pix, __float_as_uint(0.f));
+  q.ps[idx] = make_uint4(one | (one << 16), one, (uint32_t)pix, __float_as_uint(0.f));   // generation 0
 }
 
 // ----------------------------------------------------------------- shade
@@ -146,7 +166,7 @@ BNX_D bool sample_dir(LightSample &ls, const bnx_dir_light *lights, int n, v3 N,
   return weights[i] != 0.f;
 }
 
-BNX_D bool sample_lights(LightSample &ls, const ShadeParams &P, v3 N, Random2 &random, bool &needs_mis,
+BNX_D bool sample_lights(LightSample &ls, const SlotDev &P, v3 N, Random2 &random, bool &needs_mis,
                          bool &is_dir)
 {
   needs_mis = false;
@@ -272,7 +292,7 @@ BNX_D SurfHit eval_volume_hit(float t, float4 rgb, v3 rorg, v3 rdir)
 }
 
 // primary-ray miss color (generateRays.cu:205-216): background or gradient
-BNX_D float4 miss_color(const ShadeParams &P, int pixel)
+BNX_D float4 miss_color(const SlotDev &P, int pixel)
 {
   if (P.rr.bg_color[3] >= 0.f) return make_float4(P.rr.bg_color[0], P.rr.bg_color[1], P.rr.bg_color[2], P.rr.bg_color[3]);
   int iy = pixel / P.W;
@@ -290,18 +310,21 @@ struct OutRay {
   uint4 ps;
 };
 
-BNX_D void pack_ps(uint4 &ps, v3 tp, int ndiff, int pixel, float mis)
+BNX_D void pack_ps(uint4 &ps, v3 tp, int ndiff, int gen, int pixel, float mis)
 {
   ps.x = f2h(tp.x) | (f2h(tp.y) << 16);
-  ps.y = f2h(tp.z) | ((uint32_t)ndiff << 16);
+  ps.y = f2h(tp.z) | ((uint32_t)ndiff << 16) | ((uint32_t)gen << 24);
   ps.z = (uint32_t)pixel;
   ps.w = __float_as_uint(mis);
 }
 
 constexpr int SHADE_BLOCK = 1024;
 
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P, RayQueue in, const int *in_count,
-                                                       RayQueue out, int *out_count, FrameDev fb)
+// Each ray carries its own generation, so one launch can shade rays of
+// different requests at different depths of their paths.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *slots, RayQueue in, const int *in_count,
+                                                       RayQueue out, int *out_count, unsigned int *live,
+                                                       unsigned long long *stats)
 {
   __shared__ int wave_base[SHADE_BLOCK / 64];
   __shared__ int block_base;
@@ -309,20 +332,25 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
   const int n = *in_count;
   const bool valid = i < n;
   bool w_sh = false, w_ray = false;
+  int slot_id = -1;
   OutRay o_sh, o_ray;
   if (valid) {
-    const int gen = P.generation;
     float4 ro = in.org[i], rd = in.dir[i];
     uint64_t rng = in.rng[i];
     uint4 ps = in.ps[i];
     int4 hit = in.hit[i];
     float hv = in.hit_v[i];
-    uint32_t flags = __float_as_uint(rd.w);
-    const bool is_shadow = (flags & 1u) != 0;
-    const int sample = (int)(flags >> 16);
+    const uint32_t flags = __float_as_uint(rd.w);
+    const bool is_shadow = (flags & FLAG_SHADOW) != 0;
+    const int sample = flag_sample(flags);
+    slot_id = flag_slot(flags);
+    const SlotDev &P = slots[slot_id];
+    const SceneDev &S = P.scene;
+    const FrameDev &fb = P.fb;
     const int accumID = P.accum_base + sample;
     v3 tp = v3{h2f(ps.x), h2f(ps.x >> 16), h2f(ps.y)};
-    int ndiff = (int)(ps.y >> 16);
+    int ndiff = (int)((ps.y >> 16) & 0xffu);
+    const int gen = (int)(ps.y >> 24);
     const int pixel = (int)ps.z;
     const float mis = __uint_as_float(ps.w);
     v3 rorg = xyz(ro), rdir = xyz(rd);
@@ -390,7 +418,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
             sh_tmax = ls.distance * (1.f - 2.f * off_eps);
             o_sh.org = make_float4(so.x, so.y, so.z, sh_tmax);
             o_sh.dir = make_float4(ls.direction.x, ls.direction.y, ls.direction.z,
-                                   __uint_as_float(1u | ((uint32_t)sample << 16)));
+                                   __uint_as_float(mk_flags(true, sample, slot_id)));
             o_sh.rng = rng;
             rng_next_h(rng, (uint64_t)__float_as_uint(tmax));
             float smis = 1.f;
@@ -400,7 +428,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
               smis = pl / ((pl + psd) + 1e-10f);
               if (smis < 1e-5f) sh_tmax = -1.f;
             }
-            pack_ps(o_sh.ps, quant_half(tp_sr), 0, pixel, smis);
+            pack_ps(o_sh.ps, quant_half(tp_sr), 0, gen + 1, pixel, smis);
           }
         }
         if (gen >= MAX_PATH_DEPTH || sh.phase) {
@@ -425,6 +453,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
           float spdf = local.z * ONE_OVER_PI;
           if (!(spdf > 0.f) || spdf <= 1e-6f) {
             // invalid scatter: the ray goes on unchanged (shadeRays.cu:774-775)
+            ps.y = (ps.y & 0x00ffffffu) | ((uint32_t)(gen + 1) << 24);
           } else if (ndiff >= MAX_DIFFUSE_BOUNCES) {
             ray_tmax = -1.f;
           } else {
@@ -444,11 +473,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
             } else {
               nmis = 1.f;
             }
-            pack_ps(ps, tp, ndiff, pixel, nmis);
+            pack_ps(ps, tp, ndiff, gen + 1, pixel, nmis);
           }
           if (ray_tmax > 0.f) {
             o_ray.org = make_float4(rorg.x, rorg.y, rorg.z, ray_tmax);
-            o_ray.dir = make_float4(rdir.x, rdir.y, rdir.z, __uint_as_float((uint32_t)sample << 16));
+            o_ray.dir = make_float4(rdir.x, rdir.y, rdir.z, __uint_as_float(mk_flags(false, sample, slot_id)));
             o_ray.rng = rng;
             o_ray.ps = ps;
           }
@@ -457,6 +486,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
     }
     w_sh = sh_tmax > 0.f;
     w_ray = ray_tmax > 0.f;
+    // renderer's generation limit: rays of generation max_generations are
+    // never traced, so they are not written
+    if (P.rr.max_generations > 0 && gen + 1 >= P.rr.max_generations) w_sh = w_ray = false;
 
     // ------------------------------------------------ accumulate (shadeRays.cu:993-1040)
     const float clampMax = 10.f * (float)(1 + accumID);
@@ -523,7 +555,42 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(SceneDev S, ShadeParams P
     out.rng[slot] = o_ray.rng;
     out.ps[slot] = o_ray.ps;
   }
-  if (lane_id() == 0) stat_add(P.stats, STAT_SHADOW, (uint32_t)__popcll(b1));
+  if (lane_id() == 0) stat_add(stats, STAT_SHADOW, (uint32_t)__popcll(b1));
+  if (live) {
+    // per-request counts of the rays written: one atomic per distinct
+    // request in the wave (usually one, since a wave's input rays come from
+    // one run of the queue)
+    uint64_t todo = b1 | b2;
+    const int spread = (int)(blockIdx.x & (LIVE_SPREAD - 1));
+    while (todo) {
+      const int leader = __ffsll((unsigned long long)todo) - 1;
+      const int ls = __shfl(slot_id, leader);
+      const uint64_t same = __ballot(slot_id == ls) & todo;
+      if (lane_id() == leader) {
+        atomicAdd(live + ((size_t)ls * 2 + 0) * LIVE_SPREAD + spread, (unsigned)(__popcll(b1 & same) + __popcll(b2 & same)));
+        atomicAdd(live + ((size_t)ls * 2 + 1) * LIVE_SPREAD + spread, (unsigned)__popcll(b2 & same));
+      }
+      todo &= ~same;
+    }
+  }
+}
+
+__global__ void k_live_reduce(unsigned int *live, const int *slot_list, int n, unsigned int *out)
+{
+  const int k = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
+  const int lane = threadIdx.x & 63;
+  if (k >= n) return;
+  unsigned int *p = live + (size_t)slot_list[k] * 2 * LIVE_SPREAD;
+  unsigned int v = lane < 2 * LIVE_SPREAD ? p[lane] : 0u;
+  if (lane < 2 * LIVE_SPREAD) p[lane] = 0u;
+  // lanes 0..31: all rays, lanes 32..63: path rays
+  unsigned int a = lane < LIVE_SPREAD ? v : 0u, b = lane < LIVE_SPREAD ? 0u : v;
+  a = wave_sum(a);
+  b = wave_sum(b);
+  if (lane == 0) {
+    out[2 * k] = a;
+    out[2 * k + 1] = b;
+  }
 }
 
 // ----------------------------------------------------------- frame ops
@@ -617,20 +684,28 @@ int frame_read(Frame *f, int channel, void *host)
   return BNX_OK;
 }
 
-int launch_raygen(Device *dev, const bnx_camera &cam, const bnx_renderer &rr, int W, int H, int accum_base, int S,
-                  const RayQueue &q)
+int launch_raygen(Device *dev, const RaygenChunk *chunks, int num_chunks, const SlotDev *slots, long long q_begin,
+                  long long total, const RayQueue &q)
 {
-  long total = (long)W * H * S;
-  if (total > 0) k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(cam, rr, W, H, accum_base, S, q);
+  if (total > 0 && num_chunks > 0)
+    k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(chunks, num_chunks, slots, q_begin, total, q);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }
 
-int launch_shade(Device *dev, const SceneDev &S, const ShadeParams &P, const RayQueue &in, const int *in_count,
-                 int max_in, const RayQueue &out, int *out_count, const FrameDev &fb)
+int launch_shade(Device *dev, const SlotDev *slots, const RayQueue &in, const int *in_count, int max_in,
+                 const RayQueue &out, int *out_count, unsigned int *live, unsigned long long *stats)
 {
   if (max_in > 0)
-    k_shade<<<nblk(max_in, SHADE_BLOCK), SHADE_BLOCK, 0, dev->stream>>>(S, P, in, in_count, out, out_count, fb);
+    k_shade<<<nblk(max_in, SHADE_BLOCK), SHADE_BLOCK, 0, dev->stream>>>(slots, in, in_count, out, out_count, live,
+                                                                        stats);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_live_reduce(Device *dev, unsigned int *live, const int *slot_list, int n, unsigned int *out)
+{
+  if (n > 0) k_live_reduce<<<nblk(n, 4), 256, 0, dev->stream>>>(live, slot_list, n, out);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -19,6 +19,7 @@
 //   * ties in t go to the lower (instance, geometry, primitive) ID, and a
 //     box whose entry distance equals the current best is still visited.
 #include "bnx_internal.h"
+#include "bnx_render.h"
 
 namespace bnx {
 namespace {
@@ -258,7 +259,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
 
 // one ray per thread; blocks past the queue count exit at once.
 // counters: statistics array (STAT_NODES, STAT_TRIS, STAT_SPHERES)
-__global__ __launch_bounds__(TB) void k_trace_queue(SceneDev S, TraceQueue q, const int *d_count,
+__global__ __launch_bounds__(TB) void k_trace_queue(const SlotDev *slots, TraceQueue q, const int *d_count,
                                                     unsigned long long *counters)
 {
   __shared__ uint32_t ring[SHORT * TB];
@@ -271,8 +272,11 @@ __global__ __launch_bounds__(TB) void k_trace_queue(SceneDev S, TraceQueue q, co
   if (i < n) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
-    bool shadow = (__float_as_uint(d.w) & 1u) != 0;
-    HitRec h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, st, cnt);
+    const uint32_t flags = __float_as_uint(d.w);
+    const bool shadow = (flags & FLAG_SHADOW) != 0;
+    const SceneDev &S = slots[flag_slot(flags)].scene;
+    HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
+    if (S.num_insts > 0) h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, st, cnt);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
@@ -299,10 +303,10 @@ int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, ui
   return BNX_OK;
 }
 
-int launch_trace_queue(Group *g, const TraceQueue &q, const int *d_count, int max_n, unsigned long long *counters)
+int launch_trace_queue(Device *dev, const SlotDev *slots, const TraceQueue &q, const int *d_count, int max_n,
+                       unsigned long long *counters)
 {
-  Device *dev = g->dev;
-  k_trace_queue<<<grid_for(max_n), TB, 0, dev->stream>>>(g->scene, q, d_count, counters);
+  k_trace_queue<<<grid_for(max_n), TB, 0, dev->stream>>>(slots, q, d_count, counters);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

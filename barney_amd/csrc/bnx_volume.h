@@ -2,6 +2,7 @@
 #pragma once
 
 #include "bnx_internal.h"
+#include "bnx_render.h"
 
 namespace bnx {
 
@@ -43,7 +44,9 @@ int volume_build(Volume *v);
 int volume_rebuild_majorants(Volume *v);
 // surfaces: false when the world has no surface instances (the trace kernel
 // was skipped and the hit records are not written yet)
-int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
+// upload the world's volume table (if changed) and return it
+int world_volumes(World *w, VolumeList &vl);
+int launch_volume_march(Device *dev, const SlotDev *d_slots, const RayQueue &q, const int *d_count, int max_n,
                         bool surfaces, unsigned long long *stats);
 
 // ---------------------------------------------------------------- device

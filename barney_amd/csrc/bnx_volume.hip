@@ -63,8 +63,8 @@ __global__ void k_majorants(VolumeDev v, const float2 *ranges, float *maj)
 // march every queued ray through the world's volumes, one ray per thread.
 // With no surface instances the trace kernel is skipped and every ray
 // enters here as a surface miss.
-__global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q, const int *d_count, int surfaces,
-                                                      unsigned long long *stats)
+__global__ __launch_bounds__(256) void k_volume_march(const SlotDev *slots, RayQueue q, const int *d_count,
+                                                      int surfaces, unsigned long long *stats)
 {
   const int n = *d_count;
   if ((int)(blockIdx.x * 256) >= n) return;
@@ -74,7 +74,10 @@ __global__ __launch_bounds__(256) void k_volume_march(VolumeList vl, RayQueue q,
     float4 o = q.org[i];
     float4 d = q.dir[i];
     int4 hit = surfaces ? q.hit[i] : make_int4(-1, -1, -1, 0);
-    const bool shadow = (__float_as_uint(d.w) & 1u) != 0;
+    const uint32_t flags = __float_as_uint(d.w);
+    const bool shadow = (flags & FLAG_SHADOW) != 0;
+    const SlotDev &SL = slots[flag_slot(flags)];
+    const VolumeList vl{SL.vols, SL.num_vols};
     if (!(shadow && hit.x >= 0)) {                 // an occluded shadow ray is done
       const v3 org = xyz(o), dir = fix_dir(xyz(d));
       float tcur = o.w;
@@ -140,11 +143,11 @@ int volume_rebuild_majorants(Volume *v)
   return BNX_OK;
 }
 
-int launch_volume_march(Device *dev, const VolumeList &vl, const RayQueue &q, const int *d_count, int max_n,
+int launch_volume_march(Device *dev, const SlotDev *slots, const RayQueue &q, const int *d_count, int max_n,
                         bool surfaces, unsigned long long *stats)
 {
-  if (vl.count == 0 || max_n == 0) return BNX_OK;
-  k_volume_march<<<nblk(max_n, 256), 256, 0, dev->stream>>>(vl, q, d_count, surfaces ? 1 : 0, stats);
+  if (max_n == 0) return BNX_OK;
+  k_volume_march<<<nblk(max_n, 256), 256, 0, dev->stream>>>(slots, q, d_count, surfaces ? 1 : 0, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

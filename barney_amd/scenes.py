@@ -152,7 +152,8 @@ def config1(width: int = 512, height: int = 512, spp: int = 1, num_spheres: int 
     s.dir_lights.append((d.astype(np.float32), 1.0, np.ones(3, dtype=np.float32)))
     s.ambient = 1.0
     s.bg_color = (0.0, 0.0, 0.0, 1.0)
-    s.camera = dict(position=(0.0, 0.0, 5.0), direction=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=40.0)
+    s.camera = dict(position=(0.0, 0.0, 5.0), direction=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=40.0,
+                    target=(0.0, 0.0, 0.0))
     return s
 
 
@@ -248,5 +249,53 @@ def config2(n: int = 256, width: int = 1024, height: int = 1024, spp: int = 16,
     d = -np.ones(3, dtype=np.float64) / math.sqrt(3.0)
     s.dir_lights.append((d.astype(np.float32), 2.5, np.ones(3, dtype=np.float32)))
     s.camera = dict(position=(1.9, 1.4, 2.6), direction=(0.5 - 1.9, 0.5 - 1.4, 0.5 - 2.6), up=(0.0, 1.0, 0.0),
-                    fovy=35.0)
+                    fovy=35.0, target=(0.5, 0.5, 0.5))
     return s
+
+
+# --------------------------------------------------------------------------
+# serving stream
+# --------------------------------------------------------------------------
+RESOLUTIONS = (256, 512, 1024, 2048)
+SPPS = (1, 4, 16, 64)
+
+
+def orbit_camera(sc, u_az: float, u_el: float) -> dict:
+    """camera on the scene's orbit: the default camera's distance to its
+    target, azimuth U[0, 2pi), elevation U(-30, 60) degrees, looking at the
+    target"""
+    tgt = np.asarray(sc.camera["target"], np.float64)
+    r = float(np.linalg.norm(np.asarray(sc.camera["position"], np.float64) - tgt))
+    az = 2.0 * math.pi * u_az
+    el = math.radians(-30.0 + 90.0 * u_el)
+    p = tgt + r * np.array([math.cos(el) * math.sin(az), math.sin(el), math.cos(el) * math.cos(az)])
+    return dict(position=tuple(p.tolist()), direction=tuple((tgt - p).tolist()), up=(0.0, 1.0, 0.0),
+                fovy=sc.camera["fovy"], target=tuple(tgt.tolist()))
+
+
+@dataclass
+class ServeRequest:
+    arrival_s: float
+    scene: int            # index into the resident scenes
+    width: int
+    height: int
+    spp: int
+    camera: dict
+
+
+def serving_stream(n: int, qps: float, scene_list, seed: int = SEEDS["c5"], resolutions=RESOLUTIONS,
+                   spps=SPPS) -> list:
+    """n requests with Poisson arrivals at `qps` (inter-arrival Exp(qps) by
+    inversion of a PCG32 uniform), scene, square resolution and spp each
+    uniform over the given choices, camera on the scene's seeded orbit.
+    Request i draws from stream (5 << 32) | i."""
+    u = uniform_table(seed, 5, n, 6)
+    gaps = -np.log1p(-u[:, 0]) / qps
+    t = np.cumsum(gaps) - gaps[0]
+    out = []
+    for i in range(n):
+        k = min(int(u[i, 1] * len(scene_list)), len(scene_list) - 1)
+        r = resolutions[min(int(u[i, 2] * len(resolutions)), len(resolutions) - 1)]
+        spp = spps[min(int(u[i, 3] * len(spps)), len(spps) - 1)]
+        out.append(ServeRequest(float(t[i]), k, r, r, spp, orbit_camera(scene_list[k], u[i, 4], u[i, 5])))
+    return out

@@ -303,6 +303,75 @@ int bnx_frame_destroy(bnx_frame fb);
 int bnx_render(bnx_world world, const bnx_camera *camera, const bnx_renderer *renderer,
                bnx_frame fb, int spp, bnx_render_stats *stats);
 
+/* ------------------------------------------------------------- request server
+ * Continuous batching of render requests on one device.  barney renders one
+ * frame per bnRender call (Context::renderTiles, native/Context.cpp:139-182,
+ * one raygen + trace/shade loop per call); the server keeps worlds and BVHs
+ * resident and runs ONE wavefront for all requests in flight: every
+ * generation traces, marches and shades the rays of all admitted requests in
+ * one launch per kernel, and tops the shared queue up with primary rays of
+ * admitted requests (shortest remaining work first).  Requests are admitted
+ * in arrival order as request slots and frame memory free up.  A request's
+ * result is the same as bnx_render of the same world, camera, renderer and
+ * spp into a cleared frame (up to the order of fixed-point additions, which
+ * does not change the sum).  The worlds named by requests must outlive them.
+ * Calls on one server are not thread-safe. */
+typedef struct bnx_server_s *bnx_server;
+
+typedef struct bnx_server_config {
+  size_t  frame_memory_budget;      /* bytes of request frames held at once (0: 16 GiB) */
+  int64_t max_paths_in_flight;      /* paths in the shared queue (0: 4M); the queue
+                                       holds twice as many rays                        */
+  int32_t max_active;               /* requests holding a slot at once (0: 256, <= 4096) */
+  int32_t _pad;
+} bnx_server_config;
+
+typedef struct bnx_request {
+  bnx_world    world;
+  bnx_camera   camera;
+  int32_t      width, height;
+  bnx_renderer renderer;
+  int32_t      spp;                 /* 1 .. 4096 */
+  int32_t      _pad;
+} bnx_request;
+
+enum { BNX_REQ_QUEUED = 0, BNX_REQ_ACTIVE = 1, BNX_REQ_DONE = 2 };
+
+typedef struct bnx_request_status {
+  int32_t state;                    /* BNX_REQ_*                                       */
+  int32_t _pad;
+  double  submit_ms, admit_ms, done_ms;   /* host clock, ms since server creation     */
+  int64_t paths_started;            /* primary rays generated so far                   */
+  int64_t paths_total;              /* width * height * spp                            */
+} bnx_request_status;
+
+typedef struct bnx_server_stats {
+  int64_t submitted, admitted, done, released;
+  int64_t rays_traced;              /* sum over generations of rays entering trace     */
+  int64_t shadow_rays;
+  int64_t volume_samples;
+  int64_t generations;              /* generation launches that had rays               */
+  int64_t peak_queue;               /* largest generation, rays                        */
+  int32_t active, queued;
+  size_t  frame_bytes;              /* frame memory held (admitted + pooled)           */
+  double  busy_ms;                  /* wall time spent inside bnx_server_step          */
+} bnx_server_stats;
+
+int bnx_server_create(bnx_device dev, const bnx_server_config *config, bnx_server *out);
+int bnx_server_destroy(bnx_server server);
+/* queue a request; BNX_ERR_INVALID for bad sizes or a world of another device */
+int bnx_server_submit(bnx_server server, const bnx_request *request, int64_t *ticket);
+/* one generation for all requests in flight (admission, queue top-up, trace,
+ * march, shade); writes up to max_done tickets that completed in it.  With
+ * nothing queued or active it returns at once with *num_done = 0. */
+int bnx_server_step(bnx_server server, int64_t *done_tickets, int max_done, int *num_done);
+int bnx_server_poll(bnx_server server, int64_t ticket, bnx_request_status *status);
+/* read a channel (BNX_FB_*) of a completed request, as bnx_frame_read */
+int bnx_server_read(bnx_server server, int64_t ticket, int channel, void *host);
+/* drop a completed request; its frame memory goes back to the server's pool */
+int bnx_server_release(bnx_server server, int64_t ticket);
+int bnx_server_get_stats(bnx_server server, bnx_server_stats *stats);
+
 #ifdef __cplusplus
 } /* extern "C" */
 #endif
@@ -315,6 +384,10 @@ static_assert(sizeof(bnx_camera) == 60, "bnx_camera layout");
 static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
 static_assert(sizeof(bnx_triangles_desc) == 72, "bnx_triangles_desc layout");
 static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
+static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
+static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
+static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
+static_assert(sizeof(bnx_server_stats) == 96, "bnx_server_stats layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
 static_assert(sizeof(bnx_render_stats) == 112, "bnx_render_stats layout");
 #endif
