@@ -102,14 +102,17 @@ class ServerStats(C.Structure):
     _fields_ = [("submitted", C.c_int64), ("admitted", C.c_int64), ("done", C.c_int64), ("released", C.c_int64),
                 ("rays_traced", C.c_int64), ("shadow_rays", C.c_int64), ("volume_samples", C.c_int64),
                 ("generations", C.c_int64), ("peak_queue", C.c_int64), ("active", C.c_int32),
-                ("queued", C.c_int32), ("frame_bytes", C.c_size_t), ("busy_ms", C.c_double)]
+                ("queued", C.c_int32), ("frame_bytes", C.c_size_t), ("busy_ms", C.c_double),
+                ("trace_ms", C.c_double), ("trace_launches", C.c_int64), ("march_ms", C.c_double),
+                ("march_launches", C.c_int64), ("march_rays", C.c_int64), ("node_visits", C.c_int64),
+                ("tri_tests", C.c_int64), ("sphere_tests", C.c_int64), ("mc_cells", C.c_int64)]
 
 
 BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE = 0, 1, 2
 
 assert C.sizeof(Material) == 32 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
 assert C.sizeof(ServerConfig) == 24 and C.sizeof(Request) == 136 and C.sizeof(RequestStatus) == 48
-assert C.sizeof(ServerStats) == 96 and Request.renderer.offset == 76 and Request.spp.offset == 124
+assert C.sizeof(ServerStats) == 168 and Request.renderer.offset == 76 and Request.spp.offset == 124
 
 # 64-byte BVH node: child boxes lo0,hi0,lo1,hi1 then ref[2], cnt[2]
 NODE_DTYPE = np.dtype([("lo0", "<f4", 3), ("hi0", "<f4", 3), ("lo1", "<f4", 3), ("hi1", "<f4", 3),

@@ -207,10 +207,10 @@ int build_tlas(Group *g)
         gd.color_from_attr = gm->mat.color_from_attribute;
         gd.num_prims = gm->num_prims;
         gd.color = make_float4(gm->mat.color[0], gm->mat.color[1], gm->mat.color[2], gm->mat.color[3]);
-        gd.vertices = gm->vertices.as<float>();
-        gd.indices = gm->indices.as<int>();
-        gd.colors = gm->colors.ptr ? gm->colors.as<float>() : nullptr;
-        gd.spheres = gm->spheres.as<float4>();
+        BNX_SET(gd.vertices, gm->vertices.as<float>());
+        BNX_SET(gd.indices, gm->indices.as<int>());
+        BNX_SET(gd.colors, gm->colors.ptr ? gm->colors.as<float>() : nullptr);
+        BNX_SET(gd.spheres, gm->spheres.as<float4>());
         geoms.push_back(gd);
       }
       g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.max_depth);
@@ -247,9 +247,9 @@ int build_tlas(Group *g)
   std::vector<BlasDev> blas(blas_groups.size());
   for (size_t k = 0; k < blas_groups.size(); k++) {
     Group *bg = blas_groups[k];
-    blas[k].nodes = bg->bvh.nodes.as<Node>();
-    blas[k].prims = bg->prims.as<float4>();
-    blas[k].refs = bg->refs.as<int2>();
+    BNX_SET(blas[k].nodes, bg->bvh.nodes.as<Node>());
+    BNX_SET(blas[k].prims, bg->prims.as<float4>());
+    BNX_SET(blas[k].refs, bg->refs.as<int2>());
     blas[k].type = bg->kind == GROUP_TRIANGLES ? BLAS_TRIANGLES : BLAS_SPHERES;
     blas[k].num_prims = bg->bvh.num_prims;
   }
@@ -266,11 +266,11 @@ int build_tlas(Group *g)
     set_error("bnx_group_build: BVH deeper than the traversal stack");
     return BNX_ERR_UNSUPPORTED;
   }
-  g->scene.tlas_nodes = g->bvh.nodes.as<Node>();
-  g->scene.tlas_order = g->bvh.order.as<int>();
-  g->scene.insts = g->d_insts.as<InstDev>();
-  g->scene.blas = g->d_blas.as<BlasDev>();
-  g->scene.geoms = g->d_geoms.as<GeomDev>();
+  BNX_SET(g->scene.tlas_nodes, g->bvh.nodes.as<Node>());
+  BNX_SET(g->scene.tlas_order, g->bvh.order.as<int>());
+  BNX_SET(g->scene.insts, g->d_insts.as<InstDev>());
+  BNX_SET(g->scene.blas, g->d_blas.as<BlasDev>());
+  BNX_SET(g->scene.geoms, g->d_geoms.as<GeomDev>());
   g->scene.num_insts = n;
   g->built = true;
   return BNX_OK;
@@ -294,7 +294,7 @@ int world_volumes(World *w, VolumeList &vl)
     BNX_HIP(hipStreamSynchronize(w->dev->stream));
     w->volumes_dirty = false;
   }
-  vl.vols = w->d_volumes.as<VolumeDev>();
+  BNX_SET(vl.vols, w->d_volumes.as<VolumeDev>());
   vl.count = (int)w->volumes.size();
   return BNX_OK;
 }
@@ -631,7 +631,7 @@ static int set_tf(Volume *v, const bnx_transfer_function *tf)
   BNX_CHECK(tf->domain[1] > tf->domain[0], BNX_ERR_INVALID, "transfer function domain is empty");
   int rc = upload(v->dev, v->tf, (const float4 *)tf->values, (size_t)tf->num_values);
   if (rc != BNX_OK) return rc;
-  v->dd.tf = v->tf.as<float4>();
+  BNX_SET(v->dd.tf, v->tf.as<float4>());
   v->dd.ntf = tf->num_values;
   v->dd.dom0 = tf->domain[0];
   v->dd.dom1 = tf->domain[1];
@@ -657,7 +657,7 @@ int bnx_volume_create_structured(bnx_device d, const bnx_structured_desc *f, con
     delete v;
     return rc;
   }
-  dd.scalars = v->scalars.as<float>();
+  BNX_SET(dd.scalars, v->scalars.as<float>());
   const int c = f->cells_per_mc > 0 ? f->cells_per_mc : 8;
   dd.cells_per_mc = c;
   for (int k = 0; k < 3; k++) {

@@ -14,6 +14,21 @@
 
 #define BNX_D __device__ __forceinline__
 
+// Pointer fields of the device tables (scene, BLAS, geometry, volume,
+// request-slot and frame records) point into global memory.  In the device
+// pass they carry the global address space, so loads, stores and atomics
+// through them and through pointers derived from them are global-memory
+// instructions; a pointer loaded from memory as a plain pointer would be
+// generic, and flat accesses through it also wait on the LDS counter.  The
+// host pass sees plain pointers of the same size and assigns them with
+// BNX_SET.
+#if defined(__HIP_DEVICE_COMPILE__)
+#define BNX_G __attribute__((address_space(1)))
+#else
+#define BNX_G
+#endif
+#define BNX_SET(field, value) ((field) = (decltype(field))(value))
+
 namespace bnx {
 
 // ------------------------------------------------------------------ vec3
@@ -126,9 +141,9 @@ BNX_D uint32_t enc_child(int ref, int cnt)
 enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1 };
 
 struct BlasDev {
-  const Node *nodes;
-  const float4 *prims;    // triangles: 3 per slot (v0|prim, e1|geom, e2); spheres: (c, r)
-  const int2 *refs;       // spheres: (geom, prim) per slot
+  const Node BNX_G *nodes;
+  const float4 BNX_G *prims;    // triangles: 3 per slot (v0|prim, e1|geom, e2); spheres: (c, r)
+  const int2 BNX_G *refs;  // spheres: (geom, prim) per slot
   int type;
   int num_prims;
 };
@@ -149,18 +164,18 @@ struct GeomDev {
   int color_from_attr;
   int num_prims;
   float4 color;           // constant material color
-  const float *vertices;  // triangles: float3
-  const int *indices;     //            int3
-  const float *colors;    // per-prim float3 or null
-  const float4 *spheres;  // spheres: (center, radius)
+  const float BNX_G *vertices;   // triangles: float3
+  const int BNX_G *indices;      //            int3
+  const float BNX_G *colors;     // per-prim float3 or null
+  const float4 BNX_G *spheres;   // spheres: (center, radius)
 };
 
 struct SceneDev {
-  const Node *tlas_nodes;
-  const int *tlas_order;
-  const InstDev *insts;
-  const BlasDev *blas;
-  const GeomDev *geoms;
+  const Node BNX_G *tlas_nodes;
+  const int BNX_G *tlas_order;
+  const InstDev BNX_G *insts;
+  const BlasDev BNX_G *blas;
+  const GeomDev BNX_G *geoms;
   int num_insts;
 };
 

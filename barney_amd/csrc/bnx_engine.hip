@@ -1,8 +1,16 @@
 // bnx_engine.hip -- host side of the wavefront: the per-request frame loop
 // (bnx_render) and the request server (bnx_server_*).  Both drive the same
 // generation: trace, volume march and shade of one shared ray queue, whose
-// rays may belong to any number of requests (each ray names its request
-// slot; per-request state lives in a slot table in HBM).
+// rays may belong to any number of requests.
+//
+// Queue layout: the queue of a generation is a list of segments, each a run
+// of rays of one request (Segment, bnx_render.h).  Shade writes the rays of
+// request s into s's own region of the other queue, sized for the worst case
+// (each path leaves at most one shadow ray and one continuation ray), and
+// counts them per request; the next generation's segments are those regions
+// plus the runs of new primary rays that top the queue up.  Every block of
+// every launch lies inside one segment, so a request's parameters are
+// block-uniform and live in scalar registers.
 //
 // Replaces Context::renderTiles (native/Context.cpp:139-182) and the
 // RayQueue count read-back (render/RayQueue.cpp:47-57).  The server has no
@@ -45,11 +53,13 @@ struct Pinned {
 // queues, tables and staging of one wavefront
 struct Wavefront {
   Queues Q;
-  DevMem slots, chunks, live, slot_list, live_out;
-  Pinned h_slots, h_chunks, h_list, h_live, h_count;
-  int slot_cap = 0, chunk_cap = 0;
+  DevMem slots, chunks, segs, blk256, blk1024, out_base, out_cnt;
+  Pinned h_slots, h_chunks, h_segs, h_out_base, h_out_cnt;
+  int slot_cap = 0, chunk_cap = 0, seg_cap = 0;
+  size_t blk_cap = 0;
 
-  int ensure(size_t rays, int nslots, int nchunks)
+  // rays per queue, request slots, raygen chunks, segments per generation
+  int ensure(size_t rays, int nslots, int nchunks, int nsegs)
   {
     int rc = Q.ensure(rays);
     if (rc != BNX_OK) return rc;
@@ -62,12 +72,10 @@ struct Wavefront {
     if (nslots > slot_cap) {
       BNX_HIP(slots.ensure(sizeof(SlotDev) * nslots));
       BNX_HIP(h_slots.ensure(sizeof(SlotDev) * nslots));
-      BNX_HIP(live.ensure(sizeof(unsigned) * 2 * LIVE_SPREAD * nslots));
-      BNX_HIP(hipMemset(live.ptr, 0, sizeof(unsigned) * 2 * LIVE_SPREAD * nslots));
-      BNX_HIP(slot_list.ensure(sizeof(int) * nslots));
-      BNX_HIP(h_list.ensure(sizeof(int) * nslots));
-      BNX_HIP(live_out.ensure(sizeof(unsigned) * 2 * nslots));
-      BNX_HIP(h_live.ensure(sizeof(unsigned) * 2 * nslots));
+      BNX_HIP(out_base.ensure(sizeof(int) * nslots));
+      BNX_HIP(h_out_base.ensure(sizeof(int) * nslots));
+      BNX_HIP(out_cnt.ensure(sizeof(int) * 2 * nslots));
+      BNX_HIP(h_out_cnt.ensure(sizeof(int) * 2 * nslots));
       slot_cap = nslots;
     }
     if (nchunks > chunk_cap) {
@@ -75,8 +83,32 @@ struct Wavefront {
       BNX_HIP(h_chunks.ensure(sizeof(RaygenChunk) * nchunks));
       chunk_cap = nchunks;
     }
-    BNX_HIP(h_count.ensure(64));
+    if (nsegs > seg_cap) {
+      BNX_HIP(segs.ensure(sizeof(Segment) * nsegs));
+      BNX_HIP(h_segs.ensure(sizeof(Segment) * nsegs));
+      seg_cap = nsegs;
+    }
+    // every segment adds at most one partial block
+    const size_t blocks = rays / 256 + (size_t)nsegs + 1;
+    if (blocks > blk_cap) {
+      BNX_HIP(blk256.ensure(sizeof(int) * blocks));
+      BNX_HIP(blk1024.ensure(sizeof(int) * blocks));
+      blk_cap = blocks;
+    }
     return BNX_OK;
+  }
+};
+
+// segments of one generation, written straight into pinned staging
+struct SegList {
+  Segment *v;
+  int n = 0, blocks256 = 0, blocks1024 = 0;
+  void add(int slot, int begin, int count)
+  {
+    if (count <= 0) return;
+    v[n++] = Segment{slot, begin, count, blocks256, blocks1024, {0, 0, 0}};
+    blocks256 += (count + 255) / 256;
+    blocks1024 += (count + 1023) / 1024;
   }
 };
 
@@ -85,30 +117,34 @@ struct GenTiming {
   int64_t trace_launches = 0, march_launches = 0, march_rays = 0;
 };
 
-// Enqueue one generation over queue `cur`, whose count (n on the host) is
-// already in counts[cur] on the device: trace, volume march, shade.  Shade
-// appends to queue 1 - cur, starting from count zero.
-int enqueue_generation(Device *dev, Wavefront &F, int cur, int n, bool surfaces, bool volumes, unsigned *live)
+// Enqueue one generation over queue `cur`: upload the segments, build the
+// block maps, trace, volume march, shade into queue 1 - cur.  The caller
+// has uploaded the output region bases and cleared the output counters.
+int enqueue_generation(Device *dev, Wavefront &F, int cur, const SegList &sl, bool surfaces, bool volumes)
 {
   Queues &Q = F.Q;
   RayQueue in = Q.q(cur), out = Q.q(1 - cur);
-  int *counts = Q.counts.as<int>();
   unsigned long long *st = Q.stats.as<unsigned long long>();
   const SlotDev *slots = F.slots.as<SlotDev>();
-  TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
   int rc;
+  BNX_HIP(hipMemcpyAsync(F.segs.ptr, sl.v, sizeof(Segment) * sl.n, hipMemcpyHostToDevice, dev->stream));
+  if ((rc = launch_segment_blocks(dev, F.segs.as<Segment>(), sl.n, sl.blocks256, sl.blocks1024,
+                                  F.blk256.as<int>(), F.blk1024.as<int>())) != BNX_OK)
+    return rc;
+  const SegView sv{F.segs.as<Segment>(), F.blk256.as<int>(), F.blk1024.as<int>()};
+  TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
   // with neither surfaces nor volumes the trace kernel still writes the misses
   const bool trace = surfaces || !volumes;
   BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
-  if (trace && (rc = launch_trace_queue(dev, slots, tq, counts + cur, n, st)) != BNX_OK) return rc;
+  if (trace && (rc = launch_trace_queue(dev, slots, sv, sl.blocks256, tq, st)) != BNX_OK) return rc;
   BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
   if (volumes) {
     BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
-    if ((rc = launch_volume_march(dev, slots, in, counts + cur, n, trace, st)) != BNX_OK) return rc;
+    if ((rc = launch_volume_march(dev, slots, sv, sl.blocks256, in, trace, st)) != BNX_OK) return rc;
     BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
   }
-  BNX_HIP(hipMemsetAsync(counts + (1 - cur), 0, sizeof(int), dev->stream));
-  return launch_shade(dev, slots, in, counts + cur, n, out, counts + (1 - cur), live, st);
+  const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
+  return launch_shade(dev, slots, sv, sl.blocks1024, in, out, so, st);
 }
 
 // after the stream has synchronized
@@ -157,7 +193,7 @@ int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f,
   s.scene = w->instances->scene;
   s.vols = vl.vols;
   s.num_vols = vl.count;
-  s.dir_lights = w->d_dir_lights.as<bnx_dir_light>();
+  BNX_SET(s.dir_lights, w->d_dir_lights.ptr);
   s.num_dir_lights = (int)w->dir_lights.size();
   s.rr = rr;
   s.cam = cam;
@@ -185,10 +221,11 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
   // samples per batch: about 2M paths in flight
   const int S = (int)std::max(1LL, std::min({(long long)spp, (2LL << 20) / std::max(1LL, np),
                                              (long long)MAX_BATCH_SAMPLES}));
+  BNX_CHECK(2 * np * S < (1LL << 31), BNX_ERR_UNSUPPORTED, "bnx_render: frame too large for one batch");
   int rc;
-  if ((rc = F.ensure((size_t)(2 * np * S), 1, 1)) != BNX_OK) return rc;
-  int *counts = F.Q.counts.as<int>();
-  BNX_HIP(hipMemsetAsync(counts, 0, 128, dev->stream));
+  // one batch of paths leaves at most twice as many rays (one shadow ray and
+  // one continuation ray per path) in the output region
+  if ((rc = F.ensure((size_t)(2 * np * S), 1, 1, 1)) != BNX_OK) return rc;
   BNX_HIP(hipMemsetAsync(F.Q.stats.ptr, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
   SlotDev *hs = F.h_slots.as<SlotDev>();
   if ((rc = fill_slot(w, cam, rr, f, *hs)) != BNX_OK) return rc;
@@ -196,7 +233,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
   const bool volumes = hs->num_vols > 0;
   const bool traced = surfaces || !volumes;
   RaygenChunk *hc = F.h_chunks.as<RaygenChunk>();
-  int *hn = F.h_count.as<int>();
+  int *hb = F.h_out_base.as<int>();
+  int *hcnt = F.h_out_cnt.as<int>();
+  hb[0] = 0;                           // the single request's region starts the queue
+  BNX_HIP(hipMemcpyAsync(F.out_base.ptr, hb, sizeof(int), hipMemcpyHostToDevice, dev->stream));
   int64_t rays = 0, gens = 0;
   GenTiming tm;
   for (int s0 = 0; s0 < spp; s0 += S) {
@@ -210,16 +250,17 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), 1, F.slots.as<SlotDev>(), 0, count, F.Q.q(cur))) !=
         BNX_OK)
       return rc;
-    hn[0] = count;
-    BNX_HIP(hipMemcpyAsync(counts + cur, hn, sizeof(int), hipMemcpyHostToDevice, dev->stream));
     while (count > 0) {
-      if ((rc = enqueue_generation(dev, F, cur, count, surfaces, volumes, nullptr)) != BNX_OK) return rc;
-      BNX_HIP(hipMemcpyAsync(hn + 1, counts + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, dev->stream));
+      SegList sl{F.h_segs.as<Segment>()};
+      sl.add(0, 0, count);
+      BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, 2 * sizeof(int), dev->stream));
+      if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes)) != BNX_OK) return rc;
+      BNX_HIP(hipMemcpyAsync(hcnt, F.out_cnt.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, dev->stream));
       BNX_HIP(hipStreamSynchronize(dev->stream));
       if ((rc = collect_timing(F, count, traced, volumes, tm)) != BNX_OK) return rc;
       rays += count;
       gens++;
-      count = hn[1];
+      count = hcnt[0];
       cur = 1 - cur;
     }
     f->accum_id += nS;
@@ -261,7 +302,9 @@ struct Request {
   int slot = -1;
   Frame *frame = nullptr;
   long long next_path = 0, total_paths = 0;
-  long long live_rays = 0, live_paths = 0;
+  int region = 0;                  // first queue index of its rays in the current queue
+  int live_rays = 0, live_paths = 0;
+  int added = 0;                   // primary rays added in this step
   double submit_ms = -1.0, admit_ms = -1.0, done_ms = -1.0;
   bool surfaces = false, volumes = false;
 };
@@ -280,11 +323,12 @@ struct Server {
   std::vector<int> free_slots;
   std::multimap<std::pair<int, int>, Frame *> pool;
   size_t frame_bytes = 0;
-  int cur = 0, count = 0;
-  long long paths_in_flight = 0;
+  int cur = 0;
+  int in_end = 0;                  // end of the output regions laid out in the current queue
   int64_t next_ticket = 1;
   int64_t submitted = 0, admitted = 0, done = 0, released = 0, rays = 0, gens = 0, peak = 0;
   double busy_ms = 0.0;
+  GenTiming tm;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
 
   double now_ms() const
@@ -302,7 +346,6 @@ struct Server {
   }
 
   static size_t bytes_of(int w, int h) { return FRAME_BYTES_PER_PIXEL * (size_t)w * h; }
-
   // a cleared frame of w x h from the pool or newly allocated; null if the
   // budget does not allow it right now
   int take_frame(int w, int h, Frame **out)
@@ -368,6 +411,11 @@ struct Server {
     return BNX_OK;
   }
 
+  // One generation for every request in flight.  The queue holds, per
+  // active request, the rays its last shade wrote (region, live_rays); new
+  // primary rays go after the regions, shortest remaining work first, while
+  // the paths in flight stay within max_paths.  The next output regions are
+  // laid out in the other queue, 2 rays per path.
   int step(int64_t *done_tickets, int max_done, int *num_done)
   {
     const double ts = now_ms();
@@ -375,73 +423,77 @@ struct Server {
     int rc;
     if ((rc = admit()) != BNX_OK) return rc;
     if (active.empty()) return BNX_OK;
-    // top the queue up with primary rays, shortest remaining work first
+    long long in_flight = 0;
+    for (Request *r : active) {
+      in_flight += r->live_paths;
+      r->added = 0;
+    }
     std::vector<Request *> order;
     for (Request *r : active)
       if (r->next_path < r->total_paths) order.push_back(r);
     std::stable_sort(order.begin(), order.end(), [](const Request *a, const Request *b) {
       return a->total_paths - a->next_path < b->total_paths - b->next_path;
     });
-    long long room = max_paths - paths_in_flight;
-    long long added = 0;
+    long long room = max_paths - in_flight;
     int nch = 0;
+    long long added = 0;
     RaygenChunk *hc = F.h_chunks.as<RaygenChunk>();
+    // this generation's segments: old regions, then the new primaries
+    SegList sl{F.h_segs.as<Segment>()};
+    for (Request *r : active) sl.add(r->slot, r->region, r->live_rays);
     for (Request *r : order) {
       if (room <= 0) break;
       const long long take = std::min(room, r->total_paths - r->next_path);
-      hc[nch++] = RaygenChunk{(long long)count + added, r->next_path, r->slot, 0};
+      hc[nch++] = RaygenChunk{(long long)in_end + added, r->next_path, r->slot, 0};
+      sl.add(r->slot, (int)(in_end + added), (int)take);
+      r->added = (int)take;
       r->next_path += take;
       added += take;
       room -= take;
     }
-    const int n = count + (int)added;
+    long long n = 0;
+    for (int k = 0; k < sl.n; k++) n += sl.v[k].count;
     if (n == 0) {
-      // every active request is still waiting for a first sample (cannot
-      // happen with room > 0) -- nothing to launch
       busy_ms += now_ms() - ts;
       return BNX_OK;
     }
-    int *counts = F.Q.counts.as<int>();
-    int *hn = F.h_count.as<int>();
-    if (added > 0) {
+    // output regions in the other queue
+    int *hb = F.h_out_base.as<int>();
+    long long out_end = 0;
+    bool surfaces = false, volumes = false;
+    int max_slot = 0;
+    for (Request *r : active) {
+      hb[r->slot] = (int)out_end;
+      out_end += 2LL * (r->live_paths + r->added);
+      surfaces |= r->surfaces;
+      volumes |= r->volumes;
+      max_slot = std::max(max_slot, r->slot + 1);
+    }
+    BNX_HIP(hipMemcpyAsync(F.out_base.ptr, hb, sizeof(int) * max_slot, hipMemcpyHostToDevice, dev->stream));
+    BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, sizeof(int) * 2 * max_slot, dev->stream));
+    if (nch > 0) {
       BNX_HIP(hipMemcpyAsync(F.chunks.ptr, hc, sizeof(RaygenChunk) * nch, hipMemcpyHostToDevice, dev->stream));
-      if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), nch, F.slots.as<SlotDev>(), count, added,
+      if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), nch, F.slots.as<SlotDev>(), in_end, added,
                               F.Q.q(cur))) != BNX_OK)
         return rc;
     }
-    hn[0] = n;
-    BNX_HIP(hipMemcpyAsync(counts + cur, hn, sizeof(int), hipMemcpyHostToDevice, dev->stream));
-    bool surfaces = false, volumes = false;
-    int *hl = F.h_list.as<int>();
-    const int nact = (int)active.size();
-    for (int k = 0; k < nact; k++) {
-      surfaces |= active[k]->surfaces;
-      volumes |= active[k]->volumes;
-      hl[k] = active[k]->slot;
-    }
-    BNX_HIP(hipMemcpyAsync(F.slot_list.ptr, hl, sizeof(int) * nact, hipMemcpyHostToDevice, dev->stream));
-    unsigned *live = F.live.as<unsigned>();
-    if ((rc = enqueue_generation(dev, F, cur, n, surfaces, volumes, live)) != BNX_OK) return rc;
-    if ((rc = launch_live_reduce(dev, live, F.slot_list.as<int>(), nact, F.live_out.as<unsigned>())) != BNX_OK)
-      return rc;
-    unsigned *hv = F.h_live.as<unsigned>();
-    BNX_HIP(hipMemcpyAsync(hv, F.live_out.ptr, sizeof(unsigned) * 2 * nact, hipMemcpyDeviceToHost, dev->stream));
-    BNX_HIP(hipMemcpyAsync(hn + 1, counts + (1 - cur), sizeof(int), hipMemcpyDeviceToHost, dev->stream));
+    if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes)) != BNX_OK) return rc;
+    int *hcnt = F.h_out_cnt.as<int>();
+    BNX_HIP(hipMemcpyAsync(hcnt, F.out_cnt.ptr, sizeof(int) * 2 * max_slot, hipMemcpyDeviceToHost, dev->stream));
     BNX_HIP(hipStreamSynchronize(dev->stream));
     BNX_HIP(hipGetLastError());
+    if ((rc = collect_timing(F, (int)n, surfaces || !volumes, volumes, tm)) != BNX_OK) return rc;
     rays += n;
     gens++;
     peak = std::max<int64_t>(peak, n);
-    count = hn[1];
     cur = 1 - cur;
-    paths_in_flight = 0;
+    in_end = (int)out_end;
     const double tnow = now_ms();
     std::vector<Request *> still;
-    for (int k = 0; k < nact; k++) {
-      Request *r = active[k];
-      r->live_rays = hv[2 * k];
-      r->live_paths = hv[2 * k + 1];
-      paths_in_flight += r->live_paths;
+    for (Request *r : active) {
+      r->region = hb[r->slot];
+      r->live_rays = hcnt[2 * r->slot];
+      r->live_paths = hcnt[2 * r->slot + 1];
       if (r->next_path == r->total_paths && r->live_rays == 0) {
         r->state = BNX_REQ_DONE;
         r->done_ms = tnow;
@@ -478,19 +530,19 @@ int bnx_server_create(bnx_device d, const bnx_server_config *cfg, bnx_server *ou
   if (cfg) c = *cfg;
   const long long paths = c.max_paths_in_flight > 0 ? c.max_paths_in_flight : (4LL << 20);
   const int act = c.max_active > 0 ? c.max_active : 256;
-  BNX_CHECK(act <= 4096 && act <= MAX_SLOTS, BNX_ERR_INVALID, "bnx_server_create: max_active above 4096");
+  BNX_CHECK(act <= 4096, BNX_ERR_INVALID, "bnx_server_create: max_active above 4096");
   BNX_CHECK(paths <= (1LL << 28), BNX_ERR_INVALID, "bnx_server_create: max_paths_in_flight above 2^28");
   bnx_server s = new bnx_server_s;
   s->dev = dev;
   s->max_paths = paths;
   s->budget = c.frame_memory_budget > 0 ? c.frame_memory_budget : ((size_t)16 << 30);
   s->max_active = act;
-  int rc = s->F.ensure((size_t)(2 * paths), act, act);
+  // queue: output regions (2 rays per path in flight) plus new primaries
+  int rc = s->F.ensure((size_t)(3 * paths), act, act, 2 * act + 1);
   if (rc != BNX_OK) {
     delete s;
     return rc;
   }
-  BNX_HIP(hipMemsetAsync(s->F.Q.counts.ptr, 0, 128, dev->stream));
   BNX_HIP(hipMemsetAsync(s->F.Q.stats.ptr, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
   BNX_HIP(hipStreamSynchronize(dev->stream));
   for (int k = act - 1; k >= 0; k--) s->free_slots.push_back(k);
@@ -610,6 +662,15 @@ int bnx_server_get_stats(bnx_server s, bnx_server_stats *st)
   st->queued = (int32_t)s->queued.size();
   st->frame_bytes = s->frame_bytes;
   st->busy_ms = s->busy_ms;
+  st->trace_ms = s->tm.trace_ms;
+  st->trace_launches = s->tm.trace_launches;
+  st->march_ms = s->tm.march_ms;
+  st->march_launches = s->tm.march_launches;
+  st->march_rays = s->tm.march_rays;
+  st->node_visits = (int64_t)host_st[STAT_NODES];
+  st->tri_tests = (int64_t)host_st[STAT_TRIS];
+  st->sphere_tests = (int64_t)host_st[STAT_SPHERES];
+  st->mc_cells = (int64_t)host_st[STAT_CELLS];
   return BNX_OK;
   GUARD_END
 }

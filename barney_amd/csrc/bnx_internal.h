@@ -162,7 +162,9 @@ struct TraceQueue {
 };
 int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
 struct SlotDev;
-int launch_trace_queue(Device *dev, const SlotDev *d_slots, const TraceQueue &q, const int *d_count, int max_n,
+struct SegView;
+// trace every ray of the generation's segments (blocks256 blocks of 256 threads)
+int launch_trace_queue(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, const TraceQueue &q,
                        unsigned long long *counters);
 
 } // namespace bnx

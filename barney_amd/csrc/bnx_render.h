@@ -1,4 +1,5 @@
-// bnx_render.h -- wavefront queue, shading parameters and frame views.
+// bnx_render.h -- wavefront queue, request slots, queue segments and frame
+// views shared by the raygen, trace, volume-march and shade kernels.
 #pragma once
 
 #include "bnx_internal.h"
@@ -18,41 +19,72 @@ struct RayQueue {
 };
 
 // ray flags in dir.w: bit 0 shadow ray, bits 4..15 sample index inside the
-// current batch of its request, bits 16..31 request slot
+// current batch of its request
 constexpr uint32_t FLAG_SHADOW = 1u;
 constexpr int MAX_BATCH_SAMPLES = 4096;
-constexpr int MAX_SLOTS = 65536;
-BNX_D uint32_t mk_flags(bool shadow, int sample, int slot)
+BNX_D uint32_t mk_flags(bool shadow, int sample)
 {
-  return (shadow ? FLAG_SHADOW : 0u) | ((uint32_t)sample << 4) | ((uint32_t)slot << 16);
+  return (shadow ? FLAG_SHADOW : 0u) | ((uint32_t)sample << 4);
 }
 BNX_D int flag_sample(uint32_t f) { return (int)((f >> 4) & 0xfffu); }
-BNX_D int flag_slot(uint32_t f) { return (int)(f >> 16); }
 
 struct FrameDev {
-  unsigned long long *accum;   // 4 x int64 per pixel, 2^-32 fixed point
-  int *depth;                  // ordered-int float
-  int *prim, *inst, *obj;
-  float4 *normal;
+  unsigned long long BNX_G *accum;   // 4 x int64 per pixel, 2^-32 fixed point
+  int BNX_G *depth;                  // ordered-int float
+  int BNX_G *prim, *inst, *obj;
+  float4 BNX_G *normal;
 };
 
 struct VolumeDev;
 // One request in flight: everything its rays need to be generated, traced,
 // marched, shaded and accumulated.  Rays of different requests (different
 // worlds, cameras, resolutions, frames) share one queue and one launch per
-// kernel; each ray finds its request through the slot index in its flags.
+// kernel; the queue is cut into segments of one request each (Segment).
 struct SlotDev {
   SceneDev scene;
-  const VolumeDev *vols;       // world volumes (volume k: hit geom VOLUME_GEOM_BASE + k)
+  const VolumeDev BNX_G *vols; // world volumes (volume k: hit geom VOLUME_GEOM_BASE + k)
   int num_vols;
   int num_dir_lights;
-  const bnx_dir_light *dir_lights;
+  const bnx_dir_light BNX_G *dir_lights;
   bnx_renderer rr;
   bnx_camera cam;
   int W, H;
   int accum_base;              // accumID of batch sample 0
   int pad;
   FrameDev fb;
+};
+
+// A run of queue entries that all belong to one request slot.  Launches are
+// laid out so that every block lies inside one segment: a block reads its
+// segment (and through it its request's scene, volumes, lights, camera and
+// frame) with scalar loads, and the request's parameters sit in scalar
+// registers for the whole block.  Segment k covers blocks
+// [block256, block256 + ceil(count / 256)) of a 256-thread launch and
+// [block1024, ...) of a 1024-thread launch.
+struct Segment {
+  int slot;
+  int begin;                   // first queue index
+  int count;                   // rays
+  int block256;
+  int block1024;
+  int pad[3];
+};
+static_assert(sizeof(Segment) == 32, "segment");
+
+// kernel-side view of one generation's input: segments and the block ->
+// segment maps of both launch widths
+struct SegView {
+  const Segment *segs;
+  const int *blk256;           // segment of each 256-thread block
+  const int *blk1024;          // segment of each 1024-thread block
+};
+
+// shade output of one generation: request slot s appends to its own region
+// of the output queue, starting at base[s]; cnt[2s] counts the rays written
+// and cnt[2s + 1] the path (non-shadow) rays among them
+struct SlotOut {
+  const int *base;
+  int *cnt;
 };
 
 // A run of primary rays of one request: queue entries [begin, next chunk's
@@ -65,27 +97,21 @@ struct RaygenChunk {
   int pad;
 };
 
-// per-request count of rays written by shade, spread over LIVE_SPREAD
-// counters per slot so that one generation's atomics do not serialize;
-// [slot][0][.]: all rays, [slot][1][.]: path (non-shadow) rays
-constexpr int LIVE_SPREAD = 32;
-
 inline FrameDev frame_dev(Frame *f)
 {
   FrameDev d;
-  d.accum = f->accum.as<unsigned long long>();
-  d.depth = f->depth.as<int>();
-  d.prim = f->prim.as<int>();
-  d.inst = f->inst.as<int>();
-  d.obj = f->obj.as<int>();
-  d.normal = f->normal.as<float4>();
+  BNX_SET(d.accum, f->accum.ptr);
+  BNX_SET(d.depth, f->depth.ptr);
+  BNX_SET(d.prim, f->prim.ptr);
+  BNX_SET(d.inst, f->inst.ptr);
+  BNX_SET(d.obj, f->obj.ptr);
+  BNX_SET(d.normal, f->normal.ptr);
   return d;
 }
 
-// per-device ray queues (two, ping-pong) and counters
+// ray queues (two, ping-pong) and statistics counters
 struct Queues {
   DevMem org[2], dir[2], rng[2], ps[2], hit[2], hit_v[2], vcol[2];
-  DevMem counts;      // [0], [1]: queue counts
   DevMem stats;       // STAT_COUNT x STAT_SLOTS 64-bit counters
   size_t cap = 0;
   hipEvent_t ev_begin = nullptr, ev_end = nullptr, ev_vbegin = nullptr, ev_vend = nullptr;
@@ -101,7 +127,6 @@ struct Queues {
       BNX_HIP(hit_v[k].ensure(sizeof(float) * n));
       BNX_HIP(vcol[k].ensure(sizeof(float4) * n));
     }
-    BNX_HIP(counts.ensure(128));
     BNX_HIP(stats.ensure(sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS));
     cap = n;
     return BNX_OK;
@@ -125,13 +150,14 @@ int frame_read(Frame *f, int channel, void *host);
 // primary rays for total queue entries starting at queue index q_begin
 int launch_raygen(Device *dev, const RaygenChunk *d_chunks, int num_chunks, const SlotDev *d_slots,
                   long long q_begin, long long total, const RayQueue &q);
-// shade in[0, *in_count), appending to out at *out_count; live (optional):
-// per-slot counters of written rays, see LIVE_SPREAD
-int launch_shade(Device *dev, const SlotDev *d_slots, const RayQueue &in, const int *in_count, int max_in,
-                 const RayQueue &out, int *out_count, unsigned int *live, unsigned long long *stats);
-// sum and clear the live counters of the listed slots: out[2k], out[2k+1] =
-// rays and path rays of slot slots[k]
-int launch_live_reduce(Device *dev, unsigned int *live, const int *d_slot_list, int n, unsigned int *out);
+// fill the block -> segment maps of nsegs segments (blocks256 / blocks1024
+// blocks in all)
+int launch_segment_blocks(Device *dev, const Segment *d_segs, int nsegs, int blocks256, int blocks1024,
+                          int *blk256, int *blk1024);
+// shade every ray of the segments (blocks1024 blocks of 1024 threads),
+// appending each request's shadow and continuation rays to its region of out
+int launch_shade(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks1024, const RayQueue &in,
+                 const RayQueue &out, const SlotOut &so, unsigned long long *stats);
 int trace_stack_capacity();
 
 // the per-request frame loop behind bnx_render
@@ -139,5 +165,17 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
                  bnx_render_stats *stats);
 // free the device's render queues
 void wavefront_destroy(Device *dev);
+
+// The segment of this block of a B-thread launch, and the ray this thread
+// takes (i, valid).  Every thread of a block gets the same segment.
+template <int B> BNX_D Segment segment_of(const SegView &sv, int &i, bool &valid)
+{
+  const int k = (B == 256 ? sv.blk256 : sv.blk1024)[blockIdx.x];
+  const Segment sg = sv.segs[k];
+  const int local = ((int)blockIdx.x - (B == 256 ? sg.block256 : sg.block1024)) * B + (int)threadIdx.x;
+  valid = local < sg.count;
+  i = sg.begin + local;
+  return sg;
+}
 
 } // namespace bnx

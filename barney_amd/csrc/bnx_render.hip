@@ -8,12 +8,14 @@
 //
 // Layout: the ray queue is structure-of-arrays (16-byte records per
 // field), so each wave reads and writes whole cache lines; several samples
-// per pixel, and several requests, are in flight in one queue (at
-// generation 0 a request's batch occupies begin + sample * pixels + pixel).
-// Every ray carries its request slot in its flags; per-request state
-// (scene, lights, camera, renderer, frame) lives in a slot table in HBM.
-// Shade appends its shadow and continuation rays with one atomic per block
-// (ballot + popcount prefix inside each wave, LDS scan across waves).  The frame accumulates in
+// per pixel, and several requests, are in flight in one queue (a run of
+// primary rays holds consecutive path indices sample * pixels + pixel).
+// The queue is cut into segments of one request each, and every block lies
+// inside one segment, so the request's state (scene, lights, camera,
+// renderer, frame: a slot table in HBM) is block-uniform.  Shade appends
+// each request's shadow and continuation rays to that request's region of
+// the next queue with one atomic per block (ballot + popcount prefix inside
+// each wave, LDS scan across waves).  The frame accumulates in
 // 2^-32 fixed point with 64-bit integer atomics, so a pixel's sum does not
 // depend on the order in which rays of different samples or requests
 // arrive.
@@ -88,7 +90,7 @@ __global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDe
     }
   }
   q.org[idx] = make_float4(org.x, org.y, org.z, tmax);
-  q.dir[idx] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(mk_flags(false, s, ch.slot)));
+  q.dir[idx] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(mk_flags(false, s)));
   q.rng[idx] = seed;
   uint32_t one = f2h(1.f);
   q.ps[idx] = make_uint4(one | (one << 16), one, (uint32_t)pix, __float_as_uint(0.f));   // generation 0
@@ -228,14 +230,16 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
   const float hu = __int_as_float(hit.w);
   float color[4] = {0.f, 0.f, 0.f, 1.f};
   if (G.colors) {
-    color[0] = G.colors[3 * prim + 0];
-    color[1] = G.colors[3 * prim + 1];
-    color[2] = G.colors[3 * prim + 2];
+    const float *cp = G.colors + 3 * prim;
+    color[0] = cp[0];
+    color[1] = cp[1];
+    color[2] = cp[2];
   }
   v3 P, Nw;
   if (G.type == 0) {
     const int *ix = G.indices + 3 * prim;
-    v3 v0 = ld3(G.vertices + 3 * ix[0]), v1 = ld3(G.vertices + 3 * ix[1]), v2 = ld3(G.vertices + 3 * ix[2]);
+    const float *vx = G.vertices;
+    v3 v0 = ld3(vx + 3 * ix[0]), v1 = ld3(vx + 3 * ix[1]), v2 = ld3(vx + 3 * ix[2]);
     v3 n = normalize(cross(v1 - v0, v2 - v0));
     v3 osN = normalize(n);
     v3 osP = (((1.f - hu) - hv) * v0 + hu * v1) + hv * v2;
@@ -321,18 +325,21 @@ BNX_D void pack_ps(uint4 &ps, v3 tp, int ndiff, int gen, int pixel, float mis)
 constexpr int SHADE_BLOCK = 1024;
 
 // Each ray carries its own generation, so one launch can shade rays of
-// different requests at different depths of their paths.
-__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *slots, RayQueue in, const int *in_count,
-                                                       RayQueue out, int *out_count, unsigned int *live,
-                                                       unsigned long long *stats)
+// different requests at different depths of their paths.  Each block lies
+// inside one request's segment: the request's parameters are block-uniform
+// (scalar registers) and the block appends to that request's output region.
+__global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict__ slots, SegView sv, RayQueue in,
+                                                       RayQueue out, SlotOut so, unsigned long long *stats)
 {
   __shared__ int wave_base[SHADE_BLOCK / 64];
+  __shared__ int wave_paths[SHADE_BLOCK / 64];
   __shared__ int block_base;
-  const int i = blockIdx.x * blockDim.x + threadIdx.x;
-  const int n = *in_count;
-  const bool valid = i < n;
+  int i;
+  bool valid;
+  const Segment sg = segment_of<SHADE_BLOCK>(sv, i, valid);
+  const int slot_id = sg.slot;
+  const SlotDev &P = slots[slot_id];
   bool w_sh = false, w_ray = false;
-  int slot_id = -1;
   OutRay o_sh, o_ray;
   if (valid) {
     float4 ro = in.org[i], rd = in.dir[i];
@@ -343,8 +350,6 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *slots, Ray
     const uint32_t flags = __float_as_uint(rd.w);
     const bool is_shadow = (flags & FLAG_SHADOW) != 0;
     const int sample = flag_sample(flags);
-    slot_id = flag_slot(flags);
-    const SlotDev &P = slots[slot_id];
     const SceneDev &S = P.scene;
     const FrameDev &fb = P.fb;
     const int accumID = P.accum_base + sample;
@@ -418,7 +423,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *slots, Ray
             sh_tmax = ls.distance * (1.f - 2.f * off_eps);
             o_sh.org = make_float4(so.x, so.y, so.z, sh_tmax);
             o_sh.dir = make_float4(ls.direction.x, ls.direction.y, ls.direction.z,
-                                   __uint_as_float(mk_flags(true, sample, slot_id)));
+                                   __uint_as_float(mk_flags(true, sample)));
             o_sh.rng = rng;
             rng_next_h(rng, (uint64_t)__float_as_uint(tmax));
             float smis = 1.f;
@@ -477,7 +482,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *slots, Ray
           }
           if (ray_tmax > 0.f) {
             o_ray.org = make_float4(rorg.x, rorg.y, rorg.z, ray_tmax);
-            o_ray.dir = make_float4(rdir.x, rdir.y, rdir.z, __uint_as_float(mk_flags(false, sample, slot_id)));
+            o_ray.dir = make_float4(rdir.x, rdir.y, rdir.z, __uint_as_float(mk_flags(false, sample)));
             o_ray.rng = rng;
             o_ray.ps = ps;
           }
@@ -522,74 +527,69 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *slots, Ray
   // ------------------------------------------------ compacted append
   // wave ballot + popcount gives each lane its offset inside the wave, the
   // wave totals are scanned in LDS, and one atomic per block reserves the
-  // block's range in the output queue
+  // block's range in its request's output region
   const uint64_t b1 = __ballot(w_sh);
   const uint64_t b2 = __ballot(w_ray);
   const uint64_t lt = lanemask_lt();
   const int pre = __popcll(b1 & lt) + __popcll(b2 & lt);
   const int tot = __popcll(b1) + __popcll(b2);
   const int wid = threadIdx.x >> 6;
-  if (lane_id() == 0) wave_base[wid] = tot;
+  if (lane_id() == 0) {
+    wave_base[wid] = tot;
+    wave_paths[wid] = __popcll(b2);
+  }
   __syncthreads();
   if (threadIdx.x == 0) {
-    int acc = 0;
+    int acc = 0, paths = 0;
     for (int w = 0; w < SHADE_BLOCK / 64; w++) {
       int c = wave_base[w];
       wave_base[w] = acc;
       acc += c;
+      paths += wave_paths[w];
     }
-    block_base = acc > 0 ? atomicAdd(out_count, acc) : 0;
+    block_base = so.base[slot_id] + (acc > 0 ? atomicAdd(so.cnt + 2 * slot_id, acc) : 0);
+    if (paths > 0) atomicAdd(so.cnt + 2 * slot_id + 1, paths);
   }
   __syncthreads();
-  int slot = block_base + wave_base[wid] + pre;
+  int o = block_base + wave_base[wid] + pre;
   if (w_sh) {
-    out.org[slot] = o_sh.org;
-    out.dir[slot] = o_sh.dir;
-    out.rng[slot] = o_sh.rng;
-    out.ps[slot] = o_sh.ps;
-    slot++;
+    out.org[o] = o_sh.org;
+    out.dir[o] = o_sh.dir;
+    out.rng[o] = o_sh.rng;
+    out.ps[o] = o_sh.ps;
+    o++;
   }
   if (w_ray) {
-    out.org[slot] = o_ray.org;
-    out.dir[slot] = o_ray.dir;
-    out.rng[slot] = o_ray.rng;
-    out.ps[slot] = o_ray.ps;
+    out.org[o] = o_ray.org;
+    out.dir[o] = o_ray.dir;
+    out.rng[o] = o_ray.rng;
+    out.ps[o] = o_ray.ps;
   }
   if (lane_id() == 0) stat_add(stats, STAT_SHADOW, (uint32_t)__popcll(b1));
-  if (live) {
-    // per-request counts of the rays written: one atomic per distinct
-    // request in the wave (usually one, since a wave's input rays come from
-    // one run of the queue)
-    uint64_t todo = b1 | b2;
-    const int spread = (int)(blockIdx.x & (LIVE_SPREAD - 1));
-    while (todo) {
-      const int leader = __ffsll((unsigned long long)todo) - 1;
-      const int ls = __shfl(slot_id, leader);
-      const uint64_t same = __ballot(slot_id == ls) & todo;
-      if (lane_id() == leader) {
-        atomicAdd(live + ((size_t)ls * 2 + 0) * LIVE_SPREAD + spread, (unsigned)(__popcll(b1 & same) + __popcll(b2 & same)));
-        atomicAdd(live + ((size_t)ls * 2 + 1) * LIVE_SPREAD + spread, (unsigned)__popcll(b2 & same));
-      }
-      todo &= ~same;
-    }
-  }
 }
 
-__global__ void k_live_reduce(unsigned int *live, const int *slot_list, int n, unsigned int *out)
+// block -> segment maps: one thread per block of either launch width
+__global__ void k_segment_blocks(const Segment *segs, int nsegs, int blocks256, int blocks1024, int *blk256,
+                                 int *blk1024)
 {
-  const int k = blockIdx.x * (blockDim.x / 64) + (threadIdx.x >> 6);
-  const int lane = threadIdx.x & 63;
-  if (k >= n) return;
-  unsigned int *p = live + (size_t)slot_list[k] * 2 * LIVE_SPREAD;
-  unsigned int v = lane < 2 * LIVE_SPREAD ? p[lane] : 0u;
-  if (lane < 2 * LIVE_SPREAD) p[lane] = 0u;
-  // lanes 0..31: all rays, lanes 32..63: path rays
-  unsigned int a = lane < LIVE_SPREAD ? v : 0u, b = lane < LIVE_SPREAD ? 0u : v;
-  a = wave_sum(a);
-  b = wave_sum(b);
-  if (lane == 0) {
-    out[2 * k] = a;
-    out[2 * k + 1] = b;
+  const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  if (b < blocks256) {
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segs[mid].block256 <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    blk256[b] = lo;
+  }
+  if (b < blocks1024) {
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segs[mid].block1024 <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    blk1024[b] = lo;
   }
 }
 
@@ -693,19 +693,20 @@ int launch_raygen(Device *dev, const RaygenChunk *chunks, int num_chunks, const 
   return BNX_OK;
 }
 
-int launch_shade(Device *dev, const SlotDev *slots, const RayQueue &in, const int *in_count, int max_in,
-                 const RayQueue &out, int *out_count, unsigned int *live, unsigned long long *stats)
+int launch_shade(Device *dev, const SlotDev *slots, const SegView &sv, int blocks1024, const RayQueue &in,
+                 const RayQueue &out, const SlotOut &so, unsigned long long *stats)
 {
-  if (max_in > 0)
-    k_shade<<<nblk(max_in, SHADE_BLOCK), SHADE_BLOCK, 0, dev->stream>>>(slots, in, in_count, out, out_count, live,
-                                                                        stats);
+  if (blocks1024 > 0) k_shade<<<blocks1024, SHADE_BLOCK, 0, dev->stream>>>(slots, sv, in, out, so, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }
 
-int launch_live_reduce(Device *dev, unsigned int *live, const int *slot_list, int n, unsigned int *out)
+int launch_segment_blocks(Device *dev, const Segment *segs, int nsegs, int blocks256, int blocks1024, int *blk256,
+                          int *blk1024)
 {
-  if (n > 0) k_live_reduce<<<nblk(n, 4), 256, 0, dev->stream>>>(live, slot_list, n, out);
+  if (nsegs > 0 && blocks256 > 0)
+    k_segment_blocks<<<nblk(blocks256, 256), 256, 0, dev->stream>>>(segs, nsegs, blocks256, blocks1024, blk256,
+                                                                    blk1024);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -5,8 +5,8 @@
 // kernels/traceRays.dev.cu:26-54.
 //
 // Design (wave64):
-//   * persistent waves: lane 0 of each wave takes 64 rays at a time from a
-//     device counter, so long rays do not hold a whole block hostage;
+//   * one ray per thread; blocks past the queue's device-side count exit at
+//     once, so a launch can be sized for the queue's capacity;
 //   * nearest-child-first binary BVH traversal over 64-byte nodes (both
 //     child boxes in one fetch = four 16-byte loads);
 //   * short stack: a 16-entry ring per lane in LDS laid out [entry][lane]
@@ -257,24 +257,24 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
   }
 }
 
-// one ray per thread; blocks past the queue count exit at once.
+// one ray per thread over the generation's segments; each block lies inside
+// one request's segment, so the request's scene pointers are block-uniform.
 // counters: statistics array (STAT_NODES, STAT_TRIS, STAT_SPHERES)
-__global__ __launch_bounds__(TB) void k_trace_queue(const SlotDev *slots, TraceQueue q, const int *d_count,
+__global__ __launch_bounds__(TB) void k_trace_queue(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
                                                     unsigned long long *counters)
 {
   __shared__ uint32_t ring[SHORT * TB];
-  const int n = *d_count;
-  if ((int)(blockIdx.x * TB) >= n) return;
   Stack st;
   st.lds = ring + threadIdx.x;
   Counts cnt{0, 0, 0};
-  const int i = blockIdx.x * TB + threadIdx.x;
-  if (i < n) {
+  int i;
+  bool valid;
+  const Segment sg = segment_of<TB>(sv, i, valid);
+  const SceneDev S = slots[sg.slot].scene;
+  if (valid) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
-    const uint32_t flags = __float_as_uint(d.w);
-    const bool shadow = (flags & FLAG_SHADOW) != 0;
-    const SceneDev &S = slots[flag_slot(flags)].scene;
+    const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
     if (S.num_insts > 0) h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, st, cnt);
     if (h.prim >= 0) q.org[i].w = h.t;
@@ -303,10 +303,10 @@ int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, ui
   return BNX_OK;
 }
 
-int launch_trace_queue(Device *dev, const SlotDev *slots, const TraceQueue &q, const int *d_count, int max_n,
+int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, const TraceQueue &q,
                        unsigned long long *counters)
 {
-  k_trace_queue<<<grid_for(max_n), TB, 0, dev->stream>>>(slots, q, d_count, counters);
+  if (blocks256 > 0) k_trace_queue<<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

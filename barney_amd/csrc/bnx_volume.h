@@ -10,7 +10,7 @@ namespace bnx {
 constexpr int VOLUME_GEOM_BASE = 0x40000000;
 
 struct VolumeDev {
-  const float *scalars;       // dims[0]*dims[1]*dims[2], x fastest
+  const float BNX_G *scalars;   // dims[0]*dims[1]*dims[2], x fastest
   int dims[3];
   int ncells[3];
   float origin[3];
@@ -20,8 +20,8 @@ struct VolumeDev {
   int cells_per_mc;
   float mc_origin[3];
   float mc_rcp_spacing[3];    // 1/(cells_per_mc * gridSpacing)
-  const float *majorants;
-  const float4 *tf;
+  const float BNX_G *majorants;
+  const float4 BNX_G *tf;
   int ntf;
   float dom0, dom1, base_density;
   int rtc_inst;               // rtc instance index (seeds the march RNG)
@@ -30,7 +30,7 @@ struct VolumeDev {
 };
 
 struct VolumeList {
-  const VolumeDev *vols;      // device array
+  const VolumeDev BNX_G *vols;  // device array
   int count;
 };
 
@@ -42,11 +42,12 @@ struct Volume {
 
 int volume_build(Volume *v);
 int volume_rebuild_majorants(Volume *v);
-// surfaces: false when the world has no surface instances (the trace kernel
-// was skipped and the hit records are not written yet)
 // upload the world's volume table (if changed) and return it
 int world_volumes(World *w, VolumeList &vl);
-int launch_volume_march(Device *dev, const SlotDev *d_slots, const RayQueue &q, const int *d_count, int max_n,
+// march every ray of the generation's segments (blocks256 blocks of 256
+// threads); surfaces: false when the trace kernel did not run, so the hit
+// records are not written yet
+int launch_volume_march(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, const RayQueue &q,
                         bool surfaces, unsigned long long *stats);
 
 // ---------------------------------------------------------------- device
@@ -86,10 +87,11 @@ BNX_D float volume_sample(const VolumeDev &v, v3 P)
   lerp_coord(v.dims[1], ry + .5f, y0, y1, fy);
   lerp_coord(v.dims[2], rz + .5f, z0, z1, fz);
   const size_t Nx = (size_t)v.dims[0], NxNy = (size_t)v.dims[0] * (size_t)v.dims[1];
-  const float *b00 = v.scalars + (size_t)y0 * Nx + (size_t)z0 * NxNy;
-  const float *b01 = v.scalars + (size_t)y1 * Nx + (size_t)z0 * NxNy;
-  const float *b10 = v.scalars + (size_t)y0 * Nx + (size_t)z1 * NxNy;
-  const float *b11 = v.scalars + (size_t)y1 * Nx + (size_t)z1 * NxNy;
+  const float *sc = v.scalars;
+  const float *b00 = sc + (size_t)y0 * Nx + (size_t)z0 * NxNy;
+  const float *b01 = sc + (size_t)y1 * Nx + (size_t)z0 * NxNy;
+  const float *b10 = sc + (size_t)y0 * Nx + (size_t)z1 * NxNy;
+  const float *b11 = sc + (size_t)y1 * Nx + (size_t)z1 * NxNy;
   float v00 = lerp_l(fx, b00[x0], b00[x1]);
   float v01 = lerp_l(fx, b01[x0], b01[x1]);
   float v10 = lerp_l(fx, b10[x0], b10[x1]);
@@ -109,7 +111,8 @@ BNX_D float4 tf_map(const VolumeDev &v, float s)
   int idx = (int)f;
   idx = idx < 0 ? 0 : (idx > N - 2 ? N - 2 : idx);
   f -= (float)idx;
-  float4 a = v.tf[idx], b = v.tf[idx + 1];
+  const float4 *tf = v.tf;
+  float4 a = tf[idx], b = tf[idx + 1];
   float4 r;
   r.x = (1.f - f) * a.x + f * b.x;
   r.y = (1.f - f) * a.y + f * b.y;
@@ -136,12 +139,13 @@ BNX_D float tf_majorant(const VolumeDev &v, float lo, float hi)
   i_hi = i_hi < 0 ? 0 : (i_hi > N - 2 ? N - 2 : i_hi);
   f_lo -= (float)i_lo;
   f_hi -= (float)i_hi;
-  float l_lo = (1.f - f_lo) * v.tf[i_lo].w + f_lo * v.tf[i_lo + 1].w;
-  float l_hi = (1.f - f_hi) * v.tf[i_hi].w + f_hi * v.tf[i_hi + 1].w;
+  const float4 *tf = v.tf;
+  float l_lo = (1.f - f_lo) * tf[i_lo].w + f_lo * tf[i_lo + 1].w;
+  float l_hi = (1.f - f_hi) * tf[i_hi].w + f_hi * tf[i_hi + 1].w;
   float m = 0.f;
   m = fmx(m, l_lo);
   m = fmx(m, l_hi);
-  for (int i = i_lo + 1; i <= i_hi; i++) m = fmx(m, v.tf[i].w);
+  for (int i = i_lo + 1; i <= i_hi; i++) m = fmx(m, tf[i].w);
   return m * v.base_density;
 }
 

@@ -60,24 +60,24 @@ __global__ void k_majorants(VolumeDev v, const float2 *ranges, float *maj)
   maj[tid] = tf_majorant(v, r.x, r.y);
 }
 
-// march every queued ray through the world's volumes, one ray per thread.
-// With no surface instances the trace kernel is skipped and every ray
-// enters here as a surface miss.
-__global__ __launch_bounds__(256) void k_volume_march(const SlotDev *slots, RayQueue q, const int *d_count,
+// march every ray of the generation's segments through its request's
+// volumes, one ray per thread; each block lies inside one request's segment,
+// so the volume table is block-uniform.  With no surface instances in the
+// batch the trace kernel is skipped and every ray enters here as a surface
+// miss (surfaces == 0).
+__global__ __launch_bounds__(256) void k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
                                                       int surfaces, unsigned long long *stats)
 {
-  const int n = *d_count;
-  if ((int)(blockIdx.x * 256) >= n) return;
   uint32_t ns = 0, nc = 0;
-  const int i = blockIdx.x * 256 + threadIdx.x;
-  if (i < n) {
+  int i;
+  bool valid;
+  const Segment sg = segment_of<256>(sv, i, valid);
+  const VolumeList vl{slots[sg.slot].vols, slots[sg.slot].num_vols};
+  if (valid) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
     int4 hit = surfaces ? q.hit[i] : make_int4(-1, -1, -1, 0);
-    const uint32_t flags = __float_as_uint(d.w);
-    const bool shadow = (flags & FLAG_SHADOW) != 0;
-    const SlotDev &SL = slots[flag_slot(flags)];
-    const VolumeList vl{SL.vols, SL.num_vols};
+    const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     if (!(shadow && hit.x >= 0)) {                 // an occluded shadow ray is done
       const v3 org = xyz(o), dir = fix_dir(xyz(d));
       float tcur = o.w;
@@ -124,7 +124,7 @@ int volume_build(Volume *v)
   const size_t nmc = (size_t)d.mc[0] * d.mc[1] * d.mc[2];
   BNX_HIP(v->ranges.ensure(sizeof(float2) * nmc));
   BNX_HIP(v->majorants.ensure(sizeof(float) * nmc));
-  d.majorants = v->majorants.as<float>();
+  BNX_SET(d.majorants, v->majorants.ptr);
   k_mc_ranges<<<nblk((long)nmc, 64), 64, 0, dev->stream>>>(d, v->ranges.as<float2>());
   k_majorants<<<nblk((long)nmc, 256), 256, 0, dev->stream>>>(d, v->ranges.as<float2>(), v->majorants.as<float>());
   BNX_HIP(hipStreamSynchronize(dev->stream));
@@ -143,11 +143,11 @@ int volume_rebuild_majorants(Volume *v)
   return BNX_OK;
 }
 
-int launch_volume_march(Device *dev, const SlotDev *slots, const RayQueue &q, const int *d_count, int max_n,
+int launch_volume_march(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, const RayQueue &q,
                         bool surfaces, unsigned long long *stats)
 {
-  if (max_n == 0) return BNX_OK;
-  k_volume_march<<<nblk(max_n, 256), 256, 0, dev->stream>>>(slots, q, d_count, surfaces ? 1 : 0, stats);
+  if (blocks256 > 0)
+    k_volume_march<<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

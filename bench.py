@@ -1,12 +1,24 @@
 """Benchmark: Mrays/s of barney's single-GPU ray-tracing hot path on MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c2|c1]
 
-A step renders one frame of the workload from a cleared frame buffer
-(raygen -> trace/shade generations -> accumulate), with the scene and its
-BVHs resident in HBM.  Prints one JSON line with whole-job Mrays/s, the
-frame latency percentiles, the trace kernel's roofline figures and the CPU
-oracle's throughput on the same scene (timed on this host).
+c5 (default): the serving stream.  Requests over resident worlds (config 1's
+surface scene and config 2's structured volume) arrive with seeded Poisson
+arrivals, each with a resolution in {256^2, 512^2, 1024^2, 2048^2}, spp in
+{1, 4, 16, 64} and a camera on a seeded orbit.  A step is one request.  The
+same K requests are rendered (a) plain: one bnx_render per request, in
+arrival order, and (b) served: through the continuous-batching server.
+Reported: served Mrays/s at saturation (all K requests queued at once) as
+`value`, plain Mrays/s and their ratio, p50/p99 frame latency (arrival ->
+frame readable) of both forms under the same Poisson stream, and the plain
+single-request latency.  Served and plain frames are identical (checked on
+a sample at the end).
+
+c2 / c1: one request of config 2 (1024^2, 16 spp structured volume) or
+config 1 (512^2, 1 spp surfaces) per step, from a cleared frame.
+
+Every line carries the dominant kernel's roofline figures and the CPU
+oracle's throughput on a bounded sample of the same workload.
 """
 from __future__ import annotations
 
@@ -27,6 +39,32 @@ def algorithmic_bytes(rays, nodes, tris, spheres):
     # per traced ray: 32 B ray in + 16 B hit out; 64 B per node visit,
     # 48 B per triangle record, 16 B per sphere record
     return 48.0 * rays + 64.0 * nodes + 48.0 * tris + 16.0 * spheres
+
+
+def march_bytes(samples, cells, rays):
+    # volume march kernel: 8 texels (32 B) per sample, 4 B per macrocell
+    # majorant, 32 B ray in + 16 B hit out per marched ray
+    return 32.0 * samples + 4.0 * cells + 48.0 * rays
+
+
+def roofline(tot):
+    """dominant kernel (by summed HIP-event time) and its algorithmic GB/s"""
+    if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
+        kernel, kms, launches = "k_volume_march", tot["march_ms"], tot["march_launches"]
+        byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
+    else:
+        kernel, kms, launches = "k_trace_queue", tot["trace_ms"], tot["trace_launches"]
+        byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
+    achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+            "bytes_per_launch": round(byts / max(1, launches)), "avg_launch_ms": round(kms / max(1, launches), 5),
+            "launches": launches, "kernel_ms": round(kms, 3)}
+
+
+def pct(a, q):
+    import numpy as np
+    return round(float(np.percentile(np.asarray(a, dtype=np.float64), q)), 4) if len(a) else None
 
 
 def workload_c2(args):
@@ -112,15 +150,202 @@ def run_cpu(sc, budget_s):
     return out
 
 
+# ---------------------------------------------------------------- serving
+def serve_setup(dev, n, seed=None):
+    import barney_amd as bx
+    from barney_amd import abi, scenes
+    c1, c2 = scenes.config1(), scenes.config2()
+    worlds = [(bx.SurfaceWorld(dev, c1), c1), (bx.VolumeWorld(dev, c2), c2)]
+    stream = scenes.serving_stream(n, 1.0, [c1, c2], **({} if seed is None else {"seed": seed}))
+    reqs = []
+    for r in stream:
+        gw, sc = worlds[r.scene]
+        cam = bx.camera_perspective(r.camera["position"], r.camera["direction"], r.camera["up"], r.camera["fovy"])
+        rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient)
+        reqs.append(dict(world=gw.world, cam=cam, rend=rend, w=r.width, h=r.height, spp=r.spp, a1=r.arrival_s,
+                         scene=r.scene))
+    return worlds, reqs
+
+
+def serve_plain(dev, reqs, frames):
+    """one bnx_render per request; returns service times (s) and totals"""
+    import barney_amd as bx
+    svc, tot = [], {}
+    for r in reqs:
+        fr = frames[(r["w"], r["h"])]
+        t = time.perf_counter()
+        fr.reset()
+        st = bx.render(r["world"], r["cam"], r["rend"], fr, r["spp"])
+        svc.append(time.perf_counter() - t)
+        for k, v in st.items():
+            tot[k] = tot.get(k, 0) + v
+    return svc, tot
+
+
+def serve_served(dev, reqs, arrivals=None, cfg=None):
+    """submit at the arrival times (all at once when arrivals is None) and
+    step the server until every request is done; returns per-request
+    completion times (s after the start), the makespan and server stats"""
+    import barney_amd as bx
+    srv = bx.Server(dev, **(cfg or {}))
+    done_at = {}
+    tix = {}
+    t0 = time.perf_counter()
+    i, n = 0, len(reqs)
+    while i < n or srv.pending:
+        now = time.perf_counter() - t0
+        while i < n and (arrivals is None or arrivals[i] <= now):
+            r = reqs[i]
+            tix[srv.submit(r["world"], r["cam"], r["rend"], r["w"], r["h"], r["spp"])] = i
+            i += 1
+        if srv.pending:
+            for t in srv.step():
+                done_at[tix[t]] = time.perf_counter() - t0
+                srv.release(t)
+        elif i < n:
+            time.sleep(max(0.0, min(arrivals[i] - now, 0.01)))
+    makespan = time.perf_counter() - t0
+    st = srv.stats()
+    srv.close()
+    return [done_at[k] for k in range(n)], makespan, st
+
+
+def serve_check(dev, reqs, frames, k=6):
+    """served frames equal plain frames on the first k small requests"""
+    import numpy as np
+
+    import barney_amd as bx
+    from barney_amd import abi
+    small = [r for r in reqs if r["w"] * r["h"] * r["spp"] <= 1024 * 1024 * 4][:k]
+    srv = bx.Server(dev)
+    tix = {srv.submit(r["world"], r["cam"], r["rend"], r["w"], r["h"], r["spp"]): j for j, r in enumerate(small)}
+    srv.run_until_idle()
+    same = 0
+    for t, j in tix.items():
+        r = small[j]
+        fr = frames[(r["w"], r["h"])]
+        fr.reset()
+        bx.render(r["world"], r["cam"], r["rend"], fr, r["spp"])
+        same += int(np.array_equal(fr.read(abi.BNX_FB_COLOR), srv.read(t, abi.BNX_FB_COLOR)) and
+                    np.array_equal(fr.read(abi.BNX_FB_PRIMID), srv.read(t, abi.BNX_FB_PRIMID)))
+        srv.release(t)
+    srv.close()
+    return same, len(small)
+
+
+def log(msg):
+    print(f"[bench] {msg}", file=sys.stderr, flush=True)
+
+
+def run_c5(args, n, warmup):
+    import numpy as np
+
+    import barney_amd as bx
+    dev = bx.Device(0)
+    worlds, reqs = serve_setup(dev, n + warmup)
+    log("worlds resident")
+    warm, reqs = reqs[:warmup], reqs[warmup:]
+    a1 = np.array([r["a1"] for r in reqs]) - reqs[0]["a1"]
+    frames = {}
+    for r in warm + reqs:
+        if (r["w"], r["h"]) not in frames:
+            frames[(r["w"], r["h"])] = bx.Frame(dev, r["w"], r["h"])
+    # warm-up: both forms
+    serve_plain(dev, warm, frames)
+    serve_served(dev, warm)
+    dev.sync()
+    # plain: service times alone, Mrays/s over the busy time
+    svc, ptot = serve_plain(dev, reqs, frames)
+    svc = np.array(svc)
+    plain_mrays = ptot["rays"] / svc.sum() / 1e6
+    log(f"plain: {svc.sum():.3f} s busy, {plain_mrays:.1f} Mrays/s")
+    # served at saturation
+    _, makespan, sst = serve_served(dev, reqs)
+    served_mrays = sst["rays_traced"] / makespan / 1e6
+    log(f"served at saturation: {makespan:.3f} s, {served_mrays:.1f} Mrays/s")
+    # the same stream, Poisson arrivals at a fixed rate: load args.load of
+    # the plain form's capacity
+    qps = args.load / float(svc.mean())
+    arr = a1 / qps
+    done, _, _ = serve_served(dev, reqs, arrivals=arr)
+    lat_served = (np.array(done) - arr) * 1e3
+    log(f"served at {qps:.1f} QPS: p50 {np.percentile(lat_served, 50):.2f} ms, p99 {np.percentile(lat_served, 99):.2f} ms")
+    # plain form under the same arrivals: one request at a time, in order
+    fin, lat_plain = 0.0, []
+    for a, s in zip(arr, svc):
+        fin = max(fin, a) + s
+        lat_plain.append((fin - a) * 1e3)
+    same, checked = serve_check(dev, reqs, frames)
+    for f in frames.values():
+        f.close()
+    tot = dict(rays=sst["rays_traced"], nodes=sst["node_visits"], tris=sst["tri_tests"],
+               spheres=sst["sphere_tests"], trace_ms=sst["trace_ms"], trace_launches=sst["trace_launches"],
+               volume_samples=sst["volume_samples"], march_ms=sst["march_ms"],
+               march_launches=sst["march_launches"], mc_cells=sst["mc_cells"], march_rays=sst["march_rays"])
+    res = sorted({r["w"] for r in reqs})
+    desc = {"workload": f"config 5: serving stream, {n} requests (bounded sample of the 10,000-request stream) "
+                        f"over resident worlds config 1 (20,480 triangles + 1,000 spheres) and config 2 "
+                        f"(256^3 structured volume); resolution in {{256..2048}}^2, spp in {{1,4,16,64}}, "
+                        f"seeded orbit cameras; Poisson arrivals at {qps:.2f} QPS (load {args.load} of the "
+                        f"plain form's capacity)",
+            "requests": n, "warmup_requests": warmup, "qps": round(qps, 3), "resolutions": res,
+            "form": "served (continuous batching) vs plain (one bnx_render per request)",
+            "scenes": ["config 1", "config 2"], "missing_scenes": ["config 3 (not built yet)"]}
+    extra = {
+        "frame_latency_ms": {"p50": pct(lat_served, 50), "p99": pct(lat_served, 99)},
+        "served": {"mrays_s_saturated": round(served_mrays, 3), "makespan_s": round(makespan, 4),
+                   "latency_ms": {"p50": pct(lat_served, 50), "p99": pct(lat_served, 99)},
+                   "generations": sst["generations"], "peak_queue": sst["peak_queue"]},
+        "plain": {"mrays_s": round(plain_mrays, 3), "busy_s": round(float(svc.sum()), 4),
+                  "latency_alone_ms": {"p50": pct(svc * 1e3, 50), "p99": pct(svc * 1e3, 99)},
+                  "latency_queued_ms": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)}},
+        "served_vs_plain_mrays": round(served_mrays / plain_mrays, 3),
+        "served_p99_vs_plain_alone_p99": round(pct(lat_served, 99) / pct(svc * 1e3, 99), 3),
+        "parity_sample": f"{same}/{checked} served frames identical to plain",
+        "volume_samples_per_s": round(sst["volume_samples"] / makespan, 1),
+    }
+    dev.close()
+    return served_mrays, makespan, tot, desc, extra
+
+
+def run_cpu_c5(args, budget_s):
+    """the oracle on the smallest requests of the same stream"""
+    import oracle
+    from barney_amd import abi, scenes
+    if not os.path.exists(oracle.LIB_PATH):
+        oracle.build()
+    threads = oracle.default_threads()
+    c1, c2 = scenes.config1(), scenes.config2()
+    oscenes = [oracle.surface_scene(c1), oracle.volume_scene(c2)]
+    stream = scenes.serving_stream(2000, 1.0, [c1, c2])
+    small = [r for r in stream if r.width == 256 and r.spp == 1]
+    rays, frames, t0 = 0, 0, time.perf_counter()
+    for r in small:
+        sc = (c1, c2)[r.scene]
+        cam = oracle.camera_perspective(r.camera["position"], r.camera["direction"], r.camera["up"], r.camera["fovy"])
+        fr = oracle.Frame(r.width, r.height)
+        _, st = oracle.render(oscenes[r.scene], cam, abi.renderer(bg=sc.bg_color, ambient=sc.ambient), fr, r.spp,
+                              threads=threads)
+        rays += st["rays"]
+        frames += 1
+        if time.perf_counter() - t0 > budget_s:
+            break
+    el = time.perf_counter() - t0
+    return {"value": round(rays / el / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+            "sample": f"the first {frames} 256^2 1-spp requests of the same stream (both worlds), {el:.1f} s wall "
+                      f"on {threads} threads"}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=20)
-    ap.add_argument("--warmup", type=int, default=3)
-    ap.add_argument("--workload", default="c2", choices=["c1", "c2"])
+    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--warmup", type=int, default=None)
+    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c5"])
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--height", type=int, default=0)
+    ap.add_argument("--load", type=float, default=0.7)
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
     args = ap.parse_args()
@@ -128,35 +353,34 @@ def main():
         print(f"bench.py: this build runs on one GPU only (the multi-GPU path is out of scope); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
         sys.exit(2)
-    sc, desc = (workload_c2 if args.workload == "c2" else workload_c1)(args)
-    elapsed, lat, tot = run_gpu(sc, args.steps, args.warmup)
-    mrays = tot["rays"] / elapsed / 1e6
-    if tot["march_launches"] > 0:
-        # volume march kernel: 8 texels (32 B) per sample, 4 B per macrocell
-        # majorant, 32 B ray in + 16 B hit out per marched ray
-        kernel, kms, launches = "k_volume_march", tot["march_ms"], tot["march_launches"]
-        byts = 32.0 * tot["volume_samples"] + 4.0 * tot["mc_cells"] + 48.0 * tot["march_rays"]
+    import numpy as np
+    if args.workload == "c5":
+        steps = args.steps if args.steps is not None else 200
+        warmup = args.warmup if args.warmup is not None else 8
+        mrays, elapsed, tot, desc, extra = run_c5(args, steps, warmup)
+        lat_line = extra.pop("frame_latency_ms")
     else:
-        kernel, kms, launches = "k_trace_queue", tot["trace_ms"], tot["trace_launches"]
-        byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
-    achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+        steps = args.steps if args.steps is not None else 20
+        warmup = args.warmup if args.warmup is not None else 3
+        sc, desc = (workload_c2 if args.workload == "c2" else workload_c1)(args)
+        elapsed, lat, tot = run_gpu(sc, steps, warmup)
+        mrays = tot["rays"] / elapsed / 1e6
+        lat_line = {"p50": pct(lat, 50), "p99": pct(lat, 99)}
+        extra = {"rays_per_step": tot["rays"] // steps}
+        if tot["volume_samples"]:
+            extra["volume_samples_per_s"] = round(tot["volume_samples"] / elapsed, 1)
     line = {
-        "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": args.steps,
-        "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scene)",
-        "config": desc,
-        "frame_latency_ms": {"p50": round(float(__import__("numpy").percentile(lat, 50)), 4),
-                             "p99": round(float(__import__("numpy").percentile(lat, 99)), 4)},
-        "rays_per_step": tot["rays"] // args.steps,
-        "roofline": {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-                     "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None,
-                     "kernel": kernel, "bytes_per_launch": round(byts / max(1, launches)),
-                     "avg_launch_ms": round(kms / max(1, launches), 5), "launches_per_step": launches // args.steps},
+        "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
+        "warmup": warmup, "ms_per_step": round(elapsed * 1e3 / steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes and stream)",
+        "config": desc, "frame_latency_ms": lat_line, "roofline": roofline(tot),
     }
-    if tot["volume_samples"]:
-        line["volume_samples_per_s"] = round(tot["volume_samples"] / elapsed, 1)
+    line.update(extra)
     if not args.no_cpu:
-        line["cpu_baseline"] = run_cpu(sc, args.cpu_seconds)
+        if args.workload == "c5":
+            line["cpu_baseline"] = run_cpu_c5(args, args.cpu_seconds)
+        else:
+            line["cpu_baseline"] = run_cpu(sc, args.cpu_seconds)
     print(json.dumps(line), flush=True)
 
 

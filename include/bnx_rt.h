@@ -355,6 +355,12 @@ typedef struct bnx_server_stats {
   int32_t active, queued;
   size_t  frame_bytes;              /* frame memory held (admitted + pooled)           */
   double  busy_ms;                  /* wall time spent inside bnx_server_step          */
+  double  trace_ms;                 /* summed trace-kernel time (HIP events)           */
+  int64_t trace_launches;
+  double  march_ms;                 /* summed volume-march kernel time                 */
+  int64_t march_launches;
+  int64_t march_rays;               /* rays entering the volume-march kernel           */
+  int64_t node_visits, tri_tests, sphere_tests, mc_cells;
 } bnx_server_stats;
 
 int bnx_server_create(bnx_device dev, const bnx_server_config *config, bnx_server *out);
@@ -387,7 +393,7 @@ static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
 static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
 static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
-static_assert(sizeof(bnx_server_stats) == 96, "bnx_server_stats layout");
+static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
 static_assert(sizeof(bnx_render_stats) == 112, "bnx_render_stats layout");
 #endif
