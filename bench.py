@@ -47,7 +47,26 @@ def march_bytes(samples, cells, rays):
     return 32.0 * samples + 4.0 * cells + 48.0 * rays
 
 
-def roofline(tot):
+# rocprofv3 summaries of each workload's dominant kernel; each holds a line
+# "traffic_bytes_per_launch <kernel> <bytes>" measured with FETCH_SIZE /
+# WRITE_SIZE counter runs scaled by a plain copy of known size
+PROFILES = {"c5": "k_trace_queue_config5_serving.md", "c2": "k_volume_march_config2_1024x1024x16spp.md",
+            "c1": "k_trace_queue_config1_512x512x1spp.md"}
+
+
+def profile_traffic(workload, kernel):
+    path = os.path.join(ROOT, "profiles", PROFILES.get(workload, ""))
+    if not os.path.isfile(path):
+        return None
+    with open(path) as f:
+        for line in f:
+            parts = line.split()
+            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
+                return float(parts[2])
+    return None
+
+
+def roofline(tot, workload=None):
     """dominant kernel (by summed HIP-event time) and its algorithmic GB/s"""
     if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
         kernel, kms, launches = "k_volume_march", tot["march_ms"], tot["march_launches"]
@@ -56,8 +75,9 @@ def roofline(tot):
         kernel, kms, launches = "k_trace_queue", tot["trace_ms"], tot["trace_launches"]
         byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
     achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
+    traffic = profile_traffic(workload, kernel)
     return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": None, "kernel": kernel,
+            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
             "bytes_per_launch": round(byts / max(1, launches)), "avg_launch_ms": round(kms / max(1, launches), 5),
             "launches": launches, "kernel_ms": round(kms, 3)}
 
@@ -238,6 +258,11 @@ def log(msg):
 
 
 def run_c5(args, n, warmup):
+    """forms: "all" runs the plain pass, the served pass at saturation (the
+    timed region behind `value` and `roofline`), the served pass under
+    Poisson arrivals and the parity sample; "served" runs only the served
+    pass at saturation, so that a profiler sees exactly the launches behind
+    `roofline`"""
     import numpy as np
 
     import barney_amd as bx
@@ -250,19 +275,33 @@ def run_c5(args, n, warmup):
     for r in warm + reqs:
         if (r["w"], r["h"]) not in frames:
             frames[(r["w"], r["h"])] = bx.Frame(dev, r["w"], r["h"])
+    only_served = args.forms == "served"
     # warm-up: both forms
-    serve_plain(dev, warm, frames)
-    serve_served(dev, warm)
+    if warm:
+        serve_plain(dev, warm, frames)
+        serve_served(dev, warm)
     dev.sync()
     # plain: service times alone, Mrays/s over the busy time
-    svc, ptot = serve_plain(dev, reqs, frames)
-    svc = np.array(svc)
-    plain_mrays = ptot["rays"] / svc.sum() / 1e6
-    log(f"plain: {svc.sum():.3f} s busy, {plain_mrays:.1f} Mrays/s")
+    if not only_served:
+        svc, ptot = serve_plain(dev, reqs, frames)
+        svc = np.array(svc)
+        plain_mrays = ptot["rays"] / svc.sum() / 1e6
+        log(f"plain: {svc.sum():.3f} s busy, {plain_mrays:.1f} Mrays/s")
     # served at saturation
     _, makespan, sst = serve_served(dev, reqs)
     served_mrays = sst["rays_traced"] / makespan / 1e6
     log(f"served at saturation: {makespan:.3f} s, {served_mrays:.1f} Mrays/s")
+    tot = dict(rays=sst["rays_traced"], nodes=sst["node_visits"], tris=sst["tri_tests"],
+               spheres=sst["sphere_tests"], trace_ms=sst["trace_ms"], trace_launches=sst["trace_launches"],
+               volume_samples=sst["volume_samples"], march_ms=sst["march_ms"],
+               march_launches=sst["march_launches"], mc_cells=sst["mc_cells"], march_rays=sst["march_rays"])
+    if only_served:
+        for f in frames.values():
+            f.close()
+        dev.close()
+        desc = {"workload": f"config 5 serving stream, served pass at saturation only ({n} requests)",
+                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)"}
+        return served_mrays, makespan, tot, desc, {"frame_latency_ms": {"p50": None, "p99": None}}
     # the same stream, Poisson arrivals at a fixed rate: load args.load of
     # the plain form's capacity
     qps = args.load / float(svc.mean())
@@ -278,10 +317,6 @@ def run_c5(args, n, warmup):
     same, checked = serve_check(dev, reqs, frames)
     for f in frames.values():
         f.close()
-    tot = dict(rays=sst["rays_traced"], nodes=sst["node_visits"], tris=sst["tri_tests"],
-               spheres=sst["sphere_tests"], trace_ms=sst["trace_ms"], trace_launches=sst["trace_launches"],
-               volume_samples=sst["volume_samples"], march_ms=sst["march_ms"],
-               march_launches=sst["march_launches"], mc_cells=sst["mc_cells"], march_rays=sst["march_rays"])
     res = sorted({r["w"] for r in reqs})
     desc = {"workload": f"config 5: serving stream, {n} requests (bounded sample of the 10,000-request stream) "
                         f"over resident worlds config 1 (20,480 triangles + 1,000 spheres) and config 2 "
@@ -336,6 +371,19 @@ def run_cpu_c5(args, budget_s):
                       f"on {threads} threads"}
 
 
+def copy_probe(mib):
+    """a plain device-to-device copy of `mib` MiB, five times: the reference
+    point that scales the FETCH_SIZE / WRITE_SIZE counters to bytes"""
+    import torch
+    n = mib << 20
+    a = torch.ones(n, dtype=torch.uint8, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(5):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes": n, "copies": 5}), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -346,9 +394,16 @@ def main():
     ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--height", type=int, default=0)
     ap.add_argument("--load", type=float, default=0.7)
+    ap.add_argument("--forms", default="all", choices=["all", "served"],
+                    help="c5 only: run every form, or only the served pass at saturation (for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
+                    help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
+    if args.copy_probe:
+        copy_probe(args.copy_probe)
+        return
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU only (the multi-GPU path is out of scope); "
               f"--gpus {args.gpus} is not supported", file=sys.stderr)
@@ -373,7 +428,7 @@ def main():
         "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": round(elapsed * 1e3 / steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes and stream)",
-        "config": desc, "frame_latency_ms": lat_line, "roofline": roofline(tot),
+        "config": desc, "frame_latency_ms": lat_line, "roofline": roofline(tot, args.workload),
     }
     line.update(extra)
     if not args.no_cpu:
