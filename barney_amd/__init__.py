@@ -62,7 +62,13 @@ _PROTOS = [
     ("bnx_volume_set_transfer_function", [vp, C.POINTER(abi.TransferFunction)], C.c_int),
     ("bnx_volume_majorants", [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float)], C.c_int),
     ("bnx_volume_destroy", [vp], C.c_int),
+    ("bnx_volume_create_umesh", [vp, C.POINTER(abi.UMeshDesc), C.POINTER(abi.TransferFunction), C.c_int32,
+                                 C.POINTER(vp)], C.c_int),
+    ("bnx_volume_mc_ranges", [vp, C.POINTER(C.c_int32), C.POINTER(C.c_float), C.POINTER(C.c_float)], C.c_int),
+    ("bnx_isosurface_create", [vp, C.c_float, C.POINTER(abi.Material), C.c_int32, C.POINTER(vp)], C.c_int),
+    ("bnx_isosurface_destroy", [vp], C.c_int),
     ("bnx_world_add_volume", [vp, vp, C.c_int32], C.c_int),
+    ("bnx_world_add_isosurface", [vp, vp, C.c_int32], C.c_int),
     ("bnx_world_create", [vp, vp, C.POINTER(abi.DirLight), C.c_int, C.POINTER(vp)], C.c_int),
     ("bnx_world_destroy", [vp], C.c_int),
     ("bnx_camera_perspective", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
@@ -285,14 +291,27 @@ def camera_perspective(position, direction, up, fovy_deg) -> abi.Camera:
 
 
 class Volume:
-    """A structured-data volume with its transfer function (MCVolumeAccel)."""
+    """A volume with its transfer function (MCVolumeAccel) over a structured
+    field (scenes.StructuredVolumeScene) or a tet mesh (scenes.UMeshVolumeScene)."""
 
     def __init__(self, dev: Device, vs, user_id: int = 0):
         self.dev = dev
-        d, t, keep = abi.structured_desc(vs)
         h = vp()
-        check(dev.L.bnx_volume_create_structured(dev.h, C.byref(d), C.byref(t), user_id, C.byref(h)))
+        if hasattr(vs, "tets"):
+            d, t, keep = abi.umesh_desc(vs)
+            check(dev.L.bnx_volume_create_umesh(dev.h, C.byref(d), C.byref(t), user_id, C.byref(h)))
+        else:
+            d, t, keep = abi.structured_desc(vs)
+            check(dev.L.bnx_volume_create_structured(dev.h, C.byref(d), C.byref(t), user_id, C.byref(h)))
         self.h = h
+
+    def mc_ranges(self):
+        dims = (C.c_int32 * 3)()
+        check(self.dev.L.bnx_volume_mc_ranges(self.h, dims, None, None))
+        lo = np.zeros(dims[0] * dims[1] * dims[2], np.float32)
+        hi = np.zeros_like(lo)
+        check(self.dev.L.bnx_volume_mc_ranges(self.h, dims, abi.fptr(lo), abi.fptr(hi)))
+        return np.array(list(dims)), lo, hi
 
     def set_transfer_function(self, tf, domain=(0.0, 1.0), base_density=1.0):
         tf = np.ascontiguousarray(tf, dtype=np.float32)
@@ -307,6 +326,17 @@ class Volume:
         return np.array(list(dims)), m
 
 
+class Isosurface:
+    """An isosurface over a volume's field (MCIsoSurfaceAccel), opaque matte."""
+
+    def __init__(self, field: Volume, iso: float, color=(0.8, 0.8, 0.8, 1.0), user_id: int = 0):
+        self.dev, self.field = field.dev, field
+        m = abi.matte(color=tuple(color), from_attribute=False)
+        h = vp()
+        check(field.dev.L.bnx_isosurface_create(field.h, float(iso), C.byref(m), user_id, C.byref(h)))
+        self.h = h
+
+
 class World:
     def __init__(self, dev: Device, instances: Group, dir_lights=()):
         self.dev, self.instances = dev, instances
@@ -319,6 +349,10 @@ class World:
     def add_volume(self, vol: Volume, user_inst_id: int = 0):
         check(self.dev.L.bnx_world_add_volume(self.h, vol.h, user_inst_id))
         self.volumes.append(vol)
+
+    def add_isosurface(self, iso: Isosurface, user_inst_id: int = 0):
+        check(self.dev.L.bnx_world_add_isosurface(self.h, iso.h, user_inst_id))
+        self.volumes.append(iso)
 
 
 class Frame:
@@ -465,3 +499,19 @@ class VolumeWorld:
         self.volume = Volume(dev, vs, user_id=vs.user_id)
         self.world = World(dev, self.insts, vs.dir_lights)
         self.world.add_volume(self.volume, 0)
+
+
+class UMeshWorld:
+    """GPU world for a barney_amd.scenes.UMeshVolumeScene: no surfaces, the
+    tet field as a volume (user instance 0, unless us.volume is false) and
+    the isosurface over it (user instance 1)."""
+
+    def __init__(self, dev: Device, us):
+        self.dev = dev
+        self.insts = Group.instances(dev, [], []).build()
+        self.volume = Volume(dev, us, user_id=us.user_id)
+        self.iso = Isosurface(self.volume, us.iso_value, us.iso_color, user_id=us.iso_user_id)
+        self.world = World(dev, self.insts, us.dir_lights)
+        if us.volume:
+            self.world.add_volume(self.volume, 0)
+        self.world.add_isosurface(self.iso, 1)

@@ -70,6 +70,27 @@ def structured_desc(vs):
     return d, t, (sc, tf)
 
 
+class UMeshDesc(C.Structure):
+    _fields_ = [("vertices", fp), ("scalars", fp), ("indices", ip), ("cell_offsets", ip),
+                ("cell_types", C.POINTER(C.c_uint8)), ("num_vertices", C.c_int32), ("num_indices", C.c_int32),
+                ("num_cells", C.c_int32), ("mc_dims", C.c_int32 * 3)]
+
+
+def umesh_desc(us):
+    """(UMeshDesc, TransferFunction, keep-alive arrays) for a
+    scenes.UMeshVolumeScene: one tet per cell, cell c at index 4c."""
+    v = np.ascontiguousarray(us.vertices, dtype=np.float32)
+    sc = np.ascontiguousarray(us.scalars, dtype=np.float32)
+    idx = np.ascontiguousarray(us.tets, dtype=np.int32).ravel()
+    n = len(us.tets)
+    offs = (4 * np.arange(n, dtype=np.int64)).astype(np.int32)
+    tf = np.ascontiguousarray(us.tf, dtype=np.float32)
+    d = UMeshDesc(fptr(v), fptr(sc), iptr(idx), iptr(offs), None, len(v), len(idx), n,
+                  (C.c_int32 * 3)(*us.mc_dims))
+    t = TransferFunction(fptr(tf), len(tf), (C.c_float * 2)(*us.tf_domain), us.base_density)
+    return d, t, (v, sc, idx, offs, tf)
+
+
 class RenderStats(C.Structure):
     _fields_ = [("rays_traced", C.c_int64), ("shadow_rays", C.c_int64), ("generations", C.c_int64),
                 ("volume_samples", C.c_int64), ("ms_total", C.c_double), ("node_visits", C.c_int64),
@@ -112,6 +133,7 @@ BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE = 0, 1, 2
 
 assert C.sizeof(Material) == 32 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
 assert C.sizeof(ServerConfig) == 24 and C.sizeof(Request) == 136 and C.sizeof(RequestStatus) == 48
+assert C.sizeof(UMeshDesc) == 64
 assert C.sizeof(ServerStats) == 168 and Request.renderer.offset == 76 and Request.spp.offset == 124
 
 # 64-byte BVH node: child boxes lo0,hi0,lo1,hi1 then ref[2], cnt[2]

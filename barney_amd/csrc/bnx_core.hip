@@ -280,22 +280,54 @@ int build_tlas(Group *g)
 
 int world_volumes(World *w, VolumeList &vl)
 {
-  if (w->volumes_dirty) {
+  bool dirty = w->vol_seen.size() != w->volumes.size();
+  for (size_t k = 0; !dirty && k < w->volumes.size(); k++) dirty = w->vol_seen[k] != w->volumes[k]->version;
+  if (dirty) {
     std::vector<VolumeDev> vd;
     const int nsurf = w->instances ? (int)w->instances->groups.size() : 0;
+    w->vol_seen.clear();
     for (size_t k = 0; k < w->volumes.size(); k++) {
       VolumeDev d = w->volumes[k]->dd;
       d.rtc_inst = nsurf + (int)k;
       d.user_inst = w->volume_inst_ids[k];
       vd.push_back(d);
+      w->vol_seen.push_back(w->volumes[k]->version);
     }
     int rc = upload(w->dev, w->d_volumes, vd.data(), vd.size());
     if (rc != BNX_OK) return rc;
     BNX_HIP(hipStreamSynchronize(w->dev->stream));
-    w->volumes_dirty = false;
   }
   BNX_SET(vl.vols, w->d_volumes.as<VolumeDev>());
   vl.count = (int)w->volumes.size();
+  return BNX_OK;
+}
+
+int world_isos(World *w, const IsoDev BNX_G *&isos, int &count)
+{
+  bool dirty = w->iso_seen.size() != w->isos.size();
+  for (size_t k = 0; !dirty && k < w->isos.size(); k++) dirty = w->iso_seen[k] != w->isos[k]->field->version;
+  if (dirty) {
+    std::vector<IsoDev> id;
+    const int nsurf = w->instances ? (int)w->instances->groups.size() : 0;
+    w->iso_seen.clear();
+    for (size_t k = 0; k < w->isos.size(); k++) {
+      const IsoSurface *s = w->isos[k];
+      IsoDev d{};
+      d.field = s->field->dd;
+      d.iso = s->iso;
+      for (int c = 0; c < 3; c++) d.color[c] = s->color[c];
+      d.rtc_inst = nsurf + (int)w->volumes.size() + (int)k;
+      d.user_inst = w->iso_inst_ids[k];
+      d.user_id = s->user_id;
+      id.push_back(d);
+      w->iso_seen.push_back(s->field->version);
+    }
+    int rc = upload(w->dev, w->d_isos, id.data(), id.size());
+    if (rc != BNX_OK) return rc;
+    BNX_HIP(hipStreamSynchronize(w->dev->stream));
+  }
+  BNX_SET(isos, w->d_isos.ptr);
+  count = (int)w->isos.size();
   return BNX_OK;
 }
 
@@ -310,6 +342,7 @@ struct bnx_group_s : Group {};
 struct bnx_world_s : World {};
 struct bnx_frame_s : Frame {};
 struct bnx_volume_s : Volume {};
+struct bnx_isosurface_s : IsoSurface {};
 
 extern "C" {
 
@@ -636,6 +669,7 @@ static int set_tf(Volume *v, const bnx_transfer_function *tf)
   v->dd.dom0 = tf->domain[0];
   v->dd.dom1 = tf->domain[1];
   v->dd.base_density = tf->base_density;
+  v->version++;
   return BNX_OK;
 }
 
@@ -711,7 +745,138 @@ int bnx_volume_destroy(bnx_volume v)
   return BNX_OK;
 }
 
+int bnx_volume_create_umesh(bnx_device d, const bnx_umesh_desc *f, const bnx_transfer_function *tf, int32_t user_id,
+                            bnx_volume *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && f && tf && out && f->vertices && f->scalars && f->indices && f->cell_offsets, BNX_ERR_INVALID,
+            "bnx_volume_create_umesh: null argument");
+  BNX_CHECK(f->num_vertices > 0 && f->num_cells > 0 && f->num_indices > 0, BNX_ERR_INVALID,
+            "bnx_volume_create_umesh: empty mesh");
+  const bool auto_mc = f->mc_dims[0] == 0 && f->mc_dims[1] == 0 && f->mc_dims[2] == 0;
+  BNX_CHECK(auto_mc || (f->mc_dims[0] > 0 && f->mc_dims[1] > 0 && f->mc_dims[2] > 0 && f->mc_dims[0] <= 1024 &&
+                        f->mc_dims[1] <= 1024 && f->mc_dims[2] <= 1024),
+            BNX_ERR_INVALID, "bnx_volume_create_umesh: macrocell dims must be all 0 or all in 1..1024");
+  const int nc = f->num_cells, nv = f->num_vertices;
+  std::vector<int4> tets4((size_t)nc);
+  for (int c = 0; c < nc; c++) {
+    if (f->cell_types) {
+      BNX_CHECK(f->cell_types[c] == BNX_CELL_TET || f->cell_types[c] == BNX_CELL_VTK_TET, BNX_ERR_UNSUPPORTED,
+                "bnx_volume_create_umesh: only tetrahedral cells are provided");
+    }
+    const long long o = f->cell_offsets[c];
+    BNX_CHECK(o >= 0 && o + 4 <= (long long)f->num_indices, BNX_ERR_INVALID,
+              "bnx_volume_create_umesh: cell offset out of range");
+    const int32_t *ix = f->indices + o;
+    for (int q = 0; q < 4; q++)
+      BNX_CHECK(ix[q] >= 0 && ix[q] < nv, BNX_ERR_INVALID, "bnx_volume_create_umesh: vertex index out of range");
+    tets4[c] = make_int4(ix[0], ix[1], ix[2], ix[3]);
+  }
+  std::vector<float4> verts4((size_t)nv);
+  for (int i = 0; i < nv; i++)
+    verts4[i] = make_float4(f->vertices[3 * (size_t)i], f->vertices[3 * (size_t)i + 1],
+                            f->vertices[3 * (size_t)i + 2], f->scalars[i]);
+  bnx_volume v = new bnx_volume_s;
+  v->dev = d;
+  VolumeDev &dd = v->dd;
+  dd.field = FIELD_UMESH;
+  // computeWorldBounds (UMeshField.cu:304-318): union of the cells' boxes
+  float lo[3] = {INFINITY, INFINITY, INFINITY}, hi[3] = {-INFINITY, -INFINITY, -INFINITY};
+  for (int c = 0; c < nc; c++) {
+    const int q4[4] = {tets4[c].x, tets4[c].y, tets4[c].z, tets4[c].w};
+    for (int q = 0; q < 4; q++) {
+      const float4 p = verts4[q4[q]];
+      const float pc[3] = {p.x, p.y, p.z};
+      for (int k = 0; k < 3; k++) {
+        lo[k] = std::min(lo[k], pc[k]);
+        hi[k] = std::max(hi[k], pc[k]);
+      }
+    }
+  }
+  // buildInitialMacroCells (UMeshField.cu:168-197), host float math
+  const float size[3] = {hi[0] - lo[0], hi[1] - lo[1], hi[2] - lo[2]};
+  if (auto_mc) {
+    const float maxWidth = std::max(std::max(size[0], size[1]), size[2]);
+    const int G = 200 + (int)sqrtf((float)nc / 1000.f);
+    const float s = (float)(G - 1) / maxWidth;
+    for (int k = 0; k < 3; k++) dd.mc[k] = 1 + (int)(size[k] * s);
+  } else {
+    for (int k = 0; k < 3; k++) dd.mc[k] = f->mc_dims[k];
+  }
+  for (int k = 0; k < 3; k++) {
+    dd.lo[k] = lo[k];
+    dd.hi[k] = hi[k];
+    dd.mc_origin[k] = lo[k];
+    const float spacing = size[k] * (1.f / (float)dd.mc[k]);
+    dd.mc_rcp_spacing[k] = 1.f / spacing;
+  }
+  dd.user_id = user_id;
+  int rc;
+  if ((rc = set_tf(v, tf)) != BNX_OK || (rc = volume_build_umesh(v, tets4.data(), nc, verts4.data(), nv)) != BNX_OK) {
+    delete v;
+    return rc;
+  }
+  v->version++;
+  *out = v;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_volume_mc_ranges(bnx_volume v, int32_t dims[3], float *lo, float *hi)
+{
+  BNX_CHECK(v && dims, BNX_ERR_INVALID, "bnx_volume_mc_ranges: null argument");
+  for (int k = 0; k < 3; k++) dims[k] = v->dd.mc[k];
+  const size_t n = (size_t)dims[0] * dims[1] * dims[2];
+  if (lo || hi) {
+    std::vector<float2> r(n);
+    BNX_HIP(hipMemcpy(r.data(), v->ranges.ptr, sizeof(float2) * n, hipMemcpyDeviceToHost));
+    for (size_t i = 0; i < n; i++) {
+      if (lo) lo[i] = r[i].x;
+      if (hi) hi[i] = r[i].y;
+    }
+  }
+  return BNX_OK;
+}
+
+int bnx_isosurface_create(bnx_volume field, float iso_value, const bnx_material *m, int32_t user_id,
+                          bnx_isosurface *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(field && out, BNX_ERR_INVALID, "bnx_isosurface_create: null argument");
+  BNX_CHECK(!m || (m->type == BNX_MATERIAL_MATTE && m->opacity >= 1.f), BNX_ERR_UNSUPPORTED,
+            "bnx_isosurface_create: only opaque matte materials are provided");
+  bnx_isosurface s = new bnx_isosurface_s;
+  s->dev = field->dev;
+  s->field = field;
+  s->iso = iso_value;
+  const float def[3] = {0.8f, 0.8f, 0.8f};
+  for (int k = 0; k < 3; k++) s->color[k] = m ? m->color[k] : def[k];
+  s->user_id = user_id;
+  *out = s;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_isosurface_destroy(bnx_isosurface s)
+{
+  BNX_CHECK(s, BNX_ERR_INVALID, "bnx_isosurface_destroy: null isosurface");
+  delete s;
+  return BNX_OK;
+}
+
 // ------------------------------------------------------------------- world
+int bnx_world_add_isosurface(bnx_world w, bnx_isosurface s, int32_t user_inst_id)
+{
+  GUARD_BEGIN
+  BNX_CHECK(w && s, BNX_ERR_INVALID, "bnx_world_add_isosurface: null argument");
+  BNX_CHECK(w->dev == s->dev, BNX_ERR_INVALID, "bnx_world_add_isosurface: isosurface lives on another device");
+  w->isos.push_back(s);
+  w->iso_inst_ids.push_back(user_inst_id);
+  w->iso_seen.clear();
+  return BNX_OK;
+  GUARD_END
+}
+
 int bnx_world_add_volume(bnx_world w, bnx_volume v, int32_t user_inst_id)
 {
   GUARD_BEGIN
@@ -719,7 +884,8 @@ int bnx_world_add_volume(bnx_world w, bnx_volume v, int32_t user_inst_id)
   BNX_CHECK(w->dev == v->dev, BNX_ERR_INVALID, "bnx_world_add_volume: volume lives on another device");
   w->volumes.push_back(v);
   w->volume_inst_ids.push_back(user_inst_id);
-  w->volumes_dirty = true;
+  w->vol_seen.clear();
+  w->iso_seen.clear();                 // isosurface rtc instances follow the volumes
   return BNX_OK;
   GUARD_END
 }

@@ -214,7 +214,8 @@ BNX_D uint32_t wave_sum(uint32_t x)
 // the per-wave atomics of a launch do not serialise on one L2 line; the host
 // sums the slots.
 constexpr int STAT_SLOTS = 256;
-enum { STAT_SHADOW = 0, STAT_NODES, STAT_TRIS, STAT_SPHERES, STAT_SAMPLES, STAT_CELLS, STAT_COUNT };
+enum { STAT_SHADOW = 0, STAT_NODES, STAT_TRIS, STAT_SPHERES, STAT_SAMPLES, STAT_CELLS, STAT_UNODES, STAT_UTETS,
+       STAT_COUNT };
 // call with the wave-wide total from lane 0
 BNX_D void stat_add(unsigned long long *stats, int idx, uint32_t v)
 {

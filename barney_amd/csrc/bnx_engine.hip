@@ -120,7 +120,8 @@ struct GenTiming {
 // Enqueue one generation over queue `cur`: upload the segments, build the
 // block maps, trace, volume march, shade into queue 1 - cur.  The caller
 // has uploaded the output region bases and cleared the output counters.
-int enqueue_generation(Device *dev, Wavefront &F, int cur, const SegList &sl, bool surfaces, bool volumes)
+int enqueue_generation(Device *dev, Wavefront &F, int cur, const SegList &sl, bool surfaces, bool volumes,
+                       bool full)
 {
   Queues &Q = F.Q;
   RayQueue in = Q.q(cur), out = Q.q(1 - cur);
@@ -140,7 +141,7 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, const SegList &sl, bo
   BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
   if (volumes) {
     BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
-    if ((rc = launch_volume_march(dev, slots, sv, sl.blocks256, in, trace, st)) != BNX_OK) return rc;
+    if ((rc = launch_volume_march(dev, slots, sv, sl.blocks256, in, trace, full, st)) != BNX_OK) return rc;
     BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
   }
   const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
@@ -184,7 +185,8 @@ Wavefront &wavefront_for(Device *dev)
   return *(Wavefront *)dev->queues;
 }
 
-int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, SlotDev &s)
+// full: the world has isosurfaces or tet volumes (the march kernel's full form)
+int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, SlotDev &s, bool &full)
 {
   VolumeList vl{};
   int rc = world_volumes(w, vl);
@@ -193,6 +195,9 @@ int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f,
   s.scene = w->instances->scene;
   s.vols = vl.vols;
   s.num_vols = vl.count;
+  if ((rc = world_isos(w, s.isos, s.num_isos)) != BNX_OK) return rc;
+  full = s.num_isos > 0;
+  for (const Volume *v : w->volumes) full = full || v->dd.field != FIELD_STRUCTURED;
   BNX_SET(s.dir_lights, w->d_dir_lights.ptr);
   s.num_dir_lights = (int)w->dir_lights.size();
   s.rr = rr;
@@ -228,9 +233,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
   if ((rc = F.ensure((size_t)(2 * np * S), 1, 1, 1)) != BNX_OK) return rc;
   BNX_HIP(hipMemsetAsync(F.Q.stats.ptr, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
   SlotDev *hs = F.h_slots.as<SlotDev>();
-  if ((rc = fill_slot(w, cam, rr, f, *hs)) != BNX_OK) return rc;
+  bool full = false;
+  if ((rc = fill_slot(w, cam, rr, f, *hs, full)) != BNX_OK) return rc;
   const bool surfaces = hs->scene.num_insts > 0;
-  const bool volumes = hs->num_vols > 0;
+  const bool volumes = hs->num_vols > 0 || hs->num_isos > 0;
   const bool traced = surfaces || !volumes;
   RaygenChunk *hc = F.h_chunks.as<RaygenChunk>();
   int *hb = F.h_out_base.as<int>();
@@ -254,7 +260,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       SegList sl{F.h_segs.as<Segment>()};
       sl.add(0, 0, count);
       BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, 2 * sizeof(int), dev->stream));
-      if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes)) != BNX_OK) return rc;
+      if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes, full)) != BNX_OK) return rc;
       BNX_HIP(hipMemcpyAsync(hcnt, F.out_cnt.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, dev->stream));
       BNX_HIP(hipStreamSynchronize(dev->stream));
       if ((rc = collect_timing(F, count, traced, volumes, tm)) != BNX_OK) return rc;
@@ -306,7 +312,7 @@ struct Request {
   int live_rays = 0, live_paths = 0;
   int added = 0;                   // primary rays added in this step
   double submit_ms = -1.0, admit_ms = -1.0, done_ms = -1.0;
-  bool surfaces = false, volumes = false;
+  bool surfaces = false, volumes = false, full = false;
 };
 
 } // namespace
@@ -397,12 +403,12 @@ struct Server {
       r->slot = free_slots.back();
       free_slots.pop_back();
       SlotDev *hs = F.h_slots.as<SlotDev>() + r->slot;
-      if ((rc = fill_slot(r->world, r->cam, r->rr, f, *hs)) != BNX_OK) return rc;
+      if ((rc = fill_slot(r->world, r->cam, r->rr, f, *hs, r->full)) != BNX_OK) return rc;
       hs->accum_base = 0;
       BNX_HIP(hipMemcpyAsync(F.slots.as<SlotDev>() + r->slot, hs, sizeof(SlotDev), hipMemcpyHostToDevice,
                              dev->stream));
       r->surfaces = hs->scene.num_insts > 0;
-      r->volumes = hs->num_vols > 0;
+      r->volumes = hs->num_vols > 0 || hs->num_isos > 0;
       r->state = BNX_REQ_ACTIVE;
       r->admit_ms = now_ms();
       active.push_back(r);
@@ -460,13 +466,14 @@ struct Server {
     // output regions in the other queue
     int *hb = F.h_out_base.as<int>();
     long long out_end = 0;
-    bool surfaces = false, volumes = false;
+    bool surfaces = false, volumes = false, full = false;
     int max_slot = 0;
     for (Request *r : active) {
       hb[r->slot] = (int)out_end;
       out_end += 2LL * (r->live_paths + r->added);
       surfaces |= r->surfaces;
       volumes |= r->volumes;
+      full |= r->full;
       max_slot = std::max(max_slot, r->slot + 1);
     }
     BNX_HIP(hipMemcpyAsync(F.out_base.ptr, hb, sizeof(int) * max_slot, hipMemcpyHostToDevice, dev->stream));
@@ -477,7 +484,7 @@ struct Server {
                               F.Q.q(cur))) != BNX_OK)
         return rc;
     }
-    if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes)) != BNX_OK) return rc;
+    if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes, full)) != BNX_OK) return rc;
     int *hcnt = F.h_out_cnt.as<int>();
     BNX_HIP(hipMemcpyAsync(hcnt, F.out_cnt.ptr, sizeof(int) * 2 * max_slot, hipMemcpyDeviceToHost, dev->stream));
     BNX_HIP(hipStreamSynchronize(dev->stream));

@@ -130,6 +130,7 @@ struct Group {
 };
 
 struct Volume;
+struct IsoSurface;
 struct World {
   Device *dev;
   Group *instances;
@@ -138,7 +139,12 @@ struct World {
   std::vector<Volume *> volumes;
   std::vector<int> volume_inst_ids;
   DevMem d_volumes;           // VolumeDev[volumes.size()]
-  bool volumes_dirty = true;
+  std::vector<IsoSurface *> isos;
+  std::vector<int> iso_inst_ids;
+  DevMem d_isos;              // IsoDev[isos.size()]
+  // versions of the volumes (and isosurface fields) last uploaded; empty
+  // until the first upload
+  std::vector<uint64_t> vol_seen, iso_seen;
 };
 
 struct Frame {

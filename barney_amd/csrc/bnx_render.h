@@ -15,7 +15,7 @@ struct RayQueue {
                         // generation << 24; z: pixel; w: misWeight
   int4 *hit;            // prim, geom, inst, bits(u)
   float *hit_v;
-  float4 *vcol;         // rgb of a volume event
+  float4 *vcol;         // rgb of a volume event, or the normal of an isosurface hit
 };
 
 // ray flags in dir.w: bit 0 shadow ray, bits 4..15 sample index inside the
@@ -36,6 +36,7 @@ struct FrameDev {
 };
 
 struct VolumeDev;
+struct IsoDev;
 // One request in flight: everything its rays need to be generated, traced,
 // marched, shaded and accumulated.  Rays of different requests (different
 // worlds, cameras, resolutions, frames) share one queue and one launch per
@@ -43,8 +44,11 @@ struct VolumeDev;
 struct SlotDev {
   SceneDev scene;
   const VolumeDev BNX_G *vols; // world volumes (volume k: hit geom VOLUME_GEOM_BASE + k)
+  const IsoDev BNX_G *isos;    // world isosurfaces (isosurface k: hit geom ISO_GEOM_BASE + k)
   int num_vols;
+  int num_isos;
   int num_dir_lights;
+  int pad0;
   const bnx_dir_light BNX_G *dir_lights;
   bnx_renderer rr;
   bnx_camera cam;

@@ -295,6 +295,20 @@ BNX_D SurfHit eval_volume_hit(float t, float4 rgb, v3 rorg, v3 rdir)
   return sh;
 }
 
+// MCIsoSurfaceAccel::isProg hit (volume/MCAccelerator.h:493-547): opaque
+// matte at the crossing, normal = the normalized field gradient (fp16 in the
+// ray, like every surface normal)
+BNX_D SurfHit eval_iso_hit(float t, float4 n, v3 rorg, v3 rdir, const IsoDev &iso)
+{
+  SurfHit sh;
+  v3 d = fix_dir(rdir);
+  sh.P = rorg + t * d;
+  sh.N = quant_half(v3{n.x, n.y, n.z});
+  sh.albedo = (0.85f * v3{iso.color[0], iso.color[1], iso.color[2]}) * ONE_OVER_PI;
+  sh.phase = false;
+  return sh;
+}
+
 // primary-ray miss color (generateRays.cu:205-216): background or gradient
 BNX_D float4 miss_color(const SlotDev &P, int pixel)
 {
@@ -364,8 +378,10 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
     const bool shadow_hit = hit.x >= 0;
 
     SurfHit sh{};
-    const bool vol_hit = had_hit && hit.y >= VOLUME_GEOM_BASE;
+    const bool vol_hit = had_hit && hit.y >= VOLUME_GEOM_BASE && hit.y < ISO_GEOM_BASE;
+    const bool iso_hit = had_hit && hit.y >= ISO_GEOM_BASE;
     if (vol_hit) sh = eval_volume_hit(tmax, in.vcol[i], rorg, rdir);
+    else if (iso_hit) sh = eval_iso_hit(tmax, in.vcol[i], rorg, rdir, P.isos[hit.y - ISO_GEOM_BASE]);
     else if (had_hit) sh = eval_hit(S, hit, hv, rorg, rdir);
 
     const float4 mc = miss_color(P, pixel);
@@ -503,7 +519,11 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
       if (accumID == 0) {
         fb.prim[pixel] = hit.x >= 0 ? hit.x : -1;
         int inst = -1, obj = -1;
-        if (vol_hit) {
+        if (iso_hit) {
+          const IsoDev &I = P.isos[hit.y - ISO_GEOM_BASE];
+          inst = I.user_inst;
+          obj = I.user_id;
+        } else if (vol_hit) {
           const VolumeDev &V = P.vols[hit.y - VOLUME_GEOM_BASE];
           inst = V.user_inst;
           obj = V.user_id;

@@ -1,4 +1,5 @@
-// bnx_volume.h -- structured-volume device data and the per-ray march.
+// bnx_volume.h -- volume and isosurface device data (structured and
+// tetrahedral fields) and the per-ray march.
 #pragma once
 
 #include "bnx_internal.h"
@@ -6,25 +7,55 @@
 
 namespace bnx {
 
-// hit-record geometry index of a volume event: VOLUME_GEOM_BASE + volume slot
+// hit-record geometry index of a volume event: VOLUME_GEOM_BASE + volume
+// slot; of an isosurface hit: ISO_GEOM_BASE + isosurface slot
 constexpr int VOLUME_GEOM_BASE = 0x40000000;
+constexpr int ISO_GEOM_BASE = 0x60000000;
+
+enum { FIELD_STRUCTURED = 0, FIELD_UMESH = 1 };
+
+// element-BVH point-query stack: a ring of PQ_SHORT entries per thread in
+// LDS ([entry][thread], 256 threads per block) spilling to PQ_DEEP private
+// entries; a tet mesh whose element BVH is deeper is rejected at creation
+constexpr int PQ_TB = 256;
+constexpr int PQ_SHORT = 16;
+constexpr int PQ_DEEP = 48;
 
 struct VolumeDev {
+  int field;                  // FIELD_STRUCTURED / FIELD_UMESH
+  // structured (StructuredData, volume/StructuredData.h:30-74)
   const float BNX_G *scalars;   // dims[0]*dims[1]*dims[2], x fastest
   int dims[3];
   int ncells[3];
   float origin[3];
   float rcp_spacing[3];       // 1/gridSpacing (host float division)
-  float lo[3], hi[3];         // world bounds (StructuredData::commit)
+  // tets (UMeshField, umesh/common/UMeshField.h:52-116)
+  const float4 BNX_G *uverts;   // position, scalar
+  const int4 BNX_G *utets;      // vertex indices, in element-BVH slot order
+  const Node BNX_G *ebvh;       // element BVH
+  // both
+  float lo[3], hi[3];         // world bounds
   int mc[3];
   int cells_per_mc;
   float mc_origin[3];
-  float mc_rcp_spacing[3];    // 1/(cells_per_mc * gridSpacing)
+  float mc_rcp_spacing[3];    // 1 / macrocell spacing (host float division)
+  const float2 BNX_G *ranges;   // macrocell value ranges (lo, hi)
   const float BNX_G *majorants;
   const float4 BNX_G *tf;
   int ntf;
   float dom0, dom1, base_density;
   int rtc_inst;               // rtc instance index (seeds the march RNG)
+  int user_inst;
+  int user_id;
+};
+
+// IsoSurface::DD (geometry/IsoSurface.h:31-42) with its field and an
+// opaque matte material
+struct IsoDev {
+  VolumeDev field;
+  float iso;
+  float color[3];
+  int rtc_inst;
   int user_inst;
   int user_id;
 };
@@ -36,19 +67,36 @@ struct VolumeList {
 
 struct Volume {
   Device *dev;
+  uint64_t version = 0;         // bumped whenever dd changes
   VolumeDev dd{};
   DevMem scalars, ranges, majorants, tf;
+  DevMem uverts, utets;
+  Bvh ebvh;                     // element BVH (tets)
+};
+
+struct IsoSurface {
+  Device *dev;
+  Volume *field;
+  float iso;
+  float color[3];
+  int user_id;
 };
 
 int volume_build(Volume *v);
+// tets: element BVH, macrocell ranges by rasterTet<5>, majorants; tets4 is
+// the host copy of the tets (num_tets x int4), verts4 of the vertices
+int volume_build_umesh(Volume *v, const int4 *tets4, int num_tets, const float4 *verts4, int num_verts);
 int volume_rebuild_majorants(Volume *v);
-// upload the world's volume table (if changed) and return it
+// upload the world's volume and isosurface tables (if changed) and return them
 int world_volumes(World *w, VolumeList &vl);
+int world_isos(World *w, const struct IsoDev BNX_G *&isos, int &count);
 // march every ray of the generation's segments (blocks256 blocks of 256
-// threads); surfaces: false when the trace kernel did not run, so the hit
-// records are not written yet
+// threads) through its request's isosurfaces and volumes; surfaces: false
+// when the trace kernel did not run, so the hit records are not written yet
+// full: some request of the batch has isosurfaces or tet volumes (else
+// every volume is structured and a leaner kernel runs)
 int launch_volume_march(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, const RayQueue &q,
-                        bool surfaces, unsigned long long *stats);
+                        bool surfaces, bool full, unsigned long long *stats);
 
 // ---------------------------------------------------------------- device
 BNX_D uint64_t fnv3(uint32_t a, uint32_t b, uint32_t c)
@@ -73,8 +121,16 @@ BNX_D void lerp_coord(int N, float tc, int &i0, int &i1, float &f)
 
 BNX_D float lerp_l(float f, float a, float b) { return (1.f - f) * a + f * b; }
 
+// per-thread march state: point-query stack and work counters (samples:
+// Woodcock density evaluations; cells: macrocells stepped; unodes/utets:
+// element-BVH nodes and tets tested by point queries)
+struct MarchCtx {
+  uint32_t *ring;             // &ring[0][thread]
+  uint32_t samples, cells, unodes, utets;
+};
+
 // StructuredDataSampler::DD::sample (volume/StructuredData.h:101-112)
-BNX_D float volume_sample(const VolumeDev &v, v3 P)
+BNX_D float structured_sample(const VolumeDev &v, v3 P)
 {
   float rx = (P.x - v.origin[0]) * v.rcp_spacing[0];
   float ry = (P.y - v.origin[1]) * v.rcp_spacing[1];
@@ -87,11 +143,11 @@ BNX_D float volume_sample(const VolumeDev &v, v3 P)
   lerp_coord(v.dims[1], ry + .5f, y0, y1, fy);
   lerp_coord(v.dims[2], rz + .5f, z0, z1, fz);
   const size_t Nx = (size_t)v.dims[0], NxNy = (size_t)v.dims[0] * (size_t)v.dims[1];
-  const float *sc = v.scalars;
-  const float *b00 = sc + (size_t)y0 * Nx + (size_t)z0 * NxNy;
-  const float *b01 = sc + (size_t)y1 * Nx + (size_t)z0 * NxNy;
-  const float *b10 = sc + (size_t)y0 * Nx + (size_t)z1 * NxNy;
-  const float *b11 = sc + (size_t)y1 * Nx + (size_t)z1 * NxNy;
+  const float BNX_G *sc = v.scalars;
+  const float BNX_G *b00 = sc + (size_t)y0 * Nx + (size_t)z0 * NxNy;
+  const float BNX_G *b01 = sc + (size_t)y1 * Nx + (size_t)z0 * NxNy;
+  const float BNX_G *b10 = sc + (size_t)y0 * Nx + (size_t)z1 * NxNy;
+  const float BNX_G *b11 = sc + (size_t)y1 * Nx + (size_t)z1 * NxNy;
   float v00 = lerp_l(fx, b00[x0], b00[x1]);
   float v01 = lerp_l(fx, b01[x0], b01[x1]);
   float v10 = lerp_l(fx, b10[x0], b10[x1]);
@@ -99,6 +155,86 @@ BNX_D float volume_sample(const VolumeDev &v, v3 P)
   float v0 = lerp_l(fy, v00, v01);
   float v1 = lerp_l(fy, v10, v11);
   return lerp_l(fz, v0, v1);
+}
+
+// evalToImplicitPlane (umesh/common/UMeshField.h:365-371)
+BNX_D float implicit_plane(v3 P, float4 a, float4 b, float4 c)
+{
+  const v3 N = cross(xyz(b) - xyz(a), xyz(c) - xyz(a));
+  return dot(P - xyz(a), N);
+}
+
+// UMeshField::DD::tetScalar (umesh/common/UMeshField.h:391-436)
+BNX_D bool tet_scalar(const VolumeDev &v, int slot, v3 P, float &out)
+{
+  const int4 t = v.utets[slot];
+  const float4 v0 = v.uverts[t.x], v1 = v.uverts[t.y], v2 = v.uverts[t.z], v3v = v.uverts[t.w];
+  const float t3 = implicit_plane(P, v0, v1, v2);
+  if (t3 < 0.f) return false;
+  const float t2 = implicit_plane(P, v0, v3v, v1);
+  if (t2 < 0.f) return false;
+  const float t1 = implicit_plane(P, v0, v2, v3v);
+  if (t1 < 0.f) return false;
+  const float t0 = implicit_plane(P, v1, v3v, v2);
+  if (t0 < 0.f) return false;
+  const float scale = 1.f / (((t0 + t1) + t2) + t3);
+  out = scale * (((t0 * v0.w + t1 * v1.w) + t2 * v2.w) + t3 * v3v.w);
+  return true;
+}
+
+BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz, v3 P)
+{
+  return lx <= P.x && P.x <= hx && ly <= P.y && P.y <= hy && lz <= P.z && P.z <= hz;
+}
+
+// UMeshCuBQLSampler::DD::sample (umesh/mc/UMeshCuBQLSampler.h:46-62): a
+// fixed-box query with the point P (closed box tests), depth first with
+// child 0 before child 1, leaf slots in order; the first tet that contains
+// P gives the value, NaN if none does
+BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
+{
+  uint32_t deep[PQ_DEEP];
+  int sp = 0;
+  uint32_t cur = 0;
+  while (true) {
+    if (!(cur & LEAF_BIT)) {
+      const float4 BNX_G *np = (const float4 BNX_G *)(v.ebvh + cur);
+      const float4 a = np[0], b = np[1], c = np[2];
+      const int4 r = ((const int4 BNX_G *)np)[3];
+      cx.unodes++;
+      const bool h0 = r.z >= 0 && box_holds(a.x, a.y, a.z, a.w, b.x, b.y, P);
+      const bool h1 = r.w >= 0 && box_holds(b.z, b.w, c.x, c.y, c.z, c.w, P);
+      if (h0 && h1) {
+        // push child 1
+        const int s = sp & (PQ_SHORT - 1);
+        if (sp >= PQ_SHORT) deep[sp - PQ_SHORT] = cx.ring[s * PQ_TB];
+        cx.ring[s * PQ_TB] = enc_child(r.y, r.w);
+        sp++;
+        cur = enc_child(r.x, r.z);
+        continue;
+      }
+      if (h0) { cur = enc_child(r.x, r.z); continue; }
+      if (h1) { cur = enc_child(r.y, r.w); continue; }
+    } else {
+      const int first = (int)((cur & ~LEAF_BIT) >> 3);
+      const int count = (int)(cur & 7u);
+      for (int k = 0; k < count; k++) {
+        float f;
+        cx.utets++;
+        if (tet_scalar(v, first + k, P, f)) return f;
+      }
+    }
+    if (sp == 0) return __builtin_nanf("");
+    sp--;
+    const int s = sp & (PQ_SHORT - 1);
+    cur = cx.ring[s * PQ_TB];
+    if (sp >= PQ_SHORT) cx.ring[s * PQ_TB] = deep[sp - PQ_SHORT];
+  }
+}
+
+BNX_D float volume_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
+{
+  return v.field == FIELD_UMESH ? umesh_sample(v, P, cx) : structured_sample(v, P);
 }
 
 // TransferFunction::DD::map (volume/TransferFunction.h:67-87)
@@ -111,7 +247,7 @@ BNX_D float4 tf_map(const VolumeDev &v, float s)
   int idx = (int)f;
   idx = idx < 0 ? 0 : (idx > N - 2 ? N - 2 : idx);
   f -= (float)idx;
-  const float4 *tf = v.tf;
+  const float4 BNX_G *tf = v.tf;
   float4 a = tf[idx], b = tf[idx + 1];
   float4 r;
   r.x = (1.f - f) * a.x + f * b.x;
@@ -139,7 +275,7 @@ BNX_D float tf_majorant(const VolumeDev &v, float lo, float hi)
   i_hi = i_hi < 0 ? 0 : (i_hi > N - 2 ? N - 2 : i_hi);
   f_lo -= (float)i_lo;
   f_hi -= (float)i_hi;
-  const float4 *tf = v.tf;
+  const float4 BNX_G *tf = v.tf;
   float l_lo = (1.f - f_lo) * tf[i_lo].w + f_lo * tf[i_lo + 1].w;
   float l_hi = (1.f - f_hi) * tf[i_hi].w + f_hi * tf[i_hi + 1].w;
   float m = 0.f;
@@ -162,36 +298,27 @@ BNX_D float fast_log(float f)
   return s + s;
 }
 
-// MCVolumeAccel::isProg (volume/MCAccelerator.h:550-689): box clip, RNG
-// child seeded by hash(rtc instance, geometry 0, 0), dda3 over macrocells,
-// Woodcock sampling per non-empty cell.  'dir' is the fixed trace direction.
-BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, float &t_out,
-                        float rgb[3], uint32_t &nsamples, uint32_t &ncells)
+// boxTest (render/Ray.h:169-181): clip [r0, r1] to the field's bounds
+BNX_D bool clip_bounds(const VolumeDev &v, v3 org, v3 dir, float &r0, float &r1)
 {
-  // boxTest (render/Ray.h:169-217)
-  float r0 = tmin, r1 = tmax;
-  {
-    v3 rd = v3{1.f / dir.x, 1.f / dir.y, 1.f / dir.z};
-    float tl[3] = {(v.lo[0] - org.x) * rd.x, (v.lo[1] - org.y) * rd.y, (v.lo[2] - org.z) * rd.z};
-    float th[3] = {(v.hi[0] - org.x) * rd.x, (v.hi[1] - org.y) * rd.y, (v.hi[2] - org.z) * rd.z};
-    float nr = fmx(fmx(fmn(tl[0], th[0]), fmn(tl[1], th[1])), fmn(tl[2], th[2]));
-    float fr = fmn(fmn(fmx(tl[0], th[0]), fmx(tl[1], th[1])), fmx(tl[2], th[2]));
-    r0 = fmx(r0, nr);
-    r1 = fmn(r1, fr);
-    if (!(r0 <= r1)) return false;
-  }
+  v3 rd = v3{1.f / dir.x, 1.f / dir.y, 1.f / dir.z};
+  float tl[3] = {(v.lo[0] - org.x) * rd.x, (v.lo[1] - org.y) * rd.y, (v.lo[2] - org.z) * rd.z};
+  float th[3] = {(v.hi[0] - org.x) * rd.x, (v.hi[1] - org.y) * rd.y, (v.hi[2] - org.z) * rd.z};
+  float nr = fmx(fmx(fmn(tl[0], th[0]), fmn(tl[1], th[1])), fmn(tl[2], th[2]));
+  float fr = fmn(fmn(fmx(tl[0], th[0]), fmx(tl[1], th[1])), fmx(tl[2], th[2]));
+  r0 = fmx(r0, nr);
+  r1 = fmn(r1, fr);
+  return r0 <= r1;
+}
+
+// dda3 (volume/DDA.h:60-201): walks the macrocells of v along the ray
+// (macrocell-grid space) from max(0, grid entry) to min(dda_tmax, grid
+// exit), calling fn(cell index, t0, t1); stops when fn returns false
+template <typename F> BNX_D void dda3(const VolumeDev &v, v3 org, v3 dir, float dda_tmax, F &&fn)
+{
   const v3 dorg = v3{(org.x - v.mc_origin[0]) * v.mc_rcp_spacing[0], (org.y - v.mc_origin[1]) * v.mc_rcp_spacing[1],
                      (org.z - v.mc_origin[2]) * v.mc_rcp_spacing[2]};
   const v3 ddir = v3{dir.x * v.mc_rcp_spacing[0], dir.y * v.mc_rcp_spacing[1], dir.z * v.mc_rcp_spacing[2]};
-  // Random(ray.rngSeed, hash(inst, geom, 0)) -- barney-common.h:59-61
-  uint64_t rs = parent;
-  rng_next(parent);
-  rng_next_h(rs, fnv3((uint32_t)v.rtc_inst, 0u, 0u));
-  auto rnd = [&]() -> float { return (float)(rng_next(rs) & 0x00FFFFFFu) * (1.f / (float)0x01000000); };
-  const float ray_tmax = tmax;
-
-  // dda3 (volume/DDA.h:60-201)
-  const float dda_tmax = r1;
   const float fs[3] = {(float)v.mc[0], (float)v.mc[1], (float)v.mc[2]};
   const float o3[3] = {dorg.x, dorg.y, dorg.z}, d3[3] = {ddir.x, ddir.y, ddir.z};
   float rcp[3], tn[3], tf[3];
@@ -202,14 +329,14 @@ BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
     tn[k] = fmn(a, b);
     tf[k] = fmx(a, b);
     if (d3[k] == 0.f) {
-      if (o3[k] < 0.f || o3[k] > fs[k]) return false;
+      if (o3[k] < 0.f || o3[k] > fs[k]) return;
       tn[k] = -INFINITY;
       tf[k] = INFINITY;
     }
   }
   float ray_t0 = fmx(0.f, fmx(fmx(tn[0], tn[1]), tn[2]));
   float ray_t1 = fmn(dda_tmax, fmn(fmn(tf[0], tf[1]), tf[2]));
-  if (ray_t0 > ray_t1) return false;
+  if (ray_t0 > ray_t1) return;
   float t_step[3], t_next[3];
   int stop[3], delta[3], cell[3];
   for (int k = 0; k < 3; k++) {
@@ -227,47 +354,146 @@ BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
     const float t_closest = fmn(fmn(t_next[0], t_next[1]), t_next[2]);
     const float c0 = ray_t0 + next_begin;
     const float c1 = ray_t0 + fmn(t_closest, dda_tmax);
-    const float maj = v.majorants[(size_t)cell[0] + (size_t)v.mc[0] * ((size_t)cell[1] + (size_t)v.mc[1] * cell[2])];
-    ncells++;
-    if (maj != 0.f) {
-      // Woodcock::sampleRange (volume/Volume.h:168-203)
-      const float upper = fmn(c1, ray_tmax);
-      float t = c0;
-      while (true) {
-        float r = rnd();
-        float dt = -fast_log(1.f - r) / maj;
-        t += dt;
-        if (t >= upper) break;
-        v3 P = org + t * dir;
-        float f = volume_sample(v, P);
-        nsamples++;
-        float4 s = isnan(f) ? make_float4(0.f, 0.f, 0.f, 0.f) : tf_map(v, f);
-        if (s.w >= rnd() * maj) {
-          t_out = t;
-          rgb[0] = s.x;
-          rgb[1] = s.y;
-          rgb[2] = s.z;
-          return true;
-        }
-      }
-    }
+    const size_t ci = (size_t)cell[0] + (size_t)v.mc[0] * ((size_t)cell[1] + (size_t)v.mc[1] * cell[2]);
+    if (!fn(ci, c0, c1)) return;
     next_begin = t_closest;
     if (t_next[0] == t_closest) {
       t_next[0] += t_step[0];
       cell[0] += delta[0];
-      if (cell[0] == stop[0]) return false;
+      if (cell[0] == stop[0]) return;
     }
     if (t_next[1] == t_closest) {
       t_next[1] += t_step[1];
       cell[1] += delta[1];
-      if (cell[1] == stop[1]) return false;
+      if (cell[1] == stop[1]) return;
     }
     if (t_next[2] == t_closest) {
       t_next[2] += t_step[2];
       cell[2] += delta[2];
-      if (cell[2] == stop[2]) return false;
+      if (cell[2] == stop[2]) return;
     }
   }
+}
+
+// the ray's RNG child for one intersection program: Random(ray.rngSeed,
+// hash(inst, geom, prim)) -- barney-common.h:59-61; advances the parent
+BNX_D uint64_t rng_child(uint64_t &parent, uint32_t rtc_inst)
+{
+  uint64_t rs = parent;
+  rng_next(parent);
+  rng_next_h(rs, fnv3(rtc_inst, 0u, 0u));
+  return rs;
+}
+
+// MCVolumeAccel::isProg (volume/MCAccelerator.h:550-689): box clip, RNG
+// child seeded by hash(rtc instance, geometry 0, 0), dda3 over macrocells,
+// Woodcock sampling per non-empty cell.  'dir' is the fixed trace direction.
+// ANY_FIELD: the field may be a tet mesh (else it is structured).
+template <bool ANY_FIELD>
+BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, float &t_out,
+                        float rgb[3], MarchCtx &cx)
+{
+  if (!ANY_FIELD && v.field != FIELD_STRUCTURED) return false;   // never launched that way
+  float r0 = tmin, r1 = tmax;
+  if (!clip_bounds(v, org, dir, r0, r1)) return false;
+  uint64_t rs = rng_child(parent, (uint32_t)v.rtc_inst);
+  auto rnd = [&]() -> float { return (float)(rng_next(rs) & 0x00FFFFFFu) * (1.f / (float)0x01000000); };
+  const float ray_tmax = tmax;
+  bool found = false;
+  dda3(v, org, dir, r1, [&](size_t ci, float c0, float c1) -> bool {
+    const float maj = v.majorants[ci];
+    cx.cells++;
+    if (maj == 0.f) return true;
+    // Woodcock::sampleRange (volume/Volume.h:168-203)
+    const float upper = fmn(c1, ray_tmax);
+    float t = c0;
+    while (true) {
+      float r = rnd();
+      float dt = -fast_log(1.f - r) / maj;
+      t += dt;
+      if (t >= upper) return true;
+      v3 P = org + t * dir;
+      float f = ANY_FIELD ? volume_sample(v, P, cx) : structured_sample(v, P);
+      cx.samples++;
+      float4 s = isnan(f) ? make_float4(0.f, 0.f, 0.f, 0.f) : tf_map(v, f);
+      if (s.w >= rnd() * maj) {
+        t_out = t;
+        rgb[0] = s.x;
+        rgb[1] = s.y;
+        rgb[2] = s.z;
+        found = true;
+        return false;
+      }
+    }
+  });
+  return found;
+}
+
+// MCIsoSurfaceAccel::isProg (volume/MCAccelerator.h:325-548): DDA over the
+// field's macrocell value ranges; in every cell whose range holds the iso
+// value, 10 linear sub-steps with a sample at each end; the first crossing
+// found before tmax ends the walk.  Normal: central differences with delta
+// = 0.1 |bounds| / (sum of grid dims).  Advances the parent RNG once when
+// the ray overlaps the field's bounds.  A crossing whose interpolation is
+// NaN (both ends exactly at the iso value) is ignored.
+BNX_D bool iso_march(const IsoDev &s, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, float &t_out,
+                     v3 &normal, MarchCtx &cx)
+{
+  const VolumeDev &v = s.field;
+  float r0 = tmin, r1 = tmax;
+  if (!clip_bounds(v, org, dir, r0, r1)) return false;
+  (void)rng_child(parent, (uint32_t)s.rtc_inst);   // drawn from only for opacity < 1
+  const float ray_tmax = tmax;
+  const float iso = s.iso;
+  float t_hit = ray_tmax;
+  dda3(v, org, dir, r1, [&](size_t ci, float t0, float t1) -> bool {
+    cx.cells++;
+    if (!(t0 < fmn(t1, ray_tmax))) return true;
+    const float2 vr = v.ranges[ci];
+    if (!(iso >= vr.x && iso <= vr.y)) return true;
+    float tt1 = t0;
+    float ff1 = volume_sample(v, org + tt1 * dir, cx);
+    for (int i = 1; i <= 10; i++) {
+      const float tt0 = tt1;
+      const float ff0 = ff1;
+      const float fi = (float)i / 10.f;
+      tt1 = (1.f - fi) * t0 + fi * t1;
+      ff1 = volume_sample(v, org + tt1 * dir, cx);
+      if (isnan(ff0) || isnan(ff1)) continue;
+      const float lo = fmn(ff0, ff1), hi = fmx(ff0, ff1);
+      if (iso >= lo && iso <= hi) {
+        float t = (iso - ff0) / (ff1 - ff0);
+        t = (1.f - t) * tt0 + t * tt1;
+        if (t < t_hit) t_hit = t;
+        if (t_hit < ray_tmax) return false;
+      }
+    }
+    return true;
+  });
+  if (!(t_hit < ray_tmax)) return false;
+  const v3 P = org + t_hit * dir;
+  const v3 bs = v3{v.hi[0] - v.lo[0], v.hi[1] - v.lo[1], v.hi[2] - v.lo[2]};
+  const float delta = length(bs) * .1f / (float)(v.mc[0] + v.mc[1] + v.mc[2]);
+  const float fP = volume_sample(v, P, cx);
+  float fx0 = volume_sample(v, P + v3{-delta, 0.f, 0.f}, cx);
+  float fx1 = volume_sample(v, P + v3{+delta, 0.f, 0.f}, cx);
+  float fy0 = volume_sample(v, P + v3{0.f, -delta, 0.f}, cx);
+  float fy1 = volume_sample(v, P + v3{0.f, +delta, 0.f}, cx);
+  float fz0 = volume_sample(v, P + v3{0.f, 0.f, -delta}, cx);
+  float fz1 = volume_sample(v, P + v3{0.f, 0.f, +delta}, cx);
+  float dx = 2.f, dy = 2.f, dz = 2.f;
+  if (isnan(fx0)) { dx -= 1.f; fx0 = fP; }
+  if (isnan(fx1)) { dx -= 1.f; fx1 = fP; }
+  if (isnan(fy0)) { dy -= 1.f; fy0 = fP; }
+  if (isnan(fy1)) { dy -= 1.f; fy1 = fP; }
+  if (isnan(fz0)) { dz -= 1.f; fz0 = fP; }
+  if (isnan(fz1)) { dz -= 1.f; fz1 = fP; }
+  v3 n = v3{dx == 0.f ? 0.f : (fx1 - fx0) / dx, dy == 0.f ? 0.f : (fy1 - fy0) / dy,
+            dz == 0.f ? 0.f : (fz1 - fz0) / dz};
+  if (n == splat(0.f)) n = -normalize(dir);
+  t_out = t_hit;
+  normal = normalize(n);
+  return true;
 }
 
 } // namespace bnx

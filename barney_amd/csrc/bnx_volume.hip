@@ -1,18 +1,24 @@
-// bnx_volume.hip -- structured volumes on gfx950: macrocell value ranges,
-// majorants, and the DDA + Woodcock (delta tracking) march.
+// bnx_volume.hip -- volumes and isosurfaces on gfx950: structured and
+// tetrahedral fields, macrocell value ranges, majorants, and the DDA +
+// Woodcock (delta tracking) and isosurface march.
 //
-// Replaces MCVolumeAccel<StructuredDataSampler> (volume/MCAccelerator.h:
-// 155-195, 550-689), StructuredData_computeMCs (volume/StructuredData.cu:
-// 30-62), mapMCs (volume/MCGrid.cu:96-113), dda3 (volume/DDA.h:60-201) and
+// Replaces MCVolumeAccel (volume/MCAccelerator.h:155-195, 550-689) and
+// MCIsoSurfaceAccel (:325-548) over StructuredDataSampler and
+// UMeshCuBQLSampler, StructuredData_computeMCs (volume/StructuredData.cu:
+// 30-62), umeshRasterCells (umesh/common/UMeshField.cu:28-161), mapMCs
+// (volume/MCGrid.cu:96-113), dda3 (volume/DDA.h:60-201) and
 // Woodcock::sampleRange (volume/Volume.h:154-203).
 //
-// Design: volumes are marched in their own wavefront kernel after the
-// surface trace (one thread per queued ray, persistent waves), clipped to
-// the surface hit.  The march is long, data-dependent and texture-bound, so
-// keeping it out of the BVH traversal kernel keeps that kernel's registers
-// and divergence low.  Trilinear filtering is done in software on the raw
-// float grid (hardware texture filtering uses 8-bit fixed-point weights);
-// a 256^3 field is 64 MiB and stays resident in the 256 MiB Infinity Cache.
+// Design: isosurfaces and volumes are handled in their own wavefront kernel
+// after the surface trace (one thread per queued ray), clipped to the
+// closest hit so far.  The march is long, data-dependent and gather-bound,
+// so keeping it out of the BVH traversal kernel keeps that kernel's
+// registers and divergence low.  Structured fields are filtered in software
+// on the raw float grid (hardware texture filtering uses 8-bit fixed-point
+// weights); a 256^3 field is 64 MiB and stays resident in the 256 MiB
+// Infinity Cache.  Tet fields keep vertices as float4 (position, scalar: one
+// 16-byte load per corner) and tets as int4 in element-BVH slot order; the
+// point query's stack is a 16-entry ring per thread in LDS.
 #include "bnx_internal.h"
 #include "bnx_render.h"
 #include "bnx_volume.h"
@@ -60,19 +66,145 @@ __global__ void k_majorants(VolumeDev v, const float2 *ranges, float *maj)
   maj[tid] = tf_majorant(v, r.x, r.y);
 }
 
-// march every ray of the generation's segments through its request's
-// volumes, one ray per thread; each block lies inside one request's segment,
-// so the volume table is block-uniform.  With no surface instances in the
-// batch the trace kernel is skipped and every ray enters here as a surface
-// miss (surfaces == 0).
+// ------------------------------------------------------------------ tets
+// world-space box of every tet (element-BVH build input)
+__global__ void k_tet_boxes(const int4 *tets, const float4 *verts, int n, float4 *lo, float4 *hi)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int4 q = tets[t];
+  const float4 a = verts[q.x], b = verts[q.y], c = verts[q.z], d = verts[q.w];
+  lo[t] = make_float4(fmn(fmn(fmn(a.x, b.x), c.x), d.x), fmn(fmn(fmn(a.y, b.y), c.y), d.y),
+                      fmn(fmn(fmn(a.z, b.z), c.z), d.z), 0.f);
+  hi[t] = make_float4(fmx(fmx(fmx(a.x, b.x), c.x), d.x), fmx(fmx(fmx(a.y, b.y), c.y), d.y),
+                      fmx(fmx(fmx(a.z, b.z), c.z), d.z), 0.f);
+}
+
+// tets in element-BVH slot order
+__global__ void k_tet_sort(const int4 *tets, const int *order, int n, int4 *out)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s < n) out[s] = tets[order[s]];
+}
+
+__global__ void k_fill_ranges(int *lo, int *hi, int n)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c >= n) return;
+  lo[c] = f2o(INFINITY);
+  hi[c] = f2o(-INFINITY);
+}
+
+__global__ void k_finish_ranges(const int *lo, const int *hi, int n, float2 *ranges)
+{
+  const int c = blockIdx.x * blockDim.x + threadIdx.x;
+  if (c < n) ranges[c] = make_float2(o2f(lo[c]), o2f(hi[c]));
+}
+
+BNX_D float4 mid4(float4 a, float4 b)
+{
+  return make_float4(0.5f * (a.x + b.x), 0.5f * (a.y + b.y), 0.5f * (a.z + b.z), 0.5f * (a.w + b.w));
+}
+BNX_D float len3(float4 a, float4 b) { return length(xyz(b) - xyz(a)); }
+
+// macrocell min/max of one cell: a per-thread cache of the last cell, since
+// the 32 pieces of a tet mostly fall into the same cell; flushed with
+// ordered-int atomics (exact float min/max in any order)
+struct RangeCache {
+  int cell = -1;
+  float lo = INFINITY, hi = -INFINITY;
+  BNX_D void flush(int *rlo, int *rhi)
+  {
+    if (cell < 0) return;
+    atomicMin(rlo + cell, f2o(lo));
+    atomicMax(rhi + cell, f2o(hi));
+  }
+  BNX_D void add(int c, float l, float h, int *rlo, int *rhi)
+  {
+    if (c != cell) {
+      flush(rlo, rhi);
+      cell = c;
+      lo = l;
+      hi = h;
+    } else {
+      lo = fmn(lo, l);
+      hi = fmx(hi, h);
+    }
+  }
+};
+
+// umeshRasterCells + rasterTet<5> (UMeshField.cu:28-161): each tet is split
+// 5 times on its longest edge (ties: ab, ac, ad, bc, bd, cd in that order);
+// the box of each of the 32 pieces is rasterized with rasterBox
+// (volume/MCGrid.cuh:95-122), which skips boxes flat in any dimension.
+// Piece p is reached by taking child (p >> (4 - level)) & 1 at every level.
+__global__ void k_umesh_raster(const int4 *tets, const float4 *verts, int n, VolumeDev v, int *rlo, int *rhi)
+{
+  const int t = blockIdx.x * blockDim.x + threadIdx.x;
+  if (t >= n) return;
+  const int4 q = tets[t];
+  const float4 A = verts[q.x], B = verts[q.y], Cc = verts[q.z], D = verts[q.w];
+  RangeCache cache;
+  for (int p = 0; p < 32; p++) {
+    float4 a = A, b = B, c = Cc, d = D;
+    for (int level = 0; level < 5; level++) {
+      const float lab = len3(a, b), lac = len3(a, c), lad = len3(a, d);
+      const float lbc = len3(b, c), lbd = len3(b, d), lcd = len3(c, d);
+      const float maxLen = fmx(fmx(fmx(fmx(fmx(lab, lac), lad), lbc), lbd), lcd);
+      float4 oa, ob, oc, od0, od1;
+      if (lab >= maxLen) { oa = mid4(a, b); ob = c; oc = d; od0 = a; od1 = b; }
+      else if (lac >= maxLen) { oa = mid4(a, c); ob = b; oc = d; od0 = a; od1 = c; }
+      else if (lad >= maxLen) { oa = mid4(a, d); ob = b; oc = c; od0 = a; od1 = d; }
+      else if (lbc >= maxLen) { oa = mid4(b, c); ob = a; oc = d; od0 = b; od1 = c; }
+      else if (lbd >= maxLen) { oa = mid4(b, d); ob = a; oc = c; od0 = b; od1 = d; }
+      else { oa = mid4(c, d); ob = a; oc = b; od0 = c; od1 = d; }
+      a = oa;
+      b = ob;
+      c = oc;
+      d = ((p >> (4 - level)) & 1) ? od1 : od0;
+    }
+    const float lx = fmn(fmn(fmn(a.x, b.x), c.x), d.x), hx = fmx(fmx(fmx(a.x, b.x), c.x), d.x);
+    const float ly = fmn(fmn(fmn(a.y, b.y), c.y), d.y), hy = fmx(fmx(fmx(a.y, b.y), c.y), d.y);
+    const float lz = fmn(fmn(fmn(a.z, b.z), c.z), d.z), hz = fmx(fmx(fmx(a.z, b.z), c.z), d.z);
+    const float lw = fmn(fmn(fmn(a.w, b.w), c.w), d.w), hw = fmx(fmx(fmx(a.w, b.w), c.w), d.w);
+    if (lx >= hx || ly >= hy || lz >= hz) continue;
+    int lo[3], hi[3];
+    const float bl[3] = {lx, ly, lz}, bh[3] = {hx, hy, hz};
+    for (int k = 0; k < 3; k++) {
+      lo[k] = (int)((bl[k] - v.mc_origin[k]) * v.mc_rcp_spacing[k]);
+      hi[k] = (int)((bh[k] - v.mc_origin[k]) * v.mc_rcp_spacing[k]);
+      lo[k] = min(max(lo[k], 0), v.mc[k] - 1);
+      hi[k] = min(max(hi[k], 0), v.mc[k] - 1);
+    }
+    for (int iz = lo[2]; iz <= hi[2]; iz++)
+      for (int iy = lo[1]; iy <= hi[1]; iy++)
+        for (int ix = lo[0]; ix <= hi[0]; ix++)
+          cache.add(ix + v.mc[0] * (iy + v.mc[1] * iz), lw, hw, rlo, rhi);
+  }
+  cache.flush(rlo, rhi);
+}
+
+// ----------------------------------------------------------------- march
+// March every ray of the generation's segments through its request's
+// isosurfaces (first) and volumes, one ray per thread, each clipped to the
+// closest hit so far; each block lies inside one request's segment, so the
+// tables are block-uniform.  With no surface instances in the batch the
+// trace kernel is skipped and every ray enters here as a surface miss
+// (surfaces == 0).  FULL: isosurfaces and tet fields may occur; otherwise
+// every volume of the batch is structured and there are no isosurfaces.
+template <bool FULL>
 __global__ __launch_bounds__(256) void k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
                                                       int surfaces, unsigned long long *stats)
 {
-  uint32_t ns = 0, nc = 0;
+  __shared__ uint32_t ring[FULL ? PQ_SHORT * PQ_TB : 1];
+  MarchCtx cx{ring + threadIdx.x, 0, 0, 0, 0};
   int i;
   bool valid;
   const Segment sg = segment_of<256>(sv, i, valid);
-  const VolumeList vl{slots[sg.slot].vols, slots[sg.slot].num_vols};
+  const SlotDev &SL = slots[sg.slot];
+  const VolumeList vl{SL.vols, SL.num_vols};
+  const IsoDev BNX_G *isos = SL.isos;
+  const int niso = FULL ? SL.num_isos : 0;
   if (valid) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
@@ -83,11 +215,25 @@ __global__ __launch_bounds__(256) void k_volume_march(const SlotDev *__restrict_
       float tcur = o.w;
       uint64_t rng = q.rng[i];
       const uint64_t rng0 = rng;
+      int ik = -1;
+      float it = 0.f;
+      v3 iN = splat(0.f);
+      for (int k = 0; k < niso; k++) {
+        float t;
+        v3 N;
+        if (iso_march(isos[k], org, dir, 0.f, tcur, rng, t, N, cx)) {
+          ik = k;
+          it = t;
+          iN = N;
+          tcur = t;
+          if (shadow) break;
+        }
+      }
       int vk = -1;
       float vt = 0.f, rgb[3] = {0.f, 0.f, 0.f};
-      for (int k = 0; k < vl.count; k++) {
+      for (int k = 0; k < vl.count && !(shadow && ik >= 0); k++) {
         float t, c[3];
-        if (volume_march(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, ns, nc)) {
+        if (volume_march<FULL>(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, cx)) {
           vk = k;
           vt = t;
           tcur = t;
@@ -101,6 +247,11 @@ __global__ __launch_bounds__(256) void k_volume_march(const SlotDev *__restrict_
         hit = make_int4(0, VOLUME_GEOM_BASE + vk, vl.vols[vk].rtc_inst, 0);
         q.vcol[i] = make_float4(rgb[0], rgb[1], rgb[2], 0.f);
         q.hit[i] = hit;
+      } else if (FULL && ik >= 0) {
+        q.org[i].w = it;
+        hit = make_int4(0, ISO_GEOM_BASE + ik, isos[ik].rtc_inst, 0);
+        q.vcol[i] = make_float4(iN.x, iN.y, iN.z, 0.f);
+        q.hit[i] = hit;
       } else if (!surfaces) {
         q.hit[i] = hit;
       }
@@ -108,10 +259,13 @@ __global__ __launch_bounds__(256) void k_volume_march(const SlotDev *__restrict_
       q.hit[i] = hit;
     }
   }
-  uint32_t s = wave_sum(ns), c = wave_sum(nc);
+  const uint32_t s = wave_sum(cx.samples), c = wave_sum(cx.cells);
+  const uint32_t un = wave_sum(cx.unodes), ut = wave_sum(cx.utets);
   if (lane_id() == 0) {
     stat_add(stats, STAT_SAMPLES, s);
     stat_add(stats, STAT_CELLS, c);
+    stat_add(stats, STAT_UNODES, un);
+    stat_add(stats, STAT_UTETS, ut);
   }
 }
 
@@ -125,9 +279,56 @@ int volume_build(Volume *v)
   BNX_HIP(v->ranges.ensure(sizeof(float2) * nmc));
   BNX_HIP(v->majorants.ensure(sizeof(float) * nmc));
   BNX_SET(d.majorants, v->majorants.ptr);
+  BNX_SET(d.ranges, v->ranges.ptr);
   k_mc_ranges<<<nblk((long)nmc, 64), 64, 0, dev->stream>>>(d, v->ranges.as<float2>());
   k_majorants<<<nblk((long)nmc, 256), 256, 0, dev->stream>>>(d, v->ranges.as<float2>(), v->majorants.as<float>());
   BNX_HIP(hipStreamSynchronize(dev->stream));
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *verts4, int nvert)
+{
+  Device *dev = v->dev;
+  VolumeDev &d = v->dd;
+  hipStream_t st = dev->stream;
+  BNX_HIP(v->uverts.ensure(sizeof(float4) * (size_t)nvert));
+  BNX_HIP(hipMemcpyAsync(v->uverts.ptr, verts4, sizeof(float4) * (size_t)nvert, hipMemcpyHostToDevice, st));
+  DevMem tets, lo, hi, rlo, rhi;
+  BNX_HIP(tets.ensure(sizeof(int4) * (size_t)ntet));
+  BNX_HIP(hipMemcpyAsync(tets.ptr, tets4, sizeof(int4) * (size_t)ntet, hipMemcpyHostToDevice, st));
+  // element BVH over the tets' boxes; tets stored in its slot order
+  BNX_HIP(lo.ensure(sizeof(float4) * (size_t)ntet));
+  BNX_HIP(hi.ensure(sizeof(float4) * (size_t)ntet));
+  k_tet_boxes<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), v->uverts.as<float4>(), ntet, lo.as<float4>(),
+                                               hi.as<float4>());
+  BNX_HIP(hipGetLastError());
+  int rc = build_bvh(dev, lo.as<float4>(), hi.as<float4>(), ntet, 4, v->ebvh);
+  if (rc != BNX_OK) return rc;
+  if (v->ebvh.max_depth + 1 > PQ_SHORT + PQ_DEEP) {
+    set_error("bnx_volume_create_umesh: element BVH deeper than the point-query stack");
+    return BNX_ERR_UNSUPPORTED;
+  }
+  BNX_HIP(v->utets.ensure(sizeof(int4) * (size_t)ntet));
+  k_tet_sort<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), v->ebvh.order.as<int>(), ntet, v->utets.as<int4>());
+  BNX_HIP(hipGetLastError());
+  BNX_SET(d.uverts, v->uverts.ptr);
+  BNX_SET(d.utets, v->utets.ptr);
+  BNX_SET(d.ebvh, v->ebvh.nodes.ptr);
+  // macrocell value ranges
+  const int nmc = d.mc[0] * d.mc[1] * d.mc[2];
+  BNX_HIP(rlo.ensure(sizeof(int) * (size_t)nmc));
+  BNX_HIP(rhi.ensure(sizeof(int) * (size_t)nmc));
+  BNX_HIP(v->ranges.ensure(sizeof(float2) * (size_t)nmc));
+  BNX_HIP(v->majorants.ensure(sizeof(float) * (size_t)nmc));
+  BNX_SET(d.ranges, v->ranges.ptr);
+  BNX_SET(d.majorants, v->majorants.ptr);
+  k_fill_ranges<<<nblk(nmc, 256), 256, 0, st>>>(rlo.as<int>(), rhi.as<int>(), nmc);
+  k_umesh_raster<<<nblk(ntet, 128), 128, 0, st>>>(tets.as<int4>(), v->uverts.as<float4>(), ntet, d, rlo.as<int>(),
+                                                  rhi.as<int>());
+  k_finish_ranges<<<nblk(nmc, 256), 256, 0, st>>>(rlo.as<int>(), rhi.as<int>(), nmc, v->ranges.as<float2>());
+  k_majorants<<<nblk(nmc, 256), 256, 0, st>>>(d, v->ranges.as<float2>(), v->majorants.as<float>());
+  BNX_HIP(hipStreamSynchronize(st));
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }
@@ -144,10 +345,12 @@ int volume_rebuild_majorants(Volume *v)
 }
 
 int launch_volume_march(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, const RayQueue &q,
-                        bool surfaces, unsigned long long *stats)
+                        bool surfaces, bool full, unsigned long long *stats)
 {
-  if (blocks256 > 0)
-    k_volume_march<<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
+  if (blocks256 > 0 && full)
+    k_volume_march<true><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
+  else if (blocks256 > 0)
+    k_volume_march<false><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

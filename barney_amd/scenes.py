@@ -254,6 +254,113 @@ def config2(n: int = 256, width: int = 1024, height: int = 1024, spp: int = 16,
 
 
 # --------------------------------------------------------------------------
+# config 4: unstructured tetrahedral volume + isosurface
+# --------------------------------------------------------------------------
+@dataclass
+class UMeshVolumeScene:
+    vertices: np.ndarray           # (nv, 3) float32
+    scalars: np.ndarray            # (nv,) float32, per vertex
+    tets: np.ndarray               # (nt, 4) int32, positively oriented
+    tf: np.ndarray                 # (N, 4) float32 RGBA
+    tf_domain: tuple
+    base_density: float
+    mc_dims: tuple = (32, 32, 32)
+    iso_value: float = 0.3
+    iso_color: tuple = (0.9, 0.6, 0.3, 1.0)
+    volume: bool = True            # render the field as a volume
+    dir_lights: list = field(default_factory=list)
+    ambient: float = 0.25
+    bg_color: tuple = (0.0, 0.0, 0.0, 1.0)
+    camera: dict = field(default_factory=dict)
+    width: int = 1024
+    height: int = 1024
+    spp: int = 32
+    user_id: int = 0
+    iso_user_id: int = 1
+
+
+def hex_tets(n: int) -> np.ndarray:
+    """The n^3 hexes of the lattice with vertex (i, j, k) at index
+    i + (n+1) * (j + (n+1) * k), hex (i, j, k) in x-fastest order, each split
+    into 5 tets: the central tet over four pairwise non-adjacent corners and
+    the four corner tets around it.  Hexes of even i+j+k use the corner set
+    {000, 110, 101, 011} for the central tet, odd ones {100, 010, 001, 111},
+    so neighbouring hexes cut their shared face along the same diagonal.
+    Returns (5 n^3, 4) int32 in (central, corners...) order per hex."""
+    m = n + 1
+    k, j, i = np.meshgrid(np.arange(n), np.arange(n), np.arange(n), indexing="ij")
+    i, j, k = i.ravel(), j.ravel(), k.ravel()
+
+    def c(a, b, cc):
+        return ((i + a) + m * ((j + b) + m * (k + cc))).astype(np.int64)
+
+    c000, c100, c010, c110 = c(0, 0, 0), c(1, 0, 0), c(0, 1, 0), c(1, 1, 0)
+    c001, c101, c011, c111 = c(0, 0, 1), c(1, 0, 1), c(0, 1, 1), c(1, 1, 1)
+    even = ((i + j + k) & 1) == 0
+    ev = np.stack([np.stack([c000, c110, c101, c011], 1), np.stack([c100, c000, c110, c101], 1),
+                   np.stack([c010, c000, c110, c011], 1), np.stack([c001, c000, c101, c011], 1),
+                   np.stack([c111, c110, c101, c011], 1)], 1)
+    od = np.stack([np.stack([c100, c010, c001, c111], 1), np.stack([c000, c100, c010, c001], 1),
+                   np.stack([c110, c100, c010, c111], 1), np.stack([c101, c100, c001, c111], 1),
+                   np.stack([c011, c010, c001, c111], 1)], 1)
+    t = np.where(even[:, None, None], ev, od)
+    return t.reshape(-1, 4).astype(np.int32)
+
+
+def orient_tets(vertices: np.ndarray, tets: np.ndarray) -> np.ndarray:
+    """swap the last two vertices of every tet whose fourth vertex lies on
+    the negative side of the plane (v0, v1, v2): the reference's tet test
+    (UMeshField.h:391-436) takes the inside as the positive side of all four
+    faces"""
+    v = vertices.astype(np.float64)
+    a, b, c, d = (v[tets[:, q]] for q in range(4))
+    vol = np.einsum("ij,ij->i", np.cross(b - a, c - a), d - a)
+    t = tets.copy()
+    neg = vol < 0
+    t[neg, 2], t[neg, 3] = tets[neg, 3], tets[neg, 2]
+    return t
+
+
+def jittered_lattice(n: int, seed: int, jitter: float = 0.2) -> np.ndarray:
+    """(n+1)^3 vertices on [0,1]^3 (x fastest), each coordinate moved by
+    U(-jitter, jitter) cell widths unless the vertex lies on the domain face
+    of that axis.  Vertex v draws 3 uniforms from stream (6 << 32) | v."""
+    m = n + 1
+    k, j, i = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+    idx = np.stack([i.ravel(), j.ravel(), k.ravel()], 1)
+    u = uniform_table(seed, 6, m ** 3, 3)
+    off = (2.0 * u - 1.0) * jitter
+    off[(idx == 0) | (idx == n)] = 0.0
+    return ((idx + off) / n).astype(np.float32)
+
+
+def config4(n: int = 160, width: int = 1024, height: int = 1024, spp: int = 32,
+            mc_dims=(32, 32, 32), volume: bool = True) -> UMeshVolumeScene:
+    """Unstructured volume: jittered n^3 hex lattice on [0,1]^3 split into 5
+    tets per hex (n = 160: 20,480,000 tets, 4,173,281 vertices), per-vertex
+    scalar sin(4x) cos(4y) sin(4z) + N(0, 0.05^2) (Box-Muller on PCG32,
+    stream (7 << 32) | vertex), the config-2 transfer function over the
+    field's value range with density 50, a 32^3 macrocell grid and an
+    isosurface at 0.3; lights and camera as config 2."""
+    seed = SEEDS["c4"]
+    verts = jittered_lattice(n, seed)
+    tets = orient_tets(verts, hex_tets(n))
+    u = uniform_table(seed, 7, len(verts), 2)
+    gauss = np.sqrt(-2.0 * np.log1p(-u[:, 0])) * np.cos(2.0 * np.pi * u[:, 1])
+    p = verts.astype(np.float64)
+    f = np.sin(4 * p[:, 0]) * np.cos(4 * p[:, 1]) * np.sin(4 * p[:, 2]) + 0.05 * gauss
+    f = f.astype(np.float32)
+    s = UMeshVolumeScene(vertices=verts, scalars=f, tets=tets, tf=ramp_tf(256, 0.8),
+                         tf_domain=(float(f.min()), float(f.max())), base_density=50.0,
+                         mc_dims=tuple(mc_dims), volume=volume, width=width, height=height, spp=spp)
+    d = -np.ones(3, dtype=np.float64) / math.sqrt(3.0)
+    s.dir_lights.append((d.astype(np.float32), 2.5, np.ones(3, dtype=np.float32)))
+    s.camera = dict(position=(1.9, 1.4, 2.6), direction=(0.5 - 1.9, 0.5 - 1.4, 0.5 - 2.6), up=(0.0, 1.0, 0.0),
+                    fovy=35.0, target=(0.5, 0.5, 0.5))
+    return s
+
+
+# --------------------------------------------------------------------------
 # serving stream
 # --------------------------------------------------------------------------
 RESOLUTIONS = (256, 512, 1024, 2048)

@@ -46,6 +46,7 @@ typedef struct bnx_group_s  *bnx_group;
 typedef struct bnx_world_s  *bnx_world;
 typedef struct bnx_frame_s  *bnx_frame;
 typedef struct bnx_volume_s *bnx_volume;
+typedef struct bnx_isosurface_s *bnx_isosurface;
 
 /* ------------------------------------------------------------- records */
 /* One ray of a trace batch: traceRays.dev.cu:38-50 passes (org, dir, 0, tMax). */
@@ -163,6 +164,28 @@ typedef struct bnx_transfer_function {
   float        base_density;
 } bnx_transfer_function;
 
+/* UMeshField (umesh/common/UMeshField.h:33-168, anari "vertex.position",
+ * "vertex.data", "index", "cell.index", "cell.type"): per-vertex scalars,
+ * each cell's vertex indices starting at cell_offsets[cell].  Tetrahedra
+ * only (cell type 0 = _ANARI_TET or 10 = _VTK_TET; NULL cell_types means all
+ * tets); any other type returns BNX_ERR_UNSUPPORTED.  A tet (v0, v1, v2, v3)
+ * contains P when P is on the non-negative side of its four faces as
+ * UMeshField::DD::tetScalar orients them (UMeshField.h:391-436). */
+#define BNX_CELL_TET      0
+#define BNX_CELL_VTK_TET 10
+typedef struct bnx_umesh_desc {
+  const float   *vertices;          /* float3[num_vertices]                        */
+  const float   *scalars;           /* float[num_vertices]                         */
+  const int32_t *indices;           /* int[num_indices]                            */
+  const int32_t *cell_offsets;      /* int[num_cells]                              */
+  const uint8_t *cell_types;        /* uint8[num_cells] or NULL                    */
+  int32_t        num_vertices;
+  int32_t        num_indices;
+  int32_t        num_cells;
+  int32_t        mc_dims[3];        /* macrocell grid; 0: 1 + size * (G-1)/maxWidth,
+                                       G = 200 + sqrt(cells/1000) (UMeshField.cu:181-186) */
+} bnx_umesh_desc;
+
 /* Frame-buffer channels (fb/FrameBuffer.cu:315-394 bnFrameBufferRead). */
 #define BNX_FB_COLOR   0   /* float4 per pixel: accum * 1/accumID           */
 #define BNX_FB_DEPTH   1   /* float per pixel: primary-ray tMax             */
@@ -271,6 +294,26 @@ int bnx_volume_set_transfer_function(bnx_volume vol, const bnx_transfer_function
 /* macrocell grid dims and majorants (for tests) */
 int bnx_volume_majorants(bnx_volume vol, int32_t dims[3], float *majorants);
 int bnx_volume_destroy(bnx_volume vol);
+/* Volume over an unstructured (tet) field (UMeshField + UMeshCuBQLSampler +
+ * MCVolumeAccel, umesh/mc/UMeshCuBQLSampler.cu:45-78, UMeshField.cu:168-197):
+ * builds on the GPU the element BVH (binned SAH, leaf <= 4), the macrocell
+ * value ranges by rasterizing each tet split 5 times on its longest edge
+ * (rasterTet<5>, UMeshField.cu:28-107), and the majorants.  Sampling is a
+ * point query: the first tet containing P in a depth-first walk of the
+ * element BVH (child 0 first). */
+int bnx_volume_create_umesh(bnx_device dev, const bnx_umesh_desc *field, const bnx_transfer_function *tf,
+                            int32_t user_id, bnx_volume *out);
+/* macrocell value ranges (lo, hi per cell; for tests) */
+int bnx_volume_mc_ranges(bnx_volume vol, int32_t dims[3], float *lo, float *hi);
+
+/* IsoSurface geometry (geometry/IsoSurface.h:31-42, anari "isovalue",
+ * "field") over the field of a volume, with its MCIsoSurfaceAccel
+ * (volume/MCAccelerator.h:325-548): DDA over the field's macrocell value
+ * ranges, 10 sub-steps per overlapping cell, central-difference normal.
+ * Only opaque matte materials are provided. */
+int bnx_isosurface_create(bnx_volume field, float iso_value, const bnx_material *material, int32_t user_id,
+                          bnx_isosurface *out);
+int bnx_isosurface_destroy(bnx_isosurface iso);
 
 /* --------------------------------------------------------------- world */
 /* World::DD (render/World.h:32-47): instances + lights. */
@@ -280,6 +323,10 @@ int bnx_world_create(bnx_device dev, bnx_group instances, const bnx_dir_light *d
  * groups after the geometry groups of an instance).  Volumes are marched
  * after the surface trace, in the order they were added. */
 int bnx_world_add_volume(bnx_world world, bnx_volume vol, int32_t user_inst_id);
+/* Adds an isosurface as a user instance; isosurfaces are intersected after
+ * the surface trace and before the volumes, in the order they were added,
+ * and act as closest-hit surfaces. */
+int bnx_world_add_isosurface(bnx_world world, bnx_isosurface iso, int32_t user_inst_id);
 int bnx_world_destroy(bnx_world world);
 
 /* -------------------------------------------------------------- camera */
@@ -391,6 +438,7 @@ static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
 static_assert(sizeof(bnx_triangles_desc) == 72, "bnx_triangles_desc layout");
 static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
 static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
+static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
 static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
 static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");

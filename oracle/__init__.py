@@ -44,6 +44,15 @@ def lib():
         L.orc_scene_add_volume.argtypes = [vp, C.POINTER(abi.StructuredDesc), C.POINTER(abi.TransferFunction),
                                            C.c_int, C.c_int]
         L.orc_volume_majorants.argtypes = [vp, C.c_int, abi.ip, abi.fp]
+        L.orc_scene_add_umesh.argtypes = [vp, C.POINTER(abi.UMeshDesc), C.POINTER(abi.TransferFunction), C.c_int,
+                                          C.c_int, C.c_int]
+        L.orc_scene_add_isosurface.argtypes = [vp, C.c_int, C.c_float, abi.fp, C.c_int, C.c_int]
+        L.orc_volume_mc_ranges.argtypes = [vp, C.c_int, abi.ip, abi.fp, abi.fp]
+        L.orc_volume_ebvh.argtypes = [vp, C.c_int, vp, abi.ip, abi.ip]
+        L.orc_umesh_sample.argtypes = [vp, C.c_int, abi.fp, C.POINTER(C.c_uint64)]
+        L.orc_umesh_sample.restype = C.c_float
+        L.orc_iso_march.argtypes = [vp, C.c_int, abi.fp, abi.fp, C.c_float, C.c_uint32, C.c_uint32, abi.fp]
+        L.orc_iso_march.restype = C.c_float
         L.orc_volume_sample.argtypes = [vp, C.c_int, abi.fp]
         L.orc_volume_sample.restype = C.c_float
         L.orc_volume_tf_map.argtypes = [vp, C.c_int, C.c_float, abi.fp]
@@ -127,6 +136,43 @@ class Scene:
         d, t, keep = abi.structured_desc(vs)
         k = self.L.orc_scene_add_volume(self.h, C.byref(d), C.byref(t), user_id, user_inst)
         return k
+
+    def add_umesh(self, us, user_id=0, user_inst=0, rendered=True):
+        d, t, keep = abi.umesh_desc(us)
+        return self.L.orc_scene_add_umesh(self.h, C.byref(d), C.byref(t), user_id, user_inst, int(rendered))
+
+    def add_isosurface(self, field: int, iso: float, color=(0.8, 0.8, 0.8), user_id=0, user_inst=0):
+        c = np.ascontiguousarray(color[:3], np.float32)
+        return self.L.orc_scene_add_isosurface(self.h, field, float(iso), abi.fptr(c), user_id, user_inst)
+
+    def mc_ranges(self, k=0):
+        dims = np.zeros(3, np.int32)
+        self.L.orc_volume_mc_ranges(self.h, k, abi.iptr(dims), None, None)
+        lo = np.zeros(int(np.prod(dims)), np.float32)
+        hi = np.zeros_like(lo)
+        self.L.orc_volume_mc_ranges(self.h, k, abi.iptr(dims), abi.fptr(lo), abi.fptr(hi))
+        return dims, lo, hi
+
+    def element_bvh(self, k=0):
+        info = np.zeros(3, np.int32)
+        self.L.orc_volume_ebvh(self.h, k, None, None, abi.iptr(info))
+        nodes = np.zeros(info[0], abi.NODE_DTYPE)
+        order = np.zeros(info[1], np.int32)
+        self.L.orc_volume_ebvh(self.h, k, nodes.ctypes.data, abi.iptr(order), abi.iptr(info))
+        return nodes, order, int(info[2])
+
+    def umesh_sample(self, P, k=0):
+        p = np.ascontiguousarray(P, np.float32)
+        n = C.c_uint64()
+        f = self.L.orc_umesh_sample(self.h, k, abi.fptr(p), C.byref(n))
+        return f, n.value
+
+    def iso_march(self, org, dir, tmax, seed=(1, 2), k=0):
+        o = np.ascontiguousarray(org, np.float32)
+        d = np.ascontiguousarray(dir, np.float32)
+        N = np.zeros(3, np.float32)
+        t = self.L.orc_iso_march(self.h, k, abi.fptr(o), abi.fptr(d), tmax, seed[0], seed[1], abi.fptr(N))
+        return t, N
 
     def volume_majorants(self, k=0):
         dims = np.zeros(3, np.int32)
@@ -258,5 +304,17 @@ def volume_scene(vs) -> Scene:
     s = Scene()
     s.add_volume(vs, user_id=vs.user_id, user_inst=0)
     s.set_dir_lights(vs.dir_lights)
+    s.build()
+    return s
+
+
+def umesh_scene(us) -> Scene:
+    """Oracle scene for a barney_amd.scenes.UMeshVolumeScene: the tet field as
+    a volume (user instance 0, unless us.volume is false) and the isosurface
+    over it (user instance 1), no surfaces."""
+    s = Scene()
+    k = s.add_umesh(us, user_id=us.user_id, user_inst=0, rendered=us.volume)
+    s.add_isosurface(k, us.iso_value, us.iso_color, user_id=us.iso_user_id, user_inst=1)
+    s.set_dir_lights(us.dir_lights)
     s.build()
     return s

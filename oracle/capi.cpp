@@ -110,6 +110,94 @@ int orc_scene_add_volume(orc_scene *sc, const bnx_structured_desc *f, const bnx_
   return (int)sc->s.volumes.size() - 1;
 }
 
+// unstructured (tet) volume; rendered == 0 keeps it only as the field of
+// isosurfaces (it is not marched as a volume)
+int orc_scene_add_umesh(orc_scene *sc, const bnx_umesh_desc *f, const bnx_transfer_function *tf, int user_id,
+                        int user_inst, int rendered)
+{
+  Volume v;
+  v.field = FIELD_UMESH;
+  v.rendered = rendered != 0;
+  v.verts.resize(4 * (size_t)f->num_vertices);
+  for (int i = 0; i < f->num_vertices; i++) {
+    for (int q = 0; q < 3; q++) v.verts[4 * (size_t)i + q] = f->vertices[3 * (size_t)i + q];
+    v.verts[4 * (size_t)i + 3] = f->scalars[i];
+  }
+  v.tets.resize(4 * (size_t)f->num_cells);
+  for (int c = 0; c < f->num_cells; c++)
+    for (int q = 0; q < 4; q++) v.tets[4 * (size_t)c + q] = f->indices[f->cell_offsets[c] + q];
+  for (int k = 0; k < 3; k++) v.mc_req[k] = f->mc_dims[k];
+  v.tf.assign(tf->values, tf->values + 4 * (size_t)tf->num_values);
+  v.dom0 = tf->domain[0];
+  v.dom1 = tf->domain[1];
+  v.base_density = tf->base_density;
+  v.user_id = user_id;
+  v.user_inst = user_inst;
+  build_volume(v);
+  sc->s.volumes.push_back(std::move(v));
+  return (int)sc->s.volumes.size() - 1;
+}
+
+// isosurface over the field of volume 'field' as user instance 'user_inst'
+int orc_scene_add_isosurface(orc_scene *sc, int field, float iso, const float *color, int user_id, int user_inst)
+{
+  IsoSurface s;
+  s.field = field;
+  s.iso = iso;
+  for (int k = 0; k < 3; k++) s.color[k] = color[k];
+  s.user_id = user_id;
+  s.user_inst = user_inst;
+  sc->s.isos.push_back(s);
+  return (int)sc->s.isos.size() - 1;
+}
+
+int orc_volume_mc_ranges(orc_scene *sc, int k, int32_t *dims, float *lo, float *hi)
+{
+  const Volume &v = sc->s.volumes[k];
+  for (int i = 0; i < 3; i++) dims[i] = v.mc[i];
+  if (lo) memcpy(lo, v.mc_lo.data(), v.mc_lo.size() * sizeof(float));
+  if (hi) memcpy(hi, v.mc_hi.data(), v.mc_hi.size() * sizeof(float));
+  return 0;
+}
+
+// element BVH of an unstructured volume: info = {num_nodes, num_prims, max_depth}
+int orc_volume_ebvh(orc_scene *sc, int k, void *nodes_out, int32_t *order_out, int32_t *info)
+{
+  const Volume &v = sc->s.volumes[k];
+  info[0] = (int32_t)v.ebvh.nodes.size();
+  info[1] = (int32_t)v.ebvh.order.size();
+  info[2] = v.ebvh.max_depth;
+  if (nodes_out) memcpy(nodes_out, v.ebvh.nodes.data(), v.ebvh.nodes.size() * sizeof(Node));
+  if (order_out) memcpy(order_out, v.ebvh.order.data(), v.ebvh.order.size() * sizeof(int32_t));
+  return 0;
+}
+
+// point query with the number of element-BVH nodes and tets visited
+float orc_umesh_sample(orc_scene *sc, int k, const float *P, uint64_t *visits)
+{
+  uint64_t n = 0;
+  float f = umesh_sample(sc->s.volumes[k], load3(P), &n);
+  if (visits) *visits = n;
+  return f;
+}
+
+// isosurface k along one ray with a parent seed (a, b): t (or -1) and normal
+float orc_iso_march(orc_scene *sc, int k, const float *org, const float *dir, float tmax, uint32_t a, uint32_t b,
+                    float *normal)
+{
+  RNGSeed p;
+  p.seed(a, b);
+  const IsoSurface &iso = sc->s.isos[k];
+  float t = -1.f;
+  vec3 N;
+  if (!iso_march(sc->s.volumes[iso.field], iso, 0u, load3(org), fix_dir(load3(dir)), 0.f, tmax, p, t, N))
+    return -1.f;
+  normal[0] = N.x;
+  normal[1] = N.y;
+  normal[2] = N.z;
+  return t;
+}
+
 int orc_volume_majorants(orc_scene *sc, int k, int32_t *dims, float *maj)
 {
   const Volume &v = sc->s.volumes[k];

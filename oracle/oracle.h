@@ -213,7 +213,10 @@ struct Instance {
 
 // StructuredData volume with its macrocell majorants (volume/StructuredData.*,
 // volume/MCGrid.*, volume/TransferFunction.h)
+enum { FIELD_STRUCTURED = 0, FIELD_UMESH = 1 };
 struct Volume {
+  int field = FIELD_STRUCTURED;
+  bool rendered = true;             // false: only the field of isosurfaces
   int dims[3] = {0, 0, 0};
   int cells_per_mc = 8;
   vec3 origin, spacing{1, 1, 1};
@@ -227,6 +230,19 @@ struct Volume {
   int mc[3];
   vec3 mc_origin, mc_spacing;
   std::vector<float> mc_lo, mc_hi, majorant;
+  // unstructured field (tets only): UMeshField (umesh/common/UMeshField.h)
+  std::vector<float> verts;         // x, y, z, scalar per vertex
+  std::vector<int32_t> tets;        // 4 vertex indices per tet; element-BVH slot order after build
+  Bvh ebvh;                         // element BVH over the tets' boxes
+  int mc_req[3] = {0, 0, 0};        // requested macrocell grid (0: the reference's formula)
+};
+
+// IsoSurface geometry (geometry/IsoSurface.h:31-42) over a volume's field
+struct IsoSurface {
+  int field = 0;                    // index of the volume whose field it cuts
+  float iso = 0.f;
+  float color[3] = {0.8f, 0.8f, 0.8f};
+  int user_id = 0, user_inst = 0;
 };
 void build_volume(Volume &v);
 void build_majorants(Volume &v);
@@ -239,6 +255,23 @@ float tf_majorant(const Volume &v, float lo, float hi);
 bool volume_march(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin, float tmax, RNGSeed &parent,
                   float &t_out, float rgb[3], uint64_t *samples);
 float fast_log(float f);
+// tets: UMeshField commit + buildInitialMacroCells (UMeshField.cu:168-197,
+// 320-379) + element BVH (UMeshCuBQLSampler::build, umesh/mc/
+// UMeshCuBQLSampler.cu:45-78, with this build's binned-SAH builder)
+void build_umesh(Volume &v);
+// UMeshCuBQLSampler::DD::sample (umesh/mc/UMeshCuBQLSampler.h:46-62): the
+// first tet that contains P in a depth-first walk of the element BVH
+// (child 0 first, leaf slots in order); NaN if none.  'visits' counts
+// element-BVH nodes and tet tests when non-null.
+float umesh_sample(const Volume &v, vec3 P, uint64_t *visits = nullptr);
+// tetScalar (UMeshField.h:391-436) of the tet in slot 'slot'
+bool tet_scalar(const Volume &v, int slot, vec3 P, float &out);
+// MCIsoSurfaceAccel::isProg (volume/MCAccelerator.h:325-548): t of the
+// surface f = iso along the ray within [tmin, tmax), with the unnormalized
+// central-difference gradient normal; advances 'parent' once when the ray
+// overlaps the field's bounds.
+bool iso_march(const Volume &field, const IsoSurface &iso, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin,
+               float tmax, RNGSeed &parent, float &t_out, vec3 &normal);
 
 struct Scene {
   std::vector<Geom> geoms;
@@ -247,6 +280,7 @@ struct Scene {
   Bvh tlas;
   std::vector<bnx_dir_light> dir_lights;
   std::vector<Volume> volumes;      // marched after the surface trace, in order
+  std::vector<IsoSurface> isos;     // tested after the surface trace, before the volumes
 };
 
 void build_scene(Scene &s);        // BLAS per group + TLAS

@@ -390,6 +390,27 @@ void apply_volume_hit(const Scene &s, PathRay &r, int k, float t, const float *r
   }
 }
 
+// MCIsoSurfaceAccel::isProg hit (volume/MCAccelerator.h:493-547): matte
+// material (AnariMatte.h:58-63) at the crossing, normal = the normalized
+// gradient; primID 0 and the isosurface's user IDs
+void apply_iso_hit(const Scene &s, PathRay &r, int k, float t, vec3 N, HitIDs *ids)
+{
+  const IsoSurface &iso = s.isos[k];
+  vec3 d = fix_dir(r.dir);
+  r.had_hit = true;
+  r.tmax = t;
+  r.P = r.org + t * d;
+  r.N = quantize_half(N);
+  r.albedo = (0.85f * vec3(iso.color[0], iso.color[1], iso.color[2])) * ONE_OVER_PI;
+  r.bsdf = 1;
+  if (ids) {
+    ids->depth = t;
+    ids->prim = 0;
+    ids->inst = iso.user_inst;
+    ids->obj = iso.user_id;
+  }
+}
+
 inline void atomic_add64(int64_t *p, int64_t v) { __atomic_fetch_add(p, v, __ATOMIC_RELAXED); }
 inline int64_t to_fixed(float f) { return (int64_t)(f * 0x1p32f); }
 
@@ -473,22 +494,43 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
         TraceCounters c;
         Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, false, stats ? &c : nullptr);
         if (gen == 0) ids[i] = {BNX_INF, -1, -1, -1};
-        // volumes after the surface trace, clipped to the surface hit
+        // isosurfaces, then volumes, after the surface trace, each clipped
+        // to the closest hit so far
         float tcur = h.prim >= 0 ? h.t : r.tmax;
+        const vec3 fd = fix_dir(r.dir);
+        int nvol = 0;
+        for (const Volume &v : s.volumes) nvol += v.rendered ? 1 : 0;
+        int ik = -1;
+        float it = 0.f;
+        vec3 iN;
+        for (int k = 0; k < (int)s.isos.size(); k++) {
+          float t;
+          vec3 N;
+          const IsoSurface &iso = s.isos[k];
+          if (iso_march(s.volumes[iso.field], iso, (uint32_t)(s.insts.size() + nvol + k), r.org, fd, 0.f, tcur,
+                        r.rng, t, N)) {
+            ik = k;
+            it = t;
+            iN = N;
+            tcur = t;
+          }
+        }
         int vk = -1;
         float vt = 0.f, vrgb[3] = {0.f, 0.f, 0.f};
         uint64_t vs = 0;
-        const vec3 fd = fix_dir(r.dir);
-        for (int k = 0; k < (int)s.volumes.size(); k++) {
+        for (int k = 0, rk = 0; k < (int)s.volumes.size(); k++) {
+          if (!s.volumes[k].rendered) continue;
           float t, rgb[3];
-          if (volume_march(s.volumes[k], (uint32_t)(s.insts.size() + k), r.org, fd, 0.f, tcur, r.rng, t, rgb, &vs)) {
+          if (volume_march(s.volumes[k], (uint32_t)(s.insts.size() + rk), r.org, fd, 0.f, tcur, r.rng, t, rgb, &vs)) {
             vk = k;
             vt = t;
             tcur = t;
             memcpy(vrgb, rgb, sizeof(vrgb));
           }
+          rk++;
         }
         if (vk >= 0) apply_volume_hit(s, r, vk, vt, vrgb, gen == 0 ? &ids[i] : nullptr);
+        else if (ik >= 0) apply_iso_hit(s, r, ik, it, iN, gen == 0 ? &ids[i] : nullptr);
         else apply_hit(s, r, h, gen == 0 ? &ids[i] : nullptr, compat13);
         if (stats) nvs += vs;
         if (stats) {

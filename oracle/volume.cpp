@@ -1,9 +1,10 @@
-// volume.cpp -- CPU oracle of barney's structured-volume path: field
-// sampling, transfer function, macrocell majorants, 3D DDA and Woodcock
-// (delta) tracking.
+// volume.cpp -- CPU oracle of barney's volume path: structured-field
+// sampling, transfer function, macrocell majorants, and the DDA + Woodcock
+// (delta) tracking that volumes of every field type share.
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.h).
 #include "oracle.h"
+#include "dda.h"
 
 #include <algorithm>
 
@@ -26,6 +27,7 @@ static inline float lerp_l(float f, float a, float b) { return (1.f - f) * a + f
 // the CPU tex3D trilinear filter (rtcore/cpu/Texture.cpp:339-412)
 float volume_sample(const Volume &v, vec3 P)
 {
+  if (v.field == FIELD_UMESH) return umesh_sample(v, P);
   vec3 rel = (P - v.origin) * vec3(1.f / v.spacing.x, 1.f / v.spacing.y, 1.f / v.spacing.z);
   if (rel.x < 0.f || rel.y < 0.f || rel.z < 0.f) return NAN;
   if (rel.x >= (float)v.ncells[0] || rel.y >= (float)v.ncells[1] || rel.z >= (float)v.ncells[2]) return NAN;
@@ -87,6 +89,10 @@ float tf_majorant(const Volume &v, float lo, float hi)
 // (StructuredData.cu:30-96), then majorants
 void build_volume(Volume &v)
 {
+  if (v.field == FIELD_UMESH) {
+    build_umesh(v);
+    return;
+  }
   for (int k = 0; k < 3; k++) v.ncells[k] = v.dims[k] - 1;
   v.bounds.lo = v.origin;
   v.bounds.hi = v.origin + v.spacing * vec3((float)v.ncells[0], (float)v.ncells[1], (float)v.ncells[2]);
@@ -160,78 +166,36 @@ bool volume_march(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float 
   vec3 ddir = dir * rsp;
   Random2 rng = Random2::from64(parent, fnv_hash3(rtc_inst, 0u, 0u));
   const float ray_tmax = tmax;   // ray.tMax seen by the lambda (:603)
-
-  // dda3 (volume/DDA.h:60-201) over the macrocell grid
-  const float dda_tmax = r1;
-  const vec3 f_size((float)v.mc[0], (float)v.mc[1], (float)v.mc[2]);
-  const vec3 rcp_dir(1.f / ddir.x, 1.f / ddir.y, 1.f / ddir.z);
-  vec3 t_lo = (vec3(0.f) - dorg) * rcp_dir;
-  vec3 t_hi = (f_size - dorg) * rcp_dir;
-  vec3 t_nr = vmin(t_lo, t_hi), t_fr = vmax(t_lo, t_hi);
-  for (int k = 0; k < 3; k++) {
-    if (ddir[k] == 0.f) {
-      if (dorg[k] < 0.f || dorg[k] > f_size[k]) return false;
-      t_nr[k] = -INFINITY;
-      t_fr[k] = INFINITY;
-    }
-  }
-  float ray_t0 = fmx(0.f, reduce_max(t_nr));
-  float ray_t1 = fmn(dda_tmax, reduce_min(t_fr));
-  if (ray_t0 > ray_t1) return false;
-  vec3 oiv = dorg + ray_t0 * ddir;
-  vec3 f_cell;
-  for (int k = 0; k < 3; k++) f_cell[k] = fmx(0.f, fmn(f_size[k] - 1.f, floorf(oiv[k])));
-  vec3 f_end, t_step, t_next;
-  int stop[3], delta[3], cell[3];
-  for (int k = 0; k < 3; k++) {
-    f_end[k] = ddir[k] > 0.f ? f_cell[k] + 1.f : f_cell[k];
-    t_step[k] = fabsf(rcp_dir[k]);
-    t_next[k] = (ddir[k] == 0.f) ? INFINITY : fabsf(f_end[k] - oiv[k]) * t_step[k];
-    stop[k] = ddir[k] > 0.f ? v.mc[k] : -1;
-    delta[k] = ddir[k] > 0.f ? 1 : -1;
-    cell[k] = (int)f_cell[k];
-  }
-  float next_begin = 0.f;
-  while (true) {
-    float t_closest = reduce_min(t_next);
-    const float c0 = ray_t0 + next_begin;
-    const float c1 = ray_t0 + fmn(t_closest, dda_tmax);
+  bool found = false;
+  // dda3 over the macrocell grid, clipped to the bounds overlap
+  dda3(dorg, ddir, r1, v.mc, [&](const int cell[3], float c0, float c1) -> bool {
     // lambda (:597-686)
-    const float maj = v.majorant[(size_t)cell[0] + (size_t)v.mc[0] * ((size_t)cell[1] + (size_t)v.mc[1] * cell[2])];
-    if (maj != 0.f) {
-      // Woodcock::sampleRange (volume/Volume.h:168-203)
-      float lower = c0, upper = fmn(c1, ray_tmax);
-      float t = lower;
-      while (true) {
-        float r = rng();
-        float dt = -fast_log(1.f - r) / maj;
-        t += dt;
-        if (t >= upper) break;
-        vec3 P = org + t * dir;
-        float f = volume_sample(v, P);
-        if (samples) (*samples)++;
-        float s[4] = {0.f, 0.f, 0.f, 0.f};
-        if (!std::isnan(f)) tf_map(v, f, s);
-        if (s[3] >= rng() * maj) {
-          t_out = t;
-          rgb[0] = s[0];
-          rgb[1] = s[1];
-          rgb[2] = s[2];
-          return true;
-        }
+    const float maj = v.majorant[mc_index(v, cell)];
+    if (maj == 0.f) return true;
+    // Woodcock::sampleRange (volume/Volume.h:168-203)
+    float lower = c0, upper = fmn(c1, ray_tmax);
+    float t = lower;
+    while (true) {
+      float r = rng();
+      float dt = -fast_log(1.f - r) / maj;
+      t += dt;
+      if (t >= upper) return true;
+      vec3 P = org + t * dir;
+      float f = volume_sample(v, P);
+      if (samples) (*samples)++;
+      float s[4] = {0.f, 0.f, 0.f, 0.f};
+      if (!std::isnan(f)) tf_map(v, f, s);
+      if (s[3] >= rng() * maj) {
+        t_out = t;
+        rgb[0] = s[0];
+        rgb[1] = s[1];
+        rgb[2] = s[2];
+        found = true;
+        return false;
       }
     }
-    next_begin = t_closest;
-    bool done = false;
-    for (int k = 0; k < 3 && !done; k++) {
-      if (t_next[k] == t_closest) {
-        t_next[k] += t_step[k];
-        cell[k] += delta[k];
-        if (cell[k] == stop[k]) done = true;
-      }
-    }
-    if (done) return false;
-  }
+  });
+  return found;
 }
 
 } // namespace orc
