@@ -70,6 +70,7 @@ _PROTOS = [
     ("bnx_world_add_volume", [vp, vp, C.c_int32], C.c_int),
     ("bnx_world_add_isosurface", [vp, vp, C.c_int32], C.c_int),
     ("bnx_world_create", [vp, vp, C.POINTER(abi.DirLight), C.c_int, C.POINTER(vp)], C.c_int),
+    ("bnx_world_set_env_map", [vp, C.POINTER(abi.EnvMapDesc)], C.c_int),
     ("bnx_world_destroy", [vp], C.c_int),
     ("bnx_camera_perspective", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                 C.POINTER(abi.Camera)], C.c_int),
@@ -354,6 +355,20 @@ class World:
         check(self.dev.L.bnx_world_add_isosurface(self.h, iso.h, user_inst_id))
         self.volumes.append(iso)
 
+    def set_env_map(self, radiance, direction=(1.0, 0.0, 0.0), up=(0.0, 0.0, 1.0), scale=1.0):
+        """HDRI environment light from an (height, width, 3) radiance image
+        (row 0 at theta = 0); None removes it"""
+        if radiance is None:
+            check(self.dev.L.bnx_world_set_env_map(self.h, None))
+            return
+        d, _keep = abi.env_map_desc(radiance, direction, up, scale)
+        check(self.dev.L.bnx_world_set_env_map(self.h, C.byref(d)))
+
+    def set_env(self, env):
+        """the env map of a scene description (scenes.EnvMap or None)"""
+        if env is not None:
+            self.set_env_map(env.radiance, env.direction, env.up, env.scale)
+
 
 class Frame:
     def __init__(self, dev: Device, w: int, h: int):
@@ -395,7 +410,7 @@ def render(world: World, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp:
                 volume_samples=st.volume_samples, ms=st.ms_total, nodes=st.node_visits, tris=st.tri_tests,
                 spheres=st.sphere_tests, trace_ms=st.trace_ms, trace_launches=st.trace_launches,
                 march_ms=st.march_ms, march_launches=st.march_launches, mc_cells=st.mc_cells,
-                march_rays=st.march_rays)
+                march_rays=st.march_rays, umesh_nodes=st.umesh_nodes, umesh_tets=st.umesh_tets)
 
 
 class Server:
@@ -487,6 +502,7 @@ class SurfaceWorld:
         self.groups = groups
         self.insts = Group.instances(dev, groups, [0] * len(groups)).build()
         self.world = World(dev, self.insts, sc.dir_lights)
+        self.world.set_env(getattr(sc, "env", None))
 
 
 class VolumeWorld:
@@ -498,6 +514,7 @@ class VolumeWorld:
         self.insts = Group.instances(dev, [], []).build()
         self.volume = Volume(dev, vs, user_id=vs.user_id)
         self.world = World(dev, self.insts, vs.dir_lights)
+        self.world.set_env(getattr(vs, "env", None))
         self.world.add_volume(self.volume, 0)
 
 
@@ -512,6 +529,7 @@ class UMeshWorld:
         self.volume = Volume(dev, us, user_id=us.user_id)
         self.iso = Isosurface(self.volume, us.iso_value, us.iso_color, user_id=us.iso_user_id)
         self.world = World(dev, self.insts, us.dir_lights)
+        self.world.set_env(getattr(us, "env", None))
         if us.volume:
             self.world.add_volume(self.volume, 0)
         self.world.add_isosurface(self.iso, 1)

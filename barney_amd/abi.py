@@ -50,6 +50,22 @@ class DirLight(C.Structure):
     _fields_ = [("direction", f3), ("radiance", C.c_float), ("color", f3)]
 
 
+class EnvMapDesc(C.Structure):
+    _fields_ = [("radiance", fp), ("width", C.c_int32), ("height", C.c_int32), ("direction", f3), ("up", f3),
+                ("scale", C.c_float), ("_pad", C.c_int32)]
+
+
+def env_map_desc(radiance, direction=(1.0, 0.0, 0.0), up=(0.0, 0.0, 1.0), scale=1.0):
+    """(EnvMapDesc, keep-alive array) for an (height, width, 3) float32
+    radiance image (row 0 at theta = 0)."""
+    rad = np.ascontiguousarray(radiance, dtype=np.float32)
+    if rad.ndim != 3 or rad.shape[2] != 3:
+        raise ValueError("env map radiance must be (height, width, 3)")
+    d = EnvMapDesc(fptr(rad), rad.shape[1], rad.shape[0], (C.c_float * 3)(*direction), (C.c_float * 3)(*up),
+                   float(scale), 0)
+    return d, rad
+
+
 class StructuredDesc(C.Structure):
     _fields_ = [("scalars", fp), ("dims", C.c_int32 * 3), ("cells_per_mc", C.c_int32), ("grid_origin", f3),
                 ("grid_spacing", f3)]
@@ -96,7 +112,8 @@ class RenderStats(C.Structure):
                 ("volume_samples", C.c_int64), ("ms_total", C.c_double), ("node_visits", C.c_int64),
                 ("tri_tests", C.c_int64), ("sphere_tests", C.c_int64), ("trace_ms", C.c_double),
                 ("trace_launches", C.c_int64), ("march_ms", C.c_double), ("march_launches", C.c_int64),
-                ("mc_cells", C.c_int64), ("march_rays", C.c_int64)]
+                ("mc_cells", C.c_int64), ("march_rays", C.c_int64), ("umesh_nodes", C.c_int64),
+                ("umesh_tets", C.c_int64)]
 
 
 class BvhInfo(C.Structure):

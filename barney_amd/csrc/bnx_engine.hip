@@ -200,6 +200,7 @@ int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f,
   for (const Volume *v : w->volumes) full = full || v->dd.field != FIELD_STRUCTURED;
   BNX_SET(s.dir_lights, w->d_dir_lights.ptr);
   s.num_dir_lights = (int)w->dir_lights.size();
+  s.env = env_dev(w);
   s.rr = rr;
   s.cam = cam;
   s.W = f->w;
@@ -289,6 +290,8 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     stats->march_launches = tm.march_launches;
     stats->mc_cells = (int64_t)host_st[STAT_CELLS];
     stats->march_rays = tm.march_rays;
+    stats->umesh_nodes = (int64_t)host_st[STAT_UNODES];
+    stats->umesh_tets = (int64_t)host_st[STAT_UTETS];
   }
   return BNX_OK;
 }

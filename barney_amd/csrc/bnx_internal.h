@@ -145,6 +145,12 @@ struct World {
   // versions of the volumes (and isosurface fields) last uploaded; empty
   // until the first upload
   std::vector<uint64_t> vol_seen, iso_seen;
+  // HDRI environment light (no map while env_W == 0)
+  DevMem env_tex;             // float4[W * H]
+  DevMem env_cdf_x, env_cdf_y;
+  int env_W = 0, env_H = 0;
+  float env_scale = 1.f;
+  float env_to_world[9], env_to_local[9];
 };
 
 struct Frame {

@@ -3,6 +3,7 @@
 #pragma once
 
 #include "bnx_internal.h"
+#include "bnx_env.h"
 
 namespace bnx {
 
@@ -56,7 +57,11 @@ struct SlotDev {
   int accum_base;              // accumID of batch sample 0
   int pad;
   FrameDev fb;
+  EnvDev env;                  // HDRI light (env.tex null: ambient radiance)
 };
+
+// the device view of a world's env map
+EnvDev env_dev(const World *w);
 
 // A run of queue entries that all belong to one request slot.  Launches are
 // laid out so that every block lies inside one segment: a block reads its

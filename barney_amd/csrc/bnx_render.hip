@@ -173,16 +173,20 @@ BNX_D bool sample_lights(LightSample &ls, const SlotDev &P, v3 N, Random2 &rando
 {
   needs_mis = false;
   LightSample els, dls;
-  // sampleEnvLight without texture (shadeRays.cu:265-291)
-  els.direction = random_direction(random);
+  // sampleEnvLight (shadeRays.cu:265-291): the HDRI map, else ambient
   els.distance = BNX_INF;
-  if (!(N == splat(0.f))) {
-    els.radiance = splat(TWO_PI * P.rr.ambient_radiance);
-    if (dot(els.direction, N) < 0.f) els.direction = -els.direction;
-    els.pdf = ONE_OVER_TWO_PI;
+  if (P.env.tex) {
+    env_sample(P.env, random, els.direction, els.radiance, els.pdf);
   } else {
-    els.radiance = splat(FOUR_PI * P.rr.ambient_radiance);
-    els.pdf = ONE_OVER_FOUR_PI;
+    els.direction = random_direction(random);
+    if (!(N == splat(0.f))) {
+      els.radiance = splat(TWO_PI * P.rr.ambient_radiance);
+      if (dot(els.direction, N) < 0.f) els.direction = -els.direction;
+      els.pdf = ONE_OVER_TWO_PI;
+    } else {
+      els.radiance = splat(FOUR_PI * P.rr.ambient_radiance);
+      els.pdf = ONE_OVER_FOUR_PI;
+    }
   }
   float elsW = reduce_max(els.radiance);
   float alsW = 0.f;
@@ -403,10 +407,12 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
       const bool front = dot(rdir, Ng) < 0.f;
       v3 Ngff = front ? Ng : -Ng;
       if (!had_hit) {
+        // primaryRayMissColor / radianceFromEnv (shadeRays.cu:435-470, 558-571)
+        const v3 env_l = P.env.tex ? env_radiance(P.env, rdir) : splat(P.rr.ambient_radiance);
         if (gen == 0) {
-          frag = xyz(mc);
+          frag = P.env.tex ? env_l : xyz(mc);
         } else {
-          frag = (tp * splat(P.rr.ambient_radiance)) * mis;
+          frag = (tp * env_l) * mis;
         }
         ray_tmax = -1.f;
       } else {
@@ -444,7 +450,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
             rng_next_h(rng, (uint64_t)__float_as_uint(tmax));
             float smis = 1.f;
             if (!is_dir && needs_mis) {
-              float pl = ONE_OVER_FOUR_PI;
+              float pl = env_pdf(P.env, ls.direction);
               float psd = sh.phase ? ONE_OVER_FOUR_PI : lambert_pdf(Ng, ls.direction);
               smis = pl / ((pl + psd) + 1e-10f);
               if (smis < 1e-5f) sh_tmax = -1.f;
@@ -489,7 +495,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
             float nmis;
             if (needs_mis && !isinf(spdf)) {
               float psd = lambert_pdf(Ng, rdir);
-              float pl = ONE_OVER_FOUR_PI;
+              float pl = env_pdf(P.env, rdir);
               nmis = psd / (psd + pl);
             } else {
               nmis = 1.f;

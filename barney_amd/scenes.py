@@ -116,10 +116,47 @@ class SphereSet:
 
 
 @dataclass
+class EnvMap:
+    """HDRI environment light: (height, width, 3) float32 radiance, row 0 at
+    theta = 0 (the map's local +z), column 0 at phi = 0."""
+    radiance: np.ndarray
+    direction: tuple = (1.0, 0.0, 0.0)
+    up: tuple = (0.0, 0.0, 1.0)
+    scale: float = 1.0
+
+
+def lobe_env_map(width: int = 1024, height: int = 512, num_lobes: int = 4, seed: int = SEEDS["c3"],
+                 sky: float = 0.05) -> EnvMap:
+    """Seeded HDR map of Gaussian lobes on a dim sky: lobe k has an axis
+    uniform on the sphere, angular width sigma U(0.05, 0.3) rad, peak
+    radiance U(5, 50) and colour U[0.5, 1]^3; texel (i, j) holds
+    sky + sum_k peak_k * colour_k * exp((cos(angle to axis_k) - 1) / sigma_k^2)
+    at its centre direction (theta = pi (j + .5) / height, phi = 2 pi (i + .5) / width)."""
+    u = uniform_table(seed, 7, num_lobes, 7)
+    z = 1.0 - 2.0 * u[:, 0]
+    ph = 2.0 * np.pi * u[:, 1]
+    r = np.sqrt(np.maximum(0.0, 1.0 - z * z))
+    axes = np.stack([r * np.cos(ph), r * np.sin(ph), z], 1)
+    sigma = 0.05 + 0.25 * u[:, 2]
+    peak = 5.0 + 45.0 * u[:, 3]
+    col = 0.5 + 0.5 * u[:, 4:7]
+    th = np.pi * (np.arange(height) + 0.5) / height
+    phi = 2.0 * np.pi * (np.arange(width) + 0.5) / width
+    d = np.stack([np.sin(th)[:, None] * np.cos(phi)[None, :], np.sin(th)[:, None] * np.sin(phi)[None, :],
+                  np.broadcast_to(np.cos(th)[:, None], (height, width))], 2)
+    img = np.full((height, width, 3), sky, np.float64)
+    for k in range(num_lobes):
+        c = d @ axes[k]
+        img += (peak[k] * np.exp((c - 1.0) / sigma[k] ** 2))[:, :, None] * col[k][None, None, :]
+    return EnvMap(img.astype(np.float32))
+
+
+@dataclass
 class SurfaceScene:
     meshes: list = field(default_factory=list)
     spheres: list = field(default_factory=list)
     dir_lights: list = field(default_factory=list)   # (direction(3), radiance, color(3))
+    env: EnvMap | None = None
     ambient: float = 1.0
     bg_color: tuple = (0.0, 0.0, 0.0, 1.0)
     camera: dict = field(default_factory=dict)

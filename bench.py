@@ -23,6 +23,7 @@ oracle's throughput on a bounded sample of the same workload.
 from __future__ import annotations
 
 import argparse
+import copy
 import json
 import os
 import sys
@@ -33,6 +34,7 @@ sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s + p50/p99 frame latency at fixed RMSE, served batched vs plain GPU"
 HBM_PEAK_GBS = 8000.0
+CPU_TET_RES = 64            # config 4 CPU sample: the same scene and camera at 64x64
 
 
 def algorithmic_bytes(rays, nodes, tris, spheres):
@@ -66,11 +68,20 @@ def profile_traffic(workload, kernel):
     return None
 
 
+def umesh_bytes(nodes, tets, cells, rays):
+    # tet point queries: 64 B per element-BVH node, 80 B per tet test (4 x 16 B
+    # vertex+scalar, 16 B index); 4 B per macrocell; 48 B per marched ray
+    return 64.0 * nodes + 80.0 * tets + 4.0 * cells + 48.0 * rays
+
+
 def roofline(tot, workload=None):
     """dominant kernel (by summed HIP-event time) and its algorithmic GB/s"""
     if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
         kernel, kms, launches = "k_volume_march", tot["march_ms"], tot["march_launches"]
-        byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
+        if tot.get("umesh_nodes", 0) > 0:
+            byts = umesh_bytes(tot["umesh_nodes"], tot["umesh_tets"], tot["mc_cells"], tot["march_rays"])
+        else:
+            byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
     else:
         kernel, kms, launches = "k_trace_queue", tot["trace_ms"], tot["trace_launches"]
         byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
@@ -98,6 +109,19 @@ def workload_c2(args):
     return sc, desc
 
 
+def workload_c4(args):
+    from barney_amd import scenes
+    n = args.umesh_n
+    sc = scenes.config4(n=n, width=args.width or 1024, height=args.height or 1024, spp=args.spp or 32)
+    desc = {"workload": f"config 4: unstructured volume, jittered {n}^3 hex lattice split into 5 tets per hex "
+                        f"({len(sc.tets):,} tets, {len(sc.vertices):,} vertices), per-vertex sin(4x)cos(4y)sin(4z)"
+                        f" + N(0,0.05^2), element-BVH point location, 32^3 majorant grid, delta tracking + "
+                        f"isosurface at 0.3, {sc.width}x{sc.height}, {sc.spp} spp",
+            "width": sc.width, "height": sc.height, "spp": sc.spp, "tets": int(len(sc.tets)),
+            "majorant_grid": 32, "iso": 0.3, "form": "plain single request"}
+    return sc, desc
+
+
 def workload_c1(args):
     from barney_amd import scenes
     sc = scenes.config1(width=args.width or 512, height=args.height or 512, spp=1)
@@ -114,7 +138,15 @@ def run_gpu(sc, steps, warmup):
     import barney_amd as bx
     from barney_amd import abi, scenes
     dev = bx.Device(0)
-    w = bx.VolumeWorld(dev, sc) if isinstance(sc, scenes.StructuredVolumeScene) else bx.SurfaceWorld(dev, sc)
+    t0 = time.perf_counter()
+    if isinstance(sc, scenes.StructuredVolumeScene):
+        w = bx.VolumeWorld(dev, sc)
+    elif isinstance(sc, scenes.UMeshVolumeScene):
+        w = bx.UMeshWorld(dev, sc)
+    else:
+        w = bx.SurfaceWorld(dev, sc)
+    dev.sync()
+    log(f"world built in {time.perf_counter() - t0:.2f} s")
     cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
     rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient)
     fr = bx.Frame(dev, sc.width, sc.height)
@@ -122,14 +154,17 @@ def run_gpu(sc, steps, warmup):
         fr.reset()
         bx.render(w.world, cam, rend, fr, sc.spp)
     dev.sync()
+    log("warm-up done")
     lat, tot = [], dict(rays=0, nodes=0, tris=0, spheres=0, trace_ms=0.0, trace_launches=0, volume_samples=0,
-                        march_ms=0.0, march_launches=0, mc_cells=0, march_rays=0)
+                        march_ms=0.0, march_launches=0, mc_cells=0, march_rays=0, umesh_nodes=0, umesh_tets=0)
     t0 = time.perf_counter()
     for _ in range(steps):
         ts = time.perf_counter()
         fr.reset()
         st = bx.render(w.world, cam, rend, fr, sc.spp)
         lat.append((time.perf_counter() - ts) * 1e3)
+        if lat[-1] > 1000.0:
+            log(f"frame {len(lat)}: {lat[-1]:.0f} ms")
         for k in tot:
             tot[k] += st[k]
     dev.sync()
@@ -145,8 +180,16 @@ def run_cpu(sc, budget_s):
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
     threads = oracle.default_threads()
-    vol = isinstance(sc, scenes.StructuredVolumeScene)
-    o = oracle.volume_scene(sc) if vol else oracle.surface_scene(sc)
+    vol = isinstance(sc, (scenes.StructuredVolumeScene, scenes.UMeshVolumeScene))
+    if isinstance(sc, scenes.UMeshVolumeScene):
+        t0 = time.perf_counter()
+        o = oracle.umesh_scene(sc)
+        log(f"oracle tet scene built in {time.perf_counter() - t0:.1f} s")
+        # the same scene and camera at a reduced resolution: a bounded sample
+        sc = copy.copy(sc)
+        sc.width, sc.height = CPU_TET_RES, CPU_TET_RES
+    else:
+        o = oracle.volume_scene(sc) if vol else oracle.surface_scene(sc)
     spp = 1 if vol else sc.spp
     cam = oracle.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"],
                                     sc.camera["fovy"])
@@ -162,7 +205,7 @@ def run_cpu(sc, budget_s):
         frames += 1
     el = time.perf_counter() - t0
     out = {"value": rays / el / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
-           "sample": f"{frames} full {sc.width}x{sc.height} frames at {spp} spp of the same scene "
+           "sample": f"{frames} {sc.width}x{sc.height} frames at {spp} spp of the same scene and camera "
                      f"({el:.1f} s wall on {threads} threads; shadow rays traced as closest hit as in "
                      f"the reference CPU path)"}
     if vol:
@@ -389,7 +432,8 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c5"])
+    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c4", "c5"])
+    ap.add_argument("--umesh-n", type=int, default=160, help="c4: hex lattice size (160: 20.48M tets)")
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--width", type=int, default=0)
     ap.add_argument("--height", type=int, default=0)
@@ -415,9 +459,10 @@ def main():
         mrays, elapsed, tot, desc, extra = run_c5(args, steps, warmup)
         lat_line = extra.pop("frame_latency_ms")
     else:
-        steps = args.steps if args.steps is not None else 20
-        warmup = args.warmup if args.warmup is not None else 3
-        sc, desc = (workload_c2 if args.workload == "c2" else workload_c1)(args)
+        heavy = args.workload == "c4"
+        steps = args.steps if args.steps is not None else (3 if heavy else 20)
+        warmup = args.warmup if args.warmup is not None else (1 if heavy else 3)
+        sc, desc = {"c1": workload_c1, "c2": workload_c2, "c4": workload_c4}[args.workload](args)
         elapsed, lat, tot = run_gpu(sc, steps, warmup)
         mrays = tot["rays"] / elapsed / 1e6
         lat_line = {"p50": pct(lat, 50), "p99": pct(lat, 99)}

@@ -142,6 +142,25 @@ typedef struct bnx_dir_light {
   float color[3];
 } bnx_dir_light;
 
+/* HDRI environment light: EnvMapLight (light/EnvMap.h:27-90, EnvMap.cu:
+ * 165-239) set up as anari/light/HDRI.cpp:21-80 does.  'radiance' holds
+ * width x height float3 texels, row-major; u = column / width runs over phi
+ * in [0, 2 pi) and wraps, v = row / height runs over theta in [0, pi] and
+ * clamps, with bilinear filtering.  The map's frame: toWorld.vz = -up,
+ * toWorld.vy = -normalize(cross(vz, direction)), toWorld.vx =
+ * normalize(cross(vy, vz)).  The light replaces the renderer's ambient
+ * radiance as the environment: primary rays that miss show it, light
+ * sampling draws from its importance CDFs, and its pdf enters the MIS
+ * weights. */
+typedef struct bnx_env_map_desc {
+  const float *radiance;
+  int32_t width, height;
+  float direction[3];               /* 'direction' (default 1, 0, 0)               */
+  float up[3];                      /* 'up' (default 0, 0, 1)                      */
+  float scale;                      /* 'scale' (default 1)                         */
+  int32_t _pad;
+} bnx_env_map_desc;
+
 /* StructuredData (volume/StructuredData.h:30-74, set3i "dims", set3f
  * "gridOrigin"/"gridSpacing"): scalars at dims[0]*dims[1]*dims[2] grid
  * vertices, x fastest; cells_per_mc is the macrocell edge in cells
@@ -210,6 +229,8 @@ typedef struct bnx_render_stats {
   int64_t march_launches;
   int64_t mc_cells;                 /* macrocells stepped by the volume DDA        */
   int64_t march_rays;               /* rays entering the volume-march kernel       */
+  int64_t umesh_nodes;              /* element-BVH nodes visited by tet point queries */
+  int64_t umesh_tets;               /* tets tested by tet point queries            */
 } bnx_render_stats;
 
 /* BVH summary of one BLAS / TLAS (for tests and the byte model). */
@@ -327,6 +348,13 @@ int bnx_world_add_volume(bnx_world world, bnx_volume vol, int32_t user_inst_id);
  * the surface trace and before the volumes, in the order they were added,
  * and act as closest-hit surfaces. */
 int bnx_world_add_isosurface(bnx_world world, bnx_isosurface iso, int32_t user_inst_id);
+/* Sets (or, with desc NULL or desc->radiance NULL, removes) the world's
+ * HDRI environment light (World::DD::envMapLight, render/World.h:32-47):
+ * uploads the texture and builds its importance CDFs on the device
+ * (EnvMapLight::computeCDFs, light/EnvMap.cu:183-239).  A server's
+ * admitted requests hold the map's device memory: do not change the map
+ * while requests on this world are in flight. */
+int bnx_world_set_env_map(bnx_world world, const bnx_env_map_desc *desc);
 int bnx_world_destroy(bnx_world world);
 
 /* -------------------------------------------------------------- camera */
@@ -435,6 +463,7 @@ static_assert(sizeof(bnx_hit) == 32, "bnx_hit layout");
 static_assert(sizeof(bnx_material) == 32, "bnx_material layout");
 static_assert(sizeof(bnx_camera) == 60, "bnx_camera layout");
 static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
+static_assert(sizeof(bnx_env_map_desc) == 48, "bnx_env_map_desc layout");
 static_assert(sizeof(bnx_triangles_desc) == 72, "bnx_triangles_desc layout");
 static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
 static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
@@ -443,7 +472,7 @@ static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
 static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
 static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
-static_assert(sizeof(bnx_render_stats) == 112, "bnx_render_stats layout");
+static_assert(sizeof(bnx_render_stats) == 128, "bnx_render_stats layout");
 #endif
 
 #endif /* BNX_RT_H */

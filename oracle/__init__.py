@@ -41,6 +41,16 @@ def lib():
         L.orc_scene_add_group.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
         L.orc_scene_add_instance.argtypes = [vp, C.c_int, C.c_int, abi.fp, abi.fp]
         L.orc_scene_set_dir_lights.argtypes = [vp, C.POINTER(abi.DirLight), C.c_int]
+        L.orc_scene_set_env_map.argtypes = [vp, C.POINTER(abi.EnvMapDesc)]
+        L.orc_env_cdfs.argtypes = [vp, abi.fp, abi.fp, abi.fp, abi.fp]
+        L.orc_env_tex.argtypes = [vp, abi.fp, C.c_int, abi.fp]
+        L.orc_env_pdf.argtypes = [vp, abi.fp, C.c_int, abi.fp]
+        L.orc_env_radiance.argtypes = [vp, abi.fp, C.c_int, abi.fp]
+        L.orc_env_sample.argtypes = [vp, C.c_uint32, C.c_uint32, C.c_int, abi.fp, abi.fp, abi.fp]
+        L.orc_env_uv_to_world.argtypes = [vp, abi.fp, C.c_int, abi.fp]
+        L.orc_atan2_n.argtypes = [abi.fp, abi.fp, C.c_int, abi.fp]
+        L.orc_acos_n.argtypes = [abi.fp, C.c_int, abi.fp]
+        L.orc_env_frame.argtypes = [abi.fp, abi.fp, abi.fp, abi.fp]
         L.orc_scene_add_volume.argtypes = [vp, C.POINTER(abi.StructuredDesc), C.POINTER(abi.TransferFunction),
                                            C.c_int, C.c_int]
         L.orc_volume_majorants.argtypes = [vp, C.c_int, abi.ip, abi.fp]
@@ -204,6 +214,58 @@ class Scene:
         self._keep.append(arr)
         self.L.orc_scene_set_dir_lights(self.h, arr, len(lights))
 
+    def set_env_map(self, radiance, direction=(1.0, 0.0, 0.0), up=(0.0, 0.0, 1.0), scale=1.0):
+        """HDRI light from an (height, width, 3) radiance image; None removes it"""
+        if radiance is None:
+            self.L.orc_scene_set_env_map(self.h, None)
+            self.env_shape = None
+            return
+        d, keep = abi.env_map_desc(radiance, direction, up, scale)
+        assert self.L.orc_scene_set_env_map(self.h, C.byref(d)) == 0
+        self.env_shape = keep.shape[:2]
+
+    def env_cdfs(self):
+        """(cdf_x (H, W), cdf_y (H,), to_world (3x3, columns), to_local)"""
+        H, W = self.env_shape
+        cx = np.zeros((H, W), np.float32)
+        cy = np.zeros(H, np.float32)
+        tw = np.zeros(9, np.float32)
+        tl = np.zeros(9, np.float32)
+        self.L.orc_env_cdfs(self.h, abi.fptr(cx), abi.fptr(cy), abi.fptr(tw), abi.fptr(tl))
+        return cx, cy, tw.reshape(3, 3).T.copy(), tl.reshape(3, 3).T.copy()
+
+    def env_tex(self, uv):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 4), np.float32)
+        self.L.orc_env_tex(self.h, abi.fptr(uv), len(uv), abi.fptr(out))
+        return out
+
+    def env_pdf(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros(len(d), np.float32)
+        self.L.orc_env_pdf(self.h, abi.fptr(d), len(d), abi.fptr(out))
+        return out
+
+    def env_radiance(self, dirs):
+        d = np.ascontiguousarray(dirs, np.float32).reshape(-1, 3)
+        out = np.zeros((len(d), 3), np.float32)
+        self.L.orc_env_radiance(self.h, abi.fptr(d), len(d), abi.fptr(out))
+        return out
+
+    def env_sample(self, n, seed=(1, 2)):
+        """n samples in sequence from Random2(seed): (dirs, radiance, pdf)"""
+        d = np.zeros((n, 3), np.float32)
+        r = np.zeros((n, 3), np.float32)
+        p = np.zeros(n, np.float32)
+        self.L.orc_env_sample(self.h, seed[0], seed[1], n, abi.fptr(d), abi.fptr(r), abi.fptr(p))
+        return d, r, p
+
+    def env_uv_to_world(self, uv):
+        uv = np.ascontiguousarray(uv, np.float32).reshape(-1, 2)
+        out = np.zeros((len(uv), 3), np.float32)
+        self.L.orc_env_uv_to_world(self.h, abi.fptr(uv), len(uv), abi.fptr(out))
+        return out
+
     def build(self) -> float:
         return self.L.orc_scene_build(self.h)
 
@@ -294,6 +356,9 @@ def surface_scene(sc) -> Scene:
     if tri:
         s.add_instance(s.add_group(0, tri), 0)
     s.set_dir_lights(sc.dir_lights)
+    env = getattr(sc, "env", None)
+    if env is not None:
+        s.set_env_map(env.radiance, env.direction, env.up, env.scale)
     s.build()
     return s
 
@@ -304,6 +369,9 @@ def volume_scene(vs) -> Scene:
     s = Scene()
     s.add_volume(vs, user_id=vs.user_id, user_inst=0)
     s.set_dir_lights(vs.dir_lights)
+    env = getattr(vs, "env", None)
+    if env is not None:
+        s.set_env_map(env.radiance, env.direction, env.up, env.scale)
     s.build()
     return s
 
@@ -316,5 +384,8 @@ def umesh_scene(us) -> Scene:
     k = s.add_umesh(us, user_id=us.user_id, user_inst=0, rendered=us.volume)
     s.add_isosurface(k, us.iso_value, us.iso_color, user_id=us.iso_user_id, user_inst=1)
     s.set_dir_lights(us.dir_lights)
+    env = getattr(us, "env", None)
+    if env is not None:
+        s.set_env_map(env.radiance, env.direction, env.up, env.scale)
     s.build()
     return s

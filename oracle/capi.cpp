@@ -230,6 +230,70 @@ void orc_scene_set_dir_lights(orc_scene *sc, const bnx_dir_light *l, int n)
   sc->s.dir_lights.assign(l, l + n);
 }
 
+// HDRI light as bnx_world_set_env_map sets it; radiance NULL removes it
+int orc_scene_set_env_map(orc_scene *sc, const bnx_env_map_desc *d)
+{
+  EnvMap &e = sc->s.env;
+  e = EnvMap();
+  if (!d || !d->radiance) return 0;
+  if (d->width < 1 || d->height < 1) return -1;
+  e.present = true;
+  e.W = d->width;
+  e.H = d->height;
+  e.scale = d->scale;
+  const size_t n = (size_t)e.W * e.H;
+  e.tex.assign(4 * n, 0.f);
+  for (size_t i = 0; i < n; i++)
+    for (int k = 0; k < 3; k++) e.tex[4 * i + k] = d->radiance[3 * i + k];
+  bnx_num::env_frame(d->direction, d->up, e.to_world, e.to_local);
+  env_build(e);
+  return 0;
+}
+void orc_env_cdfs(orc_scene *sc, float *cdf_x, float *cdf_y, float *to_world, float *to_local)
+{
+  const EnvMap &e = sc->s.env;
+  memcpy(cdf_x, e.cdf_x.data(), e.cdf_x.size() * 4);
+  memcpy(cdf_y, e.cdf_y.data(), e.cdf_y.size() * 4);
+  memcpy(to_world, e.to_world, 36);
+  memcpy(to_local, e.to_local, 36);
+}
+void orc_env_tex(orc_scene *sc, const float *uv, int n, float *out)
+{
+  for (int i = 0; i < n; i++) env_tex(sc->s.env, uv[2 * i], uv[2 * i + 1], out + 4 * i);
+}
+void orc_env_pdf(orc_scene *sc, const float *dirs, int n, float *out)
+{
+  for (int i = 0; i < n; i++) out[i] = env_pdf(sc->s.env, load3(dirs + 3 * i));
+}
+void orc_env_radiance(orc_scene *sc, const float *dirs, int n, float *out)
+{
+  for (int i = 0; i < n; i++) {
+    vec3 r = env_radiance(sc->s.env, load3(dirs + 3 * i));
+    out[3 * i] = r.x; out[3 * i + 1] = r.y; out[3 * i + 2] = r.z;
+  }
+}
+// n light samples drawn in sequence from Random2(a, b); also returns the
+// sampled uv of each (uv = (ix + r) / W, (iy + r) / H)
+void orc_env_sample(orc_scene *sc, uint32_t a, uint32_t b, int n, float *dirs, float *rads, float *pdfs)
+{
+  Random2 r(a, b);
+  for (int i = 0; i < n; i++) {
+    vec3 d, rad;
+    float pdf;
+    env_sample(sc->s.env, r, d, rad, pdf);
+    dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+    rads[3 * i] = rad.x; rads[3 * i + 1] = rad.y; rads[3 * i + 2] = rad.z;
+    pdfs[i] = pdf;
+  }
+}
+void orc_env_uv_to_world(orc_scene *sc, const float *uv, int n, float *out)
+{
+  for (int i = 0; i < n; i++) {
+    vec3 d = env_uv_to_world(sc->s.env, uv[2 * i], uv[2 * i + 1]);
+    out[3 * i] = d.x; out[3 * i + 1] = d.y; out[3 * i + 2] = d.z;
+  }
+}
+
 double orc_scene_build(orc_scene *sc)
 {
   auto t0 = std::chrono::steady_clock::now();
@@ -356,6 +420,18 @@ int orc_frame_read(orc_frame *fb, int channel, void *host)
 
 // --- numeric helpers for the known-answer tests
 void orc_sincos(float x, float *s, float *c) { bnx_num::sincos(x, *s, *c); }
+void orc_atan2_n(const float *y, const float *x, int n, float *out)
+{
+  for (int i = 0; i < n; i++) out[i] = bnx_num::atan2(y[i], x[i]);
+}
+void orc_acos_n(const float *x, int n, float *out)
+{
+  for (int i = 0; i < n; i++) out[i] = bnx_num::acos(x[i]);
+}
+void orc_env_frame(const float *dir, const float *up, float *to_world, float *to_local)
+{
+  bnx_num::env_frame(dir, up, to_world, to_local);
+}
 uint16_t orc_f32_to_f16(float f) { return f32_to_f16(f); }
 float orc_f16_to_f32(uint16_t h) { return f16_to_f32(h); }
 // RNG: seed(a,b) from the default state, then n outputs of next()

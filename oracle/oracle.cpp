@@ -26,6 +26,17 @@
 //   owl::common::frame(N) ........... parity unpinned against the reference
 //                                     (un-vendored OWL); restated as Duff et
 //                                     al. 2017 and pinned by orthonormality.
+//   env map (env.cpp) ............... tests/test_env_cpu.py: texture fetch
+//                                     against a float64 bilinear lookup (texel
+//                                     centres exact, seam, poles); CDFs
+//                                     against closed forms for row-only and
+//                                     column-only maps; samples map back to
+//                                     their texel with pdf_x pdf_y /
+//                                     (2 pi^2 sin theta); the pdf integrates
+//                                     to 1; E[L / pdf] = integral of L; a
+//                                     sphere's pixels match the MIS
+//                                     expectation integral.  atan2 / acos
+//                                     (bnx_numeric.h) within 3 ulp of libm.
 #include "oracle.h"
 
 #include <algorithm>

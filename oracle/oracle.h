@@ -273,11 +273,36 @@ bool tet_scalar(const Volume &v, int slot, vec3 P, float &out);
 bool iso_march(const Volume &field, const IsoSurface &iso, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin,
                float tmax, RNGSeed &parent, float &t_out, vec3 &normal);
 
+// EnvMapLight (light/EnvMap.h:27-90) over its HDRI texture: float4 texels,
+// bilinear, WRAP in u and CLAMP in v (anari/light/HDRI.cpp:43-78)
+struct EnvMap {
+  bool present = false;
+  int W = 0, H = 0;
+  std::vector<float> tex;           // rgba per texel, row-major, row 0 at theta = 0
+  float scale = 1.f;
+  float to_world[9], to_local[9];   // columns vx, vy, vz
+  std::vector<float> cdf_x;         // W per row (allCDFs_x)
+  std::vector<float> cdf_y;         // H
+};
+// EnvMapLight::computeCDFs (light/EnvMap.cu:61-138, 183-239)
+void env_build(EnvMap &e);
+// rtc::tex2D of the env texture (rtcore/cpu/Texture.cpp:24-89, 294-321)
+void env_tex(const EnvMap &e, float u, float v, float out[4]);
+// EnvMapLight::DD::pdf (light/EnvMap.h:135-160); 1/(4 pi) without a texture
+float env_pdf(const EnvMap &e, vec3 dir);
+// EnvMapLight::DD::sample (light/EnvMap.h:211-250): four draws of r
+void env_sample(const EnvMap &e, Random2 &r, vec3 &direction, vec3 &radiance, float &pdf);
+// radianceFromEnv with a texture (kernels/shadeRays.cu:435-454)
+vec3 env_radiance(const EnvMap &e, vec3 ray_dir);
+// EnvMapLight::DD::uvToWorld (light/EnvMap.h:172-182)
+vec3 env_uv_to_world(const EnvMap &e, float fx, float fy);
+
 struct Scene {
   std::vector<Geom> geoms;
   std::vector<Group> groups;
   std::vector<Instance> insts;
   Bvh tlas;
+  EnvMap env;
   std::vector<bnx_dir_light> dir_lights;
   std::vector<Volume> volumes;      // marched after the surface trace, in order
   std::vector<IsoSurface> isos;     // tested after the surface trace, before the volumes

@@ -67,9 +67,15 @@ struct LightSample {
   float pdf;
 };
 
-// sampleEnvLight without an env-map texture (shadeRays.cu:265-291)
-bool sample_env(LightSample &ls, float ambient, vec3 N, Random2 &random)
+// sampleEnvLight (shadeRays.cu:265-291): the HDRI map when there is one,
+// else uniform ambient radiance
+bool sample_env(LightSample &ls, const EnvMap &env, float ambient, vec3 N, Random2 &random)
 {
+  if (env.present) {
+    env_sample(env, random, ls.direction, ls.radiance, ls.pdf);
+    ls.distance = BNX_INF;
+    return true;
+  }
   ls.direction = random_direction(random);
   ls.distance = BNX_INF;
   if (N != vec3(0.f)) {
@@ -127,7 +133,7 @@ bool sample_lights(LightSample &ls, const Scene &s, const bnx_renderer &r, vec3 
 {
   needs_mis = false;
   LightSample els, dls;
-  float elsW = sample_env(els, r.ambient_radiance, N, random) ? reduce_max(els.radiance) : 0.f;
+  float elsW = sample_env(els, s.env, r.ambient_radiance, N, random) ? reduce_max(els.radiance) : 0.f;
   float alsW = 0.f;
   float dlsW = sample_dir(dls, s.dir_lights, N, random) ? reduce_max(dls.radiance) : 0.f;
   float plsW = 0.f;
@@ -210,8 +216,9 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
   const bool front = dot(ray.dir, Ng) < 0.f;
   vec3 Ngff = front ? Ng : -Ng;
   if (no_hit) {
-    if (depth == 0) frag = vec3(ray.miss[0], ray.miss[1], ray.miss[2]);
-    else frag = (ray.tp * vec3(rr.ambient_radiance)) * ray.mis;
+    // primaryRayMissColor / radianceFromEnv (shadeRays.cu:435-470, 558-571)
+    if (depth == 0) frag = s.env.present ? env_radiance(s.env, ray.dir) : vec3(ray.miss[0], ray.miss[1], ray.miss[2]);
+    else frag = (ray.tp * (s.env.present ? env_radiance(s.env, ray.dir) : vec3(rr.ambient_radiance))) * ray.mis;
     ray.tmax = -1.f;
     return;
   }
@@ -255,7 +262,7 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
       shadow.pixel = ray.pixel;
       shadow.mis = 1.f;
       if (!is_dir && needs_mis) {
-        float pl = ONE_OVER_FOUR_PI;      // EnvMapLight::DD::pdf without texture (light/EnvMap.h:142-151)
+        float pl = env_pdf(s.env, ls.direction);     // EnvMapLight::DD::pdf (light/EnvMap.h:135-160)
         float ps = ray.bsdf == 2 ? ONE_OVER_FOUR_PI : lambert_pdf(dg, ls.direction);
         shadow.mis = pl / ((pl + ps) + 1e-10f);
         if (shadow.mis < 1e-5f) shadow.tmax = -1.f;
@@ -296,7 +303,7 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
   ray.tp = quantize_half(ray.tp * sf);
   if (needs_mis && !std::isinf(spdf)) {
     float ps = lambert_pdf(dg, ray.dir);
-    float pl = ONE_OVER_FOUR_PI;
+    float pl = env_pdf(s.env, ray.dir);
     ray.mis = ps / (ps + pl);
   } else {
     ray.mis = 1.f;
