@@ -492,8 +492,10 @@ class SurfaceWorld:
 
     def __init__(self, dev: Device, sc):
         self.dev = dev
-        self.sph = [Geom.spheres(dev, s.centers, s.radii, s.colors, user_id=s.user_id) for s in sc.spheres]
-        self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id) for m in sc.meshes]
+        self.sph = [Geom.spheres(dev, s.centers, s.radii, s.colors, user_id=s.user_id, material=s.material)
+                    for s in sc.spheres]
+        self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
+                    for m in sc.meshes]
         groups = []
         if self.sph:
             groups.append(Group.geoms(dev, USER, self.sph).build())

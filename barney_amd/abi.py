@@ -7,6 +7,7 @@ import numpy as np
 
 BNX_OK = 0
 BNX_MATERIAL_MATTE = 0
+BNX_MATERIAL_PBR = 1
 BNX_CAMERA_PERSPECTIVE = 1
 BNX_RENDER_COMPAT_SPHERE_INSTID13 = 1
 BNX_FB_COLOR, BNX_FB_DEPTH, BNX_FB_PRIMID, BNX_FB_INSTID, BNX_FB_OBJID, BNX_FB_NORMAL = range(6)
@@ -21,7 +22,8 @@ ip = C.POINTER(C.c_int32)
 
 class Material(C.Structure):
     _fields_ = [("type", C.c_int32), ("color_from_attribute", C.c_int32), ("color", f4),
-                ("opacity", C.c_float), ("_pad", C.c_float)]
+                ("opacity", C.c_float), ("metallic", C.c_float), ("roughness", C.c_float),
+                ("transmission", C.c_float), ("ior", C.c_float), ("_pad", C.c_float)]
 
 
 class TrianglesDesc(C.Structure):
@@ -148,7 +150,7 @@ class ServerStats(C.Structure):
 
 BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE = 0, 1, 2
 
-assert C.sizeof(Material) == 32 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
+assert C.sizeof(Material) == 48 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
 assert C.sizeof(ServerConfig) == 24 and C.sizeof(Request) == 136 and C.sizeof(RequestStatus) == 48
 assert C.sizeof(UMeshDesc) == 64
 assert C.sizeof(ServerStats) == 168 and Request.renderer.offset == 76 and Request.spp.offset == 124
@@ -177,6 +179,19 @@ def matte(color=(0.8, 0.8, 0.8, 1.0), from_attribute=True, opacity=1.0) -> Mater
     m.color_from_attribute = 1 if from_attribute else 0
     m.color = f4(*color)
     m.opacity = opacity
+    m.metallic, m.roughness, m.transmission, m.ior = 1.0, 1.0, 0.0, 1.45
+    return m
+
+
+def pbr(base_color=(1.0, 1.0, 1.0, 1.0), from_attribute=True, metallic=1.0, roughness=1.0, opacity=1.0,
+        transmission=0.0, ior=1.45) -> Material:
+    """ANARI physicallyBased material (AnariPBR); defaults are ANARI's"""
+    m = Material()
+    m.type = BNX_MATERIAL_PBR
+    m.color_from_attribute = 1 if from_attribute else 0
+    m.color = f4(*base_color)
+    m.opacity = opacity
+    m.metallic, m.roughness, m.transmission, m.ior = metallic, roughness, transmission, ior
     return m
 
 

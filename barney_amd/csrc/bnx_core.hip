@@ -207,6 +207,10 @@ int build_tlas(Group *g)
         gd.color_from_attr = gm->mat.color_from_attribute;
         gd.num_prims = gm->num_prims;
         gd.color = make_float4(gm->mat.color[0], gm->mat.color[1], gm->mat.color[2], gm->mat.color[3]);
+        gd.material = gm->mat.type;
+        gd.metallic = gm->mat.metallic;
+        gd.roughness = gm->mat.roughness;
+        gd.ior = gm->mat.ior;
         BNX_SET(gd.vertices, gm->vertices.as<float>());
         BNX_SET(gd.indices, gm->indices.as<int>());
         BNX_SET(gd.colors, gm->colors.ptr ? gm->colors.as<float>() : nullptr);
@@ -444,6 +448,28 @@ size_t bnx_buffer_size(bnx_buffer b) { return b ? b->mem.bytes : 0; }
 // ------------------------------------------------------------------- geoms
 static void set_mat(Geom *g, const bnx_material &m) { g->mat = m; }
 
+// AnariMatte, or AnariPBR creating the NVisii BSDF (AnariPBR.h:62-96).
+// Not provided: the Glass BSDF (PBR with ior != 1 and transmission >=
+// 1e-3), and on triangles the stochastic opacity of a material whose
+// opacity (matte) or alpha = (1 - transmission) * baseColor.a * opacity
+// (PBR) is below 1 (Triangles.dev.cu:120-133); spheres ignore opacity.
+static int check_material(const bnx_material &m, bool triangles)
+{
+  BNX_CHECK(m.type == BNX_MATERIAL_MATTE || m.type == BNX_MATERIAL_PBR, BNX_ERR_UNSUPPORTED,
+            "only matte and physically-based materials are provided");
+  if (m.type == BNX_MATERIAL_PBR)
+    BNX_CHECK(m.ior == 1.f || m.transmission < 1e-3f, BNX_ERR_UNSUPPORTED,
+              "the Glass BSDF (PBR transmission with ior != 1) is not provided");
+  if (triangles) {
+    const float a = m.type == BNX_MATERIAL_PBR
+                        ? ((1.f - m.transmission) * (m.color_from_attribute ? 1.f : m.color[3])) * m.opacity
+                        : m.opacity;
+    BNX_CHECK(a >= 1.f, BNX_ERR_UNSUPPORTED,
+              "stochastic opacity (Triangles.dev.cu:120-133) is not provided; opacity must be 1");
+  }
+  return BNX_OK;
+}
+
 int bnx_geom_create_triangles(bnx_device d, const bnx_triangles_desc *desc, bnx_geom *out)
 {
   GUARD_BEGIN
@@ -451,9 +477,7 @@ int bnx_geom_create_triangles(bnx_device d, const bnx_triangles_desc *desc, bnx_
   BNX_CHECK(desc->num_triangles >= 0 && desc->num_vertices >= 0, BNX_ERR_INVALID, "negative count");
   BNX_CHECK(desc->num_triangles == 0 || (desc->vertices && desc->indices), BNX_ERR_INVALID,
             "bnx_geom_create_triangles: missing vertices or indices");
-  BNX_CHECK(desc->material.type == BNX_MATERIAL_MATTE, BNX_ERR_UNSUPPORTED, "only matte materials are provided");
-  BNX_CHECK(desc->material.opacity >= 1.f, BNX_ERR_UNSUPPORTED,
-            "stochastic opacity (Triangles.dev.cu:120-133) is not provided; opacity must be 1");
+  if (int rc = check_material(desc->material, true)) return rc;
   for (int i = 0; i < 3 * desc->num_triangles; i++)
     BNX_CHECK(desc->indices[i] >= 0 && desc->indices[i] < desc->num_vertices, BNX_ERR_INVALID,
               "bnx_geom_create_triangles: index out of range");
@@ -486,7 +510,7 @@ int bnx_geom_create_spheres(bnx_device d, const bnx_spheres_desc *desc, bnx_geom
   BNX_CHECK(d && desc && out, BNX_ERR_INVALID, "bnx_geom_create_spheres: null argument");
   BNX_CHECK(desc->num_spheres >= 0, BNX_ERR_INVALID, "negative count");
   BNX_CHECK(desc->num_spheres == 0 || desc->centers, BNX_ERR_INVALID, "bnx_geom_create_spheres: missing centers");
-  BNX_CHECK(desc->material.type == BNX_MATERIAL_MATTE, BNX_ERR_UNSUPPORTED, "only matte materials are provided");
+  if (int rc = check_material(desc->material, false)) return rc;
   bnx_geom g = new bnx_geom_s;
   g->dev = d;
   g->type = 1;

@@ -163,7 +163,9 @@ struct GeomDev {
   int user_id;
   int color_from_attr;
   int num_prims;
-  float4 color;           // constant material color
+  float4 color;           // constant material color (matte 'color', PBR 'baseColor')
+  int material;           // BNX_MATERIAL_MATTE or BNX_MATERIAL_PBR
+  float metallic, roughness, ior;   // PBR parameters as given (rounded to fp16 per hit)
   const float BNX_G *vertices;   // triangles: float3
   const int BNX_G *indices;      //            int3
   const float BNX_G *colors;     // per-prim float3 or null

@@ -22,6 +22,7 @@
 #include "bnx_internal.h"
 #include "bnx_render.h"
 #include "bnx_volume.h"
+#include "bnx_pbr.h"
 
 #include <chrono>
 
@@ -101,6 +102,8 @@ struct SurfHit {
   v3 P, N;          // N: fp16-rounded world normal (zero for a volume event)
   v3 albedo;        // Lambertian albedo or SimplePhase albedo
   bool phase;       // volume event: SimplePhase (packedBSDF/Phase.h:119-172)
+  bool pbr;         // AnariPBR surface: NVisii BSDF with parameters pp
+  pbr::Params pp;
 };
 
 BNX_D void onb(v3 n, v3 &b1, v3 &b2)
@@ -280,7 +283,9 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
   } else {
     base0 = G.color.x; base1 = G.color.y; base2 = G.color.z;
   }
-  sh.albedo = (0.85f * v3{base0, base1, base2}) * ONE_OVER_PI;
+  sh.pbr = G.material == BNX_MATERIAL_PBR;
+  if (sh.pbr) sh.pp = pbr::params(v3{base0, base1, base2}, G.metallic, G.roughness, G.ior);
+  else sh.albedo = (0.85f * v3{base0, base1, base2}) * ONE_OVER_PI;
   sh.P = P;
   sh.N = quant_half(Nw);
   sh.phase = false;
@@ -296,6 +301,7 @@ BNX_D SurfHit eval_volume_hit(float t, float4 rgb, v3 rorg, v3 rdir)
   sh.N = splat(0.f);
   sh.albedo = .7f * v3{rgb.x, rgb.y, rgb.z};
   sh.phase = true;
+  sh.pbr = false;
   return sh;
 }
 
@@ -310,6 +316,7 @@ BNX_D SurfHit eval_iso_hit(float t, float4 n, v3 rorg, v3 rdir, const IsoDev &is
   sh.N = quant_half(v3{n.x, n.y, n.z});
   sh.albedo = (0.85f * v3{iso.color[0], iso.color[1], iso.color[2]}) * ONE_OVER_PI;
   sh.phase = false;
+  sh.pbr = false;
   return sh;
 }
 
@@ -429,6 +436,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
           if (sh.phase) {
             fpdf = ONE_OVER_FOUR_PI;
             fv = ONE_OVER_FOUR_PI * sh.albedo;
+          } else if (sh.pbr) {
+            pbr_eval(sh.pp, Ng, wo, ls.direction, fv, fpdf);
           } else {
             float c = fmx(dot(ls.direction, Ng), 0.f);
             fpdf = c * ONE_OVER_PI;
@@ -451,7 +460,9 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
             float smis = 1.f;
             if (!is_dir && needs_mis) {
               float pl = env_pdf(P.env, ls.direction);
-              float psd = sh.phase ? ONE_OVER_FOUR_PI : lambert_pdf(Ng, ls.direction);
+              float psd = sh.phase ? ONE_OVER_FOUR_PI
+                          : sh.pbr ? pbr_pdf(sh.pp, Ng, wo, ls.direction)
+                                   : lambert_pdf(Ng, ls.direction);
               smis = pl / ((pl + psd) + 1e-10f);
               if (smis < 1e-5f) sh_tmax = -1.f;
             }
@@ -463,38 +474,47 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
           ray_tmax = -1.f;
         } else {
           ray_tmax = BNX_INF;
-          // Lambertian::scatter (packedBSDF/Lambertian.h:70-85)
-          v3 Nsc = Ng;
-          if (dot(wo, Nsc) < 0.f) Nsc = -Nsc;
-          float sx = random();
-          float sy = random();
-          float phi = TWO_PI * sx;
-          float cosT = sqrtf(sy);
-          float sinT = sqrtf(1.0f - sy);
-          float sp, cp;
-          bnx_num::sincos(phi, sp, cp);
-          v3 local = v3{cp * sinT, sp * sinT, cosT};
-          v3 b1, b2;
-          onb(Nsc, b1, b2);
-          v3 sdir = (b1 * local.x + b2 * local.y) + Nsc * local.z;
-          float spdf = local.z * ONE_OVER_PI;
+          v3 sdir, f_r;
+          float spdf;
+          bool diffuse = true;
+          if (sh.pbr) {
+            diffuse = pbr_scatter(sh.pp, Ng, wo, random, sdir, spdf, f_r);
+          } else {
+            // Lambertian::scatter (packedBSDF/Lambertian.h:70-85)
+            v3 Nsc = Ng;
+            if (dot(wo, Nsc) < 0.f) Nsc = -Nsc;
+            float sx = random();
+            float sy = random();
+            float phi = TWO_PI * sx;
+            float cosT = sqrtf(sy);
+            float sinT = sqrtf(1.0f - sy);
+            float sp, cp;
+            bnx_num::sincos(phi, sp, cp);
+            v3 local = v3{cp * sinT, sp * sinT, cosT};
+            v3 b1, b2;
+            onb(Nsc, b1, b2);
+            sdir = (b1 * local.x + b2 * local.y) + Nsc * local.z;
+            spdf = local.z * ONE_OVER_PI;
+            f_r = sh.albedo;
+          }
           if (!(spdf > 0.f) || spdf <= 1e-6f) {
             // invalid scatter: the ray goes on unchanged (shadeRays.cu:774-775)
             ps.y = (ps.y & 0x00ffffffu) | ((uint32_t)(gen + 1) << 24);
-          } else if (ndiff >= MAX_DIFFUSE_BOUNCES) {
+          } else if (diffuse && ndiff >= MAX_DIFFUSE_BOUNCES) {
+            // DIFFUSE scatters count against the diffuse bounce limit; GLOSSY ones do not
             ray_tmax = -1.f;
           } else {
-            ndiff = ndiff + 1;
+            if (diffuse) ndiff = ndiff + 1;
             rorg = dgP + ((1.f * off_eps) * ff_off);
             rdir = normalize(sdir);
             float cf = is_vol ? 1.f : fabsf(dot(Ng, normalize(rdir)));
             float den = isinf(spdf) ? 1.f : spdf + 1e-10f;
-            v3 sf = (sh.albedo * cf) / den;
+            v3 sf = (f_r * cf) / den;
             sf = vmin(sf, splat(2.5f));
             tp = quant_half(tp * sf);
             float nmis;
             if (needs_mis && !isinf(spdf)) {
-              float psd = lambert_pdf(Ng, rdir);
+              float psd = sh.pbr ? pbr_pdf(sh.pp, Ng, wo, rdir) : lambert_pdf(Ng, rdir);
               float pl = env_pdf(P.env, rdir);
               nmis = psd / (psd + pl);
             } else {

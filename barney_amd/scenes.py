@@ -105,6 +105,7 @@ class TriMesh:
     indices: np.ndarray           # (nt,3) int32
     colors: np.ndarray | None     # (nt,3) float32 per-prim
     user_id: int = 0
+    material: object = None       # abi.Material; None: matte coloured by the attribute
 
 
 @dataclass
@@ -113,6 +114,7 @@ class SphereSet:
     radii: np.ndarray             # (n,) float32
     colors: np.ndarray | None     # (n,3) float32
     user_id: int = 1
+    material: object = None
 
 
 @dataclass

@@ -77,12 +77,20 @@ typedef struct bnx_hit {
 
 /* ----------------------------------------------------------- materials */
 #define BNX_MATERIAL_MATTE 0        /* AnariMatte -> Lambertian (material/AnariMatte.h:51-76) */
+#define BNX_MATERIAL_PBR   1        /* AnariPBR -> NVisii (material/AnariPBR.h:62-96,
+                                       packedBSDF/NVisii.h)                        */
 
 typedef struct bnx_material {
   int32_t type;                     /* BNX_MATERIAL_*                              */
   int32_t color_from_attribute;     /* 1: color = geometry's per-prim 'color'      */
-  float   color[4];                 /* constant color when not from the attribute  */
-  float   opacity;                  /* AnariMatte 'opacity'                        */
+  float   color[4];                 /* matte 'color' / PBR 'baseColor' when not
+                                       from the attribute                          */
+  float   opacity;                  /* 'opacity'                                   */
+  float   metallic;                 /* PBR 'metallic' (ANARI default 1)            */
+  float   roughness;                /* PBR 'roughness' (default 1), clamped to
+                                       [0.001, 0.9] as AnariPBR does               */
+  float   transmission;             /* PBR 'transmission' (default 0)              */
+  float   ior;                      /* PBR 'ior' (default 1.45)                    */
   float   _pad;
 } bnx_material;
 
@@ -460,12 +468,12 @@ int bnx_server_get_stats(bnx_server server, bnx_server_stats *stats);
 #ifdef __cplusplus
 static_assert(sizeof(bnx_ray) == 32, "bnx_ray layout");
 static_assert(sizeof(bnx_hit) == 32, "bnx_hit layout");
-static_assert(sizeof(bnx_material) == 32, "bnx_material layout");
+static_assert(sizeof(bnx_material) == 48, "bnx_material layout");
 static_assert(sizeof(bnx_camera) == 60, "bnx_camera layout");
 static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
 static_assert(sizeof(bnx_env_map_desc) == 48, "bnx_env_map_desc layout");
-static_assert(sizeof(bnx_triangles_desc) == 72, "bnx_triangles_desc layout");
-static_assert(sizeof(bnx_spheres_desc) == 72, "bnx_spheres_desc layout");
+static_assert(sizeof(bnx_triangles_desc) == 88, "bnx_triangles_desc layout");
+static_assert(sizeof(bnx_spheres_desc) == 88, "bnx_spheres_desc layout");
 static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
 static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");

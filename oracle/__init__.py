@@ -51,6 +51,10 @@ def lib():
         L.orc_atan2_n.argtypes = [abi.fp, abi.fp, C.c_int, abi.fp]
         L.orc_acos_n.argtypes = [abi.fp, C.c_int, abi.fp]
         L.orc_env_frame.argtypes = [abi.fp, abi.fp, abi.fp, abi.fp]
+        L.orc_pbr_eval.argtypes = [abi.fp, abi.fp, C.c_int, abi.fp, abi.fp]
+        L.orc_pbr_pdf.argtypes = [abi.fp, abi.fp, C.c_int, abi.fp]
+        L.orc_pbr_scatter.argtypes = [abi.fp, abi.fp, abi.fp, C.c_uint32, C.c_uint32, C.c_int, abi.fp, abi.fp,
+                                      abi.fp, abi.ip]
         L.orc_scene_add_volume.argtypes = [vp, C.POINTER(abi.StructuredDesc), C.POINTER(abi.TransferFunction),
                                            C.c_int, C.c_int]
         L.orc_volume_majorants.argtypes = [vp, C.c_int, abi.ip, abi.fp]
@@ -94,6 +98,42 @@ def lib():
         L.orc_random2_seq.argtypes = [C.c_uint32, C.c_uint32, C.c_uint32, C.c_int, abi.fp, C.POINTER(C.c_uint64)]
         _lib = L
     return _lib
+
+
+def _mat6(base, metallic, roughness, ior):
+    return np.array([base[0], base[1], base[2], metallic, roughness, ior], np.float32)
+
+
+def pbr_eval(base, metallic, roughness, ng, wo, wi, ior=1.45):
+    """NVisii eval of an AnariPBR material for rows of (Ng, wo, wi)"""
+    m = _mat6(base, metallic, roughness, ior)
+    v = np.ascontiguousarray(np.concatenate([np.atleast_2d(ng), np.atleast_2d(wo), np.atleast_2d(wi)], 1),
+                             np.float32)
+    val = np.zeros((len(v), 3), np.float32)
+    pdf = np.zeros(len(v), np.float32)
+    lib().orc_pbr_eval(abi.fptr(m), abi.fptr(v), len(v), abi.fptr(val), abi.fptr(pdf))
+    return val, pdf
+
+
+def pbr_pdf(base, metallic, roughness, ng, wo, wi, ior=1.45):
+    m = _mat6(base, metallic, roughness, ior)
+    v = np.ascontiguousarray(np.concatenate([np.atleast_2d(ng), np.atleast_2d(wo), np.atleast_2d(wi)], 1),
+                             np.float32)
+    pdf = np.zeros(len(v), np.float32)
+    lib().orc_pbr_pdf(abi.fptr(m), abi.fptr(v), len(v), abi.fptr(pdf))
+    return pdf
+
+
+def pbr_scatter(base, metallic, roughness, ng, wo, n, seed=(1, 2), ior=1.45):
+    """n scatter samples: (dirs, pdf, f_r, diffuse flag)"""
+    m = _mat6(base, metallic, roughness, ior)
+    d = np.zeros((n, 3), np.float32)
+    p = np.zeros(n, np.float32)
+    f = np.zeros((n, 3), np.float32)
+    fl = np.zeros(n, np.int32)
+    lib().orc_pbr_scatter(abi.fptr(m), abi.fptr(np.asarray(ng, np.float32)), abi.fptr(np.asarray(wo, np.float32)),
+                          seed[0], seed[1], n, abi.fptr(d), abi.fptr(p), abi.fptr(f), abi.iptr(fl))
+    return d, p, f, fl
 
 
 def default_threads() -> int:
@@ -349,8 +389,9 @@ def surface_scene(sc) -> Scene:
     (ID 0) flattened into a user-geometry group (spheres) and a triangles
     group, in the order of native/ModelSlot.cpp:128-155."""
     s = Scene()
-    sph = [s.add_spheres(x.centers, x.radii, x.colors, user_id=x.user_id) for x in sc.spheres]
-    tri = [s.add_triangles(m.vertices, m.indices, m.colors, user_id=m.user_id) for m in sc.meshes]
+    sph = [s.add_spheres(x.centers, x.radii, x.colors, user_id=x.user_id, material=x.material) for x in sc.spheres]
+    tri = [s.add_triangles(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
+           for m in sc.meshes]
     if sph:
         s.add_instance(s.add_group(1, sph), 0)
     if tri:

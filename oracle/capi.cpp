@@ -26,6 +26,10 @@ static void set_material(Material &m, const bnx_material &d)
   m.color_from_attribute = d.color_from_attribute != 0;
   memcpy(m.color, d.color, sizeof(m.color));
   m.opacity = d.opacity;
+  m.metallic = d.metallic;
+  m.roughness = d.roughness;
+  m.transmission = d.transmission;
+  m.ior = d.ior;
 }
 
 extern "C" {
@@ -427,6 +431,52 @@ void orc_atan2_n(const float *y, const float *x, int n, float *out)
 void orc_acos_n(const float *x, int n, float *out)
 {
   for (int i = 0; i < n; i++) out[i] = bnx_num::acos(x[i]);
+}
+// NVisii of an AnariPBR material with base color, metallic, roughness, ior
+static PbrParams pbr_of(const float *mat4)
+{
+  Material m;
+  m.type = BNX_MATERIAL_PBR;
+  m.metallic = mat4[3];
+  m.roughness = mat4[4];
+  m.ior = mat4[5];
+  float base[4] = {mat4[0], mat4[1], mat4[2], 1.f};
+  return pbr_params(m, base);
+}
+// n (Ng, wo, wi) triples -> value (3) and pdf; mat = base rgb, metallic, roughness, ior
+void orc_pbr_eval(const float *mat, const float *vecs, int n, float *value, float *pdf)
+{
+  const PbrParams p = pbr_of(mat);
+  for (int i = 0; i < n; i++) {
+    const float *v = vecs + 9 * i;
+    vec3 f;
+    pbr_eval(p, load3(v), load3(v + 3), load3(v + 6), f, pdf[i]);
+    value[3 * i] = f.x; value[3 * i + 1] = f.y; value[3 * i + 2] = f.z;
+  }
+}
+void orc_pbr_pdf(const float *mat, const float *vecs, int n, float *pdf)
+{
+  const PbrParams p = pbr_of(mat);
+  for (int i = 0; i < n; i++) {
+    const float *v = vecs + 9 * i;
+    pdf[i] = pbr_pdf(p, load3(v), load3(v + 3), load3(v + 6));
+  }
+}
+// n scatter samples for one (Ng, wo) from a Random2(a, b) stream: dir (3),
+// pdf, f_r (3), lobe flag (1 diffuse)
+void orc_pbr_scatter(const float *mat, const float *ng, const float *wo, uint32_t a, uint32_t b, int n,
+                     float *dirs, float *pdfs, float *frs, int32_t *diffuse)
+{
+  const PbrParams p = pbr_of(mat);
+  Random2 r(a, b);
+  for (int i = 0; i < n; i++) {
+    vec3 d(0.f), f(0.f);
+    float pdf = 0.f;
+    diffuse[i] = pbr_scatter(p, load3(ng), load3(wo), r, d, pdf, f) ? 1 : 0;
+    dirs[3 * i] = d.x; dirs[3 * i + 1] = d.y; dirs[3 * i + 2] = d.z;
+    frs[3 * i] = f.x; frs[3 * i + 1] = f.y; frs[3 * i + 2] = f.z;
+    pdfs[i] = pdf;
+  }
 }
 void orc_env_frame(const float *dir, const float *up, float *to_world, float *to_local)
 {

@@ -174,7 +174,21 @@ struct Material {
   bool color_from_attribute = false;
   float color[4] = {0.8f, 0.8f, 0.8f, 1.f};
   float opacity = 1.f;
+  float metallic = 1.f, roughness = 1.f, transmission = 0.f, ior = 1.45f;   // AnariPBR
 };
+
+// NVisii parameters of an AnariPBR hit, fp16-rounded as the packed BSDF
+// stores them (packedBSDF/NVisii.h:1006-1024)
+struct PbrParams {
+  vec3 base;
+  float metallic = 0.f, roughness = 0.f, ior = 0.f;
+};
+PbrParams pbr_params(const Material &mat, const float base[4]);
+// NVisii::eval / pdf / scatter for geometric normal Ng (not flipped toward
+// wo) and wo = -ray direction; scatter returns true for the diffuse lobe
+void pbr_eval(const PbrParams &p, vec3 Ng, vec3 wo, vec3 wi, vec3 &value, float &pdf);
+float pbr_pdf(const PbrParams &p, vec3 Ng, vec3 wo, vec3 wi);
+bool pbr_scatter(const PbrParams &p, vec3 Ng, vec3 wo, Random2 &rng, vec3 &dir, float &pdf, vec3 &f_r);
 
 struct Geom {
   int type = 0;                     // 0 triangles, 1 spheres

@@ -22,7 +22,8 @@ struct PathRay {
   bool had_hit = false;
   vec3 P, N;                 // N holds the fp16-rounded normal (vec3h)
   vec3 albedo;               // Lambertian::albedo or SimplePhase::albedo
-  int bsdf = 0;              // 1 Lambertian, 2 SimplePhase (volume hit)
+  int bsdf = 0;              // 1 Lambertian, 2 SimplePhase (volume hit), 3 NVisii (AnariPBR)
+  PbrParams pbr;
   bool is_shadow = false, is_specular = false, in_medium = false;
   float miss[4];
   vec3 tp;                   // PathState::throughput, fp16-rounded (vec3h)
@@ -240,6 +241,8 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
       // SimplePhase::eval (packedBSDF/Phase.h:147-152)
       fpdf = ONE_OVER_FOUR_PI;
       fv = ONE_OVER_FOUR_PI * ray.albedo;
+    } else if (ray.bsdf == 3) {
+      pbr_eval(ray.pbr, dg.Ng, dg.wo, ls.direction, fv, fpdf);
     } else {
       lambert_eval(ray.albedo, dg, ls.direction, fv, fpdf);
     }
@@ -263,7 +266,9 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
       shadow.mis = 1.f;
       if (!is_dir && needs_mis) {
         float pl = env_pdf(s.env, ls.direction);     // EnvMapLight::DD::pdf (light/EnvMap.h:135-160)
-        float ps = ray.bsdf == 2 ? ONE_OVER_FOUR_PI : lambert_pdf(dg, ls.direction);
+        float ps = ray.bsdf == 2   ? ONE_OVER_FOUR_PI
+                   : ray.bsdf == 3 ? pbr_pdf(ray.pbr, dg.Ng, dg.wo, ls.direction)
+                                   : lambert_pdf(dg, ls.direction);
         shadow.mis = pl / ((pl + ps) + 1e-10f);
         if (shadow.mis < 1e-5f) shadow.tmax = -1.f;
       }
@@ -285,14 +290,18 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
   }
   vec3 sdir, f_r;
   float spdf;
-  lambert_scatter(ray.albedo, dg, random, sdir, spdf, f_r);
+  bool diffuse = true;
+  if (ray.bsdf == 3) diffuse = pbr_scatter(ray.pbr, dg.Ng, dg.wo, random, sdir, spdf, f_r);
+  else lambert_scatter(ray.albedo, dg, random, sdir, spdf, f_r);
   if (!(spdf > 0.f) || spdf <= 1e-6f) return;
-  // DIFFUSE scatter
-  if (ray.ndiff >= MAX_DIFFUSE_BOUNCES) {
-    ray.tmax = -1.f;
-    return;
+  // a DIFFUSE scatter counts against the diffuse bounce limit; GLOSSY does not
+  if (diffuse) {
+    if (ray.ndiff >= MAX_DIFFUSE_BOUNCES) {
+      ray.tmax = -1.f;
+      return;
+    }
+    ray.ndiff = ray.ndiff + 1;
   }
-  ray.ndiff = ray.ndiff + 1;
   ray.is_specular = false;
   ray.org = dg.P + ((1.f * off_eps) * ff_off);
   ray.dir = normalize(sdir);
@@ -302,7 +311,7 @@ void bounce(const Scene &s, const bnx_renderer &rr, vec3 &frag, PathRay &ray, Pa
   sf = vmin(sf, vec3(2.5f));
   ray.tp = quantize_half(ray.tp * sf);
   if (needs_mis && !std::isinf(spdf)) {
-    float ps = lambert_pdf(dg, ray.dir);
+    float ps = ray.bsdf == 3 ? pbr_pdf(ray.pbr, dg.Ng, dg.wo, ray.dir) : lambert_pdf(dg, ray.dir);
     float pl = env_pdf(s.env, ray.dir);
     ray.mis = ps / (ps + pl);
   } else {
@@ -363,13 +372,17 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
   float base[4];
   if (m.color_from_attribute) memcpy(base, color, sizeof(base));
   else memcpy(base, m.color, sizeof(base));
-  vec3 albedo = (0.85f * vec3(base[0], base[1], base[2])) * ONE_OVER_PI;   // AnariMatte.h:58-63
   r.had_hit = true;
   r.tmax = h.t;
   r.P = P;
   r.N = quantize_half(Nw);
-  r.albedo = albedo;
-  r.bsdf = 1;
+  if (m.type == BNX_MATERIAL_PBR) {
+    r.pbr = pbr_params(m, base);
+    r.bsdf = 3;
+  } else {
+    r.albedo = (0.85f * vec3(base[0], base[1], base[2])) * ONE_OVER_PI;   // AnariMatte.h:58-63
+    r.bsdf = 1;
+  }
   if (ids) {
     ids->depth = h.t;
     ids->prim = h.prim;

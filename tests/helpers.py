@@ -133,12 +133,14 @@ class SceneSpec:
         self.insts = []      # (group, user_id, o2w or None, w2o or None)
         self.lights = []
 
-    def tri(self, vertices, indices, colors=None, user_id=0):
-        self.geoms.append((0, dict(vertices=vertices, indices=indices, colors=colors, user_id=user_id)))
+    def tri(self, vertices, indices, colors=None, user_id=0, material=None):
+        self.geoms.append((0, dict(vertices=vertices, indices=indices, colors=colors, user_id=user_id,
+                                   material=material)))
         return len(self.geoms) - 1
 
-    def sph(self, centers, radii, colors=None, user_id=0):
-        self.geoms.append((1, dict(centers=centers, radii=radii, colors=colors, user_id=user_id)))
+    def sph(self, centers, radii, colors=None, user_id=0, material=None):
+        self.geoms.append((1, dict(centers=centers, radii=radii, colors=colors, user_id=user_id,
+                                   material=material)))
         return len(self.geoms) - 1
 
     def group(self, kind, geoms):
@@ -153,9 +155,11 @@ class SceneSpec:
         ids = []
         for kind, kw in self.geoms:
             if kind == 0:
-                ids.append(s.add_triangles(kw["vertices"], kw["indices"], kw["colors"], user_id=kw["user_id"]))
+                ids.append(s.add_triangles(kw["vertices"], kw["indices"], kw["colors"], user_id=kw["user_id"],
+                                           material=kw["material"]))
             else:
-                ids.append(s.add_spheres(kw["centers"], kw["radii"], kw["colors"], user_id=kw["user_id"]))
+                ids.append(s.add_spheres(kw["centers"], kw["radii"], kw["colors"], user_id=kw["user_id"],
+                                         material=kw["material"]))
         gids = [s.add_group(k, [ids[g] for g in gl]) for k, gl in self.groups]
         for g, uid, o2w, w2o in self.insts:
             s.add_instance(gids[g], uid, o2w, w2o)
@@ -169,10 +173,10 @@ class SceneSpec:
         for kind, kw in self.geoms:
             if kind == 0:
                 geoms.append(bx.Geom.triangles(dev, kw["vertices"], kw["indices"], kw["colors"],
-                                               user_id=kw["user_id"]))
+                                               user_id=kw["user_id"], material=kw["material"]))
             else:
                 geoms.append(bx.Geom.spheres(dev, kw["centers"], kw["radii"], kw["colors"],
-                                             user_id=kw["user_id"]))
+                                             user_id=kw["user_id"], material=kw["material"]))
         groups = [bx.Group.geoms(dev, k, [geoms[g] for g in gl]).build() for k, gl in self.groups]
         xf = None
         if any(o is not None for _, _, o, _ in self.insts):
@@ -186,8 +190,8 @@ class SceneSpec:
 
 def c1_spec(sc):
     spec = SceneSpec()
-    sph = [spec.sph(s.centers, s.radii, s.colors, user_id=s.user_id) for s in sc.spheres]
-    tri = [spec.tri(m.vertices, m.indices, m.colors, user_id=m.user_id) for m in sc.meshes]
+    sph = [spec.sph(s.centers, s.radii, s.colors, user_id=s.user_id, material=s.material) for s in sc.spheres]
+    tri = [spec.tri(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material) for m in sc.meshes]
     if sph:
         spec.inst(spec.group(1, sph), 0)
     if tri:

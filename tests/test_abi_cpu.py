@@ -34,8 +34,8 @@ def test_library_exports_every_declared_symbol():
 
 def test_struct_layouts_match_header():
     from barney_amd import abi
-    assert C.sizeof(abi.TrianglesDesc) == 72
-    assert C.sizeof(abi.SpheresDesc) == 72
+    assert C.sizeof(abi.TrianglesDesc) == 88
+    assert C.sizeof(abi.SpheresDesc) == 88
     assert C.sizeof(abi.Renderer) == 48
     assert C.sizeof(abi.RenderStats) == 128
     assert abi.HIT_DTYPE.itemsize == 32 and abi.NODE_DTYPE.itemsize == 64
