@@ -165,6 +165,7 @@ class SurfaceScene:
     width: int = 512
     height: int = 512
     spp: int = 1
+    max_generations: int = 0       # renderer bound on traced generations (0: the path-depth limit)
 
     @property
     def num_prims(self):
@@ -193,6 +194,98 @@ def config1(width: int = 512, height: int = 512, spp: int = 1, num_spheres: int 
     s.bg_color = (0.0, 0.0, 0.0, 1.0)
     s.camera = dict(position=(0.0, 0.0, 5.0), direction=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=40.0,
                     target=(0.0, 0.0, 0.0))
+    return s
+
+
+# --------------------------------------------------------------------------
+# config 3: fBm terrain + spheres, mixed matte / glossy, HDRI light
+# --------------------------------------------------------------------------
+def value_noise(seed: int, kind: int, freq: int, x: np.ndarray, z: np.ndarray) -> np.ndarray:
+    """lattice value noise on [0, 1]^2 at lattice spacing 1/freq: PCG32
+    values in [-1, 1] at the (freq + 1)^2 lattice points (stream (kind, j * (freq + 1) + i)),
+    blended with the quintic fade 6t^5 - 15t^4 + 10t^3"""
+    lat = (2.0 * uniform_table(seed, kind, (freq + 1) ** 2, 1)[:, 0] - 1.0).astype(np.float32)
+    lat = lat.reshape(freq + 1, freq + 1)
+    fx = x * freq
+    fz = z * freq
+    i = np.minimum(fx.astype(np.int32), freq - 1)
+    j = np.minimum(fz.astype(np.int32), freq - 1)
+    tx = (fx - i).astype(np.float32)
+    tz = (fz - j).astype(np.float32)
+    tx = tx * tx * tx * (tx * (tx * 6 - 15) + 10)
+    tz = tz * tz * tz * (tz * (tz * 6 - 15) + 10)
+    a = lat[j, i] + (lat[j, i + 1] - lat[j, i]) * tx
+    b = lat[j + 1, i] + (lat[j + 1, i + 1] - lat[j + 1, i]) * tx
+    return a + (b - a) * tz
+
+
+def fbm_height(seed: int, x: np.ndarray, z: np.ndarray, octaves: int = 8, base_freq: int = 4,
+               amplitude: float = 0.12) -> np.ndarray:
+    """8-octave fBm: sum_o amplitude * 0.5^o * noise_o(x, z) at frequency
+    base_freq * 2^o (lacunarity 2, gain 0.5), octave o from stream kind 20 + o"""
+    h = np.zeros(np.broadcast(x, z).shape, np.float32)
+    for o in range(octaves):
+        h += np.float32(amplitude * 0.5 ** o) * value_noise(seed, 20 + o, base_freq << o, x, z)
+    return h
+
+
+def config3(n: int = 4096, num_spheres: int = 4_000_000, width: int = 1920, height: int = 1080, spp: int = 64,
+            env_size=(1024, 512)) -> SurfaceScene:
+    """Large triangle scene: an n x n heightfield over [-1, 1]^2 (x, z)
+    displaced in y by seeded 8-octave fBm (2 (n - 1)^2 triangles; 33.5M at
+    n = 4096), num_spheres spheres resting on the surface (x, z U[-1, 1],
+    radius U(0.002, 0.01)), per-primitive colour U[0.2, 0.8]^3 (terrain:
+    earth tones), each primitive matte or glossy (AnariPBR: terrain metallic
+    0, roughness 0.3; spheres metallic 0.5, roughness 0.2) with probability
+    1/2, lit by the seeded lobe HDRI map only; 1920x1080, fovy 45, primary
+    rays and 4 bounce generations.  The triangles and spheres of each
+    material form their own geometry."""
+    seed = SEEDS["c3"]
+    g = np.linspace(0.0, 1.0, n, dtype=np.float32)
+    X, Z = np.meshgrid(g, g)
+    Y = fbm_height(seed, X, Z)
+    verts = np.stack([2.0 * X - 1.0, Y, 2.0 * Z - 1.0], -1).reshape(-1, 3).astype(np.float32)
+    del X, Z
+    k = np.arange(n - 1, dtype=np.int32)
+    i0 = (k[None, :] + n * k[:, None]).ravel()              # (row j, column i) -> j * n + i
+    quads = np.stack([i0, i0 + 1, i0 + n, i0 + n + 1], 1)
+    del i0
+    tris = np.empty((2 * len(quads), 3), np.int32)
+    tris[0::2] = quads[:, [0, 2, 1]]
+    tris[1::2] = quads[:, [1, 2, 3]]
+    del quads
+    nt = len(tris)
+    u = uniform_table(seed, 30, nt, 2).astype(np.float32)
+    earth = np.array([[0.35, 0.45, 0.2], [0.5, 0.42, 0.3], [0.6, 0.58, 0.5]], np.float32)
+    t = u[:, 0]
+    tri_col = np.where((t < 0.5)[:, None], earth[0] + (earth[1] - earth[0]) * (2 * t)[:, None],
+                       earth[1] + (earth[2] - earth[1]) * (2 * t - 1)[:, None]).astype(np.float32)
+    tri_col = np.clip(tri_col * (0.8 + 0.4 * uniform_table(seed, 31, nt, 1)[:, :1]), 0.2, 0.8).astype(np.float32)
+    glossy_t = u[:, 1] < 0.5
+    del u
+    us = uniform_table(seed, 32, num_spheres, 7)
+    sx = (2.0 * us[:, 0] - 1.0).astype(np.float32)
+    sz = (2.0 * us[:, 1] - 1.0).astype(np.float32)
+    rad = (0.002 + 0.008 * us[:, 2]).astype(np.float32)
+    sy = fbm_height(seed, (sx + 1) * 0.5, (sz + 1) * 0.5) + rad
+    centers = np.stack([sx, sy, sz], 1).astype(np.float32)
+    sph_col = (0.2 + 0.6 * us[:, 3:6]).astype(np.float32)
+    glossy_s = us[:, 6] < 0.5
+    del us
+    from barney_amd import abi
+    pbr_t = abi.pbr(metallic=0.0, roughness=0.3)
+    pbr_s = abi.pbr(metallic=0.5, roughness=0.2)
+    s = SurfaceScene(width=width, height=height, spp=spp)
+    s.meshes.append(TriMesh(verts, tris[~glossy_t], tri_col[~glossy_t], user_id=0))
+    s.meshes.append(TriMesh(verts, tris[glossy_t], tri_col[glossy_t], user_id=1, material=pbr_t))
+    s.spheres.append(SphereSet(centers[~glossy_s], rad[~glossy_s], sph_col[~glossy_s], user_id=2))
+    s.spheres.append(SphereSet(centers[glossy_s], rad[glossy_s], sph_col[glossy_s], user_id=3, material=pbr_s))
+    s.env = lobe_env_map(*env_size)
+    s.max_generations = 5          # primary rays + 4 bounce generations
+    s.ambient = 1.0
+    s.bg_color = (0.0, 0.0, 0.0, 1.0)
+    s.camera = dict(position=(0.0, 0.6, 0.8), direction=(0.0, -0.65, -1.3), up=(0.0, 1.0, 0.0), fovy=45.0,
+                    target=(0.0, -0.05, -0.5))
     return s
 
 

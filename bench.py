@@ -1,6 +1,6 @@
 """Benchmark: Mrays/s of barney's single-GPU ray-tracing hot path on MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c2|c1]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c1|c2|c3|c4]
 
 c5 (default): the serving stream.  Requests over resident worlds (config 1's
 surface scene and config 2's structured volume) arrive with seeded Poisson
@@ -14,8 +14,11 @@ frame readable) of both forms under the same Poisson stream, and the plain
 single-request latency.  Served and plain frames are identical (checked on
 a sample at the end).
 
-c2 / c1: one request of config 2 (1024^2, 16 spp structured volume) or
-config 1 (512^2, 1 spp surfaces) per step, from a cleared frame.
+c1 / c2 / c3 / c4: one request per step, from a cleared frame, of config 1
+(512^2, 1 spp surfaces), config 2 (1024^2, 16 spp structured volume),
+config 3 (1920x1080, 64 spp, 33.5M-triangle terrain + 4M spheres, mixed
+matte / glossy, HDRI light) or config 4 (1024^2, 32 spp, 20.5M-tet volume
++ isosurface).
 
 Every line carries the dominant kernel's roofline figures and the CPU
 oracle's throughput on a bounded sample of the same workload.
@@ -35,6 +38,7 @@ sys.path.insert(0, ROOT)
 METRIC = "Mrays/s + p50/p99 frame latency at fixed RMSE, served batched vs plain GPU"
 HBM_PEAK_GBS = 8000.0
 CPU_TET_RES = 64            # config 4 CPU sample: the same scene and camera at 64x64
+CPU_BIG_RES = (160, 90)     # config 3 CPU sample: the same scene and camera at 160x90, 1 spp
 
 
 def algorithmic_bytes(rays, nodes, tris, spheres):
@@ -122,6 +126,21 @@ def workload_c4(args):
     return sc, desc
 
 
+def workload_c3(args):
+    from barney_amd import scenes
+    n = args.terrain_n
+    sc = scenes.config3(n=n, num_spheres=args.spheres, width=args.width or 1920, height=args.height or 1080,
+                        spp=args.spp or 64)
+    ntri = sum(len(m.indices) for m in sc.meshes)
+    nsph = sum(len(x.radii) for x in sc.spheres)
+    desc = {"workload": f"config 3: large triangle scene, {n}x{n} fBm heightfield ({ntri:,} triangles) + "
+                        f"{nsph:,} spheres, 50/50 matte / glossy (AnariPBR) per primitive, 1024x512 4-lobe HDRI "
+                        f"light, {sc.width}x{sc.height}, {sc.spp} spp, primary + 4 bounce generations",
+            "width": sc.width, "height": sc.height, "spp": sc.spp, "triangles": int(ntri), "spheres": int(nsph),
+            "form": "plain single request"}
+    return sc, desc
+
+
 def workload_c1(args):
     from barney_amd import scenes
     sc = scenes.config1(width=args.width or 512, height=args.height or 512, spp=1)
@@ -148,7 +167,7 @@ def run_gpu(sc, steps, warmup):
     dev.sync()
     log(f"world built in {time.perf_counter() - t0:.2f} s")
     cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
-    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient)
+    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=getattr(sc, "max_generations", 0))
     fr = bx.Frame(dev, sc.width, sc.height)
     for _ in range(warmup):
         fr.reset()
@@ -188,12 +207,18 @@ def run_cpu(sc, budget_s):
         # the same scene and camera at a reduced resolution: a bounded sample
         sc = copy.copy(sc)
         sc.width, sc.height = CPU_TET_RES, CPU_TET_RES
+    elif isinstance(sc, scenes.SurfaceScene) and sc.num_prims > 1_000_000:
+        t0 = time.perf_counter()
+        o = oracle.surface_scene(sc)
+        log(f"oracle surface scene built in {time.perf_counter() - t0:.1f} s")
+        sc = copy.copy(sc)
+        sc.width, sc.height, sc.spp = CPU_BIG_RES[0], CPU_BIG_RES[1], 1
     else:
         o = oracle.volume_scene(sc) if vol else oracle.surface_scene(sc)
     spp = 1 if vol else sc.spp
     cam = oracle.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"],
                                     sc.camera["fovy"])
-    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient)
+    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=getattr(sc, "max_generations", 0))
     fr = oracle.Frame(sc.width, sc.height)
     rays, frames, vsamp = 0, 0, 0
     t0 = time.perf_counter()
@@ -432,7 +457,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None)
     ap.add_argument("--warmup", type=int, default=None)
-    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c4", "c5"])
+    ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
+    ap.add_argument("--terrain-n", type=int, default=4096, help="c3: heightfield size (4096: 33.5M triangles)")
+    ap.add_argument("--spheres", type=int, default=4_000_000, help="c3: sphere count")
     ap.add_argument("--umesh-n", type=int, default=160, help="c4: hex lattice size (160: 20.48M tets)")
     ap.add_argument("--spp", type=int, default=0)
     ap.add_argument("--width", type=int, default=0)
@@ -459,10 +486,11 @@ def main():
         mrays, elapsed, tot, desc, extra = run_c5(args, steps, warmup)
         lat_line = extra.pop("frame_latency_ms")
     else:
-        heavy = args.workload == "c4"
+        heavy = args.workload in ("c3", "c4")
         steps = args.steps if args.steps is not None else (3 if heavy else 20)
         warmup = args.warmup if args.warmup is not None else (1 if heavy else 3)
-        sc, desc = {"c1": workload_c1, "c2": workload_c2, "c4": workload_c4}[args.workload](args)
+        sc, desc = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3,
+                    "c4": workload_c4}[args.workload](args)
         elapsed, lat, tot = run_gpu(sc, steps, warmup)
         mrays = tot["rays"] / elapsed / 1e6
         lat_line = {"p50": pct(lat, 50), "p99": pct(lat, 99)}
