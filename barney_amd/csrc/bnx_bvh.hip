@@ -554,7 +554,156 @@ __global__ void k_root_leaf(Node *nodes, const TB *tb, int n)
 
 inline int nblocks(long n, int bs) { return (int)((n + bs - 1) / bs); }
 
+// ------------------------------------------------------- 4-wide collapse
+// One thread per 4-wide node of the current level: start from the two
+// children of binary node b, then repeatedly replace the inner child of
+// largest half area by its two children, in place, until there are four
+// children or only leaves.  Inner children become nodes of the next level (index from
+// counter[0], work item from counter[1]).  need: stack entries pushed on
+// the way down to b; counter[2] collects the largest need at a leaf.
+struct Cand {
+  float lo[3], hi[3];
+  int ref, cnt;
+};
+
+__device__ Cand cand_of(const Node &nd, int k)
+{
+  Cand c;
+  if (k == 0) {
+    c.lo[0] = nd.lo0x; c.lo[1] = nd.lo0y; c.lo[2] = nd.lo0z;
+    c.hi[0] = nd.hi0x; c.hi[1] = nd.hi0y; c.hi[2] = nd.hi0z;
+    c.ref = nd.ref0; c.cnt = nd.cnt0;
+  } else {
+    c.lo[0] = nd.lo1x; c.lo[1] = nd.lo1y; c.lo[2] = nd.lo1z;
+    c.hi[0] = nd.hi1x; c.hi[1] = nd.hi1y; c.hi[2] = nd.hi1z;
+    c.ref = nd.ref1; c.cnt = nd.cnt1;
+  }
+  return c;
+}
+
+__device__ float cand_area(const Cand &c)
+{
+  float dx = c.hi[0] - c.lo[0], dy = c.hi[1] - c.lo[1], dz = c.hi[2] - c.lo[2];
+  return (dx * dy + dy * dz) + dz * dx;
+}
+
+__global__ void k_collapse4(const Node *bin, const int4 *work, int nwork, Node4 *out, int4 *next, int *counter,
+                            int cap4)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= nwork) return;
+  const int b = work[i].x, j = work[i].y, need = work[i].z;
+  Cand c[4];
+  int m = 0;
+  const Node root = bin[b];
+  for (int k = 0; k < 2; k++) {
+    Cand x = cand_of(root, k);
+    if (x.cnt >= 0) c[m++] = x;
+  }
+  for (int round = 0; round < 3 && m < 4; round++) {
+    int best = -1;
+    float area = -1.f;
+    for (int k = 0; k < m; k++)
+      if (c[k].cnt == 0 && cand_area(c[k]) > area) {
+        area = cand_area(c[k]);
+        best = k;
+      }
+    if (best < 0) break;
+    const Node nd = bin[c[best].ref];
+    Cand a = cand_of(nd, 0), d = cand_of(nd, 1);
+    if (a.cnt >= 0 && d.cnt >= 0) {
+      // keep the binary tree's left-to-right order: a depth-first walk of
+      // the 4-wide node in slot order meets the leaves in the same order
+      for (int k = m; k > best + 1; k--) c[k] = c[k - 1];
+      c[best] = a;
+      c[best + 1] = d;
+      m++;
+    } else if (a.cnt >= 0 || d.cnt >= 0) {
+      c[best] = a.cnt >= 0 ? a : d;
+    } else {
+      for (int k = best; k + 1 < m; k++) c[k] = c[k + 1];
+      m--;
+    }
+  }
+  float lo[3][4], hi[3][4];
+  uint32_t code[4];
+  const int child_need = need + (m > 0 ? m - 1 : 0);
+  for (int k = 0; k < 4; k++) {
+    if (k < m) {
+      for (int a = 0; a < 3; a++) {
+        lo[a][k] = c[k].lo[a];
+        hi[a][k] = c[k].hi[a];
+      }
+      if (c[k].cnt > 0) {
+        code[k] = enc_child(c[k].ref, c[k].cnt);
+        atomicMax(counter + 2, child_need);
+      } else {
+        const int idx = atomicAdd(counter, 1);
+        const int w = atomicAdd(counter + 1, 1);
+        if (idx < cap4) next[w] = make_int4(c[k].ref, idx, child_need, 0);
+        code[k] = (uint32_t)idx;
+      }
+    } else {
+      for (int a = 0; a < 3; a++) {
+        lo[a][k] = INFINITY;
+        hi[a][k] = -INFINITY;
+      }
+      code[k] = EMPTY4;
+    }
+  }
+  if (j >= cap4) return;
+  Node4 n4;
+  n4.lox = make_float4(lo[0][0], lo[0][1], lo[0][2], lo[0][3]);
+  n4.hix = make_float4(hi[0][0], hi[0][1], hi[0][2], hi[0][3]);
+  n4.loy = make_float4(lo[1][0], lo[1][1], lo[1][2], lo[1][3]);
+  n4.hiy = make_float4(hi[1][0], hi[1][1], hi[1][2], hi[1][3]);
+  n4.loz = make_float4(lo[2][0], lo[2][1], lo[2][2], lo[2][3]);
+  n4.hiz = make_float4(hi[2][0], hi[2][1], hi[2][2], hi[2][3]);
+  n4.child = make_uint4(code[0], code[1], code[2], code[3]);
+  n4.pad = make_uint4(0, 0, 0, 0);
+  out[j] = n4;
+}
+
 } // namespace
+
+int collapse_bvh4(Device *dev, Bvh &bvh)
+{
+  hipStream_t st = dev->stream;
+  // every 4-wide node but the root stands for a distinct inner binary node
+  const int cap4 = bvh.num_nodes > 0 ? bvh.num_nodes : 1;
+  BNX_HIP(bvh.nodes4.ensure(sizeof(Node4) * (size_t)cap4));
+  DevMem work[2], ctr;
+  BNX_HIP(work[0].ensure(sizeof(int4) * (size_t)cap4));
+  BNX_HIP(work[1].ensure(sizeof(int4) * (size_t)cap4));
+  BNX_HIP(ctr.ensure(sizeof(int) * 4));
+  int h[4] = {1, 0, 0, 0};
+  BNX_HIP(hipMemcpyAsync(ctr.ptr, h, sizeof(h), hipMemcpyHostToDevice, st));
+  const int4 root = make_int4(0, 0, 0, 0);
+  BNX_HIP(hipMemcpyAsync(work[0].ptr, &root, sizeof(root), hipMemcpyHostToDevice, st));
+  int nwork = 1, cur = 0;
+  for (int level = 0; nwork > 0; level++) {
+    if (level > 64) {
+      set_error("bvh collapse: too many levels");
+      return BNX_ERR_STATE;
+    }
+    BNX_HIP(hipMemsetAsync((int *)ctr.ptr + 1, 0, sizeof(int), st));
+    k_collapse4<<<nblocks(nwork, 128), 128, 0, st>>>(bvh.nodes.as<Node>(), work[cur].as<int4>(), nwork,
+                                                     bvh.nodes4.as<Node4>(), work[cur ^ 1].as<int4>(),
+                                                     ctr.as<int>(), cap4);
+    BNX_HIP(hipMemcpyAsync(h, ctr.ptr, sizeof(h), hipMemcpyDeviceToHost, st));
+    BNX_HIP(hipStreamSynchronize(st));
+    BNX_HIP(hipGetLastError());
+    if (h[0] > cap4) {
+      set_error("bvh collapse: node overflow");
+      return BNX_ERR_STATE;
+    }
+    nwork = h[1];
+    cur ^= 1;
+  }
+  bvh.num_nodes4 = h[0];
+  bvh.stack_need4 = h[2];
+  return BNX_OK;
+}
 
 int build_bvh(Device *dev, const float4 *d_lo, const float4 *d_hi, int n, int max_leaf, Bvh &out)
 {

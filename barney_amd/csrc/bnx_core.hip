@@ -121,6 +121,7 @@ int build_blas(Group *g)
   if (n) k_prim_boxes<<<nblk(n, 256), 256, 0, dev->stream>>>(d_gb.as<GeomBuild>(), (int)gb.size(), n, lo.as<float4>(),
                                                              hi.as<float4>(), pgeom.as<int>());
   if ((rc = build_bvh(dev, lo.as<float4>(), hi.as<float4>(), n, 4, g->bvh)) != BNX_OK) return rc;
+  if ((rc = collapse_bvh4(dev, g->bvh)) != BNX_OK) return rc;
   size_t per = g->kind == GROUP_TRIANGLES ? 3 : 1;
   BNX_HIP(g->prims.ensure(sizeof(float4) * per * (size_t)(n ? n : 1)));
   BNX_HIP(g->refs.ensure(sizeof(int2) * (size_t)(n ? n : 1)));
@@ -217,7 +218,7 @@ int build_tlas(Group *g)
         BNX_SET(gd.spheres, gm->spheres.as<float4>());
         geoms.push_back(gd);
       }
-      g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.max_depth);
+      g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.stack_need4);
     }
     const float *x = &g->xfms[12 * i];
     InstDev &I = insts[i];
@@ -251,7 +252,7 @@ int build_tlas(Group *g)
   std::vector<BlasDev> blas(blas_groups.size());
   for (size_t k = 0; k < blas_groups.size(); k++) {
     Group *bg = blas_groups[k];
-    BNX_SET(blas[k].nodes, bg->bvh.nodes.as<Node>());
+    BNX_SET(blas[k].nodes, bg->bvh.nodes4.as<Node4>());
     BNX_SET(blas[k].prims, bg->prims.as<float4>());
     BNX_SET(blas[k].refs, bg->refs.as<int2>());
     blas[k].type = bg->kind == GROUP_TRIANGLES ? BLAS_TRIANGLES : BLAS_SPHERES;
@@ -262,15 +263,17 @@ int build_tlas(Group *g)
   if ((rc = upload(dev, d_lo, lo.data(), n)) != BNX_OK) return rc;
   if ((rc = upload(dev, d_hi, hi.data(), n)) != BNX_OK) return rc;
   if ((rc = build_bvh(dev, d_lo.as<float4>(), d_hi.as<float4>(), n, 1, g->bvh)) != BNX_OK) return rc;
+  if ((rc = collapse_bvh4(dev, g->bvh)) != BNX_OK) return rc;
   if ((rc = upload(dev, g->d_insts, insts.data(), insts.size())) != BNX_OK) return rc;
   if ((rc = upload(dev, g->d_blas, blas.data(), blas.size())) != BNX_OK) return rc;
   if ((rc = upload(dev, g->d_geoms, geoms.data(), geoms.size())) != BNX_OK) return rc;
   BNX_HIP(hipStreamSynchronize(dev->stream));
-  if (g->bvh.max_depth + 1 + g->max_blas_depth + 1 > trace_stack_capacity()) {
+  // a walk pushes at most the TLAS path's entries, the LEAVE marker and a BLAS path's entries
+  if (g->bvh.stack_need4 + 1 + g->max_blas_depth + 1 > trace_stack_capacity()) {
     set_error("bnx_group_build: BVH deeper than the traversal stack");
     return BNX_ERR_UNSUPPORTED;
   }
-  BNX_SET(g->scene.tlas_nodes, g->bvh.nodes.as<Node>());
+  BNX_SET(g->scene.tlas_nodes, g->bvh.nodes4.as<Node4>());
   BNX_SET(g->scene.tlas_order, g->bvh.order.as<int>());
   BNX_SET(g->scene.insts, g->d_insts.as<InstDev>());
   BNX_SET(g->scene.blas, g->d_blas.as<BlasDev>());

@@ -137,11 +137,22 @@ BNX_D uint32_t enc_child(int ref, int cnt)
   return cnt > 0 ? (LEAF_BIT | ((uint32_t)ref << 3) | (uint32_t)cnt) : (uint32_t)ref;
 }
 
+// 128-byte 4-wide node, collapsed from the binary BVH for traversal: child
+// k's box is component k of the six axis vectors; child codes as
+// enc_child, EMPTY4 for an unused slot
+struct __align__(128) Node4 {
+  float4 lox, hix, loy, hiy, loz, hiz;
+  uint4 child;
+  uint4 pad;
+};
+static_assert(sizeof(Node4) == 128, "node4");
+constexpr uint32_t EMPTY4 = 0x7ffffffeu;
+
 // ----------------------------------------------------------- scene data
 enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1 };
 
 struct BlasDev {
-  const Node BNX_G *nodes;
+  const Node4 BNX_G *nodes;
   const float4 BNX_G *prims;    // triangles: 3 per slot (v0|prim, e1|geom, e2); spheres: (c, r)
   const int2 BNX_G *refs;  // spheres: (geom, prim) per slot
   int type;
@@ -173,7 +184,7 @@ struct GeomDev {
 };
 
 struct SceneDev {
-  const Node BNX_G *tlas_nodes;
+  const Node4 BNX_G *tlas_nodes;
   const int BNX_G *tlas_order;
   const InstDev BNX_G *insts;
   const BlasDev BNX_G *blas;

@@ -103,6 +103,9 @@ struct Geom {
 struct Bvh {
   DevMem nodes;         // Node[num_nodes] (capacity max(1,n))
   DevMem order;         // int[num_prims]
+  DevMem nodes4;        // Node4[num_nodes4]: the 4-wide traversal form (collapse_bvh4)
+  int num_nodes4 = 0;
+  int stack_need4 = 0;  // most traversal-stack entries a root-to-leaf walk can push
   int num_nodes = 0;
   int num_prims = 0;
   int max_depth = 0;
@@ -126,7 +129,7 @@ struct Group {
   std::vector<float> xfms;   // 12 floats per instance, owl affine3f layout
   DevMem d_insts, d_blas, d_geoms;
   SceneDev scene{};
-  int max_blas_depth = 0;
+  int max_blas_depth = 0;    // largest traversal-stack need of its BLASes (4-wide walk)
 };
 
 struct Volume;
@@ -164,6 +167,8 @@ struct Frame {
 
 // builder entry (bnx_bvh.hip): boxes in d_lo/d_hi (float4, n); returns host info
 int build_bvh(Device *dev, const float4 *d_lo, const float4 *d_hi, int n, int max_leaf, Bvh &out);
+// collapse the built binary BVH into 4-wide nodes (out.nodes4)
+int collapse_bvh4(Device *dev, Bvh &bvh);
 
 // traversal entry (bnx_trace.hip)
 struct TraceQueue {

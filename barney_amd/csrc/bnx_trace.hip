@@ -7,8 +7,9 @@
 // Design (wave64):
 //   * one ray per thread; blocks past the queue's device-side count exit at
 //     once, so a launch can be sized for the queue's capacity;
-//   * nearest-child-first binary BVH traversal over 64-byte nodes (both
-//     child boxes in one fetch = four 16-byte loads);
+//   * nearest-child-first traversal of a 4-wide BVH (collapsed from the
+//     binary build) over 128-byte nodes: the four child boxes in one fetch
+//     (seven 16-byte loads), entry distances ordered by a sorting network;
 //   * short stack: a 16-entry ring per lane in LDS laid out [entry][lane]
 //     (bank-conflict free), spilling its oldest entries to a per-lane
 //     private array when deeper;
@@ -149,6 +150,26 @@ struct Counts {
   uint32_t nodes, tris, spheres;
 };
 
+// entry distance of child k of a 4-wide node, +inf when the ray misses it
+BNX_D float child_entry(v3 o, v3 inv, float lx, float hx, float ly, float hy, float lz, float hz, uint32_t code,
+                        float tmin, float tmax)
+{
+  float tn;
+  const bool h = code != EMPTY4 && box_test(o, inv, lx, ly, lz, hx, hy, hz, tmin, tmax, tn);
+  return h ? tn : __builtin_huge_valf();
+}
+
+BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
+{
+  const bool s = tb < ta;
+  const float t = s ? tb : ta;
+  const uint32_t c = s ? cb : ca;
+  tb = s ? ta : tb;
+  cb = s ? ca : cb;
+  ta = t;
+  ca = c;
+}
+
 BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float tmax, bool any_hit, Stack &st,
                       Counts &cnt)
 {
@@ -156,7 +177,7 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
   const v3 winv = v3{1.f / wdir.x, 1.f / wdir.y, 1.f / wdir.z};
   HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
   v3 o = worg, d = wdir, inv = winv;
-  const Node *nodes = S.tlas_nodes;
+  const Node4 *nodes = S.tlas_nodes;
   BlasDev blas{};
   bool in_blas = false;
   int inst = -1;
@@ -167,24 +188,31 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
   // loop trips than the scene has nodes and leaves; the cap only bounds the
   // loop if the structure were corrupted
   for (uint32_t trip = 0; trip < (1u << 28); trip++) {
-    if (cur < LEAVE_BLAS) {
-      // ------------------------------------------------------- inner node
+    if (cur < EMPTY4) {
+      // ------------------------------------------------ 4-wide inner node
       const float4 *np = (const float4 *)(nodes + cur);
-      float4 a = np[0], b = np[1], c = np[2];
-      int4 r = ((const int4 *)np)[3];
+      const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+      const uint4 ch = ((const uint4 *)np)[6];
       nvis++;
-      float tn0 = 0.f, tn1 = 0.f;
-      bool h0 = r.z >= 0 && box_test(o, inv, a.x, a.y, a.z, a.w, b.x, b.y, tmin, best.t, tn0);
-      bool h1 = r.w >= 0 && box_test(o, inv, b.z, b.w, c.x, c.y, c.z, c.w, tmin, best.t, tn1);
-      if (h0 && h1) {
-        bool n1 = tn1 < tn0;
-        uint32_t c0 = enc_child(r.x, r.z), c1 = enc_child(r.y, r.w);
-        st.push(n1 ? c0 : c1);
-        cur = n1 ? c1 : c0;
+      float t0 = child_entry(o, inv, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, tmin, best.t);
+      float t1 = child_entry(o, inv, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, tmin, best.t);
+      float t2 = child_entry(o, inv, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, tmin, best.t);
+      float t3 = child_entry(o, inv, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, tmin, best.t);
+      uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+      // nearest first: a 4-element sorting network on the entry distances
+      cswap(t0, c0, t1, c1);
+      cswap(t2, c2, t3, c3);
+      cswap(t0, c0, t2, c2);
+      cswap(t1, c1, t3, c3);
+      cswap(t1, c1, t2, c2);
+      const float INF = __builtin_huge_valf();
+      if (t0 < INF) {
+        if (t3 < INF) st.push(c3);
+        if (t2 < INF) st.push(c2);
+        if (t1 < INF) st.push(c1);
+        cur = c0;
         continue;
       }
-      if (h0) { cur = enc_child(r.x, r.z); continue; }
-      if (h1) { cur = enc_child(r.y, r.w); continue; }
     } else if (cur == LEAVE_BLAS) {
       in_blas = false;
       o = worg; d = wdir; inv = winv;

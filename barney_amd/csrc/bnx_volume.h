@@ -127,6 +127,9 @@ BNX_D float lerp_l(float f, float a, float b) { return (1.f - f) * a + f * b; }
 struct MarchCtx {
   uint32_t *ring;             // &ring[0][thread]
   uint32_t samples, cells, unodes, utets;
+  // the tet that held this ray's previous point query, and its field
+  const int4 BNX_G *last_field = nullptr;
+  int last_tet = -1;
 };
 
 // StructuredDataSampler::DD::sample (volume/StructuredData.h:101-112)
@@ -182,6 +185,26 @@ BNX_D bool tet_scalar(const VolumeDev &v, int slot, v3 P, float &out)
   return true;
 }
 
+// tet_scalar of the tet in 'slot' when P lies inside it by a margin: every
+// normalized barycentric coordinate above 1e-4.  In a mesh of
+// non-overlapping tets no other tet's test can then accept P, so the value
+// is the one the depth-first query would return, bit for bit.
+BNX_D bool tet_scalar_interior(const VolumeDev &v, int slot, v3 P, float &out)
+{
+  const int4 t = v.utets[slot];
+  const float4 v0 = v.uverts[t.x], v1 = v.uverts[t.y], v2 = v.uverts[t.z], v3v = v.uverts[t.w];
+  const float t3 = implicit_plane(P, v0, v1, v2);
+  const float t2 = implicit_plane(P, v0, v3v, v1);
+  const float t1 = implicit_plane(P, v0, v2, v3v);
+  const float t0 = implicit_plane(P, v1, v3v, v2);
+  const float sum = ((t0 + t1) + t2) + t3;
+  const float m = 1e-4f * sum;
+  if (!(t0 > m && t1 > m && t2 > m && t3 > m)) return false;
+  const float scale = 1.f / sum;
+  out = scale * (((t0 * v0.w + t1 * v1.w) + t2 * v2.w) + t3 * v3v.w);
+  return true;
+}
+
 BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz, v3 P)
 {
   return lx <= P.x && P.x <= hx && ly <= P.y && P.y <= hy && lz <= P.z && P.z <= hz;
@@ -193,6 +216,13 @@ BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz,
 // P gives the value, NaN if none does
 BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
 {
+  // consecutive queries of a ray are close together: try the tet of the
+  // previous query first
+  if (cx.last_field == v.utets && cx.last_tet >= 0) {
+    float f;
+    cx.utets++;
+    if (tet_scalar_interior(v, cx.last_tet, P, f)) return f;
+  }
   uint32_t deep[PQ_DEEP];
   int sp = 0;
   uint32_t cur = 0;
@@ -221,7 +251,11 @@ BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
       for (int k = 0; k < count; k++) {
         float f;
         cx.utets++;
-        if (tet_scalar(v, first + k, P, f)) return f;
+        if (tet_scalar(v, first + k, P, f)) {
+          cx.last_field = v.utets;
+          cx.last_tet = first + k;
+          return f;
+        }
       }
     }
     if (sp == 0) return __builtin_nanf("");

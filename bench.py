@@ -42,9 +42,10 @@ CPU_BIG_RES = (160, 90)     # config 3 CPU sample: the same scene and camera at 
 
 
 def algorithmic_bytes(rays, nodes, tris, spheres):
-    # per traced ray: 32 B ray in + 16 B hit out; 64 B per node visit,
-    # 48 B per triangle record, 16 B per sphere record
-    return 48.0 * rays + 64.0 * nodes + 48.0 * tris + 16.0 * spheres
+    # per traced ray: 32 B ray in + 16 B hit out; 112 B per 4-wide node visit
+    # (six box vectors + child codes), 48 B per triangle record, 16 B per
+    # sphere record
+    return 48.0 * rays + 112.0 * nodes + 48.0 * tris + 16.0 * spheres
 
 
 def march_bytes(samples, cells, rays):
