@@ -188,6 +188,7 @@ int build_tlas(Group *g)
   std::vector<GeomDev> geoms;
   std::vector<int> geom_base;
   g->max_blas_depth = 0;
+  g->blas_prims = 0;
   for (int i = 0; i < n; i++) {
     Group *bg = g->groups[i];
     if (!bg || !bg->built || bg->kind == GROUP_INSTANCES) {
@@ -219,6 +220,7 @@ int build_tlas(Group *g)
         geoms.push_back(gd);
       }
       g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.stack_need4);
+      g->blas_prims += bg->bvh.num_prims;
     }
     const float *x = &g->xfms[12 * i];
     InstDev &I = insts[i];
@@ -370,6 +372,10 @@ int bnx_device_create(int gpu_id, bnx_device *out)
   hipDeviceProp_t prop;
   BNX_HIP(hipGetDeviceProperties(&prop, gpu_id));
   d->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  // tuning overrides of the trace launch (pool refill threshold, blocks per CU)
+  if (const char *e = getenv("BNX_TRACE_REFILL")) d->trace_refill = std::min(64, std::max(1, atoi(e)));
+  if (const char *e = getenv("BNX_TRACE_REFILL_PRIMS")) d->trace_refill_prims = std::max(0LL, atoll(e));
+  if (const char *e = getenv("BNX_TRACE_BLOCKS_PER_CU")) d->trace_blocks_per_cu = std::min(1 << 20, std::max(1, atoi(e)));
   *out = d;
   return BNX_OK;
   GUARD_END

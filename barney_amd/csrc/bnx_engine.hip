@@ -53,7 +53,7 @@ struct Pinned {
 // queues, tables and staging of one wavefront
 struct Wavefront {
   Queues Q;
-  DevMem slots, chunks, segs, blk256, blk1024, out_base, out_cnt;
+  DevMem slots, chunks, segs, blk256, blk1024, blkT, pool, out_base, out_cnt;
   Pinned h_slots, h_chunks, h_segs, h_out_base, h_out_cnt;
   int slot_cap = 0, chunk_cap = 0, seg_cap = 0;
   size_t blk_cap = 0;
@@ -86,6 +86,7 @@ struct Wavefront {
     if (nsegs > seg_cap) {
       BNX_HIP(segs.ensure(sizeof(Segment) * nsegs));
       BNX_HIP(h_segs.ensure(sizeof(Segment) * nsegs));
+      BNX_HIP(pool.ensure(sizeof(int) * nsegs));
       seg_cap = nsegs;
     }
     // every segment adds at most one partial block
@@ -93,6 +94,7 @@ struct Wavefront {
     if (blocks > blk_cap) {
       BNX_HIP(blk256.ensure(sizeof(int) * blocks));
       BNX_HIP(blk1024.ensure(sizeof(int) * blocks));
+      BNX_HIP(blkT.ensure(sizeof(int) * blocks));
       blk_cap = blocks;
     }
     return BNX_OK;
@@ -102,13 +104,30 @@ struct Wavefront {
 // segments of one generation, written straight into pinned staging
 struct SegList {
   Segment *v;
-  int n = 0, blocks256 = 0, blocks1024 = 0;
-  void add(int slot, int begin, int count)
+  int n = 0, blocks256 = 0, blocks1024 = 0, blocksT = 0;
+  long long rays = 0;
+  // refill: idle lanes per wave that trigger a trace-pool refill.  Whole-
+  // wave refills (64) keep consecutive primary rays together; secondary
+  // rays are incoherent anyway, and refilling lanes as they finish keeps
+  // the waves busy
+  void add(int slot, int begin, int count, int refill)
   {
     if (count <= 0) return;
-    v[n++] = Segment{slot, begin, count, blocks256, blocks1024, {0, 0, 0}};
+    v[n++] = Segment{slot, begin, count, blocks256, blocks1024, 0, refill, 0};
     blocks256 += (count + 255) / 256;
     blocks1024 += (count + 1023) / 1024;
+    rays += count;
+  }
+  // trace blocks: each draws rays from its segment's pool; size the launch
+  // to about `target` blocks (a few per CU), at least one ray per thread
+  void layout_trace(int target)
+  {
+    const long long per = 256LL * std::max(1LL, (rays + 256LL * target - 1) / (256LL * target));
+    blocksT = 0;
+    for (int k = 0; k < n; k++) {
+      v[k].blockT = blocksT;
+      blocksT += (int)((v[k].count + per - 1) / per);
+    }
   }
 };
 
@@ -120,24 +139,26 @@ struct GenTiming {
 // Enqueue one generation over queue `cur`: upload the segments, build the
 // block maps, trace, volume march, shade into queue 1 - cur.  The caller
 // has uploaded the output region bases and cleared the output counters.
-int enqueue_generation(Device *dev, Wavefront &F, int cur, const SegList &sl, bool surfaces, bool volumes,
-                       bool full)
+int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool surfaces, bool volumes, bool full)
 {
   Queues &Q = F.Q;
   RayQueue in = Q.q(cur), out = Q.q(1 - cur);
   unsigned long long *st = Q.stats.as<unsigned long long>();
   const SlotDev *slots = F.slots.as<SlotDev>();
   int rc;
+  sl.layout_trace(dev->trace_blocks_per_cu * dev->num_cus);
   BNX_HIP(hipMemcpyAsync(F.segs.ptr, sl.v, sizeof(Segment) * sl.n, hipMemcpyHostToDevice, dev->stream));
-  if ((rc = launch_segment_blocks(dev, F.segs.as<Segment>(), sl.n, sl.blocks256, sl.blocks1024,
-                                  F.blk256.as<int>(), F.blk1024.as<int>())) != BNX_OK)
-    return rc;
-  const SegView sv{F.segs.as<Segment>(), F.blk256.as<int>(), F.blk1024.as<int>()};
+  const SegView sv{F.segs.as<Segment>(), F.blk256.as<int>(), F.blk1024.as<int>(), F.blkT.as<int>(),
+                   F.pool.as<int>()};
+  if ((rc = launch_segment_blocks(dev, sv, sl.n, sl.blocks256, sl.blocks1024, sl.blocksT)) != BNX_OK) return rc;
   TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
   // with neither surfaces nor volumes the trace kernel still writes the misses
   const bool trace = surfaces || !volumes;
   BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
-  if (trace && (rc = launch_trace_queue(dev, slots, sv, sl.blocks256, tq, st)) != BNX_OK) return rc;
+  bool pooled = false;
+  for (int k = 0; k < sl.n; k++) pooled = pooled || sl.v[k].refill < 64;
+  if (trace && (rc = launch_trace_queue(dev, slots, sv, sl.blocks256, sl.blocksT, pooled, tq, st)) != BNX_OK)
+    return rc;
   BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
   if (volumes) {
     BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
@@ -184,6 +205,15 @@ Wavefront &wavefront_for(Device *dev)
   if (!dev->queues) dev->queues = new Wavefront;
   return *(Wavefront *)dev->queues;
 }
+
+} // namespace
+
+int trace_refill_for(const Device *dev, const World *w)
+{
+  return w->instances->blas_prims >= dev->trace_refill_prims ? dev->trace_refill : 64;
+}
+
+namespace {
 
 // full: the world has isosurfaces or tet volumes (the march kernel's full form)
 int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, SlotDev &s, bool &full)
@@ -257,9 +287,9 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), 1, F.slots.as<SlotDev>(), 0, count, F.Q.q(cur))) !=
         BNX_OK)
       return rc;
-    while (count > 0) {
+    for (bool primary = true; count > 0; primary = false) {
       SegList sl{F.h_segs.as<Segment>()};
-      sl.add(0, 0, count);
+      sl.add(0, 0, count, primary ? 64 : trace_refill_for(dev, w));
       BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, 2 * sizeof(int), dev->stream));
       if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes, full)) != BNX_OK) return rc;
       BNX_HIP(hipMemcpyAsync(hcnt, F.out_cnt.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, dev->stream));
@@ -449,12 +479,12 @@ struct Server {
     RaygenChunk *hc = F.h_chunks.as<RaygenChunk>();
     // this generation's segments: old regions, then the new primaries
     SegList sl{F.h_segs.as<Segment>()};
-    for (Request *r : active) sl.add(r->slot, r->region, r->live_rays);
+    for (Request *r : active) sl.add(r->slot, r->region, r->live_rays, trace_refill_for(dev, r->world));
     for (Request *r : order) {
       if (room <= 0) break;
       const long long take = std::min(room, r->total_paths - r->next_path);
       hc[nch++] = RaygenChunk{(long long)in_end + added, r->next_path, r->slot, 0};
-      sl.add(r->slot, (int)(in_end + added), (int)take);
+      sl.add(r->slot, (int)(in_end + added), (int)take, 64);
       r->added = (int)take;
       r->next_path += take;
       added += take;

@@ -80,6 +80,9 @@ struct Device {
   int gpu = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  int trace_refill = 16;    // secondary rays of large scenes: idle lanes that trigger a trace-pool refill
+  long long trace_refill_prims = 1 << 20;   // "large": at least this many primitives
+  int trace_blocks_per_cu = 8;   // trace launch size: blocks per CU
   BuildScratch build;
   DevMem counters;      // small device counters (trace work queue etc.)
   void *queues = nullptr;   // render queues (bnx_render.h Queues)
@@ -130,6 +133,7 @@ struct Group {
   DevMem d_insts, d_blas, d_geoms;
   SceneDev scene{};
   int max_blas_depth = 0;    // largest traversal-stack need of its BLASes (4-wide walk)
+  long long blas_prims = 0;  // primitives in the distinct BLASes it instances
 };
 
 struct Volume;
@@ -169,6 +173,10 @@ struct Frame {
 int build_bvh(Device *dev, const float4 *d_lo, const float4 *d_hi, int n, int max_leaf, Bvh &out);
 // collapse the built binary BVH into 4-wide nodes (out.nodes4)
 int collapse_bvh4(Device *dev, Bvh &bvh);
+// trace-pool refill threshold for the secondary rays of a world: refilling
+// lanes as their walks end pays off when walks are long (large BVHs); on
+// small scenes whole-wave refills keep rays together and cost less
+int trace_refill_for(const Device *dev, const World *w);
 
 // traversal entry (bnx_trace.hip)
 struct TraceQueue {
@@ -180,8 +188,10 @@ struct TraceQueue {
 int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
 struct SlotDev;
 struct SegView;
-// trace every ray of the generation's segments (blocks256 blocks of 256 threads)
-int launch_trace_queue(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, const TraceQueue &q,
-                       unsigned long long *counters);
+// trace every ray of the generation's segments: one ray per thread
+// (blocks256 blocks of 256 threads) or, pooled, blocksT blocks drawing rays
+// from their segment's pool
+int launch_trace_queue(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, int blocksT,
+                       bool pooled, const TraceQueue &q, unsigned long long *counters);
 
 } // namespace bnx

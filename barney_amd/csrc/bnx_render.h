@@ -68,15 +68,18 @@ EnvDev env_dev(const World *w);
 // segment (and through it its request's scene, volumes, lights, camera and
 // frame) with scalar loads, and the request's parameters sit in scalar
 // registers for the whole block.  Segment k covers blocks
-// [block256, block256 + ceil(count / 256)) of a 256-thread launch and
-// [block1024, ...) of a 1024-thread launch.
+// [block256, block256 + ceil(count / 256)) of a 256-thread launch,
+// [block1024, ...) of a 1024-thread launch, and [blockT, ...) of the trace
+// launch, whose blocks draw the segment's rays from a shared pool.
 struct Segment {
   int slot;
   int begin;                   // first queue index
   int count;                   // rays
   int block256;
   int block1024;
-  int pad[3];
+  int blockT;
+  int refill;                  // trace pool: idle lanes per wave that trigger a refill
+  int pad;
 };
 static_assert(sizeof(Segment) == 32, "segment");
 
@@ -86,6 +89,8 @@ struct SegView {
   const Segment *segs;
   const int *blk256;           // segment of each 256-thread block
   const int *blk1024;          // segment of each 1024-thread block
+  const int *blkT;             // segment of each trace block
+  int *pool;                   // per segment: next ray of the trace pool
 };
 
 // shade output of one generation: request slot s appends to its own region
@@ -159,10 +164,9 @@ int frame_read(Frame *f, int channel, void *host);
 // primary rays for total queue entries starting at queue index q_begin
 int launch_raygen(Device *dev, const RaygenChunk *d_chunks, int num_chunks, const SlotDev *d_slots,
                   long long q_begin, long long total, const RayQueue &q);
-// fill the block -> segment maps of nsegs segments (blocks256 / blocks1024
-// blocks in all)
-int launch_segment_blocks(Device *dev, const Segment *d_segs, int nsegs, int blocks256, int blocks1024,
-                          int *blk256, int *blk1024);
+// fill the block -> segment maps of nsegs segments (blocks256 / blocks1024 /
+// blocksT blocks in all) and reset the segments' trace pools
+int launch_segment_blocks(Device *dev, const SegView &sv, int nsegs, int blocks256, int blocks1024, int blocksT);
 // shade every ray of the segments (blocks1024 blocks of 1024 threads),
 // appending each request's shadow and continuation rays to its region of out
 int launch_shade(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks1024, const RayQueue &in,

@@ -615,10 +615,16 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
 }
 
 // block -> segment maps: one thread per block of either launch width
-__global__ void k_segment_blocks(const Segment *segs, int nsegs, int blocks256, int blocks1024, int *blk256,
-                                 int *blk1024)
+__global__ void k_segment_blocks(SegView sv, int nsegs, int blocks256, int blocks1024, int blocksT)
 {
   const int b = blockIdx.x * blockDim.x + threadIdx.x;
+  const Segment *segs = sv.segs;
+  // the pool of segment b starts after its static part: 256 rays (one chunk
+  // per wave) for each of its trace blocks
+  if (b < nsegs) {
+    const int nb = (b + 1 < nsegs ? segs[b + 1].blockT : blocksT) - segs[b].blockT;
+    sv.pool[b] = min(segs[b].count, 256 * nb);
+  }
   if (b < blocks256) {
     int lo = 0, hi = nsegs - 1;
     while (lo < hi) {
@@ -626,7 +632,7 @@ __global__ void k_segment_blocks(const Segment *segs, int nsegs, int blocks256, 
       if (segs[mid].block256 <= b) lo = mid;
       else hi = mid - 1;
     }
-    blk256[b] = lo;
+    ((int *)sv.blk256)[b] = lo;
   }
   if (b < blocks1024) {
     int lo = 0, hi = nsegs - 1;
@@ -635,7 +641,16 @@ __global__ void k_segment_blocks(const Segment *segs, int nsegs, int blocks256, 
       if (segs[mid].block1024 <= b) lo = mid;
       else hi = mid - 1;
     }
-    blk1024[b] = lo;
+    ((int *)sv.blk1024)[b] = lo;
+  }
+  if (b < blocksT) {
+    int lo = 0, hi = nsegs - 1;
+    while (lo < hi) {
+      const int mid = (lo + hi + 1) >> 1;
+      if (segs[mid].blockT <= b) lo = mid;
+      else hi = mid - 1;
+    }
+    ((int *)sv.blkT)[b] = lo;
   }
 }
 
@@ -747,12 +762,11 @@ int launch_shade(Device *dev, const SlotDev *slots, const SegView &sv, int block
   return BNX_OK;
 }
 
-int launch_segment_blocks(Device *dev, const Segment *segs, int nsegs, int blocks256, int blocks1024, int *blk256,
-                          int *blk1024)
+int launch_segment_blocks(Device *dev, const SegView &sv, int nsegs, int blocks256, int blocks1024, int blocksT)
 {
-  if (nsegs > 0 && blocks256 > 0)
-    k_segment_blocks<<<nblk(blocks256, 256), 256, 0, dev->stream>>>(segs, nsegs, blocks256, blocks1024, blk256,
-                                                                    blk1024);
+  const int n = std::max(std::max(blocks256, blocksT), nsegs);
+  if (nsegs > 0 && n > 0)
+    k_segment_blocks<<<nblk(n, 256), 256, 0, dev->stream>>>(sv, nsegs, blocks256, blocks1024, blocksT);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -5,8 +5,10 @@
 // kernels/traceRays.dev.cu:26-54.
 //
 // Design (wave64):
-//   * one ray per thread; blocks past the queue's device-side count exit at
-//     once, so a launch can be sized for the queue's capacity;
+//   * queue traversal, one ray per thread (k_trace_static), or, for the
+//     secondary rays of large scenes, from a per-segment pool (k_trace_pool):
+//     a wave claims 64 rays at a time and refills lanes as their walks end,
+//     so short walks do not idle behind the wave's longest;
 //   * nearest-child-first traversal of a 4-wide BVH (collapsed from the
 //     binary build) over 128-byte nodes: the four child boxes in one fetch
 //     (seven 16-byte loads), entry distances ordered by a sorting network;
@@ -125,27 +127,6 @@ BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec
   return true;
 }
 
-struct Stack {
-  uint32_t *lds;         // &ring[0][lane-in-block]
-  uint32_t deep[DEEP];
-  int sp;
-  BNX_D void push(uint32_t v)
-  {
-    int s = sp & (SHORT - 1);
-    if (sp >= SHORT) deep[sp - SHORT] = lds[s * TB];
-    lds[s * TB] = v;
-    sp++;
-  }
-  BNX_D uint32_t pop()
-  {
-    sp--;
-    int s = sp & (SHORT - 1);
-    uint32_t v = lds[s * TB];
-    if (sp >= SHORT) lds[s * TB] = deep[sp - SHORT];
-    return v;
-  }
-};
-
 struct Counts {
   uint32_t nodes, tris, spheres;
 };
@@ -170,9 +151,26 @@ BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
   ca = c;
 }
 
-BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float tmax, bool any_hit, Stack &st,
+// lds: this lane's ring, &ring[0][lane]; the stack depth and the spill
+// array are locals so the depth stays in a register
+BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float tmax, bool any_hit, uint32_t *lds,
                       Counts &cnt)
 {
+  uint32_t deep[DEEP];
+  int sp = 0;
+  auto push = [&](uint32_t v) {
+    const int s = sp & (SHORT - 1);
+    if (sp >= SHORT) deep[sp - SHORT] = lds[s * TB];
+    lds[s * TB] = v;
+    sp++;
+  };
+  auto pop = [&]() -> uint32_t {
+    sp--;
+    const int s = sp & (SHORT - 1);
+    const uint32_t v = lds[s * TB];
+    if (sp >= SHORT) lds[s * TB] = deep[sp - SHORT];
+    return v;
+  };
   const v3 wdir = fix_dir(wdir_in);
   const v3 winv = v3{1.f / wdir.x, 1.f / wdir.y, 1.f / wdir.z};
   HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
@@ -181,7 +179,6 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
   BlasDev blas{};
   bool in_blas = false;
   int inst = -1;
-  st.sp = 0;
   uint32_t cur = 0;
   uint32_t nvis = 0;
   // every node is entered at most once per ray, so a ray can take no more
@@ -207,9 +204,9 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
       cswap(t1, c1, t2, c2);
       const float INF = __builtin_huge_valf();
       if (t0 < INF) {
-        if (t3 < INF) st.push(c3);
-        if (t2 < INF) st.push(c2);
-        if (t1 < INF) st.push(c1);
+        if (t3 < INF) push(c3);
+        if (t2 < INF) push(c2);
+        if (t1 < INF) push(c1);
         cur = c0;
         continue;
       }
@@ -230,7 +227,7 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
           d = fix_dir(xfm_vector_rows(I.w2o, wdir));
           inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
         }
-        st.push(LEAVE_BLAS);
+        push(LEAVE_BLAS);
         in_blas = true;
         nodes = blas.nodes;
         cur = 0;
@@ -247,8 +244,8 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
       }
       if (any_hit && found) break;
     }
-    if (st.sp == 0) break;
-    cur = st.pop();
+    if (sp == 0) break;
+    cur = pop();
   }
   cnt.nodes += nvis;
   return best;
@@ -258,15 +255,14 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
                                                   uint32_t flags)
 {
   __shared__ uint32_t ring[SHORT * TB];
-  Stack st;
-  st.lds = ring + threadIdx.x;
+  uint32_t *lds = ring + threadIdx.x;
   {
     const int i = blockIdx.x * TB + threadIdx.x;
     if (i >= n) return;
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
-    HitRec h = traverse(S, xyz(r0), xyz(r1), r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, st, cnt);
+    HitRec h = traverse(S, xyz(r0), xyz(r1), r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
     bnx_hit out;
     out.t = h.t;
     out.u = h.prim >= 0 ? h.u : 0.f;
@@ -285,15 +281,15 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
   }
 }
 
-// one ray per thread over the generation's segments; each block lies inside
-// one request's segment, so the request's scene pointers are block-uniform.
-// counters: statistics array (STAT_NODES, STAT_TRIS, STAT_SPHERES)
-__global__ __launch_bounds__(TB) void k_trace_queue(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
-                                                    unsigned long long *counters)
+// one ray per thread over the generation's segments (256-thread block map);
+// each block lies inside one request's segment, so the request's scene
+// pointers are block-uniform.  counters: statistics array (STAT_NODES,
+// STAT_TRIS, STAT_SPHERES)
+__global__ __launch_bounds__(TB) void k_trace_static(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
+                                                     unsigned long long *counters)
 {
   __shared__ uint32_t ring[SHORT * TB];
-  Stack st;
-  st.lds = ring + threadIdx.x;
+  uint32_t *lds = ring + threadIdx.x;
   Counts cnt{0, 0, 0};
   int i;
   bool valid;
@@ -304,10 +300,184 @@ __global__ __launch_bounds__(TB) void k_trace_queue(const SlotDev *__restrict__ 
     float4 d = q.dir[i];
     const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
-    if (S.num_insts > 0) h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, st, cnt);
+    if (S.num_insts > 0) h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, lds, cnt);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
+  }
+  uint32_t a = wave_sum(cnt.nodes), b = wave_sum(cnt.tris), c = wave_sum(cnt.spheres);
+  if (lane_id() == 0) {
+    stat_add(counters, STAT_NODES, a);
+    stat_add(counters, STAT_TRIS, b);
+    stat_add(counters, STAT_SPHERES, c);
+  }
+}
+
+// The generation's rays, segment by segment: every block lies inside one
+// request's segment (its scene pointers are block-uniform) and draws rays
+// from the segment's pool.  A wave claims POOL_CHUNK ray indices at a time
+// (one atomic per chunk) and hands them to its lanes as they finish, so a
+// lane whose walk ends early takes the next ray instead of idling until the
+// wave's longest walk is over.  The walk is the one of traverse() above,
+// taken one node or leaf per loop trip.
+constexpr int POOL_CHUNK = 64;
+
+__global__ __launch_bounds__(TB) void k_trace_pool(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
+                                                   unsigned long long *counters)
+{
+  __shared__ uint32_t ring[SHORT * TB];
+  uint32_t *lds = ring + threadIdx.x;
+  uint32_t deep[DEEP];
+  int sp = 0;
+  auto push = [&](uint32_t v) {
+    const int s = sp & (SHORT - 1);
+    if (sp >= SHORT) deep[sp - SHORT] = lds[s * TB];
+    lds[s * TB] = v;
+    sp++;
+  };
+  auto pop = [&]() -> uint32_t {
+    sp--;
+    const int s = sp & (SHORT - 1);
+    const uint32_t v = lds[s * TB];
+    if (sp >= SHORT) lds[s * TB] = deep[sp - SHORT];
+    return v;
+  };
+  const int seg = sv.blkT[blockIdx.x];
+  const Segment sg = sv.segs[seg];
+  const SceneDev S = slots[sg.slot].scene;
+  const bool empty_scene = S.num_insts == 0;
+  const int refill_at = sg.refill;
+  Counts cnt{0, 0, 0};
+  // per-lane walk state
+  bool active = false, any_hit = false, in_blas = false;
+  int ray = 0, inst = -1;
+  v3 worg{0.f, 0.f, 0.f}, wdir{1.f, 1.f, 1.f}, winv{1.f, 1.f, 1.f};
+  v3 o = worg, d = wdir, inv = winv;
+  HitRec best{0.f, 0.f, 0.f, -1, -1, -1};
+  const Node4 *nodes = S.tlas_nodes;
+  BlasDev blas{};
+  uint32_t cur = 0;
+  // wave-uniform chunk [q_next, q_end) of segment-local ray indices: first
+  // the wave's own chunk of the segment's static part, then chunks from the
+  // pool (whose counter starts where the static part ends)
+  int q_next = (((int)blockIdx.x - sg.blockT) * (TB / 64) + (int)(threadIdx.x >> 6)) * POOL_CHUNK;
+  int q_end = min(q_next + POOL_CHUNK, sg.count);
+  bool dry = false;
+  for (;;) {
+    // ------------------------------------------------------- refill
+    const bool need = !active;
+    const uint64_t mask = __ballot(need);
+    if (mask != 0 && !dry) {
+      if (q_next >= q_end) {
+        int base = 0;
+        if (lane_id() == 0) base = atomicAdd(sv.pool + seg, POOL_CHUNK);
+        base = __builtin_amdgcn_readfirstlane(base);
+        if (base >= sg.count) {
+          dry = true;
+          q_next = q_end = 0;
+        } else {
+          q_next = base;
+          q_end = min(base + POOL_CHUNK, sg.count);
+        }
+      }
+      const int avail = max(0, q_end - q_next);
+      const int rank = __popcll(mask & lanemask_lt());
+      if (need && rank < avail) {
+        ray = sg.begin + q_next + rank;
+        const float4 ro = q.org[ray], rd = q.dir[ray];
+        any_hit = (__float_as_uint(rd.w) & FLAG_SHADOW) != 0;
+        worg = xyz(ro);
+        wdir = fix_dir(xyz(rd));
+        winv = v3{1.f / wdir.x, 1.f / wdir.y, 1.f / wdir.z};
+        o = worg; d = wdir; inv = winv;
+        best = HitRec{ro.w, 0.f, 0.f, -1, -1, -1};
+        nodes = S.tlas_nodes;
+        in_blas = false;
+        inst = -1;
+        cur = 0;
+        sp = 0;
+        active = true;
+      }
+      q_next += min(__popcll(mask), avail);
+    }
+    if (__ballot(active) == 0) break;
+    // ---------------------------------------- walk until enough lanes idle
+    while (active) {
+      bool done = false;
+      if (empty_scene) {
+        done = true;
+      } else if (cur < EMPTY4) {
+        const float4 *np = (const float4 *)(nodes + cur);
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const uint4 ch = ((const uint4 *)np)[6];
+        cnt.nodes++;
+        float t0 = child_entry(o, inv, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, 0.f, best.t);
+        float t1 = child_entry(o, inv, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, 0.f, best.t);
+        float t2 = child_entry(o, inv, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, 0.f, best.t);
+        float t3 = child_entry(o, inv, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, 0.f, best.t);
+        uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+        cswap(t0, c0, t1, c1);
+        cswap(t2, c2, t3, c3);
+        cswap(t0, c0, t2, c2);
+        cswap(t1, c1, t3, c3);
+        cswap(t1, c1, t2, c2);
+        const float INF = __builtin_huge_valf();
+        if (t0 < INF) {
+          if (t3 < INF) push(c3);
+          if (t2 < INF) push(c2);
+          if (t1 < INF) push(c1);
+          cur = c0;
+        } else if (sp == 0) {
+          done = true;
+        } else {
+          cur = pop();
+        }
+      } else if (cur == LEAVE_BLAS) {
+        in_blas = false;
+        o = worg; d = wdir; inv = winv;
+        nodes = S.tlas_nodes;
+        if (sp == 0) done = true;
+        else cur = pop();
+      } else {
+        const uint32_t first = (cur & ~LEAF_BIT) >> 3;
+        const int count = (int)(cur & 7u);
+        if (!in_blas) {
+          inst = S.tlas_order[first];                  // TLAS leaves hold one instance
+          const InstDev &I = S.insts[inst];
+          blas = S.blas[I.blas];
+          if (!I.identity) {
+            o = xfm_point_rows(I.w2o, worg);
+            d = fix_dir(xfm_vector_rows(I.w2o, wdir));
+            inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+          }
+          push(LEAVE_BLAS);
+          in_blas = true;
+          nodes = blas.nodes;
+          cur = 0;
+        } else {
+          bool found = false;
+          if (blas.type == BLAS_TRIANGLES) {
+            cnt.tris += count;
+            for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, 0.f, best, inst);
+          } else {
+            cnt.spheres += count;
+            for (int k = 0; k < count; k++)
+              found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, 0.f, best, inst);
+          }
+          if ((any_hit && found) || sp == 0) done = true;
+          else cur = pop();
+        }
+      }
+      if (done) {
+        if (best.prim >= 0) q.org[ray].w = best.t;
+        q.hit[ray] = make_int4(best.prim, best.geom, best.inst, __float_as_int(best.u));
+        q.hit_v[ray] = best.v;
+        active = false;
+      }
+      // leave the walk for a refill once refill_at lanes are idle (the test
+      // is wave-uniform, so the lanes still walking leave together)
+      if (!dry && 64 - __popcll(__ballot(true)) >= refill_at) break;
+    }
   }
   uint32_t a = wave_sum(cnt.nodes), b = wave_sum(cnt.tris), c = wave_sum(cnt.spheres);
   if (lane_id() == 0) {
@@ -331,10 +501,14 @@ int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, ui
   return BNX_OK;
 }
 
-int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, const TraceQueue &q,
-                       unsigned long long *counters)
+int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, int blocksT, bool pooled,
+                       const TraceQueue &q, unsigned long long *counters)
 {
-  if (blocks256 > 0) k_trace_queue<<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  if (pooled) {
+    if (blocksT > 0) k_trace_pool<<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  } else {
+    if (blocks256 > 0) k_trace_static<<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  }
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

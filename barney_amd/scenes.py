@@ -235,9 +235,8 @@ def config3(n: int = 4096, num_spheres: int = 4_000_000, width: int = 1920, heig
     displaced in y by seeded 8-octave fBm (2 (n - 1)^2 triangles; 33.5M at
     n = 4096), num_spheres spheres resting on the surface (x, z U[-1, 1],
     radius U(0.002, 0.01)), per-primitive colour U[0.2, 0.8]^3 (terrain:
-    earth tones), each primitive matte or glossy (AnariPBR: terrain metallic
-    0, roughness 0.3; spheres metallic 0.5, roughness 0.2) with probability
-    1/2, lit by the seeded lobe HDRI map only; 1920x1080, fovy 45, primary
+    earth tones), each primitive matte or glossy (AnariPBR, metallic 0,
+    roughness 0.2) with probability 1/2, lit by the seeded lobe HDRI map only; 1920x1080, fovy 45, primary
     rays and 4 bounce generations.  The triangles and spheres of each
     material form their own geometry."""
     seed = SEEDS["c3"]
@@ -273,8 +272,8 @@ def config3(n: int = 4096, num_spheres: int = 4_000_000, width: int = 1920, heig
     glossy_s = us[:, 6] < 0.5
     del us
     from barney_amd import abi
-    pbr_t = abi.pbr(metallic=0.0, roughness=0.3)
-    pbr_s = abi.pbr(metallic=0.5, roughness=0.2)
+    pbr_t = abi.pbr(metallic=0.0, roughness=0.2)
+    pbr_s = abi.pbr(metallic=0.0, roughness=0.2)
     s = SurfaceScene(width=width, height=height, spp=spp)
     s.meshes.append(TriMesh(verts, tris[~glossy_t], tri_col[~glossy_t], user_id=0))
     s.meshes.append(TriMesh(verts, tris[glossy_t], tri_col[glossy_t], user_id=1, material=pbr_t))
