@@ -230,20 +230,24 @@ def fbm_height(seed: int, x: np.ndarray, z: np.ndarray, octaves: int = 8, base_f
 
 
 def config3(n: int = 4096, num_spheres: int = 4_000_000, width: int = 1920, height: int = 1080, spp: int = 64,
-            env_size=(1024, 512)) -> SurfaceScene:
-    """Large triangle scene: an n x n heightfield over [-1, 1]^2 (x, z)
-    displaced in y by seeded 8-octave fBm (2 (n - 1)^2 triangles; 33.5M at
-    n = 4096), num_spheres spheres resting on the surface (x, z U[-1, 1],
-    radius U(0.002, 0.01)), per-primitive colour U[0.2, 0.8]^3 (terrain:
+            env_size=(1024, 512), extent: float = 16.0) -> SurfaceScene:
+    """Large triangle scene: an n x n heightfield over [-E, E]^2 (x, z),
+    E = extent, displaced in y by E times seeded 8-octave fBm (2 (n - 1)^2
+    triangles; 33.5M at n = 4096), num_spheres spheres resting on the surface
+    (x, z U[-E, E], radius U(0.002, 0.01); at E = 16 the 4M spheres cover
+    about half of the ground, so both terrain and spheres stay in view),
+    per-primitive colour U[0.2, 0.8]^3 (terrain:
     earth tones), each primitive matte or glossy (AnariPBR, metallic 0,
-    roughness 0.2) with probability 1/2, lit by the seeded lobe HDRI map only; 1920x1080, fovy 45, primary
+    roughness 0.2) with probability 1/2, lit by the seeded lobe HDRI map
+    only; 1920x1080, fovy 45 from (0, E/4, E/2) toward the origin, primary
     rays and 4 bounce generations.  The triangles and spheres of each
     material form their own geometry."""
     seed = SEEDS["c3"]
     g = np.linspace(0.0, 1.0, n, dtype=np.float32)
     X, Z = np.meshgrid(g, g)
-    Y = fbm_height(seed, X, Z)
-    verts = np.stack([2.0 * X - 1.0, Y, 2.0 * Z - 1.0], -1).reshape(-1, 3).astype(np.float32)
+    E = np.float32(extent)
+    Y = E * fbm_height(seed, X, Z)
+    verts = np.stack([E * (2.0 * X - 1.0), Y, E * (2.0 * Z - 1.0)], -1).reshape(-1, 3).astype(np.float32)
     del X, Z
     k = np.arange(n - 1, dtype=np.int32)
     i0 = (k[None, :] + n * k[:, None]).ravel()              # (row j, column i) -> j * n + i
@@ -263,10 +267,12 @@ def config3(n: int = 4096, num_spheres: int = 4_000_000, width: int = 1920, heig
     glossy_t = u[:, 1] < 0.5
     del u
     us = uniform_table(seed, 32, num_spheres, 7)
-    sx = (2.0 * us[:, 0] - 1.0).astype(np.float32)
-    sz = (2.0 * us[:, 1] - 1.0).astype(np.float32)
+    ux = us[:, 0].astype(np.float32)
+    uz = us[:, 1].astype(np.float32)
+    sx = (E * (2.0 * ux - 1.0)).astype(np.float32)
+    sz = (E * (2.0 * uz - 1.0)).astype(np.float32)
     rad = (0.002 + 0.008 * us[:, 2]).astype(np.float32)
-    sy = fbm_height(seed, (sx + 1) * 0.5, (sz + 1) * 0.5) + rad
+    sy = E * fbm_height(seed, ux, uz) + rad
     centers = np.stack([sx, sy, sz], 1).astype(np.float32)
     sph_col = (0.2 + 0.6 * us[:, 3:6]).astype(np.float32)
     glossy_s = us[:, 6] < 0.5
@@ -283,8 +289,9 @@ def config3(n: int = 4096, num_spheres: int = 4_000_000, width: int = 1920, heig
     s.max_generations = 5          # primary rays + 4 bounce generations
     s.ambient = 1.0
     s.bg_color = (0.0, 0.0, 0.0, 1.0)
-    s.camera = dict(position=(0.0, 0.6, 0.8), direction=(0.0, -0.65, -1.3), up=(0.0, 1.0, 0.0), fovy=45.0,
-                    target=(0.0, -0.05, -0.5))
+    e = float(extent)
+    s.camera = dict(position=(0.0, 0.25 * e, 0.5 * e), direction=(0.0, -0.25 * e, -0.5 * e), up=(0.0, 1.0, 0.0),
+                    fovy=45.0, target=(0.0, 0.0, 0.0))
     return s
 
 

@@ -57,8 +57,9 @@ def march_bytes(samples, cells, rays):
 # rocprofv3 summaries of each workload's dominant kernel; each holds a line
 # "traffic_bytes_per_launch <kernel> <bytes>" measured with FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by a plain copy of known size
-PROFILES = {"c5": "k_trace_queue_config5_serving.md", "c2": "k_volume_march_config2_1024x1024x16spp.md",
-            "c1": "k_trace_queue_config1_512x512x1spp.md"}
+PROFILES = {"c5": "k_trace_static_config5_serving.md", "c2": "k_volume_march_config2_1024x1024x16spp.md",
+            "c1": "k_trace_static_config1_512x512x1spp.md", "c3": "k_trace_config3_1920x1080x64spp.md",
+            "c4": "k_volume_march_config4_1024x1024x32spp.md"}
 
 
 def profile_traffic(workload, kernel):
@@ -88,7 +89,10 @@ def roofline(tot, workload=None):
         else:
             byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
     else:
-        kernel, kms, launches = "k_trace_queue", tot["trace_ms"], tot["trace_launches"]
+        # queue traversal: one ray per thread, or pooled for the secondary rays
+        # of large scenes (config 3); the HIP events bracket whichever ran
+        kernel = "k_trace_static+k_trace_pool" if workload == "c3" else "k_trace_static"
+        kms, launches = tot["trace_ms"], tot["trace_launches"]
         byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
     achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     traffic = profile_traffic(workload, kernel)

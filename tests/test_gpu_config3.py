@@ -1,6 +1,6 @@
 """Config 3 (fBm terrain + spheres, 50/50 matte / glossy, HDRI light,
 primary + 4 bounce generations) on the GPU against the CPU oracle at a
-reduced size: a 160^2 heightfield and 20,000 spheres.  Gates: primary IDs
+reduced size: a 160^2 heightfield over [-2, 2]^2 and 20,000 spheres.  Gates: primary IDs
 bit-exact, 1-spp pixels within 1e-3 absolute, converged RMSE within 1e-3."""
 import numpy as np
 import pytest
@@ -14,7 +14,7 @@ pytestmark = pytest.mark.gpu
 
 @pytest.fixture(scope="module")
 def small_c3():
-    return scenes.config3(n=160, num_spheres=20000, width=192, height=108, spp=1, env_size=(256, 128))
+    return scenes.config3(n=160, num_spheres=20000, width=192, height=108, spp=1, env_size=(256, 128), extent=2.0)
 
 
 def _render_both(dev, orc, sc, spp):
