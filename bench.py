@@ -83,7 +83,10 @@ def umesh_bytes(nodes, tets, cells, rays):
 def roofline(tot, workload=None):
     """dominant kernel (by summed HIP-event time) and its algorithmic GB/s"""
     if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
-        kernel, kms, launches = "k_volume_march", tot["march_ms"], tot["march_launches"]
+        # the full variant runs for tet fields and isosurfaces, the lean one otherwise
+        full = tot.get("umesh_nodes", 0) > 0 or workload == "c4"
+        kernel = "k_volume_march<true>" if full else "k_volume_march<false>"
+        kms, launches = tot["march_ms"], tot["march_launches"]
         if tot.get("umesh_nodes", 0) > 0:
             byts = umesh_bytes(tot["umesh_nodes"], tot["umesh_tets"], tot["mc_cells"], tot["march_rays"])
         else:
