@@ -28,6 +28,12 @@ namespace bnx {
 namespace {
 
 constexpr int TB = 256;        // threads per block
+// waves per SIMD the queue-trace kernels are compiled for (register budget:
+// 512 / waves VGPRs); the walks are latency bound, so more resident waves
+// hide more of each dependent node fetch
+#ifndef BNX_TRACE_WAVES
+#define BNX_TRACE_WAVES 7
+#endif
 constexpr int SHORT = 16;      // LDS ring entries per lane (power of two)
 constexpr int DEEP = 176;      // private spill entries per lane
 constexpr int STACK_CAP = SHORT + DEEP;
@@ -285,7 +291,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
 // each block lies inside one request's segment, so the request's scene
 // pointers are block-uniform.  counters: statistics array (STAT_NODES,
 // STAT_TRIS, STAT_SPHERES)
-__global__ __launch_bounds__(TB) void k_trace_static(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
                                                      unsigned long long *counters)
 {
   __shared__ uint32_t ring[SHORT * TB];
@@ -322,7 +328,7 @@ __global__ __launch_bounds__(TB) void k_trace_static(const SlotDev *__restrict__
 // taken one node or leaf per loop trip.
 constexpr int POOL_CHUNK = 64;
 
-__global__ __launch_bounds__(TB) void k_trace_pool(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
                                                    unsigned long long *counters)
 {
   __shared__ uint32_t ring[SHORT * TB];

@@ -192,8 +192,17 @@ __global__ void k_umesh_raster(const int4 *tets, const float4 *verts, int n, Vol
 // trace kernel is skipped and every ray enters here as a surface miss
 // (surfaces == 0).  FULL: isosurfaces and tet fields may occur; otherwise
 // every volume of the batch is structured and there are no isosurfaces.
+// waves per SIMD the march kernel is compiled for, lean and full variant
+#ifndef BNX_MARCH_WAVES_LEAN
+#define BNX_MARCH_WAVES_LEAN 5
+#endif
+#ifndef BNX_MARCH_WAVES_FULL
+#define BNX_MARCH_WAVES_FULL 6
+#endif
 template <bool FULL>
-__global__ __launch_bounds__(256) void k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? BNX_MARCH_WAVES_FULL
+                                                                          : BNX_MARCH_WAVES_LEAN))) void
+k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
                                                       int surfaces, unsigned long long *stats)
 {
   __shared__ uint32_t ring[FULL ? PQ_SHORT * PQ_TB : 1];
