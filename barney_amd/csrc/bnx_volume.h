@@ -32,6 +32,7 @@ struct VolumeDev {
   // tets (UMeshField, umesh/common/UMeshField.h:52-116)
   const float4 BNX_G *uverts;   // position, scalar
   const int4 BNX_G *utets;      // vertex indices, in element-BVH slot order
+  const int4 BNX_G *unbr;       // per tet slot: the slot across the face opposite vertex k (-1: boundary)
   const Node BNX_G *ebvh;       // element BVH
   // both
   float lo[3], hi[3];         // world bounds
@@ -70,7 +71,7 @@ struct Volume {
   uint64_t version = 0;         // bumped whenever dd changes
   VolumeDev dd{};
   DevMem scalars, ranges, majorants, tf;
-  DevMem uverts, utets;
+  DevMem uverts, utets, unbr;
   Bvh ebvh;                     // element BVH (tets)
 };
 
@@ -185,26 +186,6 @@ BNX_D bool tet_scalar(const VolumeDev &v, int slot, v3 P, float &out)
   return true;
 }
 
-// tet_scalar of the tet in 'slot' when P lies inside it by a margin: every
-// normalized barycentric coordinate above 1e-4.  In a mesh of
-// non-overlapping tets no other tet's test can then accept P, so the value
-// is the one the depth-first query would return, bit for bit.
-BNX_D bool tet_scalar_interior(const VolumeDev &v, int slot, v3 P, float &out)
-{
-  const int4 t = v.utets[slot];
-  const float4 v0 = v.uverts[t.x], v1 = v.uverts[t.y], v2 = v.uverts[t.z], v3v = v.uverts[t.w];
-  const float t3 = implicit_plane(P, v0, v1, v2);
-  const float t2 = implicit_plane(P, v0, v3v, v1);
-  const float t1 = implicit_plane(P, v0, v2, v3v);
-  const float t0 = implicit_plane(P, v1, v3v, v2);
-  const float sum = ((t0 + t1) + t2) + t3;
-  const float m = 1e-4f * sum;
-  if (!(t0 > m && t1 > m && t2 > m && t3 > m)) return false;
-  const float scale = 1.f / sum;
-  out = scale * (((t0 * v0.w + t1 * v1.w) + t2 * v2.w) + t3 * v3v.w);
-  return true;
-}
-
 BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz, v3 P)
 {
   return lx <= P.x && P.x <= hx && ly <= P.y && P.y <= hy && lz <= P.z && P.z <= hz;
@@ -214,14 +195,62 @@ BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz,
 // fixed-box query with the point P (closed box tests), depth first with
 // child 0 before child 1, leaf slots in order; the first tet that contains
 // P gives the value, NaN if none does
+// The four plane values of P against the tet in 'slot' (t_k: against the face
+// opposite vertex k, positive inside), their sum and the interpolated value
+struct TetPlanes {
+  float t[4];
+  float sum;
+  float value;
+};
+BNX_D TetPlanes tet_planes(const VolumeDev &v, int slot, v3 P)
+{
+  const int4 t = v.utets[slot];
+  const float4 v0 = v.uverts[t.x], v1 = v.uverts[t.y], v2 = v.uverts[t.z], v3v = v.uverts[t.w];
+  TetPlanes r;
+  r.t[3] = implicit_plane(P, v0, v1, v2);
+  r.t[2] = implicit_plane(P, v0, v3v, v1);
+  r.t[1] = implicit_plane(P, v0, v2, v3v);
+  r.t[0] = implicit_plane(P, v1, v3v, v2);
+  r.sum = ((r.t[0] + r.t[1]) + r.t[2]) + r.t[3];
+  r.value = (1.f / r.sum) * (((r.t[0] * v0.w + r.t[1] * v1.w) + r.t[2] * v2.w) + r.t[3] * v3v.w);
+  return r;
+}
+// P lies inside the tet by a margin: every normalized barycentric coordinate
+// above 1e-4.  In a mesh of non-overlapping tets no other tet's test can
+// then accept P, so the value is the one the depth-first query would
+// return, bit for bit (tet_scalar's arithmetic, in the same order).
+BNX_D bool planes_interior(const TetPlanes &r)
+{
+  const float m = 1e-4f * r.sum;
+  return r.t[0] > m && r.t[1] > m && r.t[2] > m && r.t[3] > m;
+}
+
+constexpr int TET_WALK_STEPS = 6;
+
 BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
 {
   // consecutive queries of a ray are close together: try the tet of the
-  // previous query first
+  // previous query, then walk across the face P lies furthest behind, a few
+  // steps; a tet that holds P with margin is the only one that can (see
+  // planes_interior), else the depth-first query decides
   if (cx.last_field == v.utets && cx.last_tet >= 0) {
-    float f;
-    cx.utets++;
-    if (tet_scalar_interior(v, cx.last_tet, P, f)) return f;
+    int cur = cx.last_tet;
+    for (int step = 0; step <= TET_WALK_STEPS; step++) {
+      cx.utets++;
+      const TetPlanes r = tet_planes(v, cur, P);
+      if (planes_interior(r)) {
+        cx.last_tet = cur;
+        return r.value;
+      }
+      if (step == TET_WALK_STEPS || !(r.sum > 0.f)) break;
+      int k = 0;
+      for (int j = 1; j < 4; j++)
+        if (r.t[j] < r.t[k]) k = j;
+      const int4 nb = v.unbr[cur];
+      const int next = k == 0 ? nb.x : (k == 1 ? nb.y : (k == 2 ? nb.z : nb.w));
+      if (next < 0) break;
+      cur = next;
+    }
   }
   uint32_t deep[PQ_DEEP];
   int sp = 0;

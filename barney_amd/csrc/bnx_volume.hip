@@ -23,6 +23,8 @@
 #include "bnx_render.h"
 #include "bnx_volume.h"
 
+#include <hipcub/hipcub.hpp>
+
 namespace bnx {
 namespace {
 
@@ -296,6 +298,90 @@ int volume_build(Volume *v)
   return BNX_OK;
 }
 
+// ------------------------------------------------- tet face adjacency
+// Face k of a tet is the one opposite its vertex k.  Every face gets a
+// 64-bit hash of its sorted vertex triple; after sorting the hashes, a run of
+// exactly two faces with the same triple is a shared face.  Hash collisions
+// and non-manifold faces leave the neighbour at -1 (the walk then falls back
+// to the depth-first query), so the table only ever names true neighbours.
+__device__ uint64_t mix64(uint64_t x)
+{
+  x ^= x >> 30;
+  x *= 0xbf58476d1ce4e5b9ull;
+  x ^= x >> 27;
+  x *= 0x94d049bb133111ebull;
+  x ^= x >> 31;
+  return x;
+}
+
+__device__ int3 face_of(const int4 t, int k)
+{
+  int a, b, c;
+  if (k == 0) { a = t.y; b = t.z; c = t.w; }
+  else if (k == 1) { a = t.x; b = t.z; c = t.w; }
+  else if (k == 2) { a = t.x; b = t.y; c = t.w; }
+  else { a = t.x; b = t.y; c = t.z; }
+  // sort the triple
+  if (a > b) { int x = a; a = b; b = x; }
+  if (b > c) { int x = b; b = c; c = x; }
+  if (a > b) { int x = a; a = b; b = x; }
+  return make_int3(a, b, c);
+}
+
+__global__ void k_face_keys(const int4 *tets, int ntet, uint64_t *keys, int *vals, int4 *nbr)
+{
+  const int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= ntet) return;
+  const int4 t = tets[i];
+  for (int k = 0; k < 4; k++) {
+    const int3 f = face_of(t, k);
+    keys[4 * (size_t)i + k] =
+        mix64(((uint64_t)(uint32_t)f.x << 32 | (uint32_t)f.y) ^ mix64((uint64_t)(uint32_t)f.z + 0x9e3779b97f4a7c15ull));
+    vals[4 * (size_t)i + k] = 4 * i + k;
+  }
+  nbr[i] = make_int4(-1, -1, -1, -1);
+}
+
+__global__ void k_face_match(const int4 *tets, const uint64_t *keys, const int *vals, long long nf, int4 *nbr)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i + 1 >= nf) return;
+  if (keys[i] != keys[i + 1]) return;
+  if (i > 0 && keys[i - 1] == keys[i]) return;         // not the first of its run
+  if (i + 2 < nf && keys[i + 2] == keys[i]) return;    // a run of more than two
+  const int fa = vals[i], fb = vals[i + 1];
+  const int ta = fa >> 2, ka = fa & 3, tb = fb >> 2, kb = fb & 3;
+  const int3 A = face_of(tets[ta], ka), B = face_of(tets[tb], kb);
+  if (A.x != B.x || A.y != B.y || A.z != B.z || ta == tb) return;
+  ((int *)nbr)[4 * (size_t)ta + ka] = tb;
+  ((int *)nbr)[4 * (size_t)tb + kb] = ta;
+}
+
+int build_tet_neighbors(Device *dev, const int4 *tets, int ntet, DevMem &nbr)
+{
+  hipStream_t st = dev->stream;
+  const size_t nf = 4 * (size_t)ntet;
+  DevMem keys, keys2, vals, vals2, tmp;
+  BNX_HIP(nbr.ensure(sizeof(int4) * (size_t)(ntet > 0 ? ntet : 1)));
+  if (ntet == 0) return BNX_OK;
+  BNX_HIP(keys.ensure(sizeof(uint64_t) * nf));
+  BNX_HIP(keys2.ensure(sizeof(uint64_t) * nf));
+  BNX_HIP(vals.ensure(sizeof(int) * nf));
+  BNX_HIP(vals2.ensure(sizeof(int) * nf));
+  k_face_keys<<<nblk(ntet, 256), 256, 0, st>>>(tets, ntet, keys.as<uint64_t>(), vals.as<int>(), nbr.as<int4>());
+  size_t tmp_bytes = 0;
+  BNX_HIP(hipcub::DeviceRadixSort::SortPairs(nullptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                             vals.as<int>(), vals2.as<int>(), (int)nf, 0, 64, st));
+  BNX_HIP(tmp.ensure(tmp_bytes));
+  BNX_HIP(hipcub::DeviceRadixSort::SortPairs(tmp.ptr, tmp_bytes, keys.as<uint64_t>(), keys2.as<uint64_t>(),
+                                             vals.as<int>(), vals2.as<int>(), (int)nf, 0, 64, st));
+  k_face_match<<<nblk((long)nf, 256), 256, 0, st>>>(tets, keys2.as<uint64_t>(), vals2.as<int>(), (long long)nf,
+                                                   nbr.as<int4>());
+  BNX_HIP(hipGetLastError());
+  BNX_HIP(hipStreamSynchronize(st));
+  return BNX_OK;
+}
+
 int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *verts4, int nvert)
 {
   Device *dev = v->dev;
@@ -321,6 +407,9 @@ int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *ver
   BNX_HIP(v->utets.ensure(sizeof(int4) * (size_t)ntet));
   k_tet_sort<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), v->ebvh.order.as<int>(), ntet, v->utets.as<int4>());
   BNX_HIP(hipGetLastError());
+  // face neighbours of the tets, in slot order, for the walking point query
+  if ((rc = build_tet_neighbors(dev, v->utets.as<int4>(), ntet, v->unbr)) != BNX_OK) return rc;
+  BNX_SET(d.unbr, v->unbr.ptr);
   BNX_SET(d.uverts, v->uverts.ptr);
   BNX_SET(d.utets, v->utets.ptr);
   BNX_SET(d.ebvh, v->ebvh.nodes.ptr);
