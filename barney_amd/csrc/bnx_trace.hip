@@ -137,12 +137,21 @@ struct Counts {
   uint32_t nodes, tris, spheres;
 };
 
-// entry distance of child k of a 4-wide node, +inf when the ray misses it
+// entry distance of child k of a 4-wide node, +inf when the ray misses it.
+// Branch-free, so the six box vectors of a node are loaded together up front
+// (a short-circuit on the child code let the compiler sink each child's box
+// loads behind the code load, doubling the dependent fetches per node).  The
+// slab distances are finite (|d| >= 1e-6 after fix_dir; empty slots hold
+// inverted infinite boxes), so fminf/fmaxf give the same values as compares.
 BNX_D float child_entry(v3 o, v3 inv, float lx, float hx, float ly, float hy, float lz, float hz, uint32_t code,
                         float tmin, float tmax)
 {
-  float tn;
-  const bool h = code != EMPTY4 && box_test(o, inv, lx, ly, lz, hx, hy, hz, tmin, tmax, tn);
+  const float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
+  const float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
+  const float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
+  const float tn = fmaxf(fmaxf(fmaxf(tmin, fminf(t0x, t1x)), fminf(t0y, t1y)), fminf(t0z, t1z));
+  const float tf = fminf(fminf(fminf(tmax, fmaxf(t0x, t1x)), fmaxf(t0y, t1y)), fmaxf(t0z, t1z));
+  const bool h = (tn <= tf * 1.00001f) & (code != EMPTY4);
   return h ? tn : __builtin_huge_valf();
 }
 
