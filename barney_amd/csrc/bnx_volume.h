@@ -186,9 +186,11 @@ BNX_D bool tet_scalar(const VolumeDev &v, int slot, v3 P, float &out)
   return true;
 }
 
+// closed box test, branch-free (bitwise and) so that a node's loads are
+// issued together rather than behind each other
 BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz, v3 P)
 {
-  return lx <= P.x && P.x <= hx && ly <= P.y && P.y <= hy && lz <= P.z && P.z <= hz;
+  return (lx <= P.x) & (P.x <= hx) & (ly <= P.y) & (P.y <= hy) & (lz <= P.z) & (P.z <= hz);
 }
 
 // UMeshCuBQLSampler::DD::sample (umesh/mc/UMeshCuBQLSampler.h:46-62): a
@@ -261,8 +263,8 @@ BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
       const float4 a = np[0], b = np[1], c = np[2];
       const int4 r = ((const int4 BNX_G *)np)[3];
       cx.unodes++;
-      const bool h0 = r.z >= 0 && box_holds(a.x, a.y, a.z, a.w, b.x, b.y, P);
-      const bool h1 = r.w >= 0 && box_holds(b.z, b.w, c.x, c.y, c.z, c.w, P);
+      const bool h0 = (r.z >= 0) & box_holds(a.x, a.y, a.z, a.w, b.x, b.y, P);
+      const bool h1 = (r.w >= 0) & box_holds(b.z, b.w, c.x, c.y, c.z, c.w, P);
       if (h0 && h1) {
         // push child 1
         const int s = sp & (PQ_SHORT - 1);
