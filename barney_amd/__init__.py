@@ -81,6 +81,10 @@ _PROTOS = [
     ("bnx_frame_destroy", [vp], C.c_int),
     ("bnx_render", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.POINTER(abi.RenderStats)],
      C.c_int),
+    ("bnx_device_get_tuning", [vp, C.POINTER(abi.Tuning)], C.c_int),
+    ("bnx_device_set_tuning", [vp, C.POINTER(abi.Tuning)], C.c_int),
+    ("bnx_autotune", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), C.c_int, C.c_int, C.c_int,
+                      C.POINTER(abi.Tuning), C.POINTER(abi.AutotuneReport)], C.c_int),
     ("bnx_server_create", [vp, C.POINTER(abi.ServerConfig), C.POINTER(vp)], C.c_int),
     ("bnx_server_destroy", [vp], C.c_int),
     ("bnx_server_submit", [vp, C.POINTER(abi.Request), C.POINTER(C.c_int64)], C.c_int),
@@ -136,6 +140,22 @@ class Device:
         f, t = C.c_size_t(), C.c_size_t()
         check(self.L.bnx_device_mem_info(self.h, C.byref(f), C.byref(t)))
         return f.value, t.value
+
+    def tuning(self) -> dict:
+        """the launch tuning in place (bnx_device_get_tuning)"""
+        t = abi.Tuning()
+        check(self.L.bnx_device_get_tuning(self.h, C.byref(t)))
+        return {k: getattr(t, k) for k in abi.TUNING_FIELDS}
+
+    def set_tuning(self, **fields):
+        """change some fields of the launch tuning (bnx_device_set_tuning)"""
+        cur = self.tuning()
+        unknown = set(fields) - set(cur)
+        if unknown:
+            raise ValueError(f"unknown tuning fields {sorted(unknown)}")
+        cur.update(fields)
+        t = abi.Tuning(**cur)
+        check(self.L.bnx_device_set_tuning(self.h, C.byref(t)))
 
     def close(self):
         if self.h:
@@ -403,6 +423,16 @@ class Frame:
             self.h_ = None
 
 
+def autotune(world: World, cam: abi.Camera, rend: abi.Renderer, width: int, height: int, probe_spp: int = 1):
+    """bnx_autotune: choose and set the device's launch tuning for requests
+    like this one; returns (tuning dict, report dict)"""
+    L = world.dev.L
+    t, rep = abi.Tuning(), abi.AutotuneReport()
+    check(L.bnx_autotune(world.h, C.byref(cam), C.byref(rend), width, height, probe_spp, C.byref(t), C.byref(rep)))
+    return ({k: getattr(t, k) for k in abi.TUNING_FIELDS},
+            {"ms_before": rep.ms_before, "ms_after": rep.ms_after, "probes": rep.probes})
+
+
 def render(world: World, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp: int) -> dict:
     st = abi.RenderStats()
     check(world.dev.L.bnx_render(world.h, C.byref(cam), C.byref(rend), frame.h_, spp, C.byref(st)))
@@ -417,9 +447,10 @@ class Server:
     """Continuous-batching request server (bnx_server_*): requests of any
     resident world, resolution and spp share one wavefront on the device."""
 
-    def __init__(self, dev: Device, frame_memory_budget=0, max_paths_in_flight=0, max_active=0):
+    def __init__(self, dev: Device, frame_memory_budget=0, max_paths_in_flight=0, max_active=0,
+                 deadline_slack=0.0):
         self.dev, self.L = dev, dev.L
-        cfg = abi.ServerConfig(frame_memory_budget, max_paths_in_flight, max_active, 0)
+        cfg = abi.ServerConfig(frame_memory_budget, max_paths_in_flight, max_active, float(deadline_slack))
         h = vp()
         check(self.L.bnx_server_create(dev.h, C.byref(cfg), C.byref(h)))
         self.h = h

@@ -44,8 +44,30 @@ constexpr float ONE_OVER_FOUR_PI = 1.f / FOUR_PI;
 // One thread per (request, sample, pixel) of the batch.  The chunk table
 // lists each request's first queue index in ascending order; a thread finds
 // its request by binary search over it.
+// pixel of the pix-th path of a sample: row-major when tw == 0; otherwise
+// bands of th rows, each cut into tiles tw pixels wide (the last band and
+// tile of a row may be narrower), a tile's pixels row-major.  A bijection on
+// [0, W H): only the order of the primary rays in the queue changes, never a
+// pixel's rays or random numbers.
+BNX_D void tile_pixel(int pix, int W, int H, int tw, int th, int &ix, int &iy)
+{
+  if (tw <= 0) {
+    ix = pix % W;
+    iy = pix / W;
+    return;
+  }
+  const int y0 = (int)(pix / ((long long)th * W)) * th;
+  const int bh = min(th, H - y0);
+  const long long pb = pix - (long long)y0 * W;
+  const int x0 = (int)(pb / (tw * bh)) * tw;
+  const int w = min(tw, W - x0);
+  const int loc = (int)(pb - (long long)x0 * bh);
+  ix = x0 + loc % w;
+  iy = y0 + loc / w;
+}
+
 __global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDev *slots, long long q_begin,
-                         long long total, RayQueue q)
+                         long long total, RayQueue q, int tile_w, int tile_h)
 {
   const long long idx = q_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= q_begin + total) return;
@@ -64,7 +86,8 @@ __global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDe
   const long long path = ch.first + (idx - ch.begin);
   const int s = (int)(path / np), pix = (int)(path - (long long)s * np);
   const int accum_base = SL.accum_base;
-  int ix = pix % W, iy = pix / W;
+  int ix, iy;
+  tile_pixel(pix, W, H, tile_w, tile_h, ix, iy);
   int accumID = accum_base + s;
   Random2 rnd((uint32_t)(ix + W * accumID), (uint32_t)(iy + H * accumID));
   uint64_t seed = 0xcafef00dd15ea5e5ull;
@@ -94,7 +117,7 @@ __global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDe
   q.dir[idx] = make_float4(dir.x, dir.y, dir.z, __uint_as_float(mk_flags(false, s)));
   q.rng[idx] = seed;
   uint32_t one = f2h(1.f);
-  q.ps[idx] = make_uint4(one | (one << 16), one, (uint32_t)pix, __float_as_uint(0.f));   // generation 0
+  q.ps[idx] = make_uint4(one | (one << 16), one, (uint32_t)(ix + iy * W), __float_as_uint(0.f));   // generation 0
 }
 
 // ----------------------------------------------------------------- shade
@@ -749,7 +772,8 @@ int launch_raygen(Device *dev, const RaygenChunk *chunks, int num_chunks, const 
                   long long total, const RayQueue &q)
 {
   if (total > 0 && num_chunks > 0)
-    k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(chunks, num_chunks, slots, q_begin, total, q);
+    k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(chunks, num_chunks, slots, q_begin, total, q,
+                                                         dev->raygen_tile_w, dev->raygen_tile_h);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -34,9 +34,11 @@ constexpr int TB = 256;        // threads per block
 #ifndef BNX_TRACE_WAVES
 #define BNX_TRACE_WAVES 7
 #endif
-constexpr int SHORT = 16;      // LDS ring entries per lane (power of two)
-constexpr int DEEP = 176;      // private spill entries per lane
-constexpr int STACK_CAP = SHORT + DEEP;
+// traversal stack: the top SH entries per lane in an LDS ring (SH a power
+// of two: 8, 16 or 32, chosen per device; 16 by default), the rest spilled
+// to a private array, STACK_CAP entries in all whatever SH is
+constexpr int STACK_CAP = 192;
+constexpr int API_SHORT = 16;  // the ray-array entry point's ring
 
 struct HitRec {
   float t, u, v;
@@ -168,22 +170,23 @@ BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 
 // lds: this lane's ring, &ring[0][lane]; the stack depth and the spill
 // array are locals so the depth stays in a register
+template <int SH>
 BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float tmax, bool any_hit, uint32_t *lds,
                       Counts &cnt)
 {
-  uint32_t deep[DEEP];
+  uint32_t deep[STACK_CAP - SH];
   int sp = 0;
   auto push = [&](uint32_t v) {
-    const int s = sp & (SHORT - 1);
-    if (sp >= SHORT) deep[sp - SHORT] = lds[s * TB];
+    const int s = sp & (SH - 1);
+    if (sp >= SH) deep[sp - SH] = lds[s * TB];
     lds[s * TB] = v;
     sp++;
   };
   auto pop = [&]() -> uint32_t {
     sp--;
-    const int s = sp & (SHORT - 1);
+    const int s = sp & (SH - 1);
     const uint32_t v = lds[s * TB];
-    if (sp >= SHORT) lds[s * TB] = deep[sp - SHORT];
+    if (sp >= SH) lds[s * TB] = deep[sp - SH];
     return v;
   };
   const v3 wdir = fix_dir(wdir_in);
@@ -269,7 +272,7 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
 __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *rays, int n, bnx_hit *hits,
                                                   uint32_t flags)
 {
-  __shared__ uint32_t ring[SHORT * TB];
+  __shared__ uint32_t ring[API_SHORT * TB];
   uint32_t *lds = ring + threadIdx.x;
   {
     const int i = blockIdx.x * TB + threadIdx.x;
@@ -277,7 +280,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
-    HitRec h = traverse(S, xyz(r0), xyz(r1), r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
+    HitRec h = traverse<API_SHORT>(S, xyz(r0), xyz(r1), r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
     bnx_hit out;
     out.t = h.t;
     out.u = h.prim >= 0 ? h.u : 0.f;
@@ -300,10 +303,11 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
 // each block lies inside one request's segment, so the request's scene
 // pointers are block-uniform.  counters: statistics array (STAT_NODES,
 // STAT_TRIS, STAT_SPHERES)
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
-                                                     unsigned long long *counters)
+template <int SH>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(
+    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
-  __shared__ uint32_t ring[SHORT * TB];
+  __shared__ uint32_t ring[SH * TB];
   uint32_t *lds = ring + threadIdx.x;
   Counts cnt{0, 0, 0};
   int i;
@@ -315,7 +319,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
     float4 d = q.dir[i];
     const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
-    if (S.num_insts > 0) h = traverse(S, xyz(o), xyz(d), 0.f, o.w, shadow, lds, cnt);
+    if (S.num_insts > 0) h = traverse<SH>(S, xyz(o), xyz(d), 0.f, o.w, shadow, lds, cnt);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
@@ -337,24 +341,25 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
 // taken one node or leaf per loop trip.
 constexpr int POOL_CHUNK = 64;
 
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool(const SlotDev *__restrict__ slots, SegView sv, TraceQueue q,
-                                                   unsigned long long *counters)
+template <int SH>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool(
+    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
-  __shared__ uint32_t ring[SHORT * TB];
+  __shared__ uint32_t ring[SH * TB];
   uint32_t *lds = ring + threadIdx.x;
-  uint32_t deep[DEEP];
+  uint32_t deep[STACK_CAP - SH];
   int sp = 0;
   auto push = [&](uint32_t v) {
-    const int s = sp & (SHORT - 1);
-    if (sp >= SHORT) deep[sp - SHORT] = lds[s * TB];
+    const int s = sp & (SH - 1);
+    if (sp >= SH) deep[sp - SH] = lds[s * TB];
     lds[s * TB] = v;
     sp++;
   };
   auto pop = [&]() -> uint32_t {
     sp--;
-    const int s = sp & (SHORT - 1);
+    const int s = sp & (SH - 1);
     const uint32_t v = lds[s * TB];
-    if (sp >= SHORT) lds[s * TB] = deep[sp - SHORT];
+    if (sp >= SH) lds[s * TB] = deep[sp - SH];
     return v;
   };
   const int seg = sv.blkT[blockIdx.x];
@@ -519,10 +524,19 @@ int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, ui
 int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, int blocksT, bool pooled,
                        const TraceQueue &q, unsigned long long *counters)
 {
+  const int sh = dev->trace_stack_lds;
   if (pooled) {
-    if (blocksT > 0) k_trace_pool<<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
+    if (blocksT > 0) {
+      if (sh == 8) k_trace_pool<8><<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      else if (sh == 32) k_trace_pool<32><<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      else k_trace_pool<16><<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
+    }
   } else {
-    if (blocks256 > 0) k_trace_static<<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+    if (blocks256 > 0) {
+      if (sh == 8) k_trace_static<8><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      else if (sh == 32) k_trace_static<32><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      else k_trace_static<16><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+    }
   }
   BNX_HIP(hipGetLastError());
   return BNX_OK;

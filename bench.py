@@ -3,7 +3,8 @@
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c5|c1|c2|c3|c4]
 
 c5 (default): the serving stream.  Requests over resident worlds (config 1's
-surface scene and config 2's structured volume) arrive with seeded Poisson
+surface scene, config 2's structured volume and config 3's 33.5M-triangle
+terrain with 4M spheres and an HDRI light) arrive with seeded Poisson
 arrivals, each with a resolution in {256^2, 512^2, 1024^2, 2048^2}, spp in
 {1, 4, 16, 64} and a camera on a seeded orbit.  A step is one request.  The
 same K requests are rendered (a) plain: one bnx_render per request, in
@@ -57,7 +58,7 @@ def march_bytes(samples, cells, rays):
 # rocprofv3 summaries of each workload's dominant kernel; each holds a line
 # "traffic_bytes_per_launch <kernel> <bytes>" measured with FETCH_SIZE /
 # WRITE_SIZE counter runs scaled by a plain copy of known size
-PROFILES = {"c5": "k_trace_static_config5_serving.md", "c2": "k_volume_march_config2_1024x1024x16spp.md",
+PROFILES = {"c5": "k_trace_config5_serving.md", "c2": "k_volume_march_config2_1024x1024x16spp.md",
             "c1": "k_trace_static_config1_512x512x1spp.md", "c3": "k_trace_config3_1920x1080x64spp.md",
             "c4": "k_volume_march_config4_1024x1024x32spp.md"}
 
@@ -93,8 +94,9 @@ def roofline(tot, workload=None):
             byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
     else:
         # queue traversal: one ray per thread, or pooled for the secondary rays
-        # of large scenes (config 3); the HIP events bracket whichever ran
-        kernel = "k_trace_static+k_trace_pool" if workload == "c3" else "k_trace_static"
+        # of large scenes (config 3, also resident in the serving stream); the
+        # HIP events bracket whichever ran
+        kernel = "k_trace_static+k_trace_pool" if workload in ("c3", "c5") else "k_trace_static"
         kms, launches = tot["trace_ms"], tot["trace_launches"]
         byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
     achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
@@ -159,7 +161,45 @@ def workload_c1(args):
     return sc, desc
 
 
-def run_gpu(sc, steps, warmup):
+def autotune(dev, world, cam, rend, width, height, spp):
+    """bnx_autotune on probe renders of this request shape, at up to 16M
+    paths per probe (untimed set-up, like the BVH build)"""
+    import barney_amd as bx
+    pspp = min(spp, max(1, (16 << 20) // (width * height)))
+    t0 = time.perf_counter()
+    chosen, rep = bx.autotune(world, cam, rend, width, height, pspp)
+    el = time.perf_counter() - t0
+    log(f"autotune: {el:.1f} s, {rep['probes']} probes at {width}x{height}x{pspp}: "
+        f"{rep['ms_before']:.2f} -> {rep['ms_after']:.2f} ms; {chosen}")
+    return {"tuning": chosen, "autotune": {"seconds": round(el, 2), "probe": f"{width}x{height}, {pspp} spp",
+                                           "probes": rep["probes"], "probe_ms_before": round(rep["ms_before"], 3),
+                                           "probe_ms_after": round(rep["ms_after"], 3)}}
+
+
+def parse_tuning(text):
+    """--tuning "k=v,k=v": launch-tuning fields to set; "default": the
+    device's defaults; "": run the autotuner"""
+    if text in ("", None):
+        return None
+    if text == "default":
+        return {}
+    out = {}
+    for item in text.split(","):
+        k, v = item.split("=")
+        out[k.strip()] = int(v)
+    return out
+
+
+def set_tuning(dev, tuning):
+    """apply a given tuning (no autotuner); None when the autotuner should run"""
+    if tuning is None:
+        return None
+    if tuning:
+        dev.set_tuning(**tuning)
+    return {"tuning": dev.tuning(), "autotune": "not run (tuning given)"}
+
+
+def run_gpu(sc, steps, warmup, tuning=None):
     import numpy as np
 
     import barney_amd as bx
@@ -176,6 +216,7 @@ def run_gpu(sc, steps, warmup):
     log(f"world built in {time.perf_counter() - t0:.2f} s")
     cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
     rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=getattr(sc, "max_generations", 0))
+    tune = set_tuning(dev, tuning) or autotune(dev, w.world, cam, rend, sc.width, sc.height, sc.spp)
     fr = bx.Frame(dev, sc.width, sc.height)
     for _ in range(warmup):
         fr.reset()
@@ -198,7 +239,7 @@ def run_gpu(sc, steps, warmup):
     elapsed = time.perf_counter() - t0
     fr.close()
     dev.close()
-    return elapsed, np.array(lat), tot
+    return elapsed, np.array(lat), tot, tune
 
 
 def run_cpu(sc, budget_s):
@@ -247,17 +288,38 @@ def run_cpu(sc, budget_s):
 
 
 # ---------------------------------------------------------------- serving
-def serve_setup(dev, n, seed=None):
+_C5_SCENES = {}
+
+
+def serve_scenes(args):
+    """the resident scenes of the serving stream: config 1's surface scene,
+    config 2's structured volume and config 3's large triangle scene (built
+    once per process; the CPU sample reuses them)"""
+    from barney_amd import scenes
+    key = (args.terrain_n, args.spheres)
+    if key not in _C5_SCENES:
+        _C5_SCENES.clear()
+        _C5_SCENES[key] = [scenes.config1(), scenes.config2(),
+                           scenes.config3(n=args.terrain_n, num_spheres=args.spheres)]
+    return _C5_SCENES[key]
+
+
+def serve_renderer(sc):
+    from barney_amd import abi
+    return abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=getattr(sc, "max_generations", 0))
+
+
+def serve_setup(dev, args, n, seed=None):
     import barney_amd as bx
-    from barney_amd import abi, scenes
-    c1, c2 = scenes.config1(), scenes.config2()
-    worlds = [(bx.SurfaceWorld(dev, c1), c1), (bx.VolumeWorld(dev, c2), c2)]
-    stream = scenes.serving_stream(n, 1.0, [c1, c2], **({} if seed is None else {"seed": seed}))
+    from barney_amd import scenes
+    c1, c2, c3 = serve_scenes(args)
+    worlds = [(bx.SurfaceWorld(dev, c1), c1), (bx.VolumeWorld(dev, c2), c2), (bx.SurfaceWorld(dev, c3), c3)]
+    stream = scenes.serving_stream(n, 1.0, [c1, c2, c3], **({} if seed is None else {"seed": seed}))
     reqs = []
     for r in stream:
         gw, sc = worlds[r.scene]
         cam = bx.camera_perspective(r.camera["position"], r.camera["direction"], r.camera["up"], r.camera["fovy"])
-        rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient)
+        rend = serve_renderer(sc)
         reqs.append(dict(world=gw.world, cam=cam, rend=rend, w=r.width, h=r.height, spp=r.spp, a1=r.arrival_s,
                          scene=r.scene))
     return worlds, reqs
@@ -343,8 +405,16 @@ def run_c5(args, n, warmup):
 
     import barney_amd as bx
     dev = bx.Device(0)
-    worlds, reqs = serve_setup(dev, n + warmup)
+    worlds, reqs = serve_setup(dev, args, n + warmup)
     log("worlds resident")
+    # one launch tuning for both forms: bnx_autotune on the stream's heaviest
+    # world (config 3) at a 1024^2 request with 16 spp
+    tune = set_tuning(dev, parse_tuning(args.tuning))
+    if tune is None:
+        gw3, c3 = worlds[2]
+        cam3 = bx.camera_perspective(c3.camera["position"], c3.camera["direction"], c3.camera["up"],
+                                     c3.camera["fovy"])
+        tune = autotune(dev, gw3.world, cam3, serve_renderer(c3), 1024, 1024, 16)
     warm, reqs = reqs[:warmup], reqs[warmup:]
     a1 = np.array([r["a1"] for r in reqs]) - reqs[0]["a1"]
     frames = {}
@@ -376,7 +446,7 @@ def run_c5(args, n, warmup):
             f.close()
         dev.close()
         desc = {"workload": f"config 5 serving stream, served pass at saturation only ({n} requests)",
-                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)"}
+                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)", **tune}
         return served_mrays, makespan, tot, desc, {"frame_latency_ms": {"p50": None, "p99": None}}
     # the same stream, Poisson arrivals at a fixed rate: load args.load of
     # the plain form's capacity
@@ -394,14 +464,19 @@ def run_c5(args, n, warmup):
     for f in frames.values():
         f.close()
     res = sorted({r["w"] for r in reqs})
+    c3 = worlds[2][1]
+    ntri = sum(len(m.indices) for m in c3.meshes)
+    nsph = sum(len(x.radii) for x in c3.spheres)
+    per_scene = [sum(1 for r in reqs if r["scene"] == k) for k in range(3)]
     desc = {"workload": f"config 5: serving stream, {n} requests (bounded sample of the 10,000-request stream) "
-                        f"over resident worlds config 1 (20,480 triangles + 1,000 spheres) and config 2 "
-                        f"(256^3 structured volume); resolution in {{256..2048}}^2, spp in {{1,4,16,64}}, "
-                        f"seeded orbit cameras; Poisson arrivals at {qps:.2f} QPS (load {args.load} of the "
-                        f"plain form's capacity)",
+                        f"over resident worlds config 1 (20,480 triangles + 1,000 spheres), config 2 "
+                        f"(256^3 structured volume) and config 3 ({ntri:,} triangles + {nsph:,} spheres, "
+                        f"matte / glossy, HDRI light, 5 generations); resolution in {{256..2048}}^2, spp in "
+                        f"{{1,4,16,64}}, seeded orbit cameras; Poisson arrivals at {qps:.2f} QPS (load "
+                        f"{args.load} of the plain form's capacity)",
             "requests": n, "warmup_requests": warmup, "qps": round(qps, 3), "resolutions": res,
             "form": "served (continuous batching) vs plain (one bnx_render per request)",
-            "scenes": ["config 1", "config 2"], "missing_scenes": ["config 3 (not built yet)"]}
+            "scenes": ["config 1", "config 2", "config 3"], "requests_per_scene": per_scene, **tune}
     extra = {
         "frame_latency_ms": {"p50": pct(lat_served, 50), "p99": pct(lat_served, 99)},
         "served": {"mrays_s_saturated": round(served_mrays, 3), "makespan_s": round(makespan, 4),
@@ -422,28 +497,37 @@ def run_c5(args, n, warmup):
 def run_cpu_c5(args, budget_s):
     """the oracle on the smallest requests of the same stream"""
     import oracle
-    from barney_amd import abi, scenes
+    from barney_amd import scenes
     if not os.path.exists(oracle.LIB_PATH):
         oracle.build()
     threads = oracle.default_threads()
-    c1, c2 = scenes.config1(), scenes.config2()
-    oscenes = [oracle.surface_scene(c1), oracle.volume_scene(c2)]
-    stream = scenes.serving_stream(2000, 1.0, [c1, c2])
+    c1, c2, c3 = serve_scenes(args)
+    built = {}
+
+    def oscene(k):
+        # the oracle's scene (and BVH) of world k, built on first use, outside
+        # the timed sample
+        if k not in built:
+            built[k] = (oracle.volume_scene if k == 1 else oracle.surface_scene)((c1, c2, c3)[k])
+        return built[k]
+    stream = scenes.serving_stream(2000, 1.0, [c1, c2, c3])
     small = [r for r in stream if r.width == 256 and r.spp == 1]
+    for k in sorted({r.scene for r in small}):
+        oscene(k)
     rays, frames, t0 = 0, 0, time.perf_counter()
     for r in small:
-        sc = (c1, c2)[r.scene]
+        sc = (c1, c2, c3)[r.scene]
+        os_ = oscene(r.scene)
         cam = oracle.camera_perspective(r.camera["position"], r.camera["direction"], r.camera["up"], r.camera["fovy"])
         fr = oracle.Frame(r.width, r.height)
-        _, st = oracle.render(oscenes[r.scene], cam, abi.renderer(bg=sc.bg_color, ambient=sc.ambient), fr, r.spp,
-                              threads=threads)
+        _, st = oracle.render(os_, cam, serve_renderer(sc), fr, r.spp, threads=threads)
         rays += st["rays"]
         frames += 1
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
     return {"value": round(rays / el / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"the first {frames} 256^2 1-spp requests of the same stream (both worlds), {el:.1f} s wall "
+            "sample": f"the first {frames} 256^2 1-spp requests of the same stream (all three worlds), {el:.1f} s wall "
                       f"on {threads} threads"}
 
 
@@ -477,6 +561,9 @@ def main():
                     help="c5 only: run every form, or only the served pass at saturation (for profiling)")
     ap.add_argument("--cpu-seconds", type=float, default=8.0)
     ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--tuning", default="",
+                    help='launch tuning instead of running bnx_autotune at set-up: "default" (the device\'s '
+                         'defaults) or "field=value,..." (bnx_tuning fields)')
     ap.add_argument("--copy-probe", type=int, default=0, metavar="MIB",
                     help="only run a plain device copy of MIB MiB (counter calibration)")
     args = ap.parse_args()
@@ -499,7 +586,8 @@ def main():
         warmup = args.warmup if args.warmup is not None else (1 if heavy else 3)
         sc, desc = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3,
                     "c4": workload_c4}[args.workload](args)
-        elapsed, lat, tot = run_gpu(sc, steps, warmup)
+        elapsed, lat, tot, tune = run_gpu(sc, steps, warmup, parse_tuning(args.tuning))
+        desc.update(tune)
         mrays = tot["rays"] / elapsed / 1e6
         lat_line = {"p50": pct(lat, 50), "p99": pct(lat, 99)}
         extra = {"rays_per_step": tot["rays"] // steps}

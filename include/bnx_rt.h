@@ -261,6 +261,56 @@ int bnx_device_sync(bnx_device dev);
 /* free / total device memory in bytes (used by memory-based admission) */
 int bnx_device_mem_info(bnx_device dev, size_t *free_bytes, size_t *total_bytes);
 
+/* ------------------------------------------------------------- tuning
+ * Launch parameters of the render path, per device.  None of them changes a
+ * result: they set the order in which primary rays enter the queue, how the
+ * traversal stack is split between LDS and private memory, the size of the
+ * pooled trace launch and how many paths bnx_render keeps in flight.  No
+ * counterpart in the reference, whose launch shapes are fixed (one thread
+ * per ray in blocks of 256, kernels/traceRays.cu:20-121; all of a frame's
+ * pixels per generation, native/Context.cpp:139-182); bnx_autotune chooses
+ * them per scene and request shape. */
+typedef struct bnx_tuning {
+  int32_t raygen_tile_w;            /* primary rays in tiles of w x h pixels, each   */
+  int32_t raygen_tile_h;            /* tile row-major; 0 x 0: the image row-major
+                                       (w, h: both 0, or both in 1..256)             */
+  int32_t trace_stack_lds;          /* traversal-stack entries per lane kept in LDS:
+                                       8, 16 or 32 (of 192; the rest in private
+                                       memory)                                       */
+  int32_t trace_blocks_per_cu;      /* pooled trace launch: blocks per CU, 1..64     */
+  int32_t trace_refill;             /* pooled trace: idle lanes of a wave that make
+                                       it draw new rays, 1..64                       */
+  int32_t _pad;
+  int64_t trace_refill_prims;       /* secondary generations of worlds with at least
+                                       this many primitives use the pooled trace     */
+  int64_t render_paths;             /* bnx_render: paths per batch of samples,
+                                       2^16 .. 2^26                                  */
+} bnx_tuning;
+/* a new device starts with 8 x 8 tiles, 16 LDS entries, 8 blocks per CU,
+ * refill 16 for worlds of 2^20 primitives or more, and 2^23 paths per batch */
+
+int bnx_device_get_tuning(bnx_device dev, bnx_tuning *out);
+/* BNX_ERR_INVALID (and no change) for a field out of range */
+int bnx_device_set_tuning(bnx_device dev, const bnx_tuning *tuning);
+
+typedef struct bnx_autotune_report {
+  double  ms_before;                /* probe render time with the tuning in place    */
+  double  ms_after;                 /* probe render time with the tuning chosen      */
+  int32_t probes;                   /* probe renders timed                           */
+  int32_t _pad;
+} bnx_autotune_report;
+
+/* Choose the device's tuning for requests like this one: coordinate descent,
+ * two passes, over the fields of bnx_tuning (raygen tile, LDS stack entries,
+ * refill threshold, pooled-launch size, paths per batch), each candidate timed on
+ * probe renders of width x height at probe_spp samples into a scratch frame
+ * (best of two; a candidate must be 1% faster to replace the current one).
+ * Sets the tuning chosen on the device (the one in place if nothing was
+ * faster) and returns it with the timings.  No counterpart in the
+ * reference. */
+int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *renderer, int width, int height,
+                 int probe_spp, bnx_tuning *chosen, bnx_autotune_report *report);
+
 /* ------------------------------------------------------------- buffers */
 /* rtc::Device::createBuffer / rtc::Buffer::upload / freeBuffer */
 int   bnx_buffer_create(bnx_device dev, size_t bytes, const void *init, bnx_buffer *out);
@@ -393,7 +443,8 @@ int bnx_render(bnx_world world, const bnx_camera *camera, const bnx_renderer *re
  * resident and runs ONE wavefront for all requests in flight: every
  * generation traces, marches and shades the rays of all admitted requests in
  * one launch per kernel, and tops the shared queue up with primary rays of
- * admitted requests (shortest remaining work first).  Requests are admitted
+ * admitted requests (shortest remaining work first, or earliest deadline
+ * first, see bnx_server_config).  Requests are admitted
  * in arrival order as request slots and frame memory free up.  A request's
  * result is the same as bnx_render of the same world, camera, renderer and
  * spp into a cleared frame (up to the order of fixed-point additions, which
@@ -403,10 +454,17 @@ typedef struct bnx_server_s *bnx_server;
 
 typedef struct bnx_server_config {
   size_t  frame_memory_budget;      /* bytes of request frames held at once (0: 16 GiB) */
-  int64_t max_paths_in_flight;      /* paths in the shared queue (0: 4M); the queue
-                                       holds twice as many rays                        */
+  int64_t max_paths_in_flight;      /* paths in the shared queue (0: the device's
+                                       bnx_tuning.render_paths at creation); the
+                                       queue holds three times as many rays           */
   int32_t max_active;               /* requests holding a slot at once (0: 256, <= 4096) */
-  int32_t _pad;
+  float   deadline_slack;           /* order of the primary top-up: s > 0 =
+                                       earliest deadline first, a request's
+                                       deadline being its submit time + s x its
+                                       estimated time alone (its world's measured
+                                       cost per path, from completed requests, x
+                                       its paths); 0: s = 2; s < 0: shortest
+                                       remaining work first                      */
 } bnx_server_config;
 
 typedef struct bnx_request {
@@ -479,6 +537,8 @@ static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
 static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
 static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");
+static_assert(sizeof(bnx_tuning) == 40, "bnx_tuning layout");
+static_assert(sizeof(bnx_autotune_report) == 24, "bnx_autotune_report layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
 static_assert(sizeof(bnx_render_stats) == 128, "bnx_render_stats layout");
 #endif

@@ -5,9 +5,11 @@ bnx_render into a cleared frame: the per-pixel RNG streams depend only on
 (pixel, sample), the fixed-point accumulation does not depend on the order
 in which rays of different requests arrive, and the primary-hit channels are
 written by sample 0's primary ray only.  So the comparison is bit-exact, for
-every channel, while requests of different worlds, resolutions and spp share
-the queue, arrive while others are in flight, wait for request slots and
-frame memory, and are split into partial-sample runs of primary rays."""
+every channel, while requests of different worlds (matte surfaces, a
+structured volume, PBR surfaces under an HDRI map), resolutions and spp share
+the queue under either order of the primary top-up, arrive while others are
+in flight, wait for request slots and frame memory, and are split into
+partial-sample runs of primary rays."""
 import numpy as np
 import pytest
 
@@ -24,9 +26,11 @@ CHANNELS = (abi.BNX_FB_COLOR, abi.BNX_FB_DEPTH, abi.BNX_FB_PRIMID, abi.BNX_FB_IN
 def worlds(dev):
     c1 = scenes.config1(width=64, height=64)
     c2 = scenes.config2(n=48, width=64, height=64, spp=1, mc_grid=8)
+    c3 = scenes.config3(n=64, num_spheres=400, width=64, height=64, spp=1, env_size=(64, 32))
     sw = bx.SurfaceWorld(dev, c1)
     vw = bx.VolumeWorld(dev, c2)
-    return {"c1": (sw, c1), "c2": (vw, c2)}
+    tw = bx.SurfaceWorld(dev, c3)
+    return {"c1": (sw, c1), "c2": (vw, c2), "c3": (tw, c3)}
 
 
 def _orbit_camera(sc, k):
@@ -48,11 +52,11 @@ def _mix(worlds):
     sizes = [(64, 64), (48, 32), (96, 80), (17, 23)]
     spps = [1, 4, 2, 7]
     for k in range(12):
-        name = "c1" if k % 3 else "c2"
+        name = ("c2", "c1", "c3")[k % 3]
         gw, sc = worlds[name]
         cam = _orbit_camera(sc, k) if k % 4 else bx.camera_perspective(
             sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
-        mg = 2 if k == 5 else 0
+        mg = 2 if k == 5 else getattr(sc, "max_generations", 0)
         rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=mg)
         w, h = sizes[k % len(sizes)]
         reqs.append((gw.world, cam, rend, w, h, spps[k % len(spps)]))
@@ -65,6 +69,7 @@ def _check_same(got, ref, tag):
 
 
 @pytest.mark.parametrize("cfg", [dict(), dict(max_paths_in_flight=3000, max_active=3),
+                                 dict(max_paths_in_flight=3000, deadline_slack=-1.0),
                                  dict(max_paths_in_flight=20000, frame_memory_budget=80 * 96 * 80 + 80 * 64 * 64)])
 def test_served_equals_alone(dev, worlds, cfg):
     reqs = _mix(worlds)
