@@ -372,6 +372,9 @@ int bnx_device_create(int gpu_id, bnx_device *out)
   hipDeviceProp_t prop;
   BNX_HIP(hipGetDeviceProperties(&prop, gpu_id));
   d->num_cus = prop.multiProcessorCount > 0 ? prop.multiProcessorCount : 256;
+  int khz = 0;
+  if (hipDeviceGetAttribute(&khz, hipDeviceAttributeWallClockRate, gpu_id) == hipSuccess && khz > 0)
+    d->wall_clock_khz = khz;
   // tuning overrides of the trace launch (pool refill threshold, blocks per CU)
   if (const char *e = getenv("BNX_TRACE_REFILL")) d->trace_refill = std::min(64, std::max(1, atoi(e)));
   if (const char *e = getenv("BNX_TRACE_REFILL_PRIMS")) d->trace_refill_prims = std::max(0LL, atoll(e));

@@ -24,7 +24,9 @@
 #include <chrono>
 #include <deque>
 #include <map>
+#include <tuple>
 #include <unordered_map>
+#include <vector>
 
 namespace bnx {
 namespace {
@@ -50,12 +52,75 @@ struct Pinned {
   template <typename T> T *as() const { return (T *)p; }
 };
 
+// device-driven generations of one batch at most (a captured bounce loop)
+constexpr int MAX_DD_GENS = 16;
+
+// what the launches of a device-driven batch depend on, besides the
+// wavefront's buffers
+struct BatchShape {
+  long long count0 = 0, cap = 0;
+  bool surfaces = false, volumes = false, full = false;
+  int refill = 64, stack = 16, blocks_per_cu = 8;
+  bool operator<(const BatchShape &o) const
+  {
+    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, blocks_per_cu) <
+           std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.blocks_per_cu);
+  }
+};
+
+// a captured bounce loop: G generations of one batch shape
+struct BatchGraph {
+  BatchShape shape;
+  int G = 0, cur0 = 0;
+  hipGraph_t graph = nullptr;
+  hipGraphExec_t exec = nullptr;
+};
+
 // queues, tables and staging of one wavefront
 struct Wavefront {
   Queues Q;
   DevMem slots, chunks, segs, blk256, blk1024, blkT, pool, out_base, out_cnt;
   Pinned h_slots, h_chunks, h_segs, h_out_base, h_out_cnt;
   int slot_cap = 0, chunk_cap = 0, seg_cap = 0;
+  // device-driven bounce loops: per-generation counts, timing events (four
+  // per generation), generations observed per batch shape, captured graphs
+  // (valid for the buffer addresses they were captured with)
+  DevMem gen_counts, stamps;
+  Pinned h_gen_counts;
+  std::map<BatchShape, int> gens_seen;
+  std::vector<BatchGraph> graphs;
+  std::vector<const void *> graph_bufs;
+  bool graphs_off = false;      // capture failed once: enqueue directly
+  ~Wavefront() { drop_graphs(); }
+  void drop_graphs()
+  {
+    for (BatchGraph &g : graphs) {
+      if (g.exec) (void)hipGraphExecDestroy(g.exec);
+      if (g.graph) (void)hipGraphDestroy(g.graph);
+    }
+    graphs.clear();
+  }
+  // the addresses captured graphs refer to; a reallocation drops the graphs
+  void check_graph_bufs()
+  {
+    std::vector<const void *> b = {Q.org[0].ptr, Q.org[1].ptr, Q.dir[0].ptr, Q.dir[1].ptr, Q.rng[0].ptr,
+                                   Q.rng[1].ptr, Q.ps[0].ptr, Q.ps[1].ptr, Q.hit[0].ptr, Q.hit[1].ptr,
+                                   Q.hit_v[0].ptr, Q.hit_v[1].ptr, Q.vcol[0].ptr, Q.vcol[1].ptr, Q.stats.ptr,
+                                   slots.ptr, segs.ptr, blk256.ptr, blk1024.ptr, blkT.ptr, pool.ptr,
+                                   out_base.ptr, out_cnt.ptr, gen_counts.ptr, stamps.ptr};
+    if (b != graph_bufs) {
+      drop_graphs();
+      graph_bufs = b;
+    }
+  }
+  // host staging: G counts, 4 G stamps, then the two output counters
+  int ensure_dd()
+  {
+    BNX_HIP(gen_counts.ensure(sizeof(long long) * MAX_DD_GENS));
+    BNX_HIP(stamps.ensure(sizeof(unsigned long long) * 4 * MAX_DD_GENS));
+    BNX_HIP(h_gen_counts.ensure(sizeof(long long) * 5 * MAX_DD_GENS + sizeof(int) * 2));
+    return BNX_OK;
+  }
   size_t blk_cap = 0;
 
   // rays per queue, request slots, raygen chunks, segments per generation
@@ -167,6 +232,96 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool sur
   }
   const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
   return launch_shade(dev, slots, sv, sl.blocks1024, in, out, so, st);
+}
+
+// Enqueue G device-driven generations of one request's batch (slot 0, the
+// primary rays in queue cur0, count0 of them): each generation's segment
+// and counter reset come from k_gen_setup, and every launch is sized for
+// the generation's largest possible count (count0, then cap), the blocks
+// past the actual count finding no ray.  No host read-back in between, so
+// the sequence can be captured as a graph.  Launch timing: wall-clock
+// stamps[4 g ..] before and after the trace and the march launches.
+int enqueue_dd(Device *dev, Wavefront &F, int cur0, int G, const BatchShape &b)
+{
+  Queues &Q = F.Q;
+  unsigned long long *st = Q.stats.as<unsigned long long>();
+  const SlotDev *slots = F.slots.as<SlotDev>();
+  const SegView sv{F.segs.as<Segment>(), F.blk256.as<int>(), F.blk1024.as<int>(), F.blkT.as<int>(),
+                   F.pool.as<int>()};
+  const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
+  const bool trace = b.surfaces || !b.volumes;
+  const long long target = (long long)dev->trace_blocks_per_cu * dev->num_cus;
+  int rc;
+  for (int g = 0; g < G; g++) {
+    const int cur = (cur0 + g) & 1;
+    // primary rays one per thread (whole-wave refills), as in the host loop
+    const int refill = g == 0 ? 64 : b.refill;
+    const bool pooled = refill < 64;
+    const long long c = g == 0 ? b.count0 : b.cap;
+    const int blocks256 = (int)((c + 255) / 256), blocks1024 = (int)((c + 1023) / 1024);
+    int blocksT = 0;
+    if (pooled) {
+      const long long per = 256LL * std::max(1LL, (c + 256LL * target - 1) / (256LL * target));
+      blocksT = (int)((c + per - 1) / per);
+    }
+    if ((rc = launch_gen_setup(dev, F.segs.as<Segment>(), F.out_cnt.as<int>(), F.gen_counts.as<long long>(), g,
+                               (int)b.count0, refill)) != BNX_OK)
+      return rc;
+    if ((rc = launch_segment_blocks(dev, sv, 1, blocks256, blocks1024, blocksT)) != BNX_OK) return rc;
+    RayQueue in = Q.q(cur), out = Q.q(1 - cur);
+    TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
+    unsigned long long *ts = F.stamps.as<unsigned long long>() + 4 * g;
+    if (trace) {
+      if ((rc = launch_stamp(dev, ts)) != BNX_OK) return rc;
+      if ((rc = launch_trace_queue(dev, slots, sv, blocks256, blocksT, pooled, tq, st)) != BNX_OK) return rc;
+      if ((rc = launch_stamp(dev, ts + 1)) != BNX_OK) return rc;
+    }
+    if (b.volumes) {
+      if ((rc = launch_stamp(dev, ts + 2)) != BNX_OK) return rc;
+      if ((rc = launch_volume_march(dev, slots, sv, blocks256, in, trace, b.full, st)) != BNX_OK) return rc;
+      if ((rc = launch_stamp(dev, ts + 3)) != BNX_OK) return rc;
+    }
+    if ((rc = launch_shade(dev, slots, sv, blocks1024, in, out, so, st)) != BNX_OK) return rc;
+  }
+  return BNX_OK;
+}
+
+// the captured bounce loop of shape b with G generations starting in queue
+// cur0, captured on first use; nullptr when capture is not available
+int batch_graph(Device *dev, Wavefront &F, const BatchShape &b, int G, int cur0, hipGraphExec_t &exec)
+{
+  exec = nullptr;
+  if (F.graphs_off) return BNX_OK;
+  F.check_graph_bufs();
+  for (BatchGraph &g : F.graphs)
+    if (!(g.shape < b) && !(b < g.shape) && g.G == G && g.cur0 == cur0) {
+      exec = g.exec;
+      return BNX_OK;
+    }
+  if (F.graphs.size() >= 32) F.drop_graphs();
+  BatchGraph bg;
+  bg.shape = b;
+  bg.G = G;
+  bg.cur0 = cur0;
+  if (hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+    (void)hipGetLastError();
+    F.graphs_off = true;
+    return BNX_OK;
+  }
+  const int rc = enqueue_dd(dev, F, cur0, G, b);
+  hipGraph_t graph = nullptr;
+  const hipError_t e = hipStreamEndCapture(dev->stream, &graph);
+  if (rc != BNX_OK || e != hipSuccess || !graph ||
+      hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+    if (graph) (void)hipGraphDestroy(graph);
+    (void)hipGetLastError();
+    F.graphs_off = true;
+    return BNX_OK;
+  }
+  bg.graph = graph;
+  F.graphs.push_back(bg);
+  exec = bg.exec;
+  return BNX_OK;
 }
 
 // after the stream has synchronized
@@ -287,7 +442,58 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     if ((rc = launch_raygen(dev, F.chunks.as<RaygenChunk>(), 1, F.slots.as<SlotDev>(), 0, count, F.Q.q(cur))) !=
         BNX_OK)
       return rc;
-    for (bool primary = true; count > 0; primary = false) {
+    bool primary = true;
+    // batches of up to graph_paths paths: once the generations a batch of
+    // this shape takes are known (from a first batch through the host loop
+    // below), run that many as one captured bounce loop with one read-back
+    const bool dd = dev->graph_paths > 0 && np * nS <= dev->graph_paths;
+    const BatchShape shape{np * nS, 2 * np * nS, surfaces, volumes, full, trace_refill_for(dev, w),
+                           dev->trace_stack_lds, dev->trace_blocks_per_cu};
+    int dd_gens = 0;
+    if (dd) {
+      auto it = F.gens_seen.find(shape);
+      if (it != F.gens_seen.end()) {
+        const int G = it->second;
+        if ((rc = F.ensure_dd()) != BNX_OK) return rc;
+        hipGraphExec_t exec = nullptr;
+        if ((rc = batch_graph(dev, F, shape, G, cur, exec)) != BNX_OK) return rc;
+        if (exec) {
+          BNX_HIP(hipGraphLaunch(exec, dev->stream));
+        } else if ((rc = enqueue_dd(dev, F, cur, G, shape)) != BNX_OK) {
+          return rc;
+        }
+        long long *hg = F.h_gen_counts.as<long long>();
+        unsigned long long *hts = (unsigned long long *)(hg + MAX_DD_GENS);
+        int *hc2 = (int *)(hg + 5 * MAX_DD_GENS);
+        BNX_HIP(hipMemcpyAsync(hg, F.gen_counts.ptr, sizeof(long long) * G, hipMemcpyDeviceToHost, dev->stream));
+        BNX_HIP(hipMemcpyAsync(hts, F.stamps.ptr, sizeof(unsigned long long) * 4 * G, hipMemcpyDeviceToHost,
+                               dev->stream));
+        BNX_HIP(hipMemcpyAsync(hc2, F.out_cnt.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, dev->stream));
+        BNX_HIP(hipStreamSynchronize(dev->stream));
+        const double tick_ms = 1.0 / (double)dev->wall_clock_khz;
+        for (int g = 0; g < G; g++) {
+          if (traced) {
+            tm.trace_ms += (double)(hts[4 * g + 1] - hts[4 * g]) * tick_ms;
+            tm.trace_launches++;
+          }
+          if (volumes) {
+            tm.march_ms += (double)(hts[4 * g + 3] - hts[4 * g + 2]) * tick_ms;
+            tm.march_launches++;
+            tm.march_rays += hg[g];
+          }
+          rays += hg[g];
+          gens += hg[g] > 0;
+        }
+        count = hc2[0];
+        cur = (cur + G) & 1;
+        primary = false;
+        dd_gens = G;
+      }
+    }
+    // the host loop: one read-back per generation (and the generations a
+    // captured loop left over)
+    int host_gens = 0;
+    for (; count > 0; primary = false) {
       SegList sl{F.h_segs.as<Segment>()};
       sl.add(0, 0, count, primary ? 64 : trace_refill_for(dev, w));
       BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, 2 * sizeof(int), dev->stream));
@@ -297,9 +503,11 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       if ((rc = collect_timing(F, count, traced, volumes, tm)) != BNX_OK) return rc;
       rays += count;
       gens++;
+      host_gens++;
       count = hcnt[0];
       cur = 1 - cur;
     }
+    if (dd && host_gens > 0) F.gens_seen[shape] = std::min(MAX_DD_GENS, dd_gens + host_gens);
     f->accum_id += nS;
   }
   unsigned long long host_st[STAT_COUNT];

@@ -80,6 +80,7 @@ struct Device {
   int gpu = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  int wall_clock_khz = 100000;   // rate of the device wall clock (launch timing in graphs)
   // launch tuning (bnx_device_set_tuning, bnx_autotune); none of it changes results
   int raygen_tile_w = 8, raygen_tile_h = 8;   // primary-ray pixel order: tiles (0: row-major)
   int trace_stack_lds = 16;  // LDS stack entries per lane in the queue-trace kernels (8, 16, 32)
@@ -87,6 +88,7 @@ struct Device {
   long long trace_refill_prims = 1 << 20;   // "large": at least this many primitives
   int trace_blocks_per_cu = 8;   // trace launch size: blocks per CU
   long long render_paths = 8LL << 20;       // bnx_render: paths per batch
+  long long graph_paths = 4LL << 20;        // bnx_render: batches this small run a captured bounce loop
   BuildScratch build;
   DevMem counters;      // small device counters (trace work queue etc.)
   void *queues = nullptr;   // render queues (bnx_render.h Queues)

@@ -164,6 +164,13 @@ int frame_read(Frame *f, int channel, void *host);
 // primary rays for total queue entries starting at queue index q_begin
 int launch_raygen(Device *dev, const RaygenChunk *d_chunks, int num_chunks, const SlotDev *d_slots,
                   long long q_begin, long long total, const RayQueue &q);
+// a device-driven generation's single segment from the previous shade's
+// output counter (count0 when g == 0); clears the counter, keeps the count
+// in gen_counts[g]
+int launch_gen_setup(Device *dev, Segment *segs, int *out_cnt, long long *gen_counts, int g, int count0, int refill);
+// write the device wall clock (ticks of wall_clock_khz) to *t once the
+// stream reaches this point
+int launch_stamp(Device *dev, unsigned long long *t);
 // fill the block -> segment maps of nsegs segments (blocks256 / blocks1024 /
 // blocksT blocks in all) and reset the segments' trace pools
 int launch_segment_blocks(Device *dev, const SegView &sv, int nsegs, int blocks256, int blocks1024, int blocksT);

@@ -677,6 +677,28 @@ __global__ void k_segment_blocks(SegView sv, int nsegs, int blocks256, int block
   }
 }
 
+// One generation's set-up for a device-driven bounce loop (one request,
+// one segment, no read-back between generations): the segment takes its
+// count from the previous shade's output counter (count0 for the primary
+// generation), the counter is cleared for this generation's shade, and the
+// count is kept in gen_counts[g] for the host.
+__global__ void k_gen_setup(Segment *segs, int *out_cnt, long long *gen_counts, int g, int count0, int refill)
+{
+  if (threadIdx.x != 0) return;
+  const int count = g == 0 ? count0 : out_cnt[0];
+  segs[0] = Segment{0, 0, count, 0, 0, 0, refill, 0};
+  out_cnt[0] = 0;
+  out_cnt[1] = 0;
+  gen_counts[g] = count;
+}
+
+// the device's constant-rate wall clock, for timing launches inside a
+// captured graph (where event timestamps are not available)
+__global__ void k_stamp(unsigned long long *t)
+{
+  if (threadIdx.x == 0) *t = wall_clock64();
+}
+
 // ----------------------------------------------------------- frame ops
 __global__ void k_frame_clear(FrameDev fb, int np)
 {
@@ -782,6 +804,20 @@ int launch_shade(Device *dev, const SlotDev *slots, const SegView &sv, int block
                  const RayQueue &out, const SlotOut &so, unsigned long long *stats)
 {
   if (blocks1024 > 0) k_shade<<<blocks1024, SHADE_BLOCK, 0, dev->stream>>>(slots, sv, in, out, so, stats);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_gen_setup(Device *dev, Segment *segs, int *out_cnt, long long *gen_counts, int g, int count0, int refill)
+{
+  k_gen_setup<<<1, 64, 0, dev->stream>>>(segs, out_cnt, gen_counts, g, count0, refill);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_stamp(Device *dev, unsigned long long *t)
+{
+  k_stamp<<<1, 64, 0, dev->stream>>>(t);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

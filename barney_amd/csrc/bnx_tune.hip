@@ -1,7 +1,8 @@
 // bnx_tune.hip -- launch tuning of the render path (include/bnx_rt.h,
 // bnx_tuning) and its autotuner: coordinate descent over the raygen tile,
 // the LDS share of the traversal stack, the pooled trace's refill threshold
-// and launch size, and bnx_render's paths per batch, each candidate timed on
+// and launch size, bnx_render's paths per batch and the largest batch run as
+// a captured bounce loop, each candidate timed on
 // probe renders of the caller's request shape.  Host code only.
 #include "bnx_internal.h"
 #include "bnx_render.h"
@@ -26,6 +27,7 @@ bnx_tuning tuning_of(const Device *d)
   t.trace_refill = d->trace_refill;
   t.trace_refill_prims = d->trace_refill_prims;
   t.render_paths = d->render_paths;
+  t.graph_paths = d->graph_paths;
   return t;
 }
 
@@ -40,6 +42,7 @@ const char *tuning_error(const bnx_tuning &t)
   if (t.trace_refill < 1 || t.trace_refill > 64) return "trace_refill: 1..64";
   if (t.trace_refill_prims < 0) return "trace_refill_prims: >= 0";
   if (t.render_paths < (1LL << 16) || t.render_paths > (1LL << 26)) return "render_paths: 2^16 .. 2^26";
+  if (t.graph_paths < 0 || t.graph_paths > (1LL << 26)) return "graph_paths: 0 .. 2^26";
   return nullptr;
 }
 
@@ -52,6 +55,7 @@ void apply(Device *d, const bnx_tuning &t)
   d->trace_refill = t.trace_refill;
   d->trace_refill_prims = t.trace_refill_prims;
   d->render_paths = t.render_paths;
+  d->graph_paths = t.graph_paths;
 }
 
 // one probe: the request rendered into the cleared scratch frame; wall time
@@ -170,6 +174,7 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
   const std::vector<int> refills = {64, 32, 16, 8, 4};
   const std::vector<int> blocks = {2, 4, 8, 16};
   const std::vector<long long> paths = {2LL << 20, 4LL << 20, 8LL << 20, 16LL << 20};
+  const std::vector<long long> graphs = {0LL, 1LL << 20, 4LL << 20, 16LL << 20};
   const long long prims = w->instances->blas_prims;
   // two passes: a field's best can move once the fields after it changed
   for (int pass = 0; pass < 2 && rc == BNX_OK; pass++) {
@@ -194,6 +199,7 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
     if (rc == BNX_OK && pooled) rc = sweep([](bnx_tuning &t, int v) { t.trace_blocks_per_cu = v; }, blocks);
     if (rc == BNX_OK && (long long)np * probe_spp > (1LL << 20))
       rc = sweep([](bnx_tuning &t, long long v) { t.render_paths = v; }, paths);
+    if (rc == BNX_OK) rc = sweep([](bnx_tuning &t, long long v) { t.graph_paths = v; }, graphs);
   }
   if (rc != BNX_OK) {
     apply(dev, start);

@@ -285,9 +285,14 @@ typedef struct bnx_tuning {
                                        this many primitives use the pooled trace     */
   int64_t render_paths;             /* bnx_render: paths per batch of samples,
                                        2^16 .. 2^26                                  */
+  int64_t graph_paths;              /* bnx_render: a batch of at most this many
+                                       paths runs its bounce loop as a captured
+                                       hipGraph, one read-back per batch instead
+                                       of one per generation (0: never; <= 2^26)    */
 } bnx_tuning;
 /* a new device starts with 8 x 8 tiles, 16 LDS entries, 8 blocks per CU,
- * refill 16 for worlds of 2^20 primitives or more, and 2^23 paths per batch */
+ * refill 16 for worlds of 2^20 primitives or more, 2^23 paths per batch and
+ * captured bounce loops for batches of up to 2^22 paths */
 
 int bnx_device_get_tuning(bnx_device dev, bnx_tuning *out);
 /* BNX_ERR_INVALID (and no change) for a field out of range */
@@ -302,7 +307,8 @@ typedef struct bnx_autotune_report {
 
 /* Choose the device's tuning for requests like this one: coordinate descent,
  * two passes, over the fields of bnx_tuning (raygen tile, LDS stack entries,
- * refill threshold, pooled-launch size, paths per batch), each candidate timed on
+ * refill threshold, pooled-launch size, paths per batch, captured-loop
+ * limit), each candidate timed on
  * probe renders of width x height at probe_spp samples into a scratch frame
  * (best of two; a candidate must be 1% faster to replace the current one).
  * Sets the tuning chosen on the device (the one in place if nothing was
@@ -537,7 +543,7 @@ static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
 static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
 static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");
-static_assert(sizeof(bnx_tuning) == 40, "bnx_tuning layout");
+static_assert(sizeof(bnx_tuning) == 48, "bnx_tuning layout");
 static_assert(sizeof(bnx_autotune_report) == 24, "bnx_autotune_report layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
 static_assert(sizeof(bnx_render_stats) == 128, "bnx_render_stats layout");

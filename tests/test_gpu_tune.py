@@ -64,6 +64,8 @@ TUNINGS = [
     dict(trace_refill=4, trace_refill_prims=0, trace_blocks_per_cu=1),
     dict(trace_refill=64, trace_refill_prims=0),
     dict(render_paths=1 << 16, raygen_tile_w=16, raygen_tile_h=4),
+    dict(graph_paths=0),
+    dict(graph_paths=1 << 26, render_paths=1 << 16),
 ]
 
 
@@ -125,3 +127,48 @@ def test_autotune_sets_valid_tuning_and_keeps_results(dev, worlds, keep_tuning):
     with pytest.raises(bx.BnxError):
         bx.autotune(gw.world, cam, rend, 0, 64, probe_spp=1)
     assert dev.tuning() == chosen
+
+
+def _render_mg(dev, gw, sc, w, h, spp, mg):
+    cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
+    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=mg)
+    fr = bx.Frame(dev, w, h)
+    bx.render(gw.world, cam, rend, fr, spp)
+    out = {ch: fr.read(ch) for ch in CHANNELS}
+    fr.close()
+    return out
+
+
+@pytest.mark.parametrize("name", ["c1", "c3", "c2"])
+def test_captured_bounce_loop_replays(dev, worlds, keep_tuning, name):
+    """A batch shape's first render runs the host loop (one read-back per
+    generation) and records its generation count; later renders of that
+    shape run the generations as a captured graph (captured on the second
+    render, replayed after).  Every replay, with eager renders of other
+    shapes in between, must equal the host loop's frame bit for bit."""
+    gw, sc = worlds[name]
+    w, h, spp = 53, 41, 3
+    dev.set_tuning(graph_paths=0)
+    ref = _render(dev, gw, sc, w, h, spp)
+    other = _render(dev, gw, sc, 29, 31, 2)
+    dev.set_tuning(graph_paths=1 << 26)
+    for k in range(4):
+        _same(_render(dev, gw, sc, w, h, spp), ref, (name, "replay", k))
+        _same(_render(dev, gw, sc, 29, 31, 2), other, (name, "other", k))
+
+
+def test_captured_loop_with_too_few_generations(dev, worlds, keep_tuning):
+    """The generation count recorded for a batch shape is only a hint: a
+    request whose paths live longer than the captured loop's generations
+    finishes them through the host loop, and one whose paths end sooner
+    runs empty generations."""
+    gw, sc = worlds["c3"]
+    w, h, spp = 47, 33, 2
+    dev.set_tuning(graph_paths=0)
+    ref_full = _render_mg(dev, gw, sc, w, h, spp, 0)
+    ref_two = _render_mg(dev, gw, sc, w, h, spp, 2)
+    dev.set_tuning(graph_paths=1 << 26)
+    _same(_render_mg(dev, gw, sc, w, h, spp, 2), ref_two, "records 2 generations")
+    _same(_render_mg(dev, gw, sc, w, h, spp, 0), ref_full, "graph of 2, rest on the host loop")
+    _same(_render_mg(dev, gw, sc, w, h, spp, 0), ref_full, "graph of the full count")
+    _same(_render_mg(dev, gw, sc, w, h, spp, 2), ref_two, "graph longer than the paths")
