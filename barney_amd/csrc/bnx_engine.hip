@@ -507,7 +507,10 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       count = hcnt[0];
       cur = 1 - cur;
     }
-    if (dd && host_gens > 0) F.gens_seen[shape] = std::min(MAX_DD_GENS, dd_gens + host_gens);
+    if (dd && host_gens > 0) {
+      if (F.gens_seen.size() >= 256) F.gens_seen.clear();
+      F.gens_seen[shape] = std::min(MAX_DD_GENS, dd_gens + host_gens);
+    }
     f->accum_id += nS;
   }
   unsigned long long host_st[STAT_COUNT];
