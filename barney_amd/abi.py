@@ -130,7 +130,7 @@ class ServerConfig(C.Structure):
 
 class Tuning(C.Structure):
     _fields_ = [("raygen_tile_w", C.c_int32), ("raygen_tile_h", C.c_int32), ("trace_stack_lds", C.c_int32),
-                ("trace_blocks_per_cu", C.c_int32), ("trace_refill", C.c_int32), ("_pad", C.c_int32),
+                ("trace_blocks_per_cu", C.c_int32), ("trace_refill", C.c_int32), ("trace_walk", C.c_int32),
                 ("trace_refill_prims", C.c_int64), ("render_paths", C.c_int64), ("graph_paths", C.c_int64)]
 
 
@@ -138,7 +138,7 @@ class AutotuneReport(C.Structure):
     _fields_ = [("ms_before", C.c_double), ("ms_after", C.c_double), ("probes", C.c_int32), ("_pad", C.c_int32)]
 
 
-TUNING_FIELDS = ("raygen_tile_w", "raygen_tile_h", "trace_stack_lds", "trace_blocks_per_cu", "trace_refill",
+TUNING_FIELDS = ("raygen_tile_w", "raygen_tile_h", "trace_stack_lds", "trace_blocks_per_cu", "trace_refill", "trace_walk",
                  "trace_refill_prims", "render_paths", "graph_paths")
 
 

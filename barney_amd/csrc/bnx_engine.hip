@@ -56,15 +56,17 @@ struct Pinned {
 constexpr int MAX_DD_GENS = 16;
 
 // what the launches of a device-driven batch depend on, besides the
-// wavefront's buffers
+// wavefront's buffers: every tuning field that selects a kernel variant or a
+// launch size inside the captured loop (a graph replays the kernels it was
+// captured with)
 struct BatchShape {
   long long count0 = 0, cap = 0;
   bool surfaces = false, volumes = false, full = false;
-  int refill = 64, stack = 16, blocks_per_cu = 8;
+  int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8;
   bool operator<(const BatchShape &o) const
   {
-    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, blocks_per_cu) <
-           std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.blocks_per_cu);
+    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu) <
+           std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.walk, o.blocks_per_cu);
   }
 };
 
@@ -447,8 +449,9 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     // this shape takes are known (from a first batch through the host loop
     // below), run that many as one captured bounce loop with one read-back
     const bool dd = dev->graph_paths > 0 && np * nS <= dev->graph_paths;
-    const BatchShape shape{np * nS, 2 * np * nS, surfaces, volumes, full, trace_refill_for(dev, w),
-                           dev->trace_stack_lds, dev->trace_blocks_per_cu};
+    const BatchShape shape{np * nS,          2 * np * nS,           surfaces,
+                           volumes,          full,                  trace_refill_for(dev, w),
+                           dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu};
     int dd_gens = 0;
     if (dd) {
       auto it = F.gens_seen.find(shape);

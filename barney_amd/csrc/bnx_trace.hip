@@ -139,12 +139,34 @@ struct Counts {
   uint32_t nodes, tris, spheres;
 };
 
+// The ray in the world frame, read again from its record (origin at rorg,
+// direction at rdir) when a walk leaves an instance, rather than held in
+// registers for the whole walk: the walks are short of registers, and a
+// ray changes frame only on entering and leaving an instance.
+BNX_D void world_ray(const float4 *rorg, const float4 *rdir, v3 &o, v3 &d, v3 &inv)
+{
+  o = xyz(*rorg);
+  d = fix_dir(xyz(*rdir));
+  inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+}
+
+// the ray in an instance's object frame (identity instances keep the world
+// frame the walk is in)
+BNX_D void instance_ray(const InstDev &I, const float4 *rorg, const float4 *rdir, v3 &o, v3 &d, v3 &inv)
+{
+  if (I.identity) return;
+  o = xfm_point_rows(I.w2o, xyz(*rorg));
+  d = fix_dir(xfm_vector_rows(I.w2o, fix_dir(xyz(*rdir))));
+  inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+}
+
 // entry distance of child k of a 4-wide node, +inf when the ray misses it.
 // Branch-free, so the six box vectors of a node are loaded together up front
 // (a short-circuit on the child code let the compiler sink each child's box
-// loads behind the code load, doubling the dependent fetches per node).  The
-// slab distances are finite (|d| >= 1e-6 after fix_dir; empty slots hold
-// inverted infinite boxes), so fminf/fmaxf give the same values as compares.
+// loads behind the code load, doubling the dependent fetches per node).
+// |inv| <= 1e6 (|d| >= 1e-6 after fix_dir) and empty slots hold inverted
+// infinite boxes, so no slab distance is NaN and fminf/fmaxf give the same
+// values as compares.
 BNX_D float child_entry(v3 o, v3 inv, float lx, float hx, float ly, float hy, float lz, float hz, uint32_t code,
                         float tmin, float tmax)
 {
@@ -171,7 +193,8 @@ BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 // lds: this lane's ring, &ring[0][lane]; the stack depth and the spill
 // array are locals so the depth stays in a register
 template <int SH>
-BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float tmax, bool any_hit, uint32_t *lds,
+BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax, bool any_hit,
+                      uint32_t *lds,
                       Counts &cnt)
 {
   uint32_t deep[STACK_CAP - SH];
@@ -189,10 +212,9 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
     if (sp >= SH) lds[s * TB] = deep[sp - SH];
     return v;
   };
-  const v3 wdir = fix_dir(wdir_in);
-  const v3 winv = v3{1.f / wdir.x, 1.f / wdir.y, 1.f / wdir.z};
   HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
-  v3 o = worg, d = wdir, inv = winv;
+  v3 o, d, inv;
+  world_ray(rorg, rdir, o, d, inv);
   const Node4 *nodes = S.tlas_nodes;
   BlasDev blas{};
   bool in_blas = false;
@@ -230,7 +252,7 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
       }
     } else if (cur == LEAVE_BLAS) {
       in_blas = false;
-      o = worg; d = wdir; inv = winv;
+      world_ray(rorg, rdir, o, d, inv);
       nodes = S.tlas_nodes;
     } else {
       // ------------------------------------------------------------ leaf
@@ -241,9 +263,7 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
         const InstDev &I = S.insts[inst];
         blas = S.blas[I.blas];
         if (!I.identity) {
-          o = xfm_point_rows(I.w2o, worg);
-          d = fix_dir(xfm_vector_rows(I.w2o, wdir));
-          inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+          instance_ray(I, rorg, rdir, o, d, inv);
         }
         push(LEAVE_BLAS);
         in_blas = true;
@@ -269,6 +289,127 @@ BNX_D HitRec traverse(const SceneDev &S, v3 worg, v3 wdir_in, float tmin, float 
   return best;
 }
 
+// The walk of traverse() in speculative while-while order (Aila and Laine,
+// "Understanding the Efficiency of Ray Traversal on GPUs", HPG 2009): a
+// lane that reaches a primitive leaf sets it aside and keeps stepping inner
+// nodes from its stack while other lanes of the wave still step nodes, until
+// every lane with a node to step already holds a leaf (or none has a node);
+// then the lanes test their leaves together.  Each trip of the wave thus
+// serves either node steps or leaf tests for most of its lanes, where the
+// plain walk pays for both whenever its lanes are split between them.  A
+// node is culled against the best hit known when it is stepped, so a lane
+// may step nodes that a hit in its pending leaf would have culled; the hit
+// returned is the same (equal t resolves by ID).  cur == EMPTY4: the stack
+// is exhausted.
+template <int SH>
+BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax,
+                           bool any_hit,
+                           uint32_t *lds, Counts &cnt)
+{
+  uint32_t deep[STACK_CAP - SH];
+  int sp = 0;
+  auto push = [&](uint32_t v) {
+    const int s = sp & (SH - 1);
+    if (sp >= SH) deep[sp - SH] = lds[s * TB];
+    lds[s * TB] = v;
+    sp++;
+  };
+  auto pop = [&]() -> uint32_t {
+    if (sp == 0) return EMPTY4;
+    sp--;
+    const int s = sp & (SH - 1);
+    const uint32_t v = lds[s * TB];
+    if (sp >= SH) lds[s * TB] = deep[sp - SH];
+    return v;
+  };
+  HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
+  v3 o, d, inv;
+  world_ray(rorg, rdir, o, d, inv);
+  const Node4 *nodes = S.tlas_nodes;
+  BlasDev blas{};
+  bool in_blas = false;
+  int inst = -1;
+  uint32_t cur = 0;
+  uint32_t leaf = 0;             // primitive leaf set aside (leaf codes carry LEAF_BIT)
+  uint32_t nvis = 0;
+  // every node is entered at most once per ray; the cap only bounds the
+  // loops if the structure were corrupted
+  uint32_t trips = 0;
+  while (trips < (1u << 28)) {
+    // ------------------------------------------------------- node phase
+    for (;; trips++) {
+      if (leaf == 0 && in_blas && (cur & LEAF_BIT)) {
+        leaf = cur;
+        cur = pop();
+      }
+      const bool step = cur < EMPTY4;
+      if (!__any(step && leaf == 0) || trips >= (1u << 28)) break;
+      if (step) {
+        const float4 *np = (const float4 *)(nodes + cur);
+        const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
+        const uint4 ch = ((const uint4 *)np)[6];
+        nvis++;
+        float t0 = child_entry(o, inv, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, tmin, best.t);
+        float t1 = child_entry(o, inv, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, tmin, best.t);
+        float t2 = child_entry(o, inv, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, tmin, best.t);
+        float t3 = child_entry(o, inv, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, tmin, best.t);
+        uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
+        cswap(t0, c0, t1, c1);
+        cswap(t2, c2, t3, c3);
+        cswap(t0, c0, t2, c2);
+        cswap(t1, c1, t3, c3);
+        cswap(t1, c1, t2, c2);
+        const float INF = __builtin_huge_valf();
+        if (t0 < INF) {
+          if (t3 < INF) push(c3);
+          if (t2 < INF) push(c2);
+          if (t1 < INF) push(c1);
+          cur = c0;
+        } else {
+          cur = pop();
+        }
+      }
+    }
+    // ------------------------------------------------------- leaf phase
+    bool found = false;
+    if (leaf != 0) {
+      const uint32_t first = (leaf & ~LEAF_BIT) >> 3;
+      const int count = (int)(leaf & 7u);
+      if (blas.type == BLAS_TRIANGLES) {
+        cnt.tris += count;
+        for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, tmin, best, inst);
+      } else {
+        cnt.spheres += count;
+        for (int k = 0; k < count; k++)
+          found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst);
+      }
+      leaf = 0;
+    } else if ((cur & LEAF_BIT) && !in_blas) {
+      // a TLAS leaf (the node phase sets every BLAS leaf aside): enter its
+      // instance
+      inst = S.tlas_order[(cur & ~LEAF_BIT) >> 3];
+      const InstDev &I = S.insts[inst];
+      blas = S.blas[I.blas];
+      if (!I.identity) {
+        instance_ray(I, rorg, rdir, o, d, inv);
+      }
+      push(LEAVE_BLAS);
+      in_blas = true;
+      nodes = blas.nodes;
+      cur = 0;
+    } else if (cur == LEAVE_BLAS) {
+      in_blas = false;
+      world_ray(rorg, rdir, o, d, inv);
+      nodes = S.tlas_nodes;
+      cur = pop();
+    }
+    if ((any_hit && found) || (cur == EMPTY4 && leaf == 0)) break;
+    trips++;
+  }
+  cnt.nodes += nvis;
+  return best;
+}
+
 __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *rays, int n, bnx_hit *hits,
                                                   uint32_t flags)
 {
@@ -280,7 +421,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
-    HitRec h = traverse<API_SHORT>(S, xyz(r0), xyz(r1), r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
+    HitRec h = traverse<API_SHORT>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
     bnx_hit out;
     out.t = h.t;
     out.u = h.prim >= 0 ? h.u : 0.f;
@@ -303,7 +444,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
 // each block lies inside one request's segment, so the request's scene
 // pointers are block-uniform.  counters: statistics array (STAT_NODES,
 // STAT_TRIS, STAT_SPHERES)
-template <int SH>
+template <int SH, bool SPEC>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
@@ -319,7 +460,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
     float4 d = q.dir[i];
     const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
-    if (S.num_insts > 0) h = traverse<SH>(S, xyz(o), xyz(d), 0.f, o.w, shadow, lds, cnt);
+    if (S.num_insts > 0)
+      h = SPEC ? traverse_spec<SH>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt)
+               : traverse<SH>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
@@ -371,8 +514,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
   // per-lane walk state
   bool active = false, any_hit = false, in_blas = false;
   int ray = 0, inst = -1;
-  v3 worg{0.f, 0.f, 0.f}, wdir{1.f, 1.f, 1.f}, winv{1.f, 1.f, 1.f};
-  v3 o = worg, d = wdir, inv = winv;
+  v3 o{0.f, 0.f, 0.f}, d{1.f, 1.f, 1.f}, inv{1.f, 1.f, 1.f};
   HitRec best{0.f, 0.f, 0.f, -1, -1, -1};
   const Node4 *nodes = S.tlas_nodes;
   BlasDev blas{};
@@ -406,10 +548,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
         ray = sg.begin + q_next + rank;
         const float4 ro = q.org[ray], rd = q.dir[ray];
         any_hit = (__float_as_uint(rd.w) & FLAG_SHADOW) != 0;
-        worg = xyz(ro);
-        wdir = fix_dir(xyz(rd));
-        winv = v3{1.f / wdir.x, 1.f / wdir.y, 1.f / wdir.z};
-        o = worg; d = wdir; inv = winv;
+        o = xyz(ro);
+        d = fix_dir(xyz(rd));
+        inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
         best = HitRec{ro.w, 0.f, 0.f, -1, -1, -1};
         nodes = S.tlas_nodes;
         in_blas = false;
@@ -454,7 +595,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
         }
       } else if (cur == LEAVE_BLAS) {
         in_blas = false;
-        o = worg; d = wdir; inv = winv;
+        world_ray(q.org + ray, q.dir + ray, o, d, inv);
         nodes = S.tlas_nodes;
         if (sp == 0) done = true;
         else cur = pop();
@@ -466,9 +607,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
           const InstDev &I = S.insts[inst];
           blas = S.blas[I.blas];
           if (!I.identity) {
-            o = xfm_point_rows(I.w2o, worg);
-            d = fix_dir(xfm_vector_rows(I.w2o, wdir));
-            inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
+            instance_ray(I, q.org + ray, q.dir + ray, o, d, inv);
           }
           push(LEAVE_BLAS);
           in_blas = true;
@@ -533,9 +672,17 @@ int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int
     }
   } else {
     if (blocks256 > 0) {
-      if (sh == 8) k_trace_static<8><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
-      else if (sh == 32) k_trace_static<32><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
-      else k_trace_static<16><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      const bool spec = dev->trace_walk == 1;
+      if (sh == 8) {
+        if (spec) k_trace_static<8, true><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+        else k_trace_static<8, false><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      } else if (sh == 32) {
+        if (spec) k_trace_static<32, true><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+        else k_trace_static<32, false><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      } else {
+        if (spec) k_trace_static<16, true><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+        else k_trace_static<16, false><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      }
     }
   }
   BNX_HIP(hipGetLastError());

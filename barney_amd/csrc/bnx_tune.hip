@@ -1,6 +1,6 @@
 // bnx_tune.hip -- launch tuning of the render path (include/bnx_rt.h,
 // bnx_tuning) and its autotuner: coordinate descent over the raygen tile,
-// the LDS share of the traversal stack, the pooled trace's refill threshold
+// the LDS share of the traversal stack, the walk order, the pooled trace's refill threshold
 // and launch size, bnx_render's paths per batch and the largest batch run as
 // a captured bounce loop, each candidate timed on
 // probe renders of the caller's request shape.  Host code only.
@@ -9,6 +9,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cmath>
 #include <cstdint>
 #include <cstring>
 #include <string>
@@ -25,6 +26,7 @@ bnx_tuning tuning_of(const Device *d)
   t.trace_stack_lds = d->trace_stack_lds;
   t.trace_blocks_per_cu = d->trace_blocks_per_cu;
   t.trace_refill = d->trace_refill;
+  t.trace_walk = d->trace_walk;
   t.trace_refill_prims = d->trace_refill_prims;
   t.render_paths = d->render_paths;
   t.graph_paths = d->graph_paths;
@@ -40,6 +42,7 @@ const char *tuning_error(const bnx_tuning &t)
     return "trace_stack_lds: 8, 16 or 32";
   if (t.trace_blocks_per_cu < 1 || t.trace_blocks_per_cu > 64) return "trace_blocks_per_cu: 1..64";
   if (t.trace_refill < 1 || t.trace_refill > 64) return "trace_refill: 1..64";
+  if (t.trace_walk != 0 && t.trace_walk != 1) return "trace_walk: 0 or 1";
   if (t.trace_refill_prims < 0) return "trace_refill_prims: >= 0";
   if (t.render_paths < (1LL << 16) || t.render_paths > (1LL << 26)) return "render_paths: 2^16 .. 2^26";
   if (t.graph_paths < 0 || t.graph_paths > (1LL << 26)) return "graph_paths: 0 .. 2^26";
@@ -53,6 +56,7 @@ void apply(Device *d, const bnx_tuning &t)
   d->trace_stack_lds = t.trace_stack_lds;
   d->trace_blocks_per_cu = t.trace_blocks_per_cu;
   d->trace_refill = t.trace_refill;
+  d->trace_walk = t.trace_walk;
   d->trace_refill_prims = t.trace_refill_prims;
   d->render_paths = t.render_paths;
   d->graph_paths = t.graph_paths;
@@ -78,18 +82,21 @@ struct Tuner {
   bnx_renderer rr;
   Frame *f;
   int spp;
+  int reps = 2;
   int probes = 0;
 
-  // best of two probes with tuning t
+  // best of `reps` probes with tuning t
   int time(const bnx_tuning &t, double &ms)
   {
     apply(dev, t);
-    double a = 0.0, b = 0.0;
-    int rc;
-    if ((rc = probe_ms(w, cam, rr, f, spp, a)) != BNX_OK) return rc;
-    if ((rc = probe_ms(w, cam, rr, f, spp, b)) != BNX_OK) return rc;
-    probes += 2;
-    ms = std::min(a, b);
+    ms = 1e300;
+    for (int k = 0; k < reps; k++) {
+      double a = 0.0;
+      const int rc = probe_ms(w, cam, rr, f, spp, a);
+      if (rc != BNX_OK) return rc;
+      ms = std::min(ms, a);
+    }
+    probes += reps;
     return BNX_OK;
   }
 };
@@ -149,6 +156,11 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
   // the tuning in place, after one untimed render (first-use allocations)
   double warm = 0.0;
   int rc = probe_ms(w, T.cam, T.rr, &f, probe_spp, warm);
+  // short probes are timed more often: at least about 10 ms per candidate,
+  // judged on a third render (the second of a batch shape captures its
+  // bounce loop)
+  for (int k = 0; k < 2 && rc == BNX_OK; k++) rc = probe_ms(w, T.cam, T.rr, &f, probe_spp, warm);
+  T.reps = std::max(2, std::min(20, (int)std::ceil(10.0 / std::max(1e-3, warm))));
   if (rc == BNX_OK) rc = T.time(best, best_ms);
   first_ms = best_ms;
   // every candidate of one field, the other fields at their best so far
@@ -170,7 +182,7 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
     return BNX_OK;
   };
   const std::vector<std::pair<int, int>> tiles = {{0, 0}, {8, 8}, {16, 4}, {16, 16}, {32, 8}};
-  const std::vector<int> stacks = {8, 16, 32};
+  const std::vector<std::pair<int, int>> stack_walks = {{8, 0}, {16, 0}, {32, 0}, {8, 1}, {16, 1}, {32, 1}};
   const std::vector<int> refills = {64, 32, 16, 8, 4};
   const std::vector<int> blocks = {2, 4, 8, 16};
   const std::vector<long long> paths = {2LL << 20, 4LL << 20, 8LL << 20, 16LL << 20};
@@ -180,7 +192,16 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
   for (int pass = 0; pass < 2 && rc == BNX_OK; pass++) {
     rc = sweep([](bnx_tuning &t, const std::pair<int, int> &v) { t.raygen_tile_w = v.first; t.raygen_tile_h = v.second; },
                tiles);
-    if (rc == BNX_OK) rc = sweep([](bnx_tuning &t, int v) { t.trace_stack_lds = v; }, stacks);
+    // the LDS stack depth and the walk order together: both change the
+    // trace kernel's register budget, so neither's best holds for all of the
+    // other's values
+    if (rc == BNX_OK)
+      rc = sweep(
+          [](bnx_tuning &t, const std::pair<int, int> &v) {
+            t.trace_stack_lds = v.first;
+            t.trace_walk = v.second;
+          },
+          stack_walks);
     // the refill threshold for worlds of this size: 64 keeps one ray per
     // thread for them; below 64 the pooled trace takes their secondary rays
     if (rc == BNX_OK)

@@ -81,7 +81,17 @@ def umesh_bytes(nodes, tets, cells, rays):
     return 64.0 * nodes + 80.0 * tets + 4.0 * cells + 48.0 * rays
 
 
-def roofline(tot, workload=None):
+def trace_kernel_name(tuning, max_world_prims):
+    """the queue-trace kernels that ran: the LDS-stack variant of the tuning;
+    the pooled kernel too when a world is large enough to use it"""
+    sh = tuning["trace_stack_lds"]
+    walk = "true" if tuning.get("trace_walk", 0) == 1 else "false"
+    pooled = tuning["trace_refill"] < 64 and max_world_prims >= tuning["trace_refill_prims"]
+    static = f"k_trace_static<{sh}, {walk}>"
+    return f"{static}+k_trace_pool<{sh}>" if pooled else static
+
+
+def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false>"):
     """dominant kernel (by summed HIP-event time) and its algorithmic GB/s"""
     if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
         # the full variant runs for tet fields and isosurfaces, the lean one otherwise
@@ -94,9 +104,9 @@ def roofline(tot, workload=None):
             byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
     else:
         # queue traversal: one ray per thread, or pooled for the secondary rays
-        # of large scenes (config 3, also resident in the serving stream); the
-        # HIP events bracket whichever ran
-        kernel = "k_trace_static+k_trace_pool" if workload in ("c3", "c5") else "k_trace_static"
+        # of large scenes when the tuning says so; the HIP events bracket
+        # whichever ran
+        kernel = trace_kernel
         kms, launches = tot["trace_ms"], tot["trace_launches"]
         byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
     achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
@@ -407,6 +417,7 @@ def run_c5(args, n, warmup):
     dev = bx.Device(0)
     worlds, reqs = serve_setup(dev, args, n + warmup)
     log("worlds resident")
+    max_prims = max(int(getattr(sc, "num_prims", 0)) for _, sc in worlds)
     # one launch tuning for both forms: bnx_autotune on the stream's heaviest
     # world (config 3) at a 1024^2 request with 16 spp
     tune = set_tuning(dev, parse_tuning(args.tuning))
@@ -446,7 +457,8 @@ def run_c5(args, n, warmup):
             f.close()
         dev.close()
         desc = {"workload": f"config 5 serving stream, served pass at saturation only ({n} requests)",
-                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)", **tune}
+                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)", **tune,
+                "max_world_prims": max_prims}
         return served_mrays, makespan, tot, desc, {"frame_latency_ms": {"p50": None, "p99": None}}
     # the same stream, Poisson arrivals at a fixed rate: load args.load of
     # the plain form's capacity
@@ -476,7 +488,8 @@ def run_c5(args, n, warmup):
                         f"{args.load} of the plain form's capacity)",
             "requests": n, "warmup_requests": warmup, "qps": round(qps, 3), "resolutions": res,
             "form": "served (continuous batching) vs plain (one bnx_render per request)",
-            "scenes": ["config 1", "config 2", "config 3"], "requests_per_scene": per_scene, **tune}
+            "scenes": ["config 1", "config 2", "config 3"], "requests_per_scene": per_scene, **tune,
+            "max_world_prims": max_prims}
     extra = {
         "frame_latency_ms": {"p50": pct(lat_served, 50), "p99": pct(lat_served, 99)},
         "served": {"mrays_s_saturated": round(served_mrays, 3), "makespan_s": round(makespan, 4),
@@ -588,6 +601,7 @@ def main():
                     "c4": workload_c4}[args.workload](args)
         elapsed, lat, tot, tune = run_gpu(sc, steps, warmup, parse_tuning(args.tuning))
         desc.update(tune)
+        desc["max_world_prims"] = int(getattr(sc, "num_prims", 0))
         mrays = tot["rays"] / elapsed / 1e6
         lat_line = {"p50": pct(lat, 50), "p99": pct(lat, 99)}
         extra = {"rays_per_step": tot["rays"] // steps}
@@ -597,7 +611,8 @@ def main():
         "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": round(elapsed * 1e3 / steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes and stream)",
-        "config": desc, "frame_latency_ms": lat_line, "roofline": roofline(tot, args.workload),
+        "config": desc, "frame_latency_ms": lat_line,
+        "roofline": roofline(tot, args.workload, trace_kernel_name(desc["tuning"], desc.get("max_world_prims", 0))),
     }
     line.update(extra)
     if not args.no_cpu:

@@ -280,7 +280,9 @@ typedef struct bnx_tuning {
   int32_t trace_blocks_per_cu;      /* pooled trace launch: blocks per CU, 1..64     */
   int32_t trace_refill;             /* pooled trace: idle lanes of a wave that make
                                        it draw new rays, 1..64                       */
-  int32_t _pad;
+  int32_t trace_walk;               /* one-ray-per-thread walk: 0 = a node step or a
+                                       leaf per trip; 1 = speculative while-while
+                                       (leaves set aside until every lane has one)  */
   int64_t trace_refill_prims;       /* secondary generations of worlds with at least
                                        this many primitives use the pooled trace     */
   int64_t render_paths;             /* bnx_render: paths per batch of samples,
@@ -290,7 +292,7 @@ typedef struct bnx_tuning {
                                        hipGraph, one read-back per batch instead
                                        of one per generation (0: never; <= 2^26)    */
 } bnx_tuning;
-/* a new device starts with 8 x 8 tiles, 16 LDS entries, 8 blocks per CU,
+/* a new device starts with 8 x 8 tiles, 16 LDS entries, walk 0, 8 blocks per CU,
  * refill 16 for worlds of 2^20 primitives or more, 2^23 paths per batch and
  * captured bounce loops for batches of up to 2^22 paths */
 
@@ -307,10 +309,11 @@ typedef struct bnx_autotune_report {
 
 /* Choose the device's tuning for requests like this one: coordinate descent,
  * two passes, over the fields of bnx_tuning (raygen tile, LDS stack entries,
- * refill threshold, pooled-launch size, paths per batch, captured-loop
+ * walk order, refill threshold, pooled-launch size, paths per batch, captured-loop
  * limit), each candidate timed on
  * probe renders of width x height at probe_spp samples into a scratch frame
- * (best of two; a candidate must be 1% faster to replace the current one).
+ * (best of 2 to 20, so that each candidate is timed for about 10 ms; a
+ * candidate must be 1% faster to replace the current one).
  * Sets the tuning chosen on the device (the one in place if nothing was
  * faster) and returns it with the timings.  No counterpart in the
  * reference. */

@@ -61,6 +61,8 @@ TUNINGS = [
     dict(raygen_tile_w=1, raygen_tile_h=256),
     dict(trace_stack_lds=8),
     dict(trace_stack_lds=32),
+    dict(trace_walk=1),
+    dict(trace_walk=1, trace_stack_lds=8, raygen_tile_w=0, raygen_tile_h=0),
     dict(trace_refill=4, trace_refill_prims=0, trace_blocks_per_cu=1),
     dict(trace_refill=64, trace_refill_prims=0),
     dict(render_paths=1 << 16, raygen_tile_w=16, raygen_tile_h=4),
@@ -87,7 +89,7 @@ def test_tuning_roundtrip_and_rejects(dev, keep_tuning):
     t1 = dev.tuning()
     assert (t1["raygen_tile_w"], t1["raygen_tile_h"], t1["trace_stack_lds"], t1["render_paths"]) == (8, 8, 32, 1 << 22)
     for bad in (dict(raygen_tile_w=8, raygen_tile_h=0), dict(raygen_tile_w=300, raygen_tile_h=1),
-                dict(trace_stack_lds=12), dict(trace_blocks_per_cu=0), dict(trace_refill=65),
+                dict(trace_stack_lds=12), dict(trace_blocks_per_cu=0), dict(trace_refill=65), dict(trace_walk=2),
                 dict(trace_refill_prims=-1), dict(render_paths=1000), dict(render_paths=1 << 27)):
         with pytest.raises(bx.BnxError):
             dev.set_tuning(**bad)
@@ -172,3 +174,31 @@ def test_captured_loop_with_too_few_generations(dev, worlds, keep_tuning):
     _same(_render_mg(dev, gw, sc, w, h, spp, 0), ref_full, "graph of 2, rest on the host loop")
     _same(_render_mg(dev, gw, sc, w, h, spp, 0), ref_full, "graph of the full count")
     _same(_render_mg(dev, gw, sc, w, h, spp, 2), ref_two, "graph longer than the paths")
+
+
+def test_captured_loop_follows_tuning(dev, worlds, keep_tuning):
+    """A captured bounce loop replays the kernels it was captured with, so a
+    change of tuning must not reuse it.  The plain walk's node-visit count
+    is a sum of per-ray counts, the same for the host loop, the capture, a
+    replay and after switching back; every render with the speculative walk
+    steps other nodes (it steps some that the plain walk culls; its count
+    also depends on which rays share a wave, and the order in which shade
+    appends secondary rays varies from run to run)."""
+    gw, sc = worlds["c3"]
+    cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
+    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=sc.max_generations)
+    fr = bx.Frame(dev, 57, 43)
+
+    def visits():
+        fr.reset()
+        return bx.render(gw.world, cam, rend, fr, 2)["nodes"]
+
+    dev.set_tuning(graph_paths=1 << 26, trace_walk=0)
+    n0 = [visits() for _ in range(3)]
+    dev.set_tuning(trace_walk=1)
+    n1 = [visits() for _ in range(3)]
+    dev.set_tuning(trace_walk=0)
+    n0b = visits()
+    fr.close()
+    assert n0[0] == n0[1] == n0[2] == n0b, (n0, n0b)
+    assert all(n != n0[0] for n in n1), (n0, n1)
