@@ -131,6 +131,7 @@ class ServerConfig(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("raygen_tile_w", C.c_int32), ("raygen_tile_h", C.c_int32), ("trace_stack_lds", C.c_int32),
                 ("trace_blocks_per_cu", C.c_int32), ("trace_refill", C.c_int32), ("trace_walk", C.c_int32),
+                ("sort_rays", C.c_int32), ("_pad", C.c_int32),
                 ("trace_refill_prims", C.c_int64), ("render_paths", C.c_int64), ("graph_paths", C.c_int64)]
 
 
@@ -139,6 +140,7 @@ class AutotuneReport(C.Structure):
 
 
 TUNING_FIELDS = ("raygen_tile_w", "raygen_tile_h", "trace_stack_lds", "trace_blocks_per_cu", "trace_refill", "trace_walk",
+                 "sort_rays",
                  "trace_refill_prims", "render_paths", "graph_paths")
 
 
@@ -165,7 +167,7 @@ class ServerStats(C.Structure):
 BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE = 0, 1, 2
 
 assert C.sizeof(Material) == 48 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
-assert C.sizeof(Tuning) == 48 and C.sizeof(AutotuneReport) == 24
+assert C.sizeof(Tuning) == 56 and C.sizeof(AutotuneReport) == 24
 assert C.sizeof(ServerConfig) == 24 and C.sizeof(Request) == 136 and C.sizeof(RequestStatus) == 48
 assert C.sizeof(UMeshDesc) == 64
 assert C.sizeof(ServerStats) == 168 and Request.renderer.offset == 76 and Request.spp.offset == 124

@@ -17,7 +17,7 @@ CSRC = os.path.join(HERE, "csrc")
 OBJ_DIR = os.path.join(HERE, "_build")
 LIB = os.path.join(HERE, "libbnx.so")
 SOURCES = ["bnx_core.hip", "bnx_bvh.hip", "bnx_trace.hip", "bnx_render.hip", "bnx_volume.hip", "bnx_engine.hip",
-           "bnx_env.hip", "bnx_tune.hip"]
+           "bnx_env.hip", "bnx_tune.hip", "bnx_sort.hip"]
 ARCH = os.environ.get("BNX_OFFLOAD_ARCH", "gfx950")
 
 # -ffp-contract=off and correctly rounded division / square root keep every

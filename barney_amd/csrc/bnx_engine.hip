@@ -54,6 +54,8 @@ struct Pinned {
 
 // device-driven generations of one batch at most (a captured bounce loop)
 constexpr int MAX_DD_GENS = 16;
+// smallest generation worth reordering
+constexpr int SORT_MIN_RAYS = 1 << 16;
 
 // what the launches of a device-driven batch depend on, besides the
 // wavefront's buffers: every tuning field that selects a kernel variant or a
@@ -89,6 +91,7 @@ struct Wavefront {
   // (valid for the buffer addresses they were captured with)
   DevMem gen_counts, stamps;
   Pinned h_gen_counts;
+  SortScratch sorter;
   std::map<BatchShape, int> gens_seen;
   std::vector<BatchGraph> graphs;
   std::vector<const void *> graph_bufs;
@@ -496,7 +499,16 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     // the host loop: one read-back per generation (and the generations a
     // captured loop left over)
     int host_gens = 0;
+    // secondary generations of surface worlds may be reordered first (into
+    // the other queue, whose rays are spent)
+    const Bvh &tlas = w->instances->bvh;
+    const bool sorting = dev->sort_rays != 0 && traced && tlas.num_prims > 0;
     for (; count > 0; primary = false) {
+      if (sorting && !primary && count >= SORT_MIN_RAYS) {
+        if ((rc = sort_rays(dev, F.sorter, F.Q.q(cur), F.Q.q(1 - cur), count, tlas.root_lo, tlas.root_hi)) != BNX_OK)
+          return rc;
+        cur = 1 - cur;
+      }
       SegList sl{F.h_segs.as<Segment>()};
       sl.add(0, 0, count, primary ? 64 : trace_refill_for(dev, w));
       BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, 2 * sizeof(int), dev->stream));

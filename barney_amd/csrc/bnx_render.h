@@ -171,6 +171,15 @@ int launch_gen_setup(Device *dev, Segment *segs, int *out_cnt, long long *gen_co
 // write the device wall clock (ticks of wall_clock_khz) to *t once the
 // stream reaches this point
 int launch_stamp(Device *dev, unsigned long long *t);
+
+// scratch of the ray reordering (keys, indices, radix-sort temporaries)
+struct SortScratch {
+  DevMem keys[2], idx[2], tmp;
+};
+// the n rays of src (org, dir, rng, ps) into dst in the order of a key of
+// shadow flag, direction octant and the origin's Morton code in [lo, hi]
+int sort_rays(Device *dev, SortScratch &S, const RayQueue &src, const RayQueue &dst, int n, const float lo[3],
+              const float hi[3]);
 // fill the block -> segment maps of nsegs segments (blocks256 / blocks1024 /
 // blocksT blocks in all) and reset the segments' trace pools
 int launch_segment_blocks(Device *dev, const SegView &sv, int nsegs, int blocks256, int blocks1024, int blocksT);

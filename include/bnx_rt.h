@@ -283,6 +283,11 @@ typedef struct bnx_tuning {
   int32_t trace_walk;               /* one-ray-per-thread walk: 0 = a node step or a
                                        leaf per trip; 1 = speculative while-while
                                        (leaves set aside until every lane has one)  */
+  int32_t sort_rays;                /* bnx_render: 1 = reorder each secondary
+                                       generation of 2^16 rays or more by shadow
+                                       flag, direction octant and origin Morton
+                                       code before tracing it (0: queue order)      */
+  int32_t _pad;
   int64_t trace_refill_prims;       /* secondary generations of worlds with at least
                                        this many primitives use the pooled trace     */
   int64_t render_paths;             /* bnx_render: paths per batch of samples,
@@ -292,7 +297,7 @@ typedef struct bnx_tuning {
                                        hipGraph, one read-back per batch instead
                                        of one per generation (0: never; <= 2^26)    */
 } bnx_tuning;
-/* a new device starts with 8 x 8 tiles, 16 LDS entries, walk 0, 8 blocks per CU,
+/* a new device starts with 8 x 8 tiles, 16 LDS entries, walk 0, no reordering, 8 blocks per CU,
  * refill 16 for worlds of 2^20 primitives or more, 2^23 paths per batch and
  * captured bounce loops for batches of up to 2^22 paths */
 
@@ -309,7 +314,7 @@ typedef struct bnx_autotune_report {
 
 /* Choose the device's tuning for requests like this one: coordinate descent,
  * two passes, over the fields of bnx_tuning (raygen tile, LDS stack entries,
- * walk order, refill threshold, pooled-launch size, paths per batch, captured-loop
+ * walk order, ray reordering, refill threshold, pooled-launch size, paths per batch, captured-loop
  * limit), each candidate timed on
  * probe renders of width x height at probe_spp samples into a scratch frame
  * (best of 2 to 20, so that each candidate is timed for about 10 ms; a
@@ -546,7 +551,7 @@ static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");
 static_assert(sizeof(bnx_request_status) == 48, "bnx_request_status layout");
 static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");
-static_assert(sizeof(bnx_tuning) == 48, "bnx_tuning layout");
+static_assert(sizeof(bnx_tuning) == 56, "bnx_tuning layout");
 static_assert(sizeof(bnx_autotune_report) == 24, "bnx_autotune_report layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
 static_assert(sizeof(bnx_render_stats) == 128, "bnx_render_stats layout");
