@@ -34,6 +34,12 @@ struct VolumeDev {
   const int4 BNX_G *utets;      // vertex indices, in element-BVH slot order
   const int4 BNX_G *unbr;       // per tet slot: the slot across the face opposite vertex k (-1: boundary)
   const Node BNX_G *ebvh;       // element BVH
+  // seed grid over the bounds: per cell the tet slot holding its centre (-1:
+  // none), where a point query with no nearby previous tet starts its walk
+  const int BNX_G *useed;
+  int seed_dims[3];
+  float seed_origin[3];
+  float seed_rcp[3];          // cells per unit length (host float division)
   // both
   float lo[3], hi[3];         // world bounds
   int mc[3];
@@ -71,7 +77,7 @@ struct Volume {
   uint64_t version = 0;         // bumped whenever dd changes
   VolumeDev dd{};
   DevMem scalars, ranges, majorants, tf;
-  DevMem uverts, utets, unbr;
+  DevMem uverts, utets, unbr, useed;
   Bvh ebvh;                     // element BVH (tets)
 };
 
@@ -229,31 +235,38 @@ BNX_D bool planes_interior(const TetPlanes &r)
 
 constexpr int TET_WALK_STEPS = 6;
 
-BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
+// Walk from tet slot `cur` toward P: test the tet, else cross the face P
+// lies furthest behind, up to TET_WALK_STEPS times.  Accepts only a tet that
+// holds P with margin: in a mesh of non-overlapping tets no other tet can,
+// so the value is the one the depth-first query would return (see
+// planes_interior).
+BNX_D bool umesh_walk(const VolumeDev &v, int cur, v3 P, MarchCtx &cx, float &value, int &found)
 {
-  // consecutive queries of a ray are close together: try the tet of the
-  // previous query, then walk across the face P lies furthest behind, a few
-  // steps; a tet that holds P with margin is the only one that can (see
-  // planes_interior), else the depth-first query decides
-  if (cx.last_field == v.utets && cx.last_tet >= 0) {
-    int cur = cx.last_tet;
-    for (int step = 0; step <= TET_WALK_STEPS; step++) {
-      cx.utets++;
-      const TetPlanes r = tet_planes(v, cur, P);
-      if (planes_interior(r)) {
-        cx.last_tet = cur;
-        return r.value;
-      }
-      if (step == TET_WALK_STEPS || !(r.sum > 0.f)) break;
-      int k = 0;
-      for (int j = 1; j < 4; j++)
-        if (r.t[j] < r.t[k]) k = j;
-      const int4 nb = v.unbr[cur];
-      const int next = k == 0 ? nb.x : (k == 1 ? nb.y : (k == 2 ? nb.z : nb.w));
-      if (next < 0) break;
-      cur = next;
+  for (int step = 0; step <= TET_WALK_STEPS; step++) {
+    cx.utets++;
+    const TetPlanes r = tet_planes(v, cur, P);
+    if (planes_interior(r)) {
+      found = cur;
+      value = r.value;
+      return true;
     }
+    if (step == TET_WALK_STEPS || !(r.sum > 0.f)) break;
+    int k = 0;
+    for (int j = 1; j < 4; j++)
+      if (r.t[j] < r.t[k]) k = j;
+    const int4 nb = v.unbr[cur];
+    const int next = k == 0 ? nb.x : (k == 1 ? nb.y : (k == 2 ? nb.z : nb.w));
+    if (next < 0) break;
+    cur = next;
   }
+  return false;
+}
+
+// UMeshCuBQLSampler::sample (umesh/mc/UMeshCuBQLSampler.h:46-62): the first
+// tet, depth first through the element BVH, that contains P; its slot (-1:
+// none, the value is then NaN)
+BNX_D int umesh_dfs(const VolumeDev &v, v3 P, MarchCtx &cx, float &value)
+{
   uint32_t deep[PQ_DEEP];
   int sp = 0;
   uint32_t cur = 0;
@@ -283,18 +296,60 @@ BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
         float f;
         cx.utets++;
         if (tet_scalar(v, first + k, P, f)) {
-          cx.last_field = v.utets;
-          cx.last_tet = first + k;
-          return f;
+          value = f;
+          return first + k;
         }
       }
     }
-    if (sp == 0) return __builtin_nanf("");
+    if (sp == 0) {
+      value = __builtin_nanf("");
+      return -1;
+    }
     sp--;
     const int s = sp & (PQ_SHORT - 1);
     cur = cx.ring[s * PQ_TB];
     if (sp >= PQ_SHORT) cx.ring[s * PQ_TB] = deep[sp - PQ_SHORT];
   }
+}
+
+// the seed-grid cell of P (clamped to the grid)
+BNX_D int seed_cell(const VolumeDev &v, v3 P)
+{
+  const float p3[3] = {P.x, P.y, P.z};
+  int c[3];
+  for (int k = 0; k < 3; k++) {
+    const float x = fminf(fmaxf((p3[k] - v.seed_origin[k]) * v.seed_rcp[k], 0.f), (float)(v.seed_dims[k] - 1));
+    c[k] = (int)x;
+  }
+  return (c[2] * v.seed_dims[1] + c[1]) * v.seed_dims[0] + c[0];
+}
+
+// A ray's consecutive queries are close together: walk from the tet of its
+// previous query; a query with none nearby walks from the seed of its grid
+// cell; the depth-first query decides what neither walk settles.
+BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
+{
+  float value;
+  int t;
+  const bool have_last = cx.last_field == v.utets && cx.last_tet >= 0;
+  if (have_last && umesh_walk(v, cx.last_tet, P, cx, value, t)) {
+    cx.last_tet = t;
+    return value;
+  }
+  if (v.useed) {
+    const int seed = v.useed[seed_cell(v, P)];
+    if (seed >= 0 && umesh_walk(v, seed, P, cx, value, t)) {
+      cx.last_field = v.utets;
+      cx.last_tet = t;
+      return value;
+    }
+  }
+  t = umesh_dfs(v, P, cx, value);
+  if (t >= 0) {
+    cx.last_field = v.utets;
+    cx.last_tet = t;
+  }
+  return value;
 }
 
 BNX_D float volume_sample(const VolumeDev &v, v3 P, MarchCtx &cx)

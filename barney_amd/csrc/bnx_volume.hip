@@ -23,6 +23,8 @@
 #include "bnx_render.h"
 #include "bnx_volume.h"
 
+#include <cmath>
+
 #include <hipcub/hipcub.hpp>
 
 namespace bnx {
@@ -382,6 +384,56 @@ int build_tet_neighbors(Device *dev, const int4 *tets, int ntet, DevMem &nbr)
   return BNX_OK;
 }
 
+// the seed grid: per cell, the tet (depth-first query) holding its centre
+__global__ __launch_bounds__(PQ_TB) void k_umesh_seeds(VolumeDev d, int ncells, int *seeds)
+{
+  __shared__ uint32_t ring[PQ_SHORT * PQ_TB];
+  MarchCtx cx{ring + threadIdx.x, 0, 0, 0, 0};
+  const int c = blockIdx.x * PQ_TB + threadIdx.x;
+  if (c >= ncells) return;
+  const int nx = d.seed_dims[0], ny = d.seed_dims[1];
+  const int ix = c % nx, iy = (c / nx) % ny, iz = c / (nx * ny);
+  const v3 P = v3{d.seed_origin[0] + ((float)ix + .5f) / d.seed_rcp[0],
+                  d.seed_origin[1] + ((float)iy + .5f) / d.seed_rcp[1],
+                  d.seed_origin[2] + ((float)iz + .5f) / d.seed_rcp[2]};
+  float f;
+  seeds[c] = umesh_dfs(d, P, cx, f);
+}
+
+// seed grid of about one cell per four tets over the bounds (at most 256 a
+// side); an empty axis gets one cell
+int build_seed_grid(Volume *v, int ntet)
+{
+  VolumeDev &d = v->dd;
+  double ext[3], vol = 1.0;
+  int flat = 0;
+  for (int k = 0; k < 3; k++) {
+    ext[k] = (double)d.hi[k] - (double)d.lo[k];
+    if (ext[k] > 0.0) vol *= ext[k];
+    else flat++;
+  }
+  const double h = std::cbrt(vol / std::max(1.0, ntet / 4.0));
+  long long ncells = 1;
+  for (int k = 0; k < 3; k++) {
+    int n = 1;
+    if (ext[k] > 0.0 && flat == 0) n = (int)std::min(256.0, std::max(1.0, std::ceil(ext[k] / h)));
+    d.seed_dims[k] = n;
+    d.seed_origin[k] = d.lo[k];
+    d.seed_rcp[k] = ext[k] > 0.0 ? (float)n / (float)ext[k] : 0.f;
+    ncells *= n;
+  }
+  if (flat > 0) {
+    // a flat mesh holds no tet with volume: no seeds
+    d.useed = nullptr;
+    return BNX_OK;
+  }
+  BNX_HIP(v->useed.ensure(sizeof(int) * (size_t)ncells));
+  k_umesh_seeds<<<nblk(ncells, PQ_TB), PQ_TB, 0, v->dev->stream>>>(d, (int)ncells, v->useed.as<int>());
+  BNX_HIP(hipGetLastError());
+  BNX_SET(d.useed, v->useed.ptr);
+  return BNX_OK;
+}
+
 int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *verts4, int nvert)
 {
   Device *dev = v->dev;
@@ -413,6 +465,10 @@ int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *ver
   BNX_SET(d.uverts, v->uverts.ptr);
   BNX_SET(d.utets, v->utets.ptr);
   BNX_SET(d.ebvh, v->ebvh.nodes.ptr);
+  // seeds for point queries (built with the element BVH in place, before
+  // the seed pointer is set, so the build's own queries are depth first)
+  d.useed = nullptr;
+  if ((rc = build_seed_grid(v, ntet)) != BNX_OK) return rc;
   // macrocell value ranges
   const int nmc = d.mc[0] * d.mc[1] * d.mc[2];
   BNX_HIP(rlo.ensure(sizeof(int) * (size_t)nmc));
