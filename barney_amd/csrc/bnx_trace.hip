@@ -59,19 +59,6 @@ BNX_D bool accept_t(float t, const HitRec &b, int inst, int geom, int prim)
   return b.prim < 0 ? false : id_less(inst, geom, prim, b.inst, b.geom, b.prim);
 }
 
-// slab test, (lo - o) * inv per axis, conservative by 1e-5 relative on the exit
-BNX_D bool box_test(v3 o, v3 inv, float lx, float ly, float lz, float hx, float hy, float hz, float tmin, float tmax,
-                    float &tnear)
-{
-  float t0x = (lx - o.x) * inv.x, t1x = (hx - o.x) * inv.x;
-  float t0y = (ly - o.y) * inv.y, t1y = (hy - o.y) * inv.y;
-  float t0z = (lz - o.z) * inv.z, t1z = (hz - o.z) * inv.z;
-  float tn = fmx(fmx(fmx(tmin, fmn(t0x, t1x)), fmn(t0y, t1y)), fmn(t0z, t1z));
-  float tf = fmn(fmn(fmn(tmax, fmx(t0x, t1x)), fmx(t0y, t1y)), fmx(t0z, t1z));
-  tnear = tn;
-  return tn <= tf * 1.00001f;
-}
-
 // barney's triangle test (rtcore/hiprt/TraceInterface.h:38-76) on a BVH-
 // ordered record (v0|prim, e1|geom, e2); acceptance under the tie rule
 BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int inst)

@@ -335,18 +335,29 @@ def serve_setup(dev, args, n, seed=None):
     return worlds, reqs
 
 
-def serve_plain(dev, reqs, frames):
-    """one bnx_render per request; returns service times (s) and totals"""
+def serve_plain(dev, reqs, frames, per_sample=False):
+    """one bnx_render per request; returns service times (s) and totals.
+    per_sample: each request renders one pixel wave per sample, each with its
+    own generation loop, as the reference's Context::renderTiles does
+    (native/Context.cpp:154-178) -- bnx_render with one sample per batch and
+    no captured bounce loop; otherwise bnx_render merges samples into
+    batches of up to render_paths paths"""
     import barney_amd as bx
+    saved = dev.tuning()
     svc, tot = [], {}
-    for r in reqs:
-        fr = frames[(r["w"], r["h"])]
-        t = time.perf_counter()
-        fr.reset()
-        st = bx.render(r["world"], r["cam"], r["rend"], fr, r["spp"])
-        svc.append(time.perf_counter() - t)
-        for k, v in st.items():
-            tot[k] = tot.get(k, 0) + v
+    try:
+        for r in reqs:
+            if per_sample:
+                dev.set_tuning(render_paths=max(1 << 16, r["w"] * r["h"]), graph_paths=0)
+            fr = frames[(r["w"], r["h"])]
+            t = time.perf_counter()
+            fr.reset()
+            st = bx.render(r["world"], r["cam"], r["rend"], fr, r["spp"])
+            svc.append(time.perf_counter() - t)
+            for k, v in st.items():
+                tot[k] = tot.get(k, 0) + v
+    finally:
+        dev.set_tuning(**saved)
     return svc, tot
 
 
@@ -438,12 +449,18 @@ def run_c5(args, n, warmup):
         serve_plain(dev, warm, frames)
         serve_served(dev, warm)
     dev.sync()
-    # plain: service times alone, Mrays/s over the busy time
+    # plain: service times alone, Mrays/s over the busy time; samples merged
+    # into batches (bnx_render), and one pixel wave per sample (the
+    # reference's frame loop)
     if not only_served:
         svc, ptot = serve_plain(dev, reqs, frames)
         svc = np.array(svc)
         plain_mrays = ptot["rays"] / svc.sum() / 1e6
-        log(f"plain: {svc.sum():.3f} s busy, {plain_mrays:.1f} Mrays/s")
+        log(f"plain (merged samples): {svc.sum():.3f} s busy, {plain_mrays:.1f} Mrays/s")
+        svc_ps, ptot_ps = serve_plain(dev, reqs, frames, per_sample=True)
+        svc_ps = np.array(svc_ps)
+        plain_ps_mrays = ptot_ps["rays"] / svc_ps.sum() / 1e6
+        log(f"plain (one pixel wave per sample): {svc_ps.sum():.3f} s busy, {plain_ps_mrays:.1f} Mrays/s")
     # served at saturation
     _, makespan, sst = serve_served(dev, reqs)
     served_mrays = sst["rays_traced"] / makespan / 1e6
@@ -500,6 +517,12 @@ def run_c5(args, n, warmup):
                   "latency_queued_ms": {"p50": pct(lat_plain, 50), "p99": pct(lat_plain, 99)}},
         "served_vs_plain_mrays": round(served_mrays / plain_mrays, 3),
         "served_p99_vs_plain_alone_p99": round(pct(lat_served, 99) / pct(svc * 1e3, 99), 3),
+        "plain_per_sample": {"mrays_s": round(plain_ps_mrays, 3), "busy_s": round(float(svc_ps.sum()), 4),
+                             "latency_alone_ms": {"p50": pct(svc_ps * 1e3, 50), "p99": pct(svc_ps * 1e3, 99)},
+                             "form": "one pixel wave per sample with its own generation loop, as "
+                                     "Context::renderTiles (native/Context.cpp:154-178)"},
+        "served_vs_plain_per_sample_mrays": round(served_mrays / plain_ps_mrays, 3),
+        "served_p99_vs_plain_per_sample_alone_p99": round(pct(lat_served, 99) / pct(svc_ps * 1e3, 99), 3),
         "parity_sample": f"{same}/{checked} served frames identical to plain",
         "volume_samples_per_s": round(sst["volume_samples"] / makespan, 1),
     }
