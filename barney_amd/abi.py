@@ -131,7 +131,7 @@ class ServerConfig(C.Structure):
 class Tuning(C.Structure):
     _fields_ = [("raygen_tile_w", C.c_int32), ("raygen_tile_h", C.c_int32), ("trace_stack_lds", C.c_int32),
                 ("trace_blocks_per_cu", C.c_int32), ("trace_refill", C.c_int32), ("trace_walk", C.c_int32),
-                ("sort_rays", C.c_int32), ("_pad", C.c_int32),
+                ("sort_rays", C.c_int32), ("shade_block", C.c_int32),
                 ("trace_refill_prims", C.c_int64), ("render_paths", C.c_int64), ("graph_paths", C.c_int64)]
 
 
@@ -140,7 +140,7 @@ class AutotuneReport(C.Structure):
 
 
 TUNING_FIELDS = ("raygen_tile_w", "raygen_tile_h", "trace_stack_lds", "trace_blocks_per_cu", "trace_refill", "trace_walk",
-                 "sort_rays",
+                 "sort_rays", "shade_block",
                  "trace_refill_prims", "render_paths", "graph_paths")
 
 

@@ -64,11 +64,12 @@ constexpr int SORT_MIN_RAYS = 1 << 16;
 struct BatchShape {
   long long count0 = 0, cap = 0;
   bool surfaces = false, volumes = false, full = false;
-  int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8;
+  int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8, shade_block = 1024;
   bool operator<(const BatchShape &o) const
   {
-    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu) <
-           std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.walk, o.blocks_per_cu);
+    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block) <
+           std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.walk, o.blocks_per_cu,
+                    o.shade_block);
   }
 };
 
@@ -236,7 +237,7 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool sur
     BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
   }
   const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
-  return launch_shade(dev, slots, sv, sl.blocks1024, in, out, so, st);
+  return launch_shade(dev, slots, sv, sl.blocks256, sl.blocks1024, in, out, so, st);
 }
 
 // Enqueue G device-driven generations of one request's batch (slot 0, the
@@ -286,7 +287,7 @@ int enqueue_dd(Device *dev, Wavefront &F, int cur0, int G, const BatchShape &b)
       if ((rc = launch_volume_march(dev, slots, sv, blocks256, in, trace, b.full, st)) != BNX_OK) return rc;
       if ((rc = launch_stamp(dev, ts + 3)) != BNX_OK) return rc;
     }
-    if ((rc = launch_shade(dev, slots, sv, blocks1024, in, out, so, st)) != BNX_OK) return rc;
+    if ((rc = launch_shade(dev, slots, sv, blocks256, blocks1024, in, out, so, st)) != BNX_OK) return rc;
   }
   return BNX_OK;
 }
@@ -454,7 +455,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     const bool dd = dev->graph_paths > 0 && np * nS <= dev->graph_paths;
     const BatchShape shape{np * nS,          2 * np * nS,           surfaces,
                            volumes,          full,                  trace_refill_for(dev, w),
-                           dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu};
+                           dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu, dev->shade_block};
     int dd_gens = 0;
     if (dd) {
       auto it = F.gens_seen.find(shape);

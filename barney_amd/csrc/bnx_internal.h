@@ -86,6 +86,7 @@ struct Device {
   int trace_stack_lds = 16;  // LDS stack entries per lane in the queue-trace kernels (8, 16, 32)
   int trace_walk = 0;        // one-ray-per-thread walk: 0 node step or leaf per trip, 1 speculative while-while
   int sort_rays = 0;         // bnx_render: reorder secondary generations by direction and origin
+  int shade_block = 1024;    // threads per shade block (256 or 1024)
   int trace_refill = 16;    // secondary rays of large scenes: idle lanes that trigger a trace-pool refill
   long long trace_refill_prims = 1 << 20;   // "large": at least this many primitives
   int trace_blocks_per_cu = 8;   // trace launch size: blocks per CU

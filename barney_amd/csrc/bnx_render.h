@@ -183,9 +183,10 @@ int sort_rays(Device *dev, SortScratch &S, const RayQueue &src, const RayQueue &
 // fill the block -> segment maps of nsegs segments (blocks256 / blocks1024 /
 // blocksT blocks in all) and reset the segments' trace pools
 int launch_segment_blocks(Device *dev, const SegView &sv, int nsegs, int blocks256, int blocks1024, int blocksT);
-// shade every ray of the segments (blocks1024 blocks of 1024 threads),
+// shade every ray of the segments (blocks256 blocks of 256 threads or
+// blocks1024 of 1024, by the device's shade_block),
 // appending each request's shadow and continuation rays to its region of out
-int launch_shade(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks1024, const RayQueue &in,
+int launch_shade(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, int blocks1024, const RayQueue &in,
                  const RayQueue &out, const SlotOut &so, unsigned long long *stats);
 int trace_stack_capacity();
 

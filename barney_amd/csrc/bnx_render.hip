@@ -370,12 +370,14 @@ BNX_D void pack_ps(uint4 &ps, v3 tp, int ndiff, int gen, int pixel, float mis)
   ps.w = __float_as_uint(mis);
 }
 
-constexpr int SHADE_BLOCK = 1024;
-
 // Each ray carries its own generation, so one launch can shade rays of
 // different requests at different depths of their paths.  Each block lies
 // inside one request's segment: the request's parameters are block-uniform
 // (scalar registers) and the block appends to that request's output region.
+// SHADE_BLOCK is 1024 or 256 threads (a tuning field): a block of 1024 takes
+// one atomic per 1024 rays on its request's counter but fits one block per
+// CU at this kernel's register count; 256 fits more waves per SIMD.
+template <int SHADE_BLOCK>
 __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict__ slots, SegView sv, RayQueue in,
                                                        RayQueue out, SlotOut so, unsigned long long *stats)
 {
@@ -800,10 +802,14 @@ int launch_raygen(Device *dev, const RaygenChunk *chunks, int num_chunks, const 
   return BNX_OK;
 }
 
-int launch_shade(Device *dev, const SlotDev *slots, const SegView &sv, int blocks1024, const RayQueue &in,
-                 const RayQueue &out, const SlotOut &so, unsigned long long *stats)
+int launch_shade(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, int blocks1024,
+                 const RayQueue &in, const RayQueue &out, const SlotOut &so, unsigned long long *stats)
 {
-  if (blocks1024 > 0) k_shade<<<blocks1024, SHADE_BLOCK, 0, dev->stream>>>(slots, sv, in, out, so, stats);
+  if (dev->shade_block == 256) {
+    if (blocks256 > 0) k_shade<256><<<blocks256, 256, 0, dev->stream>>>(slots, sv, in, out, so, stats);
+  } else if (blocks1024 > 0) {
+    k_shade<1024><<<blocks1024, 1024, 0, dev->stream>>>(slots, sv, in, out, so, stats);
+  }
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -1,7 +1,7 @@
 // bnx_tune.hip -- launch tuning of the render path (include/bnx_rt.h,
 // bnx_tuning) and its autotuner: coordinate descent over the raygen tile,
 // the LDS share of the traversal stack, the walk order, the reordering of
-// secondary rays, the pooled trace's refill threshold
+// secondary rays, the shade block size, the pooled trace's refill threshold
 // and launch size, bnx_render's paths per batch and the largest batch run as
 // a captured bounce loop, each candidate timed on
 // probe renders of the caller's request shape.  Host code only.
@@ -29,6 +29,7 @@ bnx_tuning tuning_of(const Device *d)
   t.trace_refill = d->trace_refill;
   t.trace_walk = d->trace_walk;
   t.sort_rays = d->sort_rays;
+  t.shade_block = d->shade_block;
   t.trace_refill_prims = d->trace_refill_prims;
   t.render_paths = d->render_paths;
   t.graph_paths = d->graph_paths;
@@ -46,6 +47,7 @@ const char *tuning_error(const bnx_tuning &t)
   if (t.trace_refill < 1 || t.trace_refill > 64) return "trace_refill: 1..64";
   if (t.trace_walk != 0 && t.trace_walk != 1) return "trace_walk: 0 or 1";
   if (t.sort_rays != 0 && t.sort_rays != 1) return "sort_rays: 0 or 1";
+  if (t.shade_block != 256 && t.shade_block != 1024) return "shade_block: 256 or 1024";
   if (t.trace_refill_prims < 0) return "trace_refill_prims: >= 0";
   if (t.render_paths < (1LL << 16) || t.render_paths > (1LL << 26)) return "render_paths: 2^16 .. 2^26";
   if (t.graph_paths < 0 || t.graph_paths > (1LL << 26)) return "graph_paths: 0 .. 2^26";
@@ -61,6 +63,7 @@ void apply(Device *d, const bnx_tuning &t)
   d->trace_refill = t.trace_refill;
   d->trace_walk = t.trace_walk;
   d->sort_rays = t.sort_rays;
+  d->shade_block = t.shade_block;
   d->trace_refill_prims = t.trace_refill_prims;
   d->render_paths = t.render_paths;
   d->graph_paths = t.graph_paths;
@@ -187,6 +190,7 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
   };
   const std::vector<std::pair<int, int>> tiles = {{0, 0}, {8, 8}, {16, 4}, {16, 16}, {32, 8}};
   const std::vector<int> onoff = {0, 1};
+  const std::vector<int> shade_blocks = {1024, 256};
   const std::vector<std::pair<int, int>> stack_walks = {{8, 0}, {16, 0}, {32, 0}, {8, 1}, {16, 1}, {32, 1}};
   const std::vector<int> refills = {64, 32, 16, 8, 4};
   const std::vector<int> blocks = {2, 4, 8, 16};
@@ -208,6 +212,7 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
           },
           stack_walks);
     if (rc == BNX_OK) rc = sweep([](bnx_tuning &t, int v) { t.sort_rays = v; }, onoff);
+    if (rc == BNX_OK) rc = sweep([](bnx_tuning &t, int v) { t.shade_block = v; }, shade_blocks);
     // the refill threshold for worlds of this size: 64 keeps one ray per
     // thread for them; below 64 the pooled trace takes their secondary rays
     if (rc == BNX_OK)

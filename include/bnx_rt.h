@@ -287,7 +287,7 @@ typedef struct bnx_tuning {
                                        generation of 2^16 rays or more by shadow
                                        flag, direction octant and origin Morton
                                        code before tracing it (0: queue order)      */
-  int32_t _pad;
+  int32_t shade_block;              /* threads per shade block: 256 or 1024          */
   int64_t trace_refill_prims;       /* secondary generations of worlds with at least
                                        this many primitives use the pooled trace     */
   int64_t render_paths;             /* bnx_render: paths per batch of samples,
@@ -297,7 +297,8 @@ typedef struct bnx_tuning {
                                        hipGraph, one read-back per batch instead
                                        of one per generation (0: never; <= 2^26)    */
 } bnx_tuning;
-/* a new device starts with 8 x 8 tiles, 16 LDS entries, walk 0, no reordering, 8 blocks per CU,
+/* a new device starts with 8 x 8 tiles, 16 LDS entries, walk 0, no reordering,
+ * 1024-thread shade blocks, 8 blocks per CU,
  * refill 16 for worlds of 2^20 primitives or more, 2^23 paths per batch and
  * captured bounce loops for batches of up to 2^22 paths */
 
@@ -314,7 +315,7 @@ typedef struct bnx_autotune_report {
 
 /* Choose the device's tuning for requests like this one: coordinate descent,
  * two passes, over the fields of bnx_tuning (raygen tile, LDS stack entries,
- * walk order, ray reordering, refill threshold, pooled-launch size, paths per batch, captured-loop
+ * walk order, ray reordering, shade block, refill threshold, pooled-launch size, paths per batch, captured-loop
  * limit), each candidate timed on
  * probe renders of width x height at probe_spp samples into a scratch frame
  * (best of 2 to 20, so that each candidate is timed for about 10 ms; a

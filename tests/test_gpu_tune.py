@@ -63,6 +63,8 @@ TUNINGS = [
     dict(trace_stack_lds=32),
     dict(trace_walk=1),
     dict(sort_rays=1, graph_paths=0),
+    dict(shade_block=256),
+    dict(shade_block=256, graph_paths=1 << 26),
     dict(sort_rays=1, graph_paths=0, trace_walk=1, render_paths=1 << 16),
     dict(trace_walk=1, trace_stack_lds=8, raygen_tile_w=0, raygen_tile_h=0),
     dict(trace_refill=4, trace_refill_prims=0, trace_blocks_per_cu=1),
@@ -91,7 +93,7 @@ def test_tuning_roundtrip_and_rejects(dev, keep_tuning):
     t1 = dev.tuning()
     assert (t1["raygen_tile_w"], t1["raygen_tile_h"], t1["trace_stack_lds"], t1["render_paths"]) == (8, 8, 32, 1 << 22)
     for bad in (dict(raygen_tile_w=8, raygen_tile_h=0), dict(raygen_tile_w=300, raygen_tile_h=1),
-                dict(trace_stack_lds=12), dict(trace_blocks_per_cu=0), dict(trace_refill=65), dict(trace_walk=2), dict(sort_rays=3),
+                dict(trace_stack_lds=12), dict(trace_blocks_per_cu=0), dict(trace_refill=65), dict(trace_walk=2), dict(sort_rays=3), dict(shade_block=512),
                 dict(trace_refill_prims=-1), dict(render_paths=1000), dict(render_paths=1 << 27)):
         with pytest.raises(bx.BnxError):
             dev.set_tuning(**bad)
