@@ -52,6 +52,7 @@ _PROTOS = [
     ("bnx_group_create_instances", [vp, C.POINTER(vp), C.POINTER(C.c_int32), C.POINTER(C.c_float), C.c_int,
                                     C.POINTER(vp)], C.c_int),
     ("bnx_group_build", [vp], C.c_int),
+    ("bnx_group_set_transforms", [vp, C.POINTER(C.c_float), C.c_int], C.c_int),
     ("bnx_group_bvh_info", [vp, C.POINTER(abi.BvhInfo)], C.c_int),
     ("bnx_group_download_bvh", [vp, vp, C.POINTER(C.c_int32)], C.c_int),
     ("bnx_group_destroy", [vp], C.c_int),
@@ -253,6 +254,13 @@ class Group:
 
     def build(self):
         check(self.dev.L.bnx_group_build(self.h))
+        return self
+
+    def set_transforms(self, xfms):
+        """new object-to-world transforms (n x 12, as at creation) for every
+        instance, and the instance BVH rebuilt (bnx_group_set_transforms)"""
+        xa = np.ascontiguousarray(xfms, dtype=np.float32).reshape(-1)
+        check(self.dev.L.bnx_group_set_transforms(self.h, (C.c_float * max(1, len(xa)))(*xa), len(xa) // 12))
         return self
 
     def info(self) -> abi.BvhInfo:

@@ -4,6 +4,7 @@
 #include "bnx_render.h"
 #include "bnx_volume.h"
 
+#include <algorithm>
 #include <chrono>
 #include <cmath>
 #include <cstring>
@@ -615,6 +616,22 @@ int bnx_group_build(bnx_group g)
   BNX_HIP(hipSetDevice(g->dev->gpu));
   if (g->kind == GROUP_INSTANCES) return build_tlas(g);
   return build_blas(g);
+  GUARD_END
+}
+
+int bnx_group_set_transforms(bnx_group g, const float *xfms, int n)
+{
+  GUARD_BEGIN
+  BNX_CHECK(g && xfms, BNX_ERR_INVALID, "bnx_group_set_transforms: null argument");
+  BNX_CHECK(g->kind == GROUP_INSTANCES, BNX_ERR_INVALID, "bnx_group_set_transforms: not an instance group");
+  BNX_CHECK(n == (int)g->groups.size(), BNX_ERR_INVALID,
+            "bnx_group_set_transforms: instance count differs from the group's");
+  for (int i = 0; i < 12 * n; i++)
+    BNX_CHECK(std::isfinite(xfms[i]), BNX_ERR_INVALID, "bnx_group_set_transforms: transform not finite");
+  std::copy(xfms, xfms + 12 * (size_t)n, g->xfms.begin());
+  BNX_HIP(hipSetDevice(g->dev->gpu));
+  // refitAccel is buildAccel in the reference's CUDA, HIP and CPU backends
+  return build_tlas(g);
   GUARD_END
 }
 

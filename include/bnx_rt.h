@@ -354,6 +354,15 @@ int bnx_group_create_user(bnx_device dev, const bnx_geom *geoms, int num_geoms, 
  * affine3f: {vx.x,vx.y,vx.z, vy.x,vy.y,vy.z, vz.x,vz.y,vz.z, p.x,p.y,p.z}. */
 int bnx_group_create_instances(bnx_device dev, const bnx_group *groups, const int32_t *inst_ids,
                                const float *xfms, int num_instances, bnx_group *out);
+/* New transforms for every instance of a built instance group (12 floats
+ * each, as at creation; the count must match), then the instance BVH built
+ * again: rtc::InstanceGroup::setTransforms + refitAccel, which is buildAccel
+ * (rtcore/cuda/Group.h:21, rtcore/cpu/Group.h:17), as bnUpdateInstanceTransforms
+ * (native/barney.cu:351-361, ModelSlot::updateInstanceTransforms,
+ * native/ModelSlot.cpp:157-180) uses them.  Worlds over the group render with
+ * the new transforms from their next request; not while requests over it are
+ * in flight on a server. */
+int bnx_group_set_transforms(bnx_group instances, const float *xfms, int num_instances);
 /* rtc::Group::buildAccel: binned-SAH BVH built on the GPU. For an instance
  * group every referenced group must have been built first. */
 int bnx_group_build(bnx_group group);

@@ -225,12 +225,31 @@ def tie_spec():
     return spec
 
 
-def instanced_spec():
+INSTANCE_OFFSETS = [(0, 0, 0), (4, 0, 0), (-4, 2, 0), (0, -4, 1)]
+
+
+def affine_pair(lin, off):
+    """object-to-world (12 floats: 3x3 column-major, then translation) and
+    its inverse, for a linear part whose inverse is its transpose with
+    entries in {-1, 0, 1} (so both are exact in float32)"""
+    L = np.array(lin, np.float64).reshape(3, 3).T          # columns are the images of x, y, z
+    t = np.array(off, np.float64)
+    Li = L.T
+    ti = -Li @ t
+    o2w = np.concatenate([L.T.reshape(-1), t]).astype(np.float32)
+    w2o = np.concatenate([Li.T.reshape(-1), ti]).astype(np.float32)
+    return o2w, w2o
+
+
+def instanced_spec(xforms=None):
+    """four instances of one triangle group; xforms: (linear 3x3 column-major,
+    translation) per instance, translations only by default"""
     verts, idx, col = triangle_soup(2, 400, extent=2.0)
     spec = SceneSpec()
     grp = spec.group(0, [spec.tri(verts, idx, col, user_id=0)])
-    for k, off in enumerate([(0, 0, 0), (4, 0, 0), (-4, 2, 0), (0, -4, 1)]):
-        o2w = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, *off], np.float32)
-        w2o = np.array([1, 0, 0, 0, 1, 0, 0, 0, 1, *[-x for x in off]], np.float32)
+    ident = (1, 0, 0, 0, 1, 0, 0, 0, 1)
+    xforms = xforms or [(ident, off) for off in INSTANCE_OFFSETS]
+    for k, (lin, off) in enumerate(xforms):
+        o2w, w2o = affine_pair(lin, off)
         spec.inst(grp, 10 + k, o2w, w2o)
     return spec

@@ -92,3 +92,29 @@ def test_edge_cases(dev, orc):
     _same_hits(g.trace(rays), o.trace(rays)[0])
     # t == tmax is not a hit (rtcore/hiprt/TraceInterface.h:51 rejects t >= tMax)
     assert g.trace(rays)["prim_id"].tolist() == [0, -1, -1, -1, -1, -1]
+
+
+def test_instance_transform_update(dev, orc):
+    """bnx_group_set_transforms (bnUpdateInstanceTransforms: new transforms,
+    then the instance BVH built again): hits after the update equal those of
+    a group built from scratch with the new transforms and the oracle's,
+    bit for bit; a count mismatch or a geometry group is rejected."""
+    rays = scenes.random_rays(8, 50000, radius=9.0, spread=5.0)
+    g = helpers.instanced_spec().gpu(dev)
+    before = g.trace(rays)
+    ident = (1, 0, 0, 0, 1, 0, 0, 0, 1)
+    rot_z = (0, 1, 0, -1, 0, 0, 0, 0, 1)              # x -> y, y -> -x
+    moved = [(ident, (1, 0, 0)), (rot_z, (4, 1, 0)), (ident, (-3, 2, -1)), (rot_z, (0, -4, 2))]
+    g.set_transforms(np.stack([helpers.affine_pair(lin, off)[0] for lin, off in moved]))
+    after = g.trace(rays)
+    spec2 = helpers.instanced_spec(moved)
+    _same_hits(after, spec2.gpu(dev).trace(rays))
+    _same_hits(after, spec2.oracle(orc).trace(rays)[0])
+    assert not np.array_equal(after["prim_id"], before["prim_id"])
+    # back to the creation transforms: the first hits again
+    g.set_transforms(np.stack([helpers.affine_pair(ident, off)[0] for off in helpers.INSTANCE_OFFSETS]))
+    _same_hits(g.trace(rays), before)
+    with pytest.raises(bx.BnxError):
+        g.set_transforms(np.zeros((3, 12), np.float32))
+    with pytest.raises(bx.BnxError):
+        g._keep[1][0].set_transforms(np.zeros((1, 12), np.float32))
