@@ -60,7 +60,26 @@ def build(verbose: bool = False) -> str:
     os.replace(tmp, LIB)
     if verbose:
         print(f"built {LIB}")
+    build_host_programs(verbose)
     return LIB
+
+
+# host programs over the C-ABI (g++, no HIP headers): the C++ mirror of
+# barney's rtc interface driven as barney's native layer would
+HOST_PROGRAMS = [("tests/cpp/rtc_trace.cpp", "tests/cpp/rtc_trace")]
+
+
+def build_host_programs(verbose: bool = False):
+    for src, out in HOST_PROGRAMS:
+        dst = os.path.join(ROOT, out)
+        cmd = ["g++", "-std=c++17", "-O2", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include"),
+               os.path.join(ROOT, src), "-L", HERE, "-lbnx", "-Wl,-rpath,$ORIGIN/../../barney_amd",
+               "-Wl,-rpath-link,/opt/rocm/lib", "-o", dst]
+        r = subprocess.run(cmd, capture_output=True, text=True)
+        if r.returncode != 0:
+            raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-8000:]}")
+        if verbose:
+            print(f"built {dst}")
 
 
 if __name__ == "__main__":

@@ -359,6 +359,14 @@ extern "C" {
 const char *bnx_last_error(void) { return g_last_error.c_str(); }
 int bnx_abi_version(void) { return BNX_ABI_VERSION; }
 
+int bnx_device_count(int *count)
+{
+  BNX_CHECK(count, BNX_ERR_INVALID, "bnx_device_count: null out");
+  *count = 0;
+  BNX_HIP(hipGetDeviceCount(count));
+  return BNX_OK;
+}
+
 int bnx_device_create(int gpu_id, bnx_device *out)
 {
   GUARD_BEGIN

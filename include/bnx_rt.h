@@ -253,6 +253,8 @@ typedef struct bnx_bvh_info {
 } bnx_bvh_info;
 
 /* ------------------------------------------------------------- device */
+/* rtc::physicalDeviceCount (rtcore/cuda/Device.cu:14-19) */
+int bnx_device_count(int *count);
 /* rtc::Device(gpuID) -- rtcore/cudaCommon/Device.h, rtcore/cuda/Device.h:23-56 */
 int bnx_device_create(int gpu_id, bnx_device *out);
 int bnx_device_destroy(bnx_device dev);

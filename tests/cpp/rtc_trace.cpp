@@ -1,0 +1,134 @@
+// rtc_trace.cpp -- a scene built and traced through the C++ mirror of
+// barney's rtc interface (include/bnx_rtc.hpp), the way barney's native
+// layer drives a backend: createBuffer/upload, GeomType::createGeom,
+// setVertices/setIndices, createTrianglesGroup/createUserGeomsGroup,
+// createInstanceGroup, buildAccel, a trace launch, then setTransforms +
+// refitAccel and a second trace.
+//
+// usage: rtc_trace <in> <out>
+//   in:  int32 nv, float3[nv], int32 nt, int3[nt], float3[nt] colours,
+//        int32 ns, float3[ns] centres, float[ns] radii, float3 move,
+//        int32 nr, bnx_ray[nr]
+//   out: bnx_hit[nr] before the move, bnx_hit[nr] after it
+// The triangles form two geometries (the first and second half), the
+// spheres one; instance 0 holds the spheres' group, instance 1 the
+// triangles' group, both with user instance ID 0; the move translates
+// instance 1.  Prints "rtc_trace ok" and exits 0 on success.
+#include "bnx_rtc.hpp"
+
+#include <cstdio>
+#include <fstream>
+#include <iostream>
+#include <vector>
+
+using namespace bnx::rtc;
+
+template <typename T> static std::vector<T> read_vec(std::ifstream &f, size_t n)
+{
+  std::vector<T> v(n);
+  f.read((char *)v.data(), sizeof(T) * n);
+  if (!f) throw std::runtime_error("rtc_trace: short input");
+  return v;
+}
+
+static int32_t read_i32(std::ifstream &f) { return read_vec<int32_t>(f, 1)[0]; }
+
+int main(int argc, char **argv)
+{
+  if (argc != 3) {
+    std::cerr << "usage: rtc_trace <in> <out>" << std::endl;
+    return 2;
+  }
+  try {
+    std::ifstream in(argv[1], std::ios::binary);
+    const int nv = read_i32(in);
+    auto verts = read_vec<vec3f>(in, nv);
+    const int nt = read_i32(in);
+    auto idx = read_vec<vec3i>(in, nt);
+    auto cols = read_vec<vec3f>(in, nt);
+    const int ns = read_i32(in);
+    auto centers = read_vec<vec3f>(in, ns);
+    auto radii = read_vec<float>(in, ns);
+    auto move = read_vec<vec3f>(in, 1)[0];
+    const int nr = read_i32(in);
+    auto rays = read_vec<bnx_ray>(in, nr);
+
+    if (physicalDeviceCount() < 1) throw std::runtime_error("rtc_trace: no GPU");
+    Device dev(0);
+    GeomType *triType = dev.createGeomType(GeomType::TRIANGLES);
+    GeomType *sphType = dev.createGeomType(GeomType::SPHERES);
+
+    Buffer *vb = dev.createBuffer(sizeof(vec3f) * nv, verts.data());
+    const int half = nt / 2;
+    Buffer *ib0 = dev.createBuffer(sizeof(vec3i) * half, idx.data());
+    Buffer *ib1 = dev.createBuffer(sizeof(vec3i) * (nt - half), idx.data() + half);
+    Buffer *cb0 = dev.createBuffer(sizeof(vec3f) * half, cols.data());
+    Buffer *cb1 = dev.createBuffer(sizeof(vec3f) * (nt - half), cols.data() + half);
+    Geom *t0 = triType->createGeom();
+    t0->setVertices(vb, nv);
+    t0->setIndices(ib0, half);
+    t0->setPrimColors(cb0);
+    t0->setUserID(0);
+    Geom *t1 = triType->createGeom();
+    t1->setVertices(vb, nv);
+    t1->setIndices(ib1, nt - half);
+    t1->setPrimColors(cb1);
+    t1->setUserID(1);
+
+    // spheres: uploaded in two parts to exercise Buffer::upload at an offset
+    Buffer *sc = dev.createBuffer(sizeof(vec3f) * ns);
+    const int h2 = ns / 2;
+    sc->upload(centers.data(), sizeof(vec3f) * h2, 0);
+    sc->upload(centers.data() + h2, sizeof(vec3f) * (ns - h2), sizeof(vec3f) * h2);
+    Buffer *sr = dev.createBuffer(sizeof(float) * ns, radii.data());
+    Geom *s0 = sphType->createGeom();
+    s0->setSpheres(sc, sr, nullptr, ns, .01f);
+    s0->setUserID(2);
+
+    Group *gs = dev.createUserGeomsGroup({s0});
+    Group *gt = dev.createTrianglesGroup({t0, t1});
+    gs->buildAccel();
+    gt->buildAccel();
+    const affine3f I{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}};
+    Group *world = dev.createInstanceGroup({gs, gt}, {0, 0}, {I, I});
+    world->buildAccel();
+
+    // the reference's argument checks throw as BARNEY_RAISE does
+    bool threw = false;
+    try {
+      Group *bad = dev.createInstanceGroup({gs, gt}, {0}, {I, I});
+      dev.freeGroup(bad);
+    } catch (const std::runtime_error &) {
+      threw = true;
+    }
+    if (!threw) throw std::runtime_error("rtc_trace: mismatched createInstanceGroup did not throw");
+
+    Buffer *rb = dev.createBuffer(sizeof(bnx_ray) * nr, rays.data());
+    Buffer *hb = dev.createBuffer(sizeof(bnx_hit) * nr);
+    std::vector<bnx_hit> hits(2 * (size_t)nr);
+    traceRays(world, (const bnx_ray *)rb->getDD(), nr, (bnx_hit *)hb->getDD());
+    hb->download(hits.data(), sizeof(bnx_hit) * nr);
+
+    const affine3f M{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, move};
+    world->setTransforms({I, M});
+    world->refitAccel();
+    traceRays(world, (const bnx_ray *)rb->getDD(), nr, (bnx_hit *)hb->getDD());
+    hb->download(hits.data() + nr, sizeof(bnx_hit) * nr);
+    dev.sync();
+
+    std::ofstream out(argv[2], std::ios::binary);
+    out.write((const char *)hits.data(), sizeof(bnx_hit) * hits.size());
+    if (!out) throw std::runtime_error("rtc_trace: write failed");
+
+    for (Group *g : {world, gt, gs}) dev.freeGroup(g);
+    for (Geom *g : {t0, t1, s0}) dev.freeGeom(g);
+    for (Buffer *b : {vb, ib0, ib1, cb0, cb1, sc, sr, rb, hb}) dev.freeBuffer(b);
+    dev.freeGeomType(triType);
+    dev.freeGeomType(sphType);
+  } catch (const std::exception &e) {
+    std::cerr << e.what() << std::endl;
+    return 1;
+  }
+  std::printf("rtc_trace ok\n");
+  return 0;
+}

@@ -200,17 +200,29 @@ def c1_spec(sc):
     return spec
 
 
-def soup_spec():
+def soup_data():
+    """the triangle soup (3000 triangles) and 500 spheres of soup_spec"""
     verts, idx, col = triangle_soup(9, 3000)
-    spec = SceneSpec()
-    g0 = spec.tri(verts, idx[:1500], col[:1500], user_id=0)
-    g1 = spec.tri(verts, idx[1500:], col[1500:], user_id=1)
     rng = np.random.default_rng(1)
     cs = ((rng.random((500, 3)) - 0.5) * 4).astype(np.float32)
     rs = (rng.random(500) * 0.1 + 0.01).astype(np.float32)
+    return verts, idx, col, cs, rs
+
+
+def soup_spec(move=None):
+    """instance 0: the spheres' group; instance 1: a group of two triangle
+    geometries (the soup's halves), translated by `move` if given"""
+    verts, idx, col, cs, rs = soup_data()
+    spec = SceneSpec()
+    g0 = spec.tri(verts, idx[:1500], col[:1500], user_id=0)
+    g1 = spec.tri(verts, idx[1500:], col[1500:], user_id=1)
     g2 = spec.sph(cs, rs, None, user_id=2)
     spec.inst(spec.group(1, [g2]), 0)
-    spec.inst(spec.group(0, [g0, g1]), 0)
+    if move is None:
+        spec.inst(spec.group(0, [g0, g1]), 0)
+    else:
+        o2w, w2o = affine_pair((1, 0, 0, 0, 1, 0, 0, 0, 1), move)
+        spec.inst(spec.group(0, [g0, g1]), 0, o2w, w2o)
     return spec
 
 

@@ -47,3 +47,13 @@ def test_missing_library_fails_loudly(tmp_path, monkeypatch):
     monkeypatch.setattr(barney_amd, "_lib", None)
     with pytest.raises(RuntimeError):
         barney_amd.lib()
+
+
+def test_cpp_rtc_mirror_compiles():
+    """include/bnx_rtc.hpp (the C++ mirror of barney's rtc interface) is
+    self-contained C++17 over bnx_rt.h"""
+    import subprocess
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-I", os.path.join(root, "include"),
+                        "-x", "c++", os.path.join(root, "include", "bnx_rtc.hpp")], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
