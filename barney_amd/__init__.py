@@ -46,6 +46,7 @@ _PROTOS = [
     ("bnx_buffer_size", [vp], C.c_size_t),
     ("bnx_geom_create_triangles", [vp, C.POINTER(abi.TrianglesDesc), C.POINTER(vp)], C.c_int),
     ("bnx_geom_create_spheres", [vp, C.POINTER(abi.SpheresDesc), C.POINTER(vp)], C.c_int),
+    ("bnx_geom_create_cylinders", [vp, C.POINTER(abi.CylindersDesc), C.POINTER(vp)], C.c_int),
     ("bnx_geom_destroy", [vp], C.c_int),
     ("bnx_group_create_triangles", [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
     ("bnx_group_create_user", [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
@@ -221,6 +222,19 @@ class Geom:
                             material or abi.matte())
         h = vp()
         check(dev.L.bnx_geom_create_spheres(dev.h, C.byref(d), C.byref(h)))
+        return Geom(dev, h, None)
+
+    @staticmethod
+    def cylinders(dev: Device, vertices, indices, radii=None, colors=None, user_id=1, material=None,
+                  default_radius=0.1) -> "Geom":
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        r = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        d = abi.CylindersDesc(abi.fptr(v), abi.iptr(i), abi.fptr(r), abi.fptr(c), len(v), len(i), default_radius,
+                              user_id, material or abi.matte())
+        h = vp()
+        check(dev.L.bnx_geom_create_cylinders(dev.h, C.byref(d), C.byref(h)))
         return Geom(dev, h, None)
 
 
@@ -528,17 +542,22 @@ class Server:
 class SurfaceWorld:
     """GPU world for a barney_amd.scenes.SurfaceScene, flattened like
     native/ModelSlot.cpp:128-155: user-geometry group (spheres) first, then
-    the triangles group, both as instances of user instance 0."""
+    the triangles group, all as instances of user instance 0.  Cylinders
+    get a user group of their own, after the spheres'."""
 
     def __init__(self, dev: Device, sc):
         self.dev = dev
         self.sph = [Geom.spheres(dev, s.centers, s.radii, s.colors, user_id=s.user_id, material=s.material)
                     for s in sc.spheres]
+        self.cyl = [Geom.cylinders(dev, c.vertices, c.indices, c.radii, c.colors, user_id=c.user_id,
+                                   material=c.material) for c in getattr(sc, "cylinders", [])]
         self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
                     for m in sc.meshes]
         groups = []
         if self.sph:
             groups.append(Group.geoms(dev, USER, self.sph).build())
+        if self.cyl:
+            groups.append(Group.geoms(dev, USER, self.cyl).build())
         if self.tri:
             groups.append(Group.geoms(dev, TRIANGLES, self.tri).build())
         self.groups = groups

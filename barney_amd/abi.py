@@ -38,6 +38,12 @@ class SpheresDesc(C.Structure):
                 ("material", Material)]
 
 
+class CylindersDesc(C.Structure):
+    _fields_ = [("vertices", fp), ("indices", ip), ("radii", fp), ("colors", fp), ("num_vertices", C.c_int32),
+                ("num_cylinders", C.c_int32), ("default_radius", C.c_float), ("user_id", C.c_int32),
+                ("material", Material)]
+
+
 class Camera(C.Structure):
     _fields_ = [("type", C.c_int32), ("dir_00", f3), ("dir_du", f3), ("dir_dv", f3), ("lens_00", f3),
                 ("aperture_radius", C.c_float), ("focus_distance", C.c_float)]

@@ -28,6 +28,7 @@ struct GeomBuild {
   const float *vertices;
   const int *indices;
   const float4 *spheres;
+  const float4 *cyls;   // cylinders: (v0, radius), (v1, 0) per cylinder
 };
 
 __global__ void k_prim_boxes(const GeomBuild *gb, int num_geoms, int n, float4 *lo, float4 *hi, int *pgeom)
@@ -52,11 +53,18 @@ __global__ void k_prim_boxes(const GeomBuild *gb, int num_geoms, int n, float4 *
     }
     lo[p] = make_float4(l[0], l[1], l[2], 0.f);
     hi[p] = make_float4(h[0], h[1], h[2], 0.f);
-  } else {
+  } else if (G.type == 1) {
     // SpheresPrograms::bounds (Spheres.dev.cu:29-40)
     float4 s = G.spheres[prim];
     lo[p] = make_float4(s.x - s.w, s.y - s.w, s.z - s.w, 0.f);
     hi[p] = make_float4(s.x + s.w, s.y + s.w, s.z + s.w, 0.f);
+  } else {
+    // CylindersPrograms::bounds (native/geometry/Cylinders.dev.cu:69-85):
+    // the boxes of both end spheres
+    const float4 a = G.cyls[2 * prim], b = G.cyls[2 * prim + 1];
+    const float r = a.w;
+    lo[p] = make_float4(fmn(a.x - r, b.x - r), fmn(a.y - r, b.y - r), fmn(a.z - r, b.z - r), 0.f);
+    hi[p] = make_float4(fmx(a.x + r, b.x + r), fmx(a.y + r, b.y + r), fmx(a.z + r, b.z + r), 0.f);
   }
 }
 
@@ -76,8 +84,12 @@ __global__ void k_prim_records(const GeomBuild *gb, const int *order, const int 
     prims[3 * s + 0] = make_float4(a[0], a[1], a[2], __int_as_float(prim));
     prims[3 * s + 1] = make_float4(b[0] - a[0], b[1] - a[1], b[2] - a[2], __int_as_float(g));
     prims[3 * s + 2] = make_float4(c[0] - a[0], c[1] - a[1], c[2] - a[2], 0.f);
-  } else {
+  } else if (G.type == 1) {
     prims[s] = G.spheres[prim];
+    refs[s] = make_int2(g, prim);
+  } else {
+    prims[2 * s] = G.cyls[2 * prim];
+    prims[2 * s + 1] = G.cyls[2 * prim + 1];
     refs[s] = make_int2(g, prim);
   }
 }
@@ -110,6 +122,7 @@ int build_blas(Group *g)
     b.vertices = geom->vertices.as<float>();
     b.indices = geom->indices.as<int>();
     b.spheres = geom->spheres.as<float4>();
+    b.cyls = geom->cyls.as<float4>();
     gb.push_back(b);
     n += geom->num_prims;
   }
@@ -123,7 +136,10 @@ int build_blas(Group *g)
                                                              hi.as<float4>(), pgeom.as<int>());
   if ((rc = build_bvh(dev, lo.as<float4>(), hi.as<float4>(), n, 4, g->bvh)) != BNX_OK) return rc;
   if ((rc = collapse_bvh4(dev, g->bvh)) != BNX_OK) return rc;
-  size_t per = g->kind == GROUP_TRIANGLES ? 3 : 1;
+  // record float4s per primitive: triangles 3, spheres 1, cylinders 2 (a
+  // user group holds one user type)
+  const int utype = g->geoms.empty() ? 1 : g->geoms[0]->type;
+  size_t per = g->kind == GROUP_TRIANGLES ? 3 : (utype == 2 ? 2 : 1);
   BNX_HIP(g->prims.ensure(sizeof(float4) * per * (size_t)(n ? n : 1)));
   BNX_HIP(g->refs.ensure(sizeof(int2) * (size_t)(n ? n : 1)));
   if (n)
@@ -218,6 +234,7 @@ int build_tlas(Group *g)
         BNX_SET(gd.indices, gm->indices.as<int>());
         BNX_SET(gd.colors, gm->colors.ptr ? gm->colors.as<float>() : nullptr);
         BNX_SET(gd.spheres, gm->spheres.as<float4>());
+        BNX_SET(gd.cyls, gm->cyls.as<float4>());
         geoms.push_back(gd);
       }
       g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.stack_need4);
@@ -258,8 +275,16 @@ int build_tlas(Group *g)
     BNX_SET(blas[k].nodes, bg->bvh.nodes4.as<Node4>());
     BNX_SET(blas[k].prims, bg->prims.as<float4>());
     BNX_SET(blas[k].refs, bg->refs.as<int2>());
-    blas[k].type = bg->kind == GROUP_TRIANGLES ? BLAS_TRIANGLES : BLAS_SPHERES;
+    blas[k].type = bg->kind == GROUP_TRIANGLES
+                       ? BLAS_TRIANGLES
+                       : ((!bg->geoms.empty() && bg->geoms[0]->type == 2) ? BLAS_CYLINDERS : BLAS_SPHERES);
     blas[k].num_prims = bg->bvh.num_prims;
+  }
+  bool has_cyl = false;
+  for (const BlasDev &b : blas) has_cyl |= b.type == BLAS_CYLINDERS;
+  if (has_cyl != g->has_cyl) {
+    dev->cyl_worlds += has_cyl ? 1 : -1;
+    g->has_cyl = has_cyl;
   }
   DevMem d_lo, d_hi;
   int rc;
@@ -525,6 +550,46 @@ int bnx_geom_create_triangles(bnx_device d, const bnx_triangles_desc *desc, bnx_
   GUARD_END
 }
 
+int bnx_geom_create_cylinders(bnx_device d, const bnx_cylinders_desc *desc, bnx_geom *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && desc && out, BNX_ERR_INVALID, "bnx_geom_create_cylinders: null argument");
+  BNX_CHECK(desc->num_cylinders >= 0 && desc->num_vertices >= 0, BNX_ERR_INVALID, "negative count");
+  BNX_CHECK(desc->num_cylinders == 0 || (desc->vertices && desc->indices), BNX_ERR_INVALID,
+            "bnx_geom_create_cylinders: missing vertices or indices");
+  if (int rc = check_material(desc->material, false)) return rc;
+  std::vector<float4> rec(2 * (size_t)desc->num_cylinders);
+  for (int i = 0; i < desc->num_cylinders; i++) {
+    const int a = desc->indices[2 * i], b = desc->indices[2 * i + 1];
+    BNX_CHECK(a >= 0 && a < desc->num_vertices && b >= 0 && b < desc->num_vertices, BNX_ERR_INVALID,
+              "bnx_geom_create_cylinders: index out of range");
+    const float r = desc->radii ? desc->radii[i] : desc->default_radius;
+    const float *va = desc->vertices + 3 * (size_t)a, *vb = desc->vertices + 3 * (size_t)b;
+    rec[2 * i] = make_float4(va[0], va[1], va[2], r);
+    rec[2 * i + 1] = make_float4(vb[0], vb[1], vb[2], 0.f);
+  }
+  bnx_geom g = new bnx_geom_s;
+  g->dev = d;
+  g->type = 2;
+  g->user_id = desc->user_id;
+  set_mat(g, desc->material);
+  g->num_prims = desc->num_cylinders;
+  g->num_vertices = desc->num_vertices;
+  int rc;
+  if ((rc = upload(d, g->cyls, rec.data(), rec.size())) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  if (desc->colors && (rc = upload(d, g->colors, desc->colors, 3 * (size_t)desc->num_cylinders)) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  *out = g;
+  return BNX_OK;
+  GUARD_END
+}
+
 int bnx_geom_create_spheres(bnx_device d, const bnx_spheres_desc *desc, bnx_geom *out)
 {
   GUARD_BEGIN
@@ -575,10 +640,16 @@ static int create_geom_group(bnx_device d, const bnx_geom *geoms, int n, int kin
   g->dev = d;
   g->kind = kind;
   for (int i = 0; i < n; i++) {
-    if (!geoms[i] || geoms[i]->type != (kind == GROUP_TRIANGLES ? 0 : 1)) {
+    const bool ok = geoms[i] && (kind == GROUP_TRIANGLES ? geoms[i]->type == 0 : geoms[i]->type != 0);
+    if (!ok) {
       delete g;
       set_error("group create: geometry type does not match the group kind");
       return BNX_ERR_INVALID;
+    }
+    if (i > 0 && geoms[i]->type != geoms[0]->type) {
+      delete g;
+      set_error("group create: a user-geometry group holds one type (spheres or cylinders)");
+      return BNX_ERR_UNSUPPORTED;
     }
     g->geoms.push_back(geoms[i]);
   }
@@ -674,6 +745,7 @@ int bnx_group_destroy(bnx_group g)
 {
   BNX_CHECK(g, BNX_ERR_INVALID, "bnx_group_destroy: null group");
   (void)hipStreamSynchronize(g->dev->stream);
+  if (g->has_cyl) g->dev->cyl_worlds--;
   delete g;
   return BNX_OK;
 }

@@ -149,7 +149,7 @@ static_assert(sizeof(Node4) == 128, "node4");
 constexpr uint32_t EMPTY4 = 0x7ffffffeu;
 
 // ----------------------------------------------------------- scene data
-enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1 };
+enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1, BLAS_CYLINDERS = 2 };
 
 struct BlasDev {
   const Node4 BNX_G *nodes;
@@ -170,7 +170,7 @@ struct InstDev {
 
 // per-geometry shading data
 struct GeomDev {
-  int type;               // 0 triangles, 1 spheres
+  int type;               // 0 triangles, 1 spheres, 2 cylinders
   int user_id;
   int color_from_attr;
   int num_prims;
@@ -181,6 +181,7 @@ struct GeomDev {
   const int BNX_G *indices;      //            int3
   const float BNX_G *colors;     // per-prim float3 or null
   const float4 BNX_G *spheres;   // spheres: (center, radius)
+  const float4 BNX_G *cyls;      // cylinders: (v0, radius), (v1, 0) per cylinder
 };
 
 struct SceneDev {
@@ -204,6 +205,64 @@ BNX_D v3 xfm_vector_rows(const float4 *m, v3 v)
 {
   return v3{(m[0].x * v.x + m[0].y * v.y) + m[0].z * v.z, (m[1].x * v.x + m[1].y * v.y) + m[1].z * v.z,
             (m[2].x * v.x + m[2].y * v.y) + m[2].z * v.z};
+}
+
+// CylindersPrograms::intersect (native/geometry/Cylinders.dev.cu:107-203),
+// the capped cylinder from v0 to v1 of the given radius, in the ray's
+// object frame: the front hit t0 when it lies in [tmin, tmax], else the back
+// hit t1 when that does; *N (if asked) the unnormalized object normal of the
+// reference (for the back hit it keeps the reference's test t0 == cap_t1).
+// Written operation by operation as the reference evaluates it.
+BNX_D bool cyl_intersect(v3 ray_org, v3 ray_dir, v3 v0, v3 v1, float radius, float tmin, float tmax, float &t,
+                         v3 *N)
+{
+  const v3 d = ray_dir;
+  const v3 s = v1 - v0;
+  const v3 sxd = cross(s, d);
+  const float a = dot(sxd, sxd);
+  if (a == 0.f) return false;
+  const v3 f = v0 - ray_org;
+  const v3 sxf = cross(s, f);
+  const float ra = 1.f / a;
+  const float ts = dot(sxd, sxf) * ra;
+  const v3 fp = f - ts * d;
+  const float s2 = dot(s, s);
+  const v3 perp = cross(s, fp);
+  const float c = (radius * radius) * s2 - dot(perp, perp);
+  if (c < 0.f) return false;
+  float td = sqrtf(c * ra);
+  const float tube_t0 = ts - td;
+  const float tube_t1 = ts + td;
+  const float sf = dot(s, f);
+  const float sd = dot(s, d);
+  float cap_t0 = -1e20f, cap_t1 = 1e20f;
+  float cap_t_v0 = cap_t0;
+  if (sd == 0.f) {
+    if (dot(ray_org - v0, v1 - v0) < 0.f) return false;
+    if (dot(ray_org - v1, v0 - v1) < 0.f) return false;
+  } else {
+    const float rsd = 1.f / sd;
+    cap_t_v0 = sf * rsd;
+    const float cap_t_v1 = cap_t_v0 + s2 * rsd;
+    cap_t0 = fmn(cap_t_v0, cap_t_v1);
+    cap_t1 = fmx(cap_t_v0, cap_t_v1);
+  }
+  const float t0 = fmx(cap_t0, tube_t0);
+  const float t1 = fmn(cap_t1, tube_t1);
+  if (t0 > t1) return false;
+  bool front;
+  if (tmin <= t0 && t0 <= tmax) front = true;
+  else if (tmin <= t1 && t1 <= tmax) front = false;
+  else return false;
+  t = front ? t0 : t1;
+  if (N) {
+    if (front) td = -td;
+    const float u = ((t * sd - sf) * 1.f) / s2;
+    const bool cap = front ? (t0 == cap_t0) : (t0 == cap_t1);
+    if (cap) *N = (cap_t0 == cap_t_v0) ? v3{-s.x, -s.y, -s.z} : s;
+    else *N = (td * d - fp) - u * s;
+  }
+  return true;
 }
 
 // ------------------------------------------------------- ordered floats

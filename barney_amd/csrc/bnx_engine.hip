@@ -65,11 +65,12 @@ struct BatchShape {
   long long count0 = 0, cap = 0;
   bool surfaces = false, volumes = false, full = false;
   int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8, shade_block = 1024;
+  bool cyl = false;   // the trace kernels with the cylinder test
   bool operator<(const BatchShape &o) const
   {
-    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block) <
+    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block, cyl) <
            std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.walk, o.blocks_per_cu,
-                    o.shade_block);
+                    o.shade_block, o.cyl);
   }
 };
 
@@ -455,7 +456,8 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     const bool dd = dev->graph_paths > 0 && np * nS <= dev->graph_paths;
     const BatchShape shape{np * nS,          2 * np * nS,           surfaces,
                            volumes,          full,                  trace_refill_for(dev, w),
-                           dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu, dev->shade_block};
+                           dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu, dev->shade_block,
+                           dev->cyl_worlds > 0};
     int dd_gens = 0;
     if (dd) {
       auto it = F.gens_seen.find(shape);

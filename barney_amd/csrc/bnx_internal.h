@@ -80,6 +80,7 @@ struct Device {
   int gpu = 0;
   hipStream_t stream = nullptr;
   int num_cus = 0;
+  int cyl_worlds = 0;            // built instance groups with a cylinders BLAS (trace kernels with the cylinder test)
   int wall_clock_khz = 100000;   // rate of the device wall clock (launch timing in graphs)
   // launch tuning (bnx_device_set_tuning, bnx_autotune); none of it changes results
   int raygen_tile_w = 8, raygen_tile_h = 8;   // primary-ray pixel order: tiles (0: row-major)
@@ -104,12 +105,12 @@ struct Buffer {
 
 struct Geom {
   Device *dev;
-  int type;             // 0 triangles, 1 spheres
+  int type;             // 0 triangles, 1 spheres, 2 cylinders
   int user_id;
   bnx_material mat;
   int num_prims;
   int num_vertices;
-  DevMem vertices, indices, colors, spheres;
+  DevMem vertices, indices, colors, spheres, cyls;
 };
 
 struct Bvh {
@@ -143,6 +144,7 @@ struct Group {
   SceneDev scene{};
   int max_blas_depth = 0;    // largest traversal-stack need of its BLASes (4-wide walk)
   long long blas_prims = 0;  // primitives in the distinct BLASes it instances
+  bool has_cyl = false;      // counted in Device::cyl_worlds
 };
 
 struct Volume;

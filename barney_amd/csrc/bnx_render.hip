@@ -283,6 +283,31 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
               (I.w2o[0].y * osN.x + I.w2o[1].y * osN.y) + I.w2o[2].y * osN.z,
               (I.w2o[0].z * osN.x + I.w2o[1].z * osN.y) + I.w2o[2].z * osN.z};
     }
+  } else if (G.type == 2) {
+    // CylindersPrograms::intersect's hit attributes (Cylinders.dev.cu:
+    // 205-236): the object normal of the hit (front or back, tube or cap),
+    // the point moved 1e-4 along it, the world normal normalized twice
+    v3 o = rorg, d = fix_dir(rdir);
+    if (!I.identity) {
+      o = xfm_point_rows(I.w2o, rorg);
+      d = fix_dir(xfm_vector_rows(I.w2o, d));
+    }
+    const float4 A = G.cyls[2 * prim], B = G.cyls[2 * prim + 1];
+    float t = hu;
+    v3 objN{1.f, 0.f, 0.f};
+    (void)cyl_intersect(o, d, xyz(A), xyz(B), A.w, 0.f, hu, t, &objN);
+    v3 objP = o + hu * d;
+    objP = objP + 1e-4f * normalize(objN);
+    const v3 on = normalize(objN);
+    if (I.identity) {
+      P = objP;
+      Nw = normalize(on);
+    } else {
+      P = xfm_point_rows(I.o2w, objP);
+      Nw = normalize(v3{(I.w2o[0].x * on.x + I.w2o[1].x * on.y) + I.w2o[2].x * on.z,
+                        (I.w2o[0].y * on.x + I.w2o[1].y * on.y) + I.w2o[2].y * on.z,
+                        (I.w2o[0].z * on.x + I.w2o[1].z * on.y) + I.w2o[2].z * on.z});
+    }
   } else {
     v3 o = rorg, d = fix_dir(rdir);
     if (!I.identity) {

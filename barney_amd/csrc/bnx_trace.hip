@@ -122,6 +122,20 @@ BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec
   return true;
 }
 
+// CylindersPrograms::intersect (native/geometry/Cylinders.dev.cu:107-260)
+// on a BVH-ordered record ((v0, radius), (v1, 0)); the reference tests
+// against the ray's current closest distance, then reports the hit
+BNX_D bool cyl_test(const float4 *rec, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst)
+{
+  const float4 A = rec[0], B = rec[1];
+  float t;
+  if (!cyl_intersect(o, d, xyz(A), xyz(B), A.w, tmin, best.t, t, nullptr)) return false;
+  const int2 r = *ref;
+  if (!accept_t(t, best, inst, r.x, r.y)) return false;
+  best = HitRec{t, t, 0.f, r.y, r.x, inst};   // u: the hit distance, for the shading
+  return true;
+}
+
 struct Counts {
   uint32_t nodes, tris, spheres;
 };
@@ -179,7 +193,7 @@ BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 
 // lds: this lane's ring, &ring[0][lane]; the stack depth and the spill
 // array are locals so the depth stays in a register
-template <int SH>
+template <int SH, bool CYL>
 BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax, bool any_hit,
                       uint32_t *lds,
                       Counts &cnt)
@@ -262,10 +276,14 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       if (blas.type == BLAS_TRIANGLES) {
         cnt.tris += count;
         for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, tmin, best, inst);
-      } else {
+      } else if (!CYL || blas.type == BLAS_SPHERES) {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
           found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst);
+      } else {
+        cnt.spheres += count;
+        for (int k = 0; k < count; k++)
+          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst);
       }
       if (any_hit && found) break;
     }
@@ -288,7 +306,7 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
 // may step nodes that a hit in its pending leaf would have culled; the hit
 // returned is the same (equal t resolves by ID).  cur == EMPTY4: the stack
 // is exhausted.
-template <int SH>
+template <int SH, bool CYL>
 BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax,
                            bool any_hit,
                            uint32_t *lds, Counts &cnt)
@@ -365,10 +383,14 @@ BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *
       if (blas.type == BLAS_TRIANGLES) {
         cnt.tris += count;
         for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, tmin, best, inst);
-      } else {
+      } else if (!CYL || blas.type == BLAS_SPHERES) {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
           found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst);
+      } else {
+        cnt.spheres += count;
+        for (int k = 0; k < count; k++)
+          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst);
       }
       leaf = 0;
     } else if ((cur & LEAF_BIT) && !in_blas) {
@@ -408,7 +430,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
-    HitRec h = traverse<API_SHORT>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
+    HitRec h = traverse<API_SHORT, true>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
     bnx_hit out;
     out.t = h.t;
     out.u = h.prim >= 0 ? h.u : 0.f;
@@ -431,9 +453,9 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
 // each block lies inside one request's segment, so the request's scene
 // pointers are block-uniform.  counters: statistics array (STAT_NODES,
 // STAT_TRIS, STAT_SPHERES)
-template <int SH, bool SPEC>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(
-    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
+template <int SH, bool SPEC, bool CYL>
+BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, const TraceQueue &q,
+                        unsigned long long *counters)
 {
   __shared__ uint32_t ring[SH * TB];
   uint32_t *lds = ring + threadIdx.x;
@@ -448,8 +470,8 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
     const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
     if (S.num_insts > 0)
-      h = SPEC ? traverse_spec<SH>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt)
-               : traverse<SH>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt);
+      h = SPEC ? traverse_spec<SH, CYL>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt)
+               : traverse<SH, CYL>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
@@ -462,6 +484,24 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
   }
 }
 
+// CYL: the leaf test knows cylinder BLASes.  The kernels without it serve
+// worlds of triangles and spheres only (the leaf loop is then the same code
+// as before cylinders existed, with the same register allocation); the
+// launcher takes the _cyl kernels while any built world holds cylinders.
+template <int SH, bool SPEC>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(
+    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
+{
+  trace_static<SH, SPEC, false>(slots, sv, q, counters);
+}
+
+template <int SH, bool SPEC>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static_cyl(
+    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
+{
+  trace_static<SH, SPEC, true>(slots, sv, q, counters);
+}
+
 // The generation's rays, segment by segment: every block lies inside one
 // request's segment (its scene pointers are block-uniform) and draws rays
 // from the segment's pool.  A wave claims POOL_CHUNK ray indices at a time
@@ -471,9 +511,9 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
 // taken one node or leaf per loop trip.
 constexpr int POOL_CHUNK = 64;
 
-template <int SH>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool(
-    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
+template <int SH, bool CYL>
+BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, const TraceQueue &q,
+                      unsigned long long *counters)
 {
   __shared__ uint32_t ring[SH * TB];
   uint32_t *lds = ring + threadIdx.x;
@@ -605,10 +645,14 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
           if (blas.type == BLAS_TRIANGLES) {
             cnt.tris += count;
             for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, 0.f, best, inst);
-          } else {
+          } else if (!CYL || blas.type == BLAS_SPHERES) {
             cnt.spheres += count;
             for (int k = 0; k < count; k++)
               found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, 0.f, best, inst);
+          } else {
+            cnt.spheres += count;
+            for (int k = 0; k < count; k++)
+              found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, 0.f, best, inst);
           }
           if ((any_hit && found) || sp == 0) done = true;
           else cur = pop();
@@ -633,6 +677,36 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
   }
 }
 
+template <int SH>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool(
+    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
+{
+  trace_pool<SH, false>(slots, sv, q, counters);
+}
+
+template <int SH>
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool_cyl(
+    const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
+{
+  trace_pool<SH, true>(slots, sv, q, counters);
+}
+
+template <int SH, bool SPEC>
+void launch_static(Device *dev, bool cyl, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
+                   unsigned long long *counters)
+{
+  if (cyl) k_trace_static_cyl<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  else k_trace_static<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+}
+
+template <int SH>
+void launch_pool(Device *dev, bool cyl, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
+                 unsigned long long *counters)
+{
+  if (cyl) k_trace_pool_cyl<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  else k_trace_pool<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+}
+
 int grid_for(int n) { return n > 0 ? (n + TB - 1) / TB : 1; }
 
 } // namespace
@@ -651,24 +725,25 @@ int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int
                        const TraceQueue &q, unsigned long long *counters)
 {
   const int sh = dev->trace_stack_lds;
+  const bool cyl = dev->cyl_worlds > 0;
   if (pooled) {
     if (blocksT > 0) {
-      if (sh == 8) k_trace_pool<8><<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
-      else if (sh == 32) k_trace_pool<32><<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
-      else k_trace_pool<16><<<blocksT, TB, 0, dev->stream>>>(slots, sv, q, counters);
+      if (sh == 8) launch_pool<8>(dev, cyl, blocksT, slots, sv, q, counters);
+      else if (sh == 32) launch_pool<32>(dev, cyl, blocksT, slots, sv, q, counters);
+      else launch_pool<16>(dev, cyl, blocksT, slots, sv, q, counters);
     }
   } else {
     if (blocks256 > 0) {
       const bool spec = dev->trace_walk == 1;
       if (sh == 8) {
-        if (spec) k_trace_static<8, true><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
-        else k_trace_static<8, false><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+        if (spec) launch_static<8, true>(dev, cyl, blocks256, slots, sv, q, counters);
+        else launch_static<8, false>(dev, cyl, blocks256, slots, sv, q, counters);
       } else if (sh == 32) {
-        if (spec) k_trace_static<32, true><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
-        else k_trace_static<32, false><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+        if (spec) launch_static<32, true>(dev, cyl, blocks256, slots, sv, q, counters);
+        else launch_static<32, false>(dev, cyl, blocks256, slots, sv, q, counters);
       } else {
-        if (spec) k_trace_static<16, true><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
-        else k_trace_static<16, false><<<blocks256, TB, 0, dev->stream>>>(slots, sv, q, counters);
+        if (spec) launch_static<16, true>(dev, cyl, blocks256, slots, sv, q, counters);
+        else launch_static<16, false>(dev, cyl, blocks256, slots, sv, q, counters);
       }
     }
   }

@@ -118,6 +118,16 @@ class SphereSet:
 
 
 @dataclass
+class CylinderSet:
+    vertices: np.ndarray          # (nv,3) float32
+    indices: np.ndarray           # (n,2) int32: the two end points of each cylinder
+    radii: np.ndarray | None      # (n,) float32; None: the default radius
+    colors: np.ndarray | None     # (n,3) float32 per cylinder
+    user_id: int = 4
+    material: object = None
+
+
+@dataclass
 class EnvMap:
     """HDRI environment light: (height, width, 3) float32 radiance, row 0 at
     theta = 0 (the map's local +z), column 0 at phi = 0."""
@@ -157,6 +167,7 @@ def lobe_env_map(width: int = 1024, height: int = 512, num_lobes: int = 4, seed:
 class SurfaceScene:
     meshes: list = field(default_factory=list)
     spheres: list = field(default_factory=list)
+    cylinders: list = field(default_factory=list)
     dir_lights: list = field(default_factory=list)   # (direction(3), radiance, color(3))
     env: EnvMap | None = None
     ambient: float = 1.0
@@ -169,7 +180,8 @@ class SurfaceScene:
 
     @property
     def num_prims(self):
-        return sum(len(m.indices) for m in self.meshes) + sum(len(s.radii) for s in self.spheres)
+        return (sum(len(m.indices) for m in self.meshes) + sum(len(s.radii) for s in self.spheres) +
+                sum(len(c.indices) for c in self.cylinders))
 
 
 def config1(width: int = 512, height: int = 512, spp: int = 1, num_spheres: int = 1000,
@@ -195,6 +207,24 @@ def config1(width: int = 512, height: int = 512, spp: int = 1, num_spheres: int 
     s.camera = dict(position=(0.0, 0.0, 5.0), direction=(0.0, 0.0, -1.0), up=(0.0, 1.0, 0.0), fovy=40.0,
                     target=(0.0, 0.0, 0.0))
     return s
+
+
+def sticks(num_cylinders: int = 500, seed: int = SEEDS["c1"], extent: float = 2.0) -> CylinderSet:
+    """Random cylinders ("sticks", as a bond or streamline renderer makes
+    them): each one joins a random point to a point up to 0.6 away;
+    about a fifth share an end point with the previous one, as a polyline
+    does.  Radius U(0.01, 0.05), colour U[0.2, 0.8]^3."""
+    g = uniform_table(seed, 40, num_cylinders, 9)
+    a = (-extent + 2.0 * extent * g[:, 0:3]).astype(np.float32)
+    b = (a + 0.6 * (g[:, 3:6] - 0.5) * 2.0).astype(np.float32)
+    verts = np.concatenate([a, b]).astype(np.float32)
+    idx = np.stack([np.arange(num_cylinders), num_cylinders + np.arange(num_cylinders)], axis=1).astype(np.int32)
+    chain = g[:, 6] < 0.2
+    chain[0] = False
+    idx[chain, 0] = idx[np.nonzero(chain)[0] - 1, 1]
+    radii = (0.01 + 0.04 * g[:, 7]).astype(np.float32)
+    col = (0.2 + 0.6 * uniform_table(seed, 41, num_cylinders, 3)).astype(np.float32)
+    return CylinderSet(verts, idx, radii, col)
 
 
 # --------------------------------------------------------------------------

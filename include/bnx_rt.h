@@ -120,6 +120,22 @@ typedef struct bnx_spheres_desc {
   bnx_material   material;
 } bnx_spheres_desc;
 
+/* Cylinders: Cylinders::DD (native/geometry/Cylinders.h:20-27), capped
+ * cylinders between two vertices each, intersected as CylindersPrograms::
+ * intersect does (native/geometry/Cylinders.dev.cu:107-260).  The "color"
+ * attribute is per cylinder (not interpolated along it). */
+typedef struct bnx_cylinders_desc {
+  const float   *vertices;          /* float3[num_vertices]                        */
+  const int32_t *indices;           /* int2[num_cylinders]: end points             */
+  const float   *radii;             /* float[num_cylinders] or NULL                */
+  const float   *colors;            /* float3[num_cylinders] or NULL               */
+  int32_t        num_vertices;
+  int32_t        num_cylinders;
+  float          default_radius;
+  int32_t        user_id;
+  bnx_material   material;
+} bnx_cylinders_desc;
+
 /* Camera::DD perspective member (native/Camera.h:30-65). */
 #define BNX_CAMERA_PERSPECTIVE 1
 typedef struct bnx_camera {
@@ -344,6 +360,10 @@ int bnx_geom_create_triangles(bnx_device dev, const bnx_triangles_desc *desc, bn
 /* ... and for the 'Spheres' user type (RTC_EXPORT_USER_GEOM, Spheres.dev.cu:205). */
 int bnx_geom_create_spheres(bnx_device dev, const bnx_spheres_desc *desc, bnx_geom *out);
 /* rtc::Device::freeGeom */
+/* GeomType::createGeom + Cylinders::commit (native/geometry/Cylinders.cpp)
+ * for the 'Cylinders' user type (RTC_EXPORT_USER_GEOM(Cylinders, ...),
+ * Cylinders.dev.cu:266) */
+int bnx_geom_create_cylinders(bnx_device dev, const bnx_cylinders_desc *desc, bnx_geom *out);
 int bnx_geom_destroy(bnx_geom geom);
 
 /* -------------------------------------------------------------- groups */
@@ -558,6 +578,7 @@ static_assert(sizeof(bnx_dir_light) == 28, "bnx_dir_light layout");
 static_assert(sizeof(bnx_env_map_desc) == 48, "bnx_env_map_desc layout");
 static_assert(sizeof(bnx_triangles_desc) == 88, "bnx_triangles_desc layout");
 static_assert(sizeof(bnx_spheres_desc) == 88, "bnx_spheres_desc layout");
+static_assert(sizeof(bnx_cylinders_desc) == 96, "bnx_cylinders_desc layout");
 static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
 static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");

@@ -38,6 +38,8 @@ def lib():
         L.orc_scene_destroy.argtypes = [vp]
         L.orc_scene_add_triangles.argtypes = [vp, C.POINTER(abi.TrianglesDesc)]
         L.orc_scene_add_spheres.argtypes = [vp, C.POINTER(abi.SpheresDesc)]
+        L.orc_scene_add_cylinders.argtypes = [vp, C.POINTER(abi.CylindersDesc)]
+        L.orc_cylinder_intersect_n.argtypes = [abi.fp, C.c_int, abi.fp, C.c_float, C.c_float, abi.fp]
         L.orc_scene_add_group.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
         L.orc_scene_add_instance.argtypes = [vp, C.c_int, C.c_int, abi.fp, abi.fp]
         L.orc_scene_set_dir_lights.argtypes = [vp, C.POINTER(abi.DirLight), C.c_int]
@@ -171,6 +173,17 @@ class Scene:
         d = abi.SpheresDesc(abi.fptr(ce), abi.fptr(r), abi.fptr(c), len(ce), default_radius, user_id, 0,
                             material or abi.matte())
         return self.L.orc_scene_add_spheres(self.h, C.byref(d))
+
+    def add_cylinders(self, vertices, indices, radii=None, colors=None, user_id=1, material=None,
+                      default_radius=0.1):
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        r = None if radii is None else np.ascontiguousarray(radii, dtype=np.float32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        self._keep += [v, i, r, c]
+        d = abi.CylindersDesc(abi.fptr(v), abi.iptr(i), abi.fptr(r), abi.fptr(c), len(v), len(i), default_radius,
+                              user_id, material or abi.matte())
+        return self.L.orc_scene_add_cylinders(self.h, C.byref(d))
 
     def add_group(self, kind: int, geoms):
         g = (C.c_int * len(geoms))(*geoms)
@@ -384,16 +397,32 @@ def render(scene: Scene, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp:
                     tris=st[5], spheres=st[6])
 
 
+def cylinder_intersect(orgs, dirs, v0, v1, radius, tmin=0.0, tmax=1e30):
+    """The oracle's capped-cylinder test (Cylinders.dev.cu:107-203) for n
+    rays against one cylinder: (t, object normal); t = -1 where missed."""
+    r = np.ascontiguousarray(np.concatenate([np.asarray(orgs, np.float32).reshape(-1, 3),
+                                             np.asarray(dirs, np.float32).reshape(-1, 3)], axis=1))
+    cyl = np.ascontiguousarray(np.concatenate([np.asarray(v0, np.float32), np.asarray(v1, np.float32),
+                                               [np.float32(radius)]]).astype(np.float32))
+    out = np.zeros((len(r), 4), np.float32)
+    lib().orc_cylinder_intersect_n(abi.fptr(r), len(r), abi.fptr(cyl), tmin, tmax, abi.fptr(out))
+    return out[:, 0], out[:, 1:]
+
+
 def surface_scene(sc) -> Scene:
     """Oracle scene for a barney_amd.scenes.SurfaceScene: one user instance
-    (ID 0) flattened into a user-geometry group (spheres) and a triangles
-    group, in the order of native/ModelSlot.cpp:128-155."""
+    (ID 0) flattened into user-geometry groups (spheres, then cylinders) and
+    a triangles group, in the order of native/ModelSlot.cpp:128-155."""
     s = Scene()
     sph = [s.add_spheres(x.centers, x.radii, x.colors, user_id=x.user_id, material=x.material) for x in sc.spheres]
+    cyl = [s.add_cylinders(x.vertices, x.indices, x.radii, x.colors, user_id=x.user_id, material=x.material)
+           for x in getattr(sc, "cylinders", [])]
     tri = [s.add_triangles(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
            for m in sc.meshes]
     if sph:
         s.add_instance(s.add_group(1, sph), 0)
+    if cyl:
+        s.add_instance(s.add_group(1, cyl), 0)
     if tri:
         s.add_instance(s.add_group(0, tri), 0)
     s.set_dir_lights(sc.dir_lights)

@@ -66,6 +66,22 @@ int orc_scene_add_spheres(orc_scene *sc, const bnx_spheres_desc *d)
   return (int)sc->s.geoms.size() - 1;
 }
 
+int orc_scene_add_cylinders(orc_scene *sc, const bnx_cylinders_desc *d)
+{
+  Geom g;
+  g.type = 2;
+  g.user_id = d->user_id;
+  set_material(g.mat, d->material);
+  g.default_radius = d->default_radius;
+  for (int i = 0; i < d->num_vertices; i++) g.vertices.push_back(load3(d->vertices + 3 * i));
+  g.indices.assign(d->indices, d->indices + 2 * (size_t)d->num_cylinders);
+  if (d->radii) g.radii.assign(d->radii, d->radii + d->num_cylinders);
+  if (d->colors)
+    for (int i = 0; i < d->num_cylinders; i++) g.colors.push_back(load3(d->colors + 3 * i));
+  sc->s.geoms.push_back(std::move(g));
+  return (int)sc->s.geoms.size() - 1;
+}
+
 int orc_scene_add_group(orc_scene *sc, int type, const int *geoms, int n)
 {
   Group g;
@@ -423,6 +439,26 @@ int orc_frame_read(orc_frame *fb, int channel, void *host)
 }
 
 // --- numeric helpers for the known-answer tests
+// one capped cylinder against n rays: ray k is org (3 floats), dir (3
+// floats); cyl = v0, v1, radius; out k = t, then the object normal (0 where
+// missed); returns the number of hits
+int orc_cylinder_intersect_n(const float *rays, int n, const float *cyl, float tmin, float tmax, float *out)
+{
+  int hits = 0;
+  const vec3 v0 = load3(cyl), v1 = load3(cyl + 3);
+  for (int k = 0; k < n; k++) {
+    float t = 0.f;
+    vec3 N(0.f, 0.f, 0.f);
+    const bool h = cylinder_intersect(load3(rays + 6 * k), load3(rays + 6 * k + 3), v0, v1, cyl[6], tmin, tmax, t, &N);
+    out[4 * k] = h ? t : -1.f;
+    out[4 * k + 1] = N.x;
+    out[4 * k + 2] = N.y;
+    out[4 * k + 3] = N.z;
+    hits += h;
+  }
+  return hits;
+}
+
 void orc_sincos(float x, float *s, float *c) { bnx_num::sincos(x, *s, *c); }
 void orc_atan2_n(const float *y, const float *x, int n, float *out)
 {

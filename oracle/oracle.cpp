@@ -283,11 +283,16 @@ void build_scene(Scene &s)
         if (geom.type == 0) {
           // createTriangleBounds (rtcore/cuda/Group.cu:91-112)
           for (int k = 0; k < 3; k++) b.grow(geom.vertices[geom.indices[3 * p + k]]);
-        } else {
+        } else if (geom.type == 1) {
           // SpheresPrograms::bounds (Spheres.dev.cu:29-40)
           vec3 ctr = geom.centers[p];
           float r = geom.radii.empty() ? geom.default_radius : geom.radii[p];
           b = Box(ctr - vec3(r), ctr + vec3(r));
+        } else {
+          // CylindersPrograms::bounds (native/geometry/Cylinders.dev.cu:69-85)
+          vec3 a = geom.vertices[geom.indices[2 * p]], c = geom.vertices[geom.indices[2 * p + 1]];
+          float r = geom.radii.empty() ? geom.default_radius : geom.radii[p];
+          b = Box(vmin(a - vec3(r), c - vec3(r)), vmax(a + vec3(r), c + vec3(r)));
         }
         boxes.push_back(b);
       }
@@ -403,6 +408,63 @@ bool sphere_test(vec3 org, vec3 dir, float tmin, vec3 center, float radius, Hit 
   return true;
 }
 
+// The reference's cylinder: s the axis, f = v0 - org; the tube's roots
+// around the ray's closest approach to the axis (ts +- td), clipped to the
+// slab between the end caps; unnormalized normal of the tube (td d - f' - u s)
+// or of a cap (+-s).
+bool cylinder_test_geom(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radius, float tmin, float tmax,
+                        float &t, vec3 *N)
+{
+  const vec3 d = ray_dir;
+  const vec3 s = v1 - v0;
+  const vec3 sxd = cross(s, d);
+  const float a = dot(sxd, sxd);
+  if (a == 0.f) return false;
+  const vec3 f = v0 - ray_org;
+  const vec3 sxf = cross(s, f);
+  const float ra = 1.f / a;
+  const float ts = dot(sxd, sxf) * ra;
+  const vec3 fp = f - ts * d;
+  const float s2 = dot(s, s);
+  const vec3 perp = cross(s, fp);
+  const float c = (radius * radius) * s2 - dot(perp, perp);
+  if (c < 0.f) return false;
+  float td = sqrtf(c * ra);
+  const float tube_t0 = ts - td;
+  const float tube_t1 = ts + td;
+  const float sf = dot(s, f);
+  const float sd = dot(s, d);
+  float cap_t0 = -1e20f, cap_t1 = 1e20f;
+  float cap_t_v0 = cap_t0;
+  if (sd == 0.f) {
+    if (dot(ray_org - v0, v1 - v0) < 0.f) return false;
+    if (dot(ray_org - v1, v0 - v1) < 0.f) return false;
+  } else {
+    const float rsd = 1.f / sd;
+    cap_t_v0 = sf * rsd;
+    const float cap_t_v1 = cap_t_v0 + s2 * rsd;
+    cap_t0 = fmn(cap_t_v0, cap_t_v1);
+    cap_t1 = fmx(cap_t_v0, cap_t_v1);
+  }
+  const float t0 = fmx(cap_t0, tube_t0);
+  const float t1 = fmn(cap_t1, tube_t1);
+  if (t0 > t1) return false;
+  bool front;
+  if (tmin <= t0 && t0 <= tmax) front = true;
+  else if (tmin <= t1 && t1 <= tmax) front = false;
+  else return false;
+  t = front ? t0 : t1;
+  if (N) {
+    if (front) td = -td;
+    const float u = ((t * sd - sf) * 1.f) / s2;
+    // the reference tests t0 == cap_t1 for the back hit (Cylinders.dev.cu:195)
+    const bool cap = front ? (t0 == cap_t0) : (t0 == cap_t1);
+    if (cap) *N = (cap_t0 == cap_t_v0) ? vec3(-s.x, -s.y, -s.z) : s;
+    else *N = (td * d - fp) - u * s;
+  }
+  return true;
+}
+
 // slab test, conservative by a relative 1e-5 on the far distance
 inline bool box_test(vec3 org, vec3 inv, const float *lo, const float *hi, float tmin, float tmax, float &tnear)
 {
@@ -429,14 +491,35 @@ bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, flo
     const int32_t *ix = &geom.indices[3 * pid];
     return tri_test(org, dir, tmin, geom.vertices[ix[0]], geom.vertices[ix[1]], geom.vertices[ix[2]], best, inst,
                     gi, pid);
-  } else {
+  } else if (geom.type == 1) {
     if (ctr) ctr->spheres++;
     float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
     return sphere_test(org, dir, tmin, geom.centers[pid], r, best, inst, gi, pid);
+  } else {
+    if (ctr) ctr->spheres++;
+    float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
+    float t;
+    if (!cylinder_intersect(org, dir, geom.vertices[geom.indices[2 * pid]], geom.vertices[geom.indices[2 * pid + 1]],
+                            r, tmin, best.t, t, nullptr))
+      return false;
+    if (!accept_t(t, best, inst, gi, pid)) return false;
+    best.t = t;
+    best.u = t;
+    best.v = 0.f;
+    best.inst = inst;
+    best.geom = gi;
+    best.prim = pid;
+    return true;
   }
 }
 
 } // namespace
+
+bool cylinder_intersect(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radius, float tmin, float tmax,
+                        float &t, vec3 *N)
+{
+  return cylinder_test_geom(ray_org, ray_dir, v0, v1, radius, tmin, tmax, t, N);
+}
 
 Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool any_hit, TraceCounters *ctr)
 {

@@ -191,17 +191,26 @@ float pbr_pdf(const PbrParams &p, vec3 Ng, vec3 wo, vec3 wi);
 bool pbr_scatter(const PbrParams &p, vec3 Ng, vec3 wo, Random2 &rng, vec3 &dir, float &pdf, vec3 &f_r);
 
 struct Geom {
-  int type = 0;                     // 0 triangles, 1 spheres
+  int type = 0;                     // 0 triangles, 1 spheres, 2 cylinders
   int user_id = 0;
   Material mat;
   std::vector<vec3> vertices;
-  std::vector<int32_t> indices;     // 3 per triangle
+  std::vector<int32_t> indices;     // 3 per triangle, 2 per cylinder
   std::vector<vec3> centers;
   std::vector<float> radii;
   float default_radius = 0.1f;
   std::vector<vec3> colors;         // per prim, may be empty
-  int num_prims() const { return type == 0 ? (int)indices.size() / 3 : (int)centers.size(); }
+  int num_prims() const
+  {
+    return type == 0 ? (int)indices.size() / 3 : (type == 2 ? (int)indices.size() / 2 : (int)centers.size());
+  }
 };
+
+// CylindersPrograms::intersect (native/geometry/Cylinders.dev.cu:107-203):
+// the capped cylinder v0-v1 in the ray's object frame; the front hit when it
+// lies in [tmin, tmax], else the back hit; *N the unnormalized object normal
+bool cylinder_intersect(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radius, float tmin, float tmax,
+                        float &t, vec3 *N);
 
 struct Group {
   int type = 0;                     // 0 triangles group, 1 user group

@@ -352,6 +352,33 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
                 w.vy.x * osN.x + w.vy.y * osN.y + w.vy.z * osN.z,
                 w.vz.x * osN.x + w.vz.y * osN.y + w.vz.z * osN.z);
     }
+  } else if (geom.type == 2) {
+    // CylindersPrograms::intersect's hit attributes (native/geometry/
+    // Cylinders.dev.cu:205-236): object normal of the hit, the point moved
+    // 1e-4 along it, world normal = normalize(transformNormal(normalize(N)))
+    vec3 o = r.org, d = fix_dir(r.dir);
+    if (!in.w2o.identity) {
+      o = xfm_point(in.w2o, r.org);
+      d = fix_dir(xfm_vector(in.w2o, d));
+    }
+    vec3 v0 = geom.vertices[geom.indices[2 * h.prim]], v1 = geom.vertices[geom.indices[2 * h.prim + 1]];
+    float rad = geom.radii.empty() ? geom.default_radius : geom.radii[h.prim];
+    float t = h.u;
+    vec3 objN(1.f, 0.f, 0.f);
+    (void)cylinder_intersect(o, d, v0, v1, rad, 0.f, h.u, t, &objN);
+    vec3 objP = o + h.u * d;
+    objP = objP + 1e-4f * normalize(objN);
+    const vec3 on = normalize(objN);
+    if (in.o2w.identity) {
+      P = objP;
+      Nw = normalize(on);
+    } else {
+      P = xfm_point(in.o2w, objP);
+      const Affine &w = in.w2o;
+      Nw = normalize(vec3(w.vx.x * on.x + w.vx.y * on.y + w.vx.z * on.z,
+                          w.vy.x * on.x + w.vy.y * on.y + w.vy.z * on.z,
+                          w.vz.x * on.x + w.vz.y * on.y + w.vz.z * on.z));
+    }
   } else {
     vec3 o = r.org, d = fix_dir(r.dir);
     if (!in.w2o.identity) {

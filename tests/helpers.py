@@ -125,7 +125,8 @@ def hits_equal(a, b):
 
 class SceneSpec:
     """A scene described once and built on both sides: groups of geometries
-    (kind 0 triangles, 1 spheres) and instances (group, user id, o2w, w2o)."""
+    (kind 0 triangles, 1 spheres, 2 cylinders; a group of kind 1 holds
+    spheres or cylinders) and instances (group, user id, o2w, w2o)."""
 
     def __init__(self):
         self.geoms = []      # (kind, kwargs)
@@ -143,6 +144,11 @@ class SceneSpec:
                                    material=material)))
         return len(self.geoms) - 1
 
+    def cyl(self, vertices, indices, radii=None, colors=None, user_id=0, material=None, default_radius=0.1):
+        self.geoms.append((2, dict(vertices=vertices, indices=indices, radii=radii, colors=colors, user_id=user_id,
+                                   material=material, default_radius=default_radius)))
+        return len(self.geoms) - 1
+
     def group(self, kind, geoms):
         self.groups.append((kind, list(geoms)))
         return len(self.groups) - 1
@@ -157,9 +163,11 @@ class SceneSpec:
             if kind == 0:
                 ids.append(s.add_triangles(kw["vertices"], kw["indices"], kw["colors"], user_id=kw["user_id"],
                                            material=kw["material"]))
-            else:
+            elif kind == 1:
                 ids.append(s.add_spheres(kw["centers"], kw["radii"], kw["colors"], user_id=kw["user_id"],
                                          material=kw["material"]))
+            else:
+                ids.append(s.add_cylinders(**kw))
         gids = [s.add_group(k, [ids[g] for g in gl]) for k, gl in self.groups]
         for g, uid, o2w, w2o in self.insts:
             s.add_instance(gids[g], uid, o2w, w2o)
@@ -174,9 +182,11 @@ class SceneSpec:
             if kind == 0:
                 geoms.append(bx.Geom.triangles(dev, kw["vertices"], kw["indices"], kw["colors"],
                                                user_id=kw["user_id"], material=kw["material"]))
-            else:
+            elif kind == 1:
                 geoms.append(bx.Geom.spheres(dev, kw["centers"], kw["radii"], kw["colors"],
                                              user_id=kw["user_id"], material=kw["material"]))
+            else:
+                geoms.append(bx.Geom.cylinders(dev, **kw))
         groups = [bx.Group.geoms(dev, k, [geoms[g] for g in gl]).build() for k, gl in self.groups]
         xf = None
         if any(o is not None for _, _, o, _ in self.insts):
@@ -191,9 +201,13 @@ class SceneSpec:
 def c1_spec(sc):
     spec = SceneSpec()
     sph = [spec.sph(s.centers, s.radii, s.colors, user_id=s.user_id, material=s.material) for s in sc.spheres]
+    cyl = [spec.cyl(c.vertices, c.indices, c.radii, c.colors, user_id=c.user_id, material=c.material)
+           for c in sc.cylinders]
     tri = [spec.tri(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material) for m in sc.meshes]
     if sph:
         spec.inst(spec.group(1, sph), 0)
+    if cyl:
+        spec.inst(spec.group(1, cyl), 0)
     if tri:
         spec.inst(spec.group(0, tri), 0)
     spec.lights = list(sc.dir_lights)
@@ -251,6 +265,27 @@ def affine_pair(lin, off):
     o2w = np.concatenate([L.T.reshape(-1), t]).astype(np.float32)
     w2o = np.concatenate([Li.T.reshape(-1), ti]).astype(np.float32)
     return o2w, w2o
+
+
+def sticks_spec(rotated=False):
+    """cylinders (two geometries: sticks with and without per-cylinder radii)
+    beside the soup's spheres and triangles; with `rotated`, the cylinders'
+    group is also instanced a second time, rotated and moved"""
+    verts, idx, col, cs, rs = soup_data()
+    st = scenes.sticks(600, extent=1.8)
+    spec = SceneSpec()
+    c0 = spec.cyl(st.vertices, st.indices[:400], st.radii[:400], st.colors[:400], user_id=3)
+    c1 = spec.cyl(st.vertices, st.indices[400:], None, None, user_id=4, default_radius=0.03)
+    gs = spec.group(1, [spec.sph(cs, rs, None, user_id=2)])
+    gc = spec.group(1, [c0, c1])
+    gt = spec.group(0, [spec.tri(verts, idx[:1000], col[:1000], user_id=0)])
+    spec.inst(gs, 0)
+    spec.inst(gc, 0)
+    spec.inst(gt, 0)
+    if rotated:
+        o2w, w2o = affine_pair((0, 1, 0, 0, 0, 1, 1, 0, 0), (0.5, -0.25, 1.0))
+        spec.inst(gc, 7, o2w, w2o)
+    return spec
 
 
 def instanced_spec(xforms=None):
