@@ -10,7 +10,8 @@
  * what was uploaded (the C-ABI builds geometries from host arrays), and a
  * geometry is created in the library when its group is created.  The
  * reference's program model (SBT records, device programs per GeomType) is
- * the library's closed set of types: "Triangles" and "Spheres". */
+ * the library's closed set of types: "Triangles", "Spheres" and
+ * "Cylinders". */
 #pragma once
 
 #include "bnx_rt.h"
@@ -81,7 +82,8 @@ struct GeomType;
 
 // rtc::Geom (rtcore/cuda/Geom.h:14-45): setVertices/setIndices for
 // triangles; setDD carries a user type's data, here the Spheres::DD fields
-// (geometry/Spheres.h:15-20) as setSpheres
+// (geometry/Spheres.h:15-20) as setSpheres and the Cylinders::DD fields
+// (geometry/Cylinders.h:16-22) as setCylinders
 struct Geom {
   explicit Geom(GeomType *gt) : gt(gt) {}
   ~Geom()
@@ -108,6 +110,20 @@ struct Geom {
     primCount = numSpheres;
     this->defaultRadius = defaultRadius;
   }
+  // vertices: vec3f[numVertices]; indices: vec2i[numCylinders] (end
+  // points); radii: float[numCylinders], or null for defaultRadius
+  void setCylinders(Buffer *vertices, int numVertices, Buffer *indices, int numCylinders, Buffer *radii,
+                    Buffer *colors, float defaultRadius)
+  {
+    this->vertices = vertices;
+    this->numVertices = numVertices;
+    this->indices = indices;
+    this->numIndices = numCylinders;
+    this->radii = radii;
+    this->colors = colors;
+    primCount = numCylinders;
+    this->defaultRadius = defaultRadius;
+  }
   void setUserID(int userID) { this->userID = userID; }
   void setMaterial(const bnx_material &m) { material = m; }
   // the library's geometry, created on first use by a group
@@ -121,9 +137,10 @@ struct Geom {
   bnx_material material{BNX_MATERIAL_MATTE, 1, {.8f, .8f, .8f, 1.f}, 1.f, 1.f, 1.f, 0.f, 1.45f, 0.f};
 };
 
-// rtc::GeomType (rtcore/cuda/GeomType.h): "Triangles" or "Spheres"
+// rtc::GeomType (rtcore/cuda/GeomType.h): "Triangles", "Spheres" or
+// "Cylinders"
 struct GeomType {
-  enum Kind { TRIANGLES = 0, SPHERES = 1 };
+  enum Kind { TRIANGLES = 0, SPHERES = 1, CYLINDERS = 2 };
   GeomType(Device *device, Kind kind) : device(device), kind(kind) {}
   Geom *createGeom() { return new Geom(this); }
   Device *const device;
@@ -234,6 +251,18 @@ inline bnx_geom Geom::handle(bnx_device dev)
     d.user_id = userID;
     d.material = material;
     check(bnx_geom_create_triangles(dev, &d, &h));
+  } else if (gt->kind == GeomType::CYLINDERS) {
+    bnx_cylinders_desc d{};
+    d.vertices = vertices ? (const float *)vertices->hostData() : nullptr;
+    d.num_vertices = numVertices;
+    d.indices = indices ? (const int32_t *)indices->hostData() : nullptr;
+    d.num_cylinders = primCount;
+    d.radii = radii ? (const float *)radii->hostData() : nullptr;
+    d.colors = colors ? (const float *)colors->hostData() : nullptr;
+    d.default_radius = defaultRadius;
+    d.user_id = userID;
+    d.material = material;
+    check(bnx_geom_create_cylinders(dev, &d, &h));
   } else {
     bnx_spheres_desc d{};
     d.centers = origins ? (const float *)origins->hostData() : nullptr;

@@ -3,17 +3,20 @@
 // layer drives a backend: createBuffer/upload, GeomType::createGeom,
 // setVertices/setIndices, createTrianglesGroup/createUserGeomsGroup,
 // createInstanceGroup, buildAccel, a trace launch, then setTransforms +
-// refitAccel and a second trace.
+// refitAccel and a second trace.  Spheres and cylinders are user
+// geometries (setSpheres / setCylinders, the Spheres::DD and Cylinders::DD
+// fields).
 //
 // usage: rtc_trace <in> <out>
 //   in:  int32 nv, float3[nv], int32 nt, int3[nt], float3[nt] colours,
 //        int32 ns, float3[ns] centres, float[ns] radii, float3 move,
-//        int32 nr, bnx_ray[nr]
+//        int32 ncv, float3[ncv] cylinder vertices, int32 nc, int2[nc]
+//        end points, float[nc] radii, int32 nr, bnx_ray[nr]
 //   out: bnx_hit[nr] before the move, bnx_hit[nr] after it
 // The triangles form two geometries (the first and second half), the
-// spheres one; instance 0 holds the spheres' group, instance 1 the
-// triangles' group, both with user instance ID 0; the move translates
-// instance 1.  Prints "rtc_trace ok" and exits 0 on success.
+// spheres one, the cylinders one; instance 0 holds the spheres' group,
+// instance 1 the triangles' group, instance 2 the cylinders' group, all
+// with user instance ID 0; the move translates instance 1.  Prints "rtc_trace ok" and exits 0 on success.
 #include "bnx_rtc.hpp"
 
 #include <cstdio>
@@ -50,6 +53,11 @@ int main(int argc, char **argv)
     auto centers = read_vec<vec3f>(in, ns);
     auto radii = read_vec<float>(in, ns);
     auto move = read_vec<vec3f>(in, 1)[0];
+    const int ncv = read_i32(in);
+    auto cverts = read_vec<vec3f>(in, ncv);
+    const int nc = read_i32(in);
+    auto cidx = read_vec<int32_t>(in, 2 * (size_t)nc);
+    auto cradii = read_vec<float>(in, nc);
     const int nr = read_i32(in);
     auto rays = read_vec<bnx_ray>(in, nr);
 
@@ -57,6 +65,7 @@ int main(int argc, char **argv)
     Device dev(0);
     GeomType *triType = dev.createGeomType(GeomType::TRIANGLES);
     GeomType *sphType = dev.createGeomType(GeomType::SPHERES);
+    GeomType *cylType = dev.createGeomType(GeomType::CYLINDERS);
 
     Buffer *vb = dev.createBuffer(sizeof(vec3f) * nv, verts.data());
     const int half = nt / 2;
@@ -85,12 +94,21 @@ int main(int argc, char **argv)
     s0->setSpheres(sc, sr, nullptr, ns, .01f);
     s0->setUserID(2);
 
+    Buffer *cv = dev.createBuffer(sizeof(vec3f) * ncv, cverts.data());
+    Buffer *ci = dev.createBuffer(sizeof(int32_t) * 2 * nc, cidx.data());
+    Buffer *cr = dev.createBuffer(sizeof(float) * nc, cradii.data());
+    Geom *c0 = cylType->createGeom();
+    c0->setCylinders(cv, ncv, ci, nc, cr, nullptr, .01f);
+    c0->setUserID(3);
+
     Group *gs = dev.createUserGeomsGroup({s0});
     Group *gt = dev.createTrianglesGroup({t0, t1});
+    Group *gc = dev.createUserGeomsGroup({c0});
     gs->buildAccel();
     gt->buildAccel();
+    gc->buildAccel();
     const affine3f I{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, {0, 0, 0}};
-    Group *world = dev.createInstanceGroup({gs, gt}, {0, 0}, {I, I});
+    Group *world = dev.createInstanceGroup({gs, gt, gc}, {0, 0, 0}, {I, I, I});
     world->buildAccel();
 
     // the reference's argument checks throw as BARNEY_RAISE does
@@ -110,7 +128,7 @@ int main(int argc, char **argv)
     hb->download(hits.data(), sizeof(bnx_hit) * nr);
 
     const affine3f M{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, move};
-    world->setTransforms({I, M});
+    world->setTransforms({I, M, I});
     world->refitAccel();
     traceRays(world, (const bnx_ray *)rb->getDD(), nr, (bnx_hit *)hb->getDD());
     hb->download(hits.data() + nr, sizeof(bnx_hit) * nr);
@@ -120,11 +138,12 @@ int main(int argc, char **argv)
     out.write((const char *)hits.data(), sizeof(bnx_hit) * hits.size());
     if (!out) throw std::runtime_error("rtc_trace: write failed");
 
-    for (Group *g : {world, gt, gs}) dev.freeGroup(g);
-    for (Geom *g : {t0, t1, s0}) dev.freeGeom(g);
-    for (Buffer *b : {vb, ib0, ib1, cb0, cb1, sc, sr, rb, hb}) dev.freeBuffer(b);
+    for (Group *g : {world, gt, gs, gc}) dev.freeGroup(g);
+    for (Geom *g : {t0, t1, s0, c0}) dev.freeGeom(g);
+    for (Buffer *b : {vb, ib0, ib1, cb0, cb1, sc, sr, cv, ci, cr, rb, hb}) dev.freeBuffer(b);
     dev.freeGeomType(triType);
     dev.freeGeomType(sphType);
+    dev.freeGeomType(cylType);
   } catch (const std::exception &e) {
     std::cerr << e.what() << std::endl;
     return 1;

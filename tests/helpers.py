@@ -223,9 +223,10 @@ def soup_data():
     return verts, idx, col, cs, rs
 
 
-def soup_spec(move=None):
+def soup_spec(move=None, sticks=None):
     """instance 0: the spheres' group; instance 1: a group of two triangle
-    geometries (the soup's halves), translated by `move` if given"""
+    geometries (the soup's halves), translated by `move` if given; with
+    `sticks` (a CylinderSet), instance 2: its cylinders (user ID 3)"""
     verts, idx, col, cs, rs = soup_data()
     spec = SceneSpec()
     g0 = spec.tri(verts, idx[:1500], col[:1500], user_id=0)
@@ -237,6 +238,8 @@ def soup_spec(move=None):
     else:
         o2w, w2o = affine_pair((1, 0, 0, 0, 1, 0, 0, 0, 1), move)
         spec.inst(spec.group(0, [g0, g1]), 0, o2w, w2o)
+    if sticks is not None:
+        spec.inst(spec.group(1, [spec.cyl(sticks.vertices, sticks.indices, sticks.radii, None, user_id=3)]), 0)
     return spec
 
 

@@ -3,8 +3,8 @@
 tests/cpp/rtc_trace (built by build()) drives the mirror the way barney's
 native layer drives a backend -- buffers, GeomType::createGeom,
 setVertices/setIndices, triangle and user-geometry groups, an instance
-group, buildAccel, a trace, then setTransforms + refitAccel and a second
-trace -- and checks that a mismatched createInstanceGroup throws.  Its hits
+group (spheres, triangles and cylinders), buildAccel, a trace, then
+setTransforms + refitAccel and a second trace -- and checks that a mismatched createInstanceGroup throws.  Its hits
 must equal, bit for bit, those of the same scene built through the Python
 binding of the C-ABI and the oracle's, before and after the move."""
 import os
@@ -31,22 +31,26 @@ def test_rtc_mirror_traces_like_the_c_abi(dev, orc, tmp_path):
     assert os.access(EXE, os.X_OK), f"{EXE} not built (run __graft_entry__.build())"
     verts, idx, col, cs, rs = helpers.soup_data()
     move = np.array([0.5, -0.25, 0.0], np.float32)
+    st = scenes.sticks(300, extent=1.8)
     rays = np.ascontiguousarray(scenes.random_rays(4, 50000, radius=5.0, spread=2.0), np.float32)
     inp, out = tmp_path / "scene.bin", tmp_path / "hits.bin"
     with open(inp, "wb") as f:
         for a in (np.int32(len(verts)), verts, np.int32(len(idx)), idx.astype(np.int32), col,
-                  np.int32(len(cs)), cs, rs, move, np.int32(len(rays)), rays):
+                  np.int32(len(cs)), cs, rs, move, np.int32(len(st.vertices)), st.vertices,
+                  np.int32(len(st.indices)), st.indices.astype(np.int32), st.radii, np.int32(len(rays)), rays):
             f.write(np.ascontiguousarray(a).tobytes())
-    print(f"rtc_trace: {len(idx)} triangles, {len(cs)} spheres, {len(rays)} rays", flush=True)
+    print(f"rtc_trace: {len(idx)} triangles, {len(cs)} spheres, {len(st.indices)} cylinders, {len(rays)} rays",
+          flush=True)
     r = subprocess.run([EXE, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "rtc_trace ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
     hits = np.fromfile(out, dtype=abi.HIT_DTYPE)
     assert len(hits) == 2 * len(rays)
     before, after = hits[: len(rays)], hits[len(rays):]
-    spec = helpers.soup_spec()
+    spec = helpers.soup_spec(sticks=st)
     _same(before, spec.gpu(dev).trace(rays))
     _same(before, spec.oracle(orc).trace(rays)[0])
-    moved = helpers.soup_spec(move=tuple(float(x) for x in move))
+    moved = helpers.soup_spec(move=tuple(float(x) for x in move), sticks=st)
     _same(after, moved.gpu(dev).trace(rays))
     _same(after, moved.oracle(orc).trace(rays)[0])
     assert not np.array_equal(before["prim_id"], after["prim_id"])
+    assert (before["obj_id"] == 3).sum() > 1000
