@@ -69,9 +69,11 @@ def profile_traffic(workload, kernel):
         return None
     with open(path) as f:
         for line in f:
-            parts = line.split()
-            if len(parts) == 3 and parts[0] == "traffic_bytes_per_launch" and parts[1] == kernel:
-                return float(parts[2])
+            # "traffic_bytes_per_launch <kernel> <bytes>"; kernel names may hold spaces
+            key, _, rest = line.strip().partition(" ")
+            name, _, value = rest.rpartition(" ")
+            if key == "traffic_bytes_per_launch" and name == kernel:
+                return float(value)
     return None
 
 

@@ -57,3 +57,18 @@ def test_cpp_rtc_mirror_compiles():
     r = subprocess.run(["g++", "-std=c++17", "-fsyntax-only", "-Wall", "-Wextra", "-I", os.path.join(root, "include"),
                         "-x", "c++", os.path.join(root, "include", "bnx_rtc.hpp")], capture_output=True, text=True)
     assert r.returncode == 0, r.stderr
+
+
+def test_bench_reads_profile_traffic_of_every_workload():
+    # the roofline's "traffic" comes from the committed rocprof summaries;
+    # trace kernel names hold a space ("k_trace_static<16, false>")
+    import sys
+    root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+    sys.path.insert(0, root)
+    import bench
+    for w, k in [("c1", "k_trace_static<32, true>"), ("c2", "k_volume_march<false>"),
+                 ("c3", "k_trace_static<16, false>"), ("c4", "k_volume_march<true>"),
+                 ("c5", "k_trace_static<16, false>")]:
+        t = bench.profile_traffic(w, k)
+        assert t is not None and t > 0, (w, k)
+    assert bench.profile_traffic("c3", "k_no_such_kernel") is None
