@@ -104,3 +104,37 @@ def test_cylinder_argument_checks(dev):
     w = bx.Group.instances(dev, [g], [0]).build()
     h = w.trace(scenes.random_rays(3, 1000))
     assert np.all(h["prim_id"] == -1)
+
+
+def test_served_cylinder_world(dev, orc):
+    """requests on a world with cylinders (bx.SurfaceWorld: spheres, then
+    cylinders, then triangles) served next to one without: each equals its
+    render alone bit for bit, and the 1-spp one matches the oracle's
+    surface_scene (the same flattening)"""
+    sc = scenes.config1(width=64, height=64)
+    sc.cylinders.append(scenes.sticks(400, extent=1.8))
+    gw = bx.SurfaceWorld(dev, sc)
+    plain_sc = scenes.config1(width=64, height=64)
+    pw = bx.SurfaceWorld(dev, plain_sc)
+    cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
+    rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient)
+    reqs = [(gw.world, 64, 64, 1), (pw.world, 48, 40, 2), (gw.world, 40, 56, 3)]
+    srv = bx.Server(dev, max_paths_in_flight=4000)
+    tickets = [srv.submit(w, cam, rend, wd, ht, spp) for w, wd, ht, spp in reqs]
+    srv.run_until_idle()
+    chans = (abi.BNX_FB_COLOR, abi.BNX_FB_DEPTH, abi.BNX_FB_PRIMID, abi.BNX_FB_INSTID, abi.BNX_FB_OBJID)
+    for t, (w, wd, ht, spp) in zip(tickets, reqs):
+        fr = bx.Frame(dev, wd, ht)
+        bx.render(w, cam, rend, fr, spp)
+        for ch in chans:
+            assert np.array_equal(srv.read(t, ch).view(np.uint8), fr.read(ch).view(np.uint8)), (wd, ht, ch)
+        fr.close()
+    o = orc.surface_scene(sc)
+    of = orc.Frame(64, 64)
+    orc.render(o, orc.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"],
+                                         sc.camera["fovy"]), rend, of, 1)
+    for ch in (abi.BNX_FB_PRIMID, abi.BNX_FB_INSTID, abi.BNX_FB_OBJID):
+        assert np.array_equal(srv.read(tickets[0], ch), of.read(ch)), ch
+    assert (of.read(abi.BNX_FB_OBJID) == 4).sum() > 200
+    assert np.abs(srv.read(tickets[0], abi.BNX_FB_COLOR) - of.read(abi.BNX_FB_COLOR)).max() <= 1e-3
+    srv.close()
