@@ -164,7 +164,7 @@ int bnx_autotune(bnx_world world, const bnx_camera *camera, const bnx_renderer *
   double warm = 0.0;
   int rc = probe_ms(w, T.cam, T.rr, &f, probe_spp, warm);
   // short probes are timed more often: at least about 10 ms per candidate,
-  // judged on a third render (the second of a batch shape captures its
+  // sized from a third render (the second of a batch shape captures its
   // bounce loop)
   for (int k = 0; k < 2 && rc == BNX_OK; k++) rc = probe_ms(w, T.cam, T.rr, &f, probe_spp, warm);
   T.reps = std::max(2, std::min(20, (int)std::ceil(10.0 / std::max(1e-3, warm))));
