@@ -38,6 +38,14 @@ _PROTOS = [
     ("bnx_device_destroy", [vp], C.c_int),
     ("bnx_device_sync", [vp], C.c_int),
     ("bnx_device_mem_info", [vp, C.POINTER(C.c_size_t), C.POINTER(C.c_size_t)], C.c_int),
+    ("bnx_mem_alloc", [vp, C.c_size_t, C.POINTER(vp)], C.c_int),
+    ("bnx_mem_free", [vp, vp], C.c_int),
+    ("bnx_host_alloc", [vp, C.c_size_t, C.POINTER(vp)], C.c_int),
+    ("bnx_host_free", [vp, vp], C.c_int),
+    ("bnx_mem_copy_async", [vp, vp, vp, C.c_size_t], C.c_int),
+    ("bnx_mem_set_async", [vp, vp, C.c_int, C.c_size_t], C.c_int),
+    ("bnx_device_set_active", [vp, C.POINTER(C.c_int)], C.c_int),
+    ("bnx_device_restore_active", [C.c_int], C.c_int),
     ("bnx_buffer_create", [vp, C.c_size_t, vp, C.POINTER(vp)], C.c_int),
     ("bnx_buffer_upload", [vp, vp, C.c_size_t, C.c_size_t], C.c_int),
     ("bnx_buffer_download", [vp, vp, C.c_size_t, C.c_size_t], C.c_int),

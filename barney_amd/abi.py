@@ -170,7 +170,7 @@ class ServerStats(C.Structure):
                 ("tri_tests", C.c_int64), ("sphere_tests", C.c_int64), ("mc_cells", C.c_int64)]
 
 
-BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE = 0, 1, 2
+BNX_REQ_QUEUED, BNX_REQ_ACTIVE, BNX_REQ_DONE, BNX_REQ_FAILED = 0, 1, 2, 3
 
 assert C.sizeof(Material) == 48 and C.sizeof(Camera) == 60 and C.sizeof(DirLight) == 28
 assert C.sizeof(Tuning) == 56 and C.sizeof(AutotuneReport) == 24
@@ -218,11 +218,13 @@ def pbr(base_color=(1.0, 1.0, 1.0, 1.0), from_attribute=True, metallic=1.0, roug
     return m
 
 
-def renderer(bg=(0.0, 0.0, 0.0, 1.0), ambient=1.0, flags=0, max_generations=0) -> Renderer:
+def renderer(bg=(0.0, 0.0, 0.0, 1.0), ambient=1.0, flags=0, max_generations=0, cut_plane=None) -> Renderer:
+    """cut_plane (nx, ny, nz, w): the renderer's 'cutPlane'; surface hits with
+    dot(n, P) + w < 0 are invisible (None: no cut plane)"""
     r = Renderer()
     r.bg_color = f4(*bg)
     r.ambient_radiance = ambient
-    r.cut_plane = f4(0.0, 0.0, 0.0, -1e30)
+    r.cut_plane = f4(0.0, 0.0, 0.0, -1e30) if cut_plane is None else f4(*[float(x) for x in cut_plane])
     r.flags = flags
     r.max_generations = max_generations
     return r

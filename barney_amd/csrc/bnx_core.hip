@@ -198,6 +198,10 @@ bool is_identity(const float *a)
 int build_tlas(Group *g)
 {
   Device *dev = g->dev;
+  // the group is rebuilt in place: it counts as unbuilt (traces, renders and
+  // new worlds refuse it) until every step below, the stack-depth check
+  // included, has succeeded
+  g->built = false;
   const int n = (int)g->groups.size();
   std::vector<Group *> blas_groups;
   std::vector<InstDev> insts(n);
@@ -440,6 +444,83 @@ int bnx_device_mem_info(bnx_device d, size_t *free_bytes, size_t *total_bytes)
   BNX_CHECK(d && free_bytes && total_bytes, BNX_ERR_INVALID, "bnx_device_mem_info: null argument");
   BNX_HIP(hipSetDevice(d->gpu));
   BNX_HIP(hipMemGetInfo(free_bytes, total_bytes));
+  return BNX_OK;
+}
+
+// ------------------------------------------------------- device memory
+int bnx_mem_alloc(bnx_device d, size_t bytes, void **d_ptr)
+{
+  BNX_CHECK(d && d_ptr, BNX_ERR_INVALID, "bnx_mem_alloc: null argument");
+  *d_ptr = nullptr;
+  BNX_HIP(hipSetDevice(d->gpu));
+  const hipError_t e = hipMalloc(d_ptr, bytes ? bytes : 16);
+  if (e != hipSuccess) {
+    *d_ptr = nullptr;
+    (void)hipGetLastError();
+    set_error(std::string("bnx_mem_alloc: ") + hipGetErrorString(e));
+    return BNX_ERR_OOM;
+  }
+  return BNX_OK;
+}
+
+int bnx_mem_free(bnx_device d, void *d_ptr)
+{
+  BNX_CHECK(d, BNX_ERR_INVALID, "bnx_mem_free: null device");
+  if (!d_ptr) return BNX_OK;
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  BNX_HIP(hipFree(d_ptr));
+  return BNX_OK;
+}
+
+int bnx_host_alloc(bnx_device d, size_t bytes, void **h_ptr)
+{
+  BNX_CHECK(d && h_ptr, BNX_ERR_INVALID, "bnx_host_alloc: null argument");
+  *h_ptr = nullptr;
+  const hipError_t e = hipHostMalloc(h_ptr, bytes ? bytes : 16, hipHostMallocDefault);
+  if (e != hipSuccess) {
+    *h_ptr = nullptr;
+    (void)hipGetLastError();
+    set_error(std::string("bnx_host_alloc: ") + hipGetErrorString(e));
+    return BNX_ERR_OOM;
+  }
+  return BNX_OK;
+}
+
+int bnx_host_free(bnx_device d, void *h_ptr)
+{
+  BNX_CHECK(d, BNX_ERR_INVALID, "bnx_host_free: null device");
+  if (!h_ptr) return BNX_OK;
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  BNX_HIP(hipHostFree(h_ptr));
+  return BNX_OK;
+}
+
+int bnx_mem_copy_async(bnx_device d, void *dst, const void *src, size_t bytes)
+{
+  BNX_CHECK(d && (bytes == 0 || (dst && src)), BNX_ERR_INVALID, "bnx_mem_copy_async: null argument");
+  if (bytes) BNX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, d->stream));
+  return BNX_OK;
+}
+
+int bnx_mem_set_async(bnx_device d, void *d_ptr, int value, size_t bytes)
+{
+  BNX_CHECK(d && (bytes == 0 || d_ptr), BNX_ERR_INVALID, "bnx_mem_set_async: null argument");
+  if (bytes) BNX_HIP(hipMemsetAsync(d_ptr, value, bytes, d->stream));
+  return BNX_OK;
+}
+
+int bnx_device_set_active(bnx_device d, int *previous_gpu)
+{
+  BNX_CHECK(d && previous_gpu, BNX_ERR_INVALID, "bnx_device_set_active: null argument");
+  BNX_HIP(hipGetDevice(previous_gpu));
+  BNX_HIP(hipSetDevice(d->gpu));
+  return BNX_OK;
+}
+
+int bnx_device_restore_active(int previous_gpu)
+{
+  if (previous_gpu < 0) return BNX_OK;
+  BNX_HIP(hipSetDevice(previous_gpu));
   return BNX_OK;
 }
 
@@ -1147,6 +1228,8 @@ int bnx_render(bnx_world w, const bnx_camera *cam, const bnx_renderer *r, bnx_fr
   GUARD_BEGIN
   BNX_CHECK(w && cam && r && f && spp >= 0, BNX_ERR_INVALID, "bnx_render: bad argument");
   BNX_CHECK(f->dev == w->dev, BNX_ERR_INVALID, "bnx_render: frame and world live on different devices");
+  BNX_CHECK(w->instances && w->instances->built, BNX_ERR_STATE,
+            "bnx_render: the world's instance group is not built (a rebuild failed)");
   BNX_CHECK(cam->type == BNX_CAMERA_PERSPECTIVE && cam->aperture_radius == 0.f, BNX_ERR_UNSUPPORTED,
             "bnx_render: only the pinhole perspective camera is provided");
   return render_frame(w, *cam, *r, f, spp, stats);

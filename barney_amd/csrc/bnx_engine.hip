@@ -65,12 +65,12 @@ struct BatchShape {
   long long count0 = 0, cap = 0;
   bool surfaces = false, volumes = false, full = false;
   int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8, shade_block = 1024;
-  bool cyl = false;   // the trace kernels with the cylinder test
+  bool ext = false;   // the trace kernels with the cylinder test and the cut plane
   bool operator<(const BatchShape &o) const
   {
-    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block, cyl) <
+    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block, ext) <
            std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.walk, o.blocks_per_cu,
-                    o.shade_block, o.cyl);
+                    o.shade_block, o.ext);
   }
 };
 
@@ -227,9 +227,13 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool sur
   // with neither surfaces nor volumes the trace kernel still writes the misses
   const bool trace = surfaces || !volumes;
   BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
-  bool pooled = false;
-  for (int k = 0; k < sl.n; k++) pooled = pooled || sl.v[k].refill < 64;
-  if (trace && (rc = launch_trace_queue(dev, slots, sv, sl.blocks256, sl.blocksT, pooled, tq, st)) != BNX_OK)
+  bool pooled = false, cut = false;
+  const SlotDev *hs = F.h_slots.as<SlotDev>();
+  for (int k = 0; k < sl.n; k++) {
+    pooled = pooled || sl.v[k].refill < 64;
+    cut = cut || has_cut_plane(hs[sl.v[k].slot].rr);
+  }
+  if (trace && (rc = launch_trace_queue(dev, slots, sv, sl.blocks256, sl.blocksT, pooled, cut, tq, st)) != BNX_OK)
     return rc;
   BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
   if (volumes) {
@@ -280,7 +284,7 @@ int enqueue_dd(Device *dev, Wavefront &F, int cur0, int G, const BatchShape &b)
     unsigned long long *ts = F.stamps.as<unsigned long long>() + 4 * g;
     if (trace) {
       if ((rc = launch_stamp(dev, ts)) != BNX_OK) return rc;
-      if ((rc = launch_trace_queue(dev, slots, sv, blocks256, blocksT, pooled, tq, st)) != BNX_OK) return rc;
+      if ((rc = launch_trace_queue(dev, slots, sv, blocks256, blocksT, pooled, b.ext, tq, st)) != BNX_OK) return rc;
       if ((rc = launch_stamp(dev, ts + 1)) != BNX_OK) return rc;
     }
     if (b.volumes) {
@@ -457,7 +461,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     const BatchShape shape{np * nS,          2 * np * nS,           surfaces,
                            volumes,          full,                  trace_refill_for(dev, w),
                            dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu, dev->shade_block,
-                           dev->cyl_worlds > 0};
+                           dev->cyl_worlds > 0 || has_cut_plane(rr)};
     int dd_gens = 0;
     if (dd) {
       auto it = F.gens_seen.find(shape);
@@ -605,7 +609,7 @@ struct Server {
   int cur = 0;
   int in_end = 0;                  // end of the output regions laid out in the current queue
   int64_t next_ticket = 1;
-  int64_t submitted = 0, admitted = 0, done = 0, released = 0, rays = 0, gens = 0, peak = 0;
+  int64_t submitted = 0, admitted = 0, done = 0, released = 0, failed = 0, rays = 0, gens = 0, peak = 0;
   double busy_ms = 0.0;
   GenTiming tm;
   std::chrono::steady_clock::time_point t0 = std::chrono::steady_clock::now();
@@ -685,7 +689,18 @@ struct Server {
       r->slot = free_slots.back();
       free_slots.pop_back();
       SlotDev *hs = F.h_slots.as<SlotDev>() + r->slot;
-      if ((rc = fill_slot(r->world, r->cam, r->rr, f, *hs, r->full)) != BNX_OK) return rc;
+      if ((rc = fill_slot(r->world, r->cam, r->rr, f, *hs, r->full)) != BNX_OK) {
+        // the request fails: its slot and its frame go back, it leaves the
+        // queue, and its ticket reads BNX_REQ_FAILED until released
+        free_slots.push_back(r->slot);
+        r->slot = -1;
+        pool.insert({{f->w, f->h}, f});
+        r->frame = nullptr;
+        r->state = BNX_REQ_FAILED;
+        r->done_ms = now_ms();
+        failed++;
+        return rc;
+      }
       hs->accum_base = 0;
       BNX_HIP(hipMemcpyAsync(F.slots.as<SlotDev>() + r->slot, hs, sizeof(SlotDev), hipMemcpyHostToDevice,
                              dev->stream));
@@ -941,8 +956,9 @@ int bnx_server_release(bnx_server s, int64_t ticket)
   auto it = s->reqs.find(ticket);
   BNX_CHECK(it != s->reqs.end(), BNX_ERR_INVALID, "bnx_server_release: unknown ticket");
   Request *r = it->second;
-  BNX_CHECK(r->state == BNX_REQ_DONE, BNX_ERR_INVALID, "bnx_server_release: request not done");
-  s->pool.insert({{r->W, r->H}, r->frame});
+  BNX_CHECK(r->state == BNX_REQ_DONE || r->state == BNX_REQ_FAILED, BNX_ERR_INVALID,
+            "bnx_server_release: request not done");
+  if (r->frame) s->pool.insert({{r->W, r->H}, r->frame});
   s->reqs.erase(it);
   delete r;
   s->released++;

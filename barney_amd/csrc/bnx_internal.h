@@ -201,8 +201,9 @@ struct SlotDev;
 struct SegView;
 // trace every ray of the generation's segments: one ray per thread
 // (blocks256 blocks of 256 threads) or, pooled, blocksT blocks drawing rays
-// from their segment's pool
+// from their segment's pool; cut: a request in the launch has a cut plane
+// (the launch then takes the kernels that apply it)
 int launch_trace_queue(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, int blocksT,
-                       bool pooled, const TraceQueue &q, unsigned long long *counters);
+                       bool pooled, bool cut, const TraceQueue &q, unsigned long long *counters);
 
 } // namespace bnx

@@ -60,6 +60,13 @@ struct SlotDev {
   EnvDev env;                  // HDRI light (env.tex null: ambient radiance)
 };
 
+// the renderer's cut plane (nx, ny, nz, w; off when w <= -1e28)
+BNX_D float4 slot_cut_plane(const SlotDev &s)
+{
+  return make_float4(s.rr.cut_plane[0], s.rr.cut_plane[1], s.rr.cut_plane[2], s.rr.cut_plane[3]);
+}
+inline bool has_cut_plane(const bnx_renderer &r) { return r.cut_plane[3] > -1e28f; }
+
 // the device view of a world's env map
 EnvDev env_dev(const World *w);
 

@@ -20,7 +20,12 @@
 //   * primitive programs are a closed type switch (triangles / spheres)
 //     instead of SBT function pointers (rtcore/cuda/Geom.h:21-36);
 //   * ties in t go to the lower (instance, geometry, primitive) ID, and a
-//     box whose entry distance equals the current best is still visited.
+//     box whose entry distance equals the current best is still visited;
+//   * the extended kernels (EXT) add the cylinder leaf test and the
+//     renderer's cut plane, which rejects surface hits on its negative side
+//     in every trace (OptixGlobals::hitOnInvisibleSide, render/
+//     OptixGlobals.h:36-45); worlds without cylinders traced without a cut
+//     plane keep the plain kernels and their register allocation.
 #include "bnx_internal.h"
 #include "bnx_render.h"
 
@@ -59,9 +64,31 @@ BNX_D bool accept_t(float t, const HitRec &b, int inst, int geom, int prim)
   return b.prim < 0 ? false : id_less(inst, geom, prim, b.inst, b.geom, b.prim);
 }
 
+// The cut plane (nx, ny, nz, w; off when w <= -1e28) and the ray's world
+// frame (its queue record: origin at rorg, direction at rdir).  Every
+// surface program of the reference rejects a hit at t whose world position
+// org + t dir lies on the plane's negative side (OptixGlobals::
+// hitOnInvisibleSide, render/OptixGlobals.h:36-45; Triangles.dev.cu:39,
+// Spheres.dev.cu:170, Cylinders.dev.cu:258).  t is the same in the object
+// frame of an instance, whose transform is affine.
+struct Cut {
+  float4 plane;
+  const float4 *rorg, *rdir;
+};
+BNX_D Cut no_cut() { return Cut{make_float4(0.f, 0.f, 0.f, -1e30f), nullptr, nullptr}; }
+template <bool EXT> BNX_D bool cut_rejects(const Cut &c, float t)
+{
+  if (!EXT || c.plane.w <= -1e28f) return false;
+  const v3 o = xyz(*c.rorg);
+  const v3 d = fix_dir(xyz(*c.rdir));
+  const v3 P = o + t * d;
+  return dot(v3{c.plane.x, c.plane.y, c.plane.z}, P) + c.plane.w < 0.f;
+}
+
 // barney's triangle test (rtcore/hiprt/TraceInterface.h:38-76) on a BVH-
 // ordered record (v0|prim, e1|geom, e2); acceptance under the tie rule
-BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int inst)
+template <bool EXT>
+BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int inst, const Cut &cut)
 {
   float4 A = rec[0], B = rec[1], C = rec[2];
   v3 v0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
@@ -89,12 +116,14 @@ BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int
   float u = (Pu * e2v - Pv * e2u) / den;
   float v = (e1u * Pv - e1v * Pu) / den;
   if (u < 0.f || v < 0.f || (u + v) >= 1.f) return false;
+  if (cut_rejects<EXT>(cut, t)) return false;
   best = HitRec{t, u, v, prim, geom, inst};
   return true;
 }
 
 // SpheresPrograms::intersect (Spheres.dev.cu:114-166), "move the origin"
-BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst)
+template <bool EXT>
+BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst, const Cut &cut)
 {
   v3 center = xyz(S);
   float radius = S.w;
@@ -118,6 +147,7 @@ BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec
   if (t > best.t) return false;
   int2 r = *ref;
   if (!accept_t(t, best, inst, r.x, r.y)) return false;
+  if (cut_rejects<EXT>(cut, t)) return false;
   best = HitRec{t, cand, 0.f, r.y, r.x, inst};
   return true;
 }
@@ -125,13 +155,15 @@ BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec
 // CylindersPrograms::intersect (native/geometry/Cylinders.dev.cu:107-260)
 // on a BVH-ordered record ((v0, radius), (v1, 0)); the reference tests
 // against the ray's current closest distance, then reports the hit
-BNX_D bool cyl_test(const float4 *rec, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst)
+BNX_D bool cyl_test(const float4 *rec, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst,
+                    const Cut &cut)
 {
   const float4 A = rec[0], B = rec[1];
   float t;
   if (!cyl_intersect(o, d, xyz(A), xyz(B), A.w, tmin, best.t, t, nullptr)) return false;
   const int2 r = *ref;
   if (!accept_t(t, best, inst, r.x, r.y)) return false;
+  if (cut_rejects<true>(cut, t)) return false;
   best = HitRec{t, t, 0.f, r.y, r.x, inst};   // u: the hit distance, for the shading
   return true;
 }
@@ -193,11 +225,11 @@ BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 
 // lds: this lane's ring, &ring[0][lane]; the stack depth and the spill
 // array are locals so the depth stays in a register
-template <int SH, bool CYL>
+template <int SH, bool EXT>
 BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax, bool any_hit,
-                      uint32_t *lds,
-                      Counts &cnt)
+                      uint32_t *lds, Counts &cnt, float4 cut_plane)
 {
+  const Cut cut{cut_plane, rorg, rdir};
   uint32_t deep[STACK_CAP - SH];
   int sp = 0;
   auto push = [&](uint32_t v) {
@@ -275,15 +307,16 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       bool found = false;
       if (blas.type == BLAS_TRIANGLES) {
         cnt.tris += count;
-        for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, tmin, best, inst);
-      } else if (!CYL || blas.type == BLAS_SPHERES) {
+        for (int k = 0; k < count; k++)
+          found |= tri_test<EXT>(blas.prims + 3 * (first + k), o, d, tmin, best, inst, cut);
+      } else if (!EXT || blas.type == BLAS_SPHERES) {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
-          found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst);
+          found |= sphere_test<EXT>(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst, cut);
       } else {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
-          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst);
+          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst, cut);
       }
       if (any_hit && found) break;
     }
@@ -306,11 +339,11 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
 // may step nodes that a hit in its pending leaf would have culled; the hit
 // returned is the same (equal t resolves by ID).  cur == EMPTY4: the stack
 // is exhausted.
-template <int SH, bool CYL>
+template <int SH, bool EXT>
 BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax,
-                           bool any_hit,
-                           uint32_t *lds, Counts &cnt)
+                           bool any_hit, uint32_t *lds, Counts &cnt, float4 cut_plane)
 {
+  const Cut cut{cut_plane, rorg, rdir};
   uint32_t deep[STACK_CAP - SH];
   int sp = 0;
   auto push = [&](uint32_t v) {
@@ -382,15 +415,16 @@ BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *
       const int count = (int)(leaf & 7u);
       if (blas.type == BLAS_TRIANGLES) {
         cnt.tris += count;
-        for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, tmin, best, inst);
-      } else if (!CYL || blas.type == BLAS_SPHERES) {
+        for (int k = 0; k < count; k++)
+          found |= tri_test<EXT>(blas.prims + 3 * (first + k), o, d, tmin, best, inst, cut);
+      } else if (!EXT || blas.type == BLAS_SPHERES) {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
-          found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst);
+          found |= sphere_test<EXT>(blas.prims[first + k], blas.refs + first + k, o, d, tmin, best, inst, cut);
       } else {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
-          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst);
+          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst, cut);
       }
       leaf = 0;
     } else if ((cur & LEAF_BIT) && !in_blas) {
@@ -430,7 +464,8 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
-    HitRec h = traverse<API_SHORT, true>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt);
+    HitRec h = traverse<API_SHORT, true>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt,
+                                         no_cut().plane);
     bnx_hit out;
     out.t = h.t;
     out.u = h.prim >= 0 ? h.u : 0.f;
@@ -453,7 +488,7 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
 // each block lies inside one request's segment, so the request's scene
 // pointers are block-uniform.  counters: statistics array (STAT_NODES,
 // STAT_TRIS, STAT_SPHERES)
-template <int SH, bool SPEC, bool CYL>
+template <int SH, bool SPEC, bool EXT>
 BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, const TraceQueue &q,
                         unsigned long long *counters)
 {
@@ -464,14 +499,15 @@ BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, co
   bool valid;
   const Segment sg = segment_of<TB>(sv, i, valid);
   const SceneDev S = slots[sg.slot].scene;
+  const float4 cut_plane = EXT ? slot_cut_plane(slots[sg.slot]) : no_cut().plane;
   if (valid) {
     float4 o = q.org[i];
     float4 d = q.dir[i];
     const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
     if (S.num_insts > 0)
-      h = SPEC ? traverse_spec<SH, CYL>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt)
-               : traverse<SH, CYL>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt);
+      h = SPEC ? traverse_spec<SH, EXT>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt, cut_plane)
+               : traverse<SH, EXT>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt, cut_plane);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
@@ -484,10 +520,10 @@ BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, co
   }
 }
 
-// CYL: the leaf test knows cylinder BLASes.  The kernels without it serve
-// worlds of triangles and spheres only (the leaf loop is then the same code
-// as before cylinders existed, with the same register allocation); the
-// launcher takes the _cyl kernels while any built world holds cylinders.
+// EXT: the leaf test knows cylinder BLASes and the cut plane of each
+// request.  The kernels without it serve worlds of triangles and spheres
+// traced without a cut plane; the launcher takes the _ext kernels while any
+// built world holds cylinders or a request in the launch has a cut plane.
 template <int SH, bool SPEC>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
@@ -496,7 +532,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
 }
 
 template <int SH, bool SPEC>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static_cyl(
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static_ext(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
   trace_static<SH, SPEC, true>(slots, sv, q, counters);
@@ -511,7 +547,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
 // taken one node or leaf per loop trip.
 constexpr int POOL_CHUNK = 64;
 
-template <int SH, bool CYL>
+template <int SH, bool EXT>
 BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, const TraceQueue &q,
                       unsigned long long *counters)
 {
@@ -535,6 +571,7 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
   const int seg = sv.blkT[blockIdx.x];
   const Segment sg = sv.segs[seg];
   const SceneDev S = slots[sg.slot].scene;
+  const float4 cut_plane = EXT ? slot_cut_plane(slots[sg.slot]) : no_cut().plane;
   const bool empty_scene = S.num_insts == 0;
   const int refill_at = sg.refill;
   Counts cnt{0, 0, 0};
@@ -642,17 +679,19 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
           cur = 0;
         } else {
           bool found = false;
+          const Cut cut{cut_plane, q.org + ray, q.dir + ray};
           if (blas.type == BLAS_TRIANGLES) {
             cnt.tris += count;
-            for (int k = 0; k < count; k++) found |= tri_test(blas.prims + 3 * (first + k), o, d, 0.f, best, inst);
-          } else if (!CYL || blas.type == BLAS_SPHERES) {
+            for (int k = 0; k < count; k++)
+              found |= tri_test<EXT>(blas.prims + 3 * (first + k), o, d, 0.f, best, inst, cut);
+          } else if (!EXT || blas.type == BLAS_SPHERES) {
             cnt.spheres += count;
             for (int k = 0; k < count; k++)
-              found |= sphere_test(blas.prims[first + k], blas.refs + first + k, o, d, 0.f, best, inst);
+              found |= sphere_test<EXT>(blas.prims[first + k], blas.refs + first + k, o, d, 0.f, best, inst, cut);
           } else {
             cnt.spheres += count;
             for (int k = 0; k < count; k++)
-              found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, 0.f, best, inst);
+              found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, 0.f, best, inst, cut);
           }
           if ((any_hit && found) || sp == 0) done = true;
           else cur = pop();
@@ -685,25 +724,25 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
 }
 
 template <int SH>
-__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool_cyl(
+__global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool_ext(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
   trace_pool<SH, true>(slots, sv, q, counters);
 }
 
 template <int SH, bool SPEC>
-void launch_static(Device *dev, bool cyl, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
+void launch_static(Device *dev, bool ext, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
                    unsigned long long *counters)
 {
-  if (cyl) k_trace_static_cyl<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  if (ext) k_trace_static_ext<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
   else k_trace_static<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
 }
 
 template <int SH>
-void launch_pool(Device *dev, bool cyl, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
+void launch_pool(Device *dev, bool ext, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
                  unsigned long long *counters)
 {
-  if (cyl) k_trace_pool_cyl<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  if (ext) k_trace_pool_ext<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
   else k_trace_pool<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
 }
 
@@ -722,28 +761,28 @@ int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, ui
 }
 
 int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, int blocksT, bool pooled,
-                       const TraceQueue &q, unsigned long long *counters)
+                       bool cut, const TraceQueue &q, unsigned long long *counters)
 {
   const int sh = dev->trace_stack_lds;
-  const bool cyl = dev->cyl_worlds > 0;
+  const bool ext = dev->cyl_worlds > 0 || cut;
   if (pooled) {
     if (blocksT > 0) {
-      if (sh == 8) launch_pool<8>(dev, cyl, blocksT, slots, sv, q, counters);
-      else if (sh == 32) launch_pool<32>(dev, cyl, blocksT, slots, sv, q, counters);
-      else launch_pool<16>(dev, cyl, blocksT, slots, sv, q, counters);
+      if (sh == 8) launch_pool<8>(dev, ext, blocksT, slots, sv, q, counters);
+      else if (sh == 32) launch_pool<32>(dev, ext, blocksT, slots, sv, q, counters);
+      else launch_pool<16>(dev, ext, blocksT, slots, sv, q, counters);
     }
   } else {
     if (blocks256 > 0) {
       const bool spec = dev->trace_walk == 1;
       if (sh == 8) {
-        if (spec) launch_static<8, true>(dev, cyl, blocks256, slots, sv, q, counters);
-        else launch_static<8, false>(dev, cyl, blocks256, slots, sv, q, counters);
+        if (spec) launch_static<8, true>(dev, ext, blocks256, slots, sv, q, counters);
+        else launch_static<8, false>(dev, ext, blocks256, slots, sv, q, counters);
       } else if (sh == 32) {
-        if (spec) launch_static<32, true>(dev, cyl, blocks256, slots, sv, q, counters);
-        else launch_static<32, false>(dev, cyl, blocks256, slots, sv, q, counters);
+        if (spec) launch_static<32, true>(dev, ext, blocks256, slots, sv, q, counters);
+        else launch_static<32, false>(dev, ext, blocks256, slots, sv, q, counters);
       } else {
-        if (spec) launch_static<16, true>(dev, cyl, blocks256, slots, sv, q, counters);
-        else launch_static<16, false>(dev, cyl, blocks256, slots, sv, q, counters);
+        if (spec) launch_static<16, true>(dev, ext, blocks256, slots, sv, q, counters);
+        else launch_static<16, false>(dev, ext, blocks256, slots, sv, q, counters);
       }
     }
   }

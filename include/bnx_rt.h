@@ -279,6 +279,22 @@ int bnx_device_sync(bnx_device dev);
 /* free / total device memory in bytes (used by memory-based admission) */
 int bnx_device_mem_info(bnx_device dev, size_t *free_bytes, size_t *total_bytes);
 
+/* ---------------------------------------------------- device memory
+ * The memory methods of rtc::CudaDeviceBase (rtcore/cudaCommon/Device.h:
+ * 21-46, implemented in rtcore/cudaCommon/Device.cu): allocMem / freeMem
+ * (device memory), allocHost / freeHost (pinned host memory), copyAsync
+ * (any direction, on the device's stream; copy = copyAsync + sync),
+ * memsetAsync, and setActive / restoreActive (the GPU made current, and
+ * the one current before).  Failures return codes instead of raising. */
+int bnx_mem_alloc(bnx_device dev, size_t bytes, void **d_ptr);
+int bnx_mem_free(bnx_device dev, void *d_ptr);
+int bnx_host_alloc(bnx_device dev, size_t bytes, void **h_ptr);
+int bnx_host_free(bnx_device dev, void *h_ptr);
+int bnx_mem_copy_async(bnx_device dev, void *dst, const void *src, size_t bytes);
+int bnx_mem_set_async(bnx_device dev, void *d_ptr, int value, size_t bytes);
+int bnx_device_set_active(bnx_device dev, int *previous_gpu);
+int bnx_device_restore_active(int previous_gpu);
+
 /* ------------------------------------------------------------- tuning
  * Launch parameters of the render path, per device.  None of them changes a
  * result: they set the order in which primary rays enter the queue, how the
@@ -522,7 +538,9 @@ typedef struct bnx_request {
   int32_t      _pad;
 } bnx_request;
 
-enum { BNX_REQ_QUEUED = 0, BNX_REQ_ACTIVE = 1, BNX_REQ_DONE = 2 };
+/* FAILED: admission could not set the request up (bnx_server_step returned
+ * the error); the request holds no slot or frame and can only be released */
+enum { BNX_REQ_QUEUED = 0, BNX_REQ_ACTIVE = 1, BNX_REQ_DONE = 2, BNX_REQ_FAILED = 3 };
 
 typedef struct bnx_request_status {
   int32_t state;                    /* BNX_REQ_*                                       */

@@ -1,17 +1,22 @@
 /* bnx_rtc.hpp -- barney's rtcore host interface (namespace BARNEY_NS::rtc,
- * rtcore/cuda/Device.h:12-69, Buffer.h, Geom.h, GeomType.h, Group.h:14-25)
- * mirrored in C++ over the C-ABI of bnx_rt.h, for the rtc adapter of an
- * "amd" backend.  Same class and method names and argument meaning as the
- * reference; every failing call throws std::runtime_error with the
- * library's message, as BARNEY_RAISE does (rtcore/cudaCommon/cuda-helper.h:
- * 9-42).  Header-only; link with libbnx.so.
+ * rtcore/cuda/Device.h:12-69, rtcore/cudaCommon/Device.h:21-46, Buffer.h,
+ * Geom.h, GeomType.h, Group.h:14-25) mirrored in C++ over the C-ABI of
+ * bnx_rt.h, for the rtc adapter of an "amd" backend.  Same class and method
+ * names and argument meaning as the reference; every failing call throws
+ * std::runtime_error with the library's message, as BARNEY_RAISE does
+ * (rtcore/cudaCommon/cuda-helper.h:9-42).  Header-only; link with libbnx.so.
  *
  * Differences that follow from the boundary: a Buffer keeps a host copy of
  * what was uploaded (the C-ABI builds geometries from host arrays), and a
- * geometry is created in the library when its group is created.  The
- * reference's program model (SBT records, device programs per GeomType) is
- * the library's closed set of types: "Triangles", "Spheres" and
- * "Cylinders". */
+ * geometry is created in the library when a group is created over it.  A
+ * geometry whose setters were called again, or one of whose buffers was
+ * uploaded to since, is created anew for the next group (Triangles::commit,
+ * native/geometry/Triangles.cpp:50-80, re-sets the buffers of the same
+ * rtc::Geom and the next Group::build creates a new group); groups created
+ * before keep the geometry they were built over, which lives until the last
+ * of them is freed.  The reference's program model (SBT records, device
+ * programs per GeomType) is the library's closed set of types:
+ * "Triangles", "Spheres" and "Cylinders". */
 #pragma once
 
 #include "bnx_rt.h"
@@ -65,6 +70,7 @@ struct Buffer {
     if (ofs + numBytes > host.size()) throw std::runtime_error("bnx: Buffer::upload past the end");
     check(bnx_buffer_upload(h, hostPtr, numBytes, ofs));
     std::memcpy(host.data() + ofs, hostPtr, numBytes);
+    version++;
   }
   void download(void *hostPtr, size_t numBytes, size_t ofs = 0) const
   {
@@ -76,6 +82,7 @@ struct Buffer {
   Device *const device;
   bnx_buffer h = nullptr;
   std::vector<uint8_t> host;
+  uint64_t version = 0;   // uploads so far (a geometry over it is rebuilt when it changes)
 };
 
 struct GeomType;
@@ -86,24 +93,38 @@ struct GeomType;
 // (geometry/Cylinders.h:16-22) as setCylinders
 struct Geom {
   explicit Geom(GeomType *gt) : gt(gt) {}
+  // groups must be freed before the geometries they hold (as in the
+  // reference, whose native layer frees groups first)
   ~Geom()
   {
     if (h) bnx_geom_destroy(h);
+    for (auto &r : retired) bnx_geom_destroy(r.first);
   }
-  void setPrimCount(int primCount) { this->primCount = primCount; }
+  void setPrimCount(int primCount)
+  {
+    this->primCount = primCount;
+    dirty = true;
+  }
   void setVertices(Buffer *vertices, int numVertices)
   {
     this->vertices = vertices;
     this->numVertices = numVertices;
+    dirty = true;
   }
   void setIndices(Buffer *indices, int numIndices)
   {
     this->indices = indices;
     this->numIndices = numIndices;
+    dirty = true;
   }
-  void setPrimColors(Buffer *colors) { this->colors = colors; }
+  void setPrimColors(Buffer *colors)
+  {
+    this->colors = colors;
+    dirty = true;
+  }
   void setSpheres(Buffer *origins, Buffer *radii, Buffer *colors, int numSpheres, float defaultRadius)
   {
+    dirty = true;
     this->origins = origins;
     this->radii = radii;
     this->colors = colors;
@@ -115,6 +136,7 @@ struct Geom {
   void setCylinders(Buffer *vertices, int numVertices, Buffer *indices, int numCylinders, Buffer *radii,
                     Buffer *colors, float defaultRadius)
   {
+    dirty = true;
     this->vertices = vertices;
     this->numVertices = numVertices;
     this->indices = indices;
@@ -124,13 +146,28 @@ struct Geom {
     primCount = numCylinders;
     this->defaultRadius = defaultRadius;
   }
-  void setUserID(int userID) { this->userID = userID; }
-  void setMaterial(const bnx_material &m) { material = m; }
-  // the library's geometry, created on first use by a group
-  bnx_geom handle(bnx_device dev);
+  void setUserID(int userID)
+  {
+    this->userID = userID;
+    dirty = true;
+  }
+  void setMaterial(const bnx_material &m)
+  {
+    material = m;
+    dirty = true;
+  }
+  // the library's geometry for a new group: created on first use, and
+  // again after a setter call or an upload to one of its buffers; the group
+  // holds it until freed (acquire / release)
+  bnx_geom acquire(bnx_device dev);
+  void release(bnx_geom g);
 
   GeomType *const gt;
   bnx_geom h = nullptr;
+  int h_groups = 0;                                     // groups holding h
+  std::vector<std::pair<bnx_geom, int>> retired;        // older geometries still held by groups
+  bool dirty = true;
+  uint64_t seen[5] = {0, 0, 0, 0, 0};                   // buffer versions h was built from
   Buffer *vertices = nullptr, *indices = nullptr, *colors = nullptr, *origins = nullptr, *radii = nullptr;
   int numVertices = 0, numIndices = 0, primCount = 0, userID = 0;
   float defaultRadius = .01f;
@@ -150,7 +187,11 @@ struct GeomType {
 // rtc::Group (rtcore/cuda/Group.h:14-25)
 struct Group {
   Group(Device *device, bnx_group h, bool instances) : device(device), h(h), instances(instances) {}
-  ~Group() { bnx_group_destroy(h); }
+  ~Group()
+  {
+    bnx_group_destroy(h);
+    for (auto &u : uses) u.first->release(u.second);
+  }
   void buildAccel() { check(bnx_group_build(h)); }
   // InstanceGroup::setTransforms keeps the transforms for the next
   // refitAccel, which builds the instance BVH again (as buildAccel does in
@@ -174,6 +215,7 @@ struct Group {
   const bnx_group h;
   const bool instances;
   std::vector<affine3f> pendingXfms;
+  std::vector<std::pair<Geom *, bnx_geom>> uses;   // the geometries it was created over
 };
 
 // rtc::Device (rtcore/cuda/Device.h:23-56)
@@ -192,21 +234,33 @@ struct Device {
   void freeGeomType(GeomType *gt) { delete gt; }
   void freeGeom(Geom *g) { delete g; }
 
-  Group *createTrianglesGroup(const std::vector<Geom *> &geoms)
+  Group *createTrianglesGroup(const std::vector<Geom *> &geoms) { return geomGroup(geoms, false); }
+  Group *createUserGeomsGroup(const std::vector<Geom *> &geoms) { return geomGroup(geoms, true); }
+  Group *geomGroup(const std::vector<Geom *> &geoms, bool user)
   {
+    std::vector<std::pair<Geom *, bnx_geom>> uses;
     std::vector<bnx_geom> hs;
-    for (Geom *g : geoms) hs.push_back(g->handle(h));
     bnx_group out = nullptr;
-    check(bnx_group_create_triangles(h, hs.data(), (int)hs.size(), &out));
-    return new Group(this, out, false);
-  }
-  Group *createUserGeomsGroup(const std::vector<Geom *> &geoms)
-  {
-    std::vector<bnx_geom> hs;
-    for (Geom *g : geoms) hs.push_back(g->handle(h));
-    bnx_group out = nullptr;
-    check(bnx_group_create_user(h, hs.data(), (int)hs.size(), &out));
-    return new Group(this, out, false);
+    int rc = BNX_OK;
+    try {
+      for (Geom *g : geoms) {
+        hs.push_back(g->acquire(h));
+        uses.push_back({g, hs.back()});
+      }
+      rc = user ? bnx_group_create_user(h, hs.data(), (int)hs.size(), &out)
+                : bnx_group_create_triangles(h, hs.data(), (int)hs.size(), &out);
+    } catch (...) {
+      for (auto &u : uses) u.first->release(u.second);
+      throw;
+    }
+    if (rc != BNX_OK) {
+      const std::string msg = std::string("bnx: ") + bnx_last_error();
+      for (auto &u : uses) u.first->release(u.second);
+      throw std::runtime_error(msg);
+    }
+    Group *g = new Group(this, out, false);
+    g->uses = uses;
+    return g;
   }
   Group *createInstanceGroup(const std::vector<Group *> &groups, const std::vector<int> &instIDs,
                              const std::vector<affine3f> &xfms)
@@ -227,7 +281,50 @@ struct Device {
   void buildPipeline() {}
   void buildSBT() {}
 
+  // rtc::CudaDeviceBase memory methods (rtcore/cudaCommon/Device.h:21-46)
+  void copyAsync(void *dst, const void *src, size_t numBytes) { check(bnx_mem_copy_async(h, dst, src, numBytes)); }
+  void copy(void *dst, const void *src, size_t numBytes)
+  {
+    copyAsync(dst, src, numBytes);
+    sync();
+  }
+  void *allocHost(size_t numBytes)
+  {
+    void *p = nullptr;
+    check(bnx_host_alloc(h, numBytes, &p));
+    return p;
+  }
+  void freeHost(void *mem) { check(bnx_host_free(h, mem)); }
+  void memsetAsync(void *mem, int value, size_t size) { check(bnx_mem_set_async(h, mem, value, size)); }
+  void *allocMem(size_t numBytes)
+  {
+    void *p = nullptr;
+    check(bnx_mem_alloc(h, numBytes, &p));
+    return p;
+  }
+  void freeMem(void *mem) { check(bnx_mem_free(h, mem)); }
+  // makes this GPU current; returns the one current before
+  int setActive() const
+  {
+    int prev = -1;
+    check(bnx_device_set_active(h, &prev));
+    return prev;
+  }
+  void restoreActive(int oldActive) const { check(bnx_device_restore_active(oldActive)); }
+
   bnx_device h = nullptr;
+};
+
+// rtc::SetActiveGPU (rtcore/cudaCommon/Device.h:49-56): RAII around
+// setActive / restoreActive
+struct SetActiveGPU {
+  explicit SetActiveGPU(const Device *device) : saved(device->setActive()) {}
+  ~SetActiveGPU() { (void)bnx_device_restore_active(saved); }
+  SetActiveGPU(const SetActiveGPU &) = delete;
+  SetActiveGPU &operator=(const SetActiveGPU &) = delete;
+
+private:
+  int saved = -1;
 };
 
 static_assert(sizeof(affine3f) == 12 * sizeof(float), "affine3f is 12 floats");
@@ -238,9 +335,38 @@ inline Buffer::Buffer(Device *device, size_t numBytes, const void *initValues) :
   if (initValues && numBytes) std::memcpy(host.data(), initValues, numBytes);
 }
 
-inline bnx_geom Geom::handle(bnx_device dev)
+inline void Geom::release(bnx_geom g)
 {
-  if (h) return h;
+  if (g == h) {
+    h_groups--;
+    return;
+  }
+  for (size_t k = 0; k < retired.size(); k++)
+    if (retired[k].first == g) {
+      if (--retired[k].second == 0) {
+        bnx_geom_destroy(g);
+        retired.erase(retired.begin() + (long)k);
+      }
+      return;
+    }
+}
+
+inline bnx_geom Geom::acquire(bnx_device dev)
+{
+  Buffer *bufs[5] = {vertices, indices, colors, origins, radii};
+  bool stale = dirty || !h;
+  for (int k = 0; k < 5; k++) stale = stale || (bufs[k] ? bufs[k]->version : 0) != seen[k];
+  if (!stale) {
+    h_groups++;
+    return h;
+  }
+  bnx_geom old = h;
+  h = nullptr;
+  if (old) {
+    if (h_groups > 0) retired.push_back({old, h_groups});
+    else bnx_geom_destroy(old);
+  }
+  h_groups = 0;
   if (gt->kind == GeomType::TRIANGLES) {
     bnx_triangles_desc d{};
     d.vertices = vertices ? (const float *)vertices->hostData() : nullptr;
@@ -274,6 +400,9 @@ inline bnx_geom Geom::handle(bnx_device dev)
     d.material = material;
     check(bnx_geom_create_spheres(dev, &d, &h));
   }
+  for (int k = 0; k < 5; k++) seen[k] = bufs[k] ? bufs[k]->version : 0;
+  dirty = false;
+  h_groups = 1;
   return h;
 }
 

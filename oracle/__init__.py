@@ -79,7 +79,7 @@ def lib():
         L.orc_volume_march.restype = C.c_float
         L.orc_scene_build.argtypes = [vp]
         L.orc_scene_build.restype = C.c_double
-        L.orc_trace.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_uint64), C.c_int]
+        L.orc_trace.argtypes = [vp, vp, C.c_int, vp, C.c_int, C.POINTER(C.c_uint64), C.c_int, abi.fp]
         L.orc_bvh_build_boxes.argtypes = [abi.fp, C.c_int, C.c_int, vp, abi.ip, abi.ip]
         L.orc_scene_bvh.argtypes = [vp, C.c_int, vp, abi.ip, abi.ip]
         L.orc_camera_perspective.argtypes = [abi.fp, abi.fp, abi.fp, C.c_float, C.POINTER(abi.Camera)]
@@ -322,13 +322,16 @@ class Scene:
     def build(self) -> float:
         return self.L.orc_scene_build(self.h)
 
-    def trace(self, rays: np.ndarray, mode: int = 0, threads: int | None = None):
+    def trace(self, rays: np.ndarray, mode: int = 0, threads: int | None = None, cut_plane=None):
+        """mode 0: BVH closest hit, 1: brute force, 2: BVH any hit.  cut_plane
+        (nx, ny, nz, w): surface hits on its negative side are rejected."""
         rays = np.ascontiguousarray(rays, dtype=np.float32)
         n = len(rays)
         hits = np.zeros(n, dtype=abi.HIT_DTYPE)
         ctr = (C.c_uint64 * 4)()
+        cp = None if cut_plane is None else np.ascontiguousarray(cut_plane, np.float32)
         self.L.orc_trace(self.h, rays.ctypes.data, n, hits.ctypes.data, mode, ctr,
-                         threads or default_threads())
+                         threads or default_threads(), None if cp is None else abi.fptr(cp))
         return hits, dict(nodes=ctr[0], tris=ctr[1], spheres=ctr[2], rays=ctr[3])
 
     def bvh(self, group: int):

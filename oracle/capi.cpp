@@ -322,16 +322,17 @@ double orc_scene_build(orc_scene *sc)
 }
 
 // mode 0: BVH closest, 1: brute force closest, 2: BVH any hit.
-// counters: nodes, tris, spheres, rays
-int orc_trace(orc_scene *sc, const bnx_ray *rays, int n, bnx_hit *hits, int mode, uint64_t *counters, int threads)
+// counters: nodes, tris, spheres, rays.  cut_plane: 4 floats or null.
+int orc_trace(orc_scene *sc, const bnx_ray *rays, int n, bnx_hit *hits, int mode, uint64_t *counters, int threads,
+              const float *cut_plane)
 {
   std::atomic<uint64_t> cn{0}, ct{0}, cs{0};
   parallel_for(n, threads, [&](int i) {
     const bnx_ray &r = rays[i];
     Hit h;
     TraceCounters c;
-    if (mode == 1) h = trace_brute(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax);
-    else h = trace_bvh(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax, mode == 2, &c);
+    if (mode == 1) h = trace_brute(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax, cut_plane);
+    else h = trace_bvh(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax, mode == 2, &c, cut_plane);
     bnx_hit &o = hits[i];
     o.t = h.t;
     o.u = h.prim >= 0 ? h.u : 0.f;

@@ -479,8 +479,24 @@ inline bool box_test(vec3 org, vec3 inv, const float *lo, const float *hi, float
   return tn <= tf * 1.00001f;
 }
 
+// OptixGlobals::hitOnInvisibleSide (native/render/OptixGlobals.h:36-45): a
+// hit at t whose world position org + t dir lies on the negative side of
+// the cut plane (nx, ny, nz, w) is rejected by every surface program
+// (Triangles.dev.cu:39, Spheres.dev.cu:170, Cylinders.dev.cu:258); the
+// plane is off when w <= -1e28
+struct CutRay {
+  const float *plane;   // null: no cut plane
+  vec3 org, dir;        // the world ray (dir after the zero fix)
+};
+inline bool cut_rejects(const CutRay &c, float t)
+{
+  if (!c.plane || c.plane[3] <= -1e28f) return false;
+  vec3 P = c.org + t * c.dir;
+  return dot(load3(c.plane), P) + c.plane[3] < 0.f;
+}
+
 bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, float tmin, Hit &best, int inst,
-               TraceCounters *ctr)
+               TraceCounters *ctr, const CutRay &cut)
 {
   int pidx = g.bvh.order[slot];
   int gi = g.prim_geom[pidx];
@@ -489,12 +505,21 @@ bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, flo
   if (geom.type == 0) {
     if (ctr) ctr->tris++;
     const int32_t *ix = &geom.indices[3 * pid];
-    return tri_test(org, dir, tmin, geom.vertices[ix[0]], geom.vertices[ix[1]], geom.vertices[ix[2]], best, inst,
-                    gi, pid);
+    Hit h = best;
+    if (!tri_test(org, dir, tmin, geom.vertices[ix[0]], geom.vertices[ix[1]], geom.vertices[ix[2]], h, inst, gi,
+                  pid))
+      return false;
+    if (cut_rejects(cut, h.t)) return false;
+    best = h;
+    return true;
   } else if (geom.type == 1) {
     if (ctr) ctr->spheres++;
     float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
-    return sphere_test(org, dir, tmin, geom.centers[pid], r, best, inst, gi, pid);
+    Hit h = best;
+    if (!sphere_test(org, dir, tmin, geom.centers[pid], r, h, inst, gi, pid)) return false;
+    if (cut_rejects(cut, h.t)) return false;
+    best = h;
+    return true;
   } else {
     if (ctr) ctr->spheres++;
     float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
@@ -503,6 +528,7 @@ bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, flo
                             r, tmin, best.t, t, nullptr))
       return false;
     if (!accept_t(t, best, inst, gi, pid)) return false;
+    if (cut_rejects(cut, t)) return false;
     best.t = t;
     best.u = t;
     best.v = 0.f;
@@ -521,9 +547,11 @@ bool cylinder_intersect(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radi
   return cylinder_test_geom(ray_org, ray_dir, v0, v1, radius, tmin, tmax, t, N);
 }
 
-Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool any_hit, TraceCounters *ctr)
+Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool any_hit, TraceCounters *ctr,
+              const float *cut_plane)
 {
   vec3 dir = fix_dir(dir0);
+  const CutRay cut{cut_plane, org, dir};
   Hit best;
   best.t = tmax;
   best.u = best.v = 0.f;
@@ -587,16 +615,17 @@ Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool 
     }
     const Group &g = s.groups[s.insts[inst].group];
     for (int k = en.ref; k < en.ref + en.cnt; k++) {
-      bool hit = prim_test(s, g, k, o, d, tmin, best, inst, ctr);
+      bool hit = prim_test(s, g, k, o, d, tmin, best, inst, ctr, cut);
       if (hit && any_hit) return best;
     }
   }
   return best;
 }
 
-Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax)
+Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, const float *cut_plane)
 {
   vec3 dir = fix_dir(dir0);
+  const CutRay cut{cut_plane, org, dir};
   Hit best;
   best.t = tmax;
   best.u = best.v = 0.f;
@@ -608,7 +637,7 @@ Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax)
       d = fix_dir(xfm_vector(in.w2o, dir));
     }
     const Group &g = s.groups[in.group];
-    for (int k = 0; k < (int)g.bvh.order.size(); k++) prim_test(s, g, k, o, d, tmin, best, inst, nullptr);
+    for (int k = 0; k < (int)g.bvh.order.size(); k++) prim_test(s, g, k, o, d, tmin, best, inst, nullptr, cut);
   }
   return best;
 }

@@ -355,9 +355,12 @@ struct Hit {
   int prim = -1, geom = -1, inst = -1;
 };
 
-// closest hit (any_hit=false) or occlusion (any_hit=true)
-Hit trace_bvh(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, bool any_hit, TraceCounters *ctr);
-Hit trace_brute(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax);
+// closest hit (any_hit=false) or occlusion (any_hit=true); with cut_plane
+// (nx, ny, nz, w; off when w <= -1e28) surface hits on its negative side are
+// rejected (OptixGlobals::hitOnInvisibleSide, native/render/OptixGlobals.h:36-45)
+Hit trace_bvh(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, bool any_hit, TraceCounters *ctr,
+              const float *cut_plane = nullptr);
+Hit trace_brute(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, const float *cut_plane = nullptr);
 
 // ------------------------------------------------------------- render
 struct Frame {

@@ -539,7 +539,7 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
       parallel_for(n, threads, [&](int i) {
         PathRay &r = queue[i];
         TraceCounters c;
-        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, false, stats ? &c : nullptr);
+        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, false, stats ? &c : nullptr, rr.cut_plane);
         if (gen == 0) ids[i] = {BNX_INF, -1, -1, -1};
         // isosurfaces, then volumes, after the surface trace, each clipped
         // to the closest hit so far

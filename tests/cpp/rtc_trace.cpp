@@ -7,12 +7,21 @@
 // geometries (setSpheres / setCylinders, the Spheres::DD and Cylinders::DD
 // fields).
 //
+// A third phase re-commits the triangles as Triangles::commit does
+// (native/geometry/Triangles.cpp:50-80): the vertex buffer is uploaded
+// again with every vertex moved, the same rtc::Geoms get setVertices /
+// setIndices again, a new triangles group and a new instance group (all
+// identity) are built, and the rays are traced again while the first
+// groups still exist.  The rays and hits live in memory from the device's
+// allocMem / allocHost and move with copyAsync / memsetAsync.
+//
 // usage: rtc_trace <in> <out>
 //   in:  int32 nv, float3[nv], int32 nt, int3[nt], float3[nt] colours,
 //        int32 ns, float3[ns] centres, float[ns] radii, float3 move,
 //        int32 ncv, float3[ncv] cylinder vertices, int32 nc, int2[nc]
 //        end points, float[nc] radii, int32 nr, bnx_ray[nr]
-//   out: bnx_hit[nr] before the move, bnx_hit[nr] after it
+//   out: bnx_hit[nr] before the move, bnx_hit[nr] after it, bnx_hit[nr]
+//        after the re-commit of moved vertices
 // The triangles form two geometries (the first and second half), the
 // spheres one, the cylinders one; instance 0 holds the spheres' group,
 // instance 1 the triangles' group, instance 2 the cylinders' group, all
@@ -121,26 +130,55 @@ int main(int argc, char **argv)
     }
     if (!threw) throw std::runtime_error("rtc_trace: mismatched createInstanceGroup did not throw");
 
-    Buffer *rb = dev.createBuffer(sizeof(bnx_ray) * nr, rays.data());
-    Buffer *hb = dev.createBuffer(sizeof(bnx_hit) * nr);
-    std::vector<bnx_hit> hits(2 * (size_t)nr);
-    traceRays(world, (const bnx_ray *)rb->getDD(), nr, (bnx_hit *)hb->getDD());
-    hb->download(hits.data(), sizeof(bnx_hit) * nr);
+    SetActiveGPU active(&dev);
+    bnx_ray *d_rays = (bnx_ray *)dev.allocMem(sizeof(bnx_ray) * nr);
+    bnx_hit *d_hits = (bnx_hit *)dev.allocMem(sizeof(bnx_hit) * nr);
+    bnx_hit *h_hits = (bnx_hit *)dev.allocHost(3 * sizeof(bnx_hit) * nr);
+    dev.copy(d_rays, rays.data(), sizeof(bnx_ray) * nr);
+    auto trace_into = [&](Group *w, int k) {
+      dev.memsetAsync(d_hits, 0xff, sizeof(bnx_hit) * nr);
+      traceRays(w, d_rays, nr, d_hits);
+      dev.copyAsync(h_hits + (size_t)k * nr, d_hits, sizeof(bnx_hit) * nr);
+      dev.sync();
+    };
+    trace_into(world, 0);
 
     const affine3f M{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}, move};
     world->setTransforms({I, M, I});
     world->refitAccel();
-    traceRays(world, (const bnx_ray *)rb->getDD(), nr, (bnx_hit *)hb->getDD());
-    hb->download(hits.data() + nr, sizeof(bnx_hit) * nr);
-    dev.sync();
+    trace_into(world, 1);
+
+    // re-commit the triangles with every vertex moved
+    std::vector<vec3f> moved(verts);
+    for (vec3f &v : moved) v = vec3f{v.x + move.x, v.y + move.y, v.z + move.z};
+    vb->upload(moved.data(), sizeof(vec3f) * nv);
+    t0->setVertices(vb, nv);
+    t0->setIndices(ib0, half);
+    t1->setVertices(vb, nv);
+    t1->setIndices(ib1, nt - half);
+    Group *gt2 = dev.createTrianglesGroup({t0, t1});
+    gt2->buildAccel();
+    Group *world2 = dev.createInstanceGroup({gs, gt2, gc}, {0, 0, 0}, {I, I, I});
+    world2->buildAccel();
+    // the first groups still hold the geometries they were built over
+    if (t0->retired.size() != 1 || t1->retired.size() != 1)
+      throw std::runtime_error("rtc_trace: re-committed geometry did not keep the old one for its group");
+    trace_into(world2, 2);
+    dev.freeGroup(world);
+    dev.freeGroup(gt);
+    if (!t0->retired.empty() || !t1->retired.empty())
+      throw std::runtime_error("rtc_trace: old geometry not freed with its last group");
 
     std::ofstream out(argv[2], std::ios::binary);
-    out.write((const char *)hits.data(), sizeof(bnx_hit) * hits.size());
+    out.write((const char *)h_hits, 3 * sizeof(bnx_hit) * nr);
     if (!out) throw std::runtime_error("rtc_trace: write failed");
+    dev.freeHost(h_hits);
+    dev.freeMem(d_hits);
+    dev.freeMem(d_rays);
 
-    for (Group *g : {world, gt, gs, gc}) dev.freeGroup(g);
+    for (Group *g : {world2, gt2, gs, gc}) dev.freeGroup(g);
     for (Geom *g : {t0, t1, s0, c0}) dev.freeGeom(g);
-    for (Buffer *b : {vb, ib0, ib1, cb0, cb1, sc, sr, cv, ci, cr, rb, hb}) dev.freeBuffer(b);
+    for (Buffer *b : {vb, ib0, ib1, cb0, cb1, sc, sr, cv, ci, cr}) dev.freeBuffer(b);
     dev.freeGeomType(triType);
     dev.freeGeomType(sphType);
     dev.freeGeomType(cylType);

@@ -223,11 +223,14 @@ def soup_data():
     return verts, idx, col, cs, rs
 
 
-def soup_spec(move=None, sticks=None):
+def soup_spec(move=None, sticks=None, vertex_move=None):
     """instance 0: the spheres' group; instance 1: a group of two triangle
     geometries (the soup's halves), translated by `move` if given; with
-    `sticks` (a CylinderSet), instance 2: its cylinders (user ID 3)"""
+    `sticks` (a CylinderSet), instance 2: its cylinders (user ID 3);
+    `vertex_move` adds an offset to every triangle vertex (float32)"""
     verts, idx, col, cs, rs = soup_data()
+    if vertex_move is not None:
+        verts = (verts + np.asarray(vertex_move, np.float32)).astype(np.float32)
     spec = SceneSpec()
     g0 = spec.tri(verts, idx[:1500], col[:1500], user_id=0)
     g1 = spec.tri(verts, idx[1500:], col[1500:], user_id=1)

@@ -4,9 +4,11 @@ tests/cpp/rtc_trace (built by build()) drives the mirror the way barney's
 native layer drives a backend -- buffers, GeomType::createGeom,
 setVertices/setIndices, triangle and user-geometry groups, an instance
 group (spheres, triangles and cylinders), buildAccel, a trace, then
-setTransforms + refitAccel and a second trace -- and checks that a mismatched createInstanceGroup throws.  Its hits
-must equal, bit for bit, those of the same scene built through the Python
-binding of the C-ABI and the oracle's, before and after the move."""
+setTransforms + refitAccel and a second trace, then a re-commit of moved
+vertices into the same rtc::Geoms and a third trace over new groups -- and
+checks that a mismatched createInstanceGroup throws.  Its hits must equal,
+bit for bit, those of the same scene built through the Python binding of the
+C-ABI and the oracle's, before and after the move and after the re-commit."""
 import os
 import subprocess
 
@@ -44,8 +46,9 @@ def test_rtc_mirror_traces_like_the_c_abi(dev, orc, tmp_path):
     r = subprocess.run([EXE, str(inp), str(out)], capture_output=True, text=True, timeout=120)
     assert r.returncode == 0 and "rtc_trace ok" in r.stdout, (r.returncode, r.stdout, r.stderr)
     hits = np.fromfile(out, dtype=abi.HIT_DTYPE)
-    assert len(hits) == 2 * len(rays)
-    before, after = hits[: len(rays)], hits[len(rays):]
+    assert len(hits) == 3 * len(rays)
+    n = len(rays)
+    before, after, recommit = hits[:n], hits[n:2 * n], hits[2 * n:]
     spec = helpers.soup_spec(sticks=st)
     _same(before, spec.gpu(dev).trace(rays))
     _same(before, spec.oracle(orc).trace(rays)[0])
@@ -54,3 +57,8 @@ def test_rtc_mirror_traces_like_the_c_abi(dev, orc, tmp_path):
     _same(after, moved.oracle(orc).trace(rays)[0])
     assert not np.array_equal(before["prim_id"], after["prim_id"])
     assert (before["obj_id"] == 3).sum() > 1000
+    # re-committed vertices (moved in the vertex buffer, not by an instance)
+    spec3 = helpers.soup_spec(sticks=st, vertex_move=tuple(float(x) for x in move))
+    _same(recommit, spec3.gpu(dev).trace(rays))
+    _same(recommit, spec3.oracle(orc).trace(rays)[0])
+    assert not np.array_equal(before["prim_id"], recommit["prim_id"])
