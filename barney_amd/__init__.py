@@ -84,6 +84,10 @@ _PROTOS = [
     ("bnx_world_destroy", [vp], C.c_int),
     ("bnx_camera_perspective", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
                                 C.POINTER(abi.Camera)], C.c_int),
+    ("bnx_camera_orthographic", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float), C.c_float,
+                                 C.c_float, C.POINTER(abi.Camera)], C.c_int),
+    ("bnx_camera_omni", [C.POINTER(C.c_float), C.POINTER(C.c_float), C.POINTER(C.c_float),
+                         C.POINTER(abi.Camera)], C.c_int),
     ("bnx_frame_create", [vp, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
     ("bnx_frame_reset", [vp], C.c_int),
     ("bnx_frame_read", [vp, C.c_int, vp], C.c_int),
@@ -332,14 +336,46 @@ def bvh_build_boxes(dev: Device, boxes: np.ndarray, max_leaf: int = 4):
     return nodes[: inf.num_nodes], order[:n], inf
 
 
-def camera_perspective(position, direction, up, fovy_deg) -> abi.Camera:
+def camera_perspective(position, direction, up, fovy_deg, aperture_radius=0.0, focus_distance=0.0) -> abi.Camera:
+    """PerspectiveCamera; depth of field when aperture_radius > 0"""
     L = lib()
     cam = abi.Camera()
     p = (C.c_float * 3)(*position)
     d = (C.c_float * 3)(*direction)
     u = (C.c_float * 3)(*up)
     check(L.bnx_camera_perspective(p, d, u, fovy_deg, C.byref(cam)))
+    if aperture_radius > 0.0:
+        cam.aperture_radius = aperture_radius
+        cam.focus_distance = focus_distance
     return cam
+
+
+def camera_orthographic(position, direction, up, height, aspect) -> abi.Camera:
+    L = lib()
+    cam = abi.Camera()
+    p, d, u = ((C.c_float * 3)(*v) for v in (position, direction, up))
+    check(L.bnx_camera_orthographic(p, d, u, float(height), float(aspect), C.byref(cam)))
+    return cam
+
+
+def camera_omni(position, direction, up) -> abi.Camera:
+    L = lib()
+    cam = abi.Camera()
+    p, d, u = ((C.c_float * 3)(*v) for v in (position, direction, up))
+    check(L.bnx_camera_omni(p, d, u, C.byref(cam)))
+    return cam
+
+
+def _channel_array(channel, h, w):
+    if channel == abi.BNX_FB_COLOR:
+        return np.zeros((h, w, 4), np.float32)
+    if channel in (abi.BNX_FB_COLOR_RGBA8, abi.BNX_FB_COLOR_SRGB8):
+        return np.zeros((h, w, 4), np.uint8)
+    if channel == abi.BNX_FB_NORMAL:
+        return np.zeros((h, w, 3), np.float32)
+    if channel == abi.BNX_FB_DEPTH:
+        return np.zeros((h, w), np.float32)
+    return np.zeros((h, w), np.int32)
 
 
 class Volume:
@@ -437,14 +473,7 @@ class Frame:
         check(self.dev.L.bnx_frame_reset(self.h_))
 
     def read(self, channel: int) -> np.ndarray:
-        if channel == abi.BNX_FB_COLOR:
-            out = np.zeros((self.hgt, self.w, 4), np.float32)
-        elif channel == abi.BNX_FB_NORMAL:
-            out = np.zeros((self.hgt, self.w, 3), np.float32)
-        elif channel == abi.BNX_FB_DEPTH:
-            out = np.zeros((self.hgt, self.w), np.float32)
-        else:
-            out = np.zeros((self.hgt, self.w), np.int32)
+        out = _channel_array(channel, self.hgt, self.w)
         check(self.dev.L.bnx_frame_read(self.h_, channel, out.ctypes.data))
         return out
 
@@ -512,14 +541,7 @@ class Server:
 
     def read(self, ticket: int, channel: int) -> np.ndarray:
         w, h = self.sizes[ticket]
-        if channel == abi.BNX_FB_COLOR:
-            out = np.zeros((h, w, 4), np.float32)
-        elif channel == abi.BNX_FB_NORMAL:
-            out = np.zeros((h, w, 3), np.float32)
-        elif channel == abi.BNX_FB_DEPTH:
-            out = np.zeros((h, w), np.float32)
-        else:
-            out = np.zeros((h, w), np.int32)
+        out = _channel_array(channel, h, w)
         check(self.L.bnx_server_read(self.h, ticket, channel, out.ctypes.data))
         return out
 

@@ -11,6 +11,8 @@ BNX_MATERIAL_PBR = 1
 BNX_CAMERA_PERSPECTIVE = 1
 BNX_RENDER_COMPAT_SPHERE_INSTID13 = 1
 BNX_FB_COLOR, BNX_FB_DEPTH, BNX_FB_PRIMID, BNX_FB_INSTID, BNX_FB_OBJID, BNX_FB_NORMAL = range(6)
+BNX_FB_COLOR_RGBA8, BNX_FB_COLOR_SRGB8 = 6, 7
+BNX_CAMERA_PERSPECTIVE, BNX_CAMERA_ORTHOGRAPHIC, BNX_CAMERA_OMNI = 1, 2, 3
 BNX_TRACE_CLOSEST = 0
 BNX_TRACE_ANY = 1
 
@@ -45,6 +47,9 @@ class CylindersDesc(C.Structure):
 
 
 class Camera(C.Structure):
+    """bnx_camera: the perspective fields by name; an orthographic or omni
+    camera (bnx_camera_orthographic / bnx_camera_omni) reuses the same 14
+    floats as its own fields"""
     _fields_ = [("type", C.c_int32), ("dir_00", f3), ("dir_du", f3), ("dir_dv", f3), ("lens_00", f3),
                 ("aperture_radius", C.c_float), ("focus_distance", C.c_float)]
 

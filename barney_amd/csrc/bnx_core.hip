@@ -1166,7 +1166,60 @@ int bnx_camera_perspective(const float position[3], const float direction[3], co
     out->lens_00[k] = position[k];
   }
   out->aperture_radius = 0.f;
-  out->focus_distance = 1.f;
+  out->focus_distance = 0.f;   // PerspectiveCamera::focusDistance default
+  return BNX_OK;
+}
+
+namespace {
+void cam_normalize(const float *v, float *o)
+{
+  float l = 1.f / sqrtf(v[0] * v[0] + v[1] * v[1] + v[2] * v[2]);
+  o[0] = v[0] * l;
+  o[1] = v[1] * l;
+  o[2] = v[2] * l;
+}
+void cam_cross(const float *a, const float *b, float *o)
+{
+  o[0] = a[1] * b[2] - a[2] * b[1];
+  o[1] = a[2] * b[0] - a[0] * b[2];
+  o[2] = a[0] * b[1] - a[1] * b[0];
+}
+} // namespace
+
+int bnx_camera_orthographic(const float position[3], const float direction[3], const float up[3], float height,
+                            float aspect, bnx_camera *out)
+{
+  BNX_CHECK(position && direction && up && out, BNX_ERR_INVALID, "bnx_camera_orthographic: null argument");
+  // OrthographicCamera::commit (native/Camera.cu:168-181), host float math
+  float d00[3], du[3], dv[3], t[3];
+  cam_normalize(direction, d00);
+  cam_cross(d00, up, t);
+  cam_normalize(t, du);
+  cam_cross(du, d00, t);
+  cam_normalize(t, dv);
+  memset(out, 0, sizeof(*out));
+  out->type = BNX_CAMERA_ORTHOGRAPHIC;
+  for (int k = 0; k < 3; k++) {
+    out->ortho.dir[k] = d00[k];
+    out->ortho.org_00[k] = position[k];
+    out->ortho.org_du[k] = du[k];
+    out->ortho.org_dv[k] = dv[k];
+  }
+  out->ortho.height = height;
+  out->ortho.aspect = aspect;
+  return BNX_OK;
+}
+
+int bnx_camera_omni(const float position[3], const float direction[3], const float up[3], bnx_camera *out)
+{
+  BNX_CHECK(position && direction && up && out, BNX_ERR_INVALID, "bnx_camera_omni: null argument");
+  // OmniCamera::commit (native/Camera.cu:230-242): the frame of the
+  // environment map's construction (EnvMap.cu:165-180)
+  float to_local[9];
+  memset(out, 0, sizeof(*out));
+  out->type = BNX_CAMERA_OMNI;
+  bnx_num::env_frame(direction, up, out->omni.to_world, to_local);
+  for (int k = 0; k < 3; k++) out->omni.position[k] = position[k];
   return BNX_OK;
 }
 
@@ -1230,8 +1283,8 @@ int bnx_render(bnx_world w, const bnx_camera *cam, const bnx_renderer *r, bnx_fr
   BNX_CHECK(f->dev == w->dev, BNX_ERR_INVALID, "bnx_render: frame and world live on different devices");
   BNX_CHECK(w->instances && w->instances->built, BNX_ERR_STATE,
             "bnx_render: the world's instance group is not built (a rebuild failed)");
-  BNX_CHECK(cam->type == BNX_CAMERA_PERSPECTIVE && cam->aperture_radius == 0.f, BNX_ERR_UNSUPPORTED,
-            "bnx_render: only the pinhole perspective camera is provided");
+  BNX_CHECK(cam->type >= BNX_CAMERA_PERSPECTIVE && cam->type <= BNX_CAMERA_OMNI, BNX_ERR_INVALID,
+            "bnx_render: unknown camera type");
   return render_frame(w, *cam, *r, f, spp, stats);
   GUARD_END
 }

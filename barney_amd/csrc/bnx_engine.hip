@@ -892,8 +892,8 @@ int bnx_server_submit(bnx_server s, const bnx_request *req, int64_t *ticket)
   BNX_CHECK(req->width > 0 && req->height > 0 && (long long)req->width * req->height <= (1LL << 26),
             BNX_ERR_INVALID, "bnx_server_submit: bad resolution");
   BNX_CHECK(req->spp >= 1 && req->spp <= MAX_BATCH_SAMPLES, BNX_ERR_INVALID, "bnx_server_submit: spp not in 1..4096");
-  BNX_CHECK(req->camera.type == BNX_CAMERA_PERSPECTIVE && req->camera.aperture_radius == 0.f, BNX_ERR_UNSUPPORTED,
-            "bnx_server_submit: only the pinhole perspective camera is provided");
+  BNX_CHECK(req->camera.type >= BNX_CAMERA_PERSPECTIVE && req->camera.type <= BNX_CAMERA_OMNI, BNX_ERR_INVALID,
+            "bnx_server_submit: unknown camera type");
   BNX_CHECK(Server::bytes_of(req->width, req->height) <= s->budget, BNX_ERR_INVALID,
             "bnx_server_submit: frame larger than the server's frame memory budget");
   Request *r = new Request;

@@ -97,10 +97,51 @@ __global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDe
   float image_u = ((float)ix + pu) / (float)W;
   float image_v = ((float)iy + pv) / (float)H;
   float aspect = (float)W / (float)H;
-  v3 org = ld3(cam.lens_00);
-  v3 d = (ld3(cam.dir_00) + ((1.f * aspect) * (image_u - .5f)) * ld3(cam.dir_du)) +
-         (1.f * (image_v - .5f)) * ld3(cam.dir_dv);
-  v3 dir = normalize(d);
+  v3 org, dir;
+  if (cam.type == BNX_CAMERA_ORTHOGRAPHIC) {
+    // generateRays.cu:130-139
+    dir = normalize(ld3(cam.ortho.dir));
+    org = (ld3(cam.ortho.org_00) + (((image_u - .5f) * cam.ortho.aspect) * cam.ortho.height) * ld3(cam.ortho.org_du)) +
+          ((image_v - .5f) * cam.ortho.height) * ld3(cam.ortho.org_dv);
+  } else if (cam.type == BNX_CAMERA_OMNI) {
+    // generateRays.cu:24-38, 140-146: uvToWorld, not normalized
+    const float phi = TWO_PI * image_u;
+    const float theta = ONE_PI * image_v;
+    float st, ct, sp, cp;
+    bnx_num::sincos(theta, st, ct);
+    bnx_num::sincos(phi, sp, cp);
+    const float *m = cam.omni.to_world;
+    const v3 l{cp * st, sp * st, ct};
+    dir = v3{(m[0] * l.x + m[3] * l.y) + m[6] * l.z, (m[1] * l.x + m[4] * l.y) + m[7] * l.z,
+             (m[2] * l.x + m[5] * l.y) + m[8] * l.z};
+    org = ld3(cam.omni.position);
+  } else {
+    org = ld3(cam.lens_00);
+    v3 d = (ld3(cam.dir_00) + ((1.f * aspect) * (image_u - .5f)) * ld3(cam.dir_du)) +
+           (1.f * (image_v - .5f)) * ld3(cam.dir_dv);
+    if (cam.aperture_radius > 0.f) {
+      // depth of field (generateRays.cu:101-126)
+      const v3 lens_du = normalize(ld3(cam.dir_du));
+      const v3 lens_dv = normalize(ld3(cam.dir_dv));
+      const v3 lens_n = cross(lens_du, lens_dv);
+      const v3 D = normalize(d);
+      const v3 on_plane = D * (cam.focus_distance / fabsf(dot(D, lens_n)));
+      float lu = 0.f, lv = 0.f;
+      if (accumID != 0) {
+        while (true) {
+          lu = 2.f * rnd() - 1.f;
+          lv = 2.f * rnd() - 1.f;
+          if (lu * lu + lv * lv > 1.f) continue;
+          break;
+        }
+      }
+      const v3 off = (cam.aperture_radius * lu) * lens_du + (cam.aperture_radius * lv) * lens_dv;
+      org = org + off;
+      dir = normalize(on_plane - off);
+    } else {
+      dir = normalize(d);
+    }
+  }
   float tmax = 1e30f;
   if (rr.cut_plane[3] > -1e28f) {
     v3 N = ld3(rr.cut_plane);
@@ -750,6 +791,29 @@ __global__ void k_frame_color(FrameDev fb, int np, float scale, float4 *out)
   out[i] = make_float4(c[0], c[1], c[2], c[3]);
 }
 
+// LinearToFixed8 (fb/FrameBuffer.cu:111-130): saturate, the sRGB curve on
+// RGB (common/math.h:61-67; powf of the device library), make_rgba
+// (common/math.h:103-125: min(255, max(0, int(256 f))) per channel)
+BNX_D uint32_t make_8bit(float f) { return (uint32_t)min(255, max(0, (int)(f * 256.f))); }
+BNX_D float linear_to_srgb(float x)
+{
+  if (x <= 0.0031308f) return 12.92f * x;
+  return 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+}
+__global__ void k_frame_rgba8(FrameDev fb, int np, float scale, bool srgb, uint32_t *out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const unsigned long long *a = fb.accum + 4 * (size_t)i;
+  float c[4];
+  for (int k = 0; k < 4; k++) {
+    c[k] = (float)((double)(long long)a[k] * 0x1p-32) * scale;
+    c[k] = fminf(fmaxf(c[k], 0.f), 1.f);
+    if (srgb && k < 3) c[k] = linear_to_srgb(c[k]);
+  }
+  out[i] = (make_8bit(c[0]) << 0) + (make_8bit(c[1]) << 8) + (make_8bit(c[2]) << 16) + (make_8bit(c[3]) << 24);
+}
+
 __global__ void k_frame_depth(FrameDev fb, int np, float *out)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -781,6 +845,15 @@ int frame_read(Frame *f, int channel, void *host)
     float scale = f->accum_id > 0 ? 1.f / (float)f->accum_id : 0.f;
     k_frame_color<<<nblk(np, 256), 256, 0, st>>>(fd, np, scale, f->out.as<float4>());
     BNX_HIP(hipMemcpyAsync(host, f->out.ptr, sizeof(float4) * (size_t)np, hipMemcpyDeviceToHost, st));
+    break;
+  }
+  case BNX_FB_COLOR_RGBA8:
+  case BNX_FB_COLOR_SRGB8: {
+    BNX_HIP(f->out.ensure(sizeof(uint32_t) * (size_t)np));
+    float scale = f->accum_id > 0 ? 1.f / (float)f->accum_id : 0.f;
+    k_frame_rgba8<<<nblk(np, 256), 256, 0, st>>>(fd, np, scale, channel == BNX_FB_COLOR_SRGB8,
+                                                 f->out.as<uint32_t>());
+    BNX_HIP(hipMemcpyAsync(host, f->out.ptr, sizeof(uint32_t) * (size_t)np, hipMemcpyDeviceToHost, st));
     break;
   }
   case BNX_FB_DEPTH:

@@ -136,16 +136,38 @@ typedef struct bnx_cylinders_desc {
   bnx_material   material;
 } bnx_cylinders_desc;
 
-/* Camera::DD perspective member (native/Camera.h:30-65). */
-#define BNX_CAMERA_PERSPECTIVE 1
+/* Camera::DD (native/Camera.h:30-65): a type and, by type, the fields of
+ * the perspective camera (with depth of field when aperture_radius > 0),
+ * the orthographic camera or the omnidirectional camera, as their commit()
+ * methods write them (native/Camera.cu:82-101, 168-181, 230-242).  Fill it
+ * with bnx_camera_perspective / _orthographic / _omni. */
+#define BNX_CAMERA_PERSPECTIVE  1
+#define BNX_CAMERA_ORTHOGRAPHIC 2
+#define BNX_CAMERA_OMNI         3
 typedef struct bnx_camera {
   int32_t type;
-  float   dir_00[3];
-  float   dir_du[3];
-  float   dir_dv[3];
-  float   lens_00[3];
-  float   aperture_radius;
-  float   focus_distance;
+  union {
+    struct {                        /* BNX_CAMERA_PERSPECTIVE                      */
+      float dir_00[3];
+      float dir_du[3];
+      float dir_dv[3];
+      float lens_00[3];
+      float aperture_radius;
+      float focus_distance;
+    };
+    struct {                        /* BNX_CAMERA_ORTHOGRAPHIC                     */
+      float org_00[3];
+      float org_du[3];
+      float org_dv[3];
+      float dir[3];
+      float height;
+      float aspect;
+    } ortho;
+    struct {                        /* BNX_CAMERA_OMNI: affine3f toWorld           */
+      float to_world[9];            /* columns vx, vy, vz                          */
+      float position[3];
+    } omni;
+  };
 } bnx_camera;
 
 /* Renderer::DD + the renderer constants of kernels/shadeRays.cu:17-34. */
@@ -236,6 +258,12 @@ typedef struct bnx_umesh_desc {
 #define BNX_FB_INSTID  3   /* int32 per pixel                               */
 #define BNX_FB_OBJID   4   /* int32 per pixel                               */
 #define BNX_FB_NORMAL  5   /* float3 per pixel (first-sample normal)        */
+/* the colour channel as 8-bit RGBA, one uint32 per pixel (R in the low
+ * byte): BN_UFIXED8_RGBA and BN_UFIXED8_RGBA_SRGB (fb/FrameBuffer.cu:111-130,
+ * 284-294): saturated to [0, 1], the sRGB curve on RGB for the second
+ * (common/math.h:61-75), then min(255, int(256 x)) per channel */
+#define BNX_FB_COLOR_RGBA8 6
+#define BNX_FB_COLOR_SRGB8 7
 
 /* Counters of one bnx_render call (device-side counts, read after sync). */
 typedef struct bnx_render_stats {
@@ -478,9 +506,19 @@ int bnx_world_set_env_map(bnx_world world, const bnx_env_map_desc *desc);
 int bnx_world_destroy(bnx_world world);
 
 /* -------------------------------------------------------------- camera */
-/* PerspectiveCamera::commit (native/Camera.cu:82-101); fovy in degrees. */
+/* PerspectiveCamera::commit (native/Camera.cu:82-101); fovy in degrees.
+ * Depth of field: set out->aperture_radius and out->focus_distance
+ * ('apertureRadius', 'focusDistance') afterwards. */
 int bnx_camera_perspective(const float position[3], const float direction[3], const float up[3],
                            float fovy_deg, bnx_camera *out);
+/* OrthographicCamera::commit (native/Camera.cu:168-181): 'height' is the
+ * image plane's height in world units, 'aspect' its width over height. */
+int bnx_camera_orthographic(const float position[3], const float direction[3], const float up[3], float height,
+                            float aspect, bnx_camera *out);
+/* OmniCamera::commit (native/Camera.cu:230-242): pixel (u, v) looks along
+ * toWorld (cos(2 pi u) sin(pi v), sin(2 pi u) sin(pi v), cos(pi v))
+ * (generateRays.cu:24-38, 140-146). */
+int bnx_camera_omni(const float position[3], const float direction[3], const float up[3], bnx_camera *out);
 
 /* --------------------------------------------------------------- frame */
 /* FrameBuffer/TiledFB resize (fb/TiledFB.cu:249-319) */

@@ -72,6 +72,11 @@ def lib():
         L.orc_volume_sample.argtypes = [vp, C.c_int, abi.fp]
         L.orc_volume_sample.restype = C.c_float
         L.orc_volume_tf_map.argtypes = [vp, C.c_int, C.c_float, abi.fp]
+        L.orc_dda_cells.argtypes = [abi.fp, abi.fp, C.c_float, abi.ip, C.c_int, abi.ip, abi.fp]
+        L.orc_volume_march_n.argtypes = [vp, C.c_int, vp, C.c_int, C.c_uint32, C.c_uint32, abi.fp, C.c_int]
+        L.orc_camera_orthographic.argtypes = [abi.fp, abi.fp, abi.fp, C.c_float, C.c_float, vp]
+        L.orc_camera_omni.argtypes = [abi.fp, abi.fp, abi.fp, vp]
+        L.orc_primary_rays.argtypes = [vp, C.c_int, C.c_int, C.c_int, vp]
         L.orc_fast_log.argtypes = [C.c_float]
         L.orc_fast_log.restype = C.c_float
         L.orc_volume_march.argtypes = [vp, C.c_int, abi.fp, abi.fp, C.c_float, C.c_uint32, C.c_uint32, abi.fp,
@@ -262,6 +267,15 @@ class Scene:
                                     C.byref(ns))
         return t, rgb, ns.value
 
+    def volume_march_n(self, rays, seed=(0, 7), k=0, threads=None):
+        """event t (or -1) of each ray through volume k; ray i marches with
+        the parent seed (seed[0] + i, seed[1])"""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(len(rays), np.float32)
+        self.L.orc_volume_march_n(self.h, k, rays.ctypes.data, len(rays), seed[0], seed[1], abi.fptr(out),
+                                  threads or default_threads())
+        return out
+
     def set_dir_lights(self, lights):
         arr = abi.dir_lights(lights)
         self._keep.append(arr)
@@ -343,6 +357,19 @@ class Scene:
         return nodes, order, int(info[1])
 
 
+def dda_cells(dorg, ddir, tmax, dims, max_cells=4096):
+    """dda3 (volume/DDA.h:60-201) of one grid-space ray: (cells n x 3, ranges n x 2)"""
+    o = np.ascontiguousarray(dorg, np.float32)
+    d = np.ascontiguousarray(ddir, np.float32)
+    dm = np.ascontiguousarray(dims, np.int32)
+    cells = np.zeros((max_cells, 3), np.int32)
+    ts = np.zeros((max_cells, 2), np.float32)
+    n = lib().orc_dda_cells(abi.fptr(o), abi.fptr(d), float(tmax), abi.iptr(dm), max_cells, abi.iptr(cells),
+                            abi.fptr(ts))
+    assert n <= max_cells
+    return cells[:n], ts[:n]
+
+
 def bvh_build_boxes(boxes: np.ndarray, max_leaf: int = 4):
     boxes = np.ascontiguousarray(boxes, dtype=np.float32).reshape(-1, 6)
     n = len(boxes)
@@ -351,6 +378,28 @@ def bvh_build_boxes(boxes: np.ndarray, max_leaf: int = 4):
     info = (C.c_int32 * 2)()
     lib().orc_bvh_build_boxes(abi.fptr(boxes), n, max_leaf, nodes.ctypes.data, abi.iptr(order), info)
     return nodes[: info[0]], order[:n], int(info[1])
+
+
+def camera_orthographic(position, direction, up, height, aspect) -> abi.Camera:
+    cam = abi.Camera()
+    p, d, u = (np.ascontiguousarray(x, np.float32) for x in (position, direction, up))
+    lib().orc_camera_orthographic(abi.fptr(p), abi.fptr(d), abi.fptr(u), float(height), float(aspect),
+                                  C.byref(cam))
+    return cam
+
+
+def camera_omni(position, direction, up) -> abi.Camera:
+    cam = abi.Camera()
+    p, d, u = (np.ascontiguousarray(x, np.float32) for x in (position, direction, up))
+    lib().orc_camera_omni(abi.fptr(p), abi.fptr(d), abi.fptr(u), C.byref(cam))
+    return cam
+
+
+def primary_rays(cam: abi.Camera, w: int, h: int, accum_id: int = 0) -> np.ndarray:
+    """primary rays (bnx_ray layout, pixel ix + w iy) of sample accum_id"""
+    out = np.zeros((w * h, 8), np.float32)
+    lib().orc_primary_rays(C.byref(cam), w, h, accum_id, out.ctypes.data)
+    return out
 
 
 def camera_perspective(position, direction, up, fovy_deg) -> abi.Camera:
@@ -381,6 +430,8 @@ class Frame:
         n = self.w * self.h
         if channel == abi.BNX_FB_COLOR:
             out = np.zeros((self.h, self.w, 4), np.float32)
+        elif channel in (abi.BNX_FB_COLOR_RGBA8, abi.BNX_FB_COLOR_SRGB8):
+            out = np.zeros((self.h, self.w, 4), np.uint8)
         elif channel == abi.BNX_FB_NORMAL:
             out = np.zeros((self.h, self.w, 3), np.float32)
         elif channel == abi.BNX_FB_DEPTH:

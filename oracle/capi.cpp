@@ -3,12 +3,19 @@
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.h).
 #include "oracle.h"
+#include "dda.h"
 
 #include <chrono>
 #include <cstdio>
 
 namespace orc {
 void camera_perspective(const float *pos, const float *dir, const float *up, float fovy, bnx_camera &out);
+void camera_orthographic(const float *pos, const float *dir, const float *up, float height, float aspect,
+                         bnx_camera &out);
+void camera_omni(const float *pos, const float *dir, const float *up, bnx_camera &out);
+float linear_to_srgb(float x);
+void primary_rays(const bnx_camera &cam, int W, int H, int accumID, bnx_ray *out);
+uint32_t make_8bit(float f);
 }
 
 using namespace orc;
@@ -230,6 +237,45 @@ float orc_volume_sample(orc_scene *sc, int k, const float *P) { return volume_sa
 void orc_volume_tf_map(orc_scene *sc, int k, float s, float *out) { tf_map(sc->s.volumes[k], s, out); }
 float orc_fast_log(float f) { return fast_log(f); }
 
+// dda3 (volume/DDA.h:60-201) of one ray in grid space over a dims grid:
+// writes up to max_cells visited cells (x, y, z) and their parametric
+// ranges (t0, t1); returns the number of cells the walk visits (all of
+// them, even past max_cells)
+int orc_dda_cells(const float *dorg, const float *ddir, float dda_tmax, const int32_t *dims, int max_cells,
+                  int32_t *cells, float *ts)
+{
+  const int d[3] = {dims[0], dims[1], dims[2]};
+  int n = 0;
+  dda3(load3(dorg), load3(ddir), dda_tmax, d, [&](const int c[3], float t0, float t1) -> bool {
+    if (n < max_cells) {
+      cells[3 * n] = c[0];
+      cells[3 * n + 1] = c[1];
+      cells[3 * n + 2] = c[2];
+      ts[2 * n] = t0;
+      ts[2 * n + 1] = t1;
+    }
+    n++;
+    return n < (1 << 20);
+  });
+  return n;
+}
+
+// volume k marched by n rays (bnx_ray layout, from tmin 0 to the ray's
+// tmax), ray i with the parent seed (seed_a + i, seed_b): event t, or -1
+void orc_volume_march_n(orc_scene *sc, int k, const bnx_ray *rays, int n, uint32_t seed_a, uint32_t seed_b,
+                        float *t_out, int threads)
+{
+  parallel_for(n, threads, [&](int i) {
+    RNGSeed p;
+    p.seed(seed_a + (uint32_t)i, seed_b);
+    float t = -1.f, rgb[3];
+    if (!volume_march(sc->s.volumes[k], (uint32_t)(sc->s.insts.size() + k), load3(rays[i].org),
+                      fix_dir(load3(rays[i].dir)), 0.f, rays[i].tmax, p, t, rgb, nullptr))
+      t = -1.f;
+    t_out[i] = t;
+  });
+}
+
 // march one ray through volume k with a parent seed (a, b): event t or -1
 float orc_volume_march(orc_scene *sc, int k, const float *org, const float *dir, float tmax, uint32_t a, uint32_t b,
                        float *rgb, uint64_t *samples)
@@ -384,6 +430,19 @@ void orc_camera_perspective(const float *pos, const float *dir, const float *up,
 {
   camera_perspective(pos, dir, up, fovy, *out);
 }
+void orc_camera_orthographic(const float *pos, const float *dir, const float *up, float height, float aspect,
+                             bnx_camera *out)
+{
+  camera_orthographic(pos, dir, up, height, aspect, *out);
+}
+void orc_camera_omni(const float *pos, const float *dir, const float *up, bnx_camera *out)
+{
+  camera_omni(pos, dir, up, *out);
+}
+void orc_primary_rays(const bnx_camera *cam, int w, int h, int accum_id, bnx_ray *out)
+{
+  primary_rays(*cam, w, h, accum_id, out);
+}
 
 orc_frame *orc_frame_create(int w, int h)
 {
@@ -433,6 +492,20 @@ int orc_frame_read(orc_frame *fb, int channel, void *host)
     memcpy(host, f.obj.data(), np * 4);
   } else if (channel == BNX_FB_NORMAL) {
     memcpy(host, f.normal.data(), np * 12);
+  } else if (channel == BNX_FB_COLOR_RGBA8 || channel == BNX_FB_COLOR_SRGB8) {
+    // LinearToFixed8 (fb/FrameBuffer.cu:111-130): saturate, sRGB on RGB,
+    // make_rgba (common/math.h:103-125)
+    uint32_t *o = (uint32_t *)host;
+    float scale = 1.f / (float)f.accum_id;
+    for (size_t i = 0; i < np; i++) {
+      float c[4];
+      for (int k = 0; k < 4; k++) {
+        c[k] = (float)((double)f.accum[4 * i + k] * 0x1p-32) * scale;
+        c[k] = std::min(std::max(c[k], 0.f), 1.f);
+        if (k < 3 && channel == BNX_FB_COLOR_SRGB8) c[k] = linear_to_srgb(c[k]);
+      }
+      o[i] = (make_8bit(c[0]) << 0) + (make_8bit(c[1]) << 8) + (make_8bit(c[2]) << 16) + (make_8bit(c[3]) << 24);
+    }
   } else {
     return -1;
   }

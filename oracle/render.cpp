@@ -478,12 +478,83 @@ void Frame::resize(int w_, int h_)
 
 void Frame::reset() { resize(w, h); }
 
+// the camera part of _generateRays (generateRays.cu:89-146): pixel
+// (ix, iy) of a W x H frame at sample accumID; 'rand' is the pixel's
+// Random(ix + W accumID, iy + H accumID), already past the pixel jitter
+void primary_ray(const bnx_camera &cam, int W, int H, int accumID, float image_u, float image_v, Random2 &rand,
+                 vec3 &org, vec3 &dir)
+{
+  const vec3 dir00 = load3(cam.dir_00), du = load3(cam.dir_du), dv = load3(cam.dir_dv), lens = load3(cam.lens_00);
+  const vec3 o00 = load3(cam.ortho.org_00), odu = load3(cam.ortho.org_du), odv = load3(cam.ortho.org_dv);
+  float aspect = (float)W / (float)H;
+  if (cam.type == BNX_CAMERA_ORTHOGRAPHIC) {
+    // generateRays.cu:130-139
+    dir = normalize(load3(cam.ortho.dir));
+    org = (o00 + (((image_u - .5f) * cam.ortho.aspect) * cam.ortho.height) * odu) +
+          ((image_v - .5f) * cam.ortho.height) * odv;
+  } else if (cam.type == BNX_CAMERA_OMNI) {
+    // uvToWorld (generateRays.cu:24-38, 140-146); not normalized
+    const float phi = TWO_PI * image_u;
+    const float theta = ONE_PI * image_v;
+    float st, ct, sp, cp;
+    bnx_num::sincos(theta, st, ct);
+    bnx_num::sincos(phi, sp, cp);
+    const float *m = cam.omni.to_world;
+    dir = vec3((m[0] * (cp * st) + m[3] * (sp * st)) + m[6] * ct,
+               (m[1] * (cp * st) + m[4] * (sp * st)) + m[7] * ct,
+               (m[2] * (cp * st) + m[5] * (sp * st)) + m[8] * ct);
+    org = load3(cam.omni.position);
+  } else {
+    org = lens;
+    vec3 d = (dir00 + ((1.f * aspect) * (image_u - .5f)) * du) + (1.f * (image_v - .5f)) * dv;
+    if (cam.aperture_radius > 0.f) {
+      // depth of field (generateRays.cu:101-126)
+      const vec3 lens_du = normalize(du), lens_dv = normalize(dv);
+      const vec3 lens_n = cross(lens_du, lens_dv);
+      const vec3 D = normalize(d);
+      const vec3 on_plane = D * (cam.focus_distance / fabsf(dot(D, lens_n)));
+      float lu = 0.f, lv = 0.f;
+      if (accumID != 0) {
+        while (true) {
+          lu = 2.f * rand() - 1.f;
+          lv = 2.f * rand() - 1.f;
+          if (lu * lu + lv * lv > 1.f) continue;
+          break;
+        }
+      }
+      const vec3 off = (cam.aperture_radius * lu) * lens_du + (cam.aperture_radius * lv) * lens_dv;
+      org = org + off;
+      dir = normalize(on_plane - off);
+    } else {
+      dir = normalize(d);
+    }
+  }
+}
+
+// the primary rays of sample accumID of a W x H frame, pixel (ix, iy) at
+// ray ix + W iy (bnx_ray layout, tmin 0, tmax 1e30, no cut plane)
+void primary_rays(const bnx_camera &cam, int W, int H, int accumID, bnx_ray *out)
+{
+  for (int iy = 0; iy < H; iy++)
+    for (int ix = 0; ix < W; ix++) {
+      Random2 rand((uint32_t)(ix + W * accumID), (uint32_t)(iy + H * accumID));
+      float pu = (accumID == 0) ? .5f : rand();
+      float pv = (accumID == 0) ? .5f : rand();
+      vec3 org, dir;
+      primary_ray(cam, W, H, accumID, ((float)ix + pu) / (float)W, ((float)iy + pv) / (float)H, rand, org, dir);
+      bnx_ray &r = out[ix + (size_t)W * iy];
+      r.org[0] = org.x; r.org[1] = org.y; r.org[2] = org.z;
+      r.dir[0] = dir.x; r.dir[1] = dir.y; r.dir[2] = dir.z;
+      r.tmin = 0.f;
+      r.tmax = 1e30f;
+    }
+}
+
 void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame &fb, int spp, int threads,
             RenderStats *stats)
 {
   const int W = fb.w, H = fb.h;
   const bool compat13 = (rr.flags & BNX_RENDER_COMPAT_SPHERE_INSTID13) != 0;
-  const vec3 dir00 = load3(cam.dir_00), du = load3(cam.dir_du), dv = load3(cam.dir_dv), lens = load3(cam.lens_00);
   for (int sidx = 0; sidx < spp; sidx++) {
     const int accumID = fb.accum_id;
     // ---------------------------------------------- _generateRays (generateRays.cu:42-245)
@@ -499,10 +570,10 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
       float pv = (accumID == 0) ? .5f : rand();
       float image_u = ((float)ix + pu) / (float)W;
       float image_v = ((float)iy + pv) / (float)H;
-      float aspect = (float)W / (float)H;
-      ray.org = lens;
-      vec3 d = (dir00 + ((1.f * aspect) * (image_u - .5f)) * du) + (1.f * (image_v - .5f)) * dv;
-      ray.dir = normalize(d);
+      vec3 org, dir;
+      primary_ray(cam, W, H, accumID, image_u, image_v, rand, org, dir);
+      ray.org = org;
+      ray.dir = dir;
       ray.had_hit = false;
       ray.is_shadow = false;
       ray.in_medium = false;
@@ -659,7 +730,51 @@ void camera_perspective(const float *pos, const float *dir, const float *up, flo
     out.lens_00[k] = pos[k];
   }
   out.aperture_radius = 0.f;
-  out.focus_distance = 1.f;
+  out.focus_distance = 0.f;
 }
+
+// OrthographicCamera::commit (native/Camera.cu:168-181)
+void camera_orthographic(const float *pos, const float *dir, const float *up, float height, float aspect,
+                         bnx_camera &out)
+{
+  vec3 d00 = normalize(load3(dir));
+  vec3 ddu = normalize(cross(d00, load3(up)));
+  vec3 ddv = normalize(cross(ddu, d00));
+  memset(&out, 0, sizeof(out));
+  out.type = BNX_CAMERA_ORTHOGRAPHIC;
+  for (int k = 0; k < 3; k++) {
+    out.ortho.dir[k] = d00[k];
+    out.ortho.org_00[k] = pos[k];
+    out.ortho.org_du[k] = ddu[k];
+    out.ortho.org_dv[k] = ddv[k];
+  }
+  out.ortho.height = height;
+  out.ortho.aspect = aspect;
+}
+
+// OmniCamera::commit (native/Camera.cu:230-242): toWorld vz = -normalize(up),
+// vy = -normalize(cross(vz, direction)), vx = normalize(cross(vy, vz))
+void camera_omni(const float *pos, const float *dir, const float *up, bnx_camera &out)
+{
+  memset(&out, 0, sizeof(out));
+  out.type = BNX_CAMERA_OMNI;
+  vec3 vz = -normalize(load3(up));
+  vec3 vy = -normalize(cross(vz, load3(dir)));
+  vec3 vx = normalize(cross(vy, vz));
+  for (int k = 0; k < 3; k++) {
+    out.omni.to_world[k] = vx[k];
+    out.omni.to_world[3 + k] = vy[k];
+    out.omni.to_world[6 + k] = vz[k];
+    out.omni.position[k] = pos[k];
+  }
+}
+
+// linear_to_srgb (common/math.h:61-67) and make_8bit (:103-106)
+float linear_to_srgb(float x)
+{
+  if (x <= 0.0031308f) return 12.92f * x;
+  return 1.055f * powf(x, 1.f / 2.4f) - 0.055f;
+}
+uint32_t make_8bit(float f) { return (uint32_t)std::min(255, std::max(0, (int)(f * 256.f))); }
 
 } // namespace orc

@@ -1,6 +1,7 @@
 """Shared helpers for the parity tests (scene construction on both sides,
 canonical BVH comparison)."""
 import math
+import os
 
 import numpy as np
 
@@ -306,3 +307,87 @@ def instanced_spec(xforms=None):
         o2w, w2o = affine_pair(lin, off)
         spec.inst(grp, 10 + k, o2w, w2o)
     return spec
+
+
+# ---------------------------------------------------------------------------
+# the scene of the reference's unitTests/omnicamera.py (its fixture is
+# tests/golden/reference_unitTest_omnicamera.png): three matte spheres of
+# radius 0.2 at camera_from + right / front / top, an omnidirectional camera
+# at camera_from, ambient radiance 0.8, black background, 2048 x 1024
+OMNI_FROM = (-3.0, 1.5, -2.0)
+OMNI_DIR = (1.0, 0.1, 0.3)
+OMNI_UP = (0.0, 1.0, 0.0)
+OMNI_SIZE = (2048, 1024)
+OMNI_PNG = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "reference_unitTest_omnicamera.png")
+
+
+def _nrm64(a):
+    a = np.asarray(a, np.float64)
+    return a / math.sqrt(float(a @ a))
+
+
+def omni_spec():
+    """the spheres as omnicamera.py places them (positions in float64, then
+    float32 as its numpy arrays are); one user group, one instance"""
+    front = _nrm64(OMNI_DIR)
+    right = _nrm64(np.cross(front, OMNI_UP))
+    top = _nrm64(np.cross(right, front))
+    spec = SceneSpec()
+    gs = []
+    for k, (off, col) in enumerate(((right, (0.7, 0.1, 0.1)), (front, (0.1, 0.7, 0.1)), (top, (0.1, 0.1, 0.7)))):
+        pos = np.array([np.add(OMNI_FROM, off)], np.float32)
+        gs.append(spec.sph(pos, np.array([0.2], np.float32), None, user_id=k,
+                           material=abi.matte(col + (1.0,), from_attribute=False)))
+    spec.inst(spec.group(1, gs), 0)
+    return spec
+
+
+def omni_camera_args():
+    """position, direction, up as the ANARI camera passes them on
+    (anari/Camera.cpp:37-46 normalizes direction and up, and bnSet3f of
+    'direction' normalizes again, native/Camera.cu:226-229)"""
+    d = _nrm64(OMNI_DIR).astype(np.float32)
+    d = (d / np.float32(math.sqrt(float(d.astype(np.float64) @ d.astype(np.float64))))).astype(np.float32)
+    u = _nrm64(OMNI_UP).astype(np.float32)
+    return np.array(OMNI_FROM, np.float32), d, u
+
+
+def srgb_to_linear(c8):
+    c = np.asarray(c8, np.float64) / 255.0
+    return np.where(c <= 0.04045, c / 12.92, ((c + 0.055) / 1.055) ** 2.4)
+
+
+def check_against_omni_fixture(srgb8, objid):
+    """srgb8: (H, W, 4) uint8 frame in framebuffer row order (row 0 at the
+    bottom), objid: (H, W) object IDs.  Compares with the reference PNG:
+    every pixel hit is lit in the fixture; every lit fixture pixel is hit or
+    next to a hit pixel (the denoiser's bleed); per-sphere mean colour over
+    interior pixels, in linear space, within 1.5%."""
+    from PIL import Image
+    ref = np.array(Image.open(OMNI_PNG).convert("RGB")).astype(np.int64)[::-1]   # back to framebuffer rows
+    img = np.asarray(srgb8)[..., :3].astype(np.int64)
+    assert ref.shape == img.shape, (ref.shape, img.shape)
+    hit = objid >= 0
+    lit = (ref != 0).any(-1)
+    assert not (hit & ~lit).any(), int((hit & ~lit).sum())
+    grown = hit.copy()
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            grown |= np.roll(np.roll(hit, dy, 0), dx, 1)
+    assert not (lit & ~grown).any(), int((lit & ~grown).sum())
+    assert hit.sum() > 150000 and (lit & ~hit).sum() < 0.02 * hit.sum()
+    interior = hit.copy()
+    for dy in (-1, 0, 1):
+        for dx in (-1, 0, 1):
+            interior &= np.roll(np.roll(objid, dy, 0), dx, 1) == objid
+    for k, col in enumerate(((0.7, 0.1, 0.1), (0.1, 0.7, 0.1), (0.1, 0.1, 0.7))):
+        m = interior & (objid == k)
+        assert m.sum() > 10000, (k, int(m.sum()))
+        ours = srgb_to_linear(img[m]).mean(0)
+        theirs = srgb_to_linear(ref[m]).mean(0)
+        rel = ours / theirs - 1.0
+        print(f"omni fixture sphere {k}: ours {ours}, reference {theirs}, rel {rel}", flush=True)
+        assert np.all(np.abs(rel) < 0.015), (k, rel)
+        # the colour ratio of the material survives: 0.7 / 0.1 per channel
+        big = int(np.argmax(col))
+        assert 6.0 < ours[big] / np.delete(ours, big).mean() < 8.0

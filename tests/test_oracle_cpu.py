@@ -82,6 +82,125 @@ def test_triangle_closed_form(orc):
     assert (mh["prim_id"] == -1).all() and (mh["t"] == np.float32(1e30)).all()
 
 
+def _plane_quad_scene(orc, axis):
+    """the quad of _quad_scene with its coordinates permuted so that it
+    lies in the plane x = 0 (axis 0) or y = 0 (axis 1): the triangle test's
+    x- and y-dominant projections (rtcore/hiprt/TraceInterface.h:57-66)"""
+    perm = {0: [2, 0, 1], 1: [1, 2, 0]}[axis]   # quad's (x, y, 0) -> world
+    q = np.array([[-1, -1, 0], [1, -1, 0], [1, 1, 0], [-1, 1, 0]], np.float32)
+    v = np.zeros_like(q)
+    for k in range(3):
+        v[:, perm[k]] = q[:, k]
+    s = orc.Scene()
+    g = s.add_triangles(v, np.array([[0, 1, 2], [0, 2, 3]], np.int32), None, user_id=7)
+    s.add_instance(s.add_group(0, [g]), 3)
+    s.build()
+    return s, perm
+
+
+@pytest.mark.parametrize("axis", [0, 1])
+def test_triangle_closed_form_x_y_planes(orc, axis):
+    s, perm = _plane_quad_scene(orc, axis)
+    xs = np.array([-0.75, -0.3, 0.2, 0.6, 0.9], np.float32)
+    X, Y = np.meshgrid(xs, xs)
+    x, y = X.ravel().astype(np.float64), Y.ravel().astype(np.float64)
+    rays = np.zeros((25, 8), np.float32)
+    # quad coordinates (x, y, 2) mapped to the world, looking down the plane normal
+    rays[:, perm[0]], rays[:, perm[1]], rays[:, perm[2]] = x, y, 2.0
+    rays[:, 4 + perm[2]] = -1.0
+    rays[:, 7] = 1e30
+    hits, _ = s.trace(rays)
+    assert (hits["t"] == 2.0).all()
+    upper = y > x
+    assert (hits["prim_id"] == np.where(upper, 1, 0)).all()
+    u = np.where(upper, (x + 1) / 2.0, (x - y) / 2.0)
+    v = np.where(upper, (y - x) / 2.0, (y + 1) / 2.0)
+    assert np.allclose(hits["u"], u, atol=1e-6) and np.allclose(hits["v"], v, atol=1e-6)
+    # rays from the other side hit the same points at the same t
+    back = rays.copy()
+    back[:, perm[2]] = -2.0
+    back[:, 4 + perm[2]] = 1.0
+    hb, _ = s.trace(back)
+    assert (hb["t"] == 2.0).all() and (hb["prim_id"] == hits["prim_id"]).all()
+
+
+def test_triangle_rotated_quads(orc):
+    """random triangles of every orientation: t, u, v equal a float64 solve
+    of o + t d = v0 + u e1 + v e2, and hits fall exactly where the float64
+    solution says, away from the edges"""
+    rng = np.random.default_rng(3)
+    n = 400
+    c = (rng.random((n, 3)) - 0.5) * 4
+    v0 = c + rng.normal(size=(n, 3)) * 0.3
+    v1 = c + rng.normal(size=(n, 3)) * 0.3
+    v2 = c + rng.normal(size=(n, 3)) * 0.3
+    verts = np.stack([v0, v1, v2], 1).reshape(-1, 3).astype(np.float32)
+    idx = np.arange(3 * n, dtype=np.int32).reshape(n, 3)
+    vv = verts.astype(np.float64).reshape(n, 3, 3)
+    e1, e2 = vv[:, 1] - vv[:, 0], vv[:, 2] - vv[:, 0]
+    N = np.cross(e1, e2)
+    dom = np.argmax(np.abs(N), 1)
+    assert min((dom == k).sum() for k in range(3)) > 80     # every projection branch
+    checked = 0
+    for i in range(n):
+        s = orc.Scene()
+        g = s.add_triangles(verts[3 * i:3 * i + 3], np.array([[0, 1, 2]], np.int32), None, user_id=1)
+        s.add_instance(s.add_group(0, [g]), 0)
+        s.build()
+        bu, bv = rng.random(8), rng.random(8)
+        flip = bu + bv > 1
+        bu[flip], bv[flip] = 1 - bu[flip], 1 - bv[flip]
+        P = vv[i, 0] + bu[:, None] * e1[i] + bv[:, None] * e2[i]
+        o = P + 3.0 * N[i] / np.linalg.norm(N[i]) + rng.normal(size=(8, 3)) * 0.5
+        d = P - o
+        d /= np.linalg.norm(d, axis=1, keepdims=True)
+        rays = np.zeros((8, 8), np.float32)
+        rays[:, 0:3], rays[:, 4:7], rays[:, 7] = o, d, 1e30
+        h, _ = s.trace(rays)
+        ro, rd = rays[:, 0:3].astype(np.float64), rays[:, 4:7].astype(np.float64)
+        for k in range(8):
+            A = np.stack([e1[i], e2[i], -rd[k]], 1)
+            uu, vq, t = np.linalg.solve(A, ro[k] - vv[i, 0])
+            margin = min(uu, vq, 1 - uu - vq)
+            if margin < 1e-3:
+                continue
+            assert h["prim_id"][k] == 0, (i, k, uu, vq)
+            assert abs(h["t"][k] - t) <= 2e-5 * t
+            assert abs(h["u"][k] - uu) < 2e-4 and abs(h["v"][k] - vq) < 2e-4
+            checked += 1
+    assert checked > 2500
+
+
+def test_icosphere_watertight(orc):
+    """rays aimed anywhere within radius 0.98 of the unit icosphere's centre
+    hit it, at |P| within the chord error of 1; the hit triangles use every
+    projection branch of the triangle test"""
+    v, f = scenes.icosphere(5)
+    s = orc.Scene()
+    g = s.add_triangles(v.astype(np.float32), f.astype(np.int32), None, user_id=0)
+    s.add_instance(s.add_group(0, [g]), 0)
+    s.build()
+    rng = np.random.default_rng(8)
+    n = 200000
+    u = rng.normal(size=(n, 3))
+    org = 4.0 * u / np.linalg.norm(u, axis=1, keepdims=True)
+    w = rng.normal(size=(n, 3))
+    w = w / np.linalg.norm(w, axis=1, keepdims=True) * (0.98 * rng.random(n) ** 0.5)[:, None]
+    d = w - org
+    d /= np.linalg.norm(d, axis=1, keepdims=True)
+    rays = np.zeros((n, 8), np.float32)
+    rays[:, 0:3], rays[:, 4:7], rays[:, 7] = org, d, 1e30
+    h, _ = s.trace(rays)
+    assert (h["prim_id"] >= 0).all(), int((h["prim_id"] < 0).sum())
+    P = rays[:, 0:3].astype(np.float64) + h["t"][:, None].astype(np.float64) * rays[:, 4:7]
+    r = np.linalg.norm(P, axis=1)
+    assert r.max() < 1.0 + 1e-5 and r.min() > 1.0 - 6e-4
+    tri = v[f[h["prim_id"]]].astype(np.float64)
+    N = np.cross(tri[:, 1] - tri[:, 0], tri[:, 2] - tri[:, 0])
+    dom = np.argmax(np.abs(N), 1)
+    assert min((dom == k).sum() for k in range(3)) > 20000
+
+
 def test_sphere_closed_form(orc):
     s = orc.Scene()
     g = s.add_spheres(np.zeros((1, 3), np.float32), np.ones(1, np.float32), None, user_id=2)
