@@ -6,12 +6,15 @@ c5 (default): the serving stream.  Requests over resident worlds (config 1's
 surface scene, config 2's structured volume and config 3's 33.5M-triangle
 terrain with 4M spheres and an HDRI light) arrive with seeded Poisson
 arrivals, each with a resolution in {256^2, 512^2, 1024^2, 2048^2}, spp in
-{1, 4, 16, 64} and a camera on a seeded orbit.  A step is one request.  The
-same K requests are rendered (a) plain: one bnx_render per request, in
-arrival order, and (b) served: through the continuous-batching server.
-Reported: served Mrays/s at saturation (all K requests queued at once) as
-`value`, plain Mrays/s and their ratio, p50/p99 frame latency (arrival ->
-frame readable) of both forms under the same Poisson stream, and the plain
+{1, 4, 16, 64} and a camera on a seeded orbit.  The timed workload is a
+fixed sample: the first 200 requests of the seeded stream (--requests), after
+8 warm-up requests; --steps does not resize it (a step is one request of the
+sample, and the line reports the sample's size as `steps`).  The same
+requests are rendered (a) plain: one bnx_render per request, in arrival
+order, and (b) served: through the continuous-batching server.  Reported:
+served Mrays/s at saturation (all requests queued at once) as `value`,
+plain Mrays/s and their ratio, p50/p99 frame latency (arrival -> frame
+readable) of both forms under the same Poisson stream, and the plain
 single-request latency.  Served and plain frames are identical (checked on
 a sample at the end).
 
@@ -22,7 +25,15 @@ matte / glossy, HDRI light) or config 4 (1024^2, 32 spp, 20.5M-tet volume
 + isosurface).
 
 Every line carries the dominant kernel's roofline figures and the CPU
-oracle's throughput on a bounded sample of the same workload.
+oracle's throughput on a bounded sample of the same workload.  The roofline's
+algorithmic bytes follow the per-ray model 32 (ray in) + 16 (hit out) +
+64 N_node + 48 N_tri + 16 N_sph, with N_node counting binary-BVH node visits,
+and the counts taken by the CPU oracle's instrumented traversal of its own
+binary binned-SAH BVH on a stated ray subset of the same workload (shadow
+rays as occlusion queries, as the GPU traces them); per volume sample
+32 B (eight texels) + 4 B per macrocell step.  Beside it: the measured HBM
+traffic of the kernel (rocprofv3 FETCH_SIZE / WRITE_SIZE, in profiles/) and
+its fraction of the HBM peak.
 """
 from __future__ import annotations
 
@@ -37,22 +48,39 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 METRIC = "Mrays/s + p50/p99 frame latency at fixed RMSE, served batched vs plain GPU"
+C5_REQUESTS = 200           # the fixed sample of the serving stream
+C5_WARMUP = 8
+BYTE_MODEL_REQUESTS = 3     # per world: the oracle counts the rays of its first requests in the sample
+BYTE_MODEL_RES = 256        # ... rendered at 256^2, 1 spp
 HBM_PEAK_GBS = 8000.0
 CPU_TET_RES = 64            # config 4 CPU sample: the same scene and camera at 64x64
 CPU_BIG_RES = (160, 90)     # config 3 CPU sample: the same scene and camera at 160x90, 1 spp
 
 
-def algorithmic_bytes(rays, nodes, tris, spheres):
-    # per traced ray: 32 B ray in + 16 B hit out; 112 B per 4-wide node visit
-    # (six box vectors + child codes), 48 B per triangle record, 16 B per
-    # sphere record
-    return 48.0 * rays + 112.0 * nodes + 48.0 * tris + 16.0 * spheres
+def ray_bytes(rays, nodes, tris, spheres):
+    """per traced ray: 32 B ray in + 16 B hit out; 64 B per binary-BVH node
+    visit (two child boxes + refs), 48 B per triangle test (12 B index + 3 x
+    12 B vertices), 16 B per sphere test (centre + radius)"""
+    return 48.0 * rays + 64.0 * nodes + 48.0 * tris + 16.0 * spheres
 
 
-def march_bytes(samples, cells, rays):
-    # volume march kernel: 8 texels (32 B) per sample, 4 B per macrocell
-    # majorant, 32 B ray in + 16 B hit out per marched ray
-    return 32.0 * samples + 4.0 * cells + 48.0 * rays
+def march_bytes(samples, cells):
+    """structured volume: 32 B per sample (eight texels) + 4 B per
+    macrocell majorant read"""
+    return 32.0 * samples + 4.0 * cells
+
+
+def oracle_ray_model(o, cam, rend, w, h, spp=1):
+    """the oracle's traversal counts on the rays of one render (shadow rays
+    as occlusion queries): rays, nodes, tris, spheres"""
+    import oracle
+    fr = oracle.Frame(w, h)
+    _, st = oracle.render(o, cam, rend, fr, spp, shadow_any_hit=True)
+    return {k: int(st[k]) for k in ("rays", "nodes", "tris", "spheres")}
+
+
+def per_ray_bytes(counts):
+    return ray_bytes(counts["rays"], counts["nodes"], counts["tris"], counts["spheres"]) / max(1, counts["rays"])
 
 
 # rocprofv3 summaries of each workload's dominant kernel; each holds a line
@@ -93,8 +121,12 @@ def trace_kernel_name(tuning, max_world_prims):
     return f"{static}+k_trace_pool<{sh}>" if pooled else static
 
 
-def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false>"):
-    """dominant kernel (by summed HIP-event time) and its algorithmic GB/s"""
+def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false>", trace_bytes=None, model=None):
+    """dominant kernel (by summed HIP-event time) and its GB/s: algorithmic
+    (the byte model) and measured (HBM traffic per launch from profiles/).
+    trace_bytes: the trace kernels' algorithmic bytes over the timed region
+    from the oracle's counts (model: how they were counted); without it the
+    GPU's own counters are priced at the same per-item sizes"""
     if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
         # the full variant runs for tet fields and isosurfaces, the lean one otherwise
         full = tot.get("umesh_nodes", 0) > 0 or workload == "c4"
@@ -102,21 +134,36 @@ def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false>"):
         kms, launches = tot["march_ms"], tot["march_launches"]
         if tot.get("umesh_nodes", 0) > 0:
             byts = umesh_bytes(tot["umesh_nodes"], tot["umesh_tets"], tot["mc_cells"], tot["march_rays"])
+            model = ("tet point queries: 64 B per element-BVH node + 80 B per tet test (the GPU walk's counts; "
+                     "the oracle's depth-first query visits more) + 4 B per macrocell + 48 B per marched ray")
         else:
-            byts = march_bytes(tot["volume_samples"], tot["mc_cells"], tot["march_rays"])
+            byts = march_bytes(tot["volume_samples"], tot["mc_cells"])
+            model = ("32 B per volume sample + 4 B per macrocell step; sample and macrocell counts equal the "
+                     "oracle's (the march is bit-exact with it)")
     else:
         # queue traversal: one ray per thread, or pooled for the secondary rays
         # of large scenes when the tuning says so; the HIP events bracket
         # whichever ran
         kernel = trace_kernel
         kms, launches = tot["trace_ms"], tot["trace_launches"]
-        byts = algorithmic_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
+        if trace_bytes is not None:
+            byts = trace_bytes
+        else:
+            byts = ray_bytes(tot["rays"], tot["nodes"], tot["tris"], tot["spheres"])
+            model = ("48 B per ray + 64 B per node fetch + 48 B per triangle + 16 B per sphere, priced on the "
+                     "GPU's own counts of its 4-wide walk (no oracle counts in this run)")
     achieved = byts / (kms * 1e-3) / 1e9 if kms > 0 else 0.0
     traffic = profile_traffic(workload, kernel)
-    return {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
-            "bytes_per_launch": round(byts / max(1, launches)), "avg_launch_ms": round(kms / max(1, launches), 5),
-            "launches": launches, "kernel_ms": round(kms, 3)}
+    avg_ms = kms / max(1, launches)
+    out = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+           "frac": round(achieved / HBM_PEAK_GBS, 5), "traffic": traffic, "kernel": kernel,
+           "bytes_per_launch": round(byts / max(1, launches)), "avg_launch_ms": round(avg_ms, 5),
+           "launches": launches, "kernel_ms": round(kms, 3), "byte_model": model}
+    if traffic:
+        dram = traffic / (avg_ms * 1e-3) / 1e9
+        out["dram_gbs"] = round(dram, 2)
+        out["dram_frac"] = round(dram / HBM_PEAK_GBS, 5)
+    return out
 
 
 def pct(a, q):
@@ -296,7 +343,8 @@ def run_cpu(sc, budget_s):
                      f"the reference CPU path)"}
     if vol:
         out["volume_samples_per_s"] = vsamp / el
-    return out
+    counts = None if vol else oracle_ray_model(o, cam, rend, sc.width, sc.height, spp)
+    return out, counts
 
 
 # ---------------------------------------------------------------- serving
@@ -333,7 +381,7 @@ def serve_setup(dev, args, n, seed=None):
         cam = bx.camera_perspective(r.camera["position"], r.camera["direction"], r.camera["up"], r.camera["fovy"])
         rend = serve_renderer(sc)
         reqs.append(dict(world=gw.world, cam=cam, rend=rend, w=r.width, h=r.height, spp=r.spp, a1=r.arrival_s,
-                         scene=r.scene))
+                         scene=r.scene, camera=r.camera))
     return worlds, reqs
 
 
@@ -346,7 +394,7 @@ def serve_plain(dev, reqs, frames, per_sample=False):
     batches of up to render_paths paths"""
     import barney_amd as bx
     saved = dev.tuning()
-    svc, tot = [], {}
+    svc, tot, rays = [], {}, []
     try:
         for r in reqs:
             if per_sample:
@@ -356,11 +404,12 @@ def serve_plain(dev, reqs, frames, per_sample=False):
             fr.reset()
             st = bx.render(r["world"], r["cam"], r["rend"], fr, r["spp"])
             svc.append(time.perf_counter() - t)
+            rays.append(st["rays"])
             for k, v in st.items():
                 tot[k] = tot.get(k, 0) + v
     finally:
         dev.set_tuning(**saved)
-    return svc, tot
+    return svc, tot, rays
 
 
 def serve_served(dev, reqs, arrivals=None, cfg=None):
@@ -428,7 +477,11 @@ def run_c5(args, n, warmup):
 
     import barney_amd as bx
     dev = bx.Device(0)
-    worlds, reqs = serve_setup(dev, args, n + warmup)
+    # the stream's first C5_WARMUP requests are the warm-up pool, the next n
+    # the timed sample, whatever the warm-up count: a run with fewer warm-up
+    # requests (a profiler's) times the same sample
+    warmup = min(warmup, C5_WARMUP)
+    worlds, reqs = serve_setup(dev, args, C5_WARMUP + n)
     log("worlds resident")
     max_prims = max(int(getattr(sc, "num_prims", 0)) for _, sc in worlds)
     # one launch tuning for both forms: bnx_autotune on the stream's heaviest
@@ -439,7 +492,7 @@ def run_c5(args, n, warmup):
         cam3 = bx.camera_perspective(c3.camera["position"], c3.camera["direction"], c3.camera["up"],
                                      c3.camera["fovy"])
         tune = autotune(dev, gw3.world, cam3, serve_renderer(c3), 1024, 1024, 16)
-    warm, reqs = reqs[:warmup], reqs[warmup:]
+    warm, reqs = reqs[C5_WARMUP - warmup:C5_WARMUP], reqs[C5_WARMUP:]
     a1 = np.array([r["a1"] for r in reqs]) - reqs[0]["a1"]
     frames = {}
     for r in warm + reqs:
@@ -455,11 +508,11 @@ def run_c5(args, n, warmup):
     # into batches (bnx_render), and one pixel wave per sample (the
     # reference's frame loop)
     if not only_served:
-        svc, ptot = serve_plain(dev, reqs, frames)
+        svc, ptot, req_rays = serve_plain(dev, reqs, frames)
         svc = np.array(svc)
         plain_mrays = ptot["rays"] / svc.sum() / 1e6
         log(f"plain (merged samples): {svc.sum():.3f} s busy, {plain_mrays:.1f} Mrays/s")
-        svc_ps, ptot_ps = serve_plain(dev, reqs, frames, per_sample=True)
+        svc_ps, ptot_ps, _ = serve_plain(dev, reqs, frames, per_sample=True)
         svc_ps = np.array(svc_ps)
         plain_ps_mrays = ptot_ps["rays"] / svc_ps.sum() / 1e6
         log(f"plain (one pixel wave per sample): {svc_ps.sum():.3f} s busy, {plain_ps_mrays:.1f} Mrays/s")
@@ -499,7 +552,9 @@ def run_c5(args, n, warmup):
     ntri = sum(len(m.indices) for m in c3.meshes)
     nsph = sum(len(x.radii) for x in c3.spheres)
     per_scene = [sum(1 for r in reqs if r["scene"] == k) for k in range(3)]
-    desc = {"workload": f"config 5: serving stream, {n} requests (bounded sample of the 10,000-request stream) "
+    desc = {"workload": f"config 5: serving stream, a fixed sample of {n} requests (requests {C5_WARMUP}.."
+                        f"{C5_WARMUP + n - 1} of the seeded 10,000-request stream; the {warmup} before them run "
+                        f"untimed as warm-up; a step is one request of the sample) "
                         f"over resident worlds config 1 (20,480 triangles + 1,000 spheres), config 2 "
                         f"(256^3 structured volume) and config 3 ({ntri:,} triangles + {nsph:,} spheres, "
                         f"matte / glossy, HDRI light, 5 generations); resolution in {{256..2048}}^2, spp in "
@@ -507,6 +562,8 @@ def run_c5(args, n, warmup):
                         f"{args.load} of the plain form's capacity)",
             "requests": n, "warmup_requests": warmup, "qps": round(qps, 3), "resolutions": res,
             "form": "served (continuous batching) vs plain (one bnx_render per request)",
+            "target_judged_against": "plain: bnx_render per request in arrival order, samples merged into "
+                                     "batches of up to render_paths paths (the stronger of the two plain forms)",
             "scenes": ["config 1", "config 2", "config 3"], "requests_per_scene": per_scene, **tune,
             "max_world_prims": max_prims}
     extra = {
@@ -527,13 +584,18 @@ def run_c5(args, n, warmup):
         "served_p99_vs_plain_per_sample_alone_p99": round(pct(lat_served, 99) / pct(svc_ps * 1e3, 99), 3),
         "parity_sample": f"{same}/{checked} served frames identical to plain",
         "volume_samples_per_s": round(sst["volume_samples"] / makespan, 1),
+        "_rays_per_world": [int(sum(x for x, r in zip(req_rays, reqs) if r["scene"] == k)) for k in range(3)],
+        "_sample_reqs": reqs,
     }
     dev.close()
     return served_mrays, makespan, tot, desc, extra
 
 
-def run_cpu_c5(args, budget_s):
-    """the oracle on the smallest requests of the same stream"""
+def run_cpu_c5(args, budget_s, sample_reqs=None):
+    """the oracle on the smallest requests of the same stream; with the timed
+    sample's requests, also the oracle's traversal counts per world (the
+    byte model's ray subset: each world's first BYTE_MODEL_REQUESTS requests
+    of the sample, rendered at BYTE_MODEL_RES^2, 1 spp)"""
     import oracle
     from barney_amd import scenes
     if not os.path.exists(oracle.LIB_PATH):
@@ -564,9 +626,23 @@ def run_cpu_c5(args, budget_s):
         if time.perf_counter() - t0 > budget_s:
             break
     el = time.perf_counter() - t0
-    return {"value": round(rays / el / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-            "sample": f"the first {frames} 256^2 1-spp requests of the same stream (all three worlds), {el:.1f} s wall "
-                      f"on {threads} threads"}
+    out = {"value": round(rays / el / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
+           "sample": f"the first {frames} 256^2 1-spp requests of the same stream (all three worlds), {el:.1f} s wall "
+                     f"on {threads} threads"}
+    counts = None
+    if sample_reqs:
+        counts = []
+        for k in range(3):
+            tot = dict(rays=0, nodes=0, tris=0, spheres=0)
+            for r in [r for r in sample_reqs if r["scene"] == k][:BYTE_MODEL_REQUESTS]:
+                c = r["camera"]
+                cam = oracle.camera_perspective(c["position"], c["direction"], c["up"], c["fovy"])
+                m = oracle_ray_model(oscene(k), cam, serve_renderer((c1, c2, c3)[k]), BYTE_MODEL_RES,
+                                     BYTE_MODEL_RES)
+                for key in tot:
+                    tot[key] += m[key]
+            counts.append(tot)
+    return out, counts
 
 
 def copy_probe(mib):
@@ -585,7 +661,9 @@ def copy_probe(mib):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=None)
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed steps (c1-c4: renders); c5 times its fixed sample (--requests) whatever the value")
+    ap.add_argument("--requests", type=int, default=C5_REQUESTS, help="c5: size of the fixed sample")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--terrain-n", type=int, default=4096, help="c3: heightfield size (4096: 33.5M triangles)")
@@ -614,10 +692,13 @@ def main():
         sys.exit(2)
     import numpy as np
     if args.workload == "c5":
-        steps = args.steps if args.steps is not None else 200
-        warmup = args.warmup if args.warmup is not None else 8
+        # the fixed sample: --steps does not resize it
+        steps = args.requests
+        warmup = args.warmup if args.warmup is not None else C5_WARMUP
         mrays, elapsed, tot, desc, extra = run_c5(args, steps, warmup)
         lat_line = extra.pop("frame_latency_ms")
+        if args.steps is not None:
+            desc["steps_requested"] = args.steps
     else:
         heavy = args.workload in ("c3", "c4")
         steps = args.steps if args.steps is not None else (3 if heavy else 20)
@@ -632,19 +713,41 @@ def main():
         extra = {"rays_per_step": tot["rays"] // steps}
         if tot["volume_samples"]:
             extra["volume_samples_per_s"] = round(tot["volume_samples"] / elapsed, 1)
+    rays_per_world = extra.pop("_rays_per_world", None)
+    sample_reqs = extra.pop("_sample_reqs", None)
+    cpu, trace_bytes, model, byte_detail = None, None, None, None
+    if not args.no_cpu:
+        if args.workload == "c5":
+            cpu, counts = run_cpu_c5(args, args.cpu_seconds, sample_reqs)
+            if counts and rays_per_world:
+                per = [per_ray_bytes(c) for c in counts]
+                trace_bytes = sum(r * b for r, b in zip(rays_per_world, per))
+                model = (f"oracle binary-BVH counts on each world's first {BYTE_MODEL_REQUESTS} requests of the "
+                         f"sample at {BYTE_MODEL_RES}^2, 1 spp (shadow rays as occlusion queries), per-ray bytes "
+                         f"times that world's rays in the timed region")
+                byte_detail = {"worlds": ["config 1", "config 2", "config 3"], "oracle_counts": counts,
+                               "bytes_per_ray": [round(b, 2) for b in per], "rays": rays_per_world,
+                               "note": "config 2 rays (volume world, no surfaces) cost 48 B in the trace launch"}
+        else:
+            cpu, counts = run_cpu(sc, args.cpu_seconds)
+            if counts:
+                trace_bytes = tot["rays"] * per_ray_bytes(counts)
+                model = ("oracle binary-BVH counts on the CPU sample's rays (shadow rays as occlusion queries): "
+                         "per-ray bytes times the GPU's rays")
+                byte_detail = {"oracle_counts": counts, "bytes_per_ray": round(per_ray_bytes(counts), 2)}
     line = {
         "metric": METRIC, "value": round(mrays, 3), "unit": "Mrays/s", "n_gpus": 1, "steps": steps,
         "warmup": warmup, "ms_per_step": round(elapsed * 1e3 / steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic (seeded PCG32 scenes and stream)",
         "config": desc, "frame_latency_ms": lat_line,
-        "roofline": roofline(tot, args.workload, trace_kernel_name(desc["tuning"], desc.get("max_world_prims", 0))),
+        "roofline": roofline(tot, args.workload, trace_kernel_name(desc["tuning"], desc.get("max_world_prims", 0)),
+                             trace_bytes, model),
     }
+    if byte_detail and line["roofline"]["kernel"].startswith("k_trace"):
+        line["roofline"]["byte_model_detail"] = byte_detail
     line.update(extra)
-    if not args.no_cpu:
-        if args.workload == "c5":
-            line["cpu_baseline"] = run_cpu_c5(args, args.cpu_seconds)
-        else:
-            line["cpu_baseline"] = run_cpu(sc, args.cpu_seconds)
+    if cpu is not None:
+        line["cpu_baseline"] = cpu
     print(json.dumps(line), flush=True)
 
 

@@ -93,7 +93,7 @@ def lib():
         L.orc_frame_destroy.argtypes = [vp]
         L.orc_frame_reset.argtypes = [vp]
         L.orc_render.argtypes = [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.c_int,
-                                 C.POINTER(C.c_uint64)]
+                                 C.POINTER(C.c_uint64), C.c_int]
         L.orc_render.restype = C.c_double
         L.orc_frame_read.argtypes = [vp, C.c_int, vp]
         L.orc_sincos.argtypes = [C.c_float, abi.fp, abi.fp]
@@ -444,9 +444,12 @@ class Frame:
 
 
 def render(scene: Scene, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp: int,
-           threads: int | None = None):
+           threads: int | None = None, shadow_any_hit: bool = False):
+    """shadow_any_hit: count shadow rays as occlusion queries (the image is
+    the same; only the traversal counters change)"""
     st = (C.c_uint64 * 7)()
-    ms = scene.L.orc_render(scene.h, C.byref(cam), C.byref(rend), frame.h_, spp, threads or default_threads(), st)
+    ms = scene.L.orc_render(scene.h, C.byref(cam), C.byref(rend), frame.h_, spp, threads or default_threads(), st,
+                            1 if shadow_any_hit else 0)
     return ms, dict(rays=st[0], shadow=st[1], generations=st[2], volume_samples=st[3], nodes=st[4],
                     tris=st[5], spheres=st[6])
 

@@ -453,11 +453,13 @@ orc_frame *orc_frame_create(int w, int h)
 void orc_frame_destroy(orc_frame *f) { delete f; }
 void orc_frame_reset(orc_frame *f) { f->f.reset(); }
 
-// stats: rays, shadow rays, generations, volume samples, nodes, tris, spheres
+// stats: rays, shadow rays, generations, volume samples, nodes, tris, spheres.
+// flags bit 0: count shadow rays as occlusion queries (RenderStats::shadow_any_hit)
 double orc_render(orc_scene *sc, const bnx_camera *cam, const bnx_renderer *r, orc_frame *fb, int spp, int threads,
-                  uint64_t *stats)
+                  uint64_t *stats, int flags)
 {
   RenderStats st;
+  st.shadow_any_hit = (flags & 1) != 0;
   auto t0 = std::chrono::steady_clock::now();
   render(sc->s, *cam, *r, fb->f, spp, threads, stats ? &st : nullptr);
   double ms = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();

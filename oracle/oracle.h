@@ -376,6 +376,9 @@ struct Frame {
 struct RenderStats {
   uint64_t rays = 0, shadow = 0, generations = 0, volume_samples = 0;
   TraceCounters ctr;
+  // input: trace shadow rays as occlusion (any-hit) queries, as the GPU
+  // path does; the image is the same, only the traversal counters change
+  bool shadow_any_hit = false;
 };
 
 void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &r, Frame &fb, int spp,

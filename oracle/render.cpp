@@ -610,7 +610,8 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
       parallel_for(n, threads, [&](int i) {
         PathRay &r = queue[i];
         TraceCounters c;
-        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, false, stats ? &c : nullptr, rr.cut_plane);
+        const bool occlusion = r.is_shadow && stats && stats->shadow_any_hit;
+        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, occlusion, stats ? &c : nullptr, rr.cut_plane);
         if (gen == 0) ids[i] = {BNX_INF, -1, -1, -1};
         // isosurfaces, then volumes, after the surface trace, each clipped
         // to the closest hit so far
