@@ -48,10 +48,27 @@ def _compile(src: str) -> str:
     return obj
 
 
+# host-only C++ over the C-ABI, linked into the same library: barney's bn*
+# API (include/bnx_barney.h)
+HOST_SOURCES = ["bnx_barney.cpp"]
+HOST_FLAGS = ["-O2", "-std=c++17", "-fPIC", "-Wall", "-Wextra", "-I", os.path.join(ROOT, "include")]
+
+
+def _compile_host(src: str) -> str:
+    obj = os.path.join(OBJ_DIR, os.path.splitext(src)[0] + ".host.o")
+    cmd = ["g++", *HOST_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+    r = subprocess.run(cmd, capture_output=True, text=True)
+    if r.returncode != 0:
+        raise RuntimeError(f"g++ failed for {src}:\n{r.stderr[-8000:]}")
+    return obj
+
+
 def build(verbose: bool = False) -> str:
     os.makedirs(OBJ_DIR, exist_ok=True)
-    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES))) as ex:
-        objs = list(ex.map(_compile, SOURCES))
+    with cf.ThreadPoolExecutor(max_workers=min(8, len(SOURCES) + len(HOST_SOURCES))) as ex:
+        hip = ex.map(_compile, SOURCES)
+        host = ex.map(_compile_host, HOST_SOURCES)
+        objs = list(hip) + list(host)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), f"--offload-arch={ARCH}", "-shared", "-fPIC", *objs, "-o", tmp]
     r = subprocess.run(cmd, capture_output=True, text=True)
@@ -66,7 +83,7 @@ def build(verbose: bool = False) -> str:
 
 # host programs over the C-ABI (g++, no HIP headers): the C++ mirror of
 # barney's rtc interface driven as barney's native layer would
-HOST_PROGRAMS = [("tests/cpp/rtc_trace.cpp", "tests/cpp/rtc_trace")]
+HOST_PROGRAMS = [("tests/cpp/rtc_trace.cpp", "tests/cpp/rtc_trace"), ("tests/cpp/bn_render.cpp", "tests/cpp/bn_render")]
 
 
 def build_host_programs(verbose: bool = False):

@@ -922,7 +922,12 @@ int bnx_volume_create_structured(bnx_device d, const bnx_structured_desc *f, con
     dd.mc_rcp_spacing[k] = 1.f / ((float)c * f->grid_spacing[k]);
   }
   dd.user_id = user_id;
-  if ((rc = set_tf(v, tf)) != BNX_OK || (rc = volume_build(v)) != BNX_OK) {
+  // the sampler reads the bricked copy (bricks of 8^3 cells unless
+  // BNX_VOLUME_BRICK says 2^shift, 0: the linear array); a layout only, the
+  // values and results are the same
+  int shift = 3;
+  if (const char *e = getenv("BNX_VOLUME_BRICK")) shift = std::min(5, std::max(0, atoi(e)));
+  if ((rc = set_tf(v, tf)) != BNX_OK || (rc = volume_build(v)) != BNX_OK || (rc = volume_brick(v, shift)) != BNX_OK) {
     delete v;
     return rc;
   }

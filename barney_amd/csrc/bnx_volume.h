@@ -25,6 +25,13 @@ struct VolumeDev {
   int field;                  // FIELD_STRUCTURED / FIELD_UMESH
   // structured (StructuredData, volume/StructuredData.h:30-74)
   const float BNX_G *scalars;   // dims[0]*dims[1]*dims[2], x fastest
+  // the same values in bricks of B^3 cells (B = 1 << brick_shift) with a
+  // one-texel apron: (B+1)^3 texels per brick, x fastest inside a brick,
+  // bricks x fastest; every trilinear footprint lies inside the brick of
+  // its lower corner (null: sample the linear array)
+  const float BNX_G *bricks;
+  int brick_shift;
+  int nbricks[3];
   int dims[3];
   int ncells[3];
   float origin[3];
@@ -76,7 +83,7 @@ struct Volume {
   Device *dev;
   uint64_t version = 0;         // bumped whenever dd changes
   VolumeDev dd{};
-  DevMem scalars, ranges, majorants, tf;
+  DevMem scalars, bricks, ranges, majorants, tf;
   DevMem uverts, utets, unbr, useed;
   Bvh ebvh;                     // element BVH (tets)
 };
@@ -90,6 +97,9 @@ struct IsoSurface {
 };
 
 int volume_build(Volume *v);
+// the structured field's bricked copy (VolumeDev::bricks) with bricks of
+// 2^shift cells; shift 0 keeps the linear layout only
+int volume_brick(Volume *v, int shift);
 // tets: element BVH, macrocell ranges by rasterTet<5>, majorants; tets4 is
 // the host copy of the tets (num_tets x int4), verts4 of the vertices
 int volume_build_umesh(Volume *v, const int4 *tets4, int num_tets, const float4 *verts4, int num_verts);
@@ -152,12 +162,29 @@ BNX_D float structured_sample(const VolumeDev &v, v3 P)
   lerp_coord(v.dims[0], rx + .5f, x0, x1, fx);
   lerp_coord(v.dims[1], ry + .5f, y0, y1, fy);
   lerp_coord(v.dims[2], rz + .5f, z0, z1, fz);
-  const size_t Nx = (size_t)v.dims[0], NxNy = (size_t)v.dims[0] * (size_t)v.dims[1];
-  const float BNX_G *sc = v.scalars;
-  const float BNX_G *b00 = sc + (size_t)y0 * Nx + (size_t)z0 * NxNy;
-  const float BNX_G *b01 = sc + (size_t)y1 * Nx + (size_t)z0 * NxNy;
-  const float BNX_G *b10 = sc + (size_t)y0 * Nx + (size_t)z1 * NxNy;
-  const float BNX_G *b11 = sc + (size_t)y1 * Nx + (size_t)z1 * NxNy;
+  const float BNX_G *b00, *b01, *b10, *b11;
+  if (v.bricks) {
+    // the brick of the lower corner holds the whole footprint (apron)
+    const int sh = v.brick_shift, E = (1 << sh) + 1;
+    const int bx = min(x0 >> sh, v.nbricks[0] - 1), by = min(y0 >> sh, v.nbricks[1] - 1),
+              bz = min(z0 >> sh, v.nbricks[2] - 1);
+    const float BNX_G *br =
+        v.bricks + (uint32_t)((bz * v.nbricks[1] + by) * v.nbricks[0] + bx) * (uint32_t)(E * E * E);
+    const int lx = x0 - (bx << sh), ly0 = y0 - (by << sh), ly1 = y1 - (by << sh), lz0 = z0 - (bz << sh),
+              lz1 = z1 - (bz << sh);
+    b00 = br + (ly0 + E * lz0) * E - (bx << sh);
+    b01 = br + (ly1 + E * lz0) * E - (bx << sh);
+    b10 = br + (ly0 + E * lz1) * E - (bx << sh);
+    b11 = br + (ly1 + E * lz1) * E - (bx << sh);
+    (void)lx;
+  } else {
+    const size_t Nx = (size_t)v.dims[0], NxNy = (size_t)v.dims[0] * (size_t)v.dims[1];
+    const float BNX_G *sc = v.scalars;
+    b00 = sc + (size_t)y0 * Nx + (size_t)z0 * NxNy;
+    b01 = sc + (size_t)y1 * Nx + (size_t)z0 * NxNy;
+    b10 = sc + (size_t)y0 * Nx + (size_t)z1 * NxNy;
+    b11 = sc + (size_t)y1 * Nx + (size_t)z1 * NxNy;
+  }
   float v00 = lerp_l(fx, b00[x0], b00[x1]);
   float v01 = lerp_l(fx, b01[x0], b01[x1]);
   float v10 = lerp_l(fx, b10[x0], b10[x1]);

@@ -196,12 +196,15 @@ __global__ void k_umesh_raster(const int4 *tets, const float4 *verts, int n, Vol
 // trace kernel is skipped and every ray enters here as a surface miss
 // (surfaces == 0).  FULL: isosurfaces and tet fields may occur; otherwise
 // every volume of the batch is structured and there are no isosurfaces.
-// waves per SIMD the march kernel is compiled for, lean and full variant
+// waves per SIMD the march kernel is compiled for, lean and full variant.
+// The full variant (tet point location + isosurfaces) needs about 126 VGPRs:
+// at 4 waves it fits without spilling; at 6 it spilled 128 VGPRs to scratch,
+// and the spill stores dominated its memory writes
 #ifndef BNX_MARCH_WAVES_LEAN
 #define BNX_MARCH_WAVES_LEAN 5
 #endif
 #ifndef BNX_MARCH_WAVES_FULL
-#define BNX_MARCH_WAVES_FULL 6
+#define BNX_MARCH_WAVES_FULL 4
 #endif
 template <bool FULL>
 __global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? BNX_MARCH_WAVES_FULL
@@ -282,7 +285,49 @@ k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
   }
 }
 
+// one thread per brick texel: the texel of the linear field at the brick's
+// cell origin + (lx, ly, lz), clamped to the field (apron texels past the
+// last vertex repeat it; no footprint reads them)
+__global__ void k_brick(const float *src, int nx, int ny, int nz, int shift, int nbx, int nby, long long total,
+                        float *dst)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  const int E = (1 << shift) + 1, E3 = E * E * E;
+  const long long b = i / E3;
+  const int t = (int)(i - b * E3);
+  const int lx = t % E, ly = (t / E) % E, lz = t / (E * E);
+  const int bx = (int)(b % nbx), by = (int)((b / nbx) % nby), bz = (int)(b / ((long long)nbx * nby));
+  const int x = min((bx << shift) + lx, nx - 1), y = min((by << shift) + ly, ny - 1),
+            z = min((bz << shift) + lz, nz - 1);
+  dst[i] = src[(size_t)x + (size_t)nx * ((size_t)y + (size_t)ny * z)];
+}
+
 } // namespace
+
+int volume_brick(Volume *v, int shift)
+{
+  VolumeDev &d = v->dd;
+  if (shift <= 0) {
+    v->bricks.release();
+    d.bricks = nullptr;
+    d.brick_shift = 0;
+    return BNX_OK;
+  }
+  const int B = 1 << shift, E = B + 1;
+  for (int k = 0; k < 3; k++) d.nbricks[k] = (d.ncells[k] + B - 1) / B;
+  const long long total = (long long)d.nbricks[0] * d.nbricks[1] * d.nbricks[2] * E * E * E;
+  BNX_CHECK(total < (1LL << 31), BNX_ERR_UNSUPPORTED, "structured field too large for the bricked layout");
+  BNX_HIP(v->bricks.ensure(sizeof(float) * (size_t)total));
+  k_brick<<<nblk(total, 256), 256, 0, v->dev->stream>>>(v->scalars.as<float>(), d.dims[0], d.dims[1], d.dims[2],
+                                                        shift, d.nbricks[0], d.nbricks[1], total,
+                                                        v->bricks.as<float>());
+  BNX_HIP(hipGetLastError());
+  BNX_SET(d.bricks, v->bricks.ptr);
+  d.brick_shift = shift;
+  v->version++;
+  return BNX_OK;
+}
 
 int volume_build(Volume *v)
 {

@@ -15,6 +15,27 @@ def header_functions():
     return sorted(set(re.findall(r"\b(bnx_[a-z0-9_]+)\s*\(", txt)))
 
 
+def barney_api_functions():
+    """the bn* functions include/bnx_barney.h declares"""
+    txt = open(os.path.join(ROOT, "include", "bnx_barney.h")).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"\b(bn[A-Z][A-Za-z0-9]+)\s*\(", txt)))
+
+
+def test_library_exports_the_barney_api():
+    """every bn* entry point of the mirrored barney.h, with C linkage"""
+    import barney_amd
+    if not os.path.exists(barney_amd.LIB_PATH):
+        pytest.skip("HIP library not built here")
+    L = C.CDLL(barney_amd.LIB_PATH)
+    fns = barney_api_functions()
+    for f in ["bnContextCreate", "bnGeometryCreate", "bnSetData", "bnGroupCreate", "bnSetInstances", "bnBuild",
+              "bnRender", "bnFrameBufferRead", "bnVolumeSetXF", "bnCommit"]:
+        assert f in fns
+    missing = [f for f in fns if not hasattr(L, f)]
+    assert not missing, missing
+
+
 def test_header_declares_core_entry_points():
     fns = header_functions()
     for f in ["bnx_device_create", "bnx_group_build", "bnx_trace", "bnx_render", "bnx_frame_read"]:
