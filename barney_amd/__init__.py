@@ -95,6 +95,8 @@ _PROTOS = [
     ("bnx_frame_destroy", [vp], C.c_int),
     ("bnx_render", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.POINTER(abi.RenderStats)],
      C.c_int),
+    ("bnx_render_noalloc", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int,
+                            C.POINTER(abi.RenderStats)], C.c_int),
     ("bnx_device_count", [C.POINTER(C.c_int)], C.c_int),
     ("bnx_device_get_tuning", [vp, C.POINTER(abi.Tuning)], C.c_int),
     ("bnx_device_set_tuning", [vp, C.POINTER(abi.Tuning)], C.c_int),

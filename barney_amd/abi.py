@@ -6,6 +6,7 @@ import ctypes as C
 import numpy as np
 
 BNX_OK = 0
+BNX_ERR_INVALID, BNX_ERR_HIP, BNX_ERR_OOM, BNX_ERR_UNSUPPORTED, BNX_ERR_STATE = -1, -2, -3, -4, -5
 BNX_MATERIAL_MATTE = 0
 BNX_MATERIAL_PBR = 1
 BNX_CAMERA_PERSPECTIVE = 1

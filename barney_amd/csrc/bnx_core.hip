@@ -417,6 +417,12 @@ int bnx_device_create(int gpu_id, bnx_device *out)
   if (const char *e = getenv("BNX_TRACE_REFILL")) d->trace_refill = std::min(64, std::max(1, atoi(e)));
   if (const char *e = getenv("BNX_TRACE_REFILL_PRIMS")) d->trace_refill_prims = std::max(0LL, atoll(e));
   if (const char *e = getenv("BNX_TRACE_BLOCKS_PER_CU")) d->trace_blocks_per_cu = std::min(1 << 20, std::max(1, atoi(e)));
+  int rc = noalloc_init(d);
+  if (rc != BNX_OK) {
+    (void)hipStreamDestroy(d->stream);
+    delete d;
+    return rc;
+  }
   *out = d;
   return BNX_OK;
   GUARD_END
@@ -427,6 +433,7 @@ int bnx_device_destroy(bnx_device d)
   BNX_CHECK(d, BNX_ERR_INVALID, "bnx_device_destroy: null device");
   (void)hipStreamSynchronize(d->stream);
   wavefront_destroy(d);
+  noalloc_destroy(d);
   (void)hipStreamDestroy(d->stream);
   delete d;
   return BNX_OK;
@@ -1092,7 +1099,10 @@ int bnx_world_add_isosurface(bnx_world w, bnx_isosurface s, int32_t user_inst_id
   w->isos.push_back(s);
   w->iso_inst_ids.push_back(user_inst_id);
   w->iso_seen.clear();
-  return BNX_OK;
+  // the world's table now (renders only refresh it in place)
+  const IsoDev BNX_G *isos = nullptr;
+  int count = 0;
+  return world_isos(w, isos, count);
   GUARD_END
 }
 
@@ -1105,7 +1115,13 @@ int bnx_world_add_volume(bnx_world w, bnx_volume v, int32_t user_inst_id)
   w->volume_inst_ids.push_back(user_inst_id);
   w->vol_seen.clear();
   w->iso_seen.clear();                 // isosurface rtc instances follow the volumes
-  return BNX_OK;
+  // the world's tables now (renders only refresh them in place)
+  VolumeList vl{};
+  int rc = world_volumes(w, vl);
+  if (rc != BNX_OK) return rc;
+  const IsoDev BNX_G *isos = nullptr;
+  int count = 0;
+  return world_isos(w, isos, count);
   GUARD_END
 }
 

@@ -31,16 +31,19 @@
 namespace bnx {
 namespace {
 
-// pinned host memory for read-backs and small uploads
+// pinned host memory for read-backs and small uploads, or (borrowed) a
+// fixed slice of host memory owned elsewhere
 struct Pinned {
   void *p = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;
   ~Pinned()
   {
-    if (p) (void)hipHostFree(p);
+    if (p && !borrowed) (void)hipHostFree(p);
   }
   hipError_t ensure(size_t b)
   {
+    if (borrowed) return b <= bytes ? hipSuccess : hipErrorOutOfMemory;
     if (b <= bytes && p) return hipSuccess;
     if (p) (void)hipHostFree(p);
     p = nullptr;
@@ -48,6 +51,12 @@ struct Pinned {
     hipError_t e = hipHostMalloc(&p, b ? b : 16, hipHostMallocDefault);
     if (e == hipSuccess) bytes = b;
     return e;
+  }
+  void borrow(void *q, size_t b)
+  {
+    p = q;
+    bytes = b;
+    borrowed = true;
   }
   template <typename T> T *as() const { return (T *)p; }
 };
@@ -98,6 +107,10 @@ struct Wavefront {
   std::vector<BatchGraph> graphs;
   std::vector<const void *> graph_bufs;
   bool graphs_off = false;      // capture failed once: enqueue directly
+  // every buffer is a fixed slice of the static scratch array (device) and
+  // of host_stage (host): nothing is allocated, no timing events exist, and
+  // the paths per batch are bounded by the slices
+  bool noalloc = false;
   ~Wavefront() { drop_graphs(); }
   void drop_graphs()
   {
@@ -135,7 +148,7 @@ struct Wavefront {
   {
     int rc = Q.ensure(rays);
     if (rc != BNX_OK) return rc;
-    if (!Q.ev_begin) {
+    if (!Q.ev_begin && !noalloc) {
       BNX_HIP(hipEventCreate(&Q.ev_begin));
       BNX_HIP(hipEventCreate(&Q.ev_end));
       BNX_HIP(hipEventCreate(&Q.ev_vbegin));
@@ -226,7 +239,8 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool sur
   TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
   // with neither surfaces nor volumes the trace kernel still writes the misses
   const bool trace = surfaces || !volumes;
-  BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
+  const bool timed = Q.ev_begin != nullptr;
+  if (timed) BNX_HIP(hipEventRecord(Q.ev_begin, dev->stream));
   bool pooled = false, cut = false;
   const SlotDev *hs = F.h_slots.as<SlotDev>();
   for (int k = 0; k < sl.n; k++) {
@@ -235,11 +249,11 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool sur
   }
   if (trace && (rc = launch_trace_queue(dev, slots, sv, sl.blocks256, sl.blocksT, pooled, cut, tq, st)) != BNX_OK)
     return rc;
-  BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
+  if (timed) BNX_HIP(hipEventRecord(Q.ev_end, dev->stream));
   if (volumes) {
-    BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
+    if (timed) BNX_HIP(hipEventRecord(Q.ev_vbegin, dev->stream));
     if ((rc = launch_volume_march(dev, slots, sv, sl.blocks256, in, trace, full, st)) != BNX_OK) return rc;
-    BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
+    if (timed) BNX_HIP(hipEventRecord(Q.ev_vend, dev->stream));
   }
   const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
   return launch_shade(dev, slots, sv, sl.blocks256, sl.blocks1024, in, out, so, st);
@@ -339,6 +353,7 @@ int batch_graph(Device *dev, Wavefront &F, const BatchShape &b, int G, int cur0,
 int collect_timing(Wavefront &F, int n, bool surfaces, bool volumes, GenTiming &t)
 {
   float ms = 0.f;
+  if (!F.Q.ev_begin) return BNX_OK;       // untimed (bnx_render_noalloc)
   if (surfaces) {
     BNX_HIP(hipEventElapsedTime(&ms, F.Q.ev_begin, F.Q.ev_end));
     t.trace_ms += ms;
@@ -414,21 +429,47 @@ void wavefront_destroy(Device *dev)
   dev->queues = nullptr;
 }
 
-int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp, bnx_render_stats *stats)
+namespace {
+
+// zero n 32-bit words: a kernel, where hipMemsetAsync would be a memset node
+// in a caller's captured graph
+__global__ void k_zero_u32(uint32_t *p, long long n)
+{
+  const long long i = (long long)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) p[i] = 0u;
+}
+
+int zero_words(Device *dev, void *p, size_t bytes, bool kernel)
+{
+  if (!kernel) {
+    BNX_HIP(hipMemsetAsync(p, 0, bytes, dev->stream));
+    return BNX_OK;
+  }
+  const long long n = (long long)(bytes / 4);
+  if (n > 0) k_zero_u32<<<(int)((n + 255) / 256), 256, 0, dev->stream>>>((uint32_t *)p, n);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+// the frame loop behind bnx_render and bnx_render_noalloc, over wavefront F;
+// max_paths: paths per batch
+int render_frame_impl(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp,
+                      bnx_render_stats *stats, Wavefront &F, long long max_paths)
 {
   auto t0 = std::chrono::steady_clock::now();
   Device *dev = w->dev;
-  Wavefront &F = wavefront_for(dev);
   const long long np = (long long)f->w * f->h;
-  // samples per batch: about dev->render_paths paths in flight
-  const int S = (int)std::max(1LL, std::min({(long long)spp, dev->render_paths / std::max(1LL, np),
+  // samples per batch: about max_paths paths in flight
+  const int S = (int)std::max(1LL, std::min({(long long)spp, max_paths / std::max(1LL, np),
                                              (long long)MAX_BATCH_SAMPLES}));
   BNX_CHECK(2 * np * S < (1LL << 31), BNX_ERR_UNSUPPORTED, "bnx_render: frame too large for one batch");
   int rc;
   // one batch of paths leaves at most twice as many rays (one shadow ray and
   // one continuation ray per path) in the output region
   if ((rc = F.ensure((size_t)(2 * np * S), 1, 1, 1)) != BNX_OK) return rc;
-  BNX_HIP(hipMemsetAsync(F.Q.stats.ptr, 0, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, dev->stream));
+  if ((rc = zero_words(dev, F.Q.stats.ptr, sizeof(unsigned long long) * STAT_COUNT * STAT_SLOTS, F.noalloc)) !=
+      BNX_OK)
+    return rc;
   SlotDev *hs = F.h_slots.as<SlotDev>();
   bool full = false;
   if ((rc = fill_slot(w, cam, rr, f, *hs, full)) != BNX_OK) return rc;
@@ -457,7 +498,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     // batches of up to graph_paths paths: once the generations a batch of
     // this shape takes are known (from a first batch through the host loop
     // below), run that many as one captured bounce loop with one read-back
-    const bool dd = dev->graph_paths > 0 && np * nS <= dev->graph_paths;
+    const bool dd = !F.noalloc && dev->graph_paths > 0 && np * nS <= dev->graph_paths;
     const BatchShape shape{np * nS,          2 * np * nS,           surfaces,
                            volumes,          full,                  trace_refill_for(dev, w),
                            dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu, dev->shade_block,
@@ -509,7 +550,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     // secondary generations of surface worlds may be reordered first (into
     // the other queue, whose rays are spent)
     const Bvh &tlas = w->instances->bvh;
-    const bool sorting = dev->sort_rays != 0 && traced && tlas.num_prims > 0;
+    const bool sorting = !F.noalloc && dev->sort_rays != 0 && traced && tlas.num_prims > 0;
     for (; count > 0; primary = false) {
       if (sorting && !primary && count >= SORT_MIN_RAYS) {
         if ((rc = sort_rays(dev, F.sorter, F.Q.q(cur), F.Q.q(1 - cur), count, tlas.root_lo, tlas.root_hi)) != BNX_OK)
@@ -518,7 +559,7 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
       }
       SegList sl{F.h_segs.as<Segment>()};
       sl.add(0, 0, count, primary ? 64 : trace_refill_for(dev, w));
-      BNX_HIP(hipMemsetAsync(F.out_cnt.ptr, 0, 2 * sizeof(int), dev->stream));
+      if ((rc = zero_words(dev, F.out_cnt.ptr, 2 * sizeof(int), F.noalloc)) != BNX_OK) return rc;
       if ((rc = enqueue_generation(dev, F, cur, sl, surfaces, volumes, full)) != BNX_OK) return rc;
       BNX_HIP(hipMemcpyAsync(hcnt, F.out_cnt.ptr, 2 * sizeof(int), hipMemcpyDeviceToHost, dev->stream));
       BNX_HIP(hipStreamSynchronize(dev->stream));
@@ -557,6 +598,108 @@ int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame 
     stats->umesh_tets = (int64_t)host_st[STAT_UTETS];
   }
   return BNX_OK;
+}
+
+} // namespace
+
+int render_frame(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, int spp, bnx_render_stats *stats)
+{
+  Device *dev = w->dev;
+  return render_frame_impl(w, cam, rr, f, spp, stats, wavefront_for(dev), dev->render_paths);
+}
+
+// ------------------------------------------------------ bnx_render_noalloc
+namespace {
+
+// Paths per batch of bnx_render_noalloc: the queues hold twice as many rays
+// (one shadow and one continuation ray per path), two queues of 92 B per
+// ray: 24 MB of the static array below.
+constexpr long long NOALLOC_PATHS = 1LL << 16;
+constexpr size_t NOALLOC_RAYS = 2 * (size_t)NOALLOC_PATHS;
+constexpr size_t NOALLOC_BLOCKS = NOALLOC_RAYS / 256 + 2;   // one segment: one partial block at most
+constexpr size_t al256(size_t b) { return (b + 255) / 256 * 256; }
+constexpr size_t NOALLOC_QUEUE_BYTES =
+    al256(16 * NOALLOC_RAYS) * 5 + al256(8 * NOALLOC_RAYS) + al256(4 * NOALLOC_RAYS);   // one queue
+constexpr size_t NOALLOC_BYTES = 2 * NOALLOC_QUEUE_BYTES + al256(8 * STAT_COUNT * STAT_SLOTS) +
+                                 al256(sizeof(SlotDev)) + al256(sizeof(RaygenChunk)) + al256(sizeof(Segment)) +
+                                 4 * 256 + 3 * al256(4 * NOALLOC_BLOCKS);
+
+// The scratch memory of bnx_render_noalloc: one copy per GPU the library
+// runs on, held until the process ends.  Host code never passes this name to
+// a kernel or copy: it uses the address hipGetSymbolAddress gives for the
+// current device (Device::noalloc_scratch).
+static __device__ __align__(256) unsigned char scratch[NOALLOC_BYTES];
+
+// lay the noalloc wavefront's buffers out in the scratch array at dev_base
+// and in its host staging
+struct NoallocWavefront {
+  Wavefront F;
+  unsigned char host_stage[4096];
+};
+static_assert(sizeof(SlotDev) + sizeof(RaygenChunk) + sizeof(Segment) + 64 <= 4096, "host staging");
+
+void noalloc_layout(NoallocWavefront &N, unsigned char *base)
+{
+  Wavefront &F = N.F;
+  size_t o = 0;
+  auto take = [&](DevMem &m, size_t b) {
+    m.borrow(base + o, b);
+    o += al256(b);
+  };
+  for (int k = 0; k < 2; k++) {
+    take(F.Q.org[k], 16 * NOALLOC_RAYS);
+    take(F.Q.dir[k], 16 * NOALLOC_RAYS);
+    take(F.Q.rng[k], 8 * NOALLOC_RAYS);
+    take(F.Q.ps[k], 16 * NOALLOC_RAYS);
+    take(F.Q.hit[k], 16 * NOALLOC_RAYS);
+    take(F.Q.hit_v[k], 4 * NOALLOC_RAYS);
+    take(F.Q.vcol[k], 16 * NOALLOC_RAYS);
+  }
+  take(F.Q.stats, 8 * STAT_COUNT * STAT_SLOTS);
+  take(F.slots, sizeof(SlotDev));
+  take(F.chunks, sizeof(RaygenChunk));
+  take(F.segs, sizeof(Segment));
+  take(F.pool, 4);
+  take(F.out_base, 4);
+  take(F.out_cnt, 8);
+  take(F.blk256, 4 * NOALLOC_BLOCKS);
+  take(F.blk1024, 4 * NOALLOC_BLOCKS);
+  take(F.blkT, 4 * NOALLOC_BLOCKS);
+  F.Q.cap = NOALLOC_RAYS;
+  F.slot_cap = F.chunk_cap = F.seg_cap = 1;
+  F.blk_cap = NOALLOC_BLOCKS;
+  size_t h = 0;
+  auto host = [&](Pinned &p, size_t b) {
+    p.borrow(N.host_stage + h, b);
+    h += (b + 15) / 16 * 16;
+  };
+  host(F.h_slots, sizeof(SlotDev));
+  host(F.h_chunks, sizeof(RaygenChunk));
+  host(F.h_segs, sizeof(Segment));
+  host(F.h_out_base, 4);
+  host(F.h_out_cnt, 8);
+  F.noalloc = true;
+  F.graphs_off = true;
+}
+
+} // namespace
+
+int noalloc_init(Device *dev)
+{
+  void *p = nullptr;
+  BNX_HIP(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+  dev->noalloc_scratch = p;
+  NoallocWavefront *N = new NoallocWavefront;
+  noalloc_layout(*N, (unsigned char *)p);
+  dev->noalloc_queues = N;
+  return BNX_OK;
+}
+
+void noalloc_destroy(Device *dev)
+{
+  delete (NoallocWavefront *)dev->noalloc_queues;
+  dev->noalloc_queues = nullptr;
+  dev->noalloc_scratch = nullptr;
 }
 
 // ------------------------------------------------------------------ server
@@ -997,6 +1140,27 @@ int bnx_server_get_stats(bnx_server s, bnx_server_stats *st)
   st->sphere_tests = (int64_t)host_st[STAT_SPHERES];
   st->mc_cells = (int64_t)host_st[STAT_CELLS];
   return BNX_OK;
+  GUARD_END
+}
+
+int bnx_render_noalloc(bnx_world world, const bnx_camera *cam, const bnx_renderer *r, bnx_frame frame, int spp,
+                       bnx_render_stats *stats)
+{
+  GUARD_BEGIN
+  World *w = (World *)world;
+  Frame *f = (Frame *)frame;
+  BNX_CHECK(w && cam && r && f && spp >= 0, BNX_ERR_INVALID, "bnx_render_noalloc: bad argument");
+  BNX_CHECK(f->dev == w->dev, BNX_ERR_INVALID, "bnx_render_noalloc: frame and world live on different devices");
+  BNX_CHECK(w->instances && w->instances->built, BNX_ERR_STATE,
+            "bnx_render_noalloc: the world's instance group is not built");
+  BNX_CHECK(cam->type >= BNX_CAMERA_PERSPECTIVE && cam->type <= BNX_CAMERA_OMNI, BNX_ERR_INVALID,
+            "bnx_render_noalloc: unknown camera type");
+  BNX_CHECK((long long)f->w * f->h <= NOALLOC_PATHS, BNX_ERR_UNSUPPORTED,
+            "bnx_render_noalloc: frame larger than 65536 pixels (the static scratch's limit)");
+  Device *dev = w->dev;
+  BNX_CHECK(dev->noalloc_queues, BNX_ERR_STATE, "bnx_render_noalloc: device without scratch");
+  return render_frame_impl(w, *cam, *r, f, spp, stats, ((NoallocWavefront *)dev->noalloc_queues)->F,
+                           NOALLOC_PATHS);
   GUARD_END
 }
 

@@ -45,25 +45,36 @@ void set_error(const std::string &msg);
     return BNX_ERR_INVALID;                                                           \
   }
 
-// device memory owned by a C++ object
+// device memory owned by a C++ object, or (borrowed) a fixed slice of
+// memory owned elsewhere that is never freed or grown
 struct DevMem {
   void *ptr = nullptr;
   size_t bytes = 0;
+  bool borrowed = false;
   ~DevMem() { release(); }
   void release()
   {
-    if (ptr) (void)hipFree(ptr);
+    if (ptr && !borrowed) (void)hipFree(ptr);
     ptr = nullptr;
     bytes = 0;
   }
-  // (re)allocate if smaller than needed; contents are not preserved
+  // (re)allocate if smaller than needed; contents are not preserved.  A
+  // borrowed slice only reports whether it is large enough.
   hipError_t ensure(size_t b)
   {
+    if (borrowed) return b <= bytes ? hipSuccess : hipErrorOutOfMemory;
     if (b <= bytes && ptr) return hipSuccess;
     release();
     hipError_t e = hipMalloc(&ptr, b ? b : 16);
     if (e == hipSuccess) bytes = b;
     return e;
+  }
+  void borrow(void *p, size_t b)
+  {
+    release();
+    ptr = p;
+    bytes = b;
+    borrowed = true;
   }
   template <typename T> T *as() const { return (T *)ptr; }
 };
@@ -96,7 +107,17 @@ struct Device {
   BuildScratch build;
   DevMem counters;      // small device counters (trace work queue etc.)
   void *queues = nullptr;   // render queues (bnx_render.h Queues)
+  // bnx_render_noalloc: this device's copy of the library's static scratch
+  // array (its address from hipGetSymbolAddress) and the queues laid out in it
+  void *noalloc_scratch = nullptr;
+  void *noalloc_queues = nullptr;
 };
+
+// at device creation, with the device current: the address of this
+// device's copy of the static scratch array of bnx_render_noalloc
+int noalloc_init(Device *dev);
+// the device's render queues (both forms), freed with the device
+void noalloc_destroy(Device *dev);
 
 struct Buffer {
   Device *dev;

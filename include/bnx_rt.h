@@ -536,6 +536,22 @@ int bnx_frame_destroy(bnx_frame fb);
 int bnx_render(bnx_world world, const bnx_camera *camera, const bnx_renderer *renderer,
                bnx_frame fb, int spp, bnx_render_stats *stats);
 
+/* bnx_render without allocation: the same frame loop and kernels, with the
+ * ray queues, counters and staging in a static device array of the library
+ * (24 MB, one copy per GPU, held until the process ends) and no memory
+ * allocated or freed (no hipMalloc, hipFree, pinned or library allocation;
+ * counters are cleared by a kernel, not a memset).  No counterpart in the
+ * reference: for callers that cannot allocate (HIP graph capture set-up
+ * around it, serving loops, harnesses that forbid it).  Limits: frames of at
+ * most 65536 pixels (BNX_ERR_UNSUPPORTED beyond), samples in batches of up
+ * to 65536 paths; no ray reordering, no captured bounce loop and no kernel
+ * timing (the stats' *_ms fields are 0).  The world's tables must exist,
+ * as they do once its volumes and isosurfaces are added.  Every call on a
+ * GPU shares that GPU's array: calls that overlap on one GPU (from two
+ * threads) race.  The results equal bnx_render's. */
+int bnx_render_noalloc(bnx_world world, const bnx_camera *camera, const bnx_renderer *renderer, bnx_frame fb,
+                       int spp, bnx_render_stats *stats);
+
 /* ------------------------------------------------------------- request server
  * Continuous batching of render requests on one device.  barney renders one
  * frame per bnRender call (Context::renderTiles, native/Context.cpp:139-182,
