@@ -62,6 +62,10 @@ _PROTOS = [
                                     C.POINTER(vp)], C.c_int),
     ("bnx_group_build", [vp], C.c_int),
     ("bnx_group_set_transforms", [vp, C.POINTER(C.c_float), C.c_int], C.c_int),
+    ("bnx_group_set_instance_attribute", [vp, C.c_int, C.POINTER(C.c_float), C.c_int], C.c_int),
+    ("bnx_geom_set_attribute", [vp, C.c_int, C.c_int, C.c_int, C.POINTER(C.c_float), C.c_int64,
+                                C.POINTER(C.c_float)], C.c_int),
+    ("bnx_geom_set_vertex_normals", [vp, C.POINTER(C.c_float), C.c_int], C.c_int),
     ("bnx_group_bvh_info", [vp, C.POINTER(abi.BvhInfo)], C.c_int),
     ("bnx_group_download_bvh", [vp, vp, C.POINTER(C.c_int32)], C.c_int),
     ("bnx_group_destroy", [vp], C.c_int),
@@ -250,6 +254,31 @@ class Geom:
         h = vp()
         check(dev.L.bnx_geom_create_cylinders(dev.h, C.byref(d), C.byref(h)))
         return Geom(dev, h, None)
+
+    def set_attribute(self, slot: int, scope: int, data=None, value=None) -> "Geom":
+        """bnx_geom_set_attribute: data (n, comps) floats, or an rgba constant"""
+        d = None if data is None else np.ascontiguousarray(data, dtype=np.float32)
+        comps = 0 if d is None else (1 if d.ndim == 1 else d.shape[1])
+        v = None if value is None else np.ascontiguousarray(value, dtype=np.float32)
+        check(self.dev.L.bnx_geom_set_attribute(self.h, slot, scope, comps, abi.fptr(d), 0 if d is None else len(d),
+                                                abi.fptr(v)))
+        return self
+
+    def set_vertex_normals(self, normals) -> "Geom":
+        n = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32)
+        check(self.dev.L.bnx_geom_set_vertex_normals(self.h, abi.fptr(n), 0 if n is None else len(n)))
+        return self
+
+    def set_attributes(self, obj) -> "Geom":
+        """the attributes (and triangles' vertex normals) of a scene container"""
+        for slot, (scope, arr) in getattr(obj, "attributes", {}).items():
+            if scope == abi.SCOPE_CONSTANT:
+                self.set_attribute(slot, scope, value=arr)
+            else:
+                self.set_attribute(slot, scope, data=arr)
+        if getattr(obj, "normals", None) is not None:
+            self.set_vertex_normals(obj.normals)
+        return self
 
 
 TRIANGLES, USER = 0, 1
@@ -502,7 +531,9 @@ def render(world: World, cam: abi.Camera, rend: abi.Renderer, frame: Frame, spp:
                 volume_samples=st.volume_samples, ms=st.ms_total, nodes=st.node_visits, tris=st.tri_tests,
                 spheres=st.sphere_tests, trace_ms=st.trace_ms, trace_launches=st.trace_launches,
                 march_ms=st.march_ms, march_launches=st.march_launches, mc_cells=st.mc_cells,
-                march_rays=st.march_rays, umesh_nodes=st.umesh_nodes, umesh_tets=st.umesh_tets)
+                march_rays=st.march_rays, umesh_nodes=st.umesh_nodes, umesh_tets=st.umesh_tets,
+                captured_batches=st.captured_batches, uncaptured_batches=st.uncaptured_batches,
+                capture_error=st.capture_error)
 
 
 class Server:
@@ -579,12 +610,12 @@ class SurfaceWorld:
 
     def __init__(self, dev: Device, sc):
         self.dev = dev
-        self.sph = [Geom.spheres(dev, s.centers, s.radii, s.colors, user_id=s.user_id, material=s.material)
-                    for s in sc.spheres]
+        self.sph = [Geom.spheres(dev, s.centers, s.radii, s.colors, user_id=s.user_id,
+                                 material=s.material).set_attributes(s) for s in sc.spheres]
         self.cyl = [Geom.cylinders(dev, c.vertices, c.indices, c.radii, c.colors, user_id=c.user_id,
-                                   material=c.material) for c in getattr(sc, "cylinders", [])]
-        self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
-                    for m in sc.meshes]
+                                   material=c.material).set_attributes(c) for c in getattr(sc, "cylinders", [])]
+        self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id,
+                                   material=m.material).set_attributes(m) for m in sc.meshes]
         groups = []
         if self.sph:
             groups.append(Group.geoms(dev, USER, self.sph).build())
@@ -593,7 +624,11 @@ class SurfaceWorld:
         if self.tri:
             groups.append(Group.geoms(dev, TRIANGLES, self.tri).build())
         self.groups = groups
-        self.insts = Group.instances(dev, groups, [0] * len(groups)).build()
+        self.insts = Group.instances(dev, groups, [0] * len(groups))
+        for slot, vals in getattr(sc, "inst_attr", {}).items():
+            v = np.ascontiguousarray(vals, dtype=np.float32)
+            check(dev.L.bnx_group_set_instance_attribute(self.insts.h, slot, abi.fptr(v), len(v)))
+        self.insts.build()
         self.world = World(dev, self.insts, sc.dir_lights)
         self.world.set_env(getattr(sc, "env", None))
 

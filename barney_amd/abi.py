@@ -127,7 +127,8 @@ class RenderStats(C.Structure):
                 ("tri_tests", C.c_int64), ("sphere_tests", C.c_int64), ("trace_ms", C.c_double),
                 ("trace_launches", C.c_int64), ("march_ms", C.c_double), ("march_launches", C.c_int64),
                 ("mc_cells", C.c_int64), ("march_rays", C.c_int64), ("umesh_nodes", C.c_int64),
-                ("umesh_tets", C.c_int64)]
+                ("umesh_tets", C.c_int64), ("captured_batches", C.c_int64), ("uncaptured_batches", C.c_int64),
+                ("capture_error", C.c_int32), ("reserved", C.c_int32)]
 
 
 class BvhInfo(C.Structure):
@@ -202,10 +203,28 @@ def iptr(a: np.ndarray):
     return a.ctypes.data_as(ip) if a is not None else None
 
 
+# where a material's color comes from (BNX_ATTRIB_*), geometry attribute
+# slots (BNX_ATTRIB_SLOT_*) and scopes (BNX_SCOPE_*)
+ATTRIB_NONE, ATTRIB_COLOR, ATTRIB_0 = 0, 1, 2
+ATTRIB_WORLD_POSITION, ATTRIB_OBJECT_POSITION, ATTRIB_WORLD_NORMAL, ATTRIB_OBJECT_NORMAL, ATTRIB_PRIMITIVE_ID = \
+    6, 7, 8, 9, 10
+ATTRIB_SLOT_0, ATTRIB_SLOT_COLOR, ATTRIB_SLOT_NORMAL = 0, 4, 5
+SCOPE_NONE, SCOPE_CONSTANT, SCOPE_PER_PRIM, SCOPE_PER_VERTEX, SCOPE_FACE_VARYING = 0, 1, 2, 3, 4
+
+
+def _color_source(from_attribute) -> int:
+    """True: the 'color' attribute; False: the value; an int: a BNX_ATTRIB_* code"""
+    if from_attribute is True:
+        return ATTRIB_COLOR
+    if from_attribute is False or from_attribute is None:
+        return ATTRIB_NONE
+    return int(from_attribute)
+
+
 def matte(color=(0.8, 0.8, 0.8, 1.0), from_attribute=True, opacity=1.0) -> Material:
     m = Material()
     m.type = BNX_MATERIAL_MATTE
-    m.color_from_attribute = 1 if from_attribute else 0
+    m.color_from_attribute = _color_source(from_attribute)
     m.color = f4(*color)
     m.opacity = opacity
     m.metallic, m.roughness, m.transmission, m.ior = 1.0, 1.0, 0.0, 1.45
@@ -217,7 +236,7 @@ def pbr(base_color=(1.0, 1.0, 1.0, 1.0), from_attribute=True, metallic=1.0, roug
     """ANARI physicallyBased material (AnariPBR); defaults are ANARI's"""
     m = Material()
     m.type = BNX_MATERIAL_PBR
-    m.color_from_attribute = 1 if from_attribute else 0
+    m.color_from_attribute = _color_source(from_attribute)
     m.color = f4(*base_color)
     m.opacity = opacity
     m.metallic, m.roughness, m.transmission, m.ior = metallic, roughness, transmission, ior

@@ -195,6 +195,25 @@ bool is_identity(const float *a)
   return true;
 }
 
+// whether a triangle geometry's alpha (Lambertian color.w * opacity;
+// NVisii (1 - transmission) * baseColor.w * opacity) can be below 1 under
+// instance group g, so that its candidates take the opacity test
+bool can_be_translucent(const Geom &gm, const Group *g)
+{
+  const bnx_material &m = gm.mat;
+  const int k = m.color_from_attribute;
+  bool w_below_1 = false;
+  if (k == BNX_ATTRIB_NONE) {
+    w_below_1 = !(m.color[0] != m.color[0]) && !(m.color[3] >= 1.f);
+  } else if (k == BNX_ATTRIB_COLOR || (k >= BNX_ATTRIB_0 && k < BNX_ATTRIB_0 + 4)) {
+    const int slot = k == BNX_ATTRIB_COLOR ? 4 : k - BNX_ATTRIB_0;
+    w_below_1 = gm.attr[slot].scope != BNX_SCOPE_NONE ? gm.attr[slot].w_below_1 : g->inst_attr_w_below_1[slot];
+  }
+  bool below = w_below_1 || !(m.opacity >= 1.f);
+  if (m.type == BNX_MATERIAL_PBR) below = below || m.transmission != 0.f;
+  return below;
+}
+
 int build_tlas(Group *g)
 {
   Device *dev = g->dev;
@@ -234,11 +253,23 @@ int build_tlas(Group *g)
         gd.metallic = gm->mat.metallic;
         gd.roughness = gm->mat.roughness;
         gd.ior = gm->mat.ior;
+        gd.opacity = gm->mat.opacity;
+        gd.transmission = gm->mat.transmission;
+        gd.translucent = gm->type == 0 && can_be_translucent(*gm, g) ? 1 : 0;
         BNX_SET(gd.vertices, gm->vertices.as<float>());
         BNX_SET(gd.indices, gm->indices.as<int>());
         BNX_SET(gd.colors, gm->colors.ptr ? gm->colors.as<float>() : nullptr);
+        BNX_SET(gd.normals, gm->normals.ptr ? gm->normals.as<float>() : nullptr);
         BNX_SET(gd.spheres, gm->spheres.as<float4>());
         BNX_SET(gd.cyls, gm->cyls.as<float4>());
+        for (int k = 0; k < ATTR_SLOTS; k++) {
+          const HostAttr &a = gm->attr[k];
+          AttrDev &o = gd.attr[k];
+          o.scope = a.scope;
+          o.comps = a.comps;
+          o.value = make_float4(a.value[0], a.value[1], a.value[2], a.value[3]);
+          BNX_SET(o.data, a.data.ptr ? a.data.as<float>() : nullptr);
+        }
         geoms.push_back(gd);
       }
       g->max_blas_depth = std::max(g->max_blas_depth, bg->bvh.stack_need4);
@@ -283,12 +314,26 @@ int build_tlas(Group *g)
                        ? BLAS_TRIANGLES
                        : ((!bg->geoms.empty() && bg->geoms[0]->type == 2) ? BLAS_CYLINDERS : BLAS_SPHERES);
     blas[k].num_prims = bg->bvh.num_prims;
+    blas[k].translucent = 0;
+    for (size_t j = 0; j < bg->geoms.size(); j++) blas[k].translucent |= geoms[geom_base[k] + j].translucent;
   }
   bool has_cyl = false;
   for (const BlasDev &b : blas) has_cyl |= b.type == BLAS_CYLINDERS;
   if (has_cyl != g->has_cyl) {
     dev->cyl_worlds += has_cyl ? 1 : -1;
     g->has_cyl = has_cyl;
+  }
+  bool has_translucent = false;
+  for (const BlasDev &b : blas) has_translucent |= b.translucent != 0;
+  if (has_translucent != g->has_translucent) {
+    dev->translucent_worlds += has_translucent ? 1 : -1;
+    g->has_translucent = has_translucent;
+  }
+  bool has_xf = false;
+  for (const InstDev &I : insts) has_xf |= !I.identity;
+  if (has_xf != g->has_xf) {
+    dev->xf_worlds += has_xf ? 1 : -1;
+    g->has_xf = has_xf;
   }
   DevMem d_lo, d_hi;
   int rc;
@@ -310,6 +355,8 @@ int build_tlas(Group *g)
   BNX_SET(g->scene.insts, g->d_insts.as<InstDev>());
   BNX_SET(g->scene.blas, g->d_blas.as<BlasDev>());
   BNX_SET(g->scene.geoms, g->d_geoms.as<GeomDev>());
+  for (int k = 0; k < 5; k++)
+    BNX_SET(g->scene.inst_attr[k], g->inst_attr[k].ptr ? g->inst_attr[k].as<float4>() : nullptr);
   g->scene.num_insts = n;
   g->built = true;
   return BNX_OK;
@@ -584,24 +631,29 @@ static void set_mat(Geom *g, const bnx_material &m) { g->mat = m; }
 
 // AnariMatte, or AnariPBR creating the NVisii BSDF (AnariPBR.h:62-96).
 // Not provided: the Glass BSDF (PBR with ior != 1 and transmission >=
-// 1e-3), and on triangles the stochastic opacity of a material whose
-// opacity (matte) or alpha = (1 - transmission) * baseColor.a * opacity
-// (PBR) is below 1 (Triangles.dev.cu:120-133); spheres ignore opacity.
-static int check_material(const bnx_material &m, bool triangles)
+// 1e-3).  Triangles whose alpha is below 1 take the stochastic opacity test
+// (Triangles.dev.cu:120-133); spheres and cylinders have no any-hit program
+// and ignore opacity.
+static int check_material(const bnx_material &m, bool)
 {
   BNX_CHECK(m.type == BNX_MATERIAL_MATTE || m.type == BNX_MATERIAL_PBR, BNX_ERR_UNSUPPORTED,
             "only matte and physically-based materials are provided");
   if (m.type == BNX_MATERIAL_PBR)
     BNX_CHECK(m.ior == 1.f || m.transmission < 1e-3f, BNX_ERR_UNSUPPORTED,
               "the Glass BSDF (PBR transmission with ior != 1) is not provided");
-  if (triangles) {
-    const float a = m.type == BNX_MATERIAL_PBR
-                        ? ((1.f - m.transmission) * (m.color_from_attribute ? 1.f : m.color[3])) * m.opacity
-                        : m.opacity;
-    BNX_CHECK(a >= 1.f, BNX_ERR_UNSUPPORTED,
-              "stochastic opacity (Triangles.dev.cu:120-133) is not provided; opacity must be 1");
-  }
+  BNX_CHECK(m.color_from_attribute >= BNX_ATTRIB_NONE && m.color_from_attribute <= BNX_ATTRIB_PRIMITIVE_ID,
+            BNX_ERR_INVALID, "material: unknown color attribute");
   return BNX_OK;
+}
+
+// the "color" attribute per primitive from a float3 array of a geometry desc
+static int set_prim_colors(Geom *g, const float *colors, int n)
+{
+  HostAttr &a = g->attr[BNX_ATTRIB_SLOT_COLOR];
+  a.scope = BNX_SCOPE_PER_PRIM;
+  a.comps = 3;
+  a.w_below_1 = false;
+  return upload(g->dev, a.data, colors, 3 * (size_t)n);
 }
 
 int bnx_geom_create_triangles(bnx_device d, const bnx_triangles_desc *desc, bnx_geom *out)
@@ -628,7 +680,7 @@ int bnx_geom_create_triangles(bnx_device d, const bnx_triangles_desc *desc, bnx_
     delete g;
     return rc;
   }
-  if (desc->prim_colors && (rc = upload(d, g->colors, desc->prim_colors, 3 * (size_t)desc->num_triangles)) != BNX_OK) {
+  if (desc->prim_colors && (rc = set_prim_colors(g, desc->prim_colors, desc->num_triangles)) != BNX_OK) {
     delete g;
     return rc;
   }
@@ -664,11 +716,12 @@ int bnx_geom_create_cylinders(bnx_device d, const bnx_cylinders_desc *desc, bnx_
   g->num_prims = desc->num_cylinders;
   g->num_vertices = desc->num_vertices;
   int rc;
-  if ((rc = upload(d, g->cyls, rec.data(), rec.size())) != BNX_OK) {
+  if ((rc = upload(d, g->cyls, rec.data(), rec.size())) != BNX_OK ||
+      (rc = upload(d, g->indices, desc->indices, 2 * (size_t)desc->num_cylinders)) != BNX_OK) {
     delete g;
     return rc;
   }
-  if (desc->colors && (rc = upload(d, g->colors, desc->colors, 3 * (size_t)desc->num_cylinders)) != BNX_OK) {
+  if (desc->colors && (rc = set_prim_colors(g, desc->colors, desc->num_cylinders)) != BNX_OK) {
     delete g;
     return rc;
   }
@@ -707,6 +760,68 @@ int bnx_geom_create_spheres(bnx_device d, const bnx_spheres_desc *desc, bnx_geom
   }
   BNX_HIP(hipStreamSynchronize(d->stream));
   *out = g;
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_geom_set_attribute(bnx_geom g, int slot, int scope, int comps, const float *data, int64_t count,
+                           const float value[4])
+{
+  GUARD_BEGIN
+  BNX_CHECK(g, BNX_ERR_INVALID, "bnx_geom_set_attribute: null geom");
+  BNX_CHECK(slot >= 0 && slot < ATTR_SLOTS, BNX_ERR_INVALID, "bnx_geom_set_attribute: slot out of range");
+  BNX_CHECK(scope >= BNX_SCOPE_NONE && scope <= BNX_SCOPE_FACE_VARYING, BNX_ERR_INVALID,
+            "bnx_geom_set_attribute: unknown scope");
+  HostAttr &a = g->attr[slot];
+  if (scope == BNX_SCOPE_NONE) {
+    a.scope = BNX_SCOPE_NONE;
+    a.comps = 0;
+    a.data.release();
+    a.w_below_1 = false;
+    return BNX_OK;
+  }
+  if (scope == BNX_SCOPE_CONSTANT) {
+    BNX_CHECK(value, BNX_ERR_INVALID, "bnx_geom_set_attribute: constant without a value");
+    a.scope = scope;
+    a.comps = 4;
+    for (int k = 0; k < 4; k++) a.value[k] = value[k];
+    a.data.release();
+    a.w_below_1 = !(value[3] >= 1.f);
+    return BNX_OK;
+  }
+  BNX_CHECK(comps >= 1 && comps <= 4, BNX_ERR_INVALID, "bnx_geom_set_attribute: 1 to 4 floats per element");
+  // elements a hit can read: per primitive, per vertex, or per corner
+  // (3 per triangle, 2 per cylinder; a sphere reads its own element)
+  int64_t need = g->num_prims;
+  if (scope == BNX_SCOPE_PER_VERTEX) need = g->type == 1 ? g->num_prims : g->num_vertices;
+  if (scope == BNX_SCOPE_FACE_VARYING) need = g->type == 0 ? 3LL * g->num_prims
+                                             : (g->type == 2 ? 2LL * g->num_prims : g->num_prims);
+  BNX_CHECK(data && count >= need, BNX_ERR_INVALID,
+            "bnx_geom_set_attribute: fewer elements than the scope needs (" + std::to_string(need) + ")");
+  int rc;
+  if ((rc = upload(g->dev, a.data, data, (size_t)comps * (size_t)count)) != BNX_OK) return rc;
+  BNX_HIP(hipStreamSynchronize(g->dev->stream));
+  a.scope = scope;
+  a.comps = comps;
+  a.w_below_1 = false;
+  if (comps == 4)
+    for (int64_t i = 0; i < count && !a.w_below_1; i++) a.w_below_1 = !(data[4 * i + 3] >= 1.f);
+  return BNX_OK;
+  GUARD_END
+}
+
+int bnx_geom_set_vertex_normals(bnx_geom g, const float *normals, int n)
+{
+  GUARD_BEGIN
+  BNX_CHECK(g && g->type == 0, BNX_ERR_INVALID, "bnx_geom_set_vertex_normals: not a triangles geometry");
+  if (!normals) {
+    g->normals.release();
+    return BNX_OK;
+  }
+  BNX_CHECK(n == g->num_vertices, BNX_ERR_INVALID, "bnx_geom_set_vertex_normals: one normal per vertex");
+  int rc;
+  if ((rc = upload(g->dev, g->normals, normals, 3 * (size_t)n)) != BNX_OK) return rc;
+  BNX_HIP(hipStreamSynchronize(g->dev->stream));
   return BNX_OK;
   GUARD_END
 }
@@ -786,6 +901,29 @@ int bnx_group_build(bnx_group g)
   GUARD_END
 }
 
+int bnx_group_set_instance_attribute(bnx_group g, int slot, const float *values, int n)
+{
+  GUARD_BEGIN
+  BNX_CHECK(g && g->kind == GROUP_INSTANCES, BNX_ERR_INVALID,
+            "bnx_group_set_instance_attribute: not an instance group");
+  BNX_CHECK(slot >= 0 && slot < 5, BNX_ERR_INVALID, "bnx_group_set_instance_attribute: slot out of range");
+  if (!values) {
+    g->inst_attr[slot].release();
+    g->inst_attr_w_below_1[slot] = false;
+    return BNX_OK;
+  }
+  BNX_CHECK(n == (int)g->groups.size(), BNX_ERR_INVALID,
+            "bnx_group_set_instance_attribute: one value per instance");
+  int rc;
+  if ((rc = upload(g->dev, g->inst_attr[slot], (const float4 *)values, (size_t)n)) != BNX_OK) return rc;
+  BNX_HIP(hipStreamSynchronize(g->dev->stream));
+  bool below = false;
+  for (int i = 0; i < n && !below; i++) below = !(values[4 * i + 3] >= 1.f);
+  g->inst_attr_w_below_1[slot] = below;
+  return BNX_OK;
+  GUARD_END
+}
+
 int bnx_group_set_transforms(bnx_group g, const float *xfms, int n)
 {
   GUARD_BEGIN
@@ -834,6 +972,8 @@ int bnx_group_destroy(bnx_group g)
   BNX_CHECK(g, BNX_ERR_INVALID, "bnx_group_destroy: null group");
   (void)hipStreamSynchronize(g->dev->stream);
   if (g->has_cyl) g->dev->cyl_worlds--;
+  if (g->has_xf) g->dev->xf_worlds--;
+  if (g->has_translucent) g->dev->translucent_worlds--;
   delete g;
   return BNX_OK;
 }

@@ -9,6 +9,8 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 #include "../../include/bnx_rt.h"
 #include "bnx_numeric.h"
 
@@ -27,7 +29,7 @@
 #else
 #define BNX_G
 #endif
-#define BNX_SET(field, value) ((field) = (decltype(field))(value))
+#define BNX_SET(field, value) ((field) = (std::remove_reference_t<decltype(field)>)(value))
 
 namespace bnx {
 
@@ -157,6 +159,8 @@ struct BlasDev {
   const int2 BNX_G *refs;  // spheres: (geom, prim) per slot
   int type;
   int num_prims;
+  int translucent;         // a geometry of it has stochastic opacity
+  int _pad;
 };
 
 struct InstDev {
@@ -168,20 +172,36 @@ struct InstDev {
   int geom_base;          // first entry of this instance's group in the geometry table
 };
 
+// one geometry attribute (GeometryAttribute::DD, render/GeometryAttributes.h:
+// 22-38): a constant, or elements of comps floats per prim / vertex / face
+// corner (scope BNX_SCOPE_*; BNX_SCOPE_NONE: not set)
+struct AttrDev {
+  float4 value;
+  const float BNX_G *data;
+  int scope;
+  int comps;
+};
+enum { ATTR_SLOTS = 6 };   // attribute0..3, color, normal (BNX_ATTRIB_SLOT_*)
+
 // per-geometry shading data
 struct GeomDev {
   int type;               // 0 triangles, 1 spheres, 2 cylinders
   int user_id;
-  int color_from_attr;
+  int color_from_attr;    // BNX_ATTRIB_*: the hit attribute the color comes from
   int num_prims;
   float4 color;           // constant material color (matte 'color', PBR 'baseColor')
   int material;           // BNX_MATERIAL_MATTE or BNX_MATERIAL_PBR
   float metallic, roughness, ior;   // PBR parameters as given (rounded to fp16 per hit)
-  const float BNX_G *vertices;   // triangles: float3
-  const int BNX_G *indices;      //            int3
-  const float BNX_G *colors;     // per-prim float3 or null
+  float opacity, transmission;      // 'opacity'; PBR 'transmission'
+  int translucent;        // triangles whose alpha can be below 1: stochastic opacity
+  int _pad;
+  const float BNX_G *vertices;   // triangles: float3; cylinders: float3
+  const int BNX_G *indices;      // triangles: int3; cylinders: int2
+  const float BNX_G *colors;     // spheres' own per-sphere float3 colors, or null
+  const float BNX_G *normals;    // triangles: per-vertex float3 normals, or null
   const float4 BNX_G *spheres;   // spheres: (center, radius)
   const float4 BNX_G *cyls;      // cylinders: (v0, radius), (v1, 0) per cylinder
+  AttrDev attr[ATTR_SLOTS];
 };
 
 struct SceneDev {
@@ -190,8 +210,111 @@ struct SceneDev {
   const InstDev BNX_G *insts;
   const BlasDev BNX_G *blas;
   const GeomDev BNX_G *geoms;
+  // World::DD::instanceAttributes: a float4 per rtc instance for
+  // attribute0..3 and color, or null
+  const float4 BNX_G *inst_attr[5];
   int num_insts;
 };
+
+// AttributeArray::DD::valueAt (render/GeometryAttributes.h:68-91)
+BNX_D float4 attr_value_at(const AttrDev &a, int i)
+{
+  const float *p = a.data + (long long)a.comps * i;
+  switch (a.comps) {
+  case 1: return make_float4(p[0], 0.f, 0.f, 1.f);
+  case 2: return make_float4(p[0], p[1], 0.f, 1.f);
+  case 3: return make_float4(p[0], p[1], p[2], 1.f);
+  case 4: return make_float4(p[0], p[1], p[2], p[3]);
+  default: return make_float4(0.f, 0.f, 0.f, 0.f);
+  }
+}
+
+// The elements a hit interpolates per-vertex (vi) or face-varying (fi)
+// attributes from, with weights w: a triangle's three corners at
+// barycentrics (1-u-v, u, v) (Triangles.dev.cu:96-111), a cylinder's two
+// ends at (1-lerp_t, lerp_t) (Cylinders.dev.cu:208-222); a sphere reads its
+// own element for both (n = 1, Spheres.dev.cu:94-100).
+struct Corners {
+  int n;
+  int vi[3], fi[3];
+  float w[3];
+};
+
+BNX_D float4 scale4(float s, float4 a) { return make_float4(s * a.x, s * a.y, s * a.z, s * a.w); }
+BNX_D float4 add4(float4 a, float4 b) { return make_float4(a.x + b.x, a.y + b.y, a.z + b.z, a.w + b.w); }
+
+BNX_D float4 attr_interp(const AttrDev &a, const Corners &c, bool face_varying)
+{
+  const int *ix = face_varying ? c.fi : c.vi;
+  if (c.n == 1) return attr_value_at(a, ix[0]);
+  float4 r = add4(scale4(c.w[0], attr_value_at(a, ix[0])), scale4(c.w[1], attr_value_at(a, ix[1])));
+  if (c.n == 3) r = add4(r, scale4(c.w[2], attr_value_at(a, ix[2])));
+  return r;
+}
+
+// the value of geometry attribute slot k at a hit, as Geometry::DD::
+// setHitAttributes sets it (geometry/Geometry.h:94-139): the geometry's own
+// (constant, per primitive, per vertex, face-varying), else its instance's
+// (slots 0-4, indexed by the rtc instance), else dflt
+BNX_D float4 hit_attribute(const SceneDev &S, const GeomDev &G, int k, int inst, int prim, const Corners &c,
+                           float4 dflt)
+{
+  const AttrDev &a = G.attr[k];
+  switch (a.scope) {
+  case BNX_SCOPE_CONSTANT: return a.value;
+  case BNX_SCOPE_PER_PRIM: return attr_value_at(a, prim);
+  case BNX_SCOPE_PER_VERTEX: return attr_interp(a, c, false);
+  case BNX_SCOPE_FACE_VARYING: return attr_interp(a, c, true);
+  default: break;
+  }
+  if (k < 5 && S.inst_attr[k]) return S.inst_attr[k][inst];
+  return dflt;
+}
+
+// The material color: PossiblyMappedParameter::DD::eval (material/
+// HostMaterial.h:100-114) of the matte 'color' / PBR 'baseColor', a value
+// (NaN: unset, (0,0,0,1)) or a hit attribute (HitAttributes::get, render/
+// HitAttributes.h:61-80).  color0: the hit's color before the attribute
+// system (spheres' own colors, else (0,0,0,1)); P / osP: world / object
+// position of the hit.
+BNX_D float4 material_color(const SceneDev &S, const GeomDev &G, int inst, int prim, const Corners &c, float4 color0,
+                            v3 P, v3 osP)
+{
+  const int k = G.color_from_attr;
+  const float4 none = make_float4(0.f, 0.f, 0.f, 1.f);
+  if (k == BNX_ATTRIB_NONE) return G.color.x != G.color.x ? none : G.color;
+  if (k == BNX_ATTRIB_COLOR) return hit_attribute(S, G, 4, inst, prim, c, color0);
+  if (k >= BNX_ATTRIB_0 && k < BNX_ATTRIB_0 + 4) return hit_attribute(S, G, k - BNX_ATTRIB_0, inst, prim, c, none);
+  if (k == BNX_ATTRIB_WORLD_POSITION) return make_float4(P.x, P.y, P.z, 1.f);
+  if (k == BNX_ATTRIB_OBJECT_POSITION) return make_float4(osP.x, osP.y, osP.z, 1.f);
+  return none;
+}
+
+// the BSDF's opacity: Lambertian alpha = color.w * opacity (AnariMatte.h:
+// 64); NVisii alpha = (1 - transmission) * baseColor.w * opacity, stored as
+// half (AnariPBR.h:87-90, NVisii.h:1082)
+BNX_D float material_alpha(const GeomDev &G, float4 base)
+{
+  if (G.material == BNX_MATERIAL_PBR) return h2f(f2h(((1.f - G.transmission) * base.w) * G.opacity));
+  return base.w * G.opacity;
+}
+
+// a triangle hit's corners for attribute interpolation
+BNX_D Corners tri_corners(const int *ix, int prim, float u, float v)
+{
+  Corners c;
+  c.n = 3;
+  c.vi[0] = ix[0];
+  c.vi[1] = ix[1];
+  c.vi[2] = ix[2];
+  c.fi[0] = 3 * prim;
+  c.fi[1] = 3 * prim + 1;
+  c.fi[2] = 3 * prim + 2;
+  c.w[0] = (1.f - u) - v;
+  c.w[1] = u;
+  c.w[2] = v;
+  return c;
+}
 
 BNX_D v3 xfm_point_rows(const float4 *m, v3 p)
 {

@@ -75,11 +75,12 @@ struct BatchShape {
   bool surfaces = false, volumes = false, full = false;
   int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8, shade_block = 1024;
   bool ext = false;   // the trace kernels with the cylinder test and the cut plane
+  bool xf = false;    // the trace kernels that change the ray's frame at instances
   bool operator<(const BatchShape &o) const
   {
-    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block, ext) <
+    return std::tie(count0, cap, surfaces, volumes, full, refill, stack, walk, blocks_per_cu, shade_block, ext, xf) <
            std::tie(o.count0, o.cap, o.surfaces, o.volumes, o.full, o.refill, o.stack, o.walk, o.blocks_per_cu,
-                    o.shade_block, o.ext);
+                    o.shade_block, o.ext, o.xf);
   }
 };
 
@@ -107,6 +108,7 @@ struct Wavefront {
   std::vector<BatchGraph> graphs;
   std::vector<const void *> graph_bufs;
   bool graphs_off = false;      // capture failed once: enqueue directly
+  hipError_t capture_err = hipSuccess;   // why (reported in every render's stats)
   // every buffer is a fixed slice of the static scratch array (device) and
   // of host_stage (host): nothing is allocated, no timing events exist, and
   // the paths per batch are bounded by the slices
@@ -236,7 +238,7 @@ int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool sur
   const SegView sv{F.segs.as<Segment>(), F.blk256.as<int>(), F.blk1024.as<int>(), F.blkT.as<int>(),
                    F.pool.as<int>()};
   if ((rc = launch_segment_blocks(dev, sv, sl.n, sl.blocks256, sl.blocks1024, sl.blocksT)) != BNX_OK) return rc;
-  TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
+  TraceQueue tq{in.org, in.dir, in.hit, in.hit_v, in.rng, in.ps};
   // with neither surfaces nor volumes the trace kernel still writes the misses
   const bool trace = surfaces || !volumes;
   const bool timed = Q.ev_begin != nullptr;
@@ -294,7 +296,7 @@ int enqueue_dd(Device *dev, Wavefront &F, int cur0, int G, const BatchShape &b)
       return rc;
     if ((rc = launch_segment_blocks(dev, sv, 1, blocks256, blocks1024, blocksT)) != BNX_OK) return rc;
     RayQueue in = Q.q(cur), out = Q.q(1 - cur);
-    TraceQueue tq{in.org, in.dir, in.hit, in.hit_v};
+    TraceQueue tq{in.org, in.dir, in.hit, in.hit_v, in.rng, in.ps};
     unsigned long long *ts = F.stamps.as<unsigned long long>() + 4 * g;
     if (trace) {
       if ((rc = launch_stamp(dev, ts)) != BNX_OK) return rc;
@@ -328,19 +330,23 @@ int batch_graph(Device *dev, Wavefront &F, const BatchShape &b, int G, int cur0,
   bg.shape = b;
   bg.G = G;
   bg.cur0 = cur0;
-  if (hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal) != hipSuccess) {
+  hipError_t e = hipStreamBeginCapture(dev->stream, hipStreamCaptureModeThreadLocal);
+  if (e != hipSuccess) {
     (void)hipGetLastError();
     F.graphs_off = true;
+    F.capture_err = e;
     return BNX_OK;
   }
   const int rc = enqueue_dd(dev, F, cur0, G, b);
   hipGraph_t graph = nullptr;
-  const hipError_t e = hipStreamEndCapture(dev->stream, &graph);
-  if (rc != BNX_OK || e != hipSuccess || !graph ||
-      hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0) != hipSuccess) {
+  e = hipStreamEndCapture(dev->stream, &graph);
+  if (rc == BNX_OK && e == hipSuccess && graph) e = hipGraphInstantiate(&bg.exec, graph, nullptr, nullptr, 0);
+  if (rc != BNX_OK || e != hipSuccess || !graph) {
     if (graph) (void)hipGraphDestroy(graph);
     (void)hipGetLastError();
+    if (rc != BNX_OK) return rc;            // an enqueue error, not a capture one
     F.graphs_off = true;
+    F.capture_err = e != hipSuccess ? e : hipErrorUnknown;
     return BNX_OK;
   }
   bg.graph = graph;
@@ -481,7 +487,7 @@ int render_frame_impl(World *w, const bnx_camera &cam, const bnx_renderer &rr, F
   int *hcnt = F.h_out_cnt.as<int>();
   hb[0] = 0;                           // the single request's region starts the queue
   BNX_HIP(hipMemcpyAsync(F.out_base.ptr, hb, sizeof(int), hipMemcpyHostToDevice, dev->stream));
-  int64_t rays = 0, gens = 0;
+  int64_t rays = 0, gens = 0, captured = 0, uncaptured = 0;
   GenTiming tm;
   for (int s0 = 0; s0 < spp; s0 += S) {
     const int nS = std::min(S, spp - s0);
@@ -502,7 +508,7 @@ int render_frame_impl(World *w, const bnx_camera &cam, const bnx_renderer &rr, F
     const BatchShape shape{np * nS,          2 * np * nS,           surfaces,
                            volumes,          full,                  trace_refill_for(dev, w),
                            dev->trace_stack_lds, dev->trace_walk, dev->trace_blocks_per_cu, dev->shade_block,
-                           dev->cyl_worlds > 0 || has_cut_plane(rr)};
+                           ext_kernels(dev, has_cut_plane(rr)), dev->xf_worlds > 0};
     int dd_gens = 0;
     if (dd) {
       auto it = F.gens_seen.find(shape);
@@ -513,8 +519,11 @@ int render_frame_impl(World *w, const bnx_camera &cam, const bnx_renderer &rr, F
         if ((rc = batch_graph(dev, F, shape, G, cur, exec)) != BNX_OK) return rc;
         if (exec) {
           BNX_HIP(hipGraphLaunch(exec, dev->stream));
+          captured++;
         } else if ((rc = enqueue_dd(dev, F, cur, G, shape)) != BNX_OK) {
           return rc;
+        } else {
+          uncaptured++;
         }
         long long *hg = F.h_gen_counts.as<long long>();
         unsigned long long *hts = (unsigned long long *)(hg + MAX_DD_GENS);
@@ -596,6 +605,10 @@ int render_frame_impl(World *w, const bnx_camera &cam, const bnx_renderer &rr, F
     stats->march_rays = tm.march_rays;
     stats->umesh_nodes = (int64_t)host_st[STAT_UNODES];
     stats->umesh_tets = (int64_t)host_st[STAT_UTETS];
+    stats->captured_batches = captured;
+    stats->uncaptured_batches = uncaptured;
+    stats->capture_error = (int32_t)F.capture_err;
+    stats->reserved = 0;
   }
   return BNX_OK;
 }

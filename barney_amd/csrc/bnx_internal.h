@@ -92,6 +92,8 @@ struct Device {
   hipStream_t stream = nullptr;
   int num_cus = 0;
   int cyl_worlds = 0;            // built instance groups with a cylinders BLAS (trace kernels with the cylinder test)
+  int xf_worlds = 0;             // built instance groups with a non-identity instance (trace kernels that change frame)
+  int translucent_worlds = 0;    // built instance groups with stochastic-opacity geometry (the EXT trace kernels)
   int wall_clock_khz = 100000;   // rate of the device wall clock (launch timing in graphs)
   // launch tuning (bnx_device_set_tuning, bnx_autotune); none of it changes results
   int raygen_tile_w = 8, raygen_tile_h = 8;   // primary-ray pixel order: tiles (0: row-major)
@@ -124,6 +126,16 @@ struct Buffer {
   DevMem mem;
 };
 
+// one geometry attribute as set (bnx_geom_set_attribute); w_below_1: some
+// value it can give has w < 1 (a stochastic-opacity alpha source)
+struct HostAttr {
+  int scope = BNX_SCOPE_NONE;
+  int comps = 0;
+  float value[4] = {0.f, 0.f, 0.f, 1.f};
+  DevMem data;
+  bool w_below_1 = false;
+};
+
 struct Geom {
   Device *dev;
   int type;             // 0 triangles, 1 spheres, 2 cylinders
@@ -132,6 +144,8 @@ struct Geom {
   int num_prims;
   int num_vertices;
   DevMem vertices, indices, colors, spheres, cyls;
+  DevMem normals;       // triangles: per-vertex normals
+  HostAttr attr[6];     // attribute0..3, color, normal
 };
 
 struct Bvh {
@@ -161,11 +175,15 @@ struct Group {
   std::vector<Group *> groups;
   std::vector<int> inst_ids;
   std::vector<float> xfms;   // 12 floats per instance, owl affine3f layout
+  DevMem inst_attr[5];       // instance attributes (attribute0..3, color): float4 per instance
+  bool inst_attr_w_below_1[5] = {false, false, false, false, false};
   DevMem d_insts, d_blas, d_geoms;
   SceneDev scene{};
   int max_blas_depth = 0;    // largest traversal-stack need of its BLASes (4-wide walk)
   long long blas_prims = 0;  // primitives in the distinct BLASes it instances
   bool has_cyl = false;      // counted in Device::cyl_worlds
+  bool has_xf = false;       // counted in Device::xf_worlds
+  bool has_translucent = false;   // counted in Device::translucent_worlds
 };
 
 struct Volume;
@@ -213,9 +231,11 @@ int trace_refill_for(const Device *dev, const World *w);
 // traversal entry (bnx_trace.hip)
 struct TraceQueue {
   float4 *org;          // org.xyz, tmax (tmax replaced by hit t on hit)
-  const float4 *dir;    // dir.xyz, flags (bit 0: shadow ray -> occlusion query)
+  float4 *dir;          // dir.xyz, flags (bit 0: shadow ray -> occlusion query)
   int4 *hit;            // prim, geom, inst, bits(u)
   float *hit_v;         // v
+  const uint64_t *rng;  // RNGSeed state (stochastic opacity)
+  const uint4 *ps;      // path state (generation, pixel: ID channels of translucent hits)
 };
 int launch_trace_api(Group *g, const bnx_ray *d_rays, int n, bnx_hit *d_hits, uint32_t flags);
 struct SlotDev;
@@ -226,5 +246,12 @@ struct SegView;
 // (the launch then takes the kernels that apply it)
 int launch_trace_queue(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, int blocksT,
                        bool pooled, bool cut, const TraceQueue &q, unsigned long long *counters);
+// whether a trace launch takes the extended kernels (cylinder test, cut
+// plane, stochastic opacity): a built world holds cylinders or translucent
+// triangles, or a request of the launch has a cut plane
+inline bool ext_kernels(const Device *dev, bool cut)
+{
+  return dev->cyl_worlds > 0 || dev->translucent_worlds > 0 || cut;
+}
 
 } // namespace bnx

@@ -22,6 +22,9 @@ struct RayQueue {
 // ray flags in dir.w: bit 0 shadow ray, bits 4..15 sample index inside the
 // current batch of its request
 constexpr uint32_t FLAG_SHADOW = 1u;
+// bit 1: the trace wrote the ray's ID channels (a translucent candidate in
+// front of its hit), so shade leaves them
+constexpr uint32_t FLAG_IDS = 2u;
 constexpr int MAX_BATCH_SAMPLES = 4096;
 BNX_D uint32_t mk_flags(bool shadow, int sample)
 {

@@ -291,7 +291,9 @@ BNX_D float lambert_pdf(v3 Ng, v3 wi)
   return c * ONE_OVER_PI;
 }
 
-// hit programs: Triangles.dev.cu:19-135 / Spheres.dev.cu:42-107 + AnariMatte
+// hit programs: Triangles.dev.cu:19-135 / Spheres.dev.cu:42-107 /
+// Cylinders.dev.cu:205-247, each with Geometry::DD::setHitAttributes
+// (geometry/Geometry.h:94-139) and the material's createBSDF
 BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
 {
   SurfHit sh;
@@ -299,21 +301,29 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
   const GeomDev &G = S.geoms[I.geom_base + hit.y];
   const int prim = hit.x;
   const float hu = __int_as_float(hit.w);
-  float color[4] = {0.f, 0.f, 0.f, 1.f};
-  if (G.colors) {
-    const float *cp = G.colors + 3 * prim;
-    color[0] = cp[0];
-    color[1] = cp[1];
-    color[2] = cp[2];
-  }
-  v3 P, Nw;
+  float4 color0 = make_float4(0.f, 0.f, 0.f, 1.f);   // HitAttributes() (render/HitAttributes.h:49-58)
+  v3 P, Nw, osP;
+  Corners cn;
   if (G.type == 0) {
     const int *ix = G.indices + 3 * prim;
     const float *vx = G.vertices;
     v3 v0 = ld3(vx + 3 * ix[0]), v1 = ld3(vx + 3 * ix[1]), v2 = ld3(vx + 3 * ix[2]);
     v3 n = normalize(cross(v1 - v0, v2 - v0));
+    if (G.normals) {
+      // interpolated vertex normals (Triangles.dev.cu:68-75)
+      const v3 Ns = (((1.f - hu) - hv) * ld3(G.normals + 3 * ix[0]) + hu * ld3(G.normals + 3 * ix[1])) +
+                    hv * ld3(G.normals + 3 * ix[2]);
+      n = normalize(Ns);
+    }
     v3 osN = normalize(n);
-    v3 osP = (((1.f - hu) - hv) * v0 + hu * v1) + hv * v2;
+    osP = (((1.f - hu) - hv) * v0 + hu * v1) + hv * v2;
+    cn = tri_corners(ix, prim, hu, hv);
+    // the 'normal' attribute replaces the object normal (Geometry.h:134),
+    // which goes to world space unnormalized (Triangles.dev.cu:112-114)
+    if (G.attr[5].scope != BNX_SCOPE_NONE) {
+      const float4 an = hit_attribute(S, G, 5, hit.z, prim, cn, make_float4(osN.x, osN.y, osN.z, 0.f));
+      osN = v3{an.x, an.y, an.z};
+    }
     if (I.identity) {
       P = osP;
       Nw = osN;
@@ -327,7 +337,9 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
   } else if (G.type == 2) {
     // CylindersPrograms::intersect's hit attributes (Cylinders.dev.cu:
     // 205-236): the object normal of the hit (front or back, tube or cap),
-    // the point moved 1e-4 along it, the world normal normalized twice
+    // the point moved 1e-4 along it, the world normal normalized twice;
+    // attributes interpolated between the two ends by the point's
+    // clamped position along the axis
     v3 o = rorg, d = fix_dir(rdir);
     if (!I.identity) {
       o = xfm_point_rows(I.w2o, rorg);
@@ -340,6 +352,17 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
     v3 objP = o + hu * d;
     objP = objP + 1e-4f * normalize(objN);
     const v3 on = normalize(objN);
+    osP = objP;
+    const v3 v0 = xyz(A), v1 = xyz(B);
+    float lerp_t = dot(objP - v0, v1 - v0) / dot(v1 - v0, v1 - v0);
+    lerp_t = fmx(0.f, fmn(1.f, lerp_t));
+    cn.n = 2;
+    cn.vi[0] = G.indices[2 * prim];
+    cn.vi[1] = G.indices[2 * prim + 1];
+    cn.fi[0] = 2 * prim;
+    cn.fi[1] = 2 * prim + 1;
+    cn.w[0] = 1.f - lerp_t;
+    cn.w[1] = lerp_t;
     if (I.identity) {
       P = objP;
       Nw = normalize(on);
@@ -365,16 +388,22 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
     v3 objN = (objP == ctr) ? v3{1.f, 0.f, 0.f} : (objP - ctr);
     v3 wN = I.identity ? objN : xfm_vector_rows(I.o2w, objN);
     Nw = normalize(wN);
+    // the object position moved off the sphere (Spheres.dev.cu:73-78),
+    // along the unnormalized offset as the reference does
+    float eps = fmx(1e-6f, 1e-6f * fabsf(rad));
+    eps = fmx(eps, 1e-6f * fmx(fmx(fabsf(objP.x), fabsf(objP.y)), fabsf(objP.z)));
+    osP = ctr + (rad + eps) * objN;
+    if (G.colors) {
+      const float *cp = G.colors + 3 * prim;
+      color0 = make_float4(cp[0], cp[1], cp[2], 1.f);
+    }
+    cn.n = 1;
+    cn.vi[0] = cn.fi[0] = prim;
   }
-  float base0, base1, base2;
-  if (G.color_from_attr) {
-    base0 = color[0]; base1 = color[1]; base2 = color[2];
-  } else {
-    base0 = G.color.x; base1 = G.color.y; base2 = G.color.z;
-  }
+  const float4 base = material_color(S, G, hit.z, prim, cn, color0, P, osP);
   sh.pbr = G.material == BNX_MATERIAL_PBR;
-  if (sh.pbr) sh.pp = pbr::params(v3{base0, base1, base2}, G.metallic, G.roughness, G.ior);
-  else sh.albedo = (0.85f * v3{base0, base1, base2}) * ONE_OVER_PI;
+  if (sh.pbr) sh.pp = pbr::params(v3{base.x, base.y, base.z}, G.metallic, G.roughness, G.ior);
+  else sh.albedo = (0.85f * v3{base.x, base.y, base.z}) * ONE_OVER_PI;
   sh.P = P;
   sh.N = quant_half(Nw);
   sh.phase = false;
@@ -633,7 +662,7 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
     frag = vmin(frag, splat(clampMax));
     unsigned long long *acc = fb.accum + 4 * (size_t)pixel;
     if (gen == 0) {
-      if (accumID == 0) {
+      if (accumID == 0 && !(flags & FLAG_IDS)) {
         fb.prim[pixel] = hit.x >= 0 ? hit.x : -1;
         int inst = -1, obj = -1;
         if (iso_hit) {
@@ -652,8 +681,8 @@ __global__ __launch_bounds__(SHADE_BLOCK) void k_shade(const SlotDev *__restrict
         }
         fb.inst[pixel] = inst;
         fb.obj[pixel] = obj;
-        fb.normal[pixel] = make_float4(inN.x, inN.y, inN.z, 0.f);
       }
+      if (accumID == 0) fb.normal[pixel] = make_float4(inN.x, inN.y, inN.z, 0.f);
       atomicMin(fb.depth + pixel, f2o(inZ));
       if (alpha > 0.f) acc_add(acc + 3, alpha);
     }

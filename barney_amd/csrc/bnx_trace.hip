@@ -71,11 +71,53 @@ BNX_D bool accept_t(float t, const HitRec &b, int inst, int geom, int prim)
 // hitOnInvisibleSide, render/OptixGlobals.h:36-45; Triangles.dev.cu:39,
 // Spheres.dev.cu:170, Cylinders.dev.cu:258).  t is the same in the object
 // frame of an instance, whose transform is affine.
+//
+// Stochastic opacity (EXT kernels): a triangle of a geometry whose alpha can
+// be below 1 (GeomDev::translucent) is kept as a candidate only with
+// probability alpha (Triangles.dev.cu:120-133).  Its draw is taken from a
+// copy of the ray's RNG state advanced by the bits of the candidate's
+// object-space position, so each candidate's fate depends on the ray and the
+// candidate alone and the closest kept candidate is the same in any
+// traversal order (the reference advances the ray's own state, in its
+// traversal order).  rng null: every surface is opaque (the ray-array entry
+// point).  The closest candidate the opacity test rejected goes to rej: the
+// ID channels report the closest candidate whatever its opacity
+// (Triangles.dev.cu:45-60).
 struct Cut {
   float4 plane;
   const float4 *rorg, *rdir;
+  const uint64_t *rng;
+  const SceneDev *S;
+  int geom_base;          // the walk's instance: its first geometry-table entry
+  bool translucent;       // its BLAS holds a translucent geometry
+  HitRec *rej;
 };
-BNX_D Cut no_cut() { return Cut{make_float4(0.f, 0.f, 0.f, -1e30f), nullptr, nullptr}; }
+BNX_D Cut no_cut()
+{
+  return Cut{make_float4(0.f, 0.f, 0.f, -1e30f), nullptr, nullptr, nullptr, nullptr, 0, false, nullptr};
+}
+
+// whether the candidate (triangle prim of geometry G at barycentrics u, v,
+// rtc instance inst) survives its opacity draw
+BNX_D bool opacity_keeps(const SceneDev &S, const GeomDev &G, int inst, int prim, float u, float v, uint64_t seed)
+{
+  const int *ix = G.indices + 3 * prim;
+  const v3 v0 = ld3(G.vertices + 3 * ix[0]), v1 = ld3(G.vertices + 3 * ix[1]), v2 = ld3(G.vertices + 3 * ix[2]);
+  const v3 osP = (((1.f - u) - v) * v0 + u * v1) + v * v2;
+  const InstDev &I = S.insts[inst];
+  const v3 P = I.identity ? osP : xfm_point_rows(I.o2w, osP);
+  const float4 base = material_color(S, G, inst, prim, tri_corners(ix, prim, u, v), make_float4(0.f, 0.f, 0.f, 1.f),
+                                     P, osP);
+  const float alpha = material_alpha(G, base);
+  if (alpha < 1.f) {
+    rng_next_h(seed, (uint64_t)__float_as_uint(osP.x));
+    rng_next_h(seed, (uint64_t)__float_as_uint(osP.y));
+    rng_next_h(seed, (uint64_t)__float_as_uint(osP.z));
+    Random2 rng(seed, 290374u);
+    if (rng() > alpha) return false;
+  }
+  return true;
+}
 template <bool EXT> BNX_D bool cut_rejects(const Cut &c, float t)
 {
   if (!EXT || c.plane.w <= -1e28f) return false;
@@ -117,6 +159,15 @@ BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int
   float v = (e1u * Pv - e1v * Pu) / den;
   if (u < 0.f || v < 0.f || (u + v) >= 1.f) return false;
   if (cut_rejects<EXT>(cut, t)) return false;
+  if (EXT && cut.translucent && cut.rng) {
+    const GeomDev &G = cut.S->geoms[cut.geom_base + geom];
+    if (G.translucent && !opacity_keeps(*cut.S, G, inst, prim, u, v, *cut.rng)) {
+      HitRec &r = *cut.rej;
+      if (t < r.t || (t == r.t && (r.prim < 0 || id_less(inst, geom, prim, r.inst, r.geom, r.prim))))
+        r = HitRec{t, u, v, prim, geom, inst};
+      return false;
+    }
+  }
   best = HitRec{t, u, v, prim, geom, inst};
   return true;
 }
@@ -224,12 +275,15 @@ BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 }
 
 // lds: this lane's ring, &ring[0][lane]; the stack depth and the spill
-// array are locals so the depth stays in a register
-template <int SH, bool EXT>
+// array are locals so the depth stays in a register.  XF: some instance
+// transforms the ray (a non-identity instance in a launched world); without
+// it the frame never changes, so the ray (o, d, inv) is loop-invariant and
+// the compiler carries no copies of it around the walk's paths.
+template <int SH, bool EXT, bool XF>
 BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax, bool any_hit,
-                      uint32_t *lds, Counts &cnt, float4 cut_plane)
+                      uint32_t *lds, Counts &cnt, float4 cut_plane, const uint64_t *rng, HitRec &rej)
 {
-  const Cut cut{cut_plane, rorg, rdir};
+  Cut cut{cut_plane, rorg, rdir, rng, &S, 0, false, &rej};
   uint32_t deep[STACK_CAP - SH];
   int sp = 0;
   auto push = [&](uint32_t v) {
@@ -285,7 +339,7 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       }
     } else if (cur == LEAVE_BLAS) {
       in_blas = false;
-      world_ray(rorg, rdir, o, d, inv);
+      if (XF) world_ray(rorg, rdir, o, d, inv);
       nodes = S.tlas_nodes;
     } else {
       // ------------------------------------------------------------ leaf
@@ -295,7 +349,11 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
         inst = S.tlas_order[first];                  // TLAS leaves hold one instance
         const InstDev &I = S.insts[inst];
         blas = S.blas[I.blas];
-        if (!I.identity) {
+        if (EXT) {
+          cut.geom_base = I.geom_base;
+          cut.translucent = blas.translucent != 0;
+        }
+        if (XF && !I.identity) {
           instance_ray(I, rorg, rdir, o, d, inv);
         }
         push(LEAVE_BLAS);
@@ -339,11 +397,12 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
 // may step nodes that a hit in its pending leaf would have culled; the hit
 // returned is the same (equal t resolves by ID).  cur == EMPTY4: the stack
 // is exhausted.
-template <int SH, bool EXT>
+template <int SH, bool EXT, bool XF>
 BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *rdir, float tmin, float tmax,
-                           bool any_hit, uint32_t *lds, Counts &cnt, float4 cut_plane)
+                           bool any_hit, uint32_t *lds, Counts &cnt, float4 cut_plane, const uint64_t *rng,
+                           HitRec &rej)
 {
-  const Cut cut{cut_plane, rorg, rdir};
+  Cut cut{cut_plane, rorg, rdir, rng, &S, 0, false, &rej};
   uint32_t deep[STACK_CAP - SH];
   int sp = 0;
   auto push = [&](uint32_t v) {
@@ -433,7 +492,11 @@ BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *
       inst = S.tlas_order[(cur & ~LEAF_BIT) >> 3];
       const InstDev &I = S.insts[inst];
       blas = S.blas[I.blas];
-      if (!I.identity) {
+      if (EXT) {
+        cut.geom_base = I.geom_base;
+        cut.translucent = blas.translucent != 0;
+      }
+      if (XF && !I.identity) {
         instance_ray(I, rorg, rdir, o, d, inv);
       }
       push(LEAVE_BLAS);
@@ -442,7 +505,7 @@ BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *
       cur = 0;
     } else if (cur == LEAVE_BLAS) {
       in_blas = false;
-      world_ray(rorg, rdir, o, d, inv);
+      if (XF) world_ray(rorg, rdir, o, d, inv);
       nodes = S.tlas_nodes;
       cur = pop();
     }
@@ -464,8 +527,9 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
     const float4 *rp = (const float4 *)(rays + i);
     float4 r0 = rp[0], r1 = rp[1];
     Counts cnt{0, 0, 0};
-    HitRec h = traverse<API_SHORT, true>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt,
-                                         no_cut().plane);
+    HitRec rej{__builtin_huge_valf(), 0.f, 0.f, -1, -1, -1};
+    HitRec h = traverse<API_SHORT, true, true>(S, rp, rp + 1, r0.w, r1.w, (flags & BNX_TRACE_ANY) != 0, lds, cnt,
+                                               no_cut().plane, nullptr, rej);
     bnx_hit out;
     out.t = h.t;
     out.u = h.prim >= 0 ? h.u : 0.f;
@@ -484,11 +548,29 @@ __global__ __launch_bounds__(TB) void k_trace_api(SceneDev S, const bnx_ray *ray
   }
 }
 
+// A translucent candidate the opacity test rejected lies in front of the
+// ray's hit: for a primary ray of a request's first sample the trace writes
+// the ID channels from it (as the reference's any-hit program writes them
+// before the opacity test, Triangles.dev.cu:45-60) and flags the ray so that
+// shade leaves them.
+BNX_D void translucent_ids(const SlotDev &P, const TraceQueue &q, int i, float4 d, const HitRec &rej)
+{
+  const uint4 ps = q.ps[i];
+  const uint32_t f = __float_as_uint(d.w);
+  if ((ps.y >> 24) != 0 || (f & FLAG_SHADOW) || P.accum_base + flag_sample(f) != 0) return;
+  const int pixel = (int)ps.z;
+  const InstDev &I = P.scene.insts[rej.inst];
+  P.fb.prim[pixel] = rej.prim;
+  P.fb.inst[pixel] = I.user_id;
+  P.fb.obj[pixel] = P.scene.geoms[I.geom_base + rej.geom].user_id;
+  q.dir[i].w = __uint_as_float(f | FLAG_IDS);
+}
+
 // one ray per thread over the generation's segments (256-thread block map);
 // each block lies inside one request's segment, so the request's scene
 // pointers are block-uniform.  counters: statistics array (STAT_NODES,
 // STAT_TRIS, STAT_SPHERES)
-template <int SH, bool SPEC, bool EXT>
+template <int SH, bool SPEC, bool EXT, bool XF>
 BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, const TraceQueue &q,
                         unsigned long long *counters)
 {
@@ -505,12 +587,16 @@ BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, co
     float4 d = q.dir[i];
     const bool shadow = (__float_as_uint(d.w) & FLAG_SHADOW) != 0;
     HitRec h{o.w, 0.f, 0.f, -1, -1, -1};
+    HitRec rej{__builtin_huge_valf(), 0.f, 0.f, -1, -1, -1};
     if (S.num_insts > 0)
-      h = SPEC ? traverse_spec<SH, EXT>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt, cut_plane)
-               : traverse<SH, EXT>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt, cut_plane);
+      h = SPEC ? traverse_spec<SH, EXT, XF>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt, cut_plane,
+                                            q.rng + i, rej)
+               : traverse<SH, EXT, XF>(S, q.org + i, q.dir + i, 0.f, o.w, shadow, lds, cnt, cut_plane, q.rng + i,
+                                       rej);
     if (h.prim >= 0) q.org[i].w = h.t;
     q.hit[i] = make_int4(h.prim, h.geom, h.inst, __float_as_int(h.u));
     q.hit_v[i] = h.v;
+    if (EXT && rej.prim >= 0 && rej.t < h.t) translucent_ids(slots[sg.slot], q, i, d, rej);
   }
   uint32_t a = wave_sum(cnt.nodes), b = wave_sum(cnt.tris), c = wave_sum(cnt.spheres);
   if (lane_id() == 0) {
@@ -524,18 +610,18 @@ BNX_D void trace_static(const SlotDev *__restrict__ slots, const SegView &sv, co
 // request.  The kernels without it serve worlds of triangles and spheres
 // traced without a cut plane; the launcher takes the _ext kernels while any
 // built world holds cylinders or a request in the launch has a cut plane.
-template <int SH, bool SPEC>
+template <int SH, bool SPEC, bool XF>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
-  trace_static<SH, SPEC, false>(slots, sv, q, counters);
+  trace_static<SH, SPEC, false, XF>(slots, sv, q, counters);
 }
 
-template <int SH, bool SPEC>
+template <int SH, bool SPEC, bool XF>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_static_ext(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
-  trace_static<SH, SPEC, true>(slots, sv, q, counters);
+  trace_static<SH, SPEC, true, XF>(slots, sv, q, counters);
 }
 
 // The generation's rays, segment by segment: every block lies inside one
@@ -547,7 +633,7 @@ __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WA
 // taken one node or leaf per loop trip.
 constexpr int POOL_CHUNK = 64;
 
-template <int SH, bool EXT>
+template <int SH, bool EXT, bool XF>
 BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, const TraceQueue &q,
                       unsigned long long *counters)
 {
@@ -580,6 +666,8 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
   int ray = 0, inst = -1;
   v3 o{0.f, 0.f, 0.f}, d{1.f, 1.f, 1.f}, inv{1.f, 1.f, 1.f};
   HitRec best{0.f, 0.f, 0.f, -1, -1, -1};
+  HitRec rej{0.f, 0.f, 0.f, -1, -1, -1};
+  int geom_base = 0;     // EXT: the lane's instance's first geometry-table entry
   const Node4 *nodes = S.tlas_nodes;
   BlasDev blas{};
   uint32_t cur = 0;
@@ -616,6 +704,7 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
         d = fix_dir(xyz(rd));
         inv = v3{1.f / d.x, 1.f / d.y, 1.f / d.z};
         best = HitRec{ro.w, 0.f, 0.f, -1, -1, -1};
+        rej = HitRec{__builtin_huge_valf(), 0.f, 0.f, -1, -1, -1};
         nodes = S.tlas_nodes;
         in_blas = false;
         inst = -1;
@@ -659,7 +748,7 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
         }
       } else if (cur == LEAVE_BLAS) {
         in_blas = false;
-        world_ray(q.org + ray, q.dir + ray, o, d, inv);
+        if (XF) world_ray(q.org + ray, q.dir + ray, o, d, inv);
         nodes = S.tlas_nodes;
         if (sp == 0) done = true;
         else cur = pop();
@@ -670,7 +759,8 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
           inst = S.tlas_order[first];                  // TLAS leaves hold one instance
           const InstDev &I = S.insts[inst];
           blas = S.blas[I.blas];
-          if (!I.identity) {
+          geom_base = I.geom_base;
+          if (XF && !I.identity) {
             instance_ray(I, q.org + ray, q.dir + ray, o, d, inv);
           }
           push(LEAVE_BLAS);
@@ -679,7 +769,7 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
           cur = 0;
         } else {
           bool found = false;
-          const Cut cut{cut_plane, q.org + ray, q.dir + ray};
+          const Cut cut{cut_plane, q.org + ray, q.dir + ray, q.rng + ray, &S, geom_base, blas.translucent != 0, &rej};
           if (blas.type == BLAS_TRIANGLES) {
             cnt.tris += count;
             for (int k = 0; k < count; k++)
@@ -701,6 +791,7 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
         if (best.prim >= 0) q.org[ray].w = best.t;
         q.hit[ray] = make_int4(best.prim, best.geom, best.inst, __float_as_int(best.u));
         q.hit_v[ray] = best.v;
+        if (EXT && rej.prim >= 0 && rej.t < best.t) translucent_ids(slots[sg.slot], q, ray, q.dir[ray], rej);
         active = false;
       }
       // leave the walk for a refill once refill_at lanes are idle (the test
@@ -716,34 +807,47 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
   }
 }
 
-template <int SH>
+template <int SH, bool XF>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
-  trace_pool<SH, false>(slots, sv, q, counters);
+  trace_pool<SH, false, XF>(slots, sv, q, counters);
 }
 
-template <int SH>
+template <int SH, bool XF>
 __global__ __launch_bounds__(TB) __attribute__((amdgpu_waves_per_eu(BNX_TRACE_WAVES))) void k_trace_pool_ext(
     const SlotDev *__restrict__ slots, SegView sv, TraceQueue q, unsigned long long *counters)
 {
-  trace_pool<SH, true>(slots, sv, q, counters);
+  trace_pool<SH, true, XF>(slots, sv, q, counters);
+}
+
+template <int SH, bool SPEC, bool XF>
+void launch_static_xf(Device *dev, bool ext, int blocks, const SlotDev *slots, const SegView &sv,
+                      const TraceQueue &q, unsigned long long *counters)
+{
+  if (ext) k_trace_static_ext<SH, SPEC, XF><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  else k_trace_static<SH, SPEC, XF><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
 }
 
 template <int SH, bool SPEC>
-void launch_static(Device *dev, bool ext, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
-                   unsigned long long *counters)
+void launch_static(Device *dev, bool ext, bool xf, int blocks, const SlotDev *slots, const SegView &sv,
+                   const TraceQueue &q, unsigned long long *counters)
 {
-  if (ext) k_trace_static_ext<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
-  else k_trace_static<SH, SPEC><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  if (xf) launch_static_xf<SH, SPEC, true>(dev, ext, blocks, slots, sv, q, counters);
+  else launch_static_xf<SH, SPEC, false>(dev, ext, blocks, slots, sv, q, counters);
 }
 
 template <int SH>
-void launch_pool(Device *dev, bool ext, int blocks, const SlotDev *slots, const SegView &sv, const TraceQueue &q,
-                 unsigned long long *counters)
+void launch_pool(Device *dev, bool ext, bool xf, int blocks, const SlotDev *slots, const SegView &sv,
+                 const TraceQueue &q, unsigned long long *counters)
 {
-  if (ext) k_trace_pool_ext<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
-  else k_trace_pool<SH><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  if (ext) {
+    if (xf) k_trace_pool_ext<SH, true><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+    else k_trace_pool_ext<SH, false><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  } else {
+    if (xf) k_trace_pool<SH, true><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+    else k_trace_pool<SH, false><<<blocks, TB, 0, dev->stream>>>(slots, sv, q, counters);
+  }
 }
 
 int grid_for(int n) { return n > 0 ? (n + TB - 1) / TB : 1; }
@@ -764,25 +868,26 @@ int launch_trace_queue(Device *dev, const SlotDev *slots, const SegView &sv, int
                        bool cut, const TraceQueue &q, unsigned long long *counters)
 {
   const int sh = dev->trace_stack_lds;
-  const bool ext = dev->cyl_worlds > 0 || cut;
+  const bool ext = ext_kernels(dev, cut);
+  const bool xf = dev->xf_worlds > 0;
   if (pooled) {
     if (blocksT > 0) {
-      if (sh == 8) launch_pool<8>(dev, ext, blocksT, slots, sv, q, counters);
-      else if (sh == 32) launch_pool<32>(dev, ext, blocksT, slots, sv, q, counters);
-      else launch_pool<16>(dev, ext, blocksT, slots, sv, q, counters);
+      if (sh == 8) launch_pool<8>(dev, ext, xf, blocksT, slots, sv, q, counters);
+      else if (sh == 32) launch_pool<32>(dev, ext, xf, blocksT, slots, sv, q, counters);
+      else launch_pool<16>(dev, ext, xf, blocksT, slots, sv, q, counters);
     }
   } else {
     if (blocks256 > 0) {
       const bool spec = dev->trace_walk == 1;
       if (sh == 8) {
-        if (spec) launch_static<8, true>(dev, ext, blocks256, slots, sv, q, counters);
-        else launch_static<8, false>(dev, ext, blocks256, slots, sv, q, counters);
+        if (spec) launch_static<8, true>(dev, ext, xf, blocks256, slots, sv, q, counters);
+        else launch_static<8, false>(dev, ext, xf, blocks256, slots, sv, q, counters);
       } else if (sh == 32) {
-        if (spec) launch_static<32, true>(dev, ext, blocks256, slots, sv, q, counters);
-        else launch_static<32, false>(dev, ext, blocks256, slots, sv, q, counters);
+        if (spec) launch_static<32, true>(dev, ext, xf, blocks256, slots, sv, q, counters);
+        else launch_static<32, false>(dev, ext, xf, blocks256, slots, sv, q, counters);
       } else {
-        if (spec) launch_static<16, true>(dev, ext, blocks256, slots, sv, q, counters);
-        else launch_static<16, false>(dev, ext, blocks256, slots, sv, q, counters);
+        if (spec) launch_static<16, true>(dev, ext, xf, blocks256, slots, sv, q, counters);
+        else launch_static<16, false>(dev, ext, xf, blocks256, slots, sv, q, counters);
       }
     }
   }

@@ -106,6 +106,9 @@ class TriMesh:
     colors: np.ndarray | None     # (nt,3) float32 per-prim
     user_id: int = 0
     material: object = None       # abi.Material; None: matte coloured by the attribute
+    normals: np.ndarray | None = None   # (nv,3) float32 vertex normals
+    # geometry attributes: slot (abi.ATTRIB_SLOT_*) -> (scope, (n, comps) data or an rgba constant)
+    attributes: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -115,6 +118,7 @@ class SphereSet:
     colors: np.ndarray | None     # (n,3) float32
     user_id: int = 1
     material: object = None
+    attributes: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -125,6 +129,7 @@ class CylinderSet:
     colors: np.ndarray | None     # (n,3) float32 per cylinder
     user_id: int = 4
     material: object = None
+    attributes: dict = field(default_factory=dict)
 
 
 @dataclass
@@ -177,6 +182,9 @@ class SurfaceScene:
     height: int = 512
     spp: int = 1
     max_generations: int = 0       # renderer bound on traced generations (0: the path-depth limit)
+    # instance attributes: slot 0-4 -> (n_rtc_instances, 4) float32, one row per rtc instance
+    # (spheres group, cylinders group, triangles group, in that order)
+    inst_attr: dict = field(default_factory=dict)
 
     @property
     def num_prims(self):

@@ -80,9 +80,24 @@ typedef struct bnx_hit {
 #define BNX_MATERIAL_PBR   1        /* AnariPBR -> NVisii (material/AnariPBR.h:62-96,
                                        packedBSDF/NVisii.h)                        */
 
+/* Where a material's color comes from (PossiblyMappedParameter, material/
+ * HostMaterial.h:100-114, HitAttributes::get, render/HitAttributes.h:61-80):
+ * its value, or a hit attribute.  Positions read (x, y, z, 1); the reference
+ * leaves the normals and the primitive ID unimplemented there, reading
+ * (0, 0, 0, 1). */
+#define BNX_ATTRIB_NONE            0
+#define BNX_ATTRIB_COLOR           1
+#define BNX_ATTRIB_0               2   /* .. BNX_ATTRIB_0 + 3: attribute0..3   */
+#define BNX_ATTRIB_WORLD_POSITION  6
+#define BNX_ATTRIB_OBJECT_POSITION 7
+#define BNX_ATTRIB_WORLD_NORMAL    8
+#define BNX_ATTRIB_OBJECT_NORMAL   9
+#define BNX_ATTRIB_PRIMITIVE_ID   10
+
 typedef struct bnx_material {
   int32_t type;                     /* BNX_MATERIAL_*                              */
-  int32_t color_from_attribute;     /* 1: color = geometry's per-prim 'color'      */
+  int32_t color_from_attribute;     /* BNX_ATTRIB_*: the hit attribute the color
+                                       (with its alpha) is read from; 0: 'color' */
   float   color[4];                 /* matte 'color' / PBR 'baseColor' when not
                                        from the attribute                          */
   float   opacity;                  /* 'opacity'                                   */
@@ -102,7 +117,8 @@ typedef struct bnx_triangles_desc {
   int32_t        num_vertices;
   int32_t        num_triangles;
   const int32_t *indices;           /* int3[num_triangles]                         */
-  const float   *prim_colors;       /* float3[num_triangles] or NULL               */
+  const float   *prim_colors;       /* float3[num_triangles] or NULL: the "color"
+                                       attribute per primitive                     */
   int32_t        user_id;           /* objID reported in hit IDs                   */
   int32_t        _pad;
   bnx_material   material;
@@ -122,8 +138,9 @@ typedef struct bnx_spheres_desc {
 
 /* Cylinders: Cylinders::DD (native/geometry/Cylinders.h:20-27), capped
  * cylinders between two vertices each, intersected as CylindersPrograms::
- * intersect does (native/geometry/Cylinders.dev.cu:107-260).  The "color"
- * attribute is per cylinder (not interpolated along it). */
+ * intersect does (native/geometry/Cylinders.dev.cu:107-260).  colors: the
+ * "color" attribute per cylinder (bnx_geom_set_attribute sets attributes per
+ * vertex, interpolated along the axis). */
 typedef struct bnx_cylinders_desc {
   const float   *vertices;          /* float3[num_vertices]                        */
   const int32_t *indices;           /* int2[num_cylinders]: end points             */
@@ -283,6 +300,11 @@ typedef struct bnx_render_stats {
   int64_t march_rays;               /* rays entering the volume-march kernel       */
   int64_t umesh_nodes;              /* element-BVH nodes visited by tet point queries */
   int64_t umesh_tets;               /* tets tested by tet point queries            */
+  int64_t captured_batches;         /* batches whose generations replayed a captured hipGraph */
+  int64_t uncaptured_batches;       /* device-driven batches enqueued directly: capture is
+                                       unavailable on this device (capture_error)  */
+  int32_t capture_error;            /* hipError_t of the device's failed capture, 0 if none */
+  int32_t reserved;
 } bnx_render_stats;
 
 /* BVH summary of one BLAS / TLAS (for tests and the byte model). */
@@ -409,6 +431,31 @@ int bnx_geom_create_spheres(bnx_device dev, const bnx_spheres_desc *desc, bnx_ge
  * Cylinders.dev.cu:266) */
 int bnx_geom_create_cylinders(bnx_device dev, const bnx_cylinders_desc *desc, bnx_geom *out);
 int bnx_geom_destroy(bnx_geom geom);
+/* Geometry attributes (GeometryAttributes, render/GeometryAttributes.h:
+ * 32-63; Geometry::DD::setHitAttributes, geometry/Geometry.h:94-139): one
+ * of attribute0..3, color or normal (BNX_ATTRIB_SLOT_*) set for the geometry
+ * with a scope: a constant value, or per primitive, per vertex (triangles:
+ * barycentric interpolation; cylinders: linear along the axis; spheres: the
+ * sphere's own) or face-varying (3 per triangle, 2 per cylinder) elements of
+ * 1-4 floats each (AttributeArray::DD::valueAt: missing y, z read 0, a
+ * missing w reads 1).  BNX_SCOPE_NONE clears it.  The 'normal' attribute
+ * replaces a triangle's object normal (Triangles.dev.cu:86-111).  The data
+ * are copied; a group built over the geometry sees them after its next
+ * bnx_group_build.  count: elements in data. */
+#define BNX_ATTRIB_SLOT_0      0
+#define BNX_ATTRIB_SLOT_COLOR  4
+#define BNX_ATTRIB_SLOT_NORMAL 5
+#define BNX_SCOPE_NONE         0   /* GeometryAttribute::INVALID      */
+#define BNX_SCOPE_CONSTANT     1
+#define BNX_SCOPE_PER_PRIM     2
+#define BNX_SCOPE_PER_VERTEX   3
+#define BNX_SCOPE_FACE_VARYING 4
+int bnx_geom_set_attribute(bnx_geom geom, int slot, int scope, int components, const float *data, int64_t count,
+                           const float value[4]);
+/* Triangles::DD::normals (geometry/Triangles.h:26-32): per-vertex normals,
+ * interpolated and normalized at the hit (Triangles.dev.cu:68-75); NULL
+ * clears them. */
+int bnx_geom_set_vertex_normals(bnx_geom geom, const float *normals, int num_vertices);
 
 /* -------------------------------------------------------------- groups */
 /* rtc::Device::createTrianglesGroup (rtcore/cuda/Group.cu:261-413) */
@@ -429,6 +476,12 @@ int bnx_group_create_instances(bnx_device dev, const bnx_group *groups, const in
  * the new transforms from their next request; not while requests over it are
  * in flight on a server. */
 int bnx_group_set_transforms(bnx_group instances, const float *xfms, int num_instances);
+/* Instance attributes (World::DD::instanceAttributes, bnSetInstanceAttributes,
+ * native/barney.cu): one float4 per instance of an instance group for
+ * attribute0..3 or color (slot 0-4); a hit on a geometry that does not set
+ * that attribute reads its instance's value (Geometry.h:103-110).  NULL
+ * clears the slot.  Worlds over the group see it from their next request. */
+int bnx_group_set_instance_attribute(bnx_group instances, int slot, const float *values, int num_instances);
 /* rtc::Group::buildAccel: binned-SAH BVH built on the GPU. For an instance
  * group every referenced group must have been built first. */
 int bnx_group_build(bnx_group group);
@@ -659,7 +712,7 @@ static_assert(sizeof(bnx_server_stats) == 168, "bnx_server_stats layout");
 static_assert(sizeof(bnx_tuning) == 56, "bnx_tuning layout");
 static_assert(sizeof(bnx_autotune_report) == 24, "bnx_autotune_report layout");
 static_assert(sizeof(bnx_renderer) == 48, "bnx_renderer layout");
-static_assert(sizeof(bnx_render_stats) == 128, "bnx_render_stats layout");
+static_assert(sizeof(bnx_render_stats) == 152, "bnx_render_stats layout");
 #endif
 
 #endif /* BNX_RT_H */

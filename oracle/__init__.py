@@ -41,6 +41,10 @@ def lib():
         L.orc_scene_add_cylinders.argtypes = [vp, C.POINTER(abi.CylindersDesc)]
         L.orc_cylinder_intersect_n.argtypes = [abi.fp, C.c_int, abi.fp, C.c_float, C.c_float, abi.fp]
         L.orc_scene_add_group.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
+        L.orc_geom_set_attribute.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, abi.fp, C.c_int64, abi.fp]
+        L.orc_geom_set_vertex_normals.argtypes = [vp, C.c_int, abi.fp, C.c_int]
+        L.orc_scene_set_instance_attribute.argtypes = [vp, C.c_int, abi.fp, C.c_int]
+        L.orc_trace_opacity.argtypes = [vp, vp, C.POINTER(C.c_uint64), C.c_int, vp, vp, C.c_int]
         L.orc_scene_add_instance.argtypes = [vp, C.c_int, C.c_int, abi.fp, abi.fp]
         L.orc_scene_set_dir_lights.argtypes = [vp, C.POINTER(abi.DirLight), C.c_int]
         L.orc_scene_set_env_map.argtypes = [vp, C.POINTER(abi.EnvMapDesc)]
@@ -189,6 +193,36 @@ class Scene:
         d = abi.CylindersDesc(abi.fptr(v), abi.iptr(i), abi.fptr(r), abi.fptr(c), len(v), len(i), default_radius,
                               user_id, material or abi.matte())
         return self.L.orc_scene_add_cylinders(self.h, C.byref(d))
+
+    def set_attribute(self, geom: int, slot: int, scope: int, data=None, value=None):
+        """a geometry attribute (bnx_geom_set_attribute): data (n, comps) floats, or a constant rgba"""
+        d = None if data is None else np.ascontiguousarray(data, dtype=np.float32)
+        comps = 0 if d is None else (1 if d.ndim == 1 else d.shape[1])
+        v = None if value is None else np.ascontiguousarray(value, dtype=np.float32)
+        self._keep += [d, v]
+        assert self.L.orc_geom_set_attribute(self.h, geom, slot, scope, comps, abi.fptr(d),
+                                             0 if d is None else len(d), abi.fptr(v)) == 0
+
+    def set_vertex_normals(self, geom: int, normals):
+        n = None if normals is None else np.ascontiguousarray(normals, dtype=np.float32)
+        self._keep += [n]
+        assert self.L.orc_geom_set_vertex_normals(self.h, geom, abi.fptr(n), 0 if n is None else len(n)) == 0
+
+    def set_instance_attribute(self, slot: int, values):
+        v = None if values is None else np.ascontiguousarray(values, dtype=np.float32)
+        self._keep += [v]
+        assert self.L.orc_scene_set_instance_attribute(self.h, slot, abi.fptr(v), 0 if v is None else len(v)) == 0
+
+    def trace_opacity(self, rays, seeds, mode=0):
+        """closest hits with stochastic opacity (ray i's RNG state seeds[i]) and the
+        closest candidates the opacity test rejected; mode 0 BVH walk, 1 brute force"""
+        r = np.ascontiguousarray(rays)
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        hits = np.zeros(len(r), abi.HIT_DTYPE)
+        rej = np.zeros(len(r), abi.HIT_DTYPE)
+        self.L.orc_trace_opacity(self.h, r.ctypes.data, s.ctypes.data_as(C.POINTER(C.c_uint64)), len(r),
+                                 hits.ctypes.data, rej.ctypes.data, mode)
+        return hits, rej
 
     def add_group(self, kind: int, geoms):
         g = (C.c_int * len(geoms))(*geoms)
@@ -476,6 +510,16 @@ def surface_scene(sc) -> Scene:
            for x in getattr(sc, "cylinders", [])]
     tri = [s.add_triangles(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
            for m in sc.meshes]
+    for gi, obj in zip(sph + cyl + tri, list(sc.spheres) + list(getattr(sc, "cylinders", [])) + list(sc.meshes)):
+        for slot, (scope, arr) in getattr(obj, "attributes", {}).items():
+            if scope == abi.SCOPE_CONSTANT:
+                s.set_attribute(gi, slot, scope, value=arr)
+            else:
+                s.set_attribute(gi, slot, scope, data=arr)
+        if getattr(obj, "normals", None) is not None:
+            s.set_vertex_normals(gi, obj.normals)
+    for slot, vals in getattr(sc, "inst_attr", {}).items():
+        s.set_instance_attribute(slot, vals)
     if sph:
         s.add_instance(s.add_group(1, sph), 0)
     if cyl:

@@ -30,7 +30,7 @@ struct orc_frame {
 static void set_material(Material &m, const bnx_material &d)
 {
   m.type = d.type;
-  m.color_from_attribute = d.color_from_attribute != 0;
+  m.color_from_attribute = d.color_from_attribute;
   memcpy(m.color, d.color, sizeof(m.color));
   m.opacity = d.opacity;
   m.metallic = d.metallic;
@@ -52,8 +52,13 @@ int orc_scene_add_triangles(orc_scene *sc, const bnx_triangles_desc *d)
   set_material(g.mat, d->material);
   for (int i = 0; i < d->num_vertices; i++) g.vertices.push_back(load3(d->vertices + 3 * i));
   g.indices.assign(d->indices, d->indices + 3 * (size_t)d->num_triangles);
-  if (d->prim_colors)
-    for (int i = 0; i < d->num_triangles; i++) g.colors.push_back(load3(d->prim_colors + 3 * i));
+  if (d->prim_colors) {
+    // the "color" attribute per primitive
+    Attribute &a = g.attr[BNX_ATTRIB_SLOT_COLOR];
+    a.scope = BNX_SCOPE_PER_PRIM;
+    a.comps = 3;
+    a.data.assign(d->prim_colors, d->prim_colors + 3 * (size_t)d->num_triangles);
+  }
   sc->s.geoms.push_back(std::move(g));
   return (int)sc->s.geoms.size() - 1;
 }
@@ -83,10 +88,50 @@ int orc_scene_add_cylinders(orc_scene *sc, const bnx_cylinders_desc *d)
   for (int i = 0; i < d->num_vertices; i++) g.vertices.push_back(load3(d->vertices + 3 * i));
   g.indices.assign(d->indices, d->indices + 2 * (size_t)d->num_cylinders);
   if (d->radii) g.radii.assign(d->radii, d->radii + d->num_cylinders);
-  if (d->colors)
-    for (int i = 0; i < d->num_cylinders; i++) g.colors.push_back(load3(d->colors + 3 * i));
+  if (d->colors) {
+    Attribute &a = g.attr[BNX_ATTRIB_SLOT_COLOR];
+    a.scope = BNX_SCOPE_PER_PRIM;
+    a.comps = 3;
+    a.data.assign(d->colors, d->colors + 3 * (size_t)d->num_cylinders);
+  }
   sc->s.geoms.push_back(std::move(g));
   return (int)sc->s.geoms.size() - 1;
+}
+
+// GeometryAttributes of geometry 'geom' (render/GeometryAttributes.cpp:11-46
+// resolves one scope per attribute); count elements of comps floats
+int orc_geom_set_attribute(orc_scene *sc, int geom, int slot, int scope, int comps, const float *data, int64_t count,
+                           const float *value)
+{
+  if (geom < 0 || geom >= (int)sc->s.geoms.size() || slot < 0 || slot >= 6) return -1;
+  Attribute &a = sc->s.geoms[geom].attr[slot];
+  a = Attribute{};
+  a.scope = scope;
+  if (scope == BNX_SCOPE_CONSTANT) {
+    a.comps = 4;
+    memcpy(a.value, value, sizeof(a.value));
+  } else if (scope != BNX_SCOPE_NONE) {
+    a.comps = comps;
+    a.data.assign(data, data + (size_t)comps * count);
+  }
+  return 0;
+}
+
+int orc_geom_set_vertex_normals(orc_scene *sc, int geom, const float *normals, int n)
+{
+  if (geom < 0 || geom >= (int)sc->s.geoms.size()) return -1;
+  Geom &g = sc->s.geoms[geom];
+  g.normals.clear();
+  for (int i = 0; normals && i < n; i++) g.normals.push_back(load3(normals + 3 * i));
+  return 0;
+}
+
+// World::DD::instanceAttributes slot (attribute0..3, color): rgba per instance
+int orc_scene_set_instance_attribute(orc_scene *sc, int slot, const float *values, int n)
+{
+  if (slot < 0 || slot >= 5) return -1;
+  sc->s.inst_attr[slot].assign(values ? values : nullptr, values ? values + 4 * (size_t)n : nullptr);
+  return 0;
 }
 
 int orc_scene_add_group(orc_scene *sc, int type, const int *geoms, int n)
@@ -369,6 +414,34 @@ double orc_scene_build(orc_scene *sc)
 
 // mode 0: BVH closest, 1: brute force closest, 2: BVH any hit.
 // counters: nodes, tris, spheres, rays.  cut_plane: 4 floats or null.
+// closest hits with stochastic opacity: ray i's RNG state seeds[i]; rej[i]
+// the closest candidate the opacity test rejected; mode 0 the BVH walk,
+// 1 brute force
+int orc_trace_opacity(orc_scene *sc, const bnx_ray *rays, const uint64_t *seeds, int n, bnx_hit *hits, bnx_hit *rej,
+                      int mode)
+{
+  auto out = [&](const Hit &h, bnx_hit &o) {
+    o.t = h.t;
+    o.u = h.prim >= 0 ? h.u : 0.f;
+    o.v = h.prim >= 0 ? h.v : 0.f;
+    o.prim_id = h.prim;
+    o.geom_id = h.prim >= 0 ? h.geom : -1;
+    o.inst = h.prim >= 0 ? h.inst : -1;
+    o.user_inst_id = h.prim >= 0 ? sc->s.insts[h.inst].user_id : -1;
+    o.obj_id = h.prim >= 0 ? sc->s.geoms[sc->s.groups[sc->s.insts[h.inst].group].geoms[h.geom]].user_id : -1;
+  };
+  for (int i = 0; i < n; i++) {
+    const bnx_ray &r = rays[i];
+    Hit rj;
+    Hit h = mode == 1 ? trace_brute(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax, nullptr, seeds + i, &rj)
+                      : trace_bvh(sc->s, load3(r.org), load3(r.dir), r.tmin, r.tmax, false, nullptr, nullptr,
+                                  seeds + i, &rj);
+    out(h, hits[i]);
+    out(rj, rej[i]);
+  }
+  return 0;
+}
+
 int orc_trace(orc_scene *sc, const bnx_ray *rays, int n, bnx_hit *hits, int mode, uint64_t *counters, int threads,
               const float *cut_plane)
 {

@@ -317,6 +317,103 @@ void build_scene(Scene &s)
 }
 
 // ======================================================================
+// hit attributes
+// ======================================================================
+namespace {
+
+// AttributeArray::DD::valueAt (render/GeometryAttributes.h:68-91)
+void value_at(const Attribute &a, int i, float out[4])
+{
+  const float *p = &a.data[(size_t)a.comps * i];
+  out[0] = out[1] = out[2] = 0.f;
+  out[3] = 1.f;
+  for (int k = 0; k < a.comps && k < 4; k++) out[k] = p[k];
+}
+
+// (w0 a + w1 b) + w2 c, component by component (vec4f operators)
+void interpolate(const Attribute &a, const Corners &c, bool face_varying, float out[4])
+{
+  const int *ix = face_varying ? c.fi : c.vi;
+  if (c.n == 1) {
+    value_at(a, ix[0], out);
+    return;
+  }
+  float va[4], vb[4], vc[4];
+  value_at(a, ix[0], va);
+  value_at(a, ix[1], vb);
+  for (int k = 0; k < 4; k++) out[k] = c.w[0] * va[k] + c.w[1] * vb[k];
+  if (c.n == 3) {
+    value_at(a, ix[2], vc);
+    for (int k = 0; k < 4; k++) out[k] = out[k] + c.w[2] * vc[k];
+  }
+}
+
+} // namespace
+
+void hit_attribute(const Scene &s, const Geom &g, int k, int inst, int prim, const Corners &c, const float dflt[4],
+                   float out[4])
+{
+  const Attribute &a = g.attr[k];
+  switch (a.scope) {
+  case BNX_SCOPE_CONSTANT: memcpy(out, a.value, 4 * sizeof(float)); return;
+  case BNX_SCOPE_PER_PRIM: value_at(a, prim, out); return;
+  case BNX_SCOPE_PER_VERTEX: interpolate(a, c, false, out); return;
+  case BNX_SCOPE_FACE_VARYING: interpolate(a, c, true, out); return;
+  default: break;
+  }
+  // no geometry attribute: the instance's value (Geometry.h:103-110)
+  if (k < 5 && !s.inst_attr[k].empty()) {
+    memcpy(out, &s.inst_attr[k][4 * (size_t)inst], 4 * sizeof(float));
+    return;
+  }
+  memcpy(out, dflt, 4 * sizeof(float));
+}
+
+void material_color(const Scene &s, const Geom &g, int inst, int prim, const Corners &c, const float color0[4],
+                    vec3 P, vec3 osP, float out[4])
+{
+  const float none[4] = {0.f, 0.f, 0.f, 1.f};
+  const int k = g.mat.color_from_attribute;
+  if (k == BNX_ATTRIB_NONE) {
+    // a value; NaN: unset (HostMaterial.h:104-106)
+    memcpy(out, std::isnan(g.mat.color[0]) ? none : g.mat.color, 4 * sizeof(float));
+  } else if (k == BNX_ATTRIB_COLOR) {
+    hit_attribute(s, g, 4, inst, prim, c, color0, out);
+  } else if (k >= BNX_ATTRIB_0 && k < BNX_ATTRIB_0 + 4) {
+    hit_attribute(s, g, k - BNX_ATTRIB_0, inst, prim, c, none, out);
+  } else if (k == BNX_ATTRIB_WORLD_POSITION || k == BNX_ATTRIB_OBJECT_POSITION) {
+    // HitAttributes::get (render/HitAttributes.h:61-80)
+    const vec3 p = k == BNX_ATTRIB_WORLD_POSITION ? P : osP;
+    out[0] = p.x;
+    out[1] = p.y;
+    out[2] = p.z;
+    out[3] = 1.f;
+  } else {
+    memcpy(out, none, sizeof(none));
+  }
+}
+
+float material_alpha(const Material &m, const float base[4])
+{
+  if (m.type == BNX_MATERIAL_PBR) return f16_to_f32(f32_to_f16(((1.f - m.transmission) * base[3]) * m.opacity));
+  return base[3] * m.opacity;
+}
+
+Corners tri_corners(const int32_t *ix, int prim, float u, float v)
+{
+  Corners c;
+  c.n = 3;
+  for (int k = 0; k < 3; k++) {
+    c.vi[k] = ix[k];
+    c.fi[k] = 3 * prim + k;
+  }
+  c.w[0] = (1.f - u) - v;
+  c.w[1] = u;
+  c.w[2] = v;
+  return c;
+}
+
+// ======================================================================
 // traversal
 // ======================================================================
 namespace {
@@ -495,8 +592,38 @@ inline bool cut_rejects(const CutRay &c, float t)
   return dot(load3(c.plane), P) + c.plane[3] < 0.f;
 }
 
+// whether a triangle candidate survives its opacity draw (Triangles.dev.cu:
+// 112-133: osP from the vertices and barycentrics, the material's alpha from
+// its hit attributes; the draw from a copy of the ray's state)
+bool opacity_keeps(const Scene &s, const Geom &geom, int inst, int prim, float u, float v, uint64_t seed)
+{
+  const int32_t *ix = &geom.indices[3 * prim];
+  const vec3 v0 = geom.vertices[ix[0]], v1 = geom.vertices[ix[1]], v2 = geom.vertices[ix[2]];
+  const vec3 osP = (((1.f - u) - v) * v0 + u * v1) + v * v2;
+  const Instance &in = s.insts[inst];
+  const vec3 P = in.o2w.identity ? osP : xfm_point(in.o2w, osP);
+  const float color0[4] = {0.f, 0.f, 0.f, 1.f};
+  float base[4];
+  material_color(s, geom, inst, prim, tri_corners(ix, prim, u, v), color0, P, osP, base);
+  const float alpha = material_alpha(geom.mat, base);
+  if (alpha < 1.f) {
+    RNGSeed r;
+    r.state = seed;
+    uint32_t b;
+    memcpy(&b, &osP.x, 4);
+    r.next_h(b);
+    memcpy(&b, &osP.y, 4);
+    r.next_h(b);
+    memcpy(&b, &osP.z, 4);
+    r.next_h(b);
+    Random2 rng(r, 290374u);
+    if (rng() > alpha) return false;
+  }
+  return true;
+}
+
 bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, float tmin, Hit &best, int inst,
-               TraceCounters *ctr, const CutRay &cut)
+               TraceCounters *ctr, const CutRay &cut, const uint64_t *seed, Hit *rej)
 {
   int pidx = g.bvh.order[slot];
   int gi = g.prim_geom[pidx];
@@ -510,6 +637,12 @@ bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, flo
                   pid))
       return false;
     if (cut_rejects(cut, h.t)) return false;
+    if (seed && !opacity_keeps(s, geom, inst, pid, h.u, h.v, *seed)) {
+      if (rej && (h.t < rej->t || (h.t == rej->t && (rej->prim < 0 || id_less(inst, gi, pid, rej->inst, rej->geom,
+                                                                                rej->prim)))))
+        *rej = h;
+      return false;
+    }
     best = h;
     return true;
   } else if (geom.type == 1) {
@@ -548,8 +681,9 @@ bool cylinder_intersect(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radi
 }
 
 Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool any_hit, TraceCounters *ctr,
-              const float *cut_plane)
+              const float *cut_plane, const uint64_t *seed, Hit *rej)
 {
+  if (rej) *rej = Hit{INFINITY, 0.f, 0.f, -1, -1, -1};
   vec3 dir = fix_dir(dir0);
   const CutRay cut{cut_plane, org, dir};
   Hit best;
@@ -615,15 +749,17 @@ Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool 
     }
     const Group &g = s.groups[s.insts[inst].group];
     for (int k = en.ref; k < en.ref + en.cnt; k++) {
-      bool hit = prim_test(s, g, k, o, d, tmin, best, inst, ctr, cut);
+      bool hit = prim_test(s, g, k, o, d, tmin, best, inst, ctr, cut, seed, rej);
       if (hit && any_hit) return best;
     }
   }
   return best;
 }
 
-Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, const float *cut_plane)
+Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, const float *cut_plane,
+                const uint64_t *seed, Hit *rej)
 {
+  if (rej) *rej = Hit{INFINITY, 0.f, 0.f, -1, -1, -1};
   vec3 dir = fix_dir(dir0);
   const CutRay cut{cut_plane, org, dir};
   Hit best;
@@ -637,7 +773,8 @@ Hit trace_brute(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, con
       d = fix_dir(xfm_vector(in.w2o, dir));
     }
     const Group &g = s.groups[in.group];
-    for (int k = 0; k < (int)g.bvh.order.size(); k++) prim_test(s, g, k, o, d, tmin, best, inst, nullptr, cut);
+    for (int k = 0; k < (int)g.bvh.order.size(); k++)
+      prim_test(s, g, k, o, d, tmin, best, inst, nullptr, cut, seed, rej);
   }
   return best;
 }

@@ -171,7 +171,7 @@ void build_bvh(const std::vector<Box> &boxes, int max_leaf, Bvh &out);
 // --------------------------------------------------------------- scene
 struct Material {
   int type = BNX_MATERIAL_MATTE;
-  bool color_from_attribute = false;
+  int color_from_attribute = BNX_ATTRIB_NONE;   // the hit attribute the color comes from
   float color[4] = {0.8f, 0.8f, 0.8f, 1.f};
   float opacity = 1.f;
   float metallic = 1.f, roughness = 1.f, transmission = 0.f, ior = 1.45f;   // AnariPBR
@@ -190,6 +190,16 @@ void pbr_eval(const PbrParams &p, vec3 Ng, vec3 wo, vec3 wi, vec3 &value, float 
 float pbr_pdf(const PbrParams &p, vec3 Ng, vec3 wo, vec3 wi);
 bool pbr_scatter(const PbrParams &p, vec3 Ng, vec3 wo, Random2 &rng, vec3 &dir, float &pdf, vec3 &f_r);
 
+// GeometryAttribute (render/GeometryAttributes.h:22-38) as resolved for the
+// device: a scope (BNX_SCOPE_*) and a constant or an array of comps floats
+// per element
+struct Attribute {
+  int scope = BNX_SCOPE_NONE;
+  int comps = 0;
+  float value[4] = {0.f, 0.f, 0.f, 1.f};
+  std::vector<float> data;
+};
+
 struct Geom {
   int type = 0;                     // 0 triangles, 1 spheres, 2 cylinders
   int user_id = 0;
@@ -199,7 +209,9 @@ struct Geom {
   std::vector<vec3> centers;
   std::vector<float> radii;
   float default_radius = 0.1f;
-  std::vector<vec3> colors;         // per prim, may be empty
+  std::vector<vec3> colors;         // spheres' own colors (Spheres::DD::colors), may be empty
+  std::vector<vec3> normals;        // triangles' vertex normals (Triangles::DD::normals), may be empty
+  Attribute attr[6];                // attribute0..3, color, normal
   int num_prims() const
   {
     return type == 0 ? (int)indices.size() / 3 : (type == 2 ? (int)indices.size() / 2 : (int)centers.size());
@@ -329,7 +341,27 @@ struct Scene {
   std::vector<bnx_dir_light> dir_lights;
   std::vector<Volume> volumes;      // marched after the surface trace, in order
   std::vector<IsoSurface> isos;     // tested after the surface trace, before the volumes
+  // World::DD::instanceAttributes (attribute0..3, color): rgba per instance, or empty
+  std::vector<float> inst_attr[5];
 };
+
+// Geometry::DD::setHitAttributes (geometry/Geometry.h:94-139) for attribute
+// slot k at a hit: elements vi (per vertex) / fi (face-varying) weighted by w
+// (n of them; n == 1: a sphere reads its own element for both)
+struct Corners {
+  int n = 1;
+  int vi[3] = {0, 0, 0}, fi[3] = {0, 0, 0};
+  float w[3] = {1.f, 0.f, 0.f};
+};
+void hit_attribute(const Scene &s, const Geom &g, int k, int inst, int prim, const Corners &c, const float dflt[4],
+                   float out[4]);
+// PossiblyMappedParameter::DD::eval (material/HostMaterial.h:100-114) of the
+// material color; color0: the hit's color before the attribute system
+void material_color(const Scene &s, const Geom &g, int inst, int prim, const Corners &c, const float color0[4],
+                    vec3 P, vec3 osP, float out[4]);
+// the BSDF's opacity (Lambertian / NVisii alpha, AnariMatte.h:64, AnariPBR.h:87-90)
+float material_alpha(const Material &m, const float base[4]);
+Corners tri_corners(const int32_t *ix, int prim, float u, float v);
 
 void build_scene(Scene &s);        // BLAS per group + TLAS
 
@@ -358,9 +390,17 @@ struct Hit {
 // closest hit (any_hit=false) or occlusion (any_hit=true); with cut_plane
 // (nx, ny, nz, w; off when w <= -1e28) surface hits on its negative side are
 // rejected (OptixGlobals::hitOnInvisibleSide, native/render/OptixGlobals.h:36-45)
+//
+// Stochastic opacity (Triangles.dev.cu:120-133): with seed (the ray's RNG
+// state) a triangle candidate whose material alpha is below 1 is kept with
+// probability alpha, its draw taken from a copy of the seed advanced by the
+// bits of its object-space position (so that the kept set does not depend on
+// traversal order); seed null: every surface is opaque.  rej (when given)
+// receives the closest candidate the opacity test rejected (prim -1: none).
 Hit trace_bvh(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, bool any_hit, TraceCounters *ctr,
-              const float *cut_plane = nullptr);
-Hit trace_brute(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, const float *cut_plane = nullptr);
+              const float *cut_plane = nullptr, const uint64_t *seed = nullptr, Hit *rej = nullptr);
+Hit trace_brute(const Scene &s, vec3 org, vec3 dir, float tmin, float tmax, const float *cut_plane = nullptr,
+                const uint64_t *seed = nullptr, Hit *rej = nullptr);
 
 // ------------------------------------------------------------- render
 struct Frame {

@@ -328,19 +328,29 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
   const Instance &in = s.insts[h.inst];
   const Group &g = s.groups[in.group];
   const Geom &geom = s.geoms[g.geoms[h.geom]];
-  vec3 P, Nw;
-  float color[4] = {0.f, 0.f, 0.f, 1.f};         // HitAttributes default (render/HitAttributes.h:49-58)
-  bool have_attr = !geom.colors.empty();
-  if (have_attr) {
-    vec3 c = geom.colors[h.prim];
-    color[0] = c.x; color[1] = c.y; color[2] = c.z;
-  }
+  vec3 P, Nw, osP;
+  float color0[4] = {0.f, 0.f, 0.f, 1.f};        // HitAttributes default (render/HitAttributes.h:49-58)
+  Corners cn;
   if (geom.type == 0) {
     const int32_t *ix = &geom.indices[3 * h.prim];
     vec3 v0 = geom.vertices[ix[0]], v1 = geom.vertices[ix[1]], v2 = geom.vertices[ix[2]];
     vec3 n = normalize(cross(v1 - v0, v2 - v0));
+    if (!geom.normals.empty()) {
+      // interpolated vertex normals (Triangles.dev.cu:68-75)
+      vec3 Ns = (((1.f - h.u) - h.v) * geom.normals[ix[0]] + h.u * geom.normals[ix[1]]) + h.v * geom.normals[ix[2]];
+      n = normalize(Ns);
+    }
     vec3 osN = normalize(n);
-    vec3 osP = (((1.f - h.u) - h.v) * v0 + h.u * v1) + h.v * v2;
+    osP = (((1.f - h.u) - h.v) * v0 + h.u * v1) + h.v * v2;
+    cn = tri_corners(ix, h.prim, h.u, h.v);
+    // the 'normal' attribute replaces the object normal (Geometry.h:134),
+    // transformed to world space unnormalized (Triangles.dev.cu:112-114)
+    if (geom.attr[5].scope != BNX_SCOPE_NONE) {
+      const float d[4] = {osN.x, osN.y, osN.z, 0.f};
+      float an[4];
+      hit_attribute(s, geom, 5, h.inst, h.prim, cn, d, an);
+      osN = vec3(an[0], an[1], an[2]);
+    }
     if (in.o2w.identity) {
       P = osP;
       Nw = osN;
@@ -355,7 +365,9 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
   } else if (geom.type == 2) {
     // CylindersPrograms::intersect's hit attributes (native/geometry/
     // Cylinders.dev.cu:205-236): object normal of the hit, the point moved
-    // 1e-4 along it, world normal = normalize(transformNormal(normalize(N)))
+    // 1e-4 along it, world normal = normalize(transformNormal(normalize(N)));
+    // attributes interpolated between the ends by the clamped position of
+    // the point along the axis (:208-222)
     vec3 o = r.org, d = fix_dir(r.dir);
     if (!in.w2o.identity) {
       o = xfm_point(in.w2o, r.org);
@@ -369,6 +381,16 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
     vec3 objP = o + h.u * d;
     objP = objP + 1e-4f * normalize(objN);
     const vec3 on = normalize(objN);
+    osP = objP;
+    float lerp_t = dot(objP - v0, v1 - v0) / dot(v1 - v0, v1 - v0);
+    lerp_t = fmx(0.f, fmn(1.f, lerp_t));
+    cn.n = 2;
+    cn.vi[0] = geom.indices[2 * h.prim];
+    cn.vi[1] = geom.indices[2 * h.prim + 1];
+    cn.fi[0] = 2 * h.prim;
+    cn.fi[1] = 2 * h.prim + 1;
+    cn.w[0] = 1.f - lerp_t;
+    cn.w[1] = lerp_t;
     if (in.o2w.identity) {
       P = objP;
       Nw = normalize(on);
@@ -394,11 +416,21 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
     vec3 objN = (objP == ctr) ? vec3(1.f, 0.f, 0.f) : (objP - ctr);
     vec3 wN = in.o2w.identity ? objN : xfm_vector(in.o2w, objN);
     Nw = normalize(wN);
+    // the object position moved off the sphere along the unnormalized
+    // offset (Spheres.dev.cu:73-78)
+    float eps = fmx(1e-6f, 1e-6f * fabsf(rad));
+    eps = fmx(eps, 1e-6f * fmx(fmx(fabsf(objP.x), fabsf(objP.y)), fabsf(objP.z)));
+    osP = ctr + (rad + eps) * objN;
+    if (!geom.colors.empty()) {
+      vec3 c = geom.colors[h.prim];
+      color0[0] = c.x; color0[1] = c.y; color0[2] = c.z;
+    }
+    cn.n = 1;
+    cn.vi[0] = cn.fi[0] = h.prim;
   }
   const Material &m = geom.mat;
   float base[4];
-  if (m.color_from_attribute) memcpy(base, color, sizeof(base));
-  else memcpy(base, m.color, sizeof(base));
+  material_color(s, geom, h.inst, h.prim, cn, color0, P, osP, base);
   r.had_hit = true;
   r.tmax = h.t;
   r.P = P;
@@ -611,7 +643,9 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
         PathRay &r = queue[i];
         TraceCounters c;
         const bool occlusion = r.is_shadow && stats && stats->shadow_any_hit;
-        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, occlusion, stats ? &c : nullptr, rr.cut_plane);
+        Hit rej;
+        Hit h = trace_bvh(s, r.org, r.dir, 0.f, r.tmax, occlusion, stats ? &c : nullptr, rr.cut_plane, &r.rng.state,
+                          &rej);
         if (gen == 0) ids[i] = {BNX_INF, -1, -1, -1};
         // isosurfaces, then volumes, after the surface trace, each clipped
         // to the closest hit so far
@@ -651,6 +685,14 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
         if (vk >= 0) apply_volume_hit(s, r, vk, vt, vrgb, gen == 0 ? &ids[i] : nullptr);
         else if (ik >= 0) apply_iso_hit(s, r, ik, it, iN, gen == 0 ? &ids[i] : nullptr);
         else apply_hit(s, r, h, gen == 0 ? &ids[i] : nullptr, compat13);
+        if (gen == 0 && rej.prim >= 0 && rej.t < h.t) {
+          // the ID channels report the closest candidate whatever its
+          // opacity (Triangles.dev.cu:45-60)
+          const Instance &in = s.insts[rej.inst];
+          ids[i].prim = rej.prim;
+          ids[i].inst = in.user_id;
+          ids[i].obj = s.geoms[s.groups[in.group].geoms[rej.geom]].user_id;
+        }
         if (stats) nvs += vs;
         if (stats) {
           nn += c.nodes;

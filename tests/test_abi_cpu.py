@@ -58,7 +58,7 @@ def test_struct_layouts_match_header():
     assert C.sizeof(abi.TrianglesDesc) == 88
     assert C.sizeof(abi.SpheresDesc) == 88
     assert C.sizeof(abi.Renderer) == 48
-    assert C.sizeof(abi.RenderStats) == 128
+    assert C.sizeof(abi.RenderStats) == 152
     assert abi.HIT_DTYPE.itemsize == 32 and abi.NODE_DTYPE.itemsize == 64
 
 

@@ -39,11 +39,13 @@ def worlds(dev):
             "c3": (bx.SurfaceWorld(dev, c3), c3)}
 
 
-def _render(dev, gw, sc, w, h, spp):
+def _render(dev, gw, sc, w, h, spp, stats=None):
     cam = bx.camera_perspective(sc.camera["position"], sc.camera["direction"], sc.camera["up"], sc.camera["fovy"])
     rend = abi.renderer(bg=sc.bg_color, ambient=sc.ambient, max_generations=getattr(sc, "max_generations", 0))
     fr = bx.Frame(dev, w, h)
-    bx.render(gw.world, cam, rend, fr, spp)
+    st = bx.render(gw.world, cam, rend, fr, spp)
+    if stats is not None:
+        stats.append(st)
     out = {ch: fr.read(ch) for ch in CHANNELS}
     fr.close()
     return out
@@ -158,9 +160,13 @@ def test_captured_bounce_loop_replays(dev, worlds, keep_tuning, name):
     ref = _render(dev, gw, sc, w, h, spp)
     other = _render(dev, gw, sc, 29, 31, 2)
     dev.set_tuning(graph_paths=1 << 26)
+    stats = []
     for k in range(4):
-        _same(_render(dev, gw, sc, w, h, spp), ref, (name, "replay", k))
+        _same(_render(dev, gw, sc, w, h, spp, stats), ref, (name, "replay", k))
         _same(_render(dev, gw, sc, 29, 31, 2), other, (name, "other", k))
+    # the captured loop ran (a failed capture is reported, not silent)
+    assert all(st["capture_error"] == 0 for st in stats), stats[-1]["capture_error"]
+    assert sum(st["captured_batches"] for st in stats) >= 3 and stats[-1]["uncaptured_batches"] == 0
 
 
 def test_captured_loop_with_too_few_generations(dev, worlds, keep_tuning):
