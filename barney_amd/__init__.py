@@ -55,6 +55,8 @@ _PROTOS = [
     ("bnx_geom_create_triangles", [vp, C.POINTER(abi.TrianglesDesc), C.POINTER(vp)], C.c_int),
     ("bnx_geom_create_spheres", [vp, C.POINTER(abi.SpheresDesc), C.POINTER(vp)], C.c_int),
     ("bnx_geom_create_cylinders", [vp, C.POINTER(abi.CylindersDesc), C.POINTER(vp)], C.c_int),
+    ("bnx_geom_create_capsules", [vp, C.POINTER(abi.CapsulesDesc), C.POINTER(vp)], C.c_int),
+    ("bnx_geom_create_cones", [vp, C.POINTER(abi.ConesDesc), C.POINTER(vp)], C.c_int),
     ("bnx_geom_destroy", [vp], C.c_int),
     ("bnx_group_create_triangles", [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
     ("bnx_group_create_user", [vp, C.POINTER(vp), C.c_int, C.POINTER(vp)], C.c_int),
@@ -253,6 +255,34 @@ class Geom:
                               user_id, material or abi.matte())
         h = vp()
         check(dev.L.bnx_geom_create_cylinders(dev.h, C.byref(d), C.byref(h)))
+        return Geom(dev, h, None)
+
+    @staticmethod
+    def capsules(dev: Device, vertices, indices, colors=None, user_id=1, material=None) -> "Geom":
+        """vertices (nv, 4): position and radius per vertex; indices (n, 2)"""
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        d = abi.CapsulesDesc(abi.fptr(v), abi.iptr(i), abi.fptr(c), len(v), len(i), user_id, 0,
+                             material or abi.matte())
+        h = vp()
+        check(dev.L.bnx_geom_create_capsules(dev.h, C.byref(d), C.byref(h)))
+        return Geom(dev, h, None)
+
+    @staticmethod
+    def cones(dev: Device, vertices, radii, indices=None, colors=None, user_id=1, material=None,
+              num_cones=None) -> "Geom":
+        """vertices (nv, 3), radii (nv,); indices (n, 2) or None (cone i
+        joins vertices 2i and 2i+1)"""
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        r = np.ascontiguousarray(radii, dtype=np.float32)
+        i = None if indices is None else np.ascontiguousarray(indices, dtype=np.int32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        n = len(i) if i is not None else (len(v) // 2 if num_cones is None else num_cones)
+        d = abi.ConesDesc(abi.fptr(v), abi.fptr(r), abi.iptr(i), abi.fptr(c), len(v), n, user_id, 0,
+                          material or abi.matte())
+        h = vp()
+        check(dev.L.bnx_geom_create_cones(dev.h, C.byref(d), C.byref(h)))
         return Geom(dev, h, None)
 
     def set_attribute(self, slot: int, scope: int, data=None, value=None) -> "Geom":
@@ -605,8 +635,9 @@ class Server:
 class SurfaceWorld:
     """GPU world for a barney_amd.scenes.SurfaceScene, flattened like
     native/ModelSlot.cpp:128-155: user-geometry group (spheres) first, then
-    the triangles group, all as instances of user instance 0.  Cylinders
-    get a user group of their own, after the spheres'."""
+    the triangles group, all as instances of user instance 0.  Cylinders,
+    capsules and cones get a user group each, after the spheres', in that
+    order."""
 
     def __init__(self, dev: Device, sc):
         self.dev = dev
@@ -614,6 +645,10 @@ class SurfaceWorld:
                                  material=s.material).set_attributes(s) for s in sc.spheres]
         self.cyl = [Geom.cylinders(dev, c.vertices, c.indices, c.radii, c.colors, user_id=c.user_id,
                                    material=c.material).set_attributes(c) for c in getattr(sc, "cylinders", [])]
+        self.cap = [Geom.capsules(dev, c.vertices, c.indices, c.colors, user_id=c.user_id,
+                                  material=c.material).set_attributes(c) for c in getattr(sc, "capsules", [])]
+        self.cone = [Geom.cones(dev, c.vertices, c.radii, c.indices, c.colors, user_id=c.user_id,
+                                material=c.material).set_attributes(c) for c in getattr(sc, "cones", [])]
         self.tri = [Geom.triangles(dev, m.vertices, m.indices, m.colors, user_id=m.user_id,
                                    material=m.material).set_attributes(m) for m in sc.meshes]
         groups = []
@@ -621,6 +656,10 @@ class SurfaceWorld:
             groups.append(Group.geoms(dev, USER, self.sph).build())
         if self.cyl:
             groups.append(Group.geoms(dev, USER, self.cyl).build())
+        if self.cap:
+            groups.append(Group.geoms(dev, USER, self.cap).build())
+        if self.cone:
+            groups.append(Group.geoms(dev, USER, self.cone).build())
         if self.tri:
             groups.append(Group.geoms(dev, TRIANGLES, self.tri).build())
         self.groups = groups

@@ -11,6 +11,7 @@ BNX_MATERIAL_MATTE = 0
 BNX_MATERIAL_PBR = 1
 BNX_CAMERA_PERSPECTIVE = 1
 BNX_RENDER_COMPAT_SPHERE_INSTID13 = 1
+BNX_RENDER_RATIO_TRACKING = 2
 BNX_FB_COLOR, BNX_FB_DEPTH, BNX_FB_PRIMID, BNX_FB_INSTID, BNX_FB_OBJID, BNX_FB_NORMAL = range(6)
 BNX_FB_COLOR_RGBA8, BNX_FB_COLOR_SRGB8 = 6, 7
 BNX_CAMERA_PERSPECTIVE, BNX_CAMERA_ORTHOGRAPHIC, BNX_CAMERA_OMNI = 1, 2, 3
@@ -45,6 +46,16 @@ class CylindersDesc(C.Structure):
     _fields_ = [("vertices", fp), ("indices", ip), ("radii", fp), ("colors", fp), ("num_vertices", C.c_int32),
                 ("num_cylinders", C.c_int32), ("default_radius", C.c_float), ("user_id", C.c_int32),
                 ("material", Material)]
+
+
+class CapsulesDesc(C.Structure):
+    _fields_ = [("vertices", fp), ("indices", ip), ("colors", fp), ("num_vertices", C.c_int32),
+                ("num_capsules", C.c_int32), ("user_id", C.c_int32), ("_pad", C.c_int32), ("material", Material)]
+
+
+class ConesDesc(C.Structure):
+    _fields_ = [("vertices", fp), ("radii", fp), ("indices", ip), ("colors", fp), ("num_vertices", C.c_int32),
+                ("num_cones", C.c_int32), ("user_id", C.c_int32), ("_pad", C.c_int32), ("material", Material)]
 
 
 class Camera(C.Structure):

@@ -292,12 +292,14 @@ struct Geom : Object {
     h_users = 0;
   }
   void commit() override;
-  int backend_kind() const   // 0 triangles, 1 spheres, 2 cylinders, 3 iso_surface
+  int backend_kind() const   // 0 triangles, 1 spheres, 2 cylinders, 3 capsules, 4 cones, 5 iso_surface
   {
     if (type == "triangles") return 0;
     if (type == "spheres") return 1;
     if (type == "cylinders") return 2;
-    return 3;
+    if (type == "capsules") return 3;
+    if (type == "cones") return 4;
+    return 5;
   }
 };
 
@@ -374,6 +376,37 @@ void Geom::commit()
     d.user_id = user_id;
     d.material = mat;
     check(bnx_geom_create_cylinders(dev, &d, &g), "cylinders");
+  } else if (type == "capsules") {
+    // Capsules::setData (native/geometry/Capsules.cpp:49-65): 'vertices'
+    // (vec4f: position, radius), 'indices' (vec2i)
+    const Data *v = data_param(this, "vertices"), *ix = data_param(this, "indices");
+    if (!v || !ix) return warn("capsules: 'vertices' and 'indices' are required");
+    if (v->dtype != BN_FLOAT32_VEC4) return warn("capsules: 'vertices' must be float4 (position, radius)");
+    bnx_capsules_desc d{};
+    d.vertices = v->fp();
+    d.num_vertices = (int)v->count;
+    d.indices = ix->ip();
+    d.num_capsules = (int)(ix->dtype == BN_INT32_VEC2 ? ix->count : ix->count / 2);
+    d.colors = colors ? col3.data() : nullptr;
+    d.user_id = user_id;
+    d.material = mat;
+    check(bnx_geom_create_capsules(dev, &d, &g), "capsules");
+  } else if (type == "cones") {
+    // Cones::setData / commit (native/geometry/Cones.cpp:18-73): 'vertices'
+    // (vec3f), 'radii' (per vertex), optional 'indices' (vec2i; without
+    // them cone i joins vertices 2i and 2i + 1)
+    const Data *v = data_param(this, "vertices"), *ix = data_param(this, "indices"), *r = data_param(this, "radii");
+    if (!v || !r) return warn("cones: 'vertices' and 'radii' are required");
+    bnx_cones_desc d{};
+    d.vertices = v->fp();
+    d.num_vertices = (int)v->count;
+    d.radii = r->fp();
+    d.indices = ix ? ix->ip() : nullptr;
+    d.num_cones = ix ? (int)(ix->dtype == BN_INT32_VEC2 ? ix->count : ix->count / 2) : (int)(v->count / 2);
+    d.colors = colors ? col3.data() : nullptr;
+    d.user_id = user_id;
+    d.material = mat;
+    check(bnx_geom_create_cones(dev, &d, &g), "cones");
   } else if (type == "iso_surface") {
     // IsoSurface (geometry/IsoSurface.cpp:44-90): 'isoValue', 'scalarField'
     const Param *p = get("scalarField", Param::OBJECT);
@@ -532,8 +565,9 @@ struct Group : Object {
   std::vector<Ref<Volume>> volumes;
   std::vector<Ref<Light>> lights;
   // built: one backend group per geometry kind (spheres, cylinders,
-  // triangles), with the geometries they were built over
-  bnx_group g[3] = {nullptr, nullptr, nullptr};
+  // capsules, cones, triangles), with the geometries they were built over
+  static constexpr int KINDS = 5;
+  bnx_group g[KINDS] = {nullptr, nullptr, nullptr, nullptr, nullptr};
   std::vector<std::pair<Geom *, bnx_geom>> held;
   std::vector<Ref<Geom>> held_refs;
   std::vector<Ref<Geom>> isos;
@@ -562,24 +596,25 @@ struct Group : Object {
   void build()
   {
     free_built();
-    // spheres, cylinders (user geometries, one type per backend group), triangles
-    std::vector<bnx_geom> per[3];
+    // spheres, cylinders, capsules, cones (user geometries, one type per
+    // backend group), triangles
+    std::vector<bnx_geom> per[KINDS];
     for (Ref<Geom> &ge : geoms) {
       if (!ge || !ge->valid) continue;
       const int k = ge->backend_kind();
-      if (k == 3) {
+      if (k == 5) {
         isos.push_back(ge);
         continue;
       }
-      const int slot = k == 0 ? 2 : k - 1;
+      const int slot = k == 0 ? KINDS - 1 : k - 1;
       bnx_geom h = ge->use();
       per[slot].push_back(h);
       held.push_back({ge.get(), h});
       held_refs.push_back(ge);
     }
-    for (int s = 0; s < 3; s++) {
+    for (int s = 0; s < KINDS; s++) {
       if (per[s].empty()) continue;
-      const int rc = s == 2 ? bnx_group_create_triangles(ctx->dev, per[s].data(), (int)per[s].size(), &g[s])
+      const int rc = s == KINDS - 1 ? bnx_group_create_triangles(ctx->dev, per[s].data(), (int)per[s].size(), &g[s])
                             : bnx_group_create_user(ctx->dev, per[s].data(), (int)per[s].size(), &g[s]);
       check(rc, "bnGroupBuild");
       check(bnx_group_build(g[s]), "bnGroupBuild");
@@ -622,13 +657,13 @@ struct Model : Object {
     return std::equal(a.begin(), a.end(), I);
   }
   // flattened instance list (ModelSlot::flattenInstancesForDevice): per
-  // instance its spheres, cylinders and triangles groups
+  // instance its spheres, cylinders, capsules, cones and triangles groups
   void flatten(std::vector<bnx_group> *gs, std::vector<int32_t> *ids, std::vector<float> &xf) const
   {
     for (size_t i = 0; i < groups.size(); i++) {
       const Group *g = groups[i].get();
       if (!g) continue;
-      for (int s = 0; s < 3; s++)
+      for (int s = 0; s < Group::KINDS; s++)
         if (g->g[s]) {
           if (gs) gs->push_back(g->g[s]);
           if (ids) ids->push_back((int32_t)i);
@@ -1048,7 +1083,8 @@ BNGeom bnGeometryCreate(BNContext context, int, const char *type)
 {
   BNB_TRY
   const std::string t = type ? type : "";
-  if (t != "triangles" && t != "spheres" && t != "cylinders" && t != "iso_surface")
+  if (t != "triangles" && t != "spheres" && t != "cylinders" && t != "capsules" && t != "cones" &&
+      t != "iso_surface")
     warn("bnGeometryCreate: geometry type '" + t + "' is not provided by this backend");
   return handle<BNGeom>(make<bnb::Geom>(ctx_of(context), "geometry", t));
   BNB_CATCH("bnGeometryCreate", nullptr)

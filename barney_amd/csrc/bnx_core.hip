@@ -28,7 +28,7 @@ struct GeomBuild {
   const float *vertices;
   const int *indices;
   const float4 *spheres;
-  const float4 *cyls;   // cylinders: (v0, radius), (v1, 0) per cylinder
+  const float4 *cyls;   // two records per prim: cylinders (v0, radius), (v1, 0); capsules, cones (v0, r0), (v1, r1)
 };
 
 __global__ void k_prim_boxes(const GeomBuild *gb, int num_geoms, int n, float4 *lo, float4 *hi, int *pgeom)
@@ -58,13 +58,23 @@ __global__ void k_prim_boxes(const GeomBuild *gb, int num_geoms, int n, float4 *
     float4 s = G.spheres[prim];
     lo[p] = make_float4(s.x - s.w, s.y - s.w, s.z - s.w, 0.f);
     hi[p] = make_float4(s.x + s.w, s.y + s.w, s.z + s.w, 0.f);
-  } else {
+  } else if (G.type == 2) {
     // CylindersPrograms::bounds (native/geometry/Cylinders.dev.cu:69-85):
     // the boxes of both end spheres
     const float4 a = G.cyls[2 * prim], b = G.cyls[2 * prim + 1];
     const float r = a.w;
     lo[p] = make_float4(fmn(a.x - r, b.x - r), fmn(a.y - r, b.y - r), fmn(a.z - r, b.z - r), 0.f);
     hi[p] = make_float4(fmx(a.x + r, b.x + r), fmx(a.y + r, b.y + r), fmx(a.z + r, b.z + r), 0.f);
+  } else {
+    // capsules and cones: the boxes of both end spheres (getBounds,
+    // Capsules.dev.cu:100-110).  ConesPrograms::bounds (Cones.dev.cu:21-42)
+    // takes the second end's lower corner from the first radius (pb - ra),
+    // a box that can miss part of the cone; a hit outside its box is found
+    // or not depending on traversal order, so cones take the box that holds
+    // them
+    const float4 a = G.cyls[2 * prim], b = G.cyls[2 * prim + 1];
+    lo[p] = make_float4(fmn(a.x - a.w, b.x - b.w), fmn(a.y - a.w, b.y - b.w), fmn(a.z - a.w, b.z - b.w), 0.f);
+    hi[p] = make_float4(fmx(a.x + a.w, b.x + b.w), fmx(a.y + a.w, b.y + b.w), fmx(a.z + a.w, b.z + b.w), 0.f);
   }
 }
 
@@ -136,10 +146,10 @@ int build_blas(Group *g)
                                                              hi.as<float4>(), pgeom.as<int>());
   if ((rc = build_bvh(dev, lo.as<float4>(), hi.as<float4>(), n, 4, g->bvh)) != BNX_OK) return rc;
   if ((rc = collapse_bvh4(dev, g->bvh)) != BNX_OK) return rc;
-  // record float4s per primitive: triangles 3, spheres 1, cylinders 2 (a
-  // user group holds one user type)
+  // record float4s per primitive: triangles 3, spheres 1, cylinders,
+  // capsules and cones 2 (a user group holds one user type)
   const int utype = g->geoms.empty() ? 1 : g->geoms[0]->type;
-  size_t per = g->kind == GROUP_TRIANGLES ? 3 : (utype == 2 ? 2 : 1);
+  size_t per = g->kind == GROUP_TRIANGLES ? 3 : (utype >= 2 ? 2 : 1);
   BNX_HIP(g->prims.ensure(sizeof(float4) * per * (size_t)(n ? n : 1)));
   BNX_HIP(g->refs.ensure(sizeof(int2) * (size_t)(n ? n : 1)));
   if (n)
@@ -255,7 +265,9 @@ int build_tlas(Group *g)
         gd.ior = gm->mat.ior;
         gd.opacity = gm->mat.opacity;
         gd.transmission = gm->mat.transmission;
-        gd.translucent = gm->type == 0 && can_be_translucent(*gm, g) ? 1 : 0;
+        // stochastic opacity: triangles and capsules (spheres, cylinders
+        // and cones have no opacity test in the reference)
+        gd.translucent = (gm->type == 0 || gm->type == 3) && can_be_translucent(*gm, g) ? 1 : 0;
         BNX_SET(gd.vertices, gm->vertices.as<float>());
         BNX_SET(gd.indices, gm->indices.as<int>());
         BNX_SET(gd.colors, gm->colors.ptr ? gm->colors.as<float>() : nullptr);
@@ -310,15 +322,15 @@ int build_tlas(Group *g)
     BNX_SET(blas[k].nodes, bg->bvh.nodes4.as<Node4>());
     BNX_SET(blas[k].prims, bg->prims.as<float4>());
     BNX_SET(blas[k].refs, bg->refs.as<int2>());
-    blas[k].type = bg->kind == GROUP_TRIANGLES
-                       ? BLAS_TRIANGLES
-                       : ((!bg->geoms.empty() && bg->geoms[0]->type == 2) ? BLAS_CYLINDERS : BLAS_SPHERES);
+    // a user group holds one user type: spheres, cylinders, capsules or cones
+    blas[k].type = bg->kind == GROUP_TRIANGLES ? BLAS_TRIANGLES
+                                               : (bg->geoms.empty() ? BLAS_SPHERES : bg->geoms[0]->type);
     blas[k].num_prims = bg->bvh.num_prims;
     blas[k].translucent = 0;
     for (size_t j = 0; j < bg->geoms.size(); j++) blas[k].translucent |= geoms[geom_base[k] + j].translucent;
   }
   bool has_cyl = false;
-  for (const BlasDev &b : blas) has_cyl |= b.type == BLAS_CYLINDERS;
+  for (const BlasDev &b : blas) has_cyl |= b.type >= BLAS_CYLINDERS;
   if (has_cyl != g->has_cyl) {
     dev->cyl_worlds += has_cyl ? 1 : -1;
     g->has_cyl = has_cyl;
@@ -731,6 +743,86 @@ int bnx_geom_create_cylinders(bnx_device d, const bnx_cylinders_desc *desc, bnx_
   GUARD_END
 }
 
+// a user geometry with two records per primitive (capsules, cones) from
+// its end-point records and index pairs
+static int create_pair_geom(bnx_device d, int type, const std::vector<float4> &rec, const std::vector<int> &idx,
+                            int num_prims, int num_vertices, const float *colors, int32_t user_id,
+                            const bnx_material &material, bnx_geom *out)
+{
+  bnx_geom g = new bnx_geom_s;
+  g->dev = d;
+  g->type = type;
+  g->user_id = user_id;
+  set_mat(g, material);
+  g->num_prims = num_prims;
+  g->num_vertices = num_vertices;
+  int rc;
+  if ((rc = upload(d, g->cyls, rec.data(), rec.size())) != BNX_OK ||
+      (rc = upload(d, g->indices, idx.data(), idx.size())) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  if (colors && (rc = set_prim_colors(g, colors, num_prims)) != BNX_OK) {
+    delete g;
+    return rc;
+  }
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  *out = g;
+  return BNX_OK;
+}
+
+int bnx_geom_create_capsules(bnx_device d, const bnx_capsules_desc *desc, bnx_geom *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && desc && out, BNX_ERR_INVALID, "bnx_geom_create_capsules: null argument");
+  BNX_CHECK(desc->num_capsules >= 0 && desc->num_vertices >= 0, BNX_ERR_INVALID, "negative count");
+  BNX_CHECK(desc->num_capsules == 0 || (desc->vertices && desc->indices), BNX_ERR_INVALID,
+            "bnx_geom_create_capsules: missing vertices or indices");
+  if (int rc = check_material(desc->material, false)) return rc;
+  const int n = desc->num_capsules;
+  std::vector<float4> rec(2 * (size_t)n);
+  std::vector<int> idx(2 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    const int a = desc->indices[2 * i], b = desc->indices[2 * i + 1];
+    BNX_CHECK(a >= 0 && a < desc->num_vertices && b >= 0 && b < desc->num_vertices, BNX_ERR_INVALID,
+              "bnx_geom_create_capsules: index out of range");
+    const float *va = desc->vertices + 4 * (size_t)a, *vb = desc->vertices + 4 * (size_t)b;
+    rec[2 * i] = make_float4(va[0], va[1], va[2], va[3]);
+    rec[2 * i + 1] = make_float4(vb[0], vb[1], vb[2], vb[3]);
+    idx[2 * i] = a;
+    idx[2 * i + 1] = b;
+  }
+  return create_pair_geom(d, 3, rec, idx, n, desc->num_vertices, desc->colors, desc->user_id, desc->material, out);
+  GUARD_END
+}
+
+int bnx_geom_create_cones(bnx_device d, const bnx_cones_desc *desc, bnx_geom *out)
+{
+  GUARD_BEGIN
+  BNX_CHECK(d && desc && out, BNX_ERR_INVALID, "bnx_geom_create_cones: null argument");
+  BNX_CHECK(desc->num_cones >= 0 && desc->num_vertices >= 0, BNX_ERR_INVALID, "negative count");
+  BNX_CHECK(desc->num_cones == 0 || (desc->vertices && desc->radii), BNX_ERR_INVALID,
+            "bnx_geom_create_cones: missing vertices or radii");
+  if (int rc = check_material(desc->material, false)) return rc;
+  const int n = desc->num_cones;
+  std::vector<float4> rec(2 * (size_t)n);
+  std::vector<int> idx(2 * (size_t)n);
+  for (int i = 0; i < n; i++) {
+    // Cones.dev.cu:81-84: without indices cone i joins vertices 2i, 2i + 1
+    const int a = desc->indices ? desc->indices[2 * i] : 2 * i;
+    const int b = desc->indices ? desc->indices[2 * i + 1] : 2 * i + 1;
+    BNX_CHECK(a >= 0 && a < desc->num_vertices && b >= 0 && b < desc->num_vertices, BNX_ERR_INVALID,
+              "bnx_geom_create_cones: index out of range");
+    const float *va = desc->vertices + 3 * (size_t)a, *vb = desc->vertices + 3 * (size_t)b;
+    rec[2 * i] = make_float4(va[0], va[1], va[2], desc->radii[a]);
+    rec[2 * i + 1] = make_float4(vb[0], vb[1], vb[2], desc->radii[b]);
+    idx[2 * i] = a;
+    idx[2 * i + 1] = b;
+  }
+  return create_pair_geom(d, 4, rec, idx, n, desc->num_vertices, desc->colors, desc->user_id, desc->material, out);
+  GUARD_END
+}
+
 int bnx_geom_create_spheres(bnx_device d, const bnx_spheres_desc *desc, bnx_geom *out)
 {
   GUARD_BEGIN
@@ -795,7 +887,7 @@ int bnx_geom_set_attribute(bnx_geom g, int slot, int scope, int comps, const flo
   int64_t need = g->num_prims;
   if (scope == BNX_SCOPE_PER_VERTEX) need = g->type == 1 ? g->num_prims : g->num_vertices;
   if (scope == BNX_SCOPE_FACE_VARYING) need = g->type == 0 ? 3LL * g->num_prims
-                                             : (g->type == 2 ? 2LL * g->num_prims : g->num_prims);
+                                             : (g->type >= 2 ? 2LL * g->num_prims : g->num_prims);
   BNX_CHECK(data && count >= need, BNX_ERR_INVALID,
             "bnx_geom_set_attribute: fewer elements than the scope needs (" + std::to_string(need) + ")");
   int rc;
@@ -851,7 +943,7 @@ static int create_geom_group(bnx_device d, const bnx_geom *geoms, int n, int kin
     }
     if (i > 0 && geoms[i]->type != geoms[0]->type) {
       delete g;
-      set_error("group create: a user-geometry group holds one type (spheres or cylinders)");
+      set_error("group create: a user-geometry group holds one type (spheres, cylinders, capsules or cones)");
       return BNX_ERR_UNSUPPORTED;
     }
     g->geoms.push_back(geoms[i]);

@@ -108,6 +108,16 @@ BNX_D uint32_t rng_next_h(uint64_t &state, uint64_t h)
   state ^= (h << 1);
   return rng_next(state);
 }
+BNX_D uint64_t fnv3(uint32_t a, uint32_t b, uint32_t c)
+{
+  // hash(v0,v1,v2) of native/common/barney-common.h:88-105
+  const uint64_t prime = 0x100000001b3ull;
+  uint64_t h = 0xcbf29ce484222325ull ^ (prime * (uint64_t)a);
+  h = h * prime ^ (uint64_t)b;
+  h = h * prime ^ (uint64_t)c;
+  return h;
+}
+
 struct Random2 {
   uint64_t state;
   BNX_D Random2(uint32_t a, uint32_t b) : state(0xcafef00dd15ea5e5ull) { rng_seed(state, a, b); }
@@ -151,7 +161,9 @@ static_assert(sizeof(Node4) == 128, "node4");
 constexpr uint32_t EMPTY4 = 0x7ffffffeu;
 
 // ----------------------------------------------------------- scene data
-enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1, BLAS_CYLINDERS = 2 };
+// BLAS types; the user types with two float4 records per primitive
+// (cylinders, capsules, cones) share the trace kernels' pair leaf test
+enum { BLAS_TRIANGLES = 0, BLAS_SPHERES = 1, BLAS_CYLINDERS = 2, BLAS_CAPSULES = 3, BLAS_CONES = 4 };
 
 struct BlasDev {
   const Node4 BNX_G *nodes;
@@ -185,7 +197,7 @@ enum { ATTR_SLOTS = 6 };   // attribute0..3, color, normal (BNX_ATTRIB_SLOT_*)
 
 // per-geometry shading data
 struct GeomDev {
-  int type;               // 0 triangles, 1 spheres, 2 cylinders
+  int type;               // 0 triangles, 1 spheres, 2 cylinders, 3 capsules, 4 cones
   int user_id;
   int color_from_attr;    // BNX_ATTRIB_*: the hit attribute the color comes from
   int num_prims;
@@ -195,12 +207,13 @@ struct GeomDev {
   float opacity, transmission;      // 'opacity'; PBR 'transmission'
   int translucent;        // triangles whose alpha can be below 1: stochastic opacity
   int _pad;
-  const float BNX_G *vertices;   // triangles: float3; cylinders: float3
-  const int BNX_G *indices;      // triangles: int3; cylinders: int2
+  const float BNX_G *vertices;   // triangles: float3; cylinders, capsules, cones: float3
+  const int BNX_G *indices;      // triangles: int3; cylinders, capsules, cones: int2
   const float BNX_G *colors;     // spheres' own per-sphere float3 colors, or null
   const float BNX_G *normals;    // triangles: per-vertex float3 normals, or null
   const float4 BNX_G *spheres;   // spheres: (center, radius)
-  const float4 BNX_G *cyls;      // cylinders: (v0, radius), (v1, 0) per cylinder
+  const float4 BNX_G *cyls;      // two records per primitive: cylinders (v0, radius), (v1, 0);
+                                 // capsules and cones (v0, r0), (v1, r1)
   AttrDev attr[ATTR_SLOTS];
 };
 
@@ -299,6 +312,20 @@ BNX_D float material_alpha(const GeomDev &G, float4 base)
   return base.w * G.opacity;
 }
 
+// the two ends of a cylinder, capsule or cone at attribute weight lerp
+BNX_D Corners pair_corners(const GeomDev &G, int prim, float lerp)
+{
+  Corners c;
+  c.n = 2;
+  c.vi[0] = G.indices[2 * prim];
+  c.vi[1] = G.indices[2 * prim + 1];
+  c.fi[0] = 2 * prim;
+  c.fi[1] = 2 * prim + 1;
+  c.w[0] = 1.f - lerp;
+  c.w[1] = lerp;
+  return c;
+}
+
 // a triangle hit's corners for attribute interpolation
 BNX_D Corners tri_corners(const int *ix, int prim, float u, float v)
 {
@@ -386,6 +413,208 @@ BNX_D bool cyl_intersect(v3 ray_org, v3 ray_dir, v3 v0, v3 v1, float radius, flo
     else *N = (td * d - fp) - u * s;
   }
   return true;
+}
+
+// CapsulesPrograms::intersect (native/geometry/Capsules.dev.cu:154-212) with
+// intersectRoundedCone (:19-98): the rounded cone between the end spheres
+// (A.xyz, A.w) and (B.xyz, B.w), in the ray's object frame.  The origin
+// first moves to 0.99 of the ray's entry distance into the box of both end
+// spheres; then the nearest of the body and the two end spheres below the
+// bound tmax is taken (each candidate must lie beyond 1e-6 from the moved
+// origin).  t: the distance from the original origin; *N: the object normal
+// (normalized as the reference normalizes it); *P: the object-space point
+// (moved origin + moved distance, as the reference computes it).  Written
+// operation by operation as the reference evaluates it.
+BNX_D bool capsule_intersect(v3 ro, v3 rd, float4 A, float4 B, float tmax, float &t, v3 *N, v3 *P)
+{
+  const v3 pa = xyz(A), pb = xyz(B);
+  const float ra = A.w, rb = B.w;
+  // getBounds (:100-110) and Ray.h's boxTest from t0 = 0, t1 = tmax
+  const v3 blo{fmn(pa.x - ra, pb.x - rb), fmn(pa.y - ra, pb.y - rb), fmn(pa.z - ra, pb.z - rb)};
+  const v3 bhi{fmx(pa.x + ra, pb.x + rb), fmx(pa.y + ra, pb.y + rb), fmx(pa.z + ra, pb.z + rb)};
+  const v3 rcp{1.f / rd.x, 1.f / rd.y, 1.f / rd.z};
+  const v3 tlo{(blo.x - ro.x) * rcp.x, (blo.y - ro.y) * rcp.y, (blo.z - ro.z) * rcp.z};
+  const v3 thi{(bhi.x - ro.x) * rcp.x, (bhi.y - ro.y) * rcp.y, (bhi.z - ro.z) * rcp.z};
+  const float t0 = fmx(0.f, fmx(fmx(fmn(tlo.x, thi.x), fmn(tlo.y, thi.y)), fmn(tlo.z, thi.z)));
+  const float t1 = fmn(tmax, fmn(fmn(fmx(tlo.x, thi.x), fmx(tlo.y, thi.y)), fmx(tlo.z, thi.z)));
+  if (!(t0 <= t1)) return false;
+  const float t_move = .99f * t0;
+  const v3 o = ro + t_move * rd;
+  const float bound = tmax - t_move;
+  float hit_t = bound;
+  // intersectRoundedCone: no capsule when one end sphere holds the other
+  const float dist_ab = length(pa - pb);
+  if (dist_ab + ra <= rb) return false;
+  if (dist_ab + rb <= ra) return false;
+  const v3 ba = pb - pa, oa = o - pa, ob = o - pb;
+  const float rr = ra - rb;
+  const float m0 = dot(ba, ba), m1 = dot(ba, oa), m2 = dot(ba, rd), m3 = dot(rd, oa);
+  const float m5 = dot(oa, oa), m6 = dot(ob, rd), m7 = dot(ob, ob);
+  const float d2 = m0 - rr * rr;
+  const float k2 = d2 - m2 * m2;
+  if (k2 == 0.f) return false;
+  const float k1 = (d2 * m3 - m1 * m2) + (m2 * rr) * ra;
+  const float k0 = ((d2 * m5 - m1 * m1) + ((m1 * rr) * ra) * 2.f) - (m0 * ra) * ra;
+  const float h = k1 * k1 - k0 * k2;
+  if (h < 0.f) return false;
+  bool had = false;
+  v3 n{0.f, 0.f, 0.f};
+  float tc = (-sqrtf(h) - k1) / k2;
+  const float y = (m1 - ra * rr) + tc * m2;
+  if (y > 0.f && y < d2 && tc > 1e-6f && tc < hit_t) {
+    hit_t = tc;
+    n = d2 * (oa + tc * rd) - ba * y;
+    had = true;
+  }
+  // the end spheres
+  const float h1 = (m3 * m3 - m5) + ra * ra;
+  if (h1 > 0.f) {
+    tc = -m3 - sqrtf(h1);
+    if (tc > 1e-6f && tc < hit_t) {
+      hit_t = tc;
+      const v3 q = (o + tc * rd) - pa;
+      n = v3{q.x / ra, q.y / ra, q.z / ra};
+      had = true;
+    }
+  }
+  const float h2 = (m6 * m6 - m7) + rb * rb;
+  if (h2 > 0.f) {
+    tc = -m6 - sqrtf(h2);
+    if (tc > 1e-6f && tc < hit_t) {
+      hit_t = tc;
+      const v3 q = (o + tc * rd) - pb;
+      n = v3{q.x / rb, q.y / rb, q.z / rb};
+      had = true;
+    }
+  }
+  if (!had) return false;
+  if (hit_t < 0.f || hit_t > bound) return false;
+  if (N) *N = normalize(n);
+  if (P) *P = o + hit_t * rd;
+  t = hit_t + t_move;
+  return true;
+}
+
+// ConesPrograms::intersect (native/geometry/Cones.dev.cu:60-266): the
+// capped cone from (p0, ra) to (p1, rb) in the ray's object frame.  The cap
+// on the side of the origin first (a hit there in (tmin, tmax) is taken
+// without looking at the body), then the body's nearer root.  *N: the
+// unnormalized object normal; *lerp: the attribute weight of the far end
+// (0 at the first cap, 1 at the second, y / m0 clamped on the body).
+BNX_D bool cone_intersect(v3 ro, v3 rd, float4 A, float4 B, float tmin, float tmax, float &t, v3 *N, float *lerp)
+{
+  const v3 p0 = xyz(A), p1 = xyz(B);
+  const float ra = A.w, rb = B.w;
+  const v3 ba = p1 - p0, oa = ro - p0, ob = ro - p1;
+  const float m0 = dot(ba, ba), m1 = dot(oa, ba), m2 = dot(ob, ba), m3 = dot(rd, ba);
+  // inversesqrt of the reference: 1. / sqrtf(f), a double division rounded to float
+  const float isq = (float)(1.0 / (double)sqrtf(m0));
+  if (m1 < 0.f) {
+    const v3 q = oa * m3 - rd * m1;
+    if (dot(q, q) < ((ra * ra) * m3) * m3) {
+      const float tc = -m1 / m3;
+      if (tc > tmin && tc < tmax) {
+        t = tc;
+        if (N) *N = -ba * isq;
+        if (lerp) *lerp = 0.f;
+        return true;
+      }
+    }
+  } else if (m2 > 0.f) {
+    const v3 q = ob * m3 - rd * m2;
+    if (dot(q, q) < ((rb * rb) * m3) * m3) {
+      const float tc = -m2 / m3;
+      if (tc > tmin && tc < tmax) {
+        t = tc;
+        if (N) *N = ba * isq;
+        if (lerp) *lerp = 1.f;
+        return true;
+      }
+    }
+  }
+  const float m4 = dot(rd, oa), m5 = dot(oa, oa);
+  const float rr = ra - rb;
+  const float hy = m0 + rr * rr;
+  const float k2 = m0 * m0 - (m3 * m3) * hy;
+  const float k1 = ((m0 * m0) * m4 - (m1 * m3) * hy) + (m0 * ra) * ((rr * m3) * 1.f);
+  const float k0 = ((m0 * m0) * m5 - (m1 * m1) * hy) + (m0 * ra) * ((rr * m1) * 2.f - m0 * ra);
+  const float h = k1 * k1 - k2 * k0;
+  if (h < 0.f) return false;
+  const float tc = (-k1 - sqrtf(h)) / k2;
+  const float y = m1 + tc * m3;
+  if (!(y > 0.f && y < m0 && tc > tmin && tc < tmax)) return false;
+  t = tc;
+  if (N) *N = m0 * (m0 * (oa + tc * rd) + (rr * ba) * ra) - (ba * hy) * y;
+  if (lerp) *lerp = fmx(0.f, fmn(1.f, y / m0));
+  return true;
+}
+
+// transformNormalFromObjectToWorldSpace: the transpose of the world-to-
+// object linear part applied to n
+BNX_D v3 xfm_normal_rows(const float4 *w2o, v3 n)
+{
+  return v3{(w2o[0].x * n.x + w2o[1].x * n.y) + w2o[2].x * n.z, (w2o[0].y * n.x + w2o[1].y * n.y) + w2o[2].y * n.z,
+            (w2o[0].z * n.x + w2o[1].z * n.y) + w2o[2].z * n.z};
+}
+
+// The hit attributes of a capsule or cone hit at distance t_hit, found
+// again from the geometry's records in the instance's object frame (o, d:
+// the object ray): the object point and normal, the world point and normal,
+// and the weight of the second end for attribute interpolation.
+struct PairHit {
+  v3 osP, osN, P, Nw;
+  float lerp;
+};
+
+// Capsules.dev.cu:212-262: the object point of capsule_intersect, world
+// point moved off the surface by max(1, |P|_inf) * 1e-5 along the world
+// normal, and the attribute weight the reference computes from the angle
+// between v0->P and v0->v1 (clamped to [0, 1])
+BNX_D void capsule_hit(const InstDev &I, float4 A, float4 B, v3 o, v3 d, float t_hit, PairHit &h)
+{
+  float t = t_hit;
+  v3 N{1.f, 0.f, 0.f}, objP = o + t_hit * d;
+  (void)capsule_intersect(o, d, A, B, t_hit * 1.0001f, t, &N, &objP);
+  const v3 v0 = xyz(A), v1 = xyz(B);
+  const float l01 = length(v1 - v0), lp0 = length(objP - v0);
+  float lerp = 0.f;
+  if (!(l01 < 1e-8f || lp0 < 1e-8f)) lerp = fmx(0.f, fmn(1.f, dot(objP - v0, v1 - v0) / (lp0 * l01)));
+  h.osP = objP;
+  h.osN = normalize(N);
+  if (I.identity) {
+    h.P = objP;
+    h.Nw = normalize(N);
+  } else {
+    h.P = xfm_point_rows(I.o2w, objP);
+    h.Nw = normalize(xfm_normal_rows(I.w2o, N));
+  }
+  float eps = 1.f;
+  eps = fmx(eps, fabsf(h.P.x));
+  eps = fmx(eps, fabsf(h.P.y));
+  eps = fmx(eps, fabsf(h.P.z));
+  eps *= 1e-5f;
+  h.P = h.P + eps * h.Nw;
+  h.lerp = lerp;
+}
+
+// Cones.dev.cu:117-262: P = o + t d (not moved off the surface), the
+// normals normalized, the attribute weight of cone_intersect
+BNX_D void cone_hit(const InstDev &I, float4 A, float4 B, v3 o, v3 d, float t_hit, PairHit &h)
+{
+  float t = t_hit, lerp = 0.f;
+  v3 N{1.f, 0.f, 0.f};
+  (void)cone_intersect(o, d, A, B, 0.f, t_hit * 1.0001f, t, &N, &lerp);
+  const v3 objP = o + t_hit * d;
+  h.osP = objP;
+  h.osN = normalize(N);
+  if (I.identity) {
+    h.P = objP;
+    h.Nw = normalize(h.osN);
+  } else {
+    h.P = xfm_point_rows(I.o2w, objP);
+    h.Nw = normalize(xfm_normal_rows(I.w2o, h.osN));
+  }
+  h.lerp = lerp;
 }
 
 // ------------------------------------------------------- ordered floats

@@ -372,6 +372,21 @@ BNX_D SurfHit eval_hit(const SceneDev &S, int4 hit, float hv, v3 rorg, v3 rdir)
                         (I.w2o[0].y * on.x + I.w2o[1].y * on.y) + I.w2o[2].y * on.z,
                         (I.w2o[0].z * on.x + I.w2o[1].z * on.y) + I.w2o[2].z * on.z});
     }
+  } else if (G.type == 3 || G.type == 4) {
+    // CapsulesPrograms / ConesPrograms::intersect's hit attributes: the hit
+    // found again from the records at the traced distance
+    v3 o = rorg, d = fix_dir(rdir);
+    if (!I.identity) {
+      o = xfm_point_rows(I.w2o, rorg);
+      d = fix_dir(xfm_vector_rows(I.w2o, d));
+    }
+    PairHit ph;
+    if (G.type == 3) capsule_hit(I, G.cyls[2 * prim], G.cyls[2 * prim + 1], o, d, hu, ph);
+    else cone_hit(I, G.cyls[2 * prim], G.cyls[2 * prim + 1], o, d, hu, ph);
+    osP = ph.osP;
+    P = ph.P;
+    Nw = ph.Nw;
+    cn = pair_corners(G, prim, ph.lerp);
   } else {
     v3 o = rorg, d = fix_dir(rdir);
     if (!I.identity) {

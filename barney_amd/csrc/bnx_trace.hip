@@ -128,36 +128,32 @@ template <bool EXT> BNX_D bool cut_rejects(const Cut &c, float t)
 }
 
 // barney's triangle test (rtcore/hiprt/TraceInterface.h:38-76) on a BVH-
-// ordered record (v0|prim, e1|geom, e2); acceptance under the tie rule
+// ordered record (v0|prim, e1|geom, e2); acceptance under the tie rule.
+// Evaluated without early exits: the reference's returns (a ray parallel to
+// the plane, t behind tmin, a degenerate projection, a point outside the
+// triangle) become one acceptance test after the three divisions, which
+// keeps a wave's lanes together through the test and lets the three record
+// loads issue at once (the values of every accepted hit are the same
+// operations as the reference's, in the same order)
 template <bool EXT>
 BNX_D bool tri_test(const float4 *rec, v3 o, v3 d, float tmin, HitRec &best, int inst, const Cut &cut)
 {
-  float4 A = rec[0], B = rec[1], C = rec[2];
-  v3 v0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
-  int prim = __float_as_int(A.w), geom = __float_as_int(B.w);
-  v3 N = cross(e1, e2);
-  float dn = dot(d, N);
-  if (fabsf(dn) < 1e-12f) return false;
-  float t = -dot(o - v0, N) / dn;
-  if (t <= tmin) return false;
-  if (!accept_t(t, best, inst, geom, prim)) return false;
-  v3 P = (o - v0) + t * d;
-  float e1u, e2u, Pu, e1v, e2v, Pv;
-  if (fabsf(N.x) >= fmx(fabsf(N.y), fabsf(N.z))) {
-    e1u = e1.y; e2u = e2.y; Pu = P.y;
-    e1v = e1.z; e2v = e2.z; Pv = P.z;
-  } else if (fabsf(N.y) > fabsf(N.z)) {
-    e1u = e1.x; e2u = e2.x; Pu = P.x;
-    e1v = e1.z; e2v = e2.z; Pv = P.z;
-  } else {
-    e1u = e1.x; e2u = e2.x; Pu = P.x;
-    e1v = e1.y; e2v = e2.y; Pv = P.y;
-  }
-  float den = e1u * e2v - e1v * e2u;
-  if (den == 0.f) return false;
-  float u = (Pu * e2v - Pv * e2u) / den;
-  float v = (e1u * Pv - e1v * Pu) / den;
-  if (u < 0.f || v < 0.f || (u + v) >= 1.f) return false;
+  const float4 A = rec[0], B = rec[1], C = rec[2];
+  const v3 v0 = xyz(A), e1 = xyz(B), e2 = xyz(C);
+  const int prim = __float_as_int(A.w), geom = __float_as_int(B.w);
+  const v3 N = cross(e1, e2);
+  const float dn = dot(d, N);
+  const float t = -dot(o - v0, N) / dn;
+  const v3 P = (o - v0) + t * d;
+  const bool px = fabsf(N.x) >= fmx(fabsf(N.y), fabsf(N.z));
+  const bool pz = !px && !(fabsf(N.y) > fabsf(N.z));
+  const float e1u = px ? e1.y : e1.x, e2u = px ? e2.y : e2.x, Pu = px ? P.y : P.x;
+  const float e1v = pz ? e1.y : e1.z, e2v = pz ? e2.y : e2.z, Pv = pz ? P.y : P.z;
+  const float den = e1u * e2v - e1v * e2u;
+  const float u = (Pu * e2v - Pv * e2u) / den;
+  const float v = (e1u * Pv - e1v * Pu) / den;
+  const bool in = !(fabsf(dn) < 1e-12f) & !(t <= tmin) & !(den == 0.f) & !(u < 0.f) & !(v < 0.f) & !((u + v) >= 1.f);
+  if (!in || !accept_t(t, best, inst, geom, prim)) return false;
   if (cut_rejects<EXT>(cut, t)) return false;
   if (EXT && cut.translucent && cut.rng) {
     const GeomDev &G = cut.S->geoms[cut.geom_base + geom];
@@ -203,19 +199,58 @@ BNX_D bool sphere_test(float4 S, const int2 *ref, v3 o, v3 d, float tmin, HitRec
   return true;
 }
 
-// CylindersPrograms::intersect (native/geometry/Cylinders.dev.cu:107-260)
-// on a BVH-ordered record ((v0, radius), (v1, 0)); the reference tests
-// against the ray's current closest distance, then reports the hit
-BNX_D bool cyl_test(const float4 *rec, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst,
-                    const Cut &cut)
+// The user geometries with two records per primitive: cylinders
+// (CylindersPrograms::intersect, native/geometry/Cylinders.dev.cu:107-260,
+// records (v0, radius), (v1, 0)), capsules (CapsulesPrograms::intersect,
+// Capsules.dev.cu:154-300, records (v0, r0), (v1, r1)) and cones
+// (ConesPrograms::intersect, Cones.dev.cu:60-266, records (p0, ra),
+// (p1, rb)).  The reference intersects them against the ray's closest hit
+// so far; capsules and cones take only candidates strictly nearer, so they
+// are given a bound just above it and a candidate at exactly the closest
+// distance reaches the tie rule (the nearest candidate below the bound does
+// not depend on the bound).  u keeps the hit distance for the shading.
+BNX_D float tie_bound(float t) { return t * 1.0001f; }
+
+// whether a capsule candidate at t survives its opacity draw (Capsules.dev.
+// cu:266-280): the material's alpha from the hit attributes, the draw from
+// Random(ray seed, hash(rtc instance, geometry, primitive)) taken on a copy
+// of the ray's state, so that the candidate's fate depends on the ray and
+// the candidate alone
+BNX_D bool capsule_keeps(const SceneDev &S, const GeomDev &G, int inst, int geom, int prim, float4 A, float4 B, v3 o,
+                         v3 d, float t, uint64_t seed)
+{
+  PairHit h;
+  capsule_hit(S.insts[inst], A, B, o, d, t, h);
+  const float4 base = material_color(S, G, inst, prim, pair_corners(G, prim, h.lerp), make_float4(0.f, 0.f, 0.f, 1.f),
+                                     h.P, h.osP);
+  const float alpha = material_alpha(G, base);
+  if (alpha < 1.f) {
+    const uint64_t hv = fnv3((uint32_t)inst, (uint32_t)geom, (uint32_t)prim);
+    rng_next_h(seed, hv ? hv : 290374ull);
+    const float r = (float)(rng_next(seed) & 0x00FFFFFFu) * (1.f / (float)0x01000000);
+    if (r > alpha) return false;
+  }
+  return true;
+}
+
+BNX_D bool pair_test(int type, const float4 *rec, const int2 *ref, v3 o, v3 d, float tmin, HitRec &best, int inst,
+                     const Cut &cut)
 {
   const float4 A = rec[0], B = rec[1];
   float t;
-  if (!cyl_intersect(o, d, xyz(A), xyz(B), A.w, tmin, best.t, t, nullptr)) return false;
+  bool h;
+  if (type == BLAS_CYLINDERS) h = cyl_intersect(o, d, xyz(A), xyz(B), A.w, tmin, best.t, t, nullptr);
+  else if (type == BLAS_CAPSULES) h = capsule_intersect(o, d, A, B, tie_bound(best.t), t, nullptr, nullptr);
+  else h = cone_intersect(o, d, A, B, tmin, tie_bound(best.t), t, nullptr, nullptr);
+  if (!h) return false;
   const int2 r = *ref;
   if (!accept_t(t, best, inst, r.x, r.y)) return false;
+  if (type == BLAS_CAPSULES && cut.translucent && cut.rng) {
+    const GeomDev &G = cut.S->geoms[cut.geom_base + r.x];
+    if (G.translucent && !capsule_keeps(*cut.S, G, inst, r.x, r.y, A, B, o, d, t, *cut.rng)) return false;
+  }
   if (cut_rejects<true>(cut, t)) return false;
-  best = HitRec{t, t, 0.f, r.y, r.x, inst};   // u: the hit distance, for the shading
+  best = HitRec{t, t, 0.f, r.y, r.x, inst};
   return true;
 }
 
@@ -263,6 +298,38 @@ BNX_D float child_entry(v3 o, v3 inv, float lx, float hx, float ly, float hy, fl
   return h ? tn : __builtin_huge_valf();
 }
 
+#if BNX_OCTANT
+// The byte offsets in a Node4 of the near-plane vectors along each axis for
+// the ray's direction signs (lo.x at 0 or hi.x at 16, lo.y at 32 or hi.y at
+// 48, lo.z at 64 or hi.z at 80); the far plane is the other one of the pair
+// (offset ^ 16).  For a valid box (lo <= hi) the near and far distances are
+// exactly the min and max of the two slab distances, so the walk is the same
+// as with min/max slab tests; an empty slot (lo = +inf, hi = -inf) gets an
+// infinite near distance on every axis and is never entered.
+struct Oct {
+  uint32_t x, y, z;
+};
+BNX_D Oct ray_oct(v3 inv) { return Oct{inv.x < 0.f ? 16u : 0u, inv.y < 0.f ? 48u : 32u, inv.z < 0.f ? 80u : 64u}; }
+BNX_D void node_entries(const Node4 *node, v3 o, v3 inv, Oct oc, float tmin, float tmax, float &t0, float &t1,
+                        float &t2, float &t3, uint4 &ch)
+{
+  const char *np = (const char *)node;
+  const float4 nx = *(const float4 *)(np + oc.x), fx = *(const float4 *)(np + (oc.x ^ 16u));
+  const float4 ny = *(const float4 *)(np + oc.y), fy = *(const float4 *)(np + (oc.y ^ 16u));
+  const float4 nz = *(const float4 *)(np + oc.z), fz = *(const float4 *)(np + (oc.z ^ 16u));
+  ch = ((const uint4 *)np)[6];
+  auto entry = [&](float nxk, float fxk, float nyk, float fyk, float nzk, float fzk) {
+    const float tn = fmaxf(fmaxf(fmaxf(tmin, (nxk - o.x) * inv.x), (nyk - o.y) * inv.y), (nzk - o.z) * inv.z);
+    const float tf = fminf(fminf(fminf(tmax, (fxk - o.x) * inv.x), (fyk - o.y) * inv.y), (fzk - o.z) * inv.z);
+    return tn <= tf * 1.00001f ? tn : __builtin_huge_valf();
+  };
+  t0 = entry(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x);
+  t1 = entry(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y);
+  t2 = entry(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z);
+  t3 = entry(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w);
+}
+#endif
+
 BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 {
   const bool s = tb < ta;
@@ -302,6 +369,9 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
   HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
   v3 o, d, inv;
   world_ray(rorg, rdir, o, d, inv);
+#if BNX_OCTANT
+  Oct oc = ray_oct(inv);
+#endif
   const Node4 *nodes = S.tlas_nodes;
   BlasDev blas{};
   bool in_blas = false;
@@ -314,14 +384,20 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
   for (uint32_t trip = 0; trip < (1u << 28); trip++) {
     if (cur < EMPTY4) {
       // ------------------------------------------------ 4-wide inner node
+      nvis++;
+#if BNX_OCTANT
+      float t0, t1, t2, t3;
+      uint4 ch;
+      node_entries(nodes + cur, o, inv, oc, tmin, best.t, t0, t1, t2, t3, ch);
+#else
       const float4 *np = (const float4 *)(nodes + cur);
       const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
       const uint4 ch = ((const uint4 *)np)[6];
-      nvis++;
       float t0 = child_entry(o, inv, lx.x, hx.x, ly.x, hy.x, lz.x, hz.x, ch.x, tmin, best.t);
       float t1 = child_entry(o, inv, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, tmin, best.t);
       float t2 = child_entry(o, inv, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, tmin, best.t);
       float t3 = child_entry(o, inv, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, tmin, best.t);
+#endif
       uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
       // nearest first: a 4-element sorting network on the entry distances
       cswap(t0, c0, t1, c1);
@@ -339,7 +415,12 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       }
     } else if (cur == LEAVE_BLAS) {
       in_blas = false;
-      if (XF) world_ray(rorg, rdir, o, d, inv);
+      if (XF) {
+        world_ray(rorg, rdir, o, d, inv);
+#if BNX_OCTANT
+        oc = ray_oct(inv);
+#endif
+      }
       nodes = S.tlas_nodes;
     } else {
       // ------------------------------------------------------------ leaf
@@ -355,6 +436,9 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
         }
         if (XF && !I.identity) {
           instance_ray(I, rorg, rdir, o, d, inv);
+#if BNX_OCTANT
+          oc = ray_oct(inv);
+#endif
         }
         push(LEAVE_BLAS);
         in_blas = true;
@@ -374,7 +458,8 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       } else {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
-          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst, cut);
+          found |= pair_test(blas.type, blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst,
+                               cut);
       }
       if (any_hit && found) break;
     }
@@ -483,7 +568,8 @@ BNX_D HitRec traverse_spec(const SceneDev &S, const float4 *rorg, const float4 *
       } else {
         cnt.spheres += count;
         for (int k = 0; k < count; k++)
-          found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst, cut);
+          found |= pair_test(blas.type, blas.prims + 2 * (first + k), blas.refs + first + k, o, d, tmin, best, inst,
+                               cut);
       }
       leaf = 0;
     } else if ((cur & LEAF_BIT) && !in_blas) {
@@ -781,7 +867,8 @@ BNX_D void trace_pool(const SlotDev *__restrict__ slots, const SegView &sv, cons
           } else {
             cnt.spheres += count;
             for (int k = 0; k < count; k++)
-              found |= cyl_test(blas.prims + 2 * (first + k), blas.refs + first + k, o, d, 0.f, best, inst, cut);
+              found |= pair_test(blas.type, blas.prims + 2 * (first + k), blas.refs + first + k, o, d, 0.f, best,
+                                   inst, cut);
           }
           if ((any_hit && found) || sp == 0) done = true;
           else cur = pop();

@@ -116,15 +116,6 @@ int launch_volume_march(Device *dev, const SlotDev *d_slots, const SegView &sv, 
                         bool surfaces, bool full, unsigned long long *stats);
 
 // ---------------------------------------------------------------- device
-BNX_D uint64_t fnv3(uint32_t a, uint32_t b, uint32_t c)
-{
-  // hash(v0,v1,v2) of native/common/barney-common.h:88-105
-  const uint64_t prime = 0x100000001b3ull;
-  uint64_t h = 0xcbf29ce484222325ull ^ (prime * (uint64_t)a);
-  h = h * prime ^ (uint64_t)b;
-  h = h * prime ^ (uint64_t)c;
-  return h;
-}
 
 // CPU texture LINEAR/CLAMP addressing (rtcore/cpu/Texture.cpp:324-336)
 BNX_D void lerp_coord(int N, float tc, int &i0, int &i1, float &f)
@@ -574,6 +565,48 @@ BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
     }
   });
   return found;
+}
+
+// Ratio tracking (Novak, Selle and Jarosz, "Residual Ratio Tracking for
+// Estimating Attenuation in Participating Media", SIGGRAPH Asia 2014,
+// section 3) of a shadow ray through one volume: the same macrocell walk,
+// RNG child and tentative collisions as volume_march (one draw per step
+// instead of two), and at each tentative collision the estimate is
+// multiplied by the null-collision probability 1 - sigma / majorant instead
+// of ending the ray with that probability.  Returns the transmittance
+// estimate in [0, 1]; its expectation is the volume's transmittance along
+// [tmin, tmax], as the expected value of delta tracking's 0/1 outcome is.
+// The reference has only delta tracking (volume/Volume.h:168-203): this is
+// an opt-in estimator (BNX_RENDER_RATIO_TRACKING).
+template <bool ANY_FIELD>
+BNX_D float ratio_track(const VolumeDev &v, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, MarchCtx &cx)
+{
+  if (!ANY_FIELD && v.field != FIELD_STRUCTURED) return 1.f;
+  float r0 = tmin, r1 = tmax;
+  if (!clip_bounds(v, org, dir, r0, r1)) return 1.f;
+  uint64_t rs = rng_child(parent, (uint32_t)v.rtc_inst);
+  auto rnd = [&]() -> float { return (float)(rng_next(rs) & 0x00FFFFFFu) * (1.f / (float)0x01000000); };
+  const float ray_tmax = tmax;
+  float T = 1.f;
+  dda3(v, org, dir, r1, [&](size_t ci, float c0, float c1) -> bool {
+    const float maj = v.majorants[ci];
+    cx.cells++;
+    if (maj == 0.f) return true;
+    const float upper = fmn(c1, ray_tmax);
+    float t = c0;
+    while (true) {
+      const float r = rnd();
+      t += -fast_log(1.f - r) / maj;
+      if (t >= upper) return true;
+      const v3 P = org + t * dir;
+      const float f = ANY_FIELD ? volume_sample(v, P, cx) : structured_sample(v, P);
+      cx.samples++;
+      const float density = isnan(f) ? 0.f : tf_map(v, f).w;
+      T *= fmx(0.f, 1.f - density / maj);
+      if (T == 0.f) return false;
+    }
+  });
+  return T;
 }
 
 // MCIsoSurfaceAccel::isProg (volume/MCAccelerator.h:325-548): DDA over the

@@ -247,8 +247,22 @@ k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
       }
       int vk = -1;
       float vt = 0.f, rgb[3] = {0.f, 0.f, 0.f};
+      // ratio tracking: a shadow ray's transmittance estimate through the
+      // volumes scales its MIS weight (the light sample it carries); zero
+      // transmittance occludes it as a delta-tracking event does
+      const bool ratio = shadow && (SL.rr.flags & BNX_RENDER_RATIO_TRACKING) != 0;
+      float T = 1.f;
       for (int k = 0; k < vl.count && !(shadow && ik >= 0); k++) {
         float t, c[3];
+        if (ratio) {
+          T *= ratio_track<FULL>(vl.vols[k], org, dir, 0.f, tcur, rng, cx);
+          if (T == 0.f) {
+            vk = k;
+            vt = tcur;
+            break;
+          }
+          continue;
+        }
         if (volume_march<FULL>(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, cx)) {
           vk = k;
           vt = t;
@@ -258,6 +272,7 @@ k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
         }
       }
       if (rng != rng0) q.rng[i] = rng;
+      if (ratio && vk < 0 && T < 1.f) q.ps[i].w = __float_as_uint(__uint_as_float(q.ps[i].w) * T);
       if (vk >= 0) {
         q.org[i].w = vt;
         hit = make_int4(0, VOLUME_GEOM_BASE + vk, vl.vols[vk].rtc_inst, 0);

@@ -133,6 +133,31 @@ class CylinderSet:
 
 
 @dataclass
+class CapsuleSet:
+    vertices: np.ndarray          # (nv,4) float32: position and radius per vertex
+    indices: np.ndarray           # (n,2) int32: the two end points of each capsule
+    colors: np.ndarray | None     # (n,3) float32 per capsule
+    user_id: int = 5
+    material: object = None
+    attributes: dict = field(default_factory=dict)
+
+
+@dataclass
+class ConeSet:
+    vertices: np.ndarray          # (nv,3) float32
+    radii: np.ndarray             # (nv,) float32 per vertex
+    indices: np.ndarray | None    # (n,2) int32, or None: cone i joins vertices 2i, 2i+1
+    colors: np.ndarray | None     # (n,3) float32 per cone
+    user_id: int = 6
+    material: object = None
+    attributes: dict = field(default_factory=dict)
+
+    @property
+    def num_cones(self):
+        return len(self.indices) if self.indices is not None else len(self.vertices) // 2
+
+
+@dataclass
 class EnvMap:
     """HDRI environment light: (height, width, 3) float32 radiance, row 0 at
     theta = 0 (the map's local +z), column 0 at phi = 0."""
@@ -173,6 +198,8 @@ class SurfaceScene:
     meshes: list = field(default_factory=list)
     spheres: list = field(default_factory=list)
     cylinders: list = field(default_factory=list)
+    capsules: list = field(default_factory=list)
+    cones: list = field(default_factory=list)
     dir_lights: list = field(default_factory=list)   # (direction(3), radiance, color(3))
     env: EnvMap | None = None
     ambient: float = 1.0
@@ -183,13 +210,14 @@ class SurfaceScene:
     spp: int = 1
     max_generations: int = 0       # renderer bound on traced generations (0: the path-depth limit)
     # instance attributes: slot 0-4 -> (n_rtc_instances, 4) float32, one row per rtc instance
-    # (spheres group, cylinders group, triangles group, in that order)
+    # (spheres, cylinders, capsules, cones and triangles groups, in that order)
     inst_attr: dict = field(default_factory=dict)
 
     @property
     def num_prims(self):
         return (sum(len(m.indices) for m in self.meshes) + sum(len(s.radii) for s in self.spheres) +
-                sum(len(c.indices) for c in self.cylinders))
+                sum(len(c.indices) for c in self.cylinders) + sum(len(c.indices) for c in self.capsules) +
+                sum(c.num_cones for c in self.cones))
 
 
 def config1(width: int = 512, height: int = 512, spp: int = 1, num_spheres: int = 1000,
@@ -233,6 +261,46 @@ def sticks(num_cylinders: int = 500, seed: int = SEEDS["c1"], extent: float = 2.
     radii = (0.01 + 0.04 * g[:, 7]).astype(np.float32)
     col = (0.2 + 0.6 * uniform_table(seed, 41, num_cylinders, 3)).astype(np.float32)
     return CylinderSet(verts, idx, radii, col)
+
+
+def capsule_chains(num_capsules: int = 400, seed: int = SEEDS["c1"], extent: float = 2.0) -> CapsuleSet:
+    """Random capsules as a streamline or tube renderer makes them: chains of
+    up to 8 links, each link from the previous end point to a point up to
+    0.5 away, the radius per vertex U(0.01, 0.06) (so most links taper);
+    colour U[0.2, 0.8]^3 per capsule."""
+    g = uniform_table(seed, 42, num_capsules, 8)
+    verts = []
+    idx = []
+    for k in range(num_capsules):
+        start = k % 8 == 0
+        if start:
+            verts.append(np.append(-extent + 2.0 * extent * g[k, 0:3], 0.01 + 0.05 * g[k, 6]))
+        a = len(verts) - 1
+        verts.append(np.append(verts[a][:3] + 0.5 * (g[k, 3:6] - 0.5) * 2.0, 0.01 + 0.05 * g[k, 7]))
+        idx.append((a, a + 1))
+    col = (0.2 + 0.6 * uniform_table(seed, 43, num_capsules, 3)).astype(np.float32)
+    return CapsuleSet(np.asarray(verts, np.float32), np.asarray(idx, np.int32), col)
+
+
+def cone_set(num_cones: int = 300, seed: int = SEEDS["c1"], extent: float = 2.0, indexed: bool = True) -> ConeSet:
+    """Random cones ("arrows"): a base at a random point, an apex up to 0.8
+    away, base radius U(0.02, 0.12) and apex radius U(0, 1) times it (a
+    tenth of them with the larger radius at the apex); colour U[0.2, 0.8]^3.
+    indexed: explicit index pairs, else the implicit pairs (2i, 2i+1)."""
+    g = uniform_table(seed, 44, num_cones, 9)
+    a = -extent + 2.0 * extent * g[:, 0:3]
+    b = a + 0.8 * (g[:, 3:6] - 0.5) * 2.0
+    ra = 0.02 + 0.10 * g[:, 6]
+    rb = ra * g[:, 7]
+    flip = g[:, 8] < 0.1
+    ra[flip], rb[flip] = rb[flip].copy(), ra[flip].copy()
+    verts = np.empty((2 * num_cones, 3), np.float32)
+    verts[0::2], verts[1::2] = a, b
+    radii = np.empty(2 * num_cones, np.float32)
+    radii[0::2], radii[1::2] = ra, rb
+    idx = np.stack([2 * np.arange(num_cones), 2 * np.arange(num_cones) + 1], 1).astype(np.int32) if indexed else None
+    col = (0.2 + 0.6 * uniform_table(seed, 45, num_cones, 3)).astype(np.float32)
+    return ConeSet(verts, radii, idx, col)
 
 
 # --------------------------------------------------------------------------

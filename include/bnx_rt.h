@@ -153,6 +153,43 @@ typedef struct bnx_cylinders_desc {
   bnx_material   material;
 } bnx_cylinders_desc;
 
+/* Capsules: Capsules::DD (native/geometry/Capsules.h:30-34), rounded cones
+ * between two vertices each, every vertex with its own radius, intersected
+ * as CapsulesPrograms::intersect does (native/geometry/Capsules.dev.cu:
+ * 19-300: the origin moved to 0.99 of the box entry, the body and both end
+ * spheres; stochastic opacity when the material's alpha can be below 1).
+ * colors: the "color" attribute per capsule. */
+typedef struct bnx_capsules_desc {
+  const float   *vertices;          /* float4[num_vertices]: position, radius    */
+  const int32_t *indices;           /* int2[num_capsules]: end points             */
+  const float   *colors;            /* float3[num_capsules] or NULL               */
+  int32_t        num_vertices;
+  int32_t        num_capsules;
+  int32_t        user_id;
+  int32_t        _pad;
+  bnx_material   material;
+} bnx_capsules_desc;
+
+/* Cones: Cones::DD (native/geometry/Cones.h:20-24), capped cones between two
+ * vertices each with a radius per vertex, intersected as ConesPrograms::
+ * intersect does (native/geometry/Cones.dev.cu:60-266).  Their BVH boxes
+ * are the boxes of both end spheres: ConesPrograms::bounds (Cones.dev.cu:
+ * 21-42) takes the second end's lower corner from the first radius
+ * (pb - ra), a box that need not hold the cone, and a hit outside its box
+ * would be found or not depending on traversal order.  indices NULL: cone i
+ * joins vertices 2i and 2i+1. */
+typedef struct bnx_cones_desc {
+  const float   *vertices;          /* float3[num_vertices]                       */
+  const float   *radii;             /* float[num_vertices]                        */
+  const int32_t *indices;           /* int2[num_cones] or NULL                    */
+  const float   *colors;            /* float3[num_cones] or NULL                  */
+  int32_t        num_vertices;
+  int32_t        num_cones;
+  int32_t        user_id;
+  int32_t        _pad;
+  bnx_material   material;
+} bnx_cones_desc;
+
 /* Camera::DD (native/Camera.h:30-65): a type and, by type, the fields of
  * the perspective camera (with depth of field when aperture_radius > 0),
  * the orthographic camera or the omnidirectional camera, as their commit()
@@ -189,6 +226,12 @@ typedef struct bnx_camera {
 
 /* Renderer::DD + the renderer constants of kernels/shadeRays.cu:17-34. */
 #define BNX_RENDER_COMPAT_SPHERE_INSTID13 1u  /* reproduce Spheres.dev.cu:193-194 */
+/* Shadow rays through volumes estimate their transmittance by ratio tracking
+ * (Novak et al. 2014) and carry it as a weight, instead of the reference's
+ * delta tracking (volume/Volume.h:168-203), whose 0/1 outcome has the same
+ * expectation and more variance.  Off by default: with it frames differ
+ * from the reference's sample by sample, not in expectation. */
+#define BNX_RENDER_RATIO_TRACKING         2u
 typedef struct bnx_renderer {
   float    bg_color[4];             /* 'background' (anari/Renderer.cpp:29)        */
   float    ambient_radiance;        /* 'ambientRadiance' (anari/Renderer.cpp:24)   */
@@ -430,13 +473,21 @@ int bnx_geom_create_spheres(bnx_device dev, const bnx_spheres_desc *desc, bnx_ge
  * for the 'Cylinders' user type (RTC_EXPORT_USER_GEOM(Cylinders, ...),
  * Cylinders.dev.cu:266) */
 int bnx_geom_create_cylinders(bnx_device dev, const bnx_cylinders_desc *desc, bnx_geom *out);
+/* ... Capsules::commit (native/geometry/Capsules.cpp:18-45) for the
+ * 'Capsules' user type (RTC_EXPORT_USER_GEOM(Capsules, ...),
+ * Capsules.dev.cu:306) */
+int bnx_geom_create_capsules(bnx_device dev, const bnx_capsules_desc *desc, bnx_geom *out);
+/* ... Cones::commit (native/geometry/Cones.cpp:18-52) for the 'Cones' user
+ * type (RTC_EXPORT_USER_GEOM(Cones, ...), Cones.dev.cu:270) */
+int bnx_geom_create_cones(bnx_device dev, const bnx_cones_desc *desc, bnx_geom *out);
 int bnx_geom_destroy(bnx_geom geom);
 /* Geometry attributes (GeometryAttributes, render/GeometryAttributes.h:
  * 32-63; Geometry::DD::setHitAttributes, geometry/Geometry.h:94-139): one
  * of attribute0..3, color or normal (BNX_ATTRIB_SLOT_*) set for the geometry
  * with a scope: a constant value, or per primitive, per vertex (triangles:
- * barycentric interpolation; cylinders: linear along the axis; spheres: the
- * sphere's own) or face-varying (3 per triangle, 2 per cylinder) elements of
+ * barycentric interpolation; cylinders, capsules and cones: linear between
+ * the two ends; spheres: the sphere's own) or face-varying (3 per triangle,
+ * 2 per cylinder, capsule or cone) elements of
  * 1-4 floats each (AttributeArray::DD::valueAt: missing y, z read 0, a
  * missing w reads 1).  BNX_SCOPE_NONE clears it.  The 'normal' attribute
  * replaces a triangle's object normal (Triangles.dev.cu:86-111).  The data
@@ -704,6 +755,8 @@ static_assert(sizeof(bnx_env_map_desc) == 48, "bnx_env_map_desc layout");
 static_assert(sizeof(bnx_triangles_desc) == 88, "bnx_triangles_desc layout");
 static_assert(sizeof(bnx_spheres_desc) == 88, "bnx_spheres_desc layout");
 static_assert(sizeof(bnx_cylinders_desc) == 96, "bnx_cylinders_desc layout");
+static_assert(sizeof(bnx_capsules_desc) == 88, "bnx_capsules_desc layout");
+static_assert(sizeof(bnx_cones_desc) == 96, "bnx_cones_desc layout");
 static_assert(sizeof(bnx_server_config) == 24, "bnx_server_config layout");
 static_assert(sizeof(bnx_umesh_desc) == 64, "bnx_umesh_desc layout");
 static_assert(sizeof(bnx_request) == 136, "bnx_request layout");

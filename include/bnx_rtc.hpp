@@ -16,7 +16,7 @@
  * before keep the geometry they were built over, which lives until the last
  * of them is freed.  The reference's program model (SBT records, device
  * programs per GeomType) is the library's closed set of types:
- * "Triangles", "Spheres" and "Cylinders". */
+ * "Triangles", "Spheres", "Cylinders", "Capsules" and "Cones". */
 #pragma once
 
 #include "bnx_rt.h"
@@ -89,8 +89,10 @@ struct GeomType;
 
 // rtc::Geom (rtcore/cuda/Geom.h:14-45): setVertices/setIndices for
 // triangles; setDD carries a user type's data, here the Spheres::DD fields
-// (geometry/Spheres.h:15-20) as setSpheres and the Cylinders::DD fields
-// (geometry/Cylinders.h:16-22) as setCylinders
+// (geometry/Spheres.h:15-20) as setSpheres, the Cylinders::DD fields
+// (geometry/Cylinders.h:16-22) as setCylinders, the Capsules::DD fields
+// (native/geometry/Capsules.h:30-34) as setCapsules and the Cones::DD
+// fields (native/geometry/Cones.h:20-24) as setCones
 struct Geom {
   explicit Geom(GeomType *gt) : gt(gt) {}
   // groups must be freed before the geometries they hold (as in the
@@ -146,6 +148,31 @@ struct Geom {
     primCount = numCylinders;
     this->defaultRadius = defaultRadius;
   }
+  // vertices: vec4f[numVertices] (position, radius); indices:
+  // vec2i[numCapsules]
+  void setCapsules(Buffer *vertices, int numVertices, Buffer *indices, int numCapsules, Buffer *colors)
+  {
+    dirty = true;
+    this->vertices = vertices;
+    this->numVertices = numVertices;
+    this->indices = indices;
+    this->numIndices = numCapsules;
+    this->colors = colors;
+    primCount = numCapsules;
+  }
+  // vertices: vec3f[numVertices]; radii: float[numVertices]; indices:
+  // vec2i[numCones], or null (cone i joins vertices 2i, 2i + 1)
+  void setCones(Buffer *vertices, int numVertices, Buffer *radii, Buffer *indices, int numCones, Buffer *colors)
+  {
+    dirty = true;
+    this->vertices = vertices;
+    this->numVertices = numVertices;
+    this->radii = radii;
+    this->indices = indices;
+    this->numIndices = numCones;
+    this->colors = colors;
+    primCount = numCones;
+  }
   void setUserID(int userID)
   {
     this->userID = userID;
@@ -174,10 +201,10 @@ struct Geom {
   bnx_material material{BNX_MATERIAL_MATTE, 1, {.8f, .8f, .8f, 1.f}, 1.f, 1.f, 1.f, 0.f, 1.45f, 0.f};
 };
 
-// rtc::GeomType (rtcore/cuda/GeomType.h): "Triangles", "Spheres" or
-// "Cylinders"
+// rtc::GeomType (rtcore/cuda/GeomType.h): "Triangles", "Spheres",
+// "Cylinders", "Capsules" or "Cones"
 struct GeomType {
-  enum Kind { TRIANGLES = 0, SPHERES = 1, CYLINDERS = 2 };
+  enum Kind { TRIANGLES = 0, SPHERES = 1, CYLINDERS = 2, CAPSULES = 3, CONES = 4 };
   GeomType(Device *device, Kind kind) : device(device), kind(kind) {}
   Geom *createGeom() { return new Geom(this); }
   Device *const device;
@@ -389,6 +416,27 @@ inline bnx_geom Geom::acquire(bnx_device dev)
     d.user_id = userID;
     d.material = material;
     check(bnx_geom_create_cylinders(dev, &d, &h));
+  } else if (gt->kind == GeomType::CAPSULES) {
+    bnx_capsules_desc d{};
+    d.vertices = vertices ? (const float *)vertices->hostData() : nullptr;
+    d.num_vertices = numVertices;
+    d.indices = indices ? (const int32_t *)indices->hostData() : nullptr;
+    d.num_capsules = primCount;
+    d.colors = colors ? (const float *)colors->hostData() : nullptr;
+    d.user_id = userID;
+    d.material = material;
+    check(bnx_geom_create_capsules(dev, &d, &h));
+  } else if (gt->kind == GeomType::CONES) {
+    bnx_cones_desc d{};
+    d.vertices = vertices ? (const float *)vertices->hostData() : nullptr;
+    d.num_vertices = numVertices;
+    d.radii = radii ? (const float *)radii->hostData() : nullptr;
+    d.indices = indices ? (const int32_t *)indices->hostData() : nullptr;
+    d.num_cones = primCount;
+    d.colors = colors ? (const float *)colors->hostData() : nullptr;
+    d.user_id = userID;
+    d.material = material;
+    check(bnx_geom_create_cones(dev, &d, &h));
   } else {
     bnx_spheres_desc d{};
     d.centers = origins ? (const float *)origins->hostData() : nullptr;

@@ -40,6 +40,12 @@ def lib():
         L.orc_scene_add_spheres.argtypes = [vp, C.POINTER(abi.SpheresDesc)]
         L.orc_scene_add_cylinders.argtypes = [vp, C.POINTER(abi.CylindersDesc)]
         L.orc_cylinder_intersect_n.argtypes = [abi.fp, C.c_int, abi.fp, C.c_float, C.c_float, abi.fp]
+        L.orc_volume_ratio_n.argtypes = [vp, C.c_int, vp, C.c_int, C.c_uint32, C.c_uint32, abi.fp, vp, C.c_int]
+        L.orc_volume_ratio_n.restype = None
+        L.orc_scene_add_capsules.argtypes = [vp, C.POINTER(abi.CapsulesDesc)]
+        L.orc_scene_add_cones.argtypes = [vp, C.POINTER(abi.ConesDesc)]
+        L.orc_capsule_intersect_n.argtypes = [abi.fp, C.c_int, abi.fp, C.c_float, abi.fp]
+        L.orc_cone_intersect_n.argtypes = [abi.fp, C.c_int, abi.fp, C.c_float, C.c_float, abi.fp]
         L.orc_scene_add_group.argtypes = [vp, C.c_int, C.POINTER(C.c_int), C.c_int]
         L.orc_geom_set_attribute.argtypes = [vp, C.c_int, C.c_int, C.c_int, C.c_int, abi.fp, C.c_int64, abi.fp]
         L.orc_geom_set_vertex_normals.argtypes = [vp, C.c_int, abi.fp, C.c_int]
@@ -194,6 +200,26 @@ class Scene:
                               user_id, material or abi.matte())
         return self.L.orc_scene_add_cylinders(self.h, C.byref(d))
 
+    def add_capsules(self, vertices, indices, colors=None, user_id=1, material=None):
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        i = np.ascontiguousarray(indices, dtype=np.int32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        self._keep += [v, i, c]
+        d = abi.CapsulesDesc(abi.fptr(v), abi.iptr(i), abi.fptr(c), len(v), len(i), user_id, 0,
+                             material or abi.matte())
+        return self.L.orc_scene_add_capsules(self.h, C.byref(d))
+
+    def add_cones(self, vertices, radii, indices=None, colors=None, user_id=1, material=None, num_cones=None):
+        v = np.ascontiguousarray(vertices, dtype=np.float32)
+        r = np.ascontiguousarray(radii, dtype=np.float32)
+        i = None if indices is None else np.ascontiguousarray(indices, dtype=np.int32)
+        c = None if colors is None else np.ascontiguousarray(colors, dtype=np.float32)
+        self._keep += [v, r, i, c]
+        n = len(i) if i is not None else (len(v) // 2 if num_cones is None else num_cones)
+        d = abi.ConesDesc(abi.fptr(v), abi.fptr(r), abi.iptr(i), abi.fptr(c), len(v), n, user_id, 0,
+                          material or abi.matte())
+        return self.L.orc_scene_add_cones(self.h, C.byref(d))
+
     def set_attribute(self, geom: int, slot: int, scope: int, data=None, value=None):
         """a geometry attribute (bnx_geom_set_attribute): data (n, comps) floats, or a constant rgba"""
         d = None if data is None else np.ascontiguousarray(data, dtype=np.float32)
@@ -309,6 +335,17 @@ class Scene:
         self.L.orc_volume_march_n(self.h, k, rays.ctypes.data, len(rays), seed[0], seed[1], abi.fptr(out),
                                   threads or default_threads())
         return out
+
+    def volume_ratio_n(self, rays, seed=(0, 7), k=0, threads=None):
+        """ratio-tracking transmittance estimate of each ray through volume
+        k (ray i with the parent seed (seed[0] + i, seed[1])), and the
+        tentative collisions each took"""
+        rays = np.ascontiguousarray(rays, np.float32).reshape(-1, 8)
+        out = np.zeros(len(rays), np.float32)
+        ns = np.zeros(len(rays), np.uint32)
+        self.L.orc_volume_ratio_n(self.h, k, rays.ctypes.data, len(rays), seed[0], seed[1], abi.fptr(out),
+                                  ns.ctypes.data, threads or default_threads())
+        return out, ns
 
     def set_dir_lights(self, lights):
         arr = abi.dir_lights(lights)
@@ -500,17 +537,51 @@ def cylinder_intersect(orgs, dirs, v0, v1, radius, tmin=0.0, tmax=1e30):
     return out[:, 0], out[:, 1:]
 
 
+def _rays(orgs, dirs):
+    return np.ascontiguousarray(np.concatenate([np.asarray(orgs, np.float32).reshape(-1, 3),
+                                                np.asarray(dirs, np.float32).reshape(-1, 3)], axis=1))
+
+
+def capsule_intersect(orgs, dirs, v0, r0, v1, r1, tmax=1e30):
+    """The oracle's capsule test (Capsules.dev.cu:154-212 with
+    intersectRoundedCone) for n rays against one capsule: (t, normalized
+    object normal, object point); t = -1 where missed."""
+    r = _rays(orgs, dirs)
+    cap = np.array([*np.asarray(v0, np.float32), r0, *np.asarray(v1, np.float32), r1], np.float32)
+    out = np.zeros((len(r), 7), np.float32)
+    lib().orc_capsule_intersect_n(abi.fptr(r), len(r), abi.fptr(cap), tmax, abi.fptr(out))
+    return out[:, 0], out[:, 1:4], out[:, 4:7]
+
+
+def cone_intersect(orgs, dirs, p0, ra, p1, rb, tmin=0.0, tmax=1e30):
+    """The oracle's cone test (Cones.dev.cu:60-266) for n rays against one
+    cone: (t, unnormalized object normal, attribute weight); t = -1 where
+    missed."""
+    r = _rays(orgs, dirs)
+    cone = np.array([*np.asarray(p0, np.float32), ra, *np.asarray(p1, np.float32), rb], np.float32)
+    out = np.zeros((len(r), 5), np.float32)
+    lib().orc_cone_intersect_n(abi.fptr(r), len(r), abi.fptr(cone), tmin, tmax, abi.fptr(out))
+    return out[:, 0], out[:, 1:4], out[:, 4]
+
+
 def surface_scene(sc) -> Scene:
     """Oracle scene for a barney_amd.scenes.SurfaceScene: one user instance
-    (ID 0) flattened into user-geometry groups (spheres, then cylinders) and
-    a triangles group, in the order of native/ModelSlot.cpp:128-155."""
+    (ID 0) flattened into user-geometry groups (spheres, cylinders, capsules,
+    cones) and a triangles group, in the order of native/ModelSlot.cpp:
+    128-155."""
     s = Scene()
     sph = [s.add_spheres(x.centers, x.radii, x.colors, user_id=x.user_id, material=x.material) for x in sc.spheres]
     cyl = [s.add_cylinders(x.vertices, x.indices, x.radii, x.colors, user_id=x.user_id, material=x.material)
            for x in getattr(sc, "cylinders", [])]
+    cap = [s.add_capsules(x.vertices, x.indices, x.colors, user_id=x.user_id, material=x.material)
+           for x in getattr(sc, "capsules", [])]
+    cone = [s.add_cones(x.vertices, x.radii, x.indices, x.colors, user_id=x.user_id, material=x.material)
+            for x in getattr(sc, "cones", [])]
     tri = [s.add_triangles(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material)
            for m in sc.meshes]
-    for gi, obj in zip(sph + cyl + tri, list(sc.spheres) + list(getattr(sc, "cylinders", [])) + list(sc.meshes)):
+    objs = (list(sc.spheres) + list(getattr(sc, "cylinders", [])) + list(getattr(sc, "capsules", [])) +
+            list(getattr(sc, "cones", [])) + list(sc.meshes))
+    for gi, obj in zip(sph + cyl + cap + cone + tri, objs):
         for slot, (scope, arr) in getattr(obj, "attributes", {}).items():
             if scope == abi.SCOPE_CONSTANT:
                 s.set_attribute(gi, slot, scope, value=arr)
@@ -524,6 +595,10 @@ def surface_scene(sc) -> Scene:
         s.add_instance(s.add_group(1, sph), 0)
     if cyl:
         s.add_instance(s.add_group(1, cyl), 0)
+    if cap:
+        s.add_instance(s.add_group(1, cap), 0)
+    if cone:
+        s.add_instance(s.add_group(1, cone), 0)
     if tri:
         s.add_instance(s.add_group(0, tri), 0)
     s.set_dir_lights(sc.dir_lights)

@@ -98,6 +98,50 @@ int orc_scene_add_cylinders(orc_scene *sc, const bnx_cylinders_desc *d)
   return (int)sc->s.geoms.size() - 1;
 }
 
+int orc_scene_add_capsules(orc_scene *sc, const bnx_capsules_desc *d)
+{
+  Geom g;
+  g.type = 3;
+  g.user_id = d->user_id;
+  set_material(g.mat, d->material);
+  for (int i = 0; i < d->num_vertices; i++) {
+    g.vertices.push_back(load3(d->vertices + 4 * i));
+    g.radii.push_back(d->vertices[4 * i + 3]);
+  }
+  g.indices.assign(d->indices, d->indices + 2 * (size_t)d->num_capsules);
+  if (d->colors) {
+    Attribute &a = g.attr[BNX_ATTRIB_SLOT_COLOR];
+    a.scope = BNX_SCOPE_PER_PRIM;
+    a.comps = 3;
+    a.data.assign(d->colors, d->colors + 3 * (size_t)d->num_capsules);
+  }
+  sc->s.geoms.push_back(std::move(g));
+  return (int)sc->s.geoms.size() - 1;
+}
+
+int orc_scene_add_cones(orc_scene *sc, const bnx_cones_desc *d)
+{
+  Geom g;
+  g.type = 4;
+  g.user_id = d->user_id;
+  set_material(g.mat, d->material);
+  for (int i = 0; i < d->num_vertices; i++) g.vertices.push_back(load3(d->vertices + 3 * i));
+  g.radii.assign(d->radii, d->radii + d->num_vertices);
+  // Cones.dev.cu:81-84: without indices cone i joins vertices 2i, 2i + 1
+  for (int i = 0; i < d->num_cones; i++) {
+    g.indices.push_back(d->indices ? d->indices[2 * i] : 2 * i);
+    g.indices.push_back(d->indices ? d->indices[2 * i + 1] : 2 * i + 1);
+  }
+  if (d->colors) {
+    Attribute &a = g.attr[BNX_ATTRIB_SLOT_COLOR];
+    a.scope = BNX_SCOPE_PER_PRIM;
+    a.comps = 3;
+    a.data.assign(d->colors, d->colors + 3 * (size_t)d->num_cones);
+  }
+  sc->s.geoms.push_back(std::move(g));
+  return (int)sc->s.geoms.size() - 1;
+}
+
 // GeometryAttributes of geometry 'geom' (render/GeometryAttributes.cpp:11-46
 // resolves one scope per attribute); count elements of comps floats
 int orc_geom_set_attribute(orc_scene *sc, int geom, int slot, int scope, int comps, const float *data, int64_t count,
@@ -318,6 +362,22 @@ void orc_volume_march_n(orc_scene *sc, int k, const bnx_ray *rays, int n, uint32
                       fix_dir(load3(rays[i].dir)), 0.f, rays[i].tmax, p, t, rgb, nullptr))
       t = -1.f;
     t_out[i] = t;
+  });
+}
+
+// ratio-tracking transmittance estimate of each ray through volume k; ray
+// i with the parent seed (seed_a + i, seed_b); samples: tentative
+// collisions per ray (may be null)
+void orc_volume_ratio_n(orc_scene *sc, int k, const bnx_ray *rays, int n, uint32_t seed_a, uint32_t seed_b,
+                        float *T_out, uint32_t *samples, int threads)
+{
+  parallel_for(n, threads, [&](int i) {
+    RNGSeed p;
+    p.seed(seed_a + (uint32_t)i, seed_b);
+    uint64_t ns = 0;
+    T_out[i] = ratio_track(sc->s.volumes[k], (uint32_t)(sc->s.insts.size() + k), load3(rays[i].org),
+                           fix_dir(load3(rays[i].dir)), 0.f, rays[i].tmax, p, &ns);
+    if (samples) samples[i] = (uint32_t)ns;
   });
 }
 
@@ -603,6 +663,44 @@ int orc_cylinder_intersect_n(const float *rays, int n, const float *cyl, float t
     out[4 * k + 1] = N.x;
     out[4 * k + 2] = N.y;
     out[4 * k + 3] = N.z;
+    hits += h;
+  }
+  return hits;
+}
+
+// one capsule (v0, r0, v1, r1 in cap) against n rays (org, dir): out k = t
+// (-1 where missed), the normalized object normal, the object point
+int orc_capsule_intersect_n(const float *rays, int n, const float *cap, float tmax, float *out)
+{
+  int hits = 0;
+  for (int k = 0; k < n; k++) {
+    float t = 0.f;
+    vec3 N(0.f), P(0.f);
+    const bool h = capsule_intersect(load3(rays + 6 * k), load3(rays + 6 * k + 3), load3(cap), cap[3], load3(cap + 4),
+                                     cap[7], tmax, t, &N, &P);
+    float *o = out + 7 * k;
+    o[0] = h ? t : -1.f;
+    o[1] = N.x; o[2] = N.y; o[3] = N.z;
+    o[4] = P.x; o[5] = P.y; o[6] = P.z;
+    hits += h;
+  }
+  return hits;
+}
+
+// one cone (p0, ra, p1, rb in cone) against n rays: out k = t (-1 where
+// missed), the unnormalized object normal, the attribute weight
+int orc_cone_intersect_n(const float *rays, int n, const float *cone, float tmin, float tmax, float *out)
+{
+  int hits = 0;
+  for (int k = 0; k < n; k++) {
+    float t = 0.f, lerp = -1.f;
+    vec3 N(0.f);
+    const bool h = cone_intersect(load3(rays + 6 * k), load3(rays + 6 * k + 3), load3(cone), cone[3],
+                                  load3(cone + 4), cone[7], tmin, tmax, t, &N, &lerp);
+    float *o = out + 5 * k;
+    o[0] = h ? t : -1.f;
+    o[1] = N.x; o[2] = N.y; o[3] = N.z;
+    o[4] = lerp;
     hits += h;
   }
   return hits;

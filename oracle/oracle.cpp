@@ -288,11 +288,21 @@ void build_scene(Scene &s)
           vec3 ctr = geom.centers[p];
           float r = geom.radii.empty() ? geom.default_radius : geom.radii[p];
           b = Box(ctr - vec3(r), ctr + vec3(r));
-        } else {
+        } else if (geom.type == 2) {
           // CylindersPrograms::bounds (native/geometry/Cylinders.dev.cu:69-85)
           vec3 a = geom.vertices[geom.indices[2 * p]], c = geom.vertices[geom.indices[2 * p + 1]];
           float r = geom.radii.empty() ? geom.default_radius : geom.radii[p];
           b = Box(vmin(a - vec3(r), c - vec3(r)), vmax(a + vec3(r), c + vec3(r)));
+        } else {
+          // capsules: getBounds (Capsules.dev.cu:100-110), the boxes of both
+          // end spheres; cones the same (ConesPrograms::bounds,
+          // Cones.dev.cu:21-42, builds the second box as (pb - ra, pb + rb),
+          // which need not hold the cone; a hit outside its primitive's box
+          // would depend on traversal order)
+          const int ia = geom.indices[2 * p], ib = geom.indices[2 * p + 1];
+          const vec3 a = geom.vertices[ia], c = geom.vertices[ib];
+          const float ra = geom.radii[ia], rb = geom.radii[ib];
+          b = Box(vmin(a - vec3(ra), c - vec3(rb)), vmax(a + vec3(ra), c + vec3(rb)));
         }
         boxes.push_back(b);
       }
@@ -562,6 +572,125 @@ bool cylinder_test_geom(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radi
   return true;
 }
 
+// Capsules.dev.cu:154-212 + intersectRoundedCone (:19-98), step by step
+bool capsule_test_geom(vec3 ro, vec3 rd, vec3 pa, float ra, vec3 pb, float rb, float tmax, float &t, vec3 *N,
+                       vec3 *P)
+{
+  // getBounds (:100-110), then Ray.h's boxTest with t0 = 0, t1 = tmax
+  const Box bb(vmin(pa - vec3(ra), pb - vec3(rb)), vmax(pa + vec3(ra), pb + vec3(rb)));
+  const vec3 rcp(1.f / rd.x, 1.f / rd.y, 1.f / rd.z);
+  const vec3 tlo = (bb.lo - ro) * rcp, thi = (bb.hi - ro) * rcp;
+  const float t0 = fmx(0.f, reduce_max(vmin(tlo, thi)));
+  const float t1 = fmn(tmax, reduce_min(vmax(tlo, thi)));
+  if (!(t0 <= t1)) return false;
+  // move the origin to just before the box entry (:191-196)
+  const float t_move = .99f * t0;
+  const vec3 o = ro + t_move * rd;
+  const float bound = tmax - t_move;
+  float hit_t = bound;
+  // one end sphere inside the other: no capsule (:30-36)
+  const float dist_ab = length(pa - pb);
+  if (dist_ab + ra <= rb) return false;
+  if (dist_ab + rb <= ra) return false;
+  const vec3 ba = pb - pa, oa = o - pa, ob = o - pb;
+  const float rr = ra - rb;
+  const float m0 = dot(ba, ba), m1 = dot(ba, oa), m2 = dot(ba, rd), m3 = dot(rd, oa);
+  const float m5 = dot(oa, oa), m6 = dot(ob, rd), m7 = dot(ob, ob);
+  const float d2 = m0 - rr * rr;
+  const float k2 = d2 - m2 * m2;
+  if (k2 == 0.f) return false;
+  const float k1 = (d2 * m3 - m1 * m2) + (m2 * rr) * ra;
+  const float k0 = ((d2 * m5 - m1 * m1) + ((m1 * rr) * ra) * 2.f) - (m0 * ra) * ra;
+  const float h = k1 * k1 - k0 * k2;
+  if (h < 0.f) return false;
+  bool had = false;
+  vec3 n(0.f);
+  // the body (:62-70)
+  float tc = (-sqrtf(h) - k1) / k2;
+  const float y = (m1 - ra * rr) + tc * m2;
+  if (y > 0.f && y < d2 && tc > 1e-6f && tc < hit_t) {
+    hit_t = tc;
+    n = d2 * (oa + tc * rd) - ba * y;
+    had = true;
+  }
+  // the two end spheres (:72-95)
+  const float h1 = (m3 * m3 - m5) + ra * ra;
+  if (h1 > 0.f) {
+    tc = -m3 - sqrtf(h1);
+    if (tc > 1e-6f && tc < hit_t) {
+      hit_t = tc;
+      n = ((o + tc * rd) - pa) / ra;
+      had = true;
+    }
+  }
+  const float h2 = (m6 * m6 - m7) + rb * rb;
+  if (h2 > 0.f) {
+    tc = -m6 - sqrtf(h2);
+    if (tc > 1e-6f && tc < hit_t) {
+      hit_t = tc;
+      n = ((o + tc * rd) - pb) / rb;
+      had = true;
+    }
+  }
+  if (!had) return false;
+  if (hit_t < 0.f || hit_t > bound) return false;
+  if (N) *N = normalize(n);
+  if (P) *P = o + hit_t * rd;
+  t = hit_t + t_move;
+  return true;
+}
+
+// Cones.dev.cu:60-266, step by step; inversesqrt (:14) is 1. / sqrtf(f),
+// a double division rounded to float
+bool cone_test_geom(vec3 ro, vec3 rd, vec3 p0, float ra, vec3 p1, float rb, float tmin, float tmax, float &t,
+                    vec3 *N, float *lerp)
+{
+  const vec3 ba = p1 - p0, oa = ro - p0, ob = ro - p1;
+  const float m0 = dot(ba, ba), m1 = dot(oa, ba), m2 = dot(ob, ba), m3 = dot(rd, ba);
+  const float isq = (float)(1.0 / (double)sqrtf(m0));
+  if (m1 < 0.f) {
+    // the first cap (:117-160)
+    const vec3 q = oa * m3 - rd * m1;
+    if (dot(q, q) < ((ra * ra) * m3) * m3) {
+      const float tc = -m1 / m3;
+      if (tc > tmin && tc < tmax) {
+        t = tc;
+        if (N) *N = -ba * isq;
+        if (lerp) *lerp = 0.f;
+        return true;
+      }
+    }
+  } else if (m2 > 0.f) {
+    // the second cap (:162-205)
+    const vec3 q = ob * m3 - rd * m2;
+    if (dot(q, q) < ((rb * rb) * m3) * m3) {
+      const float tc = -m2 / m3;
+      if (tc > tmin && tc < tmax) {
+        t = tc;
+        if (N) *N = ba * isq;
+        if (lerp) *lerp = 1.f;
+        return true;
+      }
+    }
+  }
+  // the body (:209-264)
+  const float m4 = dot(rd, oa), m5 = dot(oa, oa);
+  const float rr = ra - rb;
+  const float hy = m0 + rr * rr;
+  const float k2 = m0 * m0 - (m3 * m3) * hy;
+  const float k1 = ((m0 * m0) * m4 - (m1 * m3) * hy) + (m0 * ra) * ((rr * m3) * 1.f);
+  const float k0 = ((m0 * m0) * m5 - (m1 * m1) * hy) + (m0 * ra) * ((rr * m1) * 2.f - m0 * ra);
+  const float h = k1 * k1 - k2 * k0;
+  if (h < 0.f) return false;
+  const float tc = (-k1 - sqrtf(h)) / k2;
+  const float y = m1 + tc * m3;
+  if (!(y > 0.f && y < m0 && tc > tmin && tc < tmax)) return false;
+  t = tc;
+  if (N) *N = m0 * (m0 * (oa + tc * rd) + (rr * ba) * ra) - (ba * hy) * y;
+  if (lerp) *lerp = fmx(0.f, fmn(1.f, y / m0));
+  return true;
+}
+
 // slab test, conservative by a relative 1e-5 on the far distance
 inline bool box_test(vec3 org, vec3 inv, const float *lo, const float *hi, float tmin, float tmax, float &tnear)
 {
@@ -622,6 +751,28 @@ bool opacity_keeps(const Scene &s, const Geom &geom, int inst, int prim, float u
   return true;
 }
 
+// whether a capsule candidate survives its opacity draw (Capsules.dev.cu:
+// 266-280): alpha from the material at the hit's attributes; the draw of
+// Random(ray seed, hash(rtc instance, geometry, primitive)) on a copy of the
+// ray's state
+bool capsule_keeps(const Scene &s, const Geom &geom, int inst, int gi, int prim, vec3 org, vec3 dir, float t,
+                   uint64_t seed)
+{
+  PairHit h;
+  pair_hit(s.insts[inst], geom, prim, org, dir, t, h);
+  const float color0[4] = {0.f, 0.f, 0.f, 1.f};
+  float base[4];
+  material_color(s, geom, inst, prim, pair_corners(geom, prim, h.lerp), color0, h.P, h.osP, base);
+  const float alpha = material_alpha(geom.mat, base);
+  if (alpha < 1.f) {
+    RNGSeed copy;
+    copy.state = seed;
+    Random2 rng = Random2::from64(copy, fnv_hash3((uint32_t)inst, (uint32_t)gi, (uint32_t)prim));
+    if (rng() > alpha) return false;
+  }
+  return true;
+}
+
 bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, float tmin, Hit &best, int inst,
                TraceCounters *ctr, const CutRay &cut, const uint64_t *seed, Hit *rej)
 {
@@ -655,12 +806,26 @@ bool prim_test(const Scene &s, const Group &g, int slot, vec3 org, vec3 dir, flo
     return true;
   } else {
     if (ctr) ctr->spheres++;
-    float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
     float t;
-    if (!cylinder_intersect(org, dir, geom.vertices[geom.indices[2 * pid]], geom.vertices[geom.indices[2 * pid + 1]],
-                            r, tmin, best.t, t, nullptr))
-      return false;
+    const int ia = geom.indices[2 * pid], ib = geom.indices[2 * pid + 1];
+    if (geom.type == 2) {
+      float r = geom.radii.empty() ? geom.default_radius : geom.radii[pid];
+      if (!cylinder_intersect(org, dir, geom.vertices[ia], geom.vertices[ib], r, tmin, best.t, t, nullptr))
+        return false;
+    } else if (geom.type == 3) {
+      // the reference tests against the closest hit so far and keeps only
+      // strictly nearer candidates; the bound just above it lets a
+      // candidate at exactly that distance reach the tie rule
+      if (!capsule_intersect(org, dir, geom.vertices[ia], geom.radii[ia], geom.vertices[ib], geom.radii[ib],
+                             best.t * 1.0001f, t, nullptr, nullptr))
+        return false;
+    } else {
+      if (!cone_intersect(org, dir, geom.vertices[ia], geom.radii[ia], geom.vertices[ib], geom.radii[ib], tmin,
+                          best.t * 1.0001f, t, nullptr, nullptr))
+        return false;
+    }
     if (!accept_t(t, best, inst, gi, pid)) return false;
+    if (geom.type == 3 && seed && !capsule_keeps(s, geom, inst, gi, pid, org, dir, t, *seed)) return false;
     if (cut_rejects(cut, t)) return false;
     best.t = t;
     best.u = t;
@@ -678,6 +843,88 @@ bool cylinder_intersect(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radi
                         float &t, vec3 *N)
 {
   return cylinder_test_geom(ray_org, ray_dir, v0, v1, radius, tmin, tmax, t, N);
+}
+
+bool capsule_intersect(vec3 ro, vec3 rd, vec3 pa, float ra, vec3 pb, float rb, float tmax, float &t, vec3 *N,
+                       vec3 *P)
+{
+  return capsule_test_geom(ro, rd, pa, ra, pb, rb, tmax, t, N, P);
+}
+
+bool cone_intersect(vec3 ro, vec3 rd, vec3 p0, float ra, vec3 p1, float rb, float tmin, float tmax, float &t,
+                    vec3 *N, float *lerp)
+{
+  return cone_test_geom(ro, rd, p0, ra, p1, rb, tmin, tmax, t, N, lerp);
+}
+
+Corners pair_corners(const Geom &g, int prim, float lerp)
+{
+  Corners c;
+  c.n = 2;
+  c.vi[0] = g.indices[2 * prim];
+  c.vi[1] = g.indices[2 * prim + 1];
+  c.fi[0] = 2 * prim;
+  c.fi[1] = 2 * prim + 1;
+  c.w[0] = 1.f - lerp;
+  c.w[1] = lerp;
+  return c;
+}
+
+// transformNormalFromObjectToWorldSpace: the transpose of the world-to-
+// object linear part
+static vec3 xfm_normal(const Affine &w, vec3 n)
+{
+  return vec3(w.vx.x * n.x + w.vx.y * n.y + w.vx.z * n.z, w.vy.x * n.x + w.vy.y * n.y + w.vy.z * n.z,
+              w.vz.x * n.x + w.vz.y * n.y + w.vz.z * n.z);
+}
+
+void pair_hit(const Instance &in, const Geom &g, int prim, vec3 o, vec3 d, float t_hit, PairHit &h)
+{
+  const int ia = g.indices[2 * prim], ib = g.indices[2 * prim + 1];
+  const vec3 v0 = g.vertices[ia], v1 = g.vertices[ib];
+  const float r0 = g.radii[ia], r1 = g.radii[ib];
+  float t = t_hit;
+  vec3 N(1.f, 0.f, 0.f);
+  if (g.type == 3) {
+    // Capsules.dev.cu:212-262: the point of the moved ray, the angle-based
+    // attribute weight, the world point moved off the surface
+    vec3 objP = o + t_hit * d;
+    (void)capsule_test_geom(o, d, v0, r0, v1, r1, t_hit * 1.0001f, t, &N, &objP);
+    const float l01 = length(v1 - v0), lp0 = length(objP - v0);
+    float lerp = 0.f;
+    if (!(l01 < 1e-8f || lp0 < 1e-8f)) lerp = fmx(0.f, fmn(1.f, dot(objP - v0, v1 - v0) / (lp0 * l01)));
+    h.osP = objP;
+    h.osN = normalize(N);
+    if (in.o2w.identity) {
+      h.P = objP;
+      h.Nw = normalize(N);
+    } else {
+      h.P = xfm_point(in.o2w, objP);
+      h.Nw = normalize(xfm_normal(in.w2o, N));
+    }
+    float eps = 1.f;
+    eps = fmx(eps, fabsf(h.P.x));
+    eps = fmx(eps, fabsf(h.P.y));
+    eps = fmx(eps, fabsf(h.P.z));
+    eps *= 1e-5f;
+    h.P = h.P + eps * h.Nw;
+    h.lerp = lerp;
+  } else {
+    // Cones.dev.cu:117-262: P = o + t d, the normal normalized once
+    float lerp = 0.f;
+    (void)cone_test_geom(o, d, v0, r0, v1, r1, 0.f, t_hit * 1.0001f, t, &N, &lerp);
+    const vec3 objP = o + t_hit * d;
+    h.osP = objP;
+    h.osN = normalize(N);
+    if (in.o2w.identity) {
+      h.P = objP;
+      h.Nw = normalize(h.osN);
+    } else {
+      h.P = xfm_point(in.o2w, objP);
+      h.Nw = normalize(xfm_normal(in.w2o, h.osN));
+    }
+    h.lerp = lerp;
+  }
 }
 
 Hit trace_bvh(const Scene &s, vec3 org, vec3 dir0, float tmin, float tmax, bool any_hit, TraceCounters *ctr,

@@ -201,20 +201,20 @@ struct Attribute {
 };
 
 struct Geom {
-  int type = 0;                     // 0 triangles, 1 spheres, 2 cylinders
+  int type = 0;                     // 0 triangles, 1 spheres, 2 cylinders, 3 capsules, 4 cones
   int user_id = 0;
   Material mat;
   std::vector<vec3> vertices;
-  std::vector<int32_t> indices;     // 3 per triangle, 2 per cylinder
+  std::vector<int32_t> indices;     // 3 per triangle, 2 per cylinder, capsule or cone
   std::vector<vec3> centers;
-  std::vector<float> radii;
+  std::vector<float> radii;         // spheres, cylinders: per primitive; capsules, cones: per vertex
   float default_radius = 0.1f;
   std::vector<vec3> colors;         // spheres' own colors (Spheres::DD::colors), may be empty
   std::vector<vec3> normals;        // triangles' vertex normals (Triangles::DD::normals), may be empty
   Attribute attr[6];                // attribute0..3, color, normal
   int num_prims() const
   {
-    return type == 0 ? (int)indices.size() / 3 : (type == 2 ? (int)indices.size() / 2 : (int)centers.size());
+    return type == 0 ? (int)indices.size() / 3 : (type >= 2 ? (int)indices.size() / 2 : (int)centers.size());
   }
 };
 
@@ -223,6 +223,20 @@ struct Geom {
 // lies in [tmin, tmax], else the back hit; *N the unnormalized object normal
 bool cylinder_intersect(vec3 ray_org, vec3 ray_dir, vec3 v0, vec3 v1, float radius, float tmin, float tmax,
                         float &t, vec3 *N);
+
+// CapsulesPrograms::intersect (native/geometry/Capsules.dev.cu:154-212)
+// with intersectRoundedCone (:19-98): the rounded cone between the end
+// spheres (pa, ra), (pb, rb) in the ray's object frame, the origin moved to
+// 0.99 of the entry distance into the box of both end spheres; the nearest
+// of body and end spheres below tmax.  *N: the normalized object normal,
+// *P: the object point (moved origin + moved distance)
+bool capsule_intersect(vec3 ro, vec3 rd, vec3 pa, float ra, vec3 pb, float rb, float tmax, float &t, vec3 *N,
+                       vec3 *P);
+// ConesPrograms::intersect (native/geometry/Cones.dev.cu:60-266): the cap on
+// the origin's side first, then the body's nearer root, in (tmin, tmax).
+// *N: the unnormalized object normal; *lerp: weight of the second end
+bool cone_intersect(vec3 ro, vec3 rd, vec3 p0, float ra, vec3 p1, float rb, float tmin, float tmax, float &t,
+                    vec3 *N, float *lerp);
 
 struct Group {
   int type = 0;                     // 0 triangles group, 1 user group
@@ -290,6 +304,11 @@ float tf_majorant(const Volume &v, float lo, float hi);
 bool volume_march(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin, float tmax, RNGSeed &parent,
                   float &t_out, float rgb[3], uint64_t *samples);
 float fast_log(float f);
+// ratio tracking of a shadow ray through one volume (opt-in,
+// BNX_RENDER_RATIO_TRACKING): the transmittance estimate in [0, 1], with the
+// walk, RNG child and tentative collisions of volume_march
+float ratio_track(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin, float tmax, RNGSeed &parent,
+                  uint64_t *samples);
 // tets: UMeshField commit + buildInitialMacroCells (UMeshField.cu:168-197,
 // 320-379) + element BVH (UMeshCuBQLSampler::build, umesh/mc/
 // UMeshCuBQLSampler.cu:45-78, with this build's binned-SAH builder)
@@ -362,6 +381,17 @@ void material_color(const Scene &s, const Geom &g, int inst, int prim, const Cor
 // the BSDF's opacity (Lambertian / NVisii alpha, AnariMatte.h:64, AnariPBR.h:87-90)
 float material_alpha(const Material &m, const float base[4]);
 Corners tri_corners(const int32_t *ix, int prim, float u, float v);
+// the two ends of a cylinder, capsule or cone at attribute weight lerp
+Corners pair_corners(const Geom &g, int prim, float lerp);
+// the hit attributes of a capsule or cone hit at distance t_hit, found again
+// in the instance's object frame (o, d: the object ray): object point and
+// normal, world point (capsules: moved off the surface) and normal, and the
+// attribute weight of the second end
+struct PairHit {
+  vec3 osP, osN, P, Nw;
+  float lerp = 0.f;
+};
+void pair_hit(const Instance &in, const Geom &g, int prim, vec3 o, vec3 d, float t_hit, PairHit &h);
 
 void build_scene(Scene &s);        // BLAS per group + TLAS
 

@@ -401,6 +401,20 @@ void apply_hit(const Scene &s, PathRay &r, const Hit &h, HitIDs *ids, bool compa
                           w.vy.x * on.x + w.vy.y * on.y + w.vy.z * on.z,
                           w.vz.x * on.x + w.vz.y * on.y + w.vz.z * on.z));
     }
+  } else if (geom.type == 3 || geom.type == 4) {
+    // CapsulesPrograms / ConesPrograms::intersect's hit attributes: the hit
+    // found again at the traced distance
+    vec3 o = r.org, d = fix_dir(r.dir);
+    if (!in.w2o.identity) {
+      o = xfm_point(in.w2o, r.org);
+      d = fix_dir(xfm_vector(in.w2o, d));
+    }
+    PairHit ph;
+    pair_hit(in, geom, h.prim, o, d, h.u, ph);
+    osP = ph.osP;
+    P = ph.P;
+    Nw = ph.Nw;
+    cn = pair_corners(geom, h.prim, ph.lerp);
   } else {
     vec3 o = r.org, d = fix_dir(r.dir);
     if (!in.w2o.identity) {
@@ -671,9 +685,23 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
         int vk = -1;
         float vt = 0.f, vrgb[3] = {0.f, 0.f, 0.f};
         uint64_t vs = 0;
+        // ratio tracking (opt-in): a shadow ray's transmittance estimate
+        // scales its MIS weight; zero occludes it
+        const bool ratio = r.is_shadow && ik < 0 && (rr.flags & BNX_RENDER_RATIO_TRACKING) != 0;
+        float T = 1.f;
         for (int k = 0, rk = 0; k < (int)s.volumes.size(); k++) {
           if (!s.volumes[k].rendered) continue;
           float t, rgb[3];
+          if (ratio) {
+            T *= ratio_track(s.volumes[k], (uint32_t)(s.insts.size() + rk), r.org, fd, 0.f, tcur, r.rng, &vs);
+            rk++;
+            if (T == 0.f) {
+              vk = k;
+              vt = tcur;
+              break;
+            }
+            continue;
+          }
           if (volume_march(s.volumes[k], (uint32_t)(s.insts.size() + rk), r.org, fd, 0.f, tcur, r.rng, t, rgb, &vs)) {
             vk = k;
             vt = t;
@@ -682,6 +710,7 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
           }
           rk++;
         }
+        if (ratio && vk < 0 && T < 1.f) r.mis = r.mis * T;
         if (vk >= 0) apply_volume_hit(s, r, vk, vt, vrgb, gen == 0 ? &ids[i] : nullptr);
         else if (ik >= 0) apply_iso_hit(s, r, ik, it, iN, gen == 0 ? &ids[i] : nullptr);
         else apply_hit(s, r, h, gen == 0 ? &ids[i] : nullptr, compat13);

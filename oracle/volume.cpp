@@ -198,4 +198,41 @@ bool volume_march(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float 
   return found;
 }
 
+// Ratio tracking (Novak, Selle and Jarosz, SIGGRAPH Asia 2014, section 3)
+// of a shadow ray through one volume: the walk, RNG child and tentative
+// collisions of volume_march (one draw per step), the estimate multiplied by
+// 1 - sigma / majorant at each.  Opt-in (BNX_RENDER_RATIO_TRACKING); the
+// reference has delta tracking only (volume/Volume.h:168-203).
+float ratio_track(const Volume &v, uint32_t rtc_inst, vec3 org, vec3 dir, float tmin, float tmax, RNGSeed &parent,
+                  uint64_t *samples)
+{
+  float r0 = tmin, r1 = tmax;
+  if (!box_clip(org, dir, v.bounds, r0, r1)) return 1.f;
+  vec3 rsp(1.f / v.mc_spacing.x, 1.f / v.mc_spacing.y, 1.f / v.mc_spacing.z);
+  vec3 dorg = (org - v.mc_origin) * rsp;
+  vec3 ddir = dir * rsp;
+  Random2 rng = Random2::from64(parent, fnv_hash3(rtc_inst, 0u, 0u));
+  const float ray_tmax = tmax;
+  float T = 1.f;
+  dda3(dorg, ddir, r1, v.mc, [&](const int cell[3], float c0, float c1) -> bool {
+    const float maj = v.majorant[mc_index(v, cell)];
+    if (maj == 0.f) return true;
+    const float upper = fmn(c1, ray_tmax);
+    float t = c0;
+    while (true) {
+      const float r = rng();
+      t += -fast_log(1.f - r) / maj;
+      if (t >= upper) return true;
+      const vec3 P = org + t * dir;
+      const float f = volume_sample(v, P);
+      if (samples) (*samples)++;
+      float s[4] = {0.f, 0.f, 0.f, 0.f};
+      if (!std::isnan(f)) tf_map(v, f, s);
+      T *= fmx(0.f, 1.f - s[3] / maj);
+      if (T == 0.f) return false;
+    }
+  });
+  return T;
+}
+
 } // namespace orc

@@ -126,8 +126,8 @@ def hits_equal(a, b):
 
 class SceneSpec:
     """A scene described once and built on both sides: groups of geometries
-    (kind 0 triangles, 1 spheres, 2 cylinders; a group of kind 1 holds
-    spheres or cylinders) and instances (group, user id, o2w, w2o)."""
+    (kind 0 triangles, 1 spheres, 2 cylinders, 3 capsules, 4 cones; a group
+    of kind 1 holds one user type) and instances (group, user id, o2w, w2o)."""
 
     def __init__(self):
         self.geoms = []      # (kind, kwargs)
@@ -150,6 +150,16 @@ class SceneSpec:
                                    material=material, default_radius=default_radius)))
         return len(self.geoms) - 1
 
+    def cap(self, vertices, indices, colors=None, user_id=0, material=None):
+        self.geoms.append((3, dict(vertices=vertices, indices=indices, colors=colors, user_id=user_id,
+                                   material=material)))
+        return len(self.geoms) - 1
+
+    def cone(self, vertices, radii, indices=None, colors=None, user_id=0, material=None):
+        self.geoms.append((4, dict(vertices=vertices, radii=radii, indices=indices, colors=colors, user_id=user_id,
+                                   material=material)))
+        return len(self.geoms) - 1
+
     def group(self, kind, geoms):
         self.groups.append((kind, list(geoms)))
         return len(self.groups) - 1
@@ -167,8 +177,12 @@ class SceneSpec:
             elif kind == 1:
                 ids.append(s.add_spheres(kw["centers"], kw["radii"], kw["colors"], user_id=kw["user_id"],
                                          material=kw["material"]))
-            else:
+            elif kind == 2:
                 ids.append(s.add_cylinders(**kw))
+            elif kind == 3:
+                ids.append(s.add_capsules(**kw))
+            else:
+                ids.append(s.add_cones(**kw))
         gids = [s.add_group(k, [ids[g] for g in gl]) for k, gl in self.groups]
         for g, uid, o2w, w2o in self.insts:
             s.add_instance(gids[g], uid, o2w, w2o)
@@ -186,8 +200,12 @@ class SceneSpec:
             elif kind == 1:
                 geoms.append(bx.Geom.spheres(dev, kw["centers"], kw["radii"], kw["colors"],
                                              user_id=kw["user_id"], material=kw["material"]))
-            else:
+            elif kind == 2:
                 geoms.append(bx.Geom.cylinders(dev, **kw))
+            elif kind == 3:
+                geoms.append(bx.Geom.capsules(dev, **kw))
+            else:
+                geoms.append(bx.Geom.cones(dev, **kw))
         groups = [bx.Group.geoms(dev, k, [geoms[g] for g in gl]).build() for k, gl in self.groups]
         xf = None
         if any(o is not None for _, _, o, _ in self.insts):
@@ -204,11 +222,19 @@ def c1_spec(sc):
     sph = [spec.sph(s.centers, s.radii, s.colors, user_id=s.user_id, material=s.material) for s in sc.spheres]
     cyl = [spec.cyl(c.vertices, c.indices, c.radii, c.colors, user_id=c.user_id, material=c.material)
            for c in sc.cylinders]
+    cap = [spec.cap(c.vertices, c.indices, c.colors, user_id=c.user_id, material=c.material)
+           for c in getattr(sc, "capsules", [])]
+    cone = [spec.cone(c.vertices, c.radii, c.indices, c.colors, user_id=c.user_id, material=c.material)
+            for c in getattr(sc, "cones", [])]
     tri = [spec.tri(m.vertices, m.indices, m.colors, user_id=m.user_id, material=m.material) for m in sc.meshes]
     if sph:
         spec.inst(spec.group(1, sph), 0)
     if cyl:
         spec.inst(spec.group(1, cyl), 0)
+    if cap:
+        spec.inst(spec.group(1, cap), 0)
+    if cone:
+        spec.inst(spec.group(1, cone), 0)
     if tri:
         spec.inst(spec.group(0, tri), 0)
     spec.lights = list(sc.dir_lights)

@@ -21,8 +21,8 @@ def test_majorants_match(dev, orc, mc):
     assert np.array_equal(gm.view(np.uint32), om.view(np.uint32))
 
 
-def _render_both(dev, orc, vs, spp):
-    rend = abi.renderer(bg=vs.bg_color, ambient=vs.ambient)
+def _render_both(dev, orc, vs, spp, flags=0):
+    rend = abi.renderer(bg=vs.bg_color, ambient=vs.ambient, flags=flags)
     w = bx.VolumeWorld(dev, vs)
     cam = bx.camera_perspective(vs.camera["position"], vs.camera["direction"], vs.camera["up"], vs.camera["fovy"])
     fr = bx.Frame(dev, vs.width, vs.height)
@@ -72,3 +72,27 @@ def test_transfer_function_update_rebuilds_majorants(dev, orc):
     _, om = orc.volume_scene(vs2).volume_majorants()
     _, gm = g.majorants()
     assert np.array_equal(gm, om)
+
+
+def test_ratio_tracking_parity(dev, orc):
+    """BNX_RENDER_RATIO_TRACKING (shadow rays carry a ratio-tracking
+    transmittance estimate): GPU == oracle channel for channel at 1 and 4
+    spp, the frame differs from delta tracking's, and the two agree where
+    both have converged (same expectation)"""
+    print("ratio tracking parity", flush=True)
+    vs = scenes.config2(width=96, height=96, spp=4)
+    for spp in (1, 4):
+        out, st, ost = _render_both(dev, orc, vs, spp, flags=abi.BNX_RENDER_RATIO_TRACKING)
+        for ch in (abi.BNX_FB_PRIMID, abi.BNX_FB_INSTID, abi.BNX_FB_OBJID):
+            assert np.array_equal(*out[ch]), ch
+        gc, oc = out[abi.BNX_FB_COLOR]
+        assert np.abs(gc - oc).max() <= 1e-3, float(np.abs(gc - oc).max())
+        assert st["volume_samples"] == ost["volume_samples"]
+    delta, _, _ = _render_both(dev, orc, vs, 4)
+    assert not np.array_equal(delta[abi.BNX_FB_COLOR][0], out[abi.BNX_FB_COLOR][0])
+    vc = scenes.config2(width=32, height=32, spp=256)
+    r, _, _ = _render_both(dev, orc, vc, 256, flags=abi.BNX_RENDER_RATIO_TRACKING)
+    dl, _, _ = _render_both(dev, orc, vc, 256)
+    diff = float(np.abs(r[abi.BNX_FB_COLOR][0][..., :3].mean() - dl[abi.BNX_FB_COLOR][0][..., :3].mean()))
+    print("ratio vs delta mean diff at 256 spp", diff, flush=True)
+    assert diff < 0.01 * float(dl[abi.BNX_FB_COLOR][0][..., :3].mean()) + 1e-4
