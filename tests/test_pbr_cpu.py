@@ -200,3 +200,21 @@ def test_pbr_sphere_under_directional_light(orc):
     assert (k >= 1).all() and (k == 1).mean() > 0.5
     assert np.allclose(got, k[:, None] * exp, rtol=3e-3, atol=1e-5)
     assert np.allclose(img[hit][cosl < -0.05], 0.0)
+
+
+@pytest.mark.parametrize("mat", MATS)
+def test_eval_reciprocity(orc, mat):
+    """Helmholtz reciprocity, independent of any transcription of the
+    formulas: the Disney diffuse term, the GGX distribution, the Smith
+    masking product and the Schlick term on the half vector are each
+    symmetric in (wo, wi), so f(wo, wi) = f(wi, wo) for directions on the
+    same side of the surface; a term that takes the wrong angle breaks it"""
+    rng = np.random.default_rng(31)
+    n = np.tile([[0.0, 0.0, 1.0]], (4000, 1))
+    wo = rand_dirs(rng, 4000)
+    wi = rand_dirs(rng, 4000)
+    wo[:, 2], wi[:, 2] = np.abs(wo[:, 2]) + 0.05, np.abs(wi[:, 2]) + 0.05
+    wo, wi = unit(wo), unit(wi)
+    a, _ = orc.pbr_eval(mat[0], mat[1], mat[2], n, wo, wi)
+    b, _ = orc.pbr_eval(mat[0], mat[1], mat[2], n, wi, wo)
+    assert np.allclose(a, b, rtol=3e-5, atol=1e-7), np.abs(a - b).max()
