@@ -113,6 +113,8 @@ struct Wavefront {
   // of host_stage (host): nothing is allocated, no timing events exist, and
   // the paths per batch are bounded by the slices
   bool noalloc = false;
+  // host copy of the statistics counters (read back once per render)
+  unsigned long long h_stats[STAT_COUNT * STAT_SLOTS];
   ~Wavefront() { drop_graphs(); }
   void drop_graphs()
   {
@@ -376,9 +378,8 @@ int collect_timing(Wavefront &F, int n, bool surfaces, bool volumes, GenTiming &
 
 int read_stats(Device *dev, Wavefront &F, unsigned long long out[STAT_COUNT])
 {
-  std::vector<unsigned long long> v((size_t)STAT_COUNT * STAT_SLOTS);
-  BNX_HIP(hipMemcpyAsync(v.data(), F.Q.stats.ptr, sizeof(unsigned long long) * v.size(), hipMemcpyDeviceToHost,
-                         dev->stream));
+  unsigned long long *v = F.h_stats;
+  BNX_HIP(hipMemcpyAsync(v, F.Q.stats.ptr, sizeof(F.h_stats), hipMemcpyDeviceToHost, dev->stream));
   BNX_HIP(hipStreamSynchronize(dev->stream));
   for (int k = 0; k < STAT_COUNT; k++) {
     out[k] = 0;

@@ -25,12 +25,20 @@ def _noalloc(world, cam, rend, fr, spp):
 
 
 def _same_as_render(dev, world, cam, rend, w, h, spp):
+    """bnx_render_noalloc against bnx_render of the same request: every
+    channel bit for bit; with bnx_render's batches as large as the scratch
+    array's (2^16 paths) and no captured loop, every statistic too"""
     print(f"noalloc {w}x{h} {spp} spp", flush=True)
     fa = bx.Frame(dev, w, h)
     rc, st = _noalloc(world, cam, rend, fa, spp)
     assert rc == abi.BNX_OK, world.dev.L.bnx_last_error().decode()
     fb = bx.Frame(dev, w, h)
-    ref = bx.render(world, cam, rend, fb, spp)
+    saved = dev.tuning()
+    dev.set_tuning(render_paths=1 << 16, graph_paths=0)
+    try:
+        ref = bx.render(world, cam, rend, fb, spp)
+    finally:
+        dev.set_tuning(**saved)
     for ch in CHANNELS:
         assert np.array_equal(fa.read(ch).view(np.uint8), fb.read(ch).view(np.uint8)), (w, h, ch)
     assert st.rays_traced == ref["rays"] and st.generations == ref["generations"]
