@@ -112,6 +112,7 @@ _PROTOS = [
     ("bnx_server_destroy", [vp], C.c_int),
     ("bnx_server_submit", [vp, C.POINTER(abi.Request), C.POINTER(C.c_int64)], C.c_int),
     ("bnx_server_step", [vp, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int)], C.c_int),
+    ("bnx_server_step_n", [vp, C.c_int, C.POINTER(C.c_int64), C.c_int, C.POINTER(C.c_int)], C.c_int),
     ("bnx_server_poll", [vp, C.c_int64, C.POINTER(abi.RequestStatus)], C.c_int),
     ("bnx_server_read", [vp, C.c_int64, C.c_int, vp], C.c_int),
     ("bnx_server_release", [vp, C.c_int64], C.c_int),
@@ -589,9 +590,15 @@ class Server:
         self.pending.add(t.value)
         return t.value
 
-    def step(self) -> list:
+    def step(self, generations: int = 0) -> list:
+        """one generation read back by the host (generations=0, bnx_server_step)
+        or that many generations with the bookkeeping on the device and one
+        read-back (bnx_server_step_n); returns the tickets completed"""
         n = C.c_int()
-        check(self.L.bnx_server_step(self.h, self._done, len(self._done), C.byref(n)))
+        if generations > 0:
+            check(self.L.bnx_server_step_n(self.h, generations, self._done, len(self._done), C.byref(n)))
+        else:
+            check(self.L.bnx_server_step(self.h, self._done, len(self._done), C.byref(n)))
         done = [self._done[k] for k in range(min(n.value, len(self._done)))]
         self.pending.difference_update(done)
         return done
@@ -617,13 +624,13 @@ class Server:
         check(self.L.bnx_server_get_stats(self.h, C.byref(st)))
         return {k: getattr(st, k) for k, _ in abi.ServerStats._fields_}
 
-    def run_until_idle(self, max_steps=1_000_000) -> list:
+    def run_until_idle(self, max_steps=1_000_000, generations: int = 0) -> list:
         """step until every submitted request is done; returns the tickets in completion order"""
         done = []
         for _ in range(max_steps):
             if not self.pending:
                 break
-            done += self.step()
+            done += self.step(generations)
         return done
 
     def close(self):

@@ -119,17 +119,25 @@ class UMeshDesc(C.Structure):
 
 def umesh_desc(us):
     """(UMeshDesc, TransferFunction, keep-alive arrays) for a
-    scenes.UMeshVolumeScene: one tet per cell, cell c at index 4c."""
+    scenes.UMeshVolumeScene: its cells (indices, offsets, types) when it has
+    them, else one tet per cell, cell c at index 4c with no type array."""
     v = np.ascontiguousarray(us.vertices, dtype=np.float32)
     sc = np.ascontiguousarray(us.scalars, dtype=np.float32)
-    idx = np.ascontiguousarray(us.tets, dtype=np.int32).ravel()
-    n = len(us.tets)
-    offs = (4 * np.arange(n, dtype=np.int64)).astype(np.int32)
+    types = None
+    if getattr(us, "cells", None) is not None:
+        idx = np.ascontiguousarray(us.cells[0], dtype=np.int32)
+        offs = np.ascontiguousarray(us.cells[1], dtype=np.int32)
+        types = np.ascontiguousarray(us.cells[2], dtype=np.uint8)
+    else:
+        idx = np.ascontiguousarray(us.tets, dtype=np.int32).ravel()
+        offs = (4 * np.arange(len(us.tets), dtype=np.int64)).astype(np.int32)
+    n = len(offs)
     tf = np.ascontiguousarray(us.tf, dtype=np.float32)
-    d = UMeshDesc(fptr(v), fptr(sc), iptr(idx), iptr(offs), None, len(v), len(idx), n,
+    tp = types.ctypes.data_as(C.POINTER(C.c_uint8)) if types is not None else None
+    d = UMeshDesc(fptr(v), fptr(sc), iptr(idx), iptr(offs), tp, len(v), len(idx), n,
                   (C.c_int32 * 3)(*us.mc_dims))
     t = TransferFunction(fptr(tf), len(tf), (C.c_float * 2)(*us.tf_domain), us.base_density)
-    return d, t, (v, sc, idx, offs, tf)
+    return d, t, (v, sc, idx, offs, tf, types)
 
 
 class RenderStats(C.Structure):

@@ -72,7 +72,8 @@ constexpr int SORT_MIN_RAYS = 1 << 16;
 // captured with)
 struct BatchShape {
   long long count0 = 0, cap = 0;
-  bool surfaces = false, volumes = false, full = false;
+  bool surfaces = false, volumes = false;
+  int full = 0;       // the march kernel: 0 lean, else MARCH_FULL / MARCH_CELLS bits
   int refill = 64, stack = 16, walk = 0, blocks_per_cu = 8, shade_block = 1024;
   bool ext = false;   // the trace kernels with the cylinder test and the cut plane
   bool xf = false;    // the trace kernels that change the ray's frame at instances
@@ -228,7 +229,7 @@ struct GenTiming {
 // Enqueue one generation over queue `cur`: upload the segments, build the
 // block maps, trace, volume march, shade into queue 1 - cur.  The caller
 // has uploaded the output region bases and cleared the output counters.
-int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool surfaces, bool volumes, bool full)
+int enqueue_generation(Device *dev, Wavefront &F, int cur, SegList &sl, bool surfaces, bool volumes, int full)
 {
   Queues &Q = F.Q;
   RayQueue in = Q.q(cur), out = Q.q(1 - cur);
@@ -403,8 +404,10 @@ int trace_refill_for(const Device *dev, const World *w)
 
 namespace {
 
-// full: the world has isosurfaces or tet volumes (the march kernel's full form)
-int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, SlotDev &s, bool &full)
+// full: the march kernel the world needs (MARCH_FULL: it has isosurfaces or
+// unstructured volumes; MARCH_CELLS: some unstructured field has other cells
+// than tets)
+int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f, SlotDev &s, int &full)
 {
   VolumeList vl{};
   int rc = world_volumes(w, vl);
@@ -414,8 +417,9 @@ int fill_slot(World *w, const bnx_camera &cam, const bnx_renderer &rr, Frame *f,
   s.vols = vl.vols;
   s.num_vols = vl.count;
   if ((rc = world_isos(w, s.isos, s.num_isos)) != BNX_OK) return rc;
-  full = s.num_isos > 0;
-  for (const Volume *v : w->volumes) full = full || v->dd.field != FIELD_STRUCTURED;
+  full = s.num_isos > 0 ? MARCH_FULL : 0;
+  for (const Volume *v : w->volumes) full |= march_kind(v);
+  for (const IsoSurface *iso : w->isos) full |= march_kind(iso->field);
   BNX_SET(s.dir_lights, w->d_dir_lights.ptr);
   s.num_dir_lights = (int)w->dir_lights.size();
   s.env = env_dev(w);
@@ -478,7 +482,7 @@ int render_frame_impl(World *w, const bnx_camera &cam, const bnx_renderer &rr, F
       BNX_OK)
     return rc;
   SlotDev *hs = F.h_slots.as<SlotDev>();
-  bool full = false;
+  int full = 0;
   if ((rc = fill_slot(w, cam, rr, f, *hs, full)) != BNX_OK) return rc;
   const bool surfaces = hs->scene.num_insts > 0;
   const bool volumes = hs->num_vols > 0 || hs->num_isos > 0;
@@ -721,6 +725,190 @@ namespace {
 
 constexpr size_t FRAME_BYTES_PER_PIXEL = 80;   // 64 B of channels + 16 B read staging
 
+// ---- device-driven server steps: several generations per host read-back.
+// A request slot's state as the device keeps it through such a step; the
+// host uploads it for every active slot before the step and reads it back
+// after it.
+struct SrvSlot {
+  long long next;              // paths started
+  long long total;             // paths of the request
+  long long rays;              // rays of the slot that entered a generation in this step
+  int region;                  // first queue index of its rays in the current input queue
+  int live_rays, live_paths;   // rays there; paths they continue
+  int refill;                  // trace-pool refill of its secondary rays
+  int added;                   // primary rays added in the current generation
+  int pad;
+};
+static_assert(sizeof(SrvSlot) == 48, "SrvSlot");
+
+// one step's tables: the active slots in the host's order (their old regions
+// come first in every generation), the slots with primaries left in top-up
+// priority order, and the scalars the generations hand to each other
+struct SrvStep {
+  SrvSlot *st;
+  const int *act;
+  const int *ord;
+  int A, O;
+  Segment *segs;               // A + O: old regions, then new primaries
+  RaygenChunk *chunks;         // O
+  int *out_base, *out_cnt;
+  // [0] end of the regions in the current input queue (new primaries go
+  // there), [1] [2] the raygen range (queue begin, count) of this generation
+  long long *scal;
+  long long *gen_counts;       // rays per generation
+  long long max_paths;
+  int target;                  // trace blocks to aim for (pooled layout)
+};
+
+constexpr int SRV_TB = 1024;
+
+// exclusive prefix sum over the SRV_TB threads of the block; total: the sum
+__device__ long long srv_scan(long long v, long long &total)
+{
+  __shared__ long long wsum[SRV_TB / 64];
+  const int lane = (int)(threadIdx.x & 63), w = (int)(threadIdx.x >> 6);
+  long long x = v;
+  for (int d = 1; d < 64; d <<= 1) {
+    const long long y = __shfl_up(x, d, 64);
+    if (lane >= d) x += y;
+  }
+  if (lane == 63) wsum[w] = x;
+  __syncthreads();
+  long long off = 0, tot = 0;
+  for (int k = 0; k < SRV_TB / 64; k++) {
+    const long long s = wsum[k];
+    off += k < w ? s : 0;
+    tot += s;
+  }
+  __syncthreads();
+  total = tot;
+  return off + x - v;
+}
+
+__device__ long long srv_sum(long long v)
+{
+  long long t;
+  (void)srv_scan(v, t);
+  return t;
+}
+
+// The host step's bookkeeping for one generation, on the device (one block
+// of SRV_TB threads):
+//  - fold the previous generation's output (g > 0): each slot's rays are now
+//    the region its shade wrote, with the counts it wrote;
+//  - top up: walk the slots with primaries left in priority order, each
+//    taking min(what is left, room) paths while the paths in flight stay
+//    within max_paths; their raygen chunks follow the old regions;
+//  - segments: old regions of every active slot, then the new primaries,
+//    with their block offsets for both launch widths and the trace pool
+//    layout (about `target` blocks, as SegList::layout_trace);
+//  - output regions in the other queue, 2 rays per path in flight, in the
+//    order of the active slots, with their counters cleared.
+// fold_only: the fold alone (after a step's last generation).
+__global__ __launch_bounds__(SRV_TB) void k_srv_setup(SrvStep p, int g, int fold_only)
+{
+  const int tid = (int)threadIdx.x;
+  if (g > 0 || fold_only) {
+    for (int i = tid; i < p.A; i += SRV_TB) {
+      const int s = p.act[i];
+      p.st[s].region = p.out_base[s];
+      p.st[s].live_rays = p.out_cnt[2 * s];
+      p.st[s].live_paths = p.out_cnt[2 * s + 1];
+    }
+  }
+  if (fold_only) return;
+  for (int i = tid; i < p.A; i += SRV_TB) p.st[p.act[i]].added = 0;
+  __syncthreads();
+  const long long in_end = p.scal[0];
+  long long lp = 0;
+  for (int i = tid; i < p.A; i += SRV_TB) lp += p.st[p.act[i]].live_paths;
+  const long long room = max(0LL, p.max_paths - srv_sum(lp));
+  // top-up: the takes are the remaining paths clamped to what room is left
+  // after the slots before, so their prefix sum is min(prefix of rem, room)
+  long long carry = 0;
+  for (int base = 0; base < p.O; base += SRV_TB) {
+    const int j = base + tid;
+    int s = -1;
+    long long rem = 0;
+    if (j < p.O) {
+      s = p.ord[j];
+      rem = p.st[s].total - p.st[s].next;
+    }
+    long long tot;
+    const long long pre = carry + srv_scan(rem, tot);
+    if (j < p.O) {
+      const long long take = min(rem, max(0LL, room - pre));
+      p.chunks[j] = RaygenChunk{in_end + min(pre, room), p.st[s].next, s, 0};
+      p.st[s].next += take;
+      p.st[s].added = (int)take;
+    }
+    carry += tot;
+  }
+  const long long added = min(carry, room);
+  __syncthreads();
+  // segments
+  const int nsegs = p.A + p.O;
+  auto seg = [&](int k, int &slot, int &begin, int &refill) -> int {
+    if (k < p.A) {
+      slot = p.act[k];
+      begin = p.st[slot].region;
+      refill = p.st[slot].refill;
+      return p.st[slot].live_rays;
+    }
+    slot = p.ord[k - p.A];
+    begin = (int)p.chunks[k - p.A].begin;
+    refill = 64;                 // primary rays: whole-wave refills
+    return p.st[slot].added;
+  };
+  long long n = 0;
+  for (int k = tid; k < nsegs; k += SRV_TB) {
+    int slot, begin, refill;
+    n += seg(k, slot, begin, refill);
+  }
+  n = srv_sum(n);
+  const long long per = 256LL * max(1LL, (n + 256LL * p.target - 1) / (256LL * p.target));
+  long long c256 = 0, c1024 = 0, cT = 0;
+  for (int base = 0; base < nsegs; base += SRV_TB) {
+    const int k = base + tid;
+    int slot = 0, begin = 0, refill = 64, count = 0;
+    if (k < nsegs) count = seg(k, slot, begin, refill);
+    long long t256, t1024, tT;
+    const long long b256 = c256 + srv_scan((count + 255) / 256, t256);
+    const long long b1024 = c1024 + srv_scan((count + 1023) / 1024, t1024);
+    const long long bT = cT + srv_scan((count + per - 1) / per, tT);
+    if (k < nsegs) p.segs[k] = Segment{slot, begin, count, (int)b256, (int)b1024, (int)bT, refill, 0};
+    c256 += t256;
+    c1024 += t1024;
+    cT += tT;
+  }
+  // output regions
+  carry = 0;
+  for (int base = 0; base < p.A; base += SRV_TB) {
+    const int i = base + tid;
+    long long size = 0;
+    int s = -1;
+    if (i < p.A) {
+      s = p.act[i];
+      size = 2LL * (p.st[s].live_paths + p.st[s].added);
+    }
+    long long tot;
+    const long long pre = carry + srv_scan(size, tot);
+    if (i < p.A) {
+      p.out_base[s] = (int)pre;
+      p.out_cnt[2 * s] = 0;
+      p.out_cnt[2 * s + 1] = 0;
+      p.st[s].rays += p.st[s].live_rays + p.st[s].added;
+    }
+    carry += tot;
+  }
+  if (tid == 0) {
+    p.gen_counts[g] = n;
+    p.scal[1] = in_end;
+    p.scal[2] = added;
+    p.scal[0] = carry;           // the next generation's input regions end here
+  }
+}
+
 struct Request {
   int64_t ticket = 0;
   World *world = nullptr;
@@ -737,7 +925,8 @@ struct Request {
   double submit_ms = -1.0, admit_ms = -1.0, done_ms = -1.0;
   double cost_ms = 0.0;            // share of step time by its rays in each step
   double deadline_ms = -1.0;       // set at admission when the top-up is deadline-ordered
-  bool surfaces = false, volumes = false, full = false;
+  bool surfaces = false, volumes = false;
+  int full = 0;
 };
 
 // measured cost of a world's paths: step time attributed to completed
@@ -765,6 +954,9 @@ struct Server {
   size_t frame_bytes = 0;
   int cur = 0;
   int in_end = 0;                  // end of the output regions laid out in the current queue
+  // device-driven steps: slot states, active and top-up order, scalars
+  DevMem d_srv, d_act, d_ord, d_scal;
+  Pinned h_srv, h_act, h_ord, h_scal;
   int64_t next_ticket = 1;
   int64_t submitted = 0, admitted = 0, done = 0, released = 0, failed = 0, rays = 0, gens = 0, peak = 0;
   double busy_ms = 0.0;
@@ -872,6 +1064,49 @@ struct Server {
     return BNX_OK;
   }
 
+  // the active requests with primaries left, in top-up priority order
+  std::vector<Request *> topup_order() const
+  {
+    std::vector<Request *> order;
+    for (Request *r : active)
+      if (r->next_path < r->total_paths) order.push_back(r);
+    if (slack > 0.f)
+      std::stable_sort(order.begin(), order.end(),
+                       [](const Request *a, const Request *b) { return a->deadline_ms < b->deadline_ms; });
+    else
+      std::stable_sort(order.begin(), order.end(), [](const Request *a, const Request *b) {
+        return a->total_paths - a->next_path < b->total_paths - b->next_path;
+      });
+    return order;
+  }
+
+  // after a step: a request whose primaries are all issued and that has no
+  // rays left is done; its slot is freed
+  void settle(double tnow, int64_t *done_tickets, int max_done, int *num_done)
+  {
+    std::vector<Request *> still;
+    for (Request *r : active) {
+      if (r->next_path == r->total_paths && r->live_rays == 0) {
+        r->state = BNX_REQ_DONE;
+        r->done_ms = tnow;
+        PathCost &wc = world_cost[r->world];
+        wc.ms += r->cost_ms;
+        wc.paths += (double)r->total_paths;
+        all_cost.ms += r->cost_ms;
+        all_cost.paths += (double)r->total_paths;
+        r->frame->accum_id = r->spp;
+        free_slots.push_back(r->slot);
+        r->slot = -1;
+        done++;
+        if (*num_done < max_done && done_tickets) done_tickets[*num_done] = r->ticket;
+        (*num_done)++;
+      } else {
+        still.push_back(r);
+      }
+    }
+    active.swap(still);
+  }
+
   // One generation for every request in flight.  The queue holds, per
   // active request, the rays its last shade wrote (region, live_rays); new
   // primary rays go after the regions, shortest remaining work first, while
@@ -889,16 +1124,7 @@ struct Server {
       in_flight += r->live_paths;
       r->added = 0;
     }
-    std::vector<Request *> order;
-    for (Request *r : active)
-      if (r->next_path < r->total_paths) order.push_back(r);
-    if (slack > 0.f)
-      std::stable_sort(order.begin(), order.end(),
-                       [](const Request *a, const Request *b) { return a->deadline_ms < b->deadline_ms; });
-    else
-      std::stable_sort(order.begin(), order.end(), [](const Request *a, const Request *b) {
-        return a->total_paths - a->next_path < b->total_paths - b->next_path;
-      });
+    const std::vector<Request *> order = topup_order();
     long long room = max_paths - in_flight;
     int nch = 0;
     long long added = 0;
@@ -925,7 +1151,8 @@ struct Server {
     // output regions in the other queue
     int *hb = F.h_out_base.as<int>();
     long long out_end = 0;
-    bool surfaces = false, volumes = false, full = false;
+    bool surfaces = false, volumes = false;
+    int full = 0;
     int max_slot = 0;
     for (Request *r : active) {
       hb[r->slot] = (int)out_end;
@@ -956,31 +1183,157 @@ struct Server {
     in_end = (int)out_end;
     const double tnow = now_ms();
     const double per_ray = (tnow - ts) / (double)n;
-    std::vector<Request *> still;
     for (Request *r : active) {
       r->cost_ms += per_ray * (double)(r->live_rays + r->added);
       r->region = hb[r->slot];
       r->live_rays = hcnt[2 * r->slot];
       r->live_paths = hcnt[2 * r->slot + 1];
-      if (r->next_path == r->total_paths && r->live_rays == 0) {
-        r->state = BNX_REQ_DONE;
-        r->done_ms = tnow;
-        PathCost &wc = world_cost[r->world];
-        wc.ms += r->cost_ms;
-        wc.paths += (double)r->total_paths;
-        all_cost.ms += r->cost_ms;
-        all_cost.paths += (double)r->total_paths;
-        r->frame->accum_id = r->spp;
-        free_slots.push_back(r->slot);
-        r->slot = -1;
-        done++;
-        if (*num_done < max_done && done_tickets) done_tickets[*num_done] = r->ticket;
-        (*num_done)++;
-      } else {
-        still.push_back(r);
-      }
     }
-    active.swap(still);
+    settle(tnow, done_tickets, max_done, num_done);
+    busy_ms += now_ms() - ts;
+    return BNX_OK;
+  }
+
+  int ensure_dd()
+  {
+    const size_t act = (size_t)max_active;
+    BNX_HIP(d_srv.ensure(sizeof(SrvSlot) * act));
+    BNX_HIP(h_srv.ensure(sizeof(SrvSlot) * act));
+    BNX_HIP(d_act.ensure(sizeof(int) * act));
+    BNX_HIP(h_act.ensure(sizeof(int) * act));
+    BNX_HIP(d_ord.ensure(sizeof(int) * act));
+    BNX_HIP(h_ord.ensure(sizeof(int) * act));
+    BNX_HIP(d_scal.ensure(sizeof(long long) * 4));
+    BNX_HIP(h_scal.ensure(sizeof(long long) * 4));
+    return F.ensure_dd();
+  }
+
+  // G generations for every request in flight with the step's bookkeeping
+  // on the device (k_srv_setup before each generation) and one read-back at
+  // the end: admission happens before the first generation only, the top-up
+  // runs before every one.  Launches are sized for the largest generation
+  // the step can hold (blocks past a generation's rays find none), from
+  // bounds the host knows: the first generation's old rays exactly, then at
+  // most two rays per path in flight, P = min(max_paths, paths in flight +
+  // primaries left) paths, plus the primaries a top-up can add.
+  int step_dd(int G, int64_t *done_tickets, int max_done, int *num_done)
+  {
+    const double ts = now_ms();
+    *num_done = 0;
+    int rc;
+    if ((rc = admit()) != BNX_OK) return rc;
+    if (active.empty()) return BNX_OK;
+    if ((rc = ensure_dd()) != BNX_OK) return rc;
+    const std::vector<Request *> order = topup_order();
+    SrvSlot *hs = h_srv.as<SrvSlot>();
+    int *ha = h_act.as<int>(), *ho = h_ord.as<int>();
+    long long *hsc = h_scal.as<long long>();
+    const int A = (int)active.size(), O = (int)order.size();
+    long long rays0 = 0, paths = 0, rem = 0;
+    bool surfaces = false, volumes = false, pooled = false, cut = false;
+    int full = 0;
+    int max_slot = 0;
+    for (int i = 0; i < A; i++) {
+      Request *r = active[i];
+      const int refill = trace_refill_for(dev, r->world);
+      ha[i] = r->slot;
+      hs[r->slot] = SrvSlot{r->next_path, r->total_paths, 0, r->region, r->live_rays, r->live_paths, refill, 0, 0};
+      rays0 += r->live_rays;
+      paths += r->live_paths;
+      rem += r->total_paths - r->next_path;
+      surfaces |= r->surfaces;
+      volumes |= r->volumes;
+      full |= r->full;
+      pooled |= refill < 64;
+      cut |= has_cut_plane(r->rr);
+      max_slot = std::max(max_slot, r->slot + 1);
+    }
+    for (int j = 0; j < O; j++) ho[j] = order[j]->slot;
+    hsc[0] = in_end;
+    hipStream_t st = dev->stream;
+    BNX_HIP(hipMemcpyAsync(d_srv.ptr, hs, sizeof(SrvSlot) * max_slot, hipMemcpyHostToDevice, st));
+    BNX_HIP(hipMemcpyAsync(d_act.ptr, ha, sizeof(int) * A, hipMemcpyHostToDevice, st));
+    if (O > 0) BNX_HIP(hipMemcpyAsync(d_ord.ptr, ho, sizeof(int) * O, hipMemcpyHostToDevice, st));
+    BNX_HIP(hipMemcpyAsync(d_scal.ptr, hsc, sizeof(long long), hipMemcpyHostToDevice, st));
+    const long long added_max = std::min(max_paths, rem);
+    const long long P = std::min(max_paths, paths + rem);
+    const int nsegs = A + O;
+    const int target = dev->trace_blocks_per_cu * dev->num_cus;
+    const SrvStep p{d_srv.as<SrvSlot>(), d_act.as<int>(), d_ord.as<int>(), A, O, F.segs.as<Segment>(),
+                    F.chunks.as<RaygenChunk>(), F.out_base.as<int>(), F.out_cnt.as<int>(),
+                    d_scal.as<long long>(), F.gen_counts.as<long long>(), max_paths, target};
+    const SegView sv{F.segs.as<Segment>(), F.blk256.as<int>(), F.blk1024.as<int>(), F.blkT.as<int>(),
+                     F.pool.as<int>()};
+    const SlotOut so{F.out_base.as<int>(), F.out_cnt.as<int>()};
+    const SlotDev *slots = F.slots.as<SlotDev>();
+    unsigned long long *stat = F.Q.stats.as<unsigned long long>();
+    const bool trace = surfaces || !volumes;
+    for (int g = 0; g < G; g++) {
+      const long long N = g == 0 ? rays0 + added_max : 2 * P + added_max;
+      const int blocks256 = (int)((N + 255) / 256) + nsegs, blocks1024 = (int)((N + 1023) / 1024) + nsegs;
+      const int blocksT = pooled ? std::min(target + nsegs, blocks256) : 0;
+      k_srv_setup<<<1, SRV_TB, 0, st>>>(p, g, 0);
+      BNX_HIP(hipGetLastError());
+      RayQueue in = F.Q.q(cur), out = F.Q.q(1 - cur);
+      if (added_max > 0 && O > 0 &&
+          (rc = launch_raygen_dev(dev, p.chunks, O, slots, p.scal + 1, added_max, in)) != BNX_OK)
+        return rc;
+      if ((rc = launch_segment_blocks(dev, sv, nsegs, blocks256, blocks1024, blocksT)) != BNX_OK) return rc;
+      TraceQueue tq{in.org, in.dir, in.hit, in.hit_v, in.rng, in.ps};
+      unsigned long long *tsp = F.stamps.as<unsigned long long>() + 4 * g;
+      if (trace) {
+        if ((rc = launch_stamp(dev, tsp)) != BNX_OK) return rc;
+        if ((rc = launch_trace_queue(dev, slots, sv, blocks256, blocksT, pooled, cut, tq, stat)) != BNX_OK)
+          return rc;
+        if ((rc = launch_stamp(dev, tsp + 1)) != BNX_OK) return rc;
+      }
+      if (volumes) {
+        if ((rc = launch_stamp(dev, tsp + 2)) != BNX_OK) return rc;
+        if ((rc = launch_volume_march(dev, slots, sv, blocks256, in, trace, full, stat)) != BNX_OK) return rc;
+        if ((rc = launch_stamp(dev, tsp + 3)) != BNX_OK) return rc;
+      }
+      if ((rc = launch_shade(dev, slots, sv, blocks256, blocks1024, in, out, so, stat)) != BNX_OK) return rc;
+      cur = 1 - cur;
+    }
+    k_srv_setup<<<1, SRV_TB, 0, st>>>(p, G, 1);
+    BNX_HIP(hipGetLastError());
+    long long *hg = F.h_gen_counts.as<long long>();
+    unsigned long long *hts = (unsigned long long *)(hg + MAX_DD_GENS);
+    BNX_HIP(hipMemcpyAsync(hs, d_srv.ptr, sizeof(SrvSlot) * max_slot, hipMemcpyDeviceToHost, st));
+    BNX_HIP(hipMemcpyAsync(hsc, d_scal.ptr, sizeof(long long), hipMemcpyDeviceToHost, st));
+    BNX_HIP(hipMemcpyAsync(hg, F.gen_counts.ptr, sizeof(long long) * G, hipMemcpyDeviceToHost, st));
+    BNX_HIP(hipMemcpyAsync(hts, F.stamps.ptr, sizeof(unsigned long long) * 4 * G, hipMemcpyDeviceToHost, st));
+    BNX_HIP(hipStreamSynchronize(st));
+    BNX_HIP(hipGetLastError());
+    const double tick_ms = 1.0 / (double)dev->wall_clock_khz;
+    long long n = 0;
+    for (int g = 0; g < G; g++) {
+      if (trace) {
+        tm.trace_ms += (double)(hts[4 * g + 1] - hts[4 * g]) * tick_ms;
+        tm.trace_launches++;
+      }
+      if (volumes) {
+        tm.march_ms += (double)(hts[4 * g + 3] - hts[4 * g + 2]) * tick_ms;
+        tm.march_launches++;
+        tm.march_rays += hg[g];
+      }
+      n += hg[g];
+      gens += hg[g] > 0;
+      peak = std::max<int64_t>(peak, hg[g]);
+    }
+    rays += n;
+    in_end = (int)hsc[0];
+    const double tnow = now_ms();
+    const double per_ray = n > 0 ? (tnow - ts) / (double)n : 0.0;
+    for (Request *r : active) {
+      const SrvSlot &q = hs[r->slot];
+      r->cost_ms += per_ray * (double)q.rays;
+      r->next_path = q.next;
+      r->region = q.region;
+      r->live_rays = q.live_rays;
+      r->live_paths = q.live_paths;
+    }
+    settle(tnow, done_tickets, max_done, num_done);
     busy_ms += now_ms() - ts;
     return BNX_OK;
   }
@@ -1076,6 +1429,16 @@ int bnx_server_step(bnx_server s, int64_t *done_tickets, int max_done, int *num_
   GUARD_BEGIN
   BNX_CHECK(s && num_done && max_done >= 0, BNX_ERR_INVALID, "bnx_server_step: bad argument");
   return s->step(done_tickets, max_done, num_done);
+  GUARD_END
+}
+
+int bnx_server_step_n(bnx_server s, int generations, int64_t *done_tickets, int max_done, int *num_done)
+{
+  GUARD_BEGIN
+  BNX_CHECK(s && num_done && max_done >= 0, BNX_ERR_INVALID, "bnx_server_step_n: bad argument");
+  BNX_CHECK(generations >= 1 && generations <= MAX_DD_GENS, BNX_ERR_INVALID,
+            "bnx_server_step_n: generations not in 1..16");
+  return s->step_dd(generations, done_tickets, max_done, num_done);
   GUARD_END
 }
 

@@ -174,6 +174,11 @@ int frame_read(Frame *f, int channel, void *host);
 // primary rays for total queue entries starting at queue index q_begin
 int launch_raygen(Device *dev, const RaygenChunk *d_chunks, int num_chunks, const SlotDev *d_slots,
                   long long q_begin, long long total, const RayQueue &q);
+// the same with q_begin = d_range[0] and total = d_range[1] read on the
+// device (a generation laid out by an earlier kernel); max_total bounds the
+// launch
+int launch_raygen_dev(Device *dev, const RaygenChunk *d_chunks, int num_chunks, const SlotDev *d_slots,
+                      const long long *d_range, long long max_total, const RayQueue &q);
 // a device-driven generation's single segment from the previous shade's
 // output counter (count0 when g == 0); clears the counter, keeps the count
 // in gen_counts[g]

@@ -66,9 +66,14 @@ BNX_D void tile_pixel(int pix, int W, int H, int tw, int th, int &ix, int &iy)
   iy = y0 + loc / w;
 }
 
+// range: q_begin and total from device memory instead of the arguments
 __global__ void k_raygen(const RaygenChunk *chunks, int num_chunks, const SlotDev *slots, long long q_begin,
-                         long long total, RayQueue q, int tile_w, int tile_h)
+                         long long total, const long long *range, RayQueue q, int tile_w, int tile_h)
 {
+  if (range) {
+    q_begin = range[0];
+    total = range[1];
+  }
   const long long idx = q_begin + (long long)blockIdx.x * blockDim.x + threadIdx.x;
   if (idx >= q_begin + total) return;
   int lo = 0, hi = num_chunks - 1;
@@ -938,8 +943,18 @@ int launch_raygen(Device *dev, const RaygenChunk *chunks, int num_chunks, const 
                   long long total, const RayQueue &q)
 {
   if (total > 0 && num_chunks > 0)
-    k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(chunks, num_chunks, slots, q_begin, total, q,
+    k_raygen<<<nblk(total, 256), 256, 0, dev->stream>>>(chunks, num_chunks, slots, q_begin, total, nullptr, q,
                                                          dev->raygen_tile_w, dev->raygen_tile_h);
+  BNX_HIP(hipGetLastError());
+  return BNX_OK;
+}
+
+int launch_raygen_dev(Device *dev, const RaygenChunk *chunks, int num_chunks, const SlotDev *slots,
+                      const long long *range, long long max_total, const RayQueue &q)
+{
+  if (max_total > 0 && num_chunks > 0)
+    k_raygen<<<nblk(max_total, 256), 256, 0, dev->stream>>>(chunks, num_chunks, slots, 0, 0, range, q,
+                                                             dev->raygen_tile_w, dev->raygen_tile_h);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

@@ -298,37 +298,6 @@ BNX_D float child_entry(v3 o, v3 inv, float lx, float hx, float ly, float hy, fl
   return h ? tn : __builtin_huge_valf();
 }
 
-#if BNX_OCTANT
-// The byte offsets in a Node4 of the near-plane vectors along each axis for
-// the ray's direction signs (lo.x at 0 or hi.x at 16, lo.y at 32 or hi.y at
-// 48, lo.z at 64 or hi.z at 80); the far plane is the other one of the pair
-// (offset ^ 16).  For a valid box (lo <= hi) the near and far distances are
-// exactly the min and max of the two slab distances, so the walk is the same
-// as with min/max slab tests; an empty slot (lo = +inf, hi = -inf) gets an
-// infinite near distance on every axis and is never entered.
-struct Oct {
-  uint32_t x, y, z;
-};
-BNX_D Oct ray_oct(v3 inv) { return Oct{inv.x < 0.f ? 16u : 0u, inv.y < 0.f ? 48u : 32u, inv.z < 0.f ? 80u : 64u}; }
-BNX_D void node_entries(const Node4 *node, v3 o, v3 inv, Oct oc, float tmin, float tmax, float &t0, float &t1,
-                        float &t2, float &t3, uint4 &ch)
-{
-  const char *np = (const char *)node;
-  const float4 nx = *(const float4 *)(np + oc.x), fx = *(const float4 *)(np + (oc.x ^ 16u));
-  const float4 ny = *(const float4 *)(np + oc.y), fy = *(const float4 *)(np + (oc.y ^ 16u));
-  const float4 nz = *(const float4 *)(np + oc.z), fz = *(const float4 *)(np + (oc.z ^ 16u));
-  ch = ((const uint4 *)np)[6];
-  auto entry = [&](float nxk, float fxk, float nyk, float fyk, float nzk, float fzk) {
-    const float tn = fmaxf(fmaxf(fmaxf(tmin, (nxk - o.x) * inv.x), (nyk - o.y) * inv.y), (nzk - o.z) * inv.z);
-    const float tf = fminf(fminf(fminf(tmax, (fxk - o.x) * inv.x), (fyk - o.y) * inv.y), (fzk - o.z) * inv.z);
-    return tn <= tf * 1.00001f ? tn : __builtin_huge_valf();
-  };
-  t0 = entry(nx.x, fx.x, ny.x, fy.x, nz.x, fz.x);
-  t1 = entry(nx.y, fx.y, ny.y, fy.y, nz.y, fz.y);
-  t2 = entry(nx.z, fx.z, ny.z, fy.z, nz.z, fz.z);
-  t3 = entry(nx.w, fx.w, ny.w, fy.w, nz.w, fz.w);
-}
-#endif
 
 BNX_D void cswap(float &ta, uint32_t &ca, float &tb, uint32_t &cb)
 {
@@ -369,9 +338,6 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
   HitRec best{tmax, 0.f, 0.f, -1, -1, -1};
   v3 o, d, inv;
   world_ray(rorg, rdir, o, d, inv);
-#if BNX_OCTANT
-  Oct oc = ray_oct(inv);
-#endif
   const Node4 *nodes = S.tlas_nodes;
   BlasDev blas{};
   bool in_blas = false;
@@ -385,11 +351,6 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
     if (cur < EMPTY4) {
       // ------------------------------------------------ 4-wide inner node
       nvis++;
-#if BNX_OCTANT
-      float t0, t1, t2, t3;
-      uint4 ch;
-      node_entries(nodes + cur, o, inv, oc, tmin, best.t, t0, t1, t2, t3, ch);
-#else
       const float4 *np = (const float4 *)(nodes + cur);
       const float4 lx = np[0], hx = np[1], ly = np[2], hy = np[3], lz = np[4], hz = np[5];
       const uint4 ch = ((const uint4 *)np)[6];
@@ -397,7 +358,6 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       float t1 = child_entry(o, inv, lx.y, hx.y, ly.y, hy.y, lz.y, hz.y, ch.y, tmin, best.t);
       float t2 = child_entry(o, inv, lx.z, hx.z, ly.z, hy.z, lz.z, hz.z, ch.z, tmin, best.t);
       float t3 = child_entry(o, inv, lx.w, hx.w, ly.w, hy.w, lz.w, hz.w, ch.w, tmin, best.t);
-#endif
       uint32_t c0 = ch.x, c1 = ch.y, c2 = ch.z, c3 = ch.w;
       // nearest first: a 4-element sorting network on the entry distances
       cswap(t0, c0, t1, c1);
@@ -415,12 +375,7 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
       }
     } else if (cur == LEAVE_BLAS) {
       in_blas = false;
-      if (XF) {
-        world_ray(rorg, rdir, o, d, inv);
-#if BNX_OCTANT
-        oc = ray_oct(inv);
-#endif
-      }
+      if (XF) world_ray(rorg, rdir, o, d, inv);
       nodes = S.tlas_nodes;
     } else {
       // ------------------------------------------------------------ leaf
@@ -434,12 +389,7 @@ BNX_D HitRec traverse(const SceneDev &S, const float4 *rorg, const float4 *rdir,
           cut.geom_base = I.geom_base;
           cut.translucent = blas.translucent != 0;
         }
-        if (XF && !I.identity) {
-          instance_ray(I, rorg, rdir, o, d, inv);
-#if BNX_OCTANT
-          oc = ray_oct(inv);
-#endif
-        }
+        if (XF && !I.identity) instance_ray(I, rorg, rdir, o, d, inv);
         push(LEAVE_BLAS);
         in_blas = true;
         nodes = blas.nodes;

@@ -40,6 +40,10 @@ struct VolumeDev {
   const float4 BNX_G *uverts;   // position, scalar
   const int4 BNX_G *utets;      // vertex indices, in element-BVH slot order
   const int4 BNX_G *unbr;       // per tet slot: the slot across the face opposite vertex k (-1: boundary)
+  // meshes with other cells than tets: per slot the cell type (CELL_*) and
+  // vertex indices 4..7 (0..3 in utets); null for tets only
+  const uint8_t BNX_G *utypes;
+  const int4 BNX_G *uext;
   const Node BNX_G *ebvh;       // element BVH
   // seed grid over the bounds: per cell the tet slot holding its centre (-1:
   // none), where a point query with no nearby previous tet starts its walk
@@ -84,8 +88,8 @@ struct Volume {
   uint64_t version = 0;         // bumped whenever dd changes
   VolumeDev dd{};
   DevMem scalars, bricks, ranges, majorants, tf;
-  DevMem uverts, utets, unbr, useed;
-  Bvh ebvh;                     // element BVH (tets)
+  DevMem uverts, utets, unbr, useed, utypes, uext;
+  Bvh ebvh;                     // element BVH (cells)
 };
 
 struct IsoSurface {
@@ -100,9 +104,12 @@ int volume_build(Volume *v);
 // the structured field's bricked copy (VolumeDev::bricks) with bricks of
 // 2^shift cells; shift 0 keeps the linear layout only
 int volume_brick(Volume *v, int shift);
-// tets: element BVH, macrocell ranges by rasterTet<5>, majorants; tets4 is
-// the host copy of the tets (num_tets x int4), verts4 of the vertices
-int volume_build_umesh(Volume *v, const int4 *tets4, int num_tets, const float4 *verts4, int num_verts);
+// cells: element BVH, macrocell ranges (rasterTet<5> for tets, the cell's
+// box for the others), majorants.  tets4: the host copy of the cells'
+// vertex indices 0..3 (num_cells x int4), ext4 of 4..7 and types the cell
+// types (CELL_*; both null when every cell is a tet), verts4 of the vertices
+int volume_build_umesh(Volume *v, const int4 *tets4, const int4 *ext4, const uint8_t *types, int num_cells,
+                       const float4 *verts4, int num_verts);
 int volume_rebuild_majorants(Volume *v);
 // upload the world's volume and isosurface tables (if changed) and return them
 int world_volumes(World *w, VolumeList &vl);
@@ -110,10 +117,18 @@ int world_isos(World *w, const struct IsoDev BNX_G *&isos, int &count);
 // march every ray of the generation's segments (blocks256 blocks of 256
 // threads) through its request's isosurfaces and volumes; surfaces: false
 // when the trace kernel did not run, so the hit records are not written yet
-// full: some request of the batch has isosurfaces or tet volumes (else
-// every volume is structured and a leaner kernel runs)
+// full: march_kind bits of the batch's volumes and isosurface fields
+// the march kernel a batch needs: the lean one (structured volumes only),
+// MARCH_FULL (isosurfaces or unstructured volumes) or, with MARCH_CELLS
+// also set, the one whose point queries handle other cells than tets
+enum { MARCH_FULL = 1, MARCH_CELLS = 2 };
+inline int march_kind(const Volume *v)
+{
+  if (v->dd.field == FIELD_STRUCTURED) return 0;
+  return MARCH_FULL | (v->dd.utypes ? MARCH_CELLS : 0);
+}
 int launch_volume_march(Device *dev, const SlotDev *d_slots, const SegView &sv, int blocks256, const RayQueue &q,
-                        bool surfaces, bool full, unsigned long long *stats);
+                        bool surfaces, int full, unsigned long long *stats);
 
 // ---------------------------------------------------------------- device
 
@@ -210,6 +225,156 @@ BNX_D bool tet_scalar(const VolumeDev &v, int slot, v3 P, float &out)
   return true;
 }
 
+// ------------------------------------------------------ isoparametric cells
+// Pyramids, prisms and hexahedra (UMeshField::DD::eltScalar and pyrScalar /
+// prismScalar / hexScalar, umesh/common/UMeshField.h:372-389, 438-495, with
+// the Newton inversions of ElementIntersection.h:77-476): the cell maps
+// parametric (r, s, t) to x(r, s, t) = sum_i N_i X_i over its vertices X_i
+// in VTK order.  P is mapped back by Newton's method from (.5, .5, .5): at
+// most 10 steps, each solving J d = x - P by Cramer's rule; converged when
+// every |d_k| < 1e-4; abandoned when |det J| falls below the cell's
+// tolerance or a coordinate leaves [-1e6, 1e6].  P is inside when the
+// coordinates lie in the parameter domain, widened by 1e-6; the value is
+// sum_i N_i f_i with the N_i of the last step's starting point.
+enum { CELL_TET = 0, CELL_HEX = 1, CELL_PRISM = 2, CELL_PYR = 3 };
+constexpr float CELL_IN_LO = 0.f - 1e-6f, CELL_IN_HI = 1.f + 1e-6f;
+
+template <int TYPE> struct CellShape;
+// trilinear on the unit cube
+template <> struct CellShape<CELL_HEX> {
+  static constexpr int N = 8;
+  BNX_D static void eval(float r, float s, float t, float *w, float *dr, float *ds, float *dt)
+  {
+    const float rm = 1.f - r, sm = 1.f - s, tm = 1.f - t;
+    w[0] = rm * sm * tm; w[1] = r * sm * tm; w[2] = r * s * tm; w[3] = rm * s * tm;
+    w[4] = rm * sm * t;  w[5] = r * sm * t;  w[6] = r * s * t;  w[7] = rm * s * t;
+    dr[0] = -sm * tm; dr[1] = sm * tm; dr[2] = s * tm; dr[3] = -s * tm;
+    dr[4] = -sm * t;  dr[5] = sm * t;  dr[6] = s * t;  dr[7] = -s * t;
+    ds[0] = -rm * tm; ds[1] = -r * tm; ds[2] = r * tm; ds[3] = rm * tm;
+    ds[4] = -rm * t;  ds[5] = -r * t;  ds[6] = r * t;  ds[7] = rm * t;
+    dt[0] = -rm * sm; dt[1] = -r * sm; dt[2] = -r * s; dt[3] = -rm * s;
+    dt[4] = rm * sm;  dt[5] = r * sm;  dt[6] = r * s;  dt[7] = rm * s;
+  }
+  BNX_D static bool inside(float r, float s, float t)
+  {
+    return r >= CELL_IN_LO && r <= CELL_IN_HI && s >= CELL_IN_LO && s <= CELL_IN_HI && t >= CELL_IN_LO &&
+           t <= CELL_IN_HI;
+  }
+};
+// linear triangle (r, s) times linear t
+template <> struct CellShape<CELL_PRISM> {
+  static constexpr int N = 6;
+  BNX_D static void eval(float r, float s, float t, float *w, float *dr, float *ds, float *dt)
+  {
+    w[0] = (1.f - r - s) * (1.f - t); w[1] = r * (1.f - t); w[2] = s * (1.f - t);
+    w[3] = (1.f - r - s) * t;         w[4] = r * t;         w[5] = s * t;
+    dr[0] = -1.f + t; dr[1] = 1.f - t; dr[2] = 0.f; dr[3] = -t; dr[4] = t;   dr[5] = 0.f;
+    ds[0] = -1.f + t; ds[1] = 0.f;     ds[2] = 1.f - t; ds[3] = -t; ds[4] = 0.f; ds[5] = t;
+    dt[0] = -1.f + r + s; dt[1] = -r; dt[2] = -s; dt[3] = 1.f - r - s; dt[4] = r; dt[5] = s;
+  }
+  BNX_D static bool inside(float r, float s, float t)
+  {
+    return r >= CELL_IN_LO && r <= CELL_IN_HI && s >= CELL_IN_LO && s <= CELL_IN_HI && t >= CELL_IN_LO &&
+           t <= CELL_IN_HI && r + s <= CELL_IN_HI;
+  }
+};
+// bilinear base quad collapsing linearly in t onto the apex (vertex 4)
+template <> struct CellShape<CELL_PYR> {
+  static constexpr int N = 5;
+  BNX_D static void eval(float r, float s, float t, float *w, float *dr, float *ds, float *dt)
+  {
+    const float rm = 1.f - r, sm = 1.f - s, tm = 1.f - t;
+    w[0] = rm * sm * tm; w[1] = r * sm * tm; w[2] = r * s * tm; w[3] = rm * s * tm; w[4] = t;
+    dr[0] = -(s - 1.f) * (t - 1.f); dr[1] = (s - 1.f) * (t - 1.f); dr[2] = s - s * t; dr[3] = s * (t - 1.f);
+    dr[4] = 0.f;
+    ds[0] = -(r - 1.f) * (t - 1.f); ds[1] = r * (t - 1.f); ds[2] = r - r * t; ds[3] = (r - 1.f) * (t - 1.f);
+    ds[4] = 0.f;
+    dt[0] = -(r - 1.f) * (s - 1.f); dt[1] = r * (s - 1.f); dt[2] = -r * s; dt[3] = (r - 1.f) * s; dt[4] = 1.f;
+  }
+  BNX_D static bool inside(float r, float s, float t) { return CellShape<CELL_HEX>::inside(r, s, t); }
+};
+
+// det [a b c] (columns) = a . (b x c)
+BNX_D float det3(v3 a, v3 b, v3 c) { return dot(a, cross(b, c)); }
+
+template <int TYPE> BNX_D bool cell_invert(const float4 *V, v3 P, float tol, float &value)
+{
+  using C = CellShape<TYPE>;
+  float r = .5f, s = .5f, t = .5f;
+  float w[C::N], dr[C::N], ds[C::N], dt[C::N];
+  bool converged = false;
+  for (int it = 0; it < 10 && !converged; it++) {
+    C::eval(r, s, t, w, dr, ds, dt);
+    v3 f{0.f, 0.f, 0.f}, cr{0.f, 0.f, 0.f}, cs{0.f, 0.f, 0.f}, ct{0.f, 0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < C::N; i++) {
+      const v3 X = xyz(V[i]);
+      f = f + X * w[i];
+      cr = cr + X * dr[i];
+      cs = cs + X * ds[i];
+      ct = ct + X * dt[i];
+    }
+    f = f - P;
+    const float d = det3(cr, cs, ct);
+    if (fabsf(d) < tol) return false;
+    const float d0 = det3(f, cs, ct) / d;
+    const float d1 = det3(cr, f, ct) / d;
+    const float d2 = det3(cr, cs, f) / d;
+    r = r - d0;
+    s = s - d1;
+    t = t - d2;
+    if (fabsf(d0) < 1e-4f && fabsf(d1) < 1e-4f && fabsf(d2) < 1e-4f) converged = true;
+    else if (fabsf(r) > 1e6f || fabsf(s) > 1e6f || fabsf(t) > 1e6f) return false;
+  }
+  if (!converged || !C::inside(r, s, t)) return false;
+  float val = 0.f;
+#pragma unroll
+  for (int i = 0; i < C::N; i++) val += w[i] * V[i].w;
+  value = val;
+  return true;
+}
+
+struct CellHit {
+  float value;
+  int inside;
+};
+// a pyramid, prism or hexahedron at slot: vertex indices 0..3 in utets,
+// 4..7 in uext.  Kept out of line: the tet-only march never calls it, and
+// its registers then stay out of that kernel's budget.
+BNX_D CellHit cell_scalar(const float4 BNX_G *uverts, const int4 BNX_G *utets,
+                                                                const int4 BNX_G *uext, int type, int slot, v3 P)
+{
+  const int4 a = utets[slot], b = uext[slot];
+  float4 V[8];
+  V[0] = uverts[a.x];
+  V[1] = uverts[a.y];
+  V[2] = uverts[a.z];
+  V[3] = uverts[a.w];
+  V[4] = uverts[b.x];
+  CellHit h{0.f, 0};
+  bool in;
+  if (type == CELL_PYR) {
+    in = cell_invert<CELL_PYR>(V, P, 1e-6f, h.value);
+  } else {
+    V[5] = uverts[b.y];
+    if (type == CELL_PRISM) {
+      in = cell_invert<CELL_PRISM>(V, P, 1e-6f, h.value);
+    } else {
+      V[6] = uverts[b.z];
+      V[7] = uverts[b.w];
+      // hexahedra: the determinant tolerance scales with the cell's size
+      v3 lo = xyz(V[0]), hi = lo;
+      for (int i = 1; i < 8; i++) {
+        lo = v3{fmn(lo.x, V[i].x), fmn(lo.y, V[i].y), fmn(lo.z, V[i].z)};
+        hi = v3{fmx(hi.x, V[i].x), fmx(hi.y, V[i].y), fmx(hi.z, V[i].z)};
+      }
+      in = cell_invert<CELL_HEX>(V, P, 1e-10f * length(hi - lo), h.value);
+    }
+  }
+  h.inside = in ? 1 : 0;
+  return h;
+}
+
 // closed box test, branch-free (bitwise and) so that a node's loads are
 // issued together rather than behind each other
 BNX_D bool box_holds(float lx, float ly, float lz, float hx, float hy, float hz, v3 P)
@@ -283,7 +448,8 @@ BNX_D bool umesh_walk(const VolumeDev &v, int cur, v3 P, MarchCtx &cx, float &va
 // UMeshCuBQLSampler::sample (umesh/mc/UMeshCuBQLSampler.h:46-62): the first
 // tet, depth first through the element BVH, that contains P; its slot (-1:
 // none, the value is then NaN)
-BNX_D int umesh_dfs(const VolumeDev &v, v3 P, MarchCtx &cx, float &value)
+// CELLS: the mesh may hold other cells than tets (else every cell is one)
+template <bool CELLS = false> BNX_D int umesh_dfs(const VolumeDev &v, v3 P, MarchCtx &cx, float &value)
 {
   uint32_t deep[PQ_DEEP];
   int sp = 0;
@@ -313,7 +479,16 @@ BNX_D int umesh_dfs(const VolumeDev &v, v3 P, MarchCtx &cx, float &value)
       for (int k = 0; k < count; k++) {
         float f;
         cx.utets++;
-        if (tet_scalar(v, first + k, P, f)) {
+        bool in;
+        const int type = CELLS && v.utypes ? (int)v.utypes[first + k] : CELL_TET;
+        if (type == CELL_TET) {
+          in = tet_scalar(v, first + k, P, f);
+        } else {
+          const CellHit h = cell_scalar(v.uverts, v.utets, v.uext, type, first + k, P);
+          in = h.inside != 0;
+          f = h.value;
+        }
+        if (in) {
           value = f;
           return first + k;
         }
@@ -345,10 +520,16 @@ BNX_D int seed_cell(const VolumeDev &v, v3 P)
 // A ray's consecutive queries are close together: walk from the tet of its
 // previous query; a query with none nearby walks from the seed of its grid
 // cell; the depth-first query decides what neither walk settles.
-BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
+template <bool CELLS = false> BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
 {
   float value;
   int t;
+  if (CELLS && v.utypes) {
+    // other cells than tets: the depth-first query alone, no walk (a kernel
+    // compiled for tets only is never launched with such a field)
+    (void)umesh_dfs<CELLS>(v, P, cx, value);
+    return value;
+  }
   const bool have_last = cx.last_field == v.utets && cx.last_tet >= 0;
   if (have_last && umesh_walk(v, cx.last_tet, P, cx, value, t)) {
     cx.last_tet = t;
@@ -370,9 +551,9 @@ BNX_D float umesh_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
   return value;
 }
 
-BNX_D float volume_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
+template <bool CELLS = false> BNX_D float volume_sample(const VolumeDev &v, v3 P, MarchCtx &cx)
 {
-  return v.field == FIELD_UMESH ? umesh_sample(v, P, cx) : structured_sample(v, P);
+  return v.field == FIELD_UMESH ? umesh_sample<CELLS>(v, P, cx) : structured_sample(v, P);
 }
 
 // TransferFunction::DD::map (volume/TransferFunction.h:67-87)
@@ -526,8 +707,9 @@ BNX_D uint64_t rng_child(uint64_t &parent, uint32_t rtc_inst)
 // MCVolumeAccel::isProg (volume/MCAccelerator.h:550-689): box clip, RNG
 // child seeded by hash(rtc instance, geometry 0, 0), dda3 over macrocells,
 // Woodcock sampling per non-empty cell.  'dir' is the fixed trace direction.
-// ANY_FIELD: the field may be a tet mesh (else it is structured).
-template <bool ANY_FIELD>
+// ANY_FIELD: the field may be unstructured (else it is structured); CELLS:
+// an unstructured field may hold other cells than tets.
+template <bool ANY_FIELD, bool CELLS = false>
 BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, float &t_out,
                         float rgb[3], MarchCtx &cx)
 {
@@ -551,7 +733,7 @@ BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
       t += dt;
       if (t >= upper) return true;
       v3 P = org + t * dir;
-      float f = ANY_FIELD ? volume_sample(v, P, cx) : structured_sample(v, P);
+      float f = ANY_FIELD ? volume_sample<CELLS>(v, P, cx) : structured_sample(v, P);
       cx.samples++;
       float4 s = isnan(f) ? make_float4(0.f, 0.f, 0.f, 0.f) : tf_map(v, f);
       if (s.w >= rnd() * maj) {
@@ -578,7 +760,7 @@ BNX_D bool volume_march(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
 // [tmin, tmax], as the expected value of delta tracking's 0/1 outcome is.
 // The reference has only delta tracking (volume/Volume.h:168-203): this is
 // an opt-in estimator (BNX_RENDER_RATIO_TRACKING).
-template <bool ANY_FIELD>
+template <bool ANY_FIELD, bool CELLS = false>
 BNX_D float ratio_track(const VolumeDev &v, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, MarchCtx &cx)
 {
   if (!ANY_FIELD && v.field != FIELD_STRUCTURED) return 1.f;
@@ -599,7 +781,7 @@ BNX_D float ratio_track(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
       t += -fast_log(1.f - r) / maj;
       if (t >= upper) return true;
       const v3 P = org + t * dir;
-      const float f = ANY_FIELD ? volume_sample(v, P, cx) : structured_sample(v, P);
+      const float f = ANY_FIELD ? volume_sample<CELLS>(v, P, cx) : structured_sample(v, P);
       cx.samples++;
       const float density = isnan(f) ? 0.f : tf_map(v, f).w;
       T *= fmx(0.f, 1.f - density / maj);
@@ -616,6 +798,7 @@ BNX_D float ratio_track(const VolumeDev &v, v3 org, v3 dir, float tmin, float tm
 // = 0.1 |bounds| / (sum of grid dims).  Advances the parent RNG once when
 // the ray overlaps the field's bounds.  A crossing whose interpolation is
 // NaN (both ends exactly at the iso value) is ignored.
+template <bool CELLS = false>
 BNX_D bool iso_march(const IsoDev &s, v3 org, v3 dir, float tmin, float tmax, uint64_t &parent, float &t_out,
                      v3 &normal, MarchCtx &cx)
 {
@@ -632,13 +815,13 @@ BNX_D bool iso_march(const IsoDev &s, v3 org, v3 dir, float tmin, float tmax, ui
     const float2 vr = v.ranges[ci];
     if (!(iso >= vr.x && iso <= vr.y)) return true;
     float tt1 = t0;
-    float ff1 = volume_sample(v, org + tt1 * dir, cx);
+    float ff1 = volume_sample<CELLS>(v, org + tt1 * dir, cx);
     for (int i = 1; i <= 10; i++) {
       const float tt0 = tt1;
       const float ff0 = ff1;
       const float fi = (float)i / 10.f;
       tt1 = (1.f - fi) * t0 + fi * t1;
-      ff1 = volume_sample(v, org + tt1 * dir, cx);
+      ff1 = volume_sample<CELLS>(v, org + tt1 * dir, cx);
       if (isnan(ff0) || isnan(ff1)) continue;
       const float lo = fmn(ff0, ff1), hi = fmx(ff0, ff1);
       if (iso >= lo && iso <= hi) {
@@ -654,13 +837,13 @@ BNX_D bool iso_march(const IsoDev &s, v3 org, v3 dir, float tmin, float tmax, ui
   const v3 P = org + t_hit * dir;
   const v3 bs = v3{v.hi[0] - v.lo[0], v.hi[1] - v.lo[1], v.hi[2] - v.lo[2]};
   const float delta = length(bs) * .1f / (float)(v.mc[0] + v.mc[1] + v.mc[2]);
-  const float fP = volume_sample(v, P, cx);
-  float fx0 = volume_sample(v, P + v3{-delta, 0.f, 0.f}, cx);
-  float fx1 = volume_sample(v, P + v3{+delta, 0.f, 0.f}, cx);
-  float fy0 = volume_sample(v, P + v3{0.f, -delta, 0.f}, cx);
-  float fy1 = volume_sample(v, P + v3{0.f, +delta, 0.f}, cx);
-  float fz0 = volume_sample(v, P + v3{0.f, 0.f, -delta}, cx);
-  float fz1 = volume_sample(v, P + v3{0.f, 0.f, +delta}, cx);
+  const float fP = volume_sample<CELLS>(v, P, cx);
+  float fx0 = volume_sample<CELLS>(v, P + v3{-delta, 0.f, 0.f}, cx);
+  float fx1 = volume_sample<CELLS>(v, P + v3{+delta, 0.f, 0.f}, cx);
+  float fy0 = volume_sample<CELLS>(v, P + v3{0.f, -delta, 0.f}, cx);
+  float fy1 = volume_sample<CELLS>(v, P + v3{0.f, +delta, 0.f}, cx);
+  float fz0 = volume_sample<CELLS>(v, P + v3{0.f, 0.f, -delta}, cx);
+  float fz1 = volume_sample<CELLS>(v, P + v3{0.f, 0.f, +delta}, cx);
   float dx = 2.f, dy = 2.f, dz = 2.f;
   if (isnan(fx0)) { dx -= 1.f; fx0 = fP; }
   if (isnan(fx1)) { dx -= 1.f; fx1 = fP; }

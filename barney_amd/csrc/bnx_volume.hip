@@ -70,25 +70,57 @@ __global__ void k_majorants(VolumeDev v, const float2 *ranges, float *maj)
   maj[tid] = tf_majorant(v, r.x, r.y);
 }
 
-// ------------------------------------------------------------------ tets
-// world-space box of every tet (element-BVH build input)
-__global__ void k_tet_boxes(const int4 *tets, const float4 *verts, int n, float4 *lo, float4 *hi)
+// ------------------------------------------------------------------ cells
+BNX_D int cell_nverts(int type) { return type == CELL_HEX ? 8 : (type == CELL_PRISM ? 6 : (type == CELL_PYR ? 5 : 4)); }
+
+// the vertices (position, scalar) of cell c: k < cell_nverts
+BNX_D float4 cell_vertex(const int4 *tets, const int4 *ext, const float4 *verts, int c, int k)
+{
+  const int4 q = k < 4 ? tets[c] : ext[c];
+  const int j = k & 3;
+  return verts[j == 0 ? q.x : (j == 1 ? q.y : (j == 2 ? q.z : q.w))];
+}
+
+// cellBounds (UMeshField.h:270-345): box of the cell's vertices, position
+// in xyz and scalar range in w
+BNX_D void cell_box(const int4 *tets, const int4 *ext, const uint8_t *types, const float4 *verts, int c, float4 &lo,
+                    float4 &hi)
+{
+  const int nv = types ? cell_nverts(types[c]) : 4;
+  lo = make_float4(INFINITY, INFINITY, INFINITY, INFINITY);
+  hi = make_float4(-INFINITY, -INFINITY, -INFINITY, -INFINITY);
+  for (int k = 0; k < nv; k++) {
+    const float4 p = cell_vertex(tets, ext, verts, c, k);
+    lo = make_float4(fmn(lo.x, p.x), fmn(lo.y, p.y), fmn(lo.z, p.z), fmn(lo.w, p.w));
+    hi = make_float4(fmx(hi.x, p.x), fmx(hi.y, p.y), fmx(hi.z, p.z), fmx(hi.w, p.w));
+  }
+}
+
+// world-space box of every cell (element-BVH build input); ext and types
+// null: every cell is a tet
+__global__ void k_cell_boxes(const int4 *tets, const int4 *ext, const uint8_t *types, const float4 *verts, int n,
+                             float4 *lo, float4 *hi)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
-  const int4 q = tets[t];
-  const float4 a = verts[q.x], b = verts[q.y], c = verts[q.z], d = verts[q.w];
-  lo[t] = make_float4(fmn(fmn(fmn(a.x, b.x), c.x), d.x), fmn(fmn(fmn(a.y, b.y), c.y), d.y),
-                      fmn(fmn(fmn(a.z, b.z), c.z), d.z), 0.f);
-  hi[t] = make_float4(fmx(fmx(fmx(a.x, b.x), c.x), d.x), fmx(fmx(fmx(a.y, b.y), c.y), d.y),
-                      fmx(fmx(fmx(a.z, b.z), c.z), d.z), 0.f);
+  float4 l, h;
+  cell_box(tets, ext, types, verts, t, l, h);
+  lo[t] = make_float4(l.x, l.y, l.z, 0.f);
+  hi[t] = make_float4(h.x, h.y, h.z, 0.f);
 }
 
-// tets in element-BVH slot order
-__global__ void k_tet_sort(const int4 *tets, const int *order, int n, int4 *out)
+// cells in element-BVH slot order
+__global__ void k_cell_sort(const int4 *tets, const int4 *ext, const uint8_t *types, const int *order, int n,
+                            int4 *tets_out, int4 *ext_out, uint8_t *types_out)
 {
   const int s = blockIdx.x * blockDim.x + threadIdx.x;
-  if (s < n) out[s] = tets[order[s]];
+  if (s >= n) return;
+  const int c = order[s];
+  tets_out[s] = tets[c];
+  if (types) {
+    ext_out[s] = ext[c];
+    types_out[s] = types[c];
+  }
 }
 
 __global__ void k_fill_ranges(int *lo, int *hi, int n)
@@ -142,10 +174,37 @@ struct RangeCache {
 // the box of each of the 32 pieces is rasterized with rasterBox
 // (volume/MCGrid.cuh:95-122), which skips boxes flat in any dimension.
 // Piece p is reached by taking child (p >> (4 - level)) & 1 at every level.
-__global__ void k_umesh_raster(const int4 *tets, const float4 *verts, int n, VolumeDev v, int *rlo, int *rhi)
+// Other cells rasterize their own box (the three-argument rasterBox,
+// MCGrid.cuh:66-93, which skips only inverted boxes).
+BNX_D void raster_cell_box(const VolumeDev &v, float4 l, float4 h, RangeCache &cache, int *rlo, int *rhi)
+{
+  if (l.x > h.x || l.y > h.y || l.z > h.z) return;
+  int lo[3], hi[3];
+  const float bl[3] = {l.x, l.y, l.z}, bh[3] = {h.x, h.y, h.z};
+  for (int k = 0; k < 3; k++) {
+    lo[k] = (int)((bl[k] - v.mc_origin[k]) * v.mc_rcp_spacing[k]);
+    hi[k] = (int)((bh[k] - v.mc_origin[k]) * v.mc_rcp_spacing[k]);
+    lo[k] = min(max(lo[k], 0), v.mc[k] - 1);
+    hi[k] = min(max(hi[k], 0), v.mc[k] - 1);
+  }
+  for (int iz = lo[2]; iz <= hi[2]; iz++)
+    for (int iy = lo[1]; iy <= hi[1]; iy++)
+      for (int ix = lo[0]; ix <= hi[0]; ix++) cache.add(ix + v.mc[0] * (iy + v.mc[1] * iz), l.w, h.w, rlo, rhi);
+}
+
+__global__ void k_umesh_raster(const int4 *tets, const int4 *ext, const uint8_t *types, const float4 *verts, int n,
+                               VolumeDev v, int *rlo, int *rhi)
 {
   const int t = blockIdx.x * blockDim.x + threadIdx.x;
   if (t >= n) return;
+  if (types && types[t] != CELL_TET) {
+    RangeCache cache;
+    float4 l, h;
+    cell_box(tets, ext, types, verts, t, l, h);
+    raster_cell_box(v, l, h, cache, rlo, rhi);
+    cache.flush(rlo, rhi);
+    return;
+  }
   const int4 q = tets[t];
   const float4 A = verts[q.x], B = verts[q.y], Cc = verts[q.z], D = verts[q.w];
   RangeCache cache;
@@ -194,24 +253,32 @@ __global__ void k_umesh_raster(const int4 *tets, const float4 *verts, int n, Vol
 // closest hit so far; each block lies inside one request's segment, so the
 // tables are block-uniform.  With no surface instances in the batch the
 // trace kernel is skipped and every ray enters here as a surface miss
-// (surfaces == 0).  FULL: isosurfaces and tet fields may occur; otherwise
-// every volume of the batch is structured and there are no isosurfaces.
+// (surfaces == 0).  MODE 0: every volume of the batch is structured and
+// there are no isosurfaces; MARCH_FULL: isosurfaces and unstructured fields
+// may occur; with MARCH_CELLS, fields with other cells than tets too (a
+// kernel of its own: the cells' Newton inversions would otherwise raise the
+// register pressure of the tet kernel).
 // waves per SIMD the march kernel is compiled for, lean and full variant.
-// The full variant (tet point location + isosurfaces) needs about 126 VGPRs:
-// at 4 waves it fits without spilling; at 6 it spilled 128 VGPRs to scratch,
-// and the spill stores dominated its memory writes
+// The full variant (tet point location + isosurfaces) needs 128 VGPRs to
+// run without spilling, which allows 4 waves per SIMD.  Compiled for 6 (80
+// VGPRs) it spills 143 of them to scratch, and the spill stores then make up
+// most of its memory writes, yet on the 20.5M-tet volume with an isosurface
+// (1024^2, 32 spp) it renders 1.22x faster than at 4 waves and 1.03x faster
+// than at 5 (48 spilled): the dependent gathers of the point queries need
+// the extra waves to hide their latency more than the spills cost.
 #ifndef BNX_MARCH_WAVES_LEAN
 #define BNX_MARCH_WAVES_LEAN 5
 #endif
 #ifndef BNX_MARCH_WAVES_FULL
-#define BNX_MARCH_WAVES_FULL 4
+#define BNX_MARCH_WAVES_FULL 6
 #endif
-template <bool FULL>
-__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(FULL ? BNX_MARCH_WAVES_FULL
+template <int MODE>
+__global__ __launch_bounds__(256) __attribute__((amdgpu_waves_per_eu(MODE ? BNX_MARCH_WAVES_FULL
                                                                           : BNX_MARCH_WAVES_LEAN))) void
 k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
                                                       int surfaces, unsigned long long *stats)
 {
+  constexpr bool FULL = MODE != 0, CELLS = (MODE & MARCH_CELLS) != 0;
   __shared__ uint32_t ring[FULL ? PQ_SHORT * PQ_TB : 1];
   MarchCtx cx{ring + threadIdx.x, 0, 0, 0, 0};
   int i;
@@ -237,7 +304,7 @@ k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
       for (int k = 0; k < niso; k++) {
         float t;
         v3 N;
-        if (iso_march(isos[k], org, dir, 0.f, tcur, rng, t, N, cx)) {
+        if (iso_march<CELLS>(isos[k], org, dir, 0.f, tcur, rng, t, N, cx)) {
           ik = k;
           it = t;
           iN = N;
@@ -255,7 +322,7 @@ k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
       for (int k = 0; k < vl.count && !(shadow && ik >= 0); k++) {
         float t, c[3];
         if (ratio) {
-          T *= ratio_track<FULL>(vl.vols[k], org, dir, 0.f, tcur, rng, cx);
+          T *= ratio_track<FULL, CELLS>(vl.vols[k], org, dir, 0.f, tcur, rng, cx);
           if (T == 0.f) {
             vk = k;
             vt = tcur;
@@ -263,7 +330,7 @@ k_volume_march(const SlotDev *__restrict__ slots, SegView sv, RayQueue q,
           }
           continue;
         }
-        if (volume_march<FULL>(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, cx)) {
+        if (volume_march<FULL, CELLS>(vl.vols[k], org, dir, 0.f, tcur, rng, t, c, cx)) {
           vk = k;
           vt = t;
           tcur = t;
@@ -494,21 +561,31 @@ int build_seed_grid(Volume *v, int ntet)
   return BNX_OK;
 }
 
-int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *verts4, int nvert)
+int volume_build_umesh(Volume *v, const int4 *tets4, const int4 *ext4, const uint8_t *types, int ntet,
+                       const float4 *verts4, int nvert)
 {
   Device *dev = v->dev;
   VolumeDev &d = v->dd;
   hipStream_t st = dev->stream;
+  const bool mixed = types != nullptr;
   BNX_HIP(v->uverts.ensure(sizeof(float4) * (size_t)nvert));
   BNX_HIP(hipMemcpyAsync(v->uverts.ptr, verts4, sizeof(float4) * (size_t)nvert, hipMemcpyHostToDevice, st));
-  DevMem tets, lo, hi, rlo, rhi;
+  DevMem tets, ext, typ, lo, hi, rlo, rhi;
   BNX_HIP(tets.ensure(sizeof(int4) * (size_t)ntet));
   BNX_HIP(hipMemcpyAsync(tets.ptr, tets4, sizeof(int4) * (size_t)ntet, hipMemcpyHostToDevice, st));
-  // element BVH over the tets' boxes; tets stored in its slot order
+  if (mixed) {
+    BNX_HIP(ext.ensure(sizeof(int4) * (size_t)ntet));
+    BNX_HIP(typ.ensure((size_t)ntet));
+    BNX_HIP(hipMemcpyAsync(ext.ptr, ext4, sizeof(int4) * (size_t)ntet, hipMemcpyHostToDevice, st));
+    BNX_HIP(hipMemcpyAsync(typ.ptr, types, (size_t)ntet, hipMemcpyHostToDevice, st));
+  }
+  const int4 *ext_d = mixed ? ext.as<int4>() : nullptr;
+  const uint8_t *typ_d = mixed ? typ.as<uint8_t>() : nullptr;
+  // element BVH over the cells' boxes; cells stored in its slot order
   BNX_HIP(lo.ensure(sizeof(float4) * (size_t)ntet));
   BNX_HIP(hi.ensure(sizeof(float4) * (size_t)ntet));
-  k_tet_boxes<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), v->uverts.as<float4>(), ntet, lo.as<float4>(),
-                                               hi.as<float4>());
+  k_cell_boxes<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), ext_d, typ_d, v->uverts.as<float4>(), ntet,
+                                                lo.as<float4>(), hi.as<float4>());
   BNX_HIP(hipGetLastError());
   int rc = build_bvh(dev, lo.as<float4>(), hi.as<float4>(), ntet, 4, v->ebvh);
   if (rc != BNX_OK) return rc;
@@ -517,18 +594,42 @@ int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *ver
     return BNX_ERR_UNSUPPORTED;
   }
   BNX_HIP(v->utets.ensure(sizeof(int4) * (size_t)ntet));
-  k_tet_sort<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), v->ebvh.order.as<int>(), ntet, v->utets.as<int4>());
+  if (mixed) {
+    BNX_HIP(v->uext.ensure(sizeof(int4) * (size_t)ntet));
+    BNX_HIP(v->utypes.ensure((size_t)ntet));
+  } else {
+    v->uext.release();
+    v->utypes.release();
+  }
+  k_cell_sort<<<nblk(ntet, 256), 256, 0, st>>>(tets.as<int4>(), ext_d, typ_d, v->ebvh.order.as<int>(), ntet,
+                                               v->utets.as<int4>(), v->uext.as<int4>(), v->utypes.as<uint8_t>());
   BNX_HIP(hipGetLastError());
-  // face neighbours of the tets, in slot order, for the walking point query
-  if ((rc = build_tet_neighbors(dev, v->utets.as<int4>(), ntet, v->unbr)) != BNX_OK) return rc;
-  BNX_SET(d.unbr, v->unbr.ptr);
   BNX_SET(d.uverts, v->uverts.ptr);
   BNX_SET(d.utets, v->utets.ptr);
   BNX_SET(d.ebvh, v->ebvh.nodes.ptr);
-  // seeds for point queries (built with the element BVH in place, before
-  // the seed pointer is set, so the build's own queries are depth first)
   d.useed = nullptr;
-  if ((rc = build_seed_grid(v, ntet)) != BNX_OK) return rc;
+  d.unbr = nullptr;
+  d.uext = nullptr;
+  d.utypes = nullptr;
+  if (mixed) {
+    // other cells than tets: point queries go depth first through the
+    // element BVH (no tet walk, no seeds)
+    BNX_SET(d.uext, v->uext.ptr);
+    BNX_SET(d.utypes, v->utypes.ptr);
+    v->useed.release();
+    // no neighbours: a walk (only in a tets-only kernel, which such a field
+    // never meets) ends at its first cell
+    BNX_HIP(v->unbr.ensure(sizeof(int4) * (size_t)ntet));
+    BNX_HIP(hipMemsetAsync(v->unbr.ptr, 0xff, sizeof(int4) * (size_t)ntet, st));
+    BNX_SET(d.unbr, v->unbr.ptr);
+  } else {
+    // face neighbours of the tets, in slot order, for the walking point query
+    if ((rc = build_tet_neighbors(dev, v->utets.as<int4>(), ntet, v->unbr)) != BNX_OK) return rc;
+    BNX_SET(d.unbr, v->unbr.ptr);
+    // seeds for point queries (built with the element BVH in place, before
+    // the seed pointer is set, so the build's own queries are depth first)
+    if ((rc = build_seed_grid(v, ntet)) != BNX_OK) return rc;
+  }
   // macrocell value ranges
   const int nmc = d.mc[0] * d.mc[1] * d.mc[2];
   BNX_HIP(rlo.ensure(sizeof(int) * (size_t)nmc));
@@ -538,8 +639,8 @@ int volume_build_umesh(Volume *v, const int4 *tets4, int ntet, const float4 *ver
   BNX_SET(d.ranges, v->ranges.ptr);
   BNX_SET(d.majorants, v->majorants.ptr);
   k_fill_ranges<<<nblk(nmc, 256), 256, 0, st>>>(rlo.as<int>(), rhi.as<int>(), nmc);
-  k_umesh_raster<<<nblk(ntet, 128), 128, 0, st>>>(tets.as<int4>(), v->uverts.as<float4>(), ntet, d, rlo.as<int>(),
-                                                  rhi.as<int>());
+  k_umesh_raster<<<nblk(ntet, 128), 128, 0, st>>>(tets.as<int4>(), ext_d, typ_d, v->uverts.as<float4>(), ntet, d,
+                                                  rlo.as<int>(), rhi.as<int>());
   k_finish_ranges<<<nblk(nmc, 256), 256, 0, st>>>(rlo.as<int>(), rhi.as<int>(), nmc, v->ranges.as<float2>());
   k_majorants<<<nblk(nmc, 256), 256, 0, st>>>(d, v->ranges.as<float2>(), v->majorants.as<float>());
   BNX_HIP(hipStreamSynchronize(st));
@@ -559,12 +660,14 @@ int volume_rebuild_majorants(Volume *v)
 }
 
 int launch_volume_march(Device *dev, const SlotDev *slots, const SegView &sv, int blocks256, const RayQueue &q,
-                        bool surfaces, bool full, unsigned long long *stats)
+                        bool surfaces, int full, unsigned long long *stats)
 {
-  if (blocks256 > 0 && full)
-    k_volume_march<true><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
+  if (blocks256 > 0 && (full & MARCH_CELLS))
+    k_volume_march<MARCH_FULL | MARCH_CELLS><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
+  else if (blocks256 > 0 && full)
+    k_volume_march<MARCH_FULL><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
   else if (blocks256 > 0)
-    k_volume_march<false><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
+    k_volume_march<0><<<blocks256, 256, 0, dev->stream>>>(slots, sv, q, surfaces ? 1 : 0, stats);
   BNX_HIP(hipGetLastError());
   return BNX_OK;
 }

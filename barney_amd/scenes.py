@@ -521,6 +521,106 @@ class UMeshVolumeScene:
     spp: int = 32
     user_id: int = 0
     iso_user_id: int = 1
+    # cells of any type instead of tets: (indices int32, offsets int32,
+    # types uint8 in the ANARI codes: 0 tet, 1 hex, 2 prism, 3 pyramid)
+    cells: tuple = None
+
+
+CELL_TET, CELL_HEX, CELL_PRISM, CELL_PYR = 0, 1, 2, 3
+
+
+def mixed_cells(n: int, seed: int, kinds=(CELL_TET, CELL_HEX, CELL_PRISM, CELL_PYR), jitter: float = 0.0,
+                shear: float = 0.0):
+    """A lattice of n^3 cubes on [0,1]^3 (vertex (i, j, k) at index
+    i + (n+1) (j + (n+1) k)), each cube filled by cells of one kind drawn
+    from `kinds` (stream (8 << 32) | cube): one hexahedron, two prisms cut
+    along the diagonal plane through its (0,0,z)-(1,1,z) edges, six
+    pyramids over its faces with their apex at a vertex added at its
+    centre, or five tets (the split of hex_tets, positively oriented).  All
+    vertices in VTK order.  jitter moves interior lattice vertices by
+    U(-jitter, jitter) cell widths (faces become non-planar, neighbouring
+    cells then overlap or leave gaps); shear applies x += shear y, y +=
+    shear z, z += shear x to every vertex (an affine map: planar faces stay
+    planar).  Returns (vertices float32 (nv, 3), indices int32, offsets
+    int32, types uint8)."""
+    m = n + 1
+    lat = jittered_lattice(n, seed, jitter) if jitter > 0 else None
+    if lat is None:
+        k, j, i = np.meshgrid(np.arange(m), np.arange(m), np.arange(m), indexing="ij")
+        lat = (np.stack([i.ravel(), j.ravel(), k.ravel()], 1) / n).astype(np.float32)
+    verts = [lat.astype(np.float64)]
+    nv = len(lat)
+    pick = np.minimum((uniform_table(seed, 8, n ** 3, 1)[:, 0] * len(kinds)).astype(np.int64), len(kinds) - 1)
+    tets5 = hex_tets(n).reshape(n ** 3, 5, 4)
+    idx, offs, types = [], [], []
+
+    def add(cell, t):
+        offs.append(len(idx))
+        idx.extend(int(x) for x in cell)
+        types.append(t)
+
+    extra = []
+    for c in range(n ** 3):
+        i, j, kk = c % n, (c // n) % n, c // (n * n)
+
+        def v(a, b, cc):
+            return (i + a) + m * ((j + b) + m * (kk + cc))
+
+        kind = kinds[pick[c]]
+        c000, c100, c110, c010 = v(0, 0, 0), v(1, 0, 0), v(1, 1, 0), v(0, 1, 0)
+        c001, c101, c111, c011 = v(0, 0, 1), v(1, 0, 1), v(1, 1, 1), v(0, 1, 1)
+        if kind == CELL_HEX:
+            add((c000, c100, c110, c010, c001, c101, c111, c011), CELL_HEX)
+        elif kind == CELL_PRISM:
+            add((c000, c100, c110, c001, c101, c111), CELL_PRISM)
+            add((c000, c110, c010, c001, c111, c011), CELL_PRISM)
+        elif kind == CELL_PYR:
+            corners = np.array([lat[q] for q in (c000, c100, c110, c010, c001, c101, c111, c011)], np.float64)
+            extra.append(corners.mean(0))
+            apex = nv + len(extra) - 1
+            for quad in ((c000, c010, c110, c100), (c001, c101, c111, c011), (c000, c100, c101, c001),
+                         (c010, c011, c111, c110), (c000, c001, c011, c010), (c100, c110, c111, c101)):
+                add((*quad, apex), CELL_PYR)
+        else:
+            for t in tets5[c]:
+                add(t, CELL_TET)
+    if extra:
+        verts.append(np.array(extra))
+    V = np.concatenate(verts, 0)
+    if shear:
+        V = np.stack([V[:, 0] + shear * V[:, 1], V[:, 1] + shear * V[:, 2], V[:, 2] + shear * V[:, 0]], 1)
+    V = V.astype(np.float32)
+    idx = np.array(idx, np.int32)
+    types = np.array(types, np.uint8)
+    offs = np.array(offs, np.int32)
+    # positively oriented tets (the tet test's inside is the positive side)
+    tsel = np.nonzero(types == CELL_TET)[0]
+    if len(tsel):
+        t4 = np.stack([idx[offs[tsel] + q] for q in range(4)], 1)
+        t4 = orient_tets(V, t4)
+        for q in range(4):
+            idx[offs[tsel] + q] = t4[:, q]
+    return V, idx, offs, types
+
+
+def mixed_volume(n: int = 12, width: int = 256, height: int = 256, spp: int = 1, seed: int = 0xB4A2000A,
+                 kinds=(CELL_TET, CELL_HEX, CELL_PRISM, CELL_PYR), jitter: float = 0.15,
+                 mc_dims=(8, 8, 8), volume: bool = True) -> UMeshVolumeScene:
+    """config 4's scene on a mixed-cell mesh: mixed_cells(n) (jittered), the
+    scalar sin(4x) cos(4y) sin(4z) per vertex, the config-2 transfer
+    function over its range with density 50, an isosurface at 0.3, lights
+    and camera as config 4"""
+    V, idx, offs, types = mixed_cells(n, seed, kinds, jitter)
+    p = V.astype(np.float64)
+    f = (np.sin(4 * p[:, 0]) * np.cos(4 * p[:, 1]) * np.sin(4 * p[:, 2])).astype(np.float32)
+    s = UMeshVolumeScene(vertices=V, scalars=f, tets=np.zeros((0, 4), np.int32), tf=ramp_tf(256, 0.8),
+                         tf_domain=(float(f.min()), float(f.max())), base_density=50.0, mc_dims=tuple(mc_dims),
+                         volume=volume, width=width, height=height, spp=spp, cells=(idx, offs, types))
+    d = -np.ones(3, dtype=np.float64) / math.sqrt(3.0)
+    s.dir_lights.append((d.astype(np.float32), 2.5, np.ones(3, dtype=np.float32)))
+    s.camera = dict(position=(1.9, 1.4, 2.6), direction=(0.5 - 1.9, 0.5 - 1.4, 0.5 - 2.6), up=(0.0, 1.0, 0.0),
+                    fovy=35.0, target=(0.5, 0.5, 0.5))
+    return s
 
 
 def hex_tets(n: int) -> np.ndarray:

@@ -112,16 +112,17 @@ def umesh_bytes(nodes, tets, cells, rays):
 
 
 def trace_kernel_name(tuning, max_world_prims):
-    """the queue-trace kernels that ran: the LDS-stack variant of the tuning;
+    """the queue-trace kernels that ran: the LDS-stack and walk variant of
+    the tuning, for worlds without instance transforms (the bench worlds);
     the pooled kernel too when a world is large enough to use it"""
     sh = tuning["trace_stack_lds"]
     walk = "true" if tuning.get("trace_walk", 0) == 1 else "false"
     pooled = tuning["trace_refill"] < 64 and max_world_prims >= tuning["trace_refill_prims"]
-    static = f"k_trace_static<{sh}, {walk}>"
-    return f"{static}+k_trace_pool<{sh}>" if pooled else static
+    static = f"k_trace_static<{sh}, {walk}, false>"
+    return f"{static}+k_trace_pool<{sh}, false>" if pooled else static
 
 
-def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false>", trace_bytes=None, model=None):
+def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false, false>", trace_bytes=None, model=None):
     """dominant kernel (by summed HIP-event time) and its GB/s: algorithmic
     (the byte model) and measured (HBM traffic per launch from profiles/).
     trace_bytes: the trace kernels' algorithmic bytes over the timed region
@@ -412,10 +413,12 @@ def serve_plain(dev, reqs, frames, per_sample=False):
     return svc, tot, rays
 
 
-def serve_served(dev, reqs, arrivals=None, cfg=None):
+def serve_served(dev, reqs, arrivals=None, cfg=None, gens=0):
     """submit at the arrival times (all at once when arrivals is None) and
     step the server until every request is done; returns per-request
-    completion times (s after the start), the makespan and server stats"""
+    completion times (s after the start), the makespan and server stats.
+    gens: generations per server step (0: bnx_server_step, one host
+    read-back per generation; n: bnx_server_step_n)"""
     import barney_amd as bx
     srv = bx.Server(dev, **(cfg or {}))
     done_at = {}
@@ -429,7 +432,7 @@ def serve_served(dev, reqs, arrivals=None, cfg=None):
             tix[srv.submit(r["world"], r["cam"], r["rend"], r["w"], r["h"], r["spp"])] = i
             i += 1
         if srv.pending:
-            for t in srv.step():
+            for t in srv.step(gens):
                 done_at[tix[t]] = time.perf_counter() - t0
                 srv.release(t)
         elif i < n:
@@ -502,7 +505,7 @@ def run_c5(args, n, warmup):
     # warm-up: both forms
     if warm:
         serve_plain(dev, warm, frames)
-        serve_served(dev, warm)
+        serve_served(dev, warm, gens=args.server_generations)
     dev.sync()
     # plain: service times alone, Mrays/s over the busy time; samples merged
     # into batches (bnx_render), and one pixel wave per sample (the
@@ -517,7 +520,7 @@ def run_c5(args, n, warmup):
         plain_ps_mrays = ptot_ps["rays"] / svc_ps.sum() / 1e6
         log(f"plain (one pixel wave per sample): {svc_ps.sum():.3f} s busy, {plain_ps_mrays:.1f} Mrays/s")
     # served at saturation
-    _, makespan, sst = serve_served(dev, reqs)
+    _, makespan, sst = serve_served(dev, reqs, gens=args.server_generations)
     served_mrays = sst["rays_traced"] / makespan / 1e6
     log(f"served at saturation: {makespan:.3f} s, {served_mrays:.1f} Mrays/s")
     tot = dict(rays=sst["rays_traced"], nodes=sst["node_visits"], tris=sst["tri_tests"],
@@ -529,14 +532,15 @@ def run_c5(args, n, warmup):
             f.close()
         dev.close()
         desc = {"workload": f"config 5 serving stream, served pass at saturation only ({n} requests)",
-                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)", **tune,
+                "requests": n, "warmup_requests": warmup, "form": "served (continuous batching)",
+                "server_generations_per_step": args.server_generations, **tune,
                 "max_world_prims": max_prims}
         return served_mrays, makespan, tot, desc, {"frame_latency_ms": {"p50": None, "p99": None}}
     # the same stream, Poisson arrivals at a fixed rate: load args.load of
     # the plain form's capacity
     qps = args.load / float(svc.mean())
     arr = a1 / qps
-    done, _, _ = serve_served(dev, reqs, arrivals=arr)
+    done, _, _ = serve_served(dev, reqs, arrivals=arr, gens=args.server_generations)
     lat_served = (np.array(done) - arr) * 1e3
     log(f"served at {qps:.1f} QPS: p50 {np.percentile(lat_served, 50):.2f} ms, p99 {np.percentile(lat_served, 99):.2f} ms")
     # plain form under the same arrivals: one request at a time, in order
@@ -561,6 +565,7 @@ def run_c5(args, n, warmup):
                         f"{{1,4,16,64}}, seeded orbit cameras; Poisson arrivals at {qps:.2f} QPS (load "
                         f"{args.load} of the plain form's capacity)",
             "requests": n, "warmup_requests": warmup, "qps": round(qps, 3), "resolutions": res,
+            "server_generations_per_step": args.server_generations,
             "form": "served (continuous batching) vs plain (one bnx_render per request)",
             "target_judged_against": "plain: bnx_render per request in arrival order, samples merged into "
                                      "batches of up to render_paths paths (the stronger of the two plain forms)",
@@ -664,6 +669,9 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (c1-c4: renders); c5 times its fixed sample (--requests) whatever the value")
     ap.add_argument("--requests", type=int, default=C5_REQUESTS, help="c5: size of the fixed sample")
+    ap.add_argument("--server-generations", type=int, default=0,
+                    help="c5: generations per server step (0: one host read-back per generation, "
+                         "bnx_server_step; 1..16: bnx_server_step_n)")
     ap.add_argument("--warmup", type=int, default=None)
     ap.add_argument("--workload", default="c5", choices=["c1", "c2", "c3", "c4", "c5"])
     ap.add_argument("--terrain-n", type=int, default=4096, help="c3: heightfield size (4096: 33.5M triangles)")

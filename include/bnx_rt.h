@@ -291,13 +291,23 @@ typedef struct bnx_transfer_function {
 
 /* UMeshField (umesh/common/UMeshField.h:33-168, anari "vertex.position",
  * "vertex.data", "index", "cell.index", "cell.type"): per-vertex scalars,
- * each cell's vertex indices starting at cell_offsets[cell].  Tetrahedra
- * only (cell type 0 = _ANARI_TET or 10 = _VTK_TET; NULL cell_types means all
- * tets); any other type returns BNX_ERR_UNSUPPORTED.  A tet (v0, v1, v2, v3)
- * contains P when P is on the non-negative side of its four faces as
- * UMeshField::DD::tetScalar orients them (UMeshField.h:391-436). */
-#define BNX_CELL_TET      0
-#define BNX_CELL_VTK_TET 10
+ * each cell's vertex indices starting at cell_offsets[cell].  Cell types
+ * (UMeshField.h:17-29, ANARI or VTK codes; NULL cell_types means all tets):
+ * tetrahedra (4 vertices), hexahedra (8), prisms (6) and pyramids (5), in
+ * VTK vertex order; polyhedra (VTK 42) and other codes return
+ * BNX_ERR_UNSUPPORTED.  A tet (v0, v1, v2, v3) contains P when P is on the
+ * non-negative side of its four faces as UMeshField::DD::tetScalar orients
+ * them (UMeshField.h:391-436); the other cells invert their isoparametric
+ * map by Newton's method (ElementIntersection.h:77-476) and contain P when
+ * its parametric coordinates lie in the cell's domain (within 1e-6). */
+#define BNX_CELL_TET        0
+#define BNX_CELL_HEX        1
+#define BNX_CELL_PRISM      2
+#define BNX_CELL_PYR        3
+#define BNX_CELL_VTK_TET   10
+#define BNX_CELL_VTK_HEX   12
+#define BNX_CELL_VTK_PRISM 13
+#define BNX_CELL_VTK_PYR   14
 typedef struct bnx_umesh_desc {
   const float   *vertices;          /* float3[num_vertices]                        */
   const float   *scalars;           /* float[num_vertices]                         */
@@ -571,9 +581,10 @@ int bnx_volume_destroy(bnx_volume vol);
  * MCVolumeAccel, umesh/mc/UMeshCuBQLSampler.cu:45-78, UMeshField.cu:168-197):
  * builds on the GPU the element BVH (binned SAH, leaf <= 4), the macrocell
  * value ranges by rasterizing each tet split 5 times on its longest edge
- * (rasterTet<5>, UMeshField.cu:28-107), and the majorants.  Sampling is a
- * point query: the first tet containing P in a depth-first walk of the
- * element BVH (child 0 first). */
+ * (rasterTet<5>, UMeshField.cu:28-107) and every other cell's box
+ * (UMeshField.cu:149-152), and the majorants.  Sampling is a point query:
+ * the first cell containing P in a depth-first walk of the element BVH
+ * (child 0 first, leaf cells in order). */
 int bnx_volume_create_umesh(bnx_device dev, const bnx_umesh_desc *field, const bnx_transfer_function *tf,
                             int32_t user_id, bnx_volume *out);
 /* macrocell value ranges (lo, hi per cell; for tests) */
@@ -734,6 +745,15 @@ int bnx_server_submit(bnx_server server, const bnx_request *request, int64_t *ti
  * march, shade); writes up to max_done tickets that completed in it.  With
  * nothing queued or active it returns at once with *num_done = 0. */
 int bnx_server_step(bnx_server server, int64_t *done_tickets, int max_done, int *num_done);
+/* generations (1..16) generations for all requests in flight with one host
+ * read-back at the end instead of one per generation: admission happens
+ * before the first, and a kernel does each generation's bookkeeping on the
+ * device (the queue top-up in the same priority order, the per-request
+ * segments, the output regions).  Requests that finish inside the step are
+ * reported at its end.  Results equal bnx_server_step's and bnx_render's.
+ * No counterpart in the reference, whose frame loop (Context::renderTiles,
+ * native/Context.cpp:139-182) reads back after every generation. */
+int bnx_server_step_n(bnx_server server, int generations, int64_t *done_tickets, int max_done, int *num_done);
 int bnx_server_poll(bnx_server server, int64_t ticket, bnx_request_status *status);
 /* read a channel (BNX_FB_*) of a completed request, as bnx_frame_read */
 int bnx_server_read(bnx_server server, int64_t ticket, int channel, void *host);

@@ -75,6 +75,7 @@ def lib():
         L.orc_scene_add_isosurface.argtypes = [vp, C.c_int, C.c_float, abi.fp, C.c_int, C.c_int]
         L.orc_volume_mc_ranges.argtypes = [vp, C.c_int, abi.ip, abi.fp, abi.fp]
         L.orc_volume_ebvh.argtypes = [vp, C.c_int, vp, abi.ip, abi.ip]
+        L.orc_cell_invert_n.argtypes = [C.c_int, abi.fp, C.c_int, abi.fp, abi.ip, abi.fp]
         L.orc_umesh_sample.argtypes = [vp, C.c_int, abi.fp, C.POINTER(C.c_uint64)]
         L.orc_umesh_sample.restype = C.c_float
         L.orc_iso_march.argtypes = [vp, C.c_int, abi.fp, abi.fp, C.c_float, C.c_uint32, C.c_uint32, abi.fp]
@@ -267,7 +268,10 @@ class Scene:
 
     def add_umesh(self, us, user_id=0, user_inst=0, rendered=True):
         d, t, keep = abi.umesh_desc(us)
-        return self.L.orc_scene_add_umesh(self.h, C.byref(d), C.byref(t), user_id, user_inst, int(rendered))
+        k = self.L.orc_scene_add_umesh(self.h, C.byref(d), C.byref(t), user_id, user_inst, int(rendered))
+        if k < 0:
+            raise ValueError("oracle: unsupported cell type")
+        return k
 
     def add_isosurface(self, field: int, iso: float, color=(0.8, 0.8, 0.8), user_id=0, user_inst=0):
         c = np.ascontiguousarray(color[:3], np.float32)
@@ -620,6 +624,19 @@ def volume_scene(vs) -> Scene:
         s.set_env_map(env.radiance, env.direction, env.up, env.scale)
     s.build()
     return s
+
+
+def cell_invert(cell_type: int, verts4, P):
+    """the Newton inversion of one pyramid / prism / hexahedron (type 3 / 2 /
+    1; verts4: (n, 4) x, y, z, scalar in VTK order) at points P (m, 3):
+    (inside bool[m], value float32[m], NaN outside)"""
+    L = lib()
+    v = np.ascontiguousarray(verts4, np.float32)
+    p = np.ascontiguousarray(P, np.float32).reshape(-1, 3)
+    inside = np.zeros(len(p), np.int32)
+    val = np.zeros(len(p), np.float32)
+    L.orc_cell_invert_n(int(cell_type), abi.fptr(v), len(p), abi.fptr(p), abi.iptr(inside), abi.fptr(val))
+    return inside.astype(bool), val
 
 
 def umesh_scene(us) -> Scene:

@@ -239,9 +239,30 @@ int orc_scene_add_umesh(orc_scene *sc, const bnx_umesh_desc *f, const bnx_transf
     for (int q = 0; q < 3; q++) v.verts[4 * (size_t)i + q] = f->vertices[3 * (size_t)i + q];
     v.verts[4 * (size_t)i + 3] = f->scalars[i];
   }
-  v.tets.resize(4 * (size_t)f->num_cells);
-  for (int c = 0; c < f->num_cells; c++)
-    for (int q = 0; q < 4; q++) v.tets[4 * (size_t)c + q] = f->indices[f->cell_offsets[c] + q];
+  // cell types: the ANARI and VTK codes of tets, hexahedra, prisms and
+  // pyramids (UMeshField.h:17-29); NULL: every cell is a tet
+  const int nc = f->num_cells;
+  std::vector<uint8_t> types((size_t)nc, (uint8_t)CELL_TET);
+  bool mixed = false;
+  for (int c = 0; f->cell_types && c < nc; c++) {
+    const int t = f->cell_types[c];
+    const int k = (t == 0 || t == 10) ? CELL_TET : (t == 1 || t == 12) ? CELL_HEX : (t == 2 || t == 13) ? CELL_PRISM
+                : (t == 3 || t == 14) ? CELL_PYR : -1;
+    if (k < 0) return -1;
+    types[c] = (uint8_t)k;
+    mixed |= k != CELL_TET;
+  }
+  v.tets.assign(4 * (size_t)nc, -1);
+  if (mixed) v.ext.assign(4 * (size_t)nc, -1);
+  for (int c = 0; c < nc; c++) {
+    const int n = types[c] == CELL_HEX ? 8 : types[c] == CELL_PRISM ? 6 : types[c] == CELL_PYR ? 5 : 4;
+    for (int q = 0; q < n; q++) {
+      const int32_t i = f->indices[f->cell_offsets[c] + q];
+      if (q < 4) v.tets[4 * (size_t)c + q] = i;
+      else v.ext[4 * (size_t)c + q - 4] = i;
+    }
+  }
+  if (mixed) v.types.swap(types);
   for (int k = 0; k < 3; k++) v.mc_req[k] = f->mc_dims[k];
   v.tf.assign(tf->values, tf->values + 4 * (size_t)tf->num_values);
   v.dom0 = tf->domain[0];
@@ -286,6 +307,17 @@ int orc_volume_ebvh(orc_scene *sc, int k, void *nodes_out, int32_t *order_out, i
   if (nodes_out) memcpy(nodes_out, v.ebvh.nodes.data(), v.ebvh.nodes.size() * sizeof(Node));
   if (order_out) memcpy(order_out, v.ebvh.order.data(), v.ebvh.order.size() * sizeof(int32_t));
   return 0;
+}
+
+// the Newton inversion of one cell (type: CELL_*, verts: x, y, z, scalar per
+// vertex in VTK order) at n points: inside[i] and value[i]
+void orc_cell_invert_n(int type, const float *verts, int n, const float *P, int32_t *inside, float *value)
+{
+  for (int i = 0; i < n; i++) {
+    float f = NAN;
+    inside[i] = cell_invert(type, verts, load3(P + 3 * (size_t)i), f) ? 1 : 0;
+    value[i] = f;
+  }
 }
 
 // point query with the number of element-BVH nodes and tets visited

@@ -279,10 +279,14 @@ struct Volume {
   int mc[3];
   vec3 mc_origin, mc_spacing;
   std::vector<float> mc_lo, mc_hi, majorant;
-  // unstructured field (tets only): UMeshField (umesh/common/UMeshField.h)
+  // unstructured field: UMeshField (umesh/common/UMeshField.h)
   std::vector<float> verts;         // x, y, z, scalar per vertex
-  std::vector<int32_t> tets;        // 4 vertex indices per tet; element-BVH slot order after build
-  Bvh ebvh;                         // element BVH over the tets' boxes
+  std::vector<int32_t> tets;        // vertex indices 0..3 per cell; element-BVH slot order after build
+  // other cells than tets: per cell its type (CELL_*) and vertex indices
+  // 4..7 (-1 where unused); both empty when every cell is a tet
+  std::vector<uint8_t> types;
+  std::vector<int32_t> ext;
+  Bvh ebvh;                         // element BVH over the cells' boxes
   int mc_req[3] = {0, 0, 0};        // requested macrocell grid (0: the reference's formula)
 };
 
@@ -320,6 +324,15 @@ void build_umesh(Volume &v);
 float umesh_sample(const Volume &v, vec3 P, uint64_t *visits = nullptr);
 // tetScalar (UMeshField.h:391-436) of the tet in slot 'slot'
 bool tet_scalar(const Volume &v, int slot, vec3 P, float &out);
+// cell types (UMeshField.h:17-22, the ANARI codes)
+enum { CELL_TET = 0, CELL_HEX = 1, CELL_PRISM = 2, CELL_PYR = 3 };
+// eltScalar (UMeshField.h:372-389) of the cell in slot 'slot': tetScalar,
+// or pyrScalar / prismScalar / hexScalar (:438-495) with the Newton
+// inversions of ElementIntersection.h:77-476
+bool cell_scalar(const Volume &v, int slot, vec3 P, float &out);
+// the Newton inversion of one pyramid (5 vertices), prism (6) or hexahedron
+// (8), vertices as x, y, z, scalar in VTK order
+bool cell_invert(int type, const float *verts4, vec3 P, float &out);
 // MCIsoSurfaceAccel::isProg (volume/MCAccelerator.h:325-548): t of the
 // surface f = iso along the ray within [tmin, tmax), with the unnormalized
 // central-difference gradient normal; advances 'parent' once when the ray

@@ -689,7 +689,10 @@ void render(const Scene &s, const bnx_camera &cam, const bnx_renderer &rr, Frame
         // scales its MIS weight; zero occludes it
         const bool ratio = r.is_shadow && ik < 0 && (rr.flags & BNX_RENDER_RATIO_TRACKING) != 0;
         float T = 1.f;
-        for (int k = 0, rk = 0; k < (int)s.volumes.size(); k++) {
+        // a shadow ray is occluded by its first event: once an isosurface or
+        // a volume stopped it, the remaining volumes are not marched (their
+        // outcome would not change it; only the work counted differs)
+        for (int k = 0, rk = 0; k < (int)s.volumes.size() && !(r.is_shadow && (ik >= 0 || vk >= 0)); k++) {
           if (!s.volumes[k].rendered) continue;
           float t, rgb[3];
           if (ratio) {

@@ -1,5 +1,5 @@
-// umesh.cpp -- CPU oracle of barney's unstructured (tetrahedral) field and
-// of the isosurface intersection program.
+// umesh.cpp -- CPU oracle of barney's unstructured field (tets, pyramids,
+// prisms, hexahedra) and of the isosurface intersection program.
 //
 // TEST INFRASTRUCTURE ONLY (see oracle.h).
 #include "oracle.h"
@@ -36,14 +36,15 @@ struct Box4 {
   }
 };
 
-// rasterBox(grid, primBounds4) (volume/MCGrid.cuh:95-122): every cell the
-// 3D box touches gets min/max of the box's value range; boxes that are flat
-// in any dimension are skipped (>= tests)
-void raster_box(Volume &v, const Box4 &b, vec3 rsp)
+// rasterBox (volume/MCGrid.cuh:66-122): every cell the 3D box touches gets
+// min/max of the box's value range.  strict: boxes flat in any dimension are
+// skipped (>=, the two-argument form, for tet pieces); otherwise only
+// inverted ones (>, the three-argument form, for the other cells)
+void raster_box(Volume &v, const Box4 &b, vec3 rsp, bool strict = true)
 {
-  if (b.lo.x >= b.hi.x) return;
-  if (b.lo.y >= b.hi.y) return;
-  if (b.lo.z >= b.hi.z) return;
+  if (strict ? b.lo.x >= b.hi.x : b.lo.x > b.hi.x) return;
+  if (strict ? b.lo.y >= b.hi.y : b.lo.y > b.hi.y) return;
+  if (strict ? b.lo.z >= b.hi.z : b.lo.z > b.hi.z) return;
   int lo[3], hi[3];
   const float bl[3] = {b.lo.x, b.lo.y, b.lo.z}, bh[3] = {b.hi.x, b.hi.y, b.hi.z};
   for (int k = 0; k < 3; k++) {
@@ -96,7 +97,132 @@ inline float plane(vec3 P, vec4 a, vec4 b, vec4 c)
   return dot(P - pos(a), N);
 }
 
+int nverts_of(int type) { return type == CELL_HEX ? 8 : type == CELL_PRISM ? 6 : type == CELL_PYR ? 5 : 4; }
+int cell_type(const Volume &v, int c) { return v.types.empty() ? CELL_TET : (int)v.types[c]; }
+// vertex k of cell c (k < nverts_of(type))
+int cell_vertex(const Volume &v, int c, int k)
+{
+  return k < 4 ? v.tets[4 * (size_t)c + k] : v.ext[4 * (size_t)c + (k - 4)];
+}
+
+// ---- isoparametric cells: shape functions N_i(r, s, t) and their partial
+// derivatives in VTK vertex order (ElementIntersection.h:37-75, 171-211,
+// 286-347), with the same arithmetic
+void shape_hex(float r, float s, float t, float *w, float *dr, float *ds, float *dt)
+{
+  const float rm = 1.f - r, sm = 1.f - s, tm = 1.f - t;
+  const float W[8] = {rm * sm * tm, r * sm * tm, r * s * tm, rm * s * tm, rm * sm * t, r * sm * t, r * s * t, rm * s * t};
+  const float R[8] = {-sm * tm, sm * tm, s * tm, -s * tm, -sm * t, sm * t, s * t, -s * t};
+  const float S[8] = {-rm * tm, -r * tm, r * tm, rm * tm, -rm * t, -r * t, r * t, rm * t};
+  const float T[8] = {-rm * sm, -r * sm, -r * s, -rm * s, rm * sm, r * sm, r * s, rm * s};
+  for (int i = 0; i < 8; i++) {
+    w[i] = W[i];
+    dr[i] = R[i];
+    ds[i] = S[i];
+    dt[i] = T[i];
+  }
+}
+void shape_prism(float r, float s, float t, float *w, float *dr, float *ds, float *dt)
+{
+  const float W[6] = {(1.f - r - s) * (1.f - t), r * (1.f - t), s * (1.f - t), (1.f - r - s) * t, r * t, s * t};
+  const float R[6] = {-1.f + t, 1.f - t, 0.f, -t, t, 0.f};
+  const float S[6] = {-1.f + t, 0.f, 1.f - t, -t, 0.f, t};
+  const float T[6] = {-1.f + r + s, -r, -s, 1.f - r - s, r, s};
+  for (int i = 0; i < 6; i++) {
+    w[i] = W[i];
+    dr[i] = R[i];
+    ds[i] = S[i];
+    dt[i] = T[i];
+  }
+}
+void shape_pyr(float r, float s, float t, float *w, float *dr, float *ds, float *dt)
+{
+  const float rm = 1.f - r, sm = 1.f - s, tm = 1.f - t;
+  const float W[5] = {rm * sm * tm, r * sm * tm, r * s * tm, rm * s * tm, t};
+  const float R[5] = {-(s - 1.f) * (t - 1.f), (s - 1.f) * (t - 1.f), s - s * t, s * (t - 1.f), 0.f};
+  const float S[5] = {-(r - 1.f) * (t - 1.f), r * (t - 1.f), r - r * t, (r - 1.f) * (t - 1.f), 0.f};
+  const float T[5] = {-(r - 1.f) * (s - 1.f), r * (s - 1.f), -r * s, (r - 1.f) * s, 1.f};
+  for (int i = 0; i < 5; i++) {
+    w[i] = W[i];
+    dr[i] = R[i];
+    ds[i] = S[i];
+    dt[i] = T[i];
+  }
+}
+// det of the matrix with columns a, b, c (LinearSpace3f det: a . (b x c))
+float det3(vec3 a, vec3 b, vec3 c) { return dot(a, cross(b, c)); }
+
 } // namespace
+
+bool cell_invert(int type, const float *V, vec3 P, float &out)
+{
+  const int n = nverts_of(type);
+  float tol = 1e-6f;                 // pyramids and prisms
+  if (type == CELL_HEX) {
+    // hexahedra: 1e-10 x the diagonal of the vertices' box
+    vec3 lo(V[0], V[1], V[2]), hi = lo;
+    for (int i = 1; i < 8; i++) {
+      const vec3 p(V[4 * i], V[4 * i + 1], V[4 * i + 2]);
+      lo = vmin(lo, p);
+      hi = vmax(hi, p);
+    }
+    tol = 1e-10f * length(hi - lo);
+  }
+  float pc[3] = {.5f, .5f, .5f};
+  float w[8], dr[8], ds[8], dt[8];
+  bool converged = false;
+  for (int it = 0; it < 10; it++) {
+    if (type == CELL_HEX) shape_hex(pc[0], pc[1], pc[2], w, dr, ds, dt);
+    else if (type == CELL_PRISM) shape_prism(pc[0], pc[1], pc[2], w, dr, ds, dt);
+    else shape_pyr(pc[0], pc[1], pc[2], w, dr, ds, dt);
+    vec3 f(0.f), cr(0.f), cs(0.f), ct(0.f);
+    for (int i = 0; i < n; i++) {
+      const vec3 X(V[4 * i], V[4 * i + 1], V[4 * i + 2]);
+      f = f + X * w[i];
+      cr = cr + X * dr[i];
+      cs = cs + X * ds[i];
+      ct = ct + X * dt[i];
+    }
+    f = f - P;
+    const float d = det3(cr, cs, ct);
+    if (fabsf(d) < tol) return false;
+    const float d0 = det3(f, cs, ct) / d;
+    const float d1 = det3(cr, f, ct) / d;
+    const float d2 = det3(cr, cs, f) / d;
+    pc[0] = pc[0] - d0;
+    pc[1] = pc[1] - d1;
+    pc[2] = pc[2] - d2;
+    if (fabsf(d0) < 1e-4f && fabsf(d1) < 1e-4f && fabsf(d2) < 1e-4f) {
+      converged = true;
+      break;
+    }
+    if (fabsf(pc[0]) > 1e6f || fabsf(pc[1]) > 1e6f || fabsf(pc[2]) > 1e6f) return false;
+  }
+  if (!converged) return false;
+  const float lo = 0.f - 1e-6f, hi = 1.f + 1e-6f;
+  for (int k = 0; k < 3; k++)
+    if (!(pc[k] >= lo && pc[k] <= hi)) return false;
+  if (type == CELL_PRISM && !(pc[0] + pc[1] <= hi)) return false;
+  float val = 0.f;
+  for (int i = 0; i < n; i++) val += w[i] * V[4 * i + 3];
+  out = val;
+  return true;
+}
+
+bool cell_scalar(const Volume &v, int slot, vec3 P, float &out)
+{
+  const int type = cell_type(v, slot);
+  if (type == CELL_TET) return tet_scalar(v, slot, P, out);
+  float V[32];
+  for (int k = 0; k < nverts_of(type); k++) {
+    const vec4 p = v4(v, cell_vertex(v, slot, k));
+    V[4 * k] = p.x;
+    V[4 * k + 1] = p.y;
+    V[4 * k + 2] = p.z;
+    V[4 * k + 3] = p.w;
+  }
+  return cell_invert(type, V, P, out);
+}
 
 void build_umesh(Volume &v)
 {
@@ -106,7 +232,7 @@ void build_umesh(Volume &v)
   std::vector<Box> boxes((size_t)ntet);
   for (int t = 0; t < ntet; t++) {
     Box b;
-    for (int q = 0; q < 4; q++) b.grow(pos(v4(v, v.tets[4 * (size_t)t + q])));
+    for (int q = 0; q < nverts_of(cell_type(v, t)); q++) b.grow(pos(v4(v, cell_vertex(v, t, q))));
     boxes[t] = b;
     wb.grow(b);
   }
@@ -127,18 +253,35 @@ void build_umesh(Volume &v)
   v.mc_lo.assign(nmc, INFINITY);
   v.mc_hi.assign(nmc, -INFINITY);
   const vec3 rsp(1.f / v.mc_spacing.x, 1.f / v.mc_spacing.y, 1.f / v.mc_spacing.z);
-  // umeshRasterCells (UMeshField.cu:109-161), tets only
+  // umeshRasterCells (UMeshField.cu:109-161): tets split 5 times, every
+  // other cell by its box (position and value range)
   for (int t = 0; t < ntet; t++) {
-    const int32_t *ix = &v.tets[4 * (size_t)t];
-    raster_tet(v, 5, v4(v, ix[0]), v4(v, ix[1]), v4(v, ix[2]), v4(v, ix[3]), rsp);
+    const int type = cell_type(v, t);
+    if (type == CELL_TET) {
+      const int32_t *ix = &v.tets[4 * (size_t)t];
+      raster_tet(v, 5, v4(v, ix[0]), v4(v, ix[1]), v4(v, ix[2]), v4(v, ix[3]), rsp);
+    } else {
+      Box4 bb;
+      for (int q = 0; q < nverts_of(type); q++) bb.extend(v4(v, cell_vertex(v, t, q)));
+      raster_box(v, bb, rsp, false);
+    }
   }
   build_majorants(v);
-  // element BVH; the tets are stored in its slot order
+  // element BVH; the cells are stored in its slot order
   build_bvh(boxes, 4, v.ebvh);
-  std::vector<int32_t> sorted(v.tets.size());
-  for (int s = 0; s < ntet; s++)
-    for (int q = 0; q < 4; q++) sorted[4 * (size_t)s + q] = v.tets[4 * (size_t)v.ebvh.order[s] + q];
+  std::vector<int32_t> sorted(v.tets.size()), sorted_ext(v.ext.size());
+  std::vector<uint8_t> sorted_types(v.types.size());
+  for (int s = 0; s < ntet; s++) {
+    const int c = v.ebvh.order[s];
+    for (int q = 0; q < 4; q++) sorted[4 * (size_t)s + q] = v.tets[4 * (size_t)c + q];
+    if (!v.types.empty()) {
+      for (int q = 0; q < 4; q++) sorted_ext[4 * (size_t)s + q] = v.ext[4 * (size_t)c + q];
+      sorted_types[s] = v.types[c];
+    }
+  }
   v.tets.swap(sorted);
+  v.ext.swap(sorted_ext);
+  v.types.swap(sorted_types);
 }
 
 bool tet_scalar(const Volume &v, int slot, vec3 P, float &out)
@@ -182,7 +325,7 @@ float umesh_sample(const Volume &v, vec3 P, uint64_t *visits)
       for (int k = 0; k < e.cnt; k++) {
         float f;
         if (visits) (*visits)++;
-        if (tet_scalar(v, e.ref + k, P, f)) return f;
+        if (cell_scalar(v, e.ref + k, P, f)) return f;
       }
     }
   }

@@ -82,12 +82,12 @@ def test_cpp_rtc_mirror_compiles():
 
 def test_bench_reads_profile_traffic_of_every_workload():
     # the roofline's "traffic" comes from the committed rocprof summaries;
-    # trace kernel names hold a space ("k_trace_static<16, false>")
+    # trace kernel names hold spaces ("k_trace_static<16, false, false>")
     import sys
     root = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
     sys.path.insert(0, root)
     import bench
-    for w, k in [("c1", "k_trace_static<32, true>"), ("c2", "k_volume_march<false>"),
+    for w, k in [("c1", "k_trace_static<32, true, false>"), ("c2", "k_volume_march<false>"),
                  ("c3", "k_trace_static<16, false>"), ("c4", "k_volume_march<true>"),
                  ("c5", "k_trace_static<16, false>")]:
         t = bench.profile_traffic(w, k)

@@ -10,6 +10,8 @@ structured volume, PBR surfaces under an HDRI map), resolutions and spp share
 the queue under either order of the primary top-up, arrive while others are
 in flight, wait for request slots and frame memory, and are split into
 partial-sample runs of primary rays."""
+import ctypes as C
+
 import numpy as np
 import pytest
 
@@ -68,10 +70,21 @@ def _check_same(got, ref, tag):
         assert np.array_equal(got[ch].view(np.uint8), ref[ch].view(np.uint8)), (tag, ch)
 
 
+def _gens(mode, step):
+    """generations per server step: 0 = bnx_server_step (host read-back after
+    every generation), n = bnx_server_step_n(n), "mix" = the two alternating
+    with 2..5 device-driven generations (the request state hands over)"""
+    if mode == "mix":
+        return 0 if step % 2 == 0 else 2 + step % 4
+    return mode
+
+
 @pytest.mark.parametrize("cfg", [dict(), dict(max_paths_in_flight=3000, max_active=3),
                                  dict(max_paths_in_flight=3000, deadline_slack=-1.0),
                                  dict(max_paths_in_flight=20000, frame_memory_budget=80 * 96 * 80 + 80 * 64 * 64)])
-def test_served_equals_alone(dev, worlds, cfg):
+@pytest.mark.parametrize("mode", [0, 3, 16, "mix"])
+def test_served_equals_alone(dev, worlds, cfg, mode):
+    print(f"served equals alone cfg={cfg} generations={mode}", flush=True)
     reqs = _mix(worlds)
     refs = [_plain(dev, *r) for r in reqs]
     srv = bx.Server(dev, **cfg)
@@ -97,7 +110,7 @@ def test_served_equals_alone(dev, worlds, cfg):
         tickets[srv.submit(*reqs[k])] = k
     nxt, steps = 3, 0
     while srv.pending or nxt < len(reqs):
-        finish(srv.step())
+        finish(srv.step(_gens(mode, steps)))
         steps += 1
         if nxt < len(reqs) and steps % 2 == 0:
             tickets[srv.submit(*reqs[nxt])] = nxt
@@ -107,7 +120,31 @@ def test_served_equals_alone(dev, worlds, cfg):
     print("steps", steps, "stats", st, flush=True)
     assert st["done"] == st["released"] == len(reqs) and st["active"] == 0 and st["queued"] == 0
     assert done == set(range(len(reqs)))
+    if mode == 16:
+        # the same rays as one generation per step, in fewer steps
+        assert st["generations"] >= 5
     srv.close()
+
+
+def test_step_n_counts_and_rejects(dev, worlds):
+    """bnx_server_step_n: the stats (rays, generations, shadow rays, node
+    visits) of device-driven steps equal those of host steps over the same
+    requests; generations outside 1..16 are rejected"""
+    reqs = _mix(worlds)[:6]
+    out = {}
+    for mode in (0, 4):
+        srv = bx.Server(dev, max_paths_in_flight=4000)
+        for r in reqs:
+            srv.submit(*r)
+        srv.run_until_idle(generations=mode)
+        out[mode] = srv.stats()
+        n = C.c_int()
+        for bad in (0, 17, -3):
+            assert srv.L.bnx_server_step_n(srv.h, bad, srv._done, len(srv._done), C.byref(n)) != 0, bad
+        srv.close()
+    for k in ("rays_traced", "shadow_rays", "volume_samples", "node_visits", "tri_tests", "sphere_tests", "done"):
+        assert out[0][k] == out[4][k], k
+    assert out[4]["generations"] == out[0]["generations"]
 
 
 def test_served_oracle_parity(dev, orc, worlds):

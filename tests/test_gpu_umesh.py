@@ -100,10 +100,14 @@ def test_served_equals_alone(dev):
 
 
 def test_rejects_other_cell_types(dev):
+    """polyhedra (_VTK_POLYHEDRON = 42) and unknown codes: BNX_ERR_UNSUPPORTED;
+    a hexahedron whose 8 indices run past the index array: BNX_ERR_INVALID"""
     us = small(n=2)
     d, t, keep = abi.umesh_desc(us)
-    types = np.full(len(us.tets), 12, np.uint8)          # _VTK_HEX
-    d.cell_types = types.ctypes.data_as(abi.C.POINTER(abi.C.c_uint8))
     h = abi.C.c_void_p()
-    rc = dev.L.bnx_volume_create_umesh(dev.h, abi.C.byref(d), abi.C.byref(t), 0, abi.C.byref(h))
-    assert rc == -4                                       # BNX_ERR_UNSUPPORTED
+    for code, want in ((42, -4), (7, -4), (12, -1)):
+        types = np.zeros(len(us.tets), np.uint8)
+        types[-1] = code                                  # the last cell
+        d.cell_types = types.ctypes.data_as(abi.C.POINTER(abi.C.c_uint8))
+        rc = dev.L.bnx_volume_create_umesh(dev.h, abi.C.byref(d), abi.C.byref(t), 0, abi.C.byref(h))
+        assert rc == want, (code, rc)

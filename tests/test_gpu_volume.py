@@ -96,3 +96,20 @@ def test_ratio_tracking_parity(dev, orc):
     diff = float(np.abs(r[abi.BNX_FB_COLOR][0][..., :3].mean() - dl[abi.BNX_FB_COLOR][0][..., :3].mean()))
     print("ratio vs delta mean diff at 256 spp", diff, flush=True)
     assert diff < 0.01 * float(dl[abi.BNX_FB_COLOR][0][..., :3].mean()) + 1e-4
+
+
+@pytest.mark.parametrize("shift", [2, 3])
+def test_bricked_layout_parity(dev, orc, shift, monkeypatch):
+    """BNX_VOLUME_BRICK: the sampler reads a bricked copy of the field
+    (2^shift cells a side plus an apron texel), a layout only; frames equal
+    the oracle's as with the linear array, with the same sample count, on a
+    field whose size is not a multiple of the brick"""
+    print(f"bricked layout shift={shift}", flush=True)
+    monkeypatch.setenv("BNX_VOLUME_BRICK", str(shift))
+    vs = scenes.config2(n=45, width=96, height=96, spp=2)
+    out, st, ost = _render_both(dev, orc, vs, 2)
+    for ch in (abi.BNX_FB_PRIMID, abi.BNX_FB_INSTID, abi.BNX_FB_OBJID):
+        assert np.array_equal(*out[ch]), ch
+    gc, oc = out[abi.BNX_FB_COLOR]
+    assert np.abs(gc - oc).max() <= 1e-3, float(np.abs(gc - oc).max())
+    assert st["volume_samples"] == ost["volume_samples"] > 0
