@@ -129,9 +129,10 @@ def roofline(tot, workload=None, trace_kernel="k_trace_static<16, false, false>"
     from the oracle's counts (model: how they were counted); without it the
     GPU's own counters are priced at the same per-item sizes"""
     if tot.get("march_ms", 0.0) >= tot.get("trace_ms", 0.0) and tot.get("march_launches", 0) > 0:
-        # the full variant runs for tet fields and isosurfaces, the lean one otherwise
+        # the full variant (MARCH_FULL = 1) runs for tet fields and
+        # isosurfaces, the lean one (0) otherwise
         full = tot.get("umesh_nodes", 0) > 0 or workload == "c4"
-        kernel = "k_volume_march<true>" if full else "k_volume_march<false>"
+        kernel = "k_volume_march<1>" if full else "k_volume_march<0>"
         kms, launches = tot["march_ms"], tot["march_launches"]
         if tot.get("umesh_nodes", 0) > 0:
             byts = umesh_bytes(tot["umesh_nodes"], tot["umesh_tets"], tot["mc_cells"], tot["march_rays"])
