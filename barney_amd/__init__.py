@@ -30,6 +30,9 @@ class BnxError(RuntimeError):
         self.code = code
 
 
+# bnx_denoise_fn: (user, in_rgba, in_normal, out_rgba, width, height, blend, stream) -> int
+DENOISE_FN = C.CFUNCTYPE(C.c_int, vp, vp, vp, vp, C.c_int32, C.c_int32, C.c_float, vp)
+
 # (name, argtypes, restype)
 _PROTOS = [
     ("bnx_last_error", [], C.c_char_p),
@@ -97,6 +100,8 @@ _PROTOS = [
     ("bnx_frame_create", [vp, C.c_int, C.c_int, C.POINTER(vp)], C.c_int),
     ("bnx_frame_reset", [vp], C.c_int),
     ("bnx_frame_read", [vp, C.c_int, vp], C.c_int),
+    ("bnx_frame_set_denoiser", [vp, DENOISE_FN, vp, C.c_int32], C.c_int),
+    ("bnx_copy", [vp, vp, vp, C.c_size_t], C.c_int),
     ("bnx_frame_accum_id", [vp], C.c_int),
     ("bnx_frame_destroy", [vp], C.c_int),
     ("bnx_render", [vp, C.POINTER(abi.Camera), C.POINTER(abi.Renderer), vp, C.c_int, C.POINTER(abi.RenderStats)],
@@ -538,6 +543,14 @@ class Frame:
         out = _channel_array(channel, self.hgt, self.w)
         check(self.dev.L.bnx_frame_read(self.h_, channel, out.ctypes.data))
         return out
+
+    def set_denoiser(self, fn, fade_out: bool = False):
+        """bnx_frame_set_denoiser: fn(user, d_in_rgba, d_in_normal, d_out_rgba,
+        width, height, blend, stream) -> int, called with device pointers
+        (ints) when the colour channel is read; None removes the hook"""
+        self._denoise = DENOISE_FN(fn) if fn is not None else None
+        check(self.dev.L.bnx_frame_set_denoiser(self.h_, self._denoise if fn is not None else DENOISE_FN(),
+                                                None, int(bool(fade_out))))
 
     def close(self):
         if self.h_:

@@ -143,6 +143,11 @@ struct Context {
   bnx_device dev = nullptr;
   std::set<Obj *> live;      // every object not yet destroyed
   bool dying = false;
+  // the denoiser of the context's frame buffers (bnxContextSetDenoiser):
+  // barney's device creates its own (rtc::Device::createDenoiser, OIDN or
+  // OptiX); here the host hands one in
+  bnx_denoise_fn denoise = nullptr;
+  void *denoise_user = nullptr;
 };
 
 void release(Obj *o)
@@ -1330,8 +1335,26 @@ void bnFrameBufferRead(BNFrameBuffer fb, BNFrameBufferChannel channel, void *out
   case BN_FB_NORMAL: ch = BNX_FB_NORMAL; break;
   default: throw Failure("unknown channel");
   }
+  if (channel == BN_FB_COLOR) {
+    // FrameBuffer::set1i (fb/FrameBuffer.cu:60-75) applies "enableDenoising"
+    // and "fadeOutDenoiser" as they are set; the context's denoiser runs on
+    // the colour read when enabled ("enableUpscaling" is not provided)
+    const bool on = f->get1i("enableDenoising", 0) != 0 && f->ctx->denoise;
+    check(bnx_frame_set_denoiser(f->f, on ? f->ctx->denoise : nullptr, f->ctx->denoise_user,
+                                 f->get1i("fadeOutDenoiser", 0)),
+          "bnFrameBufferRead");
+  }
   check(bnx_frame_read(f->f, ch, out), "bnFrameBufferRead");
   BNB_CATCH("bnFrameBufferRead", )
+}
+
+void bnxContextSetDenoiser(BNContext context, bnx_denoise_fn fn, void *user)
+{
+  BNB_TRY
+  bnb::Context *c = bnb::ctx_of(context);
+  c->denoise = fn;
+  c->denoise_user = fn ? user : nullptr;
+  BNB_CATCH("bnxContextSetDenoiser", )
 }
 
 void bnRender(BNRenderer renderer, BNModel model, BNCamera camera, BNFrameBuffer fb)

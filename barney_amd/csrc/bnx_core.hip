@@ -618,6 +618,21 @@ int bnx_buffer_upload(bnx_buffer b, const void *host, size_t bytes, size_t offse
   return BNX_OK;
 }
 
+int bnx_copy(bnx_device d, void *dst, const void *src, size_t bytes)
+{
+  BNX_CHECK(dst && src, BNX_ERR_INVALID, "bnx_copy: null argument");
+  if (bytes == 0) return BNX_OK;
+  if (!d) {
+    // no device handle: after all work of the current device
+    BNX_HIP(hipDeviceSynchronize());
+    BNX_HIP(hipMemcpy(dst, src, bytes, hipMemcpyDefault));
+    return BNX_OK;
+  }
+  BNX_HIP(hipMemcpyAsync(dst, src, bytes, hipMemcpyDefault, d->stream));
+  BNX_HIP(hipStreamSynchronize(d->stream));
+  return BNX_OK;
+}
+
 int bnx_buffer_download(bnx_buffer b, void *host, size_t bytes, size_t offset)
 {
   BNX_CHECK(b && host, BNX_ERR_INVALID, "bnx_buffer_download: null argument");
@@ -1543,6 +1558,20 @@ int bnx_frame_read(bnx_frame f, int channel, void *host)
 {
   BNX_CHECK(f && host, BNX_ERR_INVALID, "bnx_frame_read: null argument");
   return frame_read(f, channel, host);
+}
+
+int bnx_frame_set_denoiser(bnx_frame f, bnx_denoise_fn fn, void *user, int32_t fade_out)
+{
+  BNX_CHECK(f, BNX_ERR_INVALID, "bnx_frame_set_denoiser: null frame");
+  f->denoise = fn;
+  f->denoise_user = fn ? user : nullptr;
+  f->denoise_fade = fn && fade_out != 0;
+  if (!fn) {
+    f->dn_in.release();
+    f->dn_out.release();
+    f->dn_normal.release();
+  }
+  return BNX_OK;
 }
 
 int bnx_frame_accum_id(bnx_frame f) { return f ? f->accum_id : -1; }

@@ -217,6 +217,12 @@ struct Frame {
   DevMem accum;         // int64 x 4 per pixel, 2^-32 fixed point
   DevMem depth, prim, inst, obj, normal;
   DevMem out;           // staging for reads
+  // denoiser hook (bnx_frame_set_denoiser) and its device images: linear
+  // colour in, normals in (float3), colour out
+  bnx_denoise_fn denoise = nullptr;
+  void *denoise_user = nullptr;
+  bool denoise_fade = false;
+  DevMem dn_in, dn_normal, dn_out;
 };
 
 // builder entry (bnx_bvh.hip): boxes in d_lo/d_hi (float4, n); returns host info

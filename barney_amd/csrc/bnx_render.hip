@@ -863,6 +863,38 @@ __global__ void k_frame_rgba8(FrameDev fb, int np, float scale, bool srgb, uint3
   out[i] = (make_8bit(c[0]) << 0) + (make_8bit(c[1]) << 8) + (make_8bit(c[2]) << 16) + (make_8bit(c[3]) << 24);
 }
 
+// 8-bit RGBA of linear float4 colours (a denoiser's output)
+__global__ void k_rgba8_from(const float4 *in, int np, bool srgb, uint32_t *out)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const float4 v = in[i];
+  float c[4] = {v.x, v.y, v.z, v.w};
+  for (int k = 0; k < 4; k++) {
+    c[k] = fminf(fmaxf(c[k], 0.f), 1.f);
+    if (srgb && k < 3) c[k] = linear_to_srgb(c[k]);
+  }
+  out[i] = (make_8bit(c[0]) << 0) + (make_8bit(c[1]) << 8) + (make_8bit(c[2]) << 16) + (make_8bit(c[3]) << 24);
+}
+
+// the denoiser's inputs (finalizeFrame's gather, fb/FrameBuffer.cu:196-240):
+// linear colour to in_rgba and out_rgba, normals to in_normal (float3)
+__global__ void k_denoise_inputs(FrameDev fb, int np, float scale, float4 *in_rgba, float4 *out_rgba, float *in_normal)
+{
+  int i = blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= np) return;
+  const unsigned long long *a = fb.accum + 4 * (size_t)i;
+  float c[4];
+  for (int k = 0; k < 4; k++) c[k] = (float)((double)(long long)a[k] * 0x1p-32) * scale;
+  const float4 v = make_float4(c[0], c[1], c[2], c[3]);
+  in_rgba[i] = v;
+  out_rgba[i] = v;
+  const float4 n = fb.normal[i];
+  in_normal[3 * (size_t)i + 0] = n.x;
+  in_normal[3 * (size_t)i + 1] = n.y;
+  in_normal[3 * (size_t)i + 2] = n.z;
+}
+
 __global__ void k_frame_depth(FrameDev fb, int np, float *out)
 {
   int i = blockIdx.x * blockDim.x + threadIdx.x;
@@ -882,12 +914,48 @@ int frame_clear(Frame *f)
   return BNX_OK;
 }
 
+// run the frame's denoiser hook: its output colour in f->dn_out
+int frame_denoise(Frame *f)
+{
+  const int np = f->w * f->h;
+  hipStream_t st = f->dev->stream;
+  BNX_HIP(f->dn_in.ensure(sizeof(float4) * (size_t)np));
+  BNX_HIP(f->dn_out.ensure(sizeof(float4) * (size_t)np));
+  BNX_HIP(f->dn_normal.ensure(sizeof(float) * 3 * (size_t)np));
+  const float scale = f->accum_id > 0 ? 1.f / (float)f->accum_id : 0.f;
+  k_denoise_inputs<<<nblk(np, 256), 256, 0, st>>>(frame_dev(f), np, scale, f->dn_in.as<float4>(),
+                                                  f->dn_out.as<float4>(), f->dn_normal.as<float>());
+  BNX_HIP(hipGetLastError());
+  // readColorChannel (fb/FrameBuffer.cu:257): fade the denoiser out as
+  // samples accumulate
+  const float blend = f->denoise_fade ? (float)(f->accum_id - 1) / ((float)f->accum_id + 100.f) : 0.f;
+  const int rc = f->denoise(f->denoise_user, f->dn_in.as<float>(), f->dn_normal.as<float>(), f->dn_out.as<float>(),
+                            f->w, f->h, blend, (void *)st);
+  BNX_CHECK(rc == 0, BNX_ERR_INVALID, "bnx_frame_read: the denoiser hook failed (" + std::to_string(rc) + ")");
+  return BNX_OK;
+}
+
 int frame_read(Frame *f, int channel, void *host)
 {
   int np = f->w * f->h;
   hipStream_t st = f->dev->stream;
   FrameDev fd = frame_dev(f);
   if (np == 0) return BNX_OK;
+  const bool color = channel == BNX_FB_COLOR || channel == BNX_FB_COLOR_RGBA8 || channel == BNX_FB_COLOR_SRGB8;
+  if (color && f->denoise) {
+    int rc = frame_denoise(f);
+    if (rc != BNX_OK) return rc;
+    if (channel == BNX_FB_COLOR) {
+      BNX_HIP(hipMemcpyAsync(host, f->dn_out.ptr, sizeof(float4) * (size_t)np, hipMemcpyDeviceToHost, st));
+    } else {
+      BNX_HIP(f->out.ensure(sizeof(uint32_t) * (size_t)np));
+      k_rgba8_from<<<nblk(np, 256), 256, 0, st>>>(f->dn_out.as<float4>(), np, channel == BNX_FB_COLOR_SRGB8,
+                                                  f->out.as<uint32_t>());
+      BNX_HIP(hipMemcpyAsync(host, f->out.ptr, sizeof(uint32_t) * (size_t)np, hipMemcpyDeviceToHost, st));
+    }
+    BNX_HIP(hipStreamSynchronize(st));
+    return BNX_OK;
+  }
   switch (channel) {
   case BNX_FB_COLOR: {
     BNX_HIP(f->out.ensure(sizeof(float4) * (size_t)np));

@@ -39,7 +39,11 @@ struct VolumeDev {
   // tets (UMeshField, umesh/common/UMeshField.h:52-116)
   const float4 BNX_G *uverts;   // position, scalar
   const int4 BNX_G *utets;      // vertex indices, in element-BVH slot order
-  const int4 BNX_G *unbr;       // per tet slot: the slot across the face opposite vertex k (-1: boundary)
+  // per tet slot an 80-byte record: its four vertices (position, scalar)
+  // and then its face neighbours (int4 bits: the slot across the face
+  // opposite vertex k, -1 at the boundary), so that a tet test or a walk
+  // step is one dependent fetch instead of an index fetch and four gathers
+  const float4 BNX_G *utetv;
   // meshes with other cells than tets: per slot the cell type (CELL_*) and
   // vertex indices 4..7 (0..3 in utets); null for tets only
   const uint8_t BNX_G *utypes;
@@ -88,7 +92,7 @@ struct Volume {
   uint64_t version = 0;         // bumped whenever dd changes
   VolumeDev dd{};
   DevMem scalars, bricks, ranges, majorants, tf;
-  DevMem uverts, utets, unbr, useed, utypes, uext;
+  DevMem uverts, utets, unbr, useed, utypes, uext, utetv;
   Bvh ebvh;                     // element BVH (cells)
 };
 
@@ -151,7 +155,7 @@ struct MarchCtx {
   uint32_t *ring;             // &ring[0][thread]
   uint32_t samples, cells, unodes, utets;
   // the tet that held this ray's previous point query, and its field
-  const int4 BNX_G *last_field = nullptr;
+  const float4 BNX_G *last_field = nullptr;
   int last_tet = -1;
 };
 
@@ -210,8 +214,8 @@ BNX_D float implicit_plane(v3 P, float4 a, float4 b, float4 c)
 // UMeshField::DD::tetScalar (umesh/common/UMeshField.h:391-436)
 BNX_D bool tet_scalar(const VolumeDev &v, int slot, v3 P, float &out)
 {
-  const int4 t = v.utets[slot];
-  const float4 v0 = v.uverts[t.x], v1 = v.uverts[t.y], v2 = v.uverts[t.z], v3v = v.uverts[t.w];
+  const float4 BNX_G *rec = v.utetv + 5 * (size_t)slot;
+  const float4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3v = rec[3];
   const float t3 = implicit_plane(P, v0, v1, v2);
   if (t3 < 0.f) return false;
   const float t2 = implicit_plane(P, v0, v3v, v1);
@@ -395,8 +399,8 @@ struct TetPlanes {
 };
 BNX_D TetPlanes tet_planes(const VolumeDev &v, int slot, v3 P)
 {
-  const int4 t = v.utets[slot];
-  const float4 v0 = v.uverts[t.x], v1 = v.uverts[t.y], v2 = v.uverts[t.z], v3v = v.uverts[t.w];
+  const float4 BNX_G *rec = v.utetv + 5 * (size_t)slot;
+  const float4 v0 = rec[0], v1 = rec[1], v2 = rec[2], v3v = rec[3];
   TetPlanes r;
   r.t[3] = implicit_plane(P, v0, v1, v2);
   r.t[2] = implicit_plane(P, v0, v3v, v1);
@@ -437,7 +441,7 @@ BNX_D bool umesh_walk(const VolumeDev &v, int cur, v3 P, MarchCtx &cx, float &va
     int k = 0;
     for (int j = 1; j < 4; j++)
       if (r.t[j] < r.t[k]) k = j;
-    const int4 nb = v.unbr[cur];
+    const int4 nb = ((const int4 BNX_G *)v.utetv)[5 * (size_t)cur + 4];
     const int next = k == 0 ? nb.x : (k == 1 ? nb.y : (k == 2 ? nb.z : nb.w));
     if (next < 0) break;
     cur = next;
@@ -530,7 +534,7 @@ template <bool CELLS = false> BNX_D float umesh_sample(const VolumeDev &v, v3 P,
     (void)umesh_dfs<CELLS>(v, P, cx, value);
     return value;
   }
-  const bool have_last = cx.last_field == v.utets && cx.last_tet >= 0;
+  const bool have_last = cx.last_field == v.utetv && cx.last_tet >= 0;
   if (have_last && umesh_walk(v, cx.last_tet, P, cx, value, t)) {
     cx.last_tet = t;
     return value;
@@ -538,14 +542,14 @@ template <bool CELLS = false> BNX_D float umesh_sample(const VolumeDev &v, v3 P,
   if (v.useed) {
     const int seed = v.useed[seed_cell(v, P)];
     if (seed >= 0 && umesh_walk(v, seed, P, cx, value, t)) {
-      cx.last_field = v.utets;
+      cx.last_field = v.utetv;
       cx.last_tet = t;
       return value;
     }
   }
   t = umesh_dfs(v, P, cx, value);
   if (t >= 0) {
-    cx.last_field = v.utets;
+    cx.last_field = v.utetv;
     cx.last_tet = t;
   }
   return value;

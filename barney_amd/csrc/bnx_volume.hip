@@ -123,6 +123,22 @@ __global__ void k_cell_sort(const int4 *tets, const int4 *ext, const uint8_t *ty
   }
 }
 
+// the 80-byte tet records (VolumeDev::utetv) in slot order: four vertices
+// and the face neighbours
+__global__ void k_tet_pack(const int4 *tets, const float4 *verts, const int4 *nbr, int n, float4 *rec)
+{
+  const int s = blockIdx.x * blockDim.x + threadIdx.x;
+  if (s >= n) return;
+  const int4 t = tets[s];
+  float4 *r = rec + 5 * (size_t)s;
+  r[0] = verts[t.x];
+  r[1] = verts[t.y];
+  r[2] = verts[t.z];
+  r[3] = verts[t.w];
+  const int4 b = nbr[s];
+  r[4] = make_float4(__int_as_float(b.x), __int_as_float(b.y), __int_as_float(b.z), __int_as_float(b.w));
+}
+
 __global__ void k_fill_ranges(int *lo, int *hi, int n)
 {
   const int c = blockIdx.x * blockDim.x + threadIdx.x;
@@ -608,7 +624,6 @@ int volume_build_umesh(Volume *v, const int4 *tets4, const int4 *ext4, const uin
   BNX_SET(d.utets, v->utets.ptr);
   BNX_SET(d.ebvh, v->ebvh.nodes.ptr);
   d.useed = nullptr;
-  d.unbr = nullptr;
   d.uext = nullptr;
   d.utypes = nullptr;
   if (mixed) {
@@ -621,11 +636,23 @@ int volume_build_umesh(Volume *v, const int4 *tets4, const int4 *ext4, const uin
     // never meets) ends at its first cell
     BNX_HIP(v->unbr.ensure(sizeof(int4) * (size_t)ntet));
     BNX_HIP(hipMemsetAsync(v->unbr.ptr, 0xff, sizeof(int4) * (size_t)ntet, st));
-    BNX_SET(d.unbr, v->unbr.ptr);
   } else {
     // face neighbours of the tets, in slot order, for the walking point query
     if ((rc = build_tet_neighbors(dev, v->utets.as<int4>(), ntet, v->unbr)) != BNX_OK) return rc;
-    BNX_SET(d.unbr, v->unbr.ptr);
+  }
+  // the tet records: vertices and neighbours in one 80-byte record per slot
+  BNX_HIP(v->utetv.ensure(sizeof(float4) * 5 * (size_t)ntet));
+  k_tet_pack<<<nblk(ntet, 256), 256, 0, st>>>(v->utets.as<int4>(), v->uverts.as<float4>(), v->unbr.as<int4>(), ntet,
+                                              v->utetv.as<float4>());
+  BNX_HIP(hipGetLastError());
+  BNX_SET(d.utetv, v->utetv.ptr);
+  BNX_HIP(hipStreamSynchronize(st));
+  // the records hold the neighbours; the index array stays only for the
+  // other cells' vertex lists
+  v->unbr.release();
+  if (!mixed) {
+    v->utets.release();
+    d.utets = nullptr;
     // seeds for point queries (built with the element BVH in place, before
     // the seed pointer is set, so the build's own queries are depth first)
     if ((rc = build_seed_grid(v, ntet)) != BNX_OK) return rc;

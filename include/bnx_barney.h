@@ -25,8 +25,9 @@
  *   - the renderer ("bgColor", "ambientRadiance", "pathsPerPixel",
  *     "cutPlane"), frame buffers (float4, 8-bit RGBA and sRGB colour, depth,
  *     prim / inst / obj IDs) and bnRender.
+ * Frame buffers denoise with a host-supplied denoiser (bnxContextSetDenoiser).
  * Unknown object types, parameters the backend does not use, and calls for
- * features out of scope (denoising, MPI) are ignored with a warning on
+ * features out of scope (upscaling, MPI) are ignored with a warning on
  * stderr, as barney's warn_unsupported_object does; errors (bad handles,
  * failed builds) print the backend's message and leave the object unchanged.
  * bnGetLastError() returns the last message.
@@ -165,6 +166,15 @@ int bnPhysicalDeviceCount(void);                               /* barney.h:263 *
 void bnRelease(BNObject object);                               /* barney.h:335 */
 void bnAddReference(BNObject object);                          /* barney.h:345 */
 const char *bnGetLastError(void);                              /* (this backend) */
+/* The denoiser of this context's frame buffers (this backend's extension:
+ * barney's device creates its own, rtc::Device::createDenoiser, from OIDN
+ * or OptiX).  A frame buffer with "enableDenoising" set runs it when its
+ * colour channel is read, with "fadeOutDenoiser" (fb/FrameBuffer.cu:60-75,
+ * 252-290); fn has bnx_frame_set_denoiser's signature and contract
+ * (bnx_rt.h); NULL removes it. */
+typedef int (*bnxDenoiseFn)(void *user, const float *d_in_rgba, const float *d_in_normal, float *d_out_rgba,
+                            int32_t width, int32_t height, float blend, void *stream);
+void bnxContextSetDenoiser(BNContext context, bnxDenoiseFn fn, void *user);
 
 /* ---------------------------------------------------- object creation */
 BNCamera bnCameraCreate(BNContext context, const char *type);  /* barney.h:237 */

@@ -379,6 +379,13 @@ int bnx_device_create(int gpu_id, bnx_device *out);
 int bnx_device_destroy(bnx_device dev);
 /* rtc::Device::sync (rtcore/cudaCommon/Device.cu) */
 int bnx_device_sync(bnx_device dev);
+/* copy bytes between host and device memory (either may be either: the
+ * runtime tells them apart) on the device's stream, and wait for it (dev
+ * NULL: after all work of the current device, for callers that hold no
+ * device handle, such as a denoiser behind barney's bn* API); for hooks that
+ * are handed device images (bnx_frame_set_denoiser).  No counterpart in the
+ * reference, whose hooks own their buffers. */
+int bnx_copy(bnx_device dev, void *dst, const void *src, size_t bytes);
 /* free / total device memory in bytes (used by memory-based admission) */
 int bnx_device_mem_info(bnx_device dev, size_t *free_bytes, size_t *total_bytes);
 
@@ -642,6 +649,22 @@ int bnx_frame_create(bnx_device dev, int width, int height, bnx_frame *out);
 int bnx_frame_reset(bnx_frame fb);
 /* bnFrameBufferRead (fb/FrameBuffer.cu:315-394) + linearize (fb/TiledFB.cu:12-111) */
 int bnx_frame_read(bnx_frame fb, int channel, void *host);
+/* Denoiser hook (rtc::Denoiser, rtcore/cpu/Denoiser.h:16-30, as
+ * FrameBuffer::finalizeFrame and readColorChannel use it, fb/FrameBuffer.cu:
+ * 196-290).  With a hook set, reading the colour channel (BNX_FB_COLOR or
+ * the 8-bit forms) first writes the linear colour (accum x 1/accumID) to
+ * in_rgba (float4 per pixel) and the first-sample normals to in_normal
+ * (float3 per pixel), copies in_rgba to out_rgba (the alpha passes through,
+ * as the OIDN denoiser leaves it), and calls fn with these device images of
+ * width x height pixels, row-major, and the device's hipStream_t; the
+ * channel is then read from out_rgba.  fn may enqueue its work on the stream
+ * or finish it before returning; a nonzero return fails the read.  blend is
+ * (accumID - 1) / (accumID + 100) when fade_out is set (fadeOutDenoiser),
+ * else 0.  fn NULL removes the hook.  barney's upscale mode (a half-size
+ * render upscaled 2x after denoising) is not provided. */
+typedef int (*bnx_denoise_fn)(void *user, const float *d_in_rgba, const float *d_in_normal, float *d_out_rgba,
+                              int32_t width, int32_t height, float blend, void *stream);
+int bnx_frame_set_denoiser(bnx_frame fb, bnx_denoise_fn fn, void *user, int32_t fade_out);
 int bnx_frame_accum_id(bnx_frame fb);
 int bnx_frame_destroy(bnx_frame fb);
 

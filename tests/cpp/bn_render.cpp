@@ -21,9 +21,13 @@
 //                scalars, int32 ntf, float4[ntf], float2 domain, float
 //                density
 //   out: float4 colour[W H], int32 primID, instID, objID [W H], float
-//        depth [W H], uint32 sRGB8 colour [W H] (rows from the bottom)
+//        depth [W H], uint32 sRGB8 colour [W H] (rows from the bottom),
+//        float4 colour [W H] read again with "enableDenoising" set and a
+//        context denoiser (bnxContextSetDenoiser) that writes 1 - rgb and
+//        keeps alpha, then float4 colour [W H] with denoising off again
 // Prints "bn_render ok" and exits 0 on success.
 #include "bnx_barney.h"
+#include "bnx_rt.h"
 
 #include <cstdio>
 #include <fstream>
@@ -40,6 +44,17 @@ template <typename T> static std::vector<T> rd(std::ifstream &f, size_t n)
   return v;
 }
 template <typename T> static T rd1(std::ifstream &f) { return rd<T>(f, 1)[0]; }
+
+// a denoiser that writes 1 - rgb and keeps alpha (bnx_copy without a
+// device handle: the bn* API exposes none)
+static int invert_rgb(void *, const float *in, const float *, float *out, int32_t w, int32_t h, float, void *)
+{
+  std::vector<float> v(4 * (size_t)w * h);
+  if (bnx_copy(nullptr, v.data(), in, sizeof(float) * v.size()) != 0) return 1;
+  for (size_t i = 0; i < v.size(); i++)
+    if (i % 4 != 3) v[i] = 1.f - v[i];
+  return bnx_copy(nullptr, out, v.data(), sizeof(float) * v.size()) != 0 ? 2 : 0;
+}
 
 static void ok_or_throw(const void *h, const char *what)
 {
@@ -184,6 +199,13 @@ int main(int argc, char **argv)
     bnFrameBufferRead(fb, BN_FB_OBJID, obj.data(), BN_INT);
     bnFrameBufferRead(fb, BN_FB_DEPTH, depth.data(), BN_FLOAT);
     bnFrameBufferRead(fb, BN_FB_COLOR, srgb.data(), BN_UFIXED8_RGBA_SRGB);
+    // denoising with a context denoiser, then off again
+    std::vector<float> denoised(4 * np), plain_again(4 * np);
+    bnxContextSetDenoiser(ctx, invert_rgb, nullptr);
+    bnSet1i(fb, "enableDenoising", 1);
+    bnFrameBufferRead(fb, BN_FB_COLOR, denoised.data(), BN_FLOAT4);
+    bnSet1i(fb, "enableDenoising", 0);
+    bnFrameBufferRead(fb, BN_FB_COLOR, plain_again.data(), BN_FLOAT4);
     if (std::string(bnGetLastError()).find("failed") != std::string::npos)
       throw std::runtime_error(std::string("bn_render: ") + bnGetLastError());
 
@@ -194,6 +216,8 @@ int main(int argc, char **argv)
     out.write((const char *)obj.data(), 4 * np);
     out.write((const char *)depth.data(), 4 * np);
     out.write((const char *)srgb.data(), 4 * np);
+    out.write((const char *)denoised.data(), sizeof(float) * denoised.size());
+    out.write((const char *)plain_again.data(), sizeof(float) * plain_again.size());
     if (!out) throw std::runtime_error("bn_render: write failed");
 
     for (BNObject o : {(BNObject)fb, (BNObject)renderer, (BNObject)camera, (BNObject)model, (BNObject)group,

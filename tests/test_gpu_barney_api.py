@@ -57,7 +57,11 @@ def _run(tmp_path, write, W, H):
     o += 12 * n
     depth = raw[o:o + 4 * n].view(np.float32).reshape(H, W)
     srgb = raw[o + 4 * n:o + 8 * n].reshape(H, W, 4)
-    return dict(color=color, prim=prim, inst=inst, obj=obj, depth=depth, srgb=srgb)
+    o += 8 * n
+    denoised = raw[o:o + 16 * n].view(np.float32).reshape(H, W, 4)
+    again = raw[o + 16 * n:o + 32 * n].view(np.float32).reshape(H, W, 4)
+    return dict(color=color, prim=prim, inst=inst, obj=obj, depth=depth, srgb=srgb, denoised=denoised,
+                again=again)
 
 
 def _check(bn, fr, of):
@@ -66,6 +70,12 @@ def _check(bn, fr, of):
              ("obj", abi.BNX_FB_OBJID), ("depth", abi.BNX_FB_DEPTH), ("srgb", abi.BNX_FB_COLOR_SRGB8)]
     for k, ch in pairs:
         assert np.array_equal(bn[k].view(np.uint8), fr.read(ch).view(np.uint8)), k
+    # "enableDenoising" with the context's denoiser (1 - rgb, alpha kept),
+    # then off again
+    want = bn["color"].copy()
+    want[..., :3] = np.float32(1.0) - want[..., :3]
+    assert np.array_equal(bn["denoised"].view(np.uint32), want.view(np.uint32))
+    assert np.array_equal(bn["again"].view(np.uint32), bn["color"].view(np.uint32))
     for k, ch in pairs[1:4]:
         assert np.array_equal(bn[k], of.read(ch)), k
     d = float(np.abs(bn["color"] - of.read(abi.BNX_FB_COLOR)).max())
