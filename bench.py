@@ -260,7 +260,7 @@ def set_tuning(dev, tuning):
     return {"tuning": dev.tuning(), "autotune": "not run (tuning given)"}
 
 
-def run_gpu(sc, steps, warmup, tuning=None):
+def run_gpu(sc, steps, warmup, tuning=None, captured_form=True):
     import numpy as np
 
     import barney_amd as bx
@@ -285,7 +285,8 @@ def run_gpu(sc, steps, warmup, tuning=None):
     dev.sync()
     log("warm-up done")
     lat, tot = [], dict(rays=0, nodes=0, tris=0, spheres=0, trace_ms=0.0, trace_launches=0, volume_samples=0,
-                        march_ms=0.0, march_launches=0, mc_cells=0, march_rays=0, umesh_nodes=0, umesh_tets=0)
+                        march_ms=0.0, march_launches=0, mc_cells=0, march_rays=0, umesh_nodes=0, umesh_tets=0,
+                        captured_batches=0, uncaptured_batches=0)
     t0 = time.perf_counter()
     for _ in range(steps):
         ts = time.perf_counter()
@@ -298,9 +299,33 @@ def run_gpu(sc, steps, warmup, tuning=None):
             tot[k] += st[k]
     dev.sync()
     elapsed = time.perf_counter() - t0
+    # the same frames with every batch in a captured bounce loop (graph_paths
+    # raised to the batch size), when the tuned form ran its batches on the
+    # host loop (none captured nor device driven): the batching form of the
+    # design, timed beside the tuned one
+    cap = None
+    if captured_form and tot["uncaptured_batches"] == 0 and tot["captured_batches"] == 0:
+        t = dict(tune["tuning"])
+        t["graph_paths"] = max(t["graph_paths"], t["render_paths"])
+        dev.set_tuning(**t)
+        fr.reset()
+        bx.render(w.world, cam, rend, fr, sc.spp)        # learns the generations, captures the loop
+        dev.sync()
+        rays, nb = 0, 0
+        t1 = time.perf_counter()
+        for _ in range(steps):
+            fr.reset()
+            st = bx.render(w.world, cam, rend, fr, sc.spp)
+            rays += st["rays"]
+            nb += st["captured_batches"]
+        dev.sync()
+        el = time.perf_counter() - t1
+        cap = {"mrays_s": round(rays / el / 1e6, 3), "ms_per_step": round(el * 1e3 / steps, 4),
+               "captured_batches_per_step": nb // steps, "graph_paths": t["graph_paths"]}
+        dev.set_tuning(**tune["tuning"])
     fr.close()
     dev.close()
-    return elapsed, np.array(lat), tot, tune
+    return elapsed, np.array(lat), tot, tune, cap
 
 
 def run_cpu(sc, budget_s):
@@ -670,6 +695,8 @@ def main():
     ap.add_argument("--steps", type=int, default=None,
                     help="timed steps (c1-c4: renders); c5 times its fixed sample (--requests) whatever the value")
     ap.add_argument("--requests", type=int, default=C5_REQUESTS, help="c5: size of the fixed sample")
+    ap.add_argument("--no-captured-form", action="store_true",
+                    help="c1-c4: skip the second timing with every batch in a captured bounce loop")
     ap.add_argument("--server-generations", type=int, default=0,
                     help="c5: generations per server step (0: one host read-back per generation, "
                          "bnx_server_step; 1..16: bnx_server_step_n)")
@@ -714,12 +741,20 @@ def main():
         warmup = args.warmup if args.warmup is not None else (1 if heavy else 3)
         sc, desc = {"c1": workload_c1, "c2": workload_c2, "c3": workload_c3,
                     "c4": workload_c4}[args.workload](args)
-        elapsed, lat, tot, tune = run_gpu(sc, steps, warmup, parse_tuning(args.tuning))
+        elapsed, lat, tot, tune, cap = run_gpu(sc, steps, warmup, parse_tuning(args.tuning),
+                                               captured_form=not args.no_captured_form)
         desc.update(tune)
         desc["max_world_prims"] = int(getattr(sc, "num_prims", 0))
         mrays = tot["rays"] / elapsed / 1e6
         lat_line = {"p50": pct(lat, 50), "p99": pct(lat, 99)}
-        extra = {"rays_per_step": tot["rays"] // steps}
+        # batches per frame that ran as a captured bounce loop, or device
+        # driven without a graph; neither: the host loop (one read-back per
+        # generation)
+        extra = {"rays_per_step": tot["rays"] // steps,
+                 "batches_per_step": {"captured": tot["captured_batches"] // steps,
+                                      "device_driven_uncaptured": tot["uncaptured_batches"] // steps}}
+        if cap is not None:
+            extra["captured_form"] = cap
         if tot["volume_samples"]:
             extra["volume_samples_per_s"] = round(tot["volume_samples"] / elapsed, 1)
     rays_per_world = extra.pop("_rays_per_world", None)

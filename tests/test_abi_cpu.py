@@ -88,8 +88,8 @@ def test_bench_reads_profile_traffic_of_every_workload():
     sys.path.insert(0, root)
     import bench
     for w, k in [("c1", "k_trace_static<32, true, false>"), ("c2", "k_volume_march<0>"),
-                 ("c3", "k_trace_static<16, false>"), ("c4", "k_volume_march<1>"),
-                 ("c5", "k_trace_static<16, false>")]:
+                 ("c3", "k_trace_static<16, false, false>"), ("c4", "k_volume_march<1>"),
+                 ("c5", "k_trace_static<16, false, false>")]:
         t = bench.profile_traffic(w, k)
         assert t is not None and t > 0, (w, k)
     assert bench.profile_traffic("c3", "k_no_such_kernel") is None
