@@ -275,13 +275,16 @@ __global__ void k_umesh_raster(const int4 *tets, const int4 *ext, const uint8_t 
 // kernel of its own: the cells' Newton inversions would otherwise raise the
 // register pressure of the tet kernel).
 // waves per SIMD the march kernel is compiled for, lean and full variant.
-// The full variant (tet point location + isosurfaces) needs 128 VGPRs to
+// The full variant (tet point location + isosurfaces) needs 126 VGPRs to
 // run without spilling, which allows 4 waves per SIMD.  Compiled for 6 (80
-// VGPRs) it spills 143 of them to scratch, and the spill stores then make up
-// most of its memory writes, yet on the 20.5M-tet volume with an isosurface
-// (1024^2, 32 spp) it renders 1.22x faster than at 4 waves and 1.03x faster
-// than at 5 (48 spilled): the dependent gathers of the point queries need
-// the extra waves to hide their latency more than the spills cost.
+// VGPRs) it spills 112 of them to scratch, and the spill stores then make up
+// most of its memory writes (4.9 GB per launch against 0.95 GB at 4 waves),
+// yet on the 20.5M-tet volume with an isosurface (1024^2, 32 spp) a launch
+// takes 24.2 ms against 27.9 ms at 4 waves and 25.2 ms at 5 (40 spilled):
+// the dependent gathers of the point queries need the extra waves to hide
+// their latency more than the spills cost.  Splitting the kernel into an
+// isosurface pass and a volume pass would not remove the spills (each alone
+// spills 46 and 64 VGPRs at 6 waves).
 #ifndef BNX_MARCH_WAVES_LEAN
 #define BNX_MARCH_WAVES_LEAN 5
 #endif

@@ -331,8 +331,7 @@ def run_gpu(sc, steps, warmup, tuning=None, captured_form=True):
 def run_cpu(sc, budget_s):
     import oracle
     from barney_amd import abi, scenes
-    if not os.path.exists(oracle.LIB_PATH):
-        oracle.build()
+    march = oracle.use_host_native()
     threads = oracle.default_threads()
     vol = isinstance(sc, (scenes.StructuredVolumeScene, scenes.UMeshVolumeScene))
     if isinstance(sc, scenes.UMeshVolumeScene):
@@ -366,8 +365,8 @@ def run_cpu(sc, budget_s):
     el = time.perf_counter() - t0
     out = {"value": rays / el / 1e6, "unit": "Mrays/s", "cores": threads, "kind": "port",
            "sample": f"{frames} {sc.width}x{sc.height} frames at {spp} spp of the same scene and camera "
-                     f"({el:.1f} s wall on {threads} threads; shadow rays traced as closest hit as in "
-                     f"the reference CPU path)"}
+                     f"({el:.1f} s wall on {threads} threads, built -march={march}; shadow rays traced as "
+                     f"closest hit as in the reference CPU path)"}
     if vol:
         out["volume_samples_per_s"] = vsamp / el
     counts = None if vol else oracle_ray_model(o, cam, rend, sc.width, sc.height, spp)
@@ -629,8 +628,7 @@ def run_cpu_c5(args, budget_s, sample_reqs=None):
     of the sample, rendered at BYTE_MODEL_RES^2, 1 spp)"""
     import oracle
     from barney_amd import scenes
-    if not os.path.exists(oracle.LIB_PATH):
-        oracle.build()
+    march = oracle.use_host_native()
     threads = oracle.default_threads()
     c1, c2, c3 = serve_scenes(args)
     built = {}
@@ -645,8 +643,11 @@ def run_cpu_c5(args, budget_s, sample_reqs=None):
     small = [r for r in stream if r.width == 256 and r.spp == 1]
     for k in sorted({r.scene for r in small}):
         oscene(k)
+    # the small requests in stream order, from the start again once all are
+    # done, until the budget is spent
     rays, frames, t0 = 0, 0, time.perf_counter()
-    for r in small:
+    while time.perf_counter() - t0 <= budget_s:
+        r = small[frames % len(small)]
         sc = (c1, c2, c3)[r.scene]
         os_ = oscene(r.scene)
         cam = oracle.camera_perspective(r.camera["position"], r.camera["direction"], r.camera["up"], r.camera["fovy"])
@@ -654,12 +655,12 @@ def run_cpu_c5(args, budget_s, sample_reqs=None):
         _, st = oracle.render(os_, cam, serve_renderer(sc), fr, r.spp, threads=threads)
         rays += st["rays"]
         frames += 1
-        if time.perf_counter() - t0 > budget_s:
-            break
     el = time.perf_counter() - t0
+    what = (f"the first {frames}" if frames <= len(small) else
+            f"{frames} renders cycling over the {len(small)}")
     out = {"value": round(rays / el / 1e6, 4), "unit": "Mrays/s", "cores": threads, "kind": "port",
-           "sample": f"the first {frames} 256^2 1-spp requests of the same stream (all three worlds), {el:.1f} s wall "
-                     f"on {threads} threads"}
+           "sample": f"{what} 256^2 1-spp requests among the stream's first 2000 (all three worlds), {el:.1f} s "
+                     f"wall on {threads} threads, built -march={march}"}
     counts = None
     if sample_reqs:
         counts = []
@@ -711,7 +712,7 @@ def main():
     ap.add_argument("--load", type=float, default=0.7)
     ap.add_argument("--forms", default="all", choices=["all", "served"],
                     help="c5 only: run every form, or only the served pass at saturation (for profiling)")
-    ap.add_argument("--cpu-seconds", type=float, default=8.0)
+    ap.add_argument("--cpu-seconds", type=float, default=12.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--tuning", default="",
                     help='launch tuning instead of running bnx_autotune at set-up: "default" (the device\'s '

@@ -27,6 +27,29 @@ def build(force: bool = False) -> str:
     return LIB_PATH
 
 
+def use_host_native() -> str:
+    """Compile the oracle for the instruction set of the machine this runs on
+    (g++ -march=native, with the same -ffp-contract=off arithmetic, so the
+    same results) into a fresh temporary folder and load that library instead
+    of the portable x86-64-v2 one.  For timing the CPU baseline on the host
+    that measures; call before anything loads the library.  Returns the
+    -march used: "native", or "x86-64-v2" when the native build fails (the
+    portable library is then kept)."""
+    global LIB_PATH
+    if _lib is not None:
+        raise RuntimeError("oracle library already loaded")
+    import tempfile
+    out = os.path.join(tempfile.mkdtemp(prefix="oracle_native_"), "liboracle.so")
+    flags = "-O3 -march=native -std=c++17 -fPIC -ffp-contract=off -fno-fast-math -Wall -Wno-unused-function"
+    r = subprocess.run(["make", "-C", _HERE, "-s", f"OUT={out}", f"CXXFLAGS={flags}"], capture_output=True)
+    if r.returncode == 0 and os.path.exists(out):
+        LIB_PATH = out
+        return "native"
+    if not os.path.exists(LIB_PATH):
+        build()
+    return "x86-64-v2"
+
+
 def lib():
     global _lib
     if _lib is None:
